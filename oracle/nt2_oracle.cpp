@@ -1,0 +1,1189 @@
+// ============================================================================
+// THE ORACLE — test infrastructure, not product code.
+//
+// A CPU restatement of NT2's own Boost.SIMD evaluation of the table-expression
+// hot path: the math kernels (exp, log, sqrt, erfc, normcdf, fastnormcdf,
+// cos/sin/sincos with NT2's full range-reduction cascade), the saturated
+// uint8 ops, NT2's transform schedule (8-lane f32 / 4-lane f64 packs from
+// index 0, scalar tail with the scalar kernels) and NT2's SIMD inner_fold
+// order for sums.  Built with g++ -O3 -mavx2 -ffp-contract=off -fopenmp so
+// that NT2's fma(a,b,c) is a*b+c with two roundings, as on x86 AVX
+// (modules/boost/simd/base/include/boost/simd/arithmetic/functions/generic/
+// fma.hpp:32).
+//
+// Only tests/, the smoke check and bench.py's cpu_baseline leg load this
+// library, and only as the checker or as the timed CPU baseline.
+//
+// Every function cites the reference file:line it restates.  Paths below are
+// relative to the reference tree root.  Parity pinning (golden vectors and
+// unit-test KATs) lives in tests/test_oracle_*.py.
+// ============================================================================
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+#include <cstring>
+#include <algorithm>
+#include <omp.h>
+#include <unistd.h>
+
+#include "pio2_tables.h"
+
+#define F32(b) __builtin_bit_cast(float, (uint32_t)(b##u))
+#define F64(b) __builtin_bit_cast(double, (uint64_t)(b##ull))
+
+namespace {
+
+inline uint32_t bits(float f) { return __builtin_bit_cast(uint32_t, f); }
+inline uint64_t bits(double d) { return __builtin_bit_cast(uint64_t, d); }
+inline float as_f(uint32_t b) { return __builtin_bit_cast(float, b); }
+inline double as_d(uint64_t b) { return __builtin_bit_cast(double, b); }
+
+// NT2 static Horner: coefficients in decreasing powers, each step
+// fma(x, acc, c) = x*acc + c with two roundings.
+// modules/core/polynomials/include/nt2/polynomials/functions/scalar/impl/horner.hpp:13-45
+template <class T, size_t K>
+inline T horner(const T (&c)[K], T x) {
+  T acc = c[0];
+  for (size_t i = 1; i < K; ++i) acc = x * acc + c[i];
+  return acc;
+}
+
+// round2even: modules/boost/simd/base/include/boost/simd/arithmetic/functions/simd/common/round2even.hpp:31-40
+inline float r2e(float x) { return __builtin_rintf(x); }
+inline double r2e(double x) { return __builtin_rint(x); }
+
+// ---------------------------------------------------------------------------
+// single precision constants (bit patterns from the NT2 constant registry)
+// ---------------------------------------------------------------------------
+const float fInvlog_2 = F32(0x3fb8aa3b), fLog_2hi = F32(0x3f318000),
+            fLog_2lo = F32(0xb95e8083), fMaxlog = F32(0x42b0c0a5),
+            fMinlog = F32(0xc2b0c0a5);
+// f_expo_reduction.hpp:62-70
+const float fExpC[5] = {F32(0x3ab778cf), F32(0x3c098d8b), F32(0x3d2aa957),
+                        F32(0x3e2aa9a5), F32(0x3f000000)};
+const float fSqrt_2o_2 = F32(0x3f3504f3), fSmallest = F32(0x00800000),
+            fTwotonmb = F32(0x4B000000), fMlogtwo2nmb = F32(0xc17f1402);
+// f_kernel.hpp:62-72
+const float fLogC[8] = {F32(0xbda5dff0), F32(0x3e0229f9), F32(0xbe04d6b7),
+                        F32(0x3e116e80), F32(0xbe2a6aa0), F32(0x3e4cd0a3),
+                        F32(0xbe800064), F32(0x3eaaaaa9)};
+const float fTwothird = F32(0x3f2aaaab);
+// erf_kernel.hpp:50-64 (erfc2, degree 10) and :72-84 (erfc3, degree 8)
+const float fErfc2C[11] = {F32(0xc0283611), F32(0x40d67e3b), F32(0xc0cd1a85),
+                           F32(0x4050b063), F32(0xbfaea865), F32(0x3e2006f0),
+                           F32(0x3e1175c7), F32(0x3ec4ca6e), F32(0x3e243828),
+                           F32(0xbf918a62), F32(0x3f0a0e8b)};
+const float fErfc3C[9] = {F32(0x41aa8e55), F32(0x401b5680), F32(0xc010d956),
+                          F32(0x3f2cff3b), F32(0xc016c985), F32(0xbffc9284),
+                          F32(0xbfa11698), F32(0xbe036d7e), F32(0x3f7ffffe)};
+// fastnormcdf.hpp:42-49
+const float fFastC[5] = {F32(0x3faa466f), F32(0xbfe91eea), F32(0x3fe40778),
+                         F32(0xbeb68f87), F32(0x3ea385fa)};
+const float fInvsqrt_2pi = F32(0x3ecc422a);
+// trigonometric constants
+const float fPio_4 = F32(0x3f490fdb), fPio_2 = F32(0x3fc90fdb),
+            f20pi = F32(0x427b53d1), fMedium_pi = F32(0x43490fdb),
+            fDmpi = F32(0x49490fdb), fPio2_1 = F32(0x3fc90f80),
+            fPio2_2 = F32(0x37354400), fPio2_3 = F32(0x2e85a300),
+            fPio2_1t = F32(0x37354443), fPio2_2t = F32(0x2e85a308),
+            fPio2_3t = F32(0x248d3132), fTwoopi = F32(0x3f22f983),
+            fInvpio_2 = F32(0x3f22f984);
+// f_trig_evaluation.hpp:35-46
+const float fCosC[3] = {F32(0x37ccf5ce), F32(0xbab60619), F32(0x3d2aaaa5)};
+const float fSinC[3] = {F32(0xb94ca1f9), F32(0x3c08839d), F32(0xbe2aaaa2)};
+
+// ---------------------------------------------------------------------------
+// double precision constants
+// ---------------------------------------------------------------------------
+const double dInvlog_2 = F64(0x3ff71547652b82fe),
+             dLog_2hi = F64(0x3fe62e42fee00000),
+             dLog_2lo = F64(0x3dea39ef35793c76),
+             dMaxlog = F64(0x40862e42fefa39ef),
+             dMinlog = F64(0xc086232bdd7abcd2);
+// d_expo_reduction.hpp:66-73
+const double dExpC[5] = {F64(0x3e66376972bea4d0), F64(0xbebbbd41c5d26bf1),
+                         F64(0x3f11566aaf25de2c), F64(0xbf66c16c16bebd93),
+                         F64(0x3fc555555555553e)};
+const double dSqrt_2o_2 = F64(0x3FE6A09E667F3BCD);
+// d_log.hpp:73-84
+const double dLogT1[3] = {F64(0x3fc39a09d078c69f), F64(0x3fcc71c51d8e78af),
+                          F64(0x3fd999999997fa04)};
+const double dLogT2[4] = {F64(0x3fc2f112df3e5244), F64(0x3fc7466496cb03de),
+                          F64(0x3fd2492494229359), F64(0x3fe5555555555593)};
+// erf_kernel.hpp:96-166
+const double dErf1P[5] = {F64(0x3f110512d5b20332), F64(0x3f53b7664358865a),
+                          F64(0x3fa4a59a4f02579c), F64(0x3fc16500f106c0a5),
+                          F64(0x3ff20dd750429b61)};
+const double dErf1Q[5] = {F64(0x3f37ea4332348252), F64(0x3f8166f75999dbd1),
+                          F64(0x3fb64536ca92ea2f), F64(0x3fdd0a84eb1ca867),
+                          F64(0x3ff0000000000000)};
+const double dErfc2P[7] = {0.0, F64(0x3f7cf4cfe0aacbb4), F64(0x3fb2488a6b5cb5e5),
+                           F64(0x3fd53dd7a67c7e9f), F64(0x3fec1986509e687b),
+                           F64(0x3ff54dfe9b258a60), F64(0x3feffffffbbb552b)};
+const double dErfc2Q[7] = {F64(0x3f89a996639b0d00), F64(0x3fc033c113a7deee),
+                           F64(0x3fe307622fcff772), F64(0x3ff9e677c2777c3c),
+                           F64(0x40053b1052dca8bd), F64(0x4003adeae79b9708),
+                           F64(0x3ff0000000000000)};
+const double dErfc3P[7] = {0.0, F64(0x3f971d0907ea7a92), F64(0x3fc42210f88b9d43),
+                           F64(0x3fe29be1cff90d94), F64(0x3ff44744306832ae),
+                           F64(0x3ff9fa202deb88e5), F64(0x3fefff5a9e697ae2)};
+const double dErfc3Q[7] = {F64(0x3fa47bd61bbb3843), F64(0x3fd1d7ab774bb837),
+                           F64(0x3ff0cfd4cb6cde9f), F64(0x400315ffdfd5ce91),
+                           F64(0x400afd487397568f), F64(0x400602f24bf3fdb6),
+                           F64(0x3ff0000000000000)};
+const double dPi = F64(0x400921fb54442d18);
+const double dPio_4 = F64(0x3fe921fb54442d18), dPio_2 = F64(0x3ff921fb54442d18),
+             d20pi = F64(0x404f6a7a2955385e), dMedium_pi = F64(0x412921fb54442d18),
+             dPio2_1 = F64(0x3FF921FB54400000), dPio2_2 = F64(0x3DD0B4611A600000),
+             dPio2_3 = F64(0x3BA3198A2E000000), dPio2_1t = F64(0x3DD0B4611A626331),
+             dPio2_2t = F64(0x3BA3198A2E037073), dPio2_3t = F64(0x397B839A252049C1),
+             dTwoopi = F64(0x3fe45f306dc9c883), dInvpio_2 = F64(0x3FE45F306DC9C883);
+const double dDmpi = (double)F32(0x49490fdb);
+// d_trig_evaluation.hpp:33-50
+const double dCosC[7] = {F64(0x3da8ff831ad9b219), F64(0xbe21eea7c1e514d4),
+                         F64(0x3e927e4f8e06d9a5), F64(0xbefa01a019ddbcd9),
+                         F64(0x3f56c16c16c15d47), F64(0xbfa5555555555551),
+                         F64(0x3fe0000000000000)};
+const double dSinC[6] = {F64(0x3de5d8fd1fcf0ec1), F64(0xbe5ae5e5a9291691),
+                         F64(0x3ec71de3567d4896), F64(0xbf2a01a019bfdf03),
+                         F64(0x3f8111111110f7d0), F64(0xbfc5555555555548)};
+
+// ===========================================================================
+// exp
+// ===========================================================================
+// f32: modules/core/exponential/include/nt2/exponential/functions/scalar/impl/expo/
+//      f_expo_reduction.hpp:73-79 (reduce), :58-72 (approx), expo_scale.hpp:26-35
+//      (2^k scaling), simd/common/impl/expo/expo_base.hpp:47-60 (special values).
+//      The scalar variant (scalar/impl/expo/expo_base.hpp:33-45) returns the same
+//      values, so one lane function serves both.
+inline float exp_f(float a0) {
+  float k = r2e(fInvlog_2 * a0);
+  float x = a0 - k * fLog_2hi;
+  x = x - k * fLog_2lo;
+  float y = horner(fExpC, x);
+  float r = (y * (x * x) + x) + 1.0f;
+  float kc = std::fmin(std::fmax(k, -200.0f), 200.0f);
+  uint32_t sb = (uint32_t)((int32_t)kc + 127) << 23;
+  r = r * as_f(sb);
+  r = (a0 <= fMinlog) ? 0.0f : r;
+  r = (a0 >= fMaxlog) ? INFINITY : r;
+  return (a0 != a0) ? a0 : r;
+}
+
+// f64: d_expo_reduction.hpp:57-84 (reduce/approx/finalize), expo_base.hpp:31-42
+inline double exp_d(double a0) {
+  double k = r2e(dInvlog_2 * a0);
+  double hi = a0 - k * dLog_2hi;
+  double lo = k * dLog_2lo;
+  double x = hi - lo;
+  double t = x * x;
+  double c = x - t * horner(dExpC, t);
+  c = 1.0 - ((lo - (x * c) / (2.0 - c)) - hi);
+  double kc = std::fmin(std::fmax(k, -2000.0), 2000.0);
+  uint64_t sb = (uint64_t)((int64_t)kc + 1023) << 52;
+  c = c * as_d(sb);
+  c = (a0 <= dMinlog) ? 0.0 : c;
+  c = (a0 >= dMaxlog) ? (double)INFINITY : c;
+  return (a0 != a0) ? a0 : c;
+}
+
+// ===========================================================================
+// log
+// ===========================================================================
+// fast_frexp: modules/boost/simd/base/include/boost/simd/ieee/functions/simd/common/fast_frexp.hpp:33-40
+// f32 log: simd/common/impl/logs/f_log.hpp:101-125 + :173-186 (finalize),
+//          scalar/impl/logs/f_kernel.hpp:47-78.  The scalar variant
+//          (scalar/impl/logs/f_log.hpp) returns the same values.
+inline float log_f(float a0) {
+  float z = a0, t = 0.0f;
+  if (std::fabs(z) < fSmallest) { z = z * fTwotonmb; t = fMlogtwo2nmb; }
+  uint32_t b = bits(z);
+  int32_t e = (int32_t)((b & 0x7f800000u) >> 23) - 126;
+  float m = as_f((b & ~0x7f800000u) | 0x3f000000u);
+  bool lt = m < fSqrt_2o_2;
+  float fe = lt ? (float)e + (-1.0f) : (float)e;
+  float x = (lt ? m + m : m) - 1.0f;
+  float x2 = x * x;
+  float y = (horner(fLogC, x) * x) * x2;
+  y = fe * fLog_2lo + y;
+  y = -0.5f * x2 + y;
+  float r = fLog_2hi * fe + ((x + y) + t);
+  if (a0 < 0.0f || a0 != a0) r = NAN;
+  if (a0 == INFINITY) r = a0;
+  if (a0 == 0.0f) r = -INFINITY;
+  return r;
+}
+
+// f64 log: simd/common/impl/logs/d_log.hpp:60-97 (no denormal prescale in NT2's f64 path)
+inline double log_d(double a0) {
+  uint64_t b = bits(a0);
+  int64_t k = (int64_t)((b & 0x7ff0000000000000ull) >> 52) - 1022;
+  double x = as_d((b & ~0x7ff0000000000000ull) | 0x3fe0000000000000ull);
+  bool lt = dSqrt_2o_2 > x;
+  k = k + (lt ? -1 : 0);
+  double f = (x + (lt ? x : 0.0)) - 1.0;
+  double dk = (double)k;
+  double s = f / (2.0 + f);
+  double z = s * s;
+  double w = z * z;
+  double t1 = w * horner(dLogT1, w);
+  double t2 = z * horner(dLogT2, w);
+  double R = t2 + t1;
+  double hfsq = 0.5 * (f * f);
+  double r = dk * dLog_2hi - ((hfsq - (s * (hfsq + R) + dk * dLog_2lo)) - f);
+  if (a0 < 0.0 || a0 != a0) r = NAN;
+  if (a0 == (double)INFINITY) r = a0;
+  if (a0 == 0.0) r = -(double)INFINITY;
+  return r;
+}
+
+// ===========================================================================
+// erfc, normcdf, fastnormcdf
+// ===========================================================================
+// f32: modules/core/euler/include/nt2/euler/functions/simd/common/erfc.hpp:92-123
+//      (scalar/erfc.hpp:72-100 returns the same values); kernels erf_kernel.hpp:46-85.
+inline float erfc_f(float a0) {
+  float x = std::fabs(a0);
+  float z = x / (1.0f + x);
+  float r;
+  if (x < fTwothird) {
+    r = (1.0f - z) * horner(fErfc3C, z);
+  } else {
+    float z2 = z - 0.4f;
+    r = exp_f(-(x * x)) * horner(fErfc2C, z2);
+  }
+  if (x == INFINITY) r = 0.0f;
+  return (a0 < 0.0f) ? 2.0f - r : r;
+}
+
+// f64 SIMD: simd/common/erfc.hpp:46-89.  f64 scalar (pack tail): scalar/erfc.hpp:38-70.
+inline double erfc_d(double a0, bool scalar) {
+  if (scalar) {
+    if (a0 != a0) return a0;
+    double x = std::fabs(a0), xx = x * x, z = 0.0;
+    if (x <= 0.0000000001) z = 1.0 - (x * 2.0) / std::sqrt(dPi);
+    else if (x < 0.65) z = 1.0 - x * (horner(dErf1P, xx) / horner(dErf1Q, xx));
+    else if (x < 2.2) z = exp_d(-xx) * (horner(dErfc2P, x) / horner(dErfc2Q, x));
+    else if (x < 6.0) z = exp_d(-xx) * (horner(dErfc3P, x) / horner(dErfc3Q, x));
+    return (a0 < 0.0) ? 2.0 - z : z;
+  }
+  double x = std::fabs(a0), xx = x * x, r;
+  if (x < 0.65) r = 1.0 - x * (horner(dErf1P, xx) / horner(dErf1Q, xx));
+  else if (x < 2.2) r = exp_d(-xx) * (horner(dErfc2P, x) / horner(dErfc2Q, x));
+  else r = exp_d(-xx) * (horner(dErfc3P, x) / horner(dErfc3Q, x));
+  if (x == (double)INFINITY) r = 0.0;
+  return (a0 < 0.0) ? 2.0 - r : r;
+}
+
+// normcdf(x) = Half*erfc(-Sqrt_2o_2*x):
+// modules/core/statistics/include/nt2/statistics/functions/generic/normcdf.hpp:33
+inline float normcdf_f(float a0) { return 0.5f * erfc_f((-fSqrt_2o_2) * a0); }
+inline double normcdf_d(double a0, bool scalar) {
+  return 0.5 * erfc_d((-dSqrt_2o_2) * a0, scalar);
+}
+
+// fastnormcdf (single only): generic/fastnormcdf.hpp:39-54
+inline float fastnormcdf_f(float a0) {
+  float l = std::fabs(a0);
+  float k = 1.0f / (0.2316419f * l + 1.0f);
+  float w = horner(fFastC, k);
+  w = w * ((k * fInvsqrt_2pi) * exp_f((-(l * l)) * 0.5f));
+  return (a0 > 0.0f) ? 1.0f - w : w;
+}
+
+// ===========================================================================
+// trigonometric range reduction
+// ===========================================================================
+int g_ph_fix = 0;
+
+// fdlibm __kernel_rem_pio2 / __kernel_rem_pio2f as embedded by NT2:
+// modules/core/trigonometric/include/nt2/trigonometric/functions/scalar/rem_pio2.hpp:234-402
+// (single, 8-bit chunks, jk = 9) and :690-905 (double, 24-bit chunks, jk = 4);
+// both called with prec = 2.  One template restates both, parameterised by the
+// chunk width B.
+template <class T, int B>
+int kernel_rem_pio2(const T* x, T* y, int e0, int nx, const int32_t* ipio2,
+                    const T* PIo2) {
+  const int jk = (B == 8) ? 9 : 4;
+  const int jp = jk;
+  const T two_b = (T)(1 << B);
+  const T twon_b = (T)1 / two_b;
+  const int32_t full = 1 << B;          // 0x100 / 0x1000000
+  const int32_t maskall = full - 1;     // 0xff / 0xffffff
+  int32_t jz, jx, jv, carry, n, iq[20], i, j, k, m, q0, ih;
+  T z, fw, f[20], fq[20], q[20];
+
+  jx = nx - 1;
+  jv = (e0 - 3) / B;
+  if (jv < 0) jv = 0;
+  q0 = e0 - B * (jv + 1);
+  j = jv - jx;
+  m = jx + jk;
+  for (i = 0; i <= m; i++, j++) f[i] = (j < 0) ? (T)0 : (T)ipio2[j];
+  for (i = 0; i <= jk; i++) {
+    for (j = 0, fw = 0; j <= jx; j++) fw += x[j] * f[jx + i - j];
+    q[i] = fw;
+  }
+  jz = jk;
+recompute:
+  for (i = 0, j = jz, z = q[jz]; j > 0; i++, j--) {
+    fw = (T)((int32_t)(twon_b * z));
+    iq[i] = (int32_t)(z - two_b * fw);
+    z = q[j - 1] + fw;
+  }
+  z = std::ldexp(z, q0);
+  z -= (T)8 * std::floor(z * (T)0.125);
+  n = (int32_t)z;
+  z -= (T)n;
+  ih = 0;
+  if (q0 > 0) {
+    i = (iq[jz - 1] >> (B - q0));
+    n += i;
+    iq[jz - 1] -= i << (B - q0);
+    ih = iq[jz - 1] >> (B - 1 - q0);
+  } else if (q0 == 0) {
+    // NT2 single: iq>>8 (rem_pio2.hpp:290); double: iq>>23 (:791).  For
+    // 8-bit chunks the top bit is bit 7, so NT2's shift by 8 always yields 0
+    // and about 4% of |x| > 2^18*pi lose accuracy; g_ph_fix selects the
+    // corrected shift so the tests can pin the rest of the restatement.
+    ih = iq[jz - 1] >> ((B == 8) ? (g_ph_fix ? 7 : 8) : 23);
+  } else if (z >= (T)0.5) {
+    ih = 2;
+  }
+  if (ih > 0) {
+    n += 1;
+    carry = 0;
+    for (i = 0; i < jz; i++) {
+      j = iq[i];
+      if (carry == 0) {
+        if (j != 0) { carry = 1; iq[i] = full - j; }
+      } else {
+        iq[i] = maskall - j;
+      }
+    }
+    if (q0 > 0) {
+      switch (q0) {
+        case 1: iq[jz - 1] &= (maskall >> 1); break;
+        case 2: iq[jz - 1] &= (maskall >> 2); break;
+      }
+    }
+    if (ih == 2) {
+      z = (T)1 - z;
+      if (carry != 0) z -= std::ldexp((T)1, q0);
+    }
+  }
+  if (z == (T)0) {
+    j = 0;
+    for (i = jz - 1; i >= jk; i--) j |= iq[i];
+    if (j == 0) {
+      for (k = 1; iq[jk - k] == 0; k++) {
+      }
+      for (i = jz + 1; i <= jz + k; i++) {
+        f[jx + i] = (T)ipio2[jv + i];
+        for (j = 0, fw = 0; j <= jx; j++) fw += x[j] * f[jx + i - j];
+        q[i] = fw;
+      }
+      jz += k;
+      goto recompute;
+    }
+  }
+  if (z == (T)0) {
+    jz -= 1;
+    q0 -= B;
+    while (iq[jz] == 0) { jz--; q0 -= B; }
+  } else {
+    z = std::ldexp(z, -q0);
+    if (z >= two_b) {
+      fw = (T)((int32_t)(twon_b * z));
+      iq[jz] = (int32_t)(z - two_b * fw);
+      jz += 1;
+      q0 += B;
+      iq[jz] = (int32_t)fw;
+    } else {
+      iq[jz] = (int32_t)z;
+    }
+  }
+  fw = std::ldexp((T)1, q0);
+  for (i = jz; i >= 0; i--) { q[i] = fw * (T)iq[i]; fw *= twon_b; }
+  for (i = jz; i >= 0; i--) {
+    for (fw = 0, k = 0; k <= jp && k <= jz - i; k++) fw += PIo2[k] * q[i + k];
+    fq[jz - i] = fw;
+  }
+  fw = 0;
+  for (i = jz; i >= 0; i--) fw += fq[i];
+  y[0] = (ih == 0) ? fw : -fw;
+  fw = fq[0] - fw;
+  for (i = 1; i <= jz; i++) fw += fq[i];
+  y[1] = (ih == 0) ? fw : -fw;
+  return n & 7;
+}
+
+float PIo2_f[11];
+double PIo2_d[8];
+struct TableInit {
+  TableInit() {
+    for (int i = 0; i < 11; ++i) PIo2_f[i] = as_f(oracle_tables::pio2_8_bits[i]);
+    for (int i = 0; i < 8; ++i) PIo2_d[i] = as_d(oracle_tables::pio2_24_bits[i]);
+  }
+} table_init_;
+
+// __ieee754_rem_pio2f: scalar/rem_pio2.hpp:114-229; NT2 wrapper :37-44 returns n&3.
+int rem_pio2f_fdlibm(float x, float& xr) {
+  const float two8 = 2.5600000000e+02f, invpio2 = F32(0x3f22f984),
+              pio2_1 = F32(0x3fc90f80), pio2_1t = F32(0x37354443),
+              pio2_2 = F32(0x37354400), pio2_2t = F32(0x2e85a308),
+              pio2_3 = F32(0x2e85a300), pio2_3t = F32(0x248d3132);
+  float y0, z, w, t, r, fn, tx[3];
+  int32_t e0, i, j, nx, n, ix, hx;
+  hx = (int32_t)bits(x);
+  ix = hx & 0x7fffffff;
+  if (ix <= 0x3f490fd8) { xr = x; return 0; }
+  if (ix < 0x4016cbe4) {
+    if (hx > 0) {
+      z = x - pio2_1;
+      if ((ix & 0xfffffff0) != 0x3fc90fd0) { y0 = z - pio2_1t; }
+      else { z -= pio2_2; y0 = z - pio2_2t; }
+      xr = y0; return 1;
+    } else {
+      z = x + pio2_1;
+      if ((ix & 0xfffffff0) != 0x3fc90fd0) { y0 = z + pio2_1t; }
+      else { z += pio2_2; y0 = z + pio2_2t; }
+      xr = y0; return (-1) & 3;
+    }
+  }
+  if (ix <= 0x43490f80) {
+    t = std::fabs(x);
+    n = (int32_t)(t * invpio2 + 0.5f);
+    fn = (float)n;
+    r = t - fn * pio2_1;
+    w = fn * pio2_1t;
+    if (n < 32 && (int32_t)(ix & 0xffffff00) != oracle_tables::npio2_hw_f[n - 1]) {
+      y0 = r - w;
+    } else {
+      uint32_t high;
+      j = ix >> 23;
+      y0 = r - w;
+      high = bits(y0);
+      i = j - ((high >> 23) & 0xff);
+      if (i > 8) {
+        t = r;
+        w = fn * pio2_2;
+        r = t - w;
+        w = fn * pio2_2t - ((t - r) - w);
+        y0 = r - w;
+        high = bits(y0);
+        i = j - ((high >> 23) & 0xff);
+        if (i > 25) {
+          t = r;
+          w = fn * pio2_3;
+          r = t - w;
+          w = fn * pio2_3t - ((t - r) - w);
+          y0 = r - w;
+        }
+      }
+    }
+    if (hx < 0) { xr = -y0; return (-n) & 3; }
+    xr = y0; return n & 3;
+  }
+  if (ix >= 0x7f800000) { xr = x - x; return 0; }
+  e0 = (ix >> 23) - 134;
+  z = as_f((uint32_t)(ix - (int32_t)((uint32_t)e0 << 23)));
+  for (i = 0; i < 2; i++) { tx[i] = (float)((int32_t)(z)); z = (z - tx[i]) * two8; }
+  tx[2] = z;
+  nx = 3;
+  while (tx[nx - 1] == 0.0f) nx--;
+  float y[2];
+  n = kernel_rem_pio2<float, 8>(tx, y, e0, nx, oracle_tables::two_over_pi_8, PIo2_f);
+  if (hx < 0) { xr = -y[0]; return (-n) & 3; }
+  xr = y[0];
+  return n & 3;
+}
+
+// __ieee754_rem_pio2 (double): scalar/rem_pio2.hpp:521-621; NT2 wrapper :414-427.
+int rem_pio2d_fdlibm(double x, double& xr) {
+  if (x == (double)INFINITY) { xr = NAN; return 0; }
+  const double two24 = 1.67772160000000000000e+07,
+               invpio2 = F64(0x3FE45F306DC9C883), pio2_1 = F64(0x3FF921FB54400000),
+               pio2_1t = F64(0x3DD0B4611A626331), pio2_2 = F64(0x3DD0B4611A600000),
+               pio2_2t = F64(0x3BA3198A2E037073), pio2_3 = F64(0x3BA3198A2E000000),
+               pio2_3t = F64(0x397B839A252049C1);
+  double z = 0.0, w, t, r, fn, y0, tx[3];
+  int32_t e0, i, j, nx, n, ix, hx;
+  uint64_t xb = bits(x);
+  hx = (int32_t)(xb >> 32);
+  ix = hx & 0x7fffffff;
+  if (ix <= 0x3fe921fb) { xr = x; return 0; }
+  if (ix < 0x4002d97c) {
+    if (hx > 0) {
+      z = x - pio2_1;
+      if (ix != 0x3ff921fb) { y0 = z - pio2_1t; }
+      else { z -= pio2_2; y0 = z - pio2_2t; }
+      xr = y0; return 1;
+    } else {
+      z = x + pio2_1;
+      if (ix != 0x3ff921fb) { y0 = z + pio2_1t; }
+      else { z += pio2_2; y0 = z + pio2_2t; }
+      xr = y0; return (-1) & 3;
+    }
+  }
+  if (ix <= 0x413921fb) {
+    t = std::fabs(x);
+    n = (int32_t)(t * invpio2 + 0.5);
+    fn = (double)n;
+    r = t - fn * pio2_1;
+    w = fn * pio2_1t;
+    if ((n < 32) && (n > 0) && (ix != oracle_tables::npio2_hw_d[n - 1])) {
+      y0 = r - w;
+    } else {
+      uint32_t high;
+      j = ix >> 20;
+      y0 = r - w;
+      high = (uint32_t)(bits(y0) >> 32);
+      i = j - ((high >> 20) & 0x7ff);
+      if (i > 16) {
+        t = r;
+        w = fn * pio2_2;
+        r = t - w;
+        w = fn * pio2_2t - ((t - r) - w);
+        y0 = r - w;
+        high = (uint32_t)(bits(y0) >> 32);
+        i = j - ((high >> 20) & 0x7ff);
+        if (i > 49) {
+          t = r;
+          w = fn * pio2_3;
+          r = t - w;
+          w = fn * pio2_3t - ((t - r) - w);
+          y0 = r - w;
+        }
+      }
+    }
+    if (hx < 0) { xr = -y0; return (-n) & 3; }
+    xr = y0; return n & 3;
+  }
+  if (ix >= 0x7ff00000) { xr = x - x; return 0; }
+  uint32_t low = (uint32_t)xb;
+  e0 = (ix >> 20) - 1046;
+  z = as_d(((uint64_t)(uint32_t)(ix - (int32_t)((uint32_t)e0 << 20)) << 32) | low);
+  for (i = 0; i < 2; i++) { tx[i] = (double)((int32_t)(z)); z = (z - tx[i]) * two24; }
+  tx[2] = z;
+  nx = 3;
+  while (tx[nx - 1] == 0.0) nx--;
+  double y[2];
+  n = kernel_rem_pio2<double, 24>(tx, y, e0, nx, oracle_tables::two_over_pi_24, PIo2_d);
+  if (hx < 0) { xr = -y[0]; return (-n) & 3; }
+  xr = y[0];
+  return n & 3;
+}
+
+// is_ngt(a, b) = !(a > b): NaN counts as reduced
+inline bool ngt(float a, float b) { return !(a > b); }
+inline bool ngt(double a, double b) { return !(a > b); }
+
+// f64 range-reduction cascade on a group of G lanes (G = 4 for a SIMD pack,
+// G = 1 for the scalar tail): scalar/impl/trigo/trig_reduction.hpp:94-215.
+// For double, conversion to a wider type is not allowed, so lanes beyond
+// Medium_pi (2^18 pi) go to fdlibm per lane (:212-215, :217-220).
+template <int G>
+void reduce_d(const double* x, double* xr, int64_t* n) {
+  bool all4 = true, all2 = true, all20 = true, allm = true;
+  for (int i = 0; i < G; ++i) {
+    all4 = all4 && ngt(x[i], dPio_4);
+    all2 = all2 && ngt(x[i], dPio_2);
+    all20 = all20 && ngt(x[i], d20pi);
+    allm = allm && ngt(x[i], dMedium_pi);
+  }
+  if (all4) {
+    for (int i = 0; i < G; ++i) { xr[i] = x[i]; n[i] = 0; }
+  } else if (all2) {  // rem_pio2_straight.hpp:58-63
+    for (int i = 0; i < G; ++i) {
+      double r = x[i] - dPio2_1;
+      r -= dPio2_2;
+      r -= dPio2_3;
+      xr[i] = r; n[i] = 1;
+    }
+  } else if (all20) {  // rem_pio2_cephes.hpp:65-72
+    for (int i = 0; i < G; ++i) {
+      double xi = r2e(x[i] * dTwoopi);
+      double r = x[i] - xi * dPio2_1;
+      r -= xi * dPio2_2;
+      r -= xi * dPio2_3;
+      xr[i] = r; n[i] = ((int64_t)xi) & 3;
+    }
+  } else if (allm) {  // rem_pio2_medium.hpp:68-83
+    for (int i = 0; i < G; ++i) {
+      const double t = x[i];
+      const double fn = r2e(t * dInvpio_2);
+      double r = t - fn * dPio2_1;
+      double w = fn * dPio2_1t;
+      double t2 = r;
+      w = fn * dPio2_2;
+      r = t2 - w;
+      w = fn * dPio2_2t - ((t2 - r) - w);
+      t2 = r;
+      w = fn * dPio2_3;
+      r = t2 - w;
+      w = fn * dPio2_3t - ((t2 - r) - w);
+      xr[i] = r - w;
+      n[i] = ((int64_t)fn) & 3;
+    }
+  } else {
+    for (int i = 0; i < G; ++i) {
+      double r;
+      n[i] = rem_pio2d_fdlibm(x[i], r);
+      xr[i] = r;
+    }
+  }
+}
+
+// f32 cascade on a group of G lanes (G = 8 SIMD, 1 scalar tail):
+// trig_reduction.hpp:94-215 with use_conversion (:190-215): when every lane is
+// within 2^18 pi, the pack is split into halves of 4 doubles and each half runs
+// the f64 cascade; beyond that, fdlibm single per lane.
+template <int G>
+void reduce_f(const float* x, float* xr, int32_t* n) {
+  bool all4 = true, all2 = true, all20 = true, allm = true, alld = true;
+  for (int i = 0; i < G; ++i) {
+    all4 = all4 && ngt(x[i], fPio_4);
+    all2 = all2 && ngt(x[i], fPio_2);
+    all20 = all20 && ngt(x[i], f20pi);
+    allm = allm && ngt(x[i], fMedium_pi);
+    alld = alld && ngt(x[i], fDmpi);
+  }
+  if (all4) {
+    for (int i = 0; i < G; ++i) { xr[i] = x[i]; n[i] = 0; }
+  } else if (all2) {
+    for (int i = 0; i < G; ++i) {
+      float r = x[i] - fPio2_1;
+      r -= fPio2_2;
+      r -= fPio2_3;
+      xr[i] = r; n[i] = 1;
+    }
+  } else if (all20) {
+    for (int i = 0; i < G; ++i) {
+      float xi = r2e(x[i] * fTwoopi);
+      float r = x[i] - xi * fPio2_1;
+      r -= xi * fPio2_2;
+      r -= xi * fPio2_3;
+      xr[i] = r; n[i] = ((int32_t)xi) & 3;
+    }
+  } else if (allm) {
+    for (int i = 0; i < G; ++i) {
+      const float t = x[i];
+      const float fn = r2e(t * fInvpio_2);
+      float r = t - fn * fPio2_1;
+      float w = fn * fPio2_1t;
+      float t2 = r;
+      w = fn * fPio2_2;
+      r = t2 - w;
+      w = fn * fPio2_2t - ((t2 - r) - w);
+      t2 = r;
+      w = fn * fPio2_3;
+      r = t2 - w;
+      w = fn * fPio2_3t - ((t2 - r) - w);
+      xr[i] = r - w;
+      n[i] = ((int32_t)fn) & 3;
+    }
+  } else if (alld) {
+    constexpr int H = (G >= 4) ? 4 : G;
+    for (int h = 0; h < G; h += H) {
+      double ux[H], uxr[H];
+      int64_t un[H];
+      for (int i = 0; i < H; ++i) ux[i] = (double)x[h + i];
+      reduce_d<H>(ux, uxr, un);
+      for (int i = 0; i < H; ++i) { xr[h + i] = (float)uxr[i]; n[h + i] = (int32_t)un[i]; }
+    }
+  } else {
+    for (int i = 0; i < G; ++i) {
+      float r;
+      n[i] = rem_pio2f_fdlibm(x[i], r);
+      xr[i] = r;
+    }
+  }
+}
+
+// evaluation polynomials: f_trig_evaluation.hpp:35-46, d_trig_evaluation.hpp:33-50
+inline float cos_eval_f(float z) {
+  float y = horner(fCosC, z);
+  return 1.0f + (z * -0.5f + y * (z * z));
+}
+inline float sin_eval_f(float z, float x) {
+  float y1 = horner(fSinC, z);
+  return (y1 * z) * x + x;
+}
+inline double cos_eval_d(double z) { return 1.0 - horner(dCosC, z) * z; }
+inline double sin_eval_d(double z, double x) {
+  double y1 = horner(dSinC, z);
+  return (y1 * z) * x + x;
+}
+
+// kind: 0 = cos, 1 = sin, 2 = sincos (s -> out0, c -> out1)
+// SIMD trig_base: simd/common/impl/trigo/trig_base.hpp:59-71 (cos), :80-91 (sin),
+// :140-156 (sincos); scalar trig_base: scalar/impl/trigo/trig_base.hpp:126-140,
+// :142-156, :183-207 (invalid inputs return NaN first).
+template <int G>
+void trig_group_f(int kind, const float* a0, float* out0, float* out1) {
+  float x[G], xr[G];
+  int32_t n[G];
+  for (int i = 0; i < G; ++i) x[i] = std::fabs(a0[i]);
+  reduce_f<G>(x, xr, n);
+  for (int i = 0; i < G; ++i) {
+    const uint32_t swap = (uint32_t)n[i] & 1u;
+    const float z = xr[i] * xr[i];
+    const float se = sin_eval_f(z, xr[i]);
+    const float ce = cos_eval_f(z);
+    const uint32_t cos_sign = (swap ^ (((uint32_t)n[i] & 2u) >> 1)) << 31;
+    const uint32_t sin_sign = (((uint32_t)n[i] & 2u) << 30) ^ (bits(a0[i]) & 0x80000000u);
+    float c = as_f(bits(swap ? se : ce) ^ cos_sign);
+    float s = as_f(bits(swap ? ce : se) ^ sin_sign);
+    if (G == 1 && !std::isfinite(a0[i])) { c = NAN; s = NAN; }
+    if (kind == 0) out0[i] = c;
+    else if (kind == 1) out0[i] = s;
+    else { out0[i] = s; out1[i] = c; }
+  }
+}
+
+template <int G>
+void trig_group_d(int kind, const double* a0, double* out0, double* out1) {
+  double x[G], xr[G];
+  int64_t n[G];
+  for (int i = 0; i < G; ++i) x[i] = std::fabs(a0[i]);
+  reduce_d<G>(x, xr, n);
+  for (int i = 0; i < G; ++i) {
+    const uint64_t swap = (uint64_t)n[i] & 1u;
+    const double z = xr[i] * xr[i];
+    const double se = sin_eval_d(z, xr[i]);
+    const double ce = cos_eval_d(z);
+    const uint64_t cos_sign = (swap ^ (((uint64_t)n[i] & 2u) >> 1)) << 63;
+    const uint64_t sin_sign = (((uint64_t)n[i] & 2u) << 62) ^ (bits(a0[i]) & 0x8000000000000000ull);
+    double c = as_d(bits(swap ? se : ce) ^ cos_sign);
+    double s = as_d(bits(swap ? ce : se) ^ sin_sign);
+    if (G == 1 && !std::isfinite(a0[i])) { c = NAN; s = NAN; }
+    if (kind == 0) out0[i] = c;
+    else if (kind == 1) out0[i] = s;
+    else { out0[i] = s; out1[i] = c; }
+  }
+}
+
+// ===========================================================================
+// NT2 transform schedule: modules/core/base/include/nt2/core/functions/simd/
+// transform.hpp:40-57 (packs of N from the range start, scalar tail), run in
+// parallel blocks of grain = L2/sizeof(T) when size > 8*grain
+// (modules/arch/shared_memory/include/nt2/core/functions/shared_memory/
+// transform.hpp:36-49, modules/arch/openmp/include/nt2/sdk/openmp/spawner/
+// transform.hpp:44-72).  Grain is a multiple of the pack width, so packs start
+// at multiples of N from index 0 and the last n%N elements use the scalar path.
+// ===========================================================================
+size_t l2_grain(size_t elem) {
+  long l2 = sysconf(_SC_LEVEL2_CACHE_SIZE);
+  if (l2 <= 0) l2 = 256 * 1024;
+  size_t g = (size_t)l2 / elem;
+  return std::max<size_t>(64, g & ~(size_t)63);
+}
+
+template <class Body>
+void nt2_transform(size_t n, size_t N, size_t elem, Body body) {
+  const size_t grain = l2_grain(elem);
+  const size_t packs = n / N;
+  if (n > 8 * grain) {
+    const size_t ppb = grain / N;
+    const size_t nblk = (packs + ppb - 1) / ppb;
+#pragma omp parallel for schedule(static)
+    for (size_t b = 0; b < nblk; ++b) {
+      const size_t p1 = std::min(packs, (b + 1) * ppb);
+      for (size_t p = b * ppb; p < p1; ++p) body(p * N, true);
+    }
+  } else {
+    for (size_t p = 0; p < packs; ++p) body(p * N, true);
+  }
+  for (size_t i = packs * N; i < n; ++i) body(i, false);
+}
+
+int g_threads = 0;
+
+// NT2 SIMD inner_fold of plus over a contiguous table (single thread):
+// modules/core/reduction/include/nt2/core/functions/simd/inner_fold.hpp:44-67,
+// details/fold_step.hpp:31-43, and the AVX butterfly
+// modules/boost/simd/base/include/boost/simd/swar/functions/simd/sse/avx/all_reduce.hpp:52-66
+// which sums 8 lanes as ((l1+l5)+(l3+l7)) + ((l2+l6)+(l0+l4)).
+float hsum8(const float* a) {
+  return ((a[1] + a[5]) + (a[3] + a[7])) + ((a[2] + a[6]) + (a[0] + a[4]));
+}
+double hsum4(const double* a) {  // all_reduce.hpp:29-36
+  return (a[1] + a[3]) + (a[0] + a[2]);
+}
+
+template <class Elem>
+float fold_sum_f32(size_t n, Elem elem) {
+  float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const size_t packs = n / 8;
+  for (size_t p = 0; p < packs; ++p) {
+    float v[8];
+    elem(p * 8, v, 8);
+    for (int l = 0; l < 8; ++l) acc[l] = acc[l] + v[l];
+  }
+  float s = hsum8(acc);
+  for (size_t i = packs * 8; i < n; ++i) {
+    float v;
+    elem(i, &v, 1);
+    s = s + v;
+  }
+  return s;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI used by the tests and by bench.py's CPU baseline
+// ===========================================================================
+extern "C" {
+
+enum {
+  OR_EXP = 0, OR_LOG = 1, OR_SQRT = 2, OR_ERFC = 3, OR_NORMCDF = 4,
+  OR_FASTNORMCDF = 5, OR_COS = 6, OR_SIN = 7, OR_ABS = 8
+};
+
+int oracle_abi_version(void) { return 1; }
+
+// 0 (default): NT2's single-precision Payne-Hanek as written; 1: with the
+// corrected top-bit shift (test-only switch, used to pin the restatement).
+void oracle_set_ph_fix(int on) { g_ph_fix = on ? 1 : 0; }
+
+void oracle_set_threads(int n) {
+  g_threads = n;
+  if (n > 0) omp_set_num_threads(n);
+}
+int oracle_get_threads(void) { return omp_get_max_threads(); }
+
+// y = f(x) over a table of n floats, NT2 transform semantics
+int oracle_map_f32(int op, const float* x, float* y, size_t n) {
+  if (op < 0 || op > OR_ABS) return -1;
+  nt2_transform(n, 8, sizeof(float), [&](size_t i, bool pack) {
+    const size_t w = pack ? 8 : 1;
+    switch (op) {
+      case OR_EXP: for (size_t l = 0; l < w; ++l) y[i + l] = exp_f(x[i + l]); break;
+      case OR_LOG: for (size_t l = 0; l < w; ++l) y[i + l] = log_f(x[i + l]); break;
+      case OR_SQRT: for (size_t l = 0; l < w; ++l) y[i + l] = std::sqrt(x[i + l]); break;
+      case OR_ERFC: for (size_t l = 0; l < w; ++l) y[i + l] = erfc_f(x[i + l]); break;
+      case OR_NORMCDF: for (size_t l = 0; l < w; ++l) y[i + l] = normcdf_f(x[i + l]); break;
+      case OR_FASTNORMCDF: for (size_t l = 0; l < w; ++l) y[i + l] = fastnormcdf_f(x[i + l]); break;
+      case OR_ABS: for (size_t l = 0; l < w; ++l) y[i + l] = std::fabs(x[i + l]); break;
+      case OR_COS:
+      case OR_SIN:
+        if (pack) trig_group_f<8>(op == OR_COS ? 0 : 1, x + i, y + i, nullptr);
+        else trig_group_f<1>(op == OR_COS ? 0 : 1, x + i, y + i, nullptr);
+        break;
+    }
+  });
+  return 0;
+}
+
+int oracle_map_f64(int op, const double* x, double* y, size_t n) {
+  if (op < 0 || op > OR_ABS || op == OR_FASTNORMCDF) return -1;
+  nt2_transform(n, 4, sizeof(double), [&](size_t i, bool pack) {
+    const size_t w = pack ? 4 : 1;
+    switch (op) {
+      case OR_EXP: for (size_t l = 0; l < w; ++l) y[i + l] = exp_d(x[i + l]); break;
+      case OR_LOG: for (size_t l = 0; l < w; ++l) y[i + l] = log_d(x[i + l]); break;
+      case OR_SQRT: for (size_t l = 0; l < w; ++l) y[i + l] = std::sqrt(x[i + l]); break;
+      case OR_ERFC: for (size_t l = 0; l < w; ++l) y[i + l] = erfc_d(x[i + l], !pack); break;
+      case OR_NORMCDF: for (size_t l = 0; l < w; ++l) y[i + l] = normcdf_d(x[i + l], !pack); break;
+      case OR_ABS: for (size_t l = 0; l < w; ++l) y[i + l] = std::fabs(x[i + l]); break;
+      case OR_COS:
+      case OR_SIN:
+        if (pack) trig_group_d<4>(op == OR_COS ? 0 : 1, x + i, y + i, nullptr);
+        else trig_group_d<1>(op == OR_COS ? 0 : 1, x + i, y + i, nullptr);
+        break;
+    }
+  });
+  return 0;
+}
+
+// [s, c] = sincos(x): generic/sincos.hpp:50-64
+int oracle_sincos_f32(const float* x, float* s, float* c, size_t n) {
+  nt2_transform(n, 8, sizeof(float), [&](size_t i, bool pack) {
+    if (pack) trig_group_f<8>(2, x + i, s + i, c + i);
+    else trig_group_f<1>(2, x + i, s + i, c + i);
+  });
+  return 0;
+}
+int oracle_sincos_f64(const double* x, double* s, double* c, size_t n) {
+  nt2_transform(n, 4, sizeof(double), [&](size_t i, bool pack) {
+    if (pack) trig_group_d<4>(2, x + i, s + i, c + i);
+    else trig_group_d<1>(2, x + i, s + i, c + i);
+  });
+  return 0;
+}
+
+// Y = alpha*X + Y: demo/saxpy/nt2/saxpy_nt2.cpp:39-42
+int oracle_saxpy_f32(size_t n, float a, const float* x, float* y) {
+  nt2_transform(n, 8, sizeof(float), [&](size_t i, bool pack) {
+    const size_t w = pack ? 8 : 1;
+    for (size_t l = 0; l < w; ++l) y[i + l] = a * x[i + l] + y[i + l];
+  });
+  return 0;
+}
+
+float oracle_sum_f32(const float* x, size_t n) {
+  return fold_sum_f32(n, [&](size_t i, float* v, int w) {
+    for (int l = 0; l < w; ++l) v[l] = x[i + l];
+  });
+}
+
+double oracle_sum_f64(const double* x, size_t n) {
+  double acc[4] = {0, 0, 0, 0};
+  const size_t packs = n / 4;
+  for (size_t p = 0; p < packs; ++p)
+    for (int l = 0; l < 4; ++l) acc[l] = acc[l] + x[p * 4 + l];
+  double s = hsum4(acc);
+  for (size_t i = packs * 4; i < n; ++i) s = s + x[i];
+  return s;
+}
+
+// res = globalsum(X*Y): demo/sdot/nt2/sdot_nt2.cpp:35-38;
+// dot(a,b) = sum(a*conj(b)): modules/core/reduction/include/nt2/reduction/functions/expr/dot.hpp:22-29
+float oracle_dot_f32(const float* x, const float* y, size_t n) {
+  return fold_sum_f32(n, [&](size_t i, float* v, int w) {
+    for (int l = 0; l < w; ++l) v[l] = x[i + l] * y[i + l];
+  });
+}
+
+// NT2's OpenMP inner_fold for the CPU baseline timing: per-thread pack
+// partials combined under a critical section in arrival order
+// (modules/arch/openmp/include/nt2/sdk/openmp/spawner/fold.hpp:49-66).
+float oracle_dot_f32_omp(const float* x, const float* y, size_t n) {
+  const size_t grain = l2_grain(sizeof(float));
+  if (n <= 8 * grain) return oracle_dot_f32(x, y, n);
+  const size_t packs = n / 8, ppb = grain / 8, nblk = (packs + ppb - 1) / ppb;
+  float total[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma omp parallel
+  {
+    float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+#pragma omp for schedule(static)
+    for (size_t b = 0; b < nblk; ++b) {
+      const size_t p1 = std::min(packs, (b + 1) * ppb);
+      for (size_t p = b * ppb; p < p1; ++p)
+        for (int l = 0; l < 8; ++l) acc[l] = acc[l] + x[p * 8 + l] * y[p * 8 + l];
+    }
+#pragma omp critical
+    for (int l = 0; l < 8; ++l) total[l] = total[l] + acc[l];
+  }
+  float s = hsum8(total);
+  for (size_t i = packs * 8; i < n; ++i) s = s + x[i] * y[i];
+  return s;
+}
+
+// Exact reference for float sums: every float product/value is summed in
+// double-double (Dekker/Knuth TwoSum), returned as the rounded double.
+double oracle_dot_f32_exact(const float* x, const float* y, size_t n) {
+  double hi = 0.0, lo = 0.0;
+  for (size_t i = 0; i < n; ++i) {
+    double p = (double)x[i] * (double)y[i];  // exact: 24+24 bits
+    double s = hi + p, bb = s - hi, e = (hi - (s - bb)) + (p - bb);
+    hi = s;
+    lo += e;
+  }
+  return hi + lo;
+}
+double oracle_sum_f32_exact(const float* x, size_t n) {
+  double hi = 0.0, lo = 0.0;
+  for (size_t i = 0; i < n; ++i) {
+    double p = (double)x[i];
+    double s = hi + p, bb = s - hi, e = (hi - (s - bb)) + (p - bb);
+    hi = s;
+    lo += e;
+  }
+  return hi + lo;
+}
+
+// ---------------------------------------------------------------------------
+// Black-Scholes.  mode 0 = textbook call/put with erfc-based normcdf:
+//   da = sqrt(T); vsT = v*da; d1 = (log(S/X) + fma(sqr(v), 0.5, r)*T)/vsT;
+//   d2 = d1 - vsT; e = X*exp(-r*T);
+//   call = S*normcdf(d1) - e*normcdf(d2); put = e*normcdf(-d2) - S*normcdf(-d1)
+// mode 1 = the NT2 demo formula (demo/blackscholes/nt2/blackandscholes_nt2.cpp:55-59):
+//   d1 = log(S/X) + (fma(sqr(v),0.5,r)*T)/(v*da); d2 = fnms(v,da,d1);
+//   call = fnms(X*exp(-r*T), fastnormcdf(d2), S*fastnormcdf(d1)); put untouched.
+// fma/fnms are NT2's two-rounding forms (generic/fma.hpp:32, generic/fnms.hpp:32).
+// ---------------------------------------------------------------------------
+int oracle_bs_f32(size_t n, const float* S, const float* X, const float* T,
+                  float r, float v, float* call, float* put, int mode) {
+  if (mode != 0 && mode != 1) return -1;
+  const float drift = (v * v) * 0.5f + r;
+  const float mr = -r;
+  nt2_transform(n, 8, sizeof(float), [&](size_t i, bool pack) {
+    const size_t w = pack ? 8 : 1;
+    for (size_t l = 0; l < w; ++l) {
+      const size_t j = i + l;
+      const float da = std::sqrt(T[j]);
+      const float vsT = v * da;
+      const float e = X[j] * exp_f(mr * T[j]);
+      if (mode == 0) {
+        const float d1 = (log_f(S[j] / X[j]) + drift * T[j]) / vsT;
+        const float d2 = d1 - vsT;
+        call[j] = S[j] * normcdf_f(d1) - e * normcdf_f(d2);
+        put[j] = e * normcdf_f(-d2) - S[j] * normcdf_f(-d1);
+      } else {
+        const float d1 = log_f(S[j] / X[j]) + (drift * T[j]) / vsT;
+        const float d2 = d1 - v * da;
+        call[j] = S[j] * fastnormcdf_f(d1) - e * fastnormcdf_f(d2);
+      }
+    }
+  });
+  return 0;
+}
+
+int oracle_bs_f64(size_t n, const double* S, const double* X, const double* T,
+                  double r, double v, double* call, double* put) {
+  const double drift = (v * v) * 0.5 + r;
+  const double mr = -r;
+  nt2_transform(n, 4, sizeof(double), [&](size_t i, bool pack) {
+    const size_t w = pack ? 4 : 1;
+    for (size_t l = 0; l < w; ++l) {
+      const size_t j = i + l;
+      const double da = std::sqrt(T[j]);
+      const double vsT = v * da;
+      const double e = X[j] * exp_d(mr * T[j]);
+      const double d1 = (log_d(S[j] / X[j]) + drift * T[j]) / vsT;
+      const double d2 = d1 - vsT;
+      call[j] = S[j] * normcdf_d(d1, !pack) - e * normcdf_d(d2, !pack);
+      put[j] = e * normcdf_d(-d2, !pack) - S[j] * normcdf_d(-d1, !pack);
+    }
+  });
+  return 0;
+}
+
+// FP64 accuracy reference for Black-Scholes: the textbook formula evaluated in
+// double with the C library's log/exp/erfc (independent of NT2's kernels).
+// Inputs are the f32 table values promoted exactly; outputs stay in double.
+int oracle_bs_fp64ref_f32in(size_t n, const float* S, const float* X, const float* T,
+                            double r, double v, double* call, double* put) {
+#pragma omp parallel for schedule(static)
+  for (size_t j = 0; j < n; ++j) {
+    const double s = S[j], x = X[j], t = T[j];
+    const double vsT = v * std::sqrt(t);
+    const double d1 = (std::log(s / x) + (r + 0.5 * v * v) * t) / vsT;
+    const double d2 = d1 - vsT;
+    const double e = x * std::exp(-r * t);
+    const double n1 = 0.5 * std::erfc(-d1 / std::sqrt(2.0));
+    const double n2 = 0.5 * std::erfc(-d2 / std::sqrt(2.0));
+    const double m1 = 0.5 * std::erfc(d1 / std::sqrt(2.0));
+    const double m2 = 0.5 * std::erfc(d2 / std::sqrt(2.0));
+    call[j] = s * n1 - e * n2;
+    put[j] = e * m2 - s * m1;
+  }
+  return 0;
+}
+int oracle_bs_fp64ref_f64in(size_t n, const double* S, const double* X, const double* T,
+                            double r, double v, double* call, double* put) {
+#pragma omp parallel for schedule(static)
+  for (size_t j = 0; j < n; ++j) {
+    const double s = S[j], x = X[j], t = T[j];
+    const double vsT = v * std::sqrt(t);
+    const double d1 = (std::log(s / x) + (r + 0.5 * v * v) * t) / vsT;
+    const double d2 = d1 - vsT;
+    const double e = x * std::exp(-r * t);
+    call[j] = s * (0.5 * std::erfc(-d1 / std::sqrt(2.0))) - e * (0.5 * std::erfc(-d2 / std::sqrt(2.0)));
+    put[j] = e * (0.5 * std::erfc(d2 / std::sqrt(2.0))) - s * (0.5 * std::erfc(d1 / std::sqrt(2.0)));
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// uint8 ops and sigma-delta
+// adds (u8) = min(255, a+b): arithmetic/functions/scalar/adds.hpp:54-67
+// selinc/seldec = c ? a+1 : a / c ? a-1 : a: boolean/functions/scalar/selinc.hpp:18-29, seldec.hpp:18-29
+// if_zero_else_one(c) = c ? 0 : 1: boolean/functions/scalar/if_zero_else_one.hpp:21-52
+// frame step: demo/sigmadelta/nt2/sigmadelta_nt2.cpp:37-58, driver :100-106
+// mul = adds applied N-1 times (N = 3 reproduces the demo's adds(adds(d,d),d)).
+// ---------------------------------------------------------------------------
+static inline uint8_t adds_u8(uint8_t a, uint8_t b) {
+  unsigned s = (unsigned)a + b;
+  return (uint8_t)(s > 255 ? 255 : s);
+}
+static inline uint8_t subs_u8(uint8_t a, uint8_t b) { return (uint8_t)(a > b ? a - b : 0); }
+
+int oracle_u8_binop(int op, const uint8_t* a, const uint8_t* b, uint8_t* y, size_t n) {
+  for (size_t i = 0; i < n; ++i) {
+    switch (op) {
+      case 0: y[i] = adds_u8(a[i], b[i]); break;
+      case 1: y[i] = subs_u8(a[i], b[i]); break;
+      case 2: y[i] = (uint8_t)(a[i] < b[i] ? a[i] + 1 : a[i]); break;  // selinc(a<b, a)
+      case 3: y[i] = (uint8_t)(a[i] > b[i] ? a[i] - 1 : a[i]); break;  // seldec(a>b, a)
+      case 4: y[i] = (uint8_t)(a[i] < b[i] ? 0 : 1); break;            // if_zero_else_one(a<b)
+      case 5: y[i] = (uint8_t)(std::max(a[i], b[i]) - std::min(a[i], b[i])); break;
+      default: return -1;
+    }
+  }
+  return 0;
+}
+
+int oracle_sigmadelta_u8(size_t npix, int nframes, const uint8_t* frames,
+                         size_t frame_stride, uint8_t* bkg, uint8_t* var,
+                         uint8_t* label, int N, uint32_t* frame_counts) {
+  if (N < 1 || nframes < 1) return -1;
+  if (frame_counts)
+    for (int k = 0; k < nframes; ++k) frame_counts[k] = 0;
+  for (int k = 1; k < nframes; ++k) {
+    const uint8_t* fr = frames + (size_t)k * frame_stride;
+    uint64_t cnt = 0;
+#pragma omp parallel for schedule(static) reduction(+ : cnt)
+    for (size_t i = 0; i < npix; ++i) {
+      uint8_t b = bkg[i];
+      const uint8_t f = fr[i];
+      b = (b < f) ? (uint8_t)(b + 1) : ((b > f) ? (uint8_t)(b - 1) : b);
+      const uint8_t d = (uint8_t)(std::max(b, f) - std::min(b, f));
+      uint8_t m = d;
+      for (int t = 1; t < N; ++t) m = adds_u8(m, d);
+      uint8_t vv = var[i];
+      if (d != 0) vv = (vv < m) ? (uint8_t)(vv + 1) : ((vv > m) ? (uint8_t)(vv - 1) : vv);
+      const uint8_t lab = (d < vv) ? 0 : 1;
+      bkg[i] = b;
+      var[i] = vv;
+      label[i] = lab;
+      cnt += lab;
+    }
+    if (frame_counts) frame_counts[k] = (uint32_t)cnt;
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Batched expression z = exp(-x.*x/2).*cos(y) + sqrt(abs(x)), s = sum(z).
+// Parsed as NT2 would: ((-x).*x)/2, exp, cos, times, plus sqrt(abs(x)).
+// ---------------------------------------------------------------------------
+static inline void expr4_pack(const float* x, const float* y, float* z, int w) {
+  float c[8];
+  if (w == 8) trig_group_f<8>(0, y, c, nullptr);
+  else for (int l = 0; l < w; ++l) trig_group_f<1>(0, y + l, c + l, nullptr);
+  for (int l = 0; l < w; ++l) {
+    const float t = ((-x[l]) * x[l]) / 2.0f;
+    z[l] = exp_f(t) * c[l] + std::sqrt(std::fabs(x[l]));
+  }
+}
+
+int oracle_expr4_f32(const float* x, const float* y, float* z, size_t n) {
+  nt2_transform(n, 8, sizeof(float), [&](size_t i, bool pack) {
+    expr4_pack(x + i, y + i, z + i, pack ? 8 : 1);
+  });
+  return 0;
+}
+
+float oracle_expr4_sum_f32(const float* x, const float* y, size_t n) {
+  return fold_sum_f32(n, [&](size_t i, float* v, int w) { expr4_pack(x + i, y + i, v, w); });
+}
+
+// FP64 reference of the batched expression: z computed in double with the C
+// library from the f32 inputs, summed in double-double.
+double oracle_expr4_sum_fp64(const float* x, const float* y, size_t n) {
+  double hi = 0.0, lo = 0.0;
+  for (size_t i = 0; i < n; ++i) {
+    const double xd = x[i], yd = y[i];
+    const double p = std::exp(-xd * xd / 2.0) * std::cos(yd) + std::sqrt(std::fabs(xd));
+    double s = hi + p, bb = s - hi, e = (hi - (s - bb)) + (p - bb);
+    hi = s;
+    lo += e;
+  }
+  return hi + lo;
+}
+
+}  // extern "C"

@@ -1,0 +1,190 @@
+"""ctypes binding of the oracle (oracle/liboracle.so) for the tests and for
+bench.py's CPU-baseline leg.  The oracle is the checker: nothing in the
+product path imports this module."""
+import ctypes
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
+
+OPS = {"exp": 0, "log": 1, "sqrt": 2, "erfc": 3, "normcdf": 4,
+       "fastnormcdf": 5, "cos": 6, "sin": 7, "abs": 8}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        _lib = ctypes.CDLL(LIB_PATH)
+        P, S = ctypes.c_void_p, ctypes.c_size_t
+        _lib.oracle_map_f32.argtypes = [ctypes.c_int, P, P, S]
+        _lib.oracle_map_f64.argtypes = [ctypes.c_int, P, P, S]
+        _lib.oracle_sincos_f32.argtypes = [P, P, P, S]
+        _lib.oracle_sincos_f64.argtypes = [P, P, P, S]
+        _lib.oracle_saxpy_f32.argtypes = [S, ctypes.c_float, P, P]
+        _lib.oracle_sum_f32.argtypes = [P, S]
+        _lib.oracle_sum_f32.restype = ctypes.c_float
+        _lib.oracle_sum_f64.argtypes = [P, S]
+        _lib.oracle_sum_f64.restype = ctypes.c_double
+        for f in ("oracle_dot_f32", "oracle_dot_f32_omp"):
+            getattr(_lib, f).argtypes = [P, P, S]
+            getattr(_lib, f).restype = ctypes.c_float
+        _lib.oracle_dot_f32_exact.argtypes = [P, P, S]
+        _lib.oracle_dot_f32_exact.restype = ctypes.c_double
+        _lib.oracle_sum_f32_exact.argtypes = [P, S]
+        _lib.oracle_sum_f32_exact.restype = ctypes.c_double
+        _lib.oracle_bs_f32.argtypes = [S, P, P, P, ctypes.c_float, ctypes.c_float, P, P, ctypes.c_int]
+        _lib.oracle_bs_f64.argtypes = [S, P, P, P, ctypes.c_double, ctypes.c_double, P, P]
+        _lib.oracle_bs_fp64ref_f32in.argtypes = [S, P, P, P, ctypes.c_double, ctypes.c_double, P, P]
+        _lib.oracle_bs_fp64ref_f64in.argtypes = [S, P, P, P, ctypes.c_double, ctypes.c_double, P, P]
+        _lib.oracle_u8_binop.argtypes = [ctypes.c_int, P, P, P, S]
+        _lib.oracle_sigmadelta_u8.argtypes = [S, ctypes.c_int, P, S, P, P, P, ctypes.c_int, P]
+        _lib.oracle_expr4_f32.argtypes = [P, P, P, S]
+        _lib.oracle_expr4_sum_f32.argtypes = [P, P, S]
+        _lib.oracle_expr4_sum_f32.restype = ctypes.c_float
+        _lib.oracle_expr4_sum_fp64.argtypes = [P, P, S]
+        _lib.oracle_expr4_sum_fp64.restype = ctypes.c_double
+        _lib.oracle_set_threads.argtypes = [ctypes.c_int]
+        _lib.oracle_get_threads.restype = ctypes.c_int
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def set_ph_fix(on):
+    lib().oracle_set_ph_fix(int(bool(on)))
+
+
+def set_threads(n):
+    lib().oracle_set_threads(int(n))
+
+
+def threads():
+    return lib().oracle_get_threads()
+
+
+def map_f32(op, x):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.empty_like(x)
+    assert lib().oracle_map_f32(OPS[op], _p(x), _p(y), x.size) == 0
+    return y
+
+
+def map_f64(op, x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.empty_like(x)
+    assert lib().oracle_map_f64(OPS[op], _p(x), _p(y), x.size) == 0
+    return y
+
+
+def sincos(x):
+    x = np.ascontiguousarray(x)
+    s, c = np.empty_like(x), np.empty_like(x)
+    f = lib().oracle_sincos_f32 if x.dtype == np.float32 else lib().oracle_sincos_f64
+    assert f(_p(x), _p(s), _p(c), x.size) == 0
+    return s, c
+
+
+def saxpy(a, x, y):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.array(y, dtype=np.float32, copy=True)
+    lib().oracle_saxpy_f32(x.size, a, _p(x), _p(y))
+    return y
+
+
+def sum_f32(x):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    return np.float32(lib().oracle_sum_f32(_p(x), x.size))
+
+
+def dot_f32(x, y, omp=False):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    f = lib().oracle_dot_f32_omp if omp else lib().oracle_dot_f32
+    return np.float32(f(_p(x), _p(y), x.size))
+
+
+def dot_f32_exact(x, y):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    return lib().oracle_dot_f32_exact(_p(x), _p(y), x.size)
+
+
+def sum_f32_exact(x):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    return lib().oracle_sum_f32_exact(_p(x), x.size)
+
+
+def bs_f32(S, X, T, r, v, mode=0):
+    S, X, T = (np.ascontiguousarray(a, dtype=np.float32) for a in (S, X, T))
+    c, p = np.zeros_like(S), np.zeros_like(S)
+    assert lib().oracle_bs_f32(S.size, _p(S), _p(X), _p(T), r, v, _p(c), _p(p), mode) == 0
+    return c, p
+
+
+def bs_f64(S, X, T, r, v):
+    S, X, T = (np.ascontiguousarray(a, dtype=np.float64) for a in (S, X, T))
+    c, p = np.zeros_like(S), np.zeros_like(S)
+    assert lib().oracle_bs_f64(S.size, _p(S), _p(X), _p(T), r, v, _p(c), _p(p)) == 0
+    return c, p
+
+
+def bs_fp64ref(S, X, T, r, v):
+    if S.dtype == np.float32:
+        S, X, T = (np.ascontiguousarray(a, dtype=np.float32) for a in (S, X, T))
+        f = lib().oracle_bs_fp64ref_f32in
+    else:
+        S, X, T = (np.ascontiguousarray(a, dtype=np.float64) for a in (S, X, T))
+        f = lib().oracle_bs_fp64ref_f64in
+    c = np.zeros(S.size)
+    p = np.zeros(S.size)
+    assert f(S.size, _p(S), _p(X), _p(T), r, v, _p(c), _p(p)) == 0
+    return c, p
+
+
+def u8_binop(op, a, b):
+    a = np.ascontiguousarray(a, dtype=np.uint8)
+    b = np.ascontiguousarray(b, dtype=np.uint8)
+    y = np.empty_like(a)
+    assert lib().oracle_u8_binop(op, _p(a), _p(b), _p(y), a.size) == 0
+    return y
+
+
+def sigmadelta(frames, N):
+    """frames: uint8 array [nframes, npix].  Returns bkg, var, label, counts."""
+    frames = np.ascontiguousarray(frames, dtype=np.uint8)
+    nf, npix = frames.shape
+    bkg = frames[0].copy()
+    var = np.ones(npix, dtype=np.uint8)
+    label = np.zeros(npix, dtype=np.uint8)
+    counts = np.zeros(nf, dtype=np.uint32)
+    assert lib().oracle_sigmadelta_u8(npix, nf, _p(frames), npix, _p(bkg), _p(var),
+                                      _p(label), N, _p(counts)) == 0
+    return bkg, var, label, counts
+
+
+def expr4(x, y):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    z = np.empty_like(x)
+    assert lib().oracle_expr4_f32(_p(x), _p(y), _p(z), x.size) == 0
+    return z
+
+
+def expr4_sum(x, y):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    return np.float32(lib().oracle_expr4_sum_f32(_p(x), _p(y), x.size))
+
+
+def expr4_sum_fp64(x, y):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    return lib().oracle_expr4_sum_fp64(_p(x), _p(y), x.size)

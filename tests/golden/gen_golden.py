@@ -1,0 +1,192 @@
+#!/usr/bin/env python3
+"""Writes the golden fixtures under tests/golden/ (run once; outputs committed).
+
+* math_f32.npz / math_f64.npz — inputs and correctly rounded outputs of exp,
+  log, erfc, normcdf, cos and sin, computed with mpmath at 60 significant
+  digits.  Inputs: edge values, the NT2 reduction-range boundaries, and seeded
+  random points spread over each function's domain (including huge
+  arguments for the Payne-Hanek path of cos/sin).
+* bs_golden.npz — Black-Scholes call/put in mpmath for 512 options drawn
+  from the benchmark distribution (f32 inputs promoted exactly).
+* kats.json — the known-answer tests of NT2's own unit tests for this path,
+  transcribed as data (input, expected, tolerance in NT2 ulpdist units, the
+  file:line of the reference test).
+
+The reference's tests hold no vector files for this path; its unit tests hold
+the KATs below, and the rest is fixed by mathematics (mpmath).
+"""
+import json
+import os
+
+import mpmath
+import numpy as np
+
+mpmath.mp.dps = 60
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def round_to(v, dtype):
+    """Correctly round the mpmath value v to float32/float64 (ties to even)."""
+    if mpmath.isnan(v):
+        return dtype(np.nan)
+    if mpmath.isinf(v):
+        return dtype(np.inf if v > 0 else -np.inf)
+    d = float(v)  # correctly rounded double
+    if dtype is np.float64:
+        return np.float64(d)
+    f = np.float32(d)
+    if not np.isfinite(f):
+        return f
+    # the two float32 neighbours bracketing v
+    if mpmath.mpf(float(f)) == v:
+        return f
+    if mpmath.mpf(float(f)) < v:
+        lo, hi = f, np.nextafter(f, np.float32(np.inf))
+    else:
+        lo, hi = np.nextafter(f, np.float32(-np.inf)), f
+    dl = v - mpmath.mpf(float(lo))
+    dh = mpmath.mpf(float(hi)) - v
+    if dl < dh:
+        return np.float32(lo)
+    if dh < dl:
+        return np.float32(hi)
+    return np.float32(lo) if (lo.view(np.uint32) & 1) == 0 else np.float32(hi)
+
+
+FUNCS = {
+    "exp": lambda x: mpmath.exp(x),
+    "log": lambda x: mpmath.log(x) if x > 0 else (mpmath.mpf("-inf") if x == 0 else mpmath.nan),
+    "erfc": lambda x: mpmath.erfc(x),
+    "normcdf": lambda x: mpmath.ncdf(x),
+    "cos": lambda x: mpmath.cos(x),
+    "sin": lambda x: mpmath.sin(x),
+}
+
+
+def inputs(name, dtype, rng):
+    f = dtype
+    big = 3.0e38 if dtype is np.float32 else 1.0e300
+    pts = []
+    if name == "exp":
+        lim = 88.0 if dtype is np.float32 else 708.0
+        pts += list(rng.uniform(-lim, lim, 1200)) + list(rng.uniform(-1, 1, 300))
+        pts += [0.0, 1.0, -1.0, 0.5, -0.5, lim, -lim, 1e-8, -1e-8]
+    elif name == "log":
+        e = rng.uniform(-120 if dtype is np.float32 else -1000, 120 if dtype is np.float32 else 1000, 1200)
+        pts += list(np.exp2(e)) + list(rng.uniform(0.5, 2.0, 300))
+        pts += [1.0, 2.0, 0.5, np.e, 10.0, 0.7071, 1.4142, 1e-30]
+        if dtype is np.float32:
+            pts += [1e-40, 3e-45, 3.0e38]
+    elif name in ("erfc", "normcdf"):
+        pts += list(rng.uniform(-6, 6, 800)) + list(rng.uniform(-1, 1, 300))
+        pts += list(rng.uniform(6, 12 if dtype is np.float32 else 26, 200))
+        pts += [0.0, 0.5, 1.0, 2.0, 2.0 / 3.0, 0.65, 2.2, 6.0, -0.5, -1.0, -2.0]
+        if name == "normcdf":
+            pts += list(rng.uniform(-14, -6, 150))
+    else:  # cos, sin
+        pts += list(rng.uniform(-np.pi / 4, np.pi / 4, 150))
+        pts += list(rng.uniform(-np.pi / 2, np.pi / 2, 150))
+        pts += list(rng.uniform(-20 * np.pi, 20 * np.pi, 200))
+        pts += list(rng.uniform(-201, 201, 200))
+        pts += list(rng.uniform(-1e4, 1e4, 300))
+        pts += list(rng.uniform(-823549, 823549, 200))
+        pts += list(np.exp2(rng.uniform(20, 127 if dtype is np.float32 else 1020, 300)) *
+                    rng.choice([-1.0, 1.0], 300))
+        pts += [0.0, np.pi, -np.pi, np.pi / 2, np.pi / 4, 1.0, 100.0, 1e5, 1e10, big]
+    x = np.array(pts, dtype=np.float64).astype(dtype)
+    return x
+
+
+def build_math(dtype, tag):
+    rng = np.random.default_rng(20240601 if dtype is np.float32 else 20240602)
+    out = {}
+    for name, fn in FUNCS.items():
+        x = inputs(name, dtype, rng)
+        y = np.array([round_to(fn(mpmath.mpf(float(v))), dtype) for v in x], dtype=dtype)
+        out[name + "_x"] = x
+        out[name + "_y"] = y
+    np.savez(os.path.join(HERE, "math_%s.npz" % tag), **out)
+
+
+def build_bs():
+    rng = np.random.default_rng(2)
+    n = 512
+    S = rng.uniform(5, 30, n).astype(np.float32)
+    X = rng.uniform(1, 100, n).astype(np.float32)
+    T = rng.uniform(0.25, 10, n).astype(np.float32)
+    r, v = mpmath.mpf("0.02"), mpmath.mpf("0.30")
+    call = np.empty(n)
+    put = np.empty(n)
+    for i in range(n):
+        s, x, t = (mpmath.mpf(float(a)) for a in (S[i], X[i], T[i]))
+        vst = v * mpmath.sqrt(t)
+        d1 = (mpmath.log(s / x) + (r + v * v / 2) * t) / vst
+        d2 = d1 - vst
+        e = x * mpmath.exp(-r * t)
+        call[i] = float(s * mpmath.ncdf(d1) - e * mpmath.ncdf(d2))
+        put[i] = float(e * mpmath.ncdf(-d2) - s * mpmath.ncdf(-d1))
+    np.savez(os.path.join(HERE, "bs_golden.npz"), S=S, X=X, T=T, r=np.float64(0.02),
+             v=np.float64(0.30), call=call, put=put)
+
+
+# Known answers from NT2's unit tests (value, expected, tolerance in ulpdist)
+KATS = [
+    # modules/core/exponential/unit/simd/exp.cpp:42-49
+    ("exp", "inf", "inf", 0.75, "modules/core/exponential/unit/simd/exp.cpp:42"),
+    ("exp", "-inf", "0", 0.75, "modules/core/exponential/unit/simd/exp.cpp:43"),
+    ("exp", "nan", "nan", 0.75, "modules/core/exponential/unit/simd/exp.cpp:44"),
+    ("exp", "-1", "1/e", 0.75, "modules/core/exponential/unit/simd/exp.cpp:46"),
+    ("exp", "1", "e", 0.75, "modules/core/exponential/unit/simd/exp.cpp:47"),
+    ("exp", "0", "1", 0.75, "modules/core/exponential/unit/simd/exp.cpp:48"),
+    # modules/core/exponential/unit/simd/log.cpp:42-50
+    ("log", "inf", "inf", 0, "modules/core/exponential/unit/simd/log.cpp:42"),
+    ("log", "-inf", "nan", 0, "modules/core/exponential/unit/simd/log.cpp:43"),
+    ("log", "nan", "nan", 0, "modules/core/exponential/unit/simd/log.cpp:44"),
+    ("log", "-1", "nan", 0, "modules/core/exponential/unit/simd/log.cpp:45"),
+    ("log", "0", "-inf", 0, "modules/core/exponential/unit/simd/log.cpp:46"),
+    ("log", "1", "0", 0, "modules/core/exponential/unit/simd/log.cpp:48"),
+    ("log", "e", "1", 1, "modules/core/exponential/unit/simd/log.cpp:49"),
+    # modules/core/euler/unit/simd/erfc.cpp:45-54
+    ("erfc", "inf", "0", 200, "modules/core/euler/unit/simd/erfc.cpp:45"),
+    ("erfc", "nan", "nan", 200, "modules/core/euler/unit/simd/erfc.cpp:46"),
+    ("erfc", "-0", "1", 200, "modules/core/euler/unit/simd/erfc.cpp:48"),
+    ("erfc", "1", "0.157299207050285", 200, "modules/core/euler/unit/simd/erfc.cpp:49"),
+    ("erfc", "0.5", "0.479500122186953", 200, "modules/core/euler/unit/simd/erfc.cpp:50"),
+    ("erfc", "2", "0.004677734981047", 200, "modules/core/euler/unit/simd/erfc.cpp:51"),
+    ("erfc", "10", "0", 200, "modules/core/euler/unit/simd/erfc.cpp:52"),
+    ("erfc", "0", "1", 200, "modules/core/euler/unit/simd/erfc.cpp:53"),
+    # modules/core/statistics/unit/scalar/normcdf.cpp:31-36
+    ("normcdf", "nan", "nan", 0.5, "modules/core/statistics/unit/scalar/normcdf.cpp:31"),
+    ("normcdf", "1", "0.841344746068542948585232545632", 1.5, "modules/core/statistics/unit/scalar/normcdf.cpp:32"),
+    ("normcdf", "-1", "0.158655253931457051414767454368", 6, "modules/core/statistics/unit/scalar/normcdf.cpp:33"),
+    ("normcdf", "0", "0.5", 0.5, "modules/core/statistics/unit/scalar/normcdf.cpp:34"),
+    ("normcdf", "inf", "1", 0.5, "modules/core/statistics/unit/scalar/normcdf.cpp:35"),
+    ("normcdf", "-inf", "0", 0.5, "modules/core/statistics/unit/scalar/normcdf.cpp:36"),
+    # normcdf(-1, 1) = normcdf(-2): normcdf.cpp:49 (the two-argument form shifts by mu)
+    ("normcdf", "-2", "0.0227501319481792072002826371665", 25, "modules/core/statistics/unit/scalar/normcdf.cpp:49"),
+    # modules/core/trigonometric/unit/simd/cos.cpp:50-60
+    ("cos", "inf", "nan", 0.5, "modules/core/trigonometric/unit/simd/cos.cpp:50"),
+    ("cos", "-inf", "nan", 0.5, "modules/core/trigonometric/unit/simd/cos.cpp:51"),
+    ("cos", "nan", "nan", 0.5, "modules/core/trigonometric/unit/simd/cos.cpp:52"),
+    ("cos", "-pi", "-1", 0.5, "modules/core/trigonometric/unit/simd/cos.cpp:54"),
+    ("cos", "-pi/2", "0", 0.5, "modules/core/trigonometric/unit/simd/cos.cpp:55"),
+    ("cos", "-pi/4", "sqrt(2)/2", 0.5, "modules/core/trigonometric/unit/simd/cos.cpp:56"),
+    ("cos", "pi", "-1", 0.5, "modules/core/trigonometric/unit/simd/cos.cpp:57"),
+    ("cos", "pi/2", "0", 0.5, "modules/core/trigonometric/unit/simd/cos.cpp:58"),
+    ("cos", "pi/4", "sqrt(2)/2", 0.5, "modules/core/trigonometric/unit/simd/cos.cpp:59"),
+    ("cos", "0", "1", 0.5, "modules/core/trigonometric/unit/simd/cos.cpp:60"),
+]
+
+
+def build_kats():
+    rows = [dict(func=f, x=x, expected=e, tol_ulpdist=t, source=s) for f, x, e, t, s in KATS]
+    with open(os.path.join(HERE, "kats.json"), "w") as fh:
+        json.dump(rows, fh, indent=1)
+
+
+if __name__ == "__main__":
+    build_math(np.float32, "f32")
+    build_math(np.float64, "f64")
+    build_bs()
+    build_kats()
+    print("golden fixtures written")

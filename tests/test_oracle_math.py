@@ -1,0 +1,199 @@
+"""Pins the oracle's math kernels (CPU restatement of NT2) against the
+reference's own unit-test KATs, mpmath golden vectors, NT2's published
+exhaustive accuracy figures, and structural properties of NT2's SIMD
+evaluation (pack grouping, scalar tail)."""
+import json
+import os
+
+import mpmath
+import numpy as np
+import pytest
+
+import _oracle as O
+from _ulp import classic_ulp, nt2_ulpdist
+
+G = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def _val(s, dt):
+    table = {"inf": np.inf, "-inf": -np.inf, "nan": np.nan, "e": np.e, "1/e": 1 / np.e,
+             "pi": np.pi, "-pi": -np.pi, "pi/2": np.pi / 2, "-pi/2": -np.pi / 2,
+             "pi/4": np.pi / 4, "-pi/4": -np.pi / 4, "sqrt(2)/2": np.sqrt(2) / 2, "-0": -0.0}
+    return dt(table[s]) if s in table else dt(float(s))
+
+
+def _iso(mapf, name, x, w):
+    """Evaluate each input in a pack of identical lanes (NT2's accuracy of the
+    function itself, independent of pack neighbours)."""
+    return mapf(name, np.repeat(x, w))[::w]
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_nt2_unit_kats(dt):
+    kats = json.load(open(os.path.join(G, "kats.json")))
+    mapf = O.map_f32 if dt is np.float32 else O.map_f64
+    W = 8 if dt is np.float32 else 4
+    for k in kats:
+        # NT2 evaluates its constants in the target precision: Pi<vT>()/2 etc.
+        x = _val(k["x"], dt)
+        exp = _val(k["expected"], dt)
+        got = _iso(mapf, k["func"], np.array([x], dtype=dt), W)[0]
+        d = nt2_ulpdist(dt(got), dt(exp))
+        assert d <= k["tol_ulpdist"], (k, got, d)
+
+
+def _bound(name, x, y, dt):
+    """Classic-ULP bounds per function and region (NT2's documented bounds,
+    plus the exp(-x^2) conditioning of erfc tails)."""
+    ax = np.abs(x.astype(np.float64))
+    if name in ("exp", "log"):
+        return np.full(x.shape, 1.0)
+    if name == "erfc":
+        if dt is np.float32:
+            # erf_kernel.hpp:75 documents 2 ulp on [0, 2/3] (measured here: 4
+            # classic ulp) and :51 16 ulp on [2/3, inf); beyond x ~ 3 the f32
+            # rounding of x*x moves exp(-x*x) by ~x^2 ulp.
+            return np.where(ax < 2.0 / 3.0, 4.0, 16.0 + 2.0 * ax * ax)
+        # NT2's f64 erfc cover bound is 12500 ulpdist = 25000 classic ulp
+        # (modules/core/euler/cover/scalar/erfc.cpp:24) on 128 samples; the
+        # rational erfc3 (erf_kernel.hpp:146-166) measures up to 3.5e4 ulp
+        # (4e-12 relative) at the left end of its interval [2.2, 6).  The SIMD
+        # form also uses erfc3 past x = 6 (checked separately, absolutely).
+        return np.where(ax < 2.2, 25000.0, np.where(ax < 6.0, 50000.0, np.inf))
+    if name == "normcdf":
+        if dt is np.float32:
+            return 16.0 + 1.5 * ax * ax
+        return np.where(ax < 2.2 * np.sqrt(2), 25000.0,
+                        np.where(ax < 6.0 * np.sqrt(2), 50000.0, np.inf))
+    # cos/sin: 1 ulp away from zeros; absolute 1 eps of the reduced argument near zeros
+    return np.where(np.abs(y.astype(np.float64)) > 1e-3, 1.0 if dt is np.float32 else 2.0, np.inf)
+
+
+@pytest.mark.parametrize("dt,tag", [(np.float32, "f32"), (np.float64, "f64")])
+@pytest.mark.parametrize("name", ["exp", "log", "erfc", "normcdf", "cos", "sin"])
+def test_golden_mpmath(dt, tag, name):
+    g = np.load(os.path.join(G, "math_%s.npz" % tag))
+    x, y = g[name + "_x"], g[name + "_y"]
+    mapf = O.map_f32 if dt is np.float32 else O.map_f64
+    W = 8 if dt is np.float32 else 4
+    if name in ("cos", "sin") and dt is np.float32:
+        O.set_ph_fix(True)  # NT2's Payne-Hanek shift is tested separately
+    try:
+        got = _iso(mapf, name, x, W)
+    finally:
+        O.set_ph_fix(False)
+    normal = (np.abs(y) >= np.finfo(dt).tiny) | (y == 0) | ~np.isfinite(y)
+    err = classic_ulp(got, y.astype(np.float64), dt)
+    b = _bound(name, x, y, dt)
+    bad = normal & (err > b)
+    assert not bad.any(), list(zip(x[bad][:5], got[bad][:5], y[bad][:5], err[bad][:5]))
+    if name in ("erfc", "normcdf") and dt is np.float64:
+        far = ~np.isfinite(b) & np.isfinite(y)
+        ae = np.abs(got.astype(np.float64) - y.astype(np.float64))
+        assert (ae[far] <= 1e-3 * np.finfo(dt).eps).all()
+    # near zeros of cos/sin: absolute error within a few eps
+    if name in ("cos", "sin"):
+        near = np.abs(y.astype(np.float64)) <= 1e-3
+        ae = np.abs(got.astype(np.float64) - y.astype(np.float64))
+        assert (ae[near] <= 4 * np.finfo(dt).eps).all()
+
+
+def _uniform_bits(lo, hi, n, rng):
+    a = np.float32(lo).view(np.int32)
+    b = np.float32(hi).view(np.int32)
+    return rng.integers(a, b, n, dtype=np.int64).astype(np.int32).view(np.float32)
+
+
+def test_exp_f32_matches_nt2_published_histogram():
+    # f_expo_reduction.hpp:65-67: over every float in [0, 88.3763] and its
+    # negative (2214928753 + 22831063 values): 98.98% exact, the rest within 1 ulp.
+    rng = np.random.default_rng(11)
+    n = 400000
+    x = np.concatenate([_uniform_bits(0.0, 88.3763, n, rng), -_uniform_bits(0.0, 87.3, n, rng)])
+    got = O.map_f32("exp", x)
+    ref = np.exp(x.astype(np.float64)).astype(np.float32)
+    exact = np.mean(got == ref)
+    assert abs(exact - 0.9898) < 0.004, exact
+    assert np.all(classic_ulp(got, ref.astype(np.float64), np.float32) <= 1.0)
+
+
+def test_log_f32_histogram():
+    """f_kernel.hpp:57-60 documents at most 1 classic ulp over [0, 3.40282e38]
+    with 99.86% exact.  In pure float arithmetic this kernel is exact on
+    99.35% of floats (measured); evaluating the same steps with x87 extended
+    intermediates gives 99.91%, so the published share was most likely taken
+    on an extended-precision scalar build.  The bound holds either way."""
+    rng = np.random.default_rng(12)
+    x = _uniform_bits(1.2e-38, 3.4e38, 600000, rng)
+    got = O.map_f32("log", x)
+    ref = np.log(x.astype(np.float64)).astype(np.float32)
+    exact = np.mean(got == ref)
+    assert 0.991 < exact < 0.996, exact
+    assert np.all(classic_ulp(got, ref.astype(np.float64), np.float32) <= 1.0)
+    d = _uniform_bits(1e-45, 1.1e-38, 20000, rng)  # denormal prescale path
+    assert np.all(classic_ulp(O.map_f32("log", d), np.log(d.astype(np.float64)).astype(np.float32).astype(np.float64), np.float32) <= 1.0)
+
+
+def test_f32_payne_hanek_nt2_shift():
+    """Beyond 2^18*pi NT2 reduces single floats with fdlibm's kernel, whose
+    q0 == 0 branch shifts the top chunk by 8 (rem_pio2.hpp:290).  With the
+    shift corrected every result is within 1 ulp (this pins the rest of the
+    restatement); as written, a few percent of inputs lose accuracy."""
+    rng = np.random.default_rng(7)
+    x = (np.exp2(rng.uniform(19.7, 127, 1500)) * rng.choice([-1, 1], 1500)).astype(np.float32)
+    mpmath.mp.dps = 60
+    ref = np.array([float(mpmath.cos(mpmath.mpf(float(v)))) for v in x])
+    O.set_ph_fix(True)
+    try:
+        fixed = _iso(O.map_f32, "cos", x, 8)
+    finally:
+        O.set_ph_fix(False)
+    nt2 = _iso(O.map_f32, "cos", x, 8)
+    ref32 = ref.astype(np.float32).astype(np.float64)
+    assert classic_ulp(fixed, ref32, np.float32).max() <= 1.0
+    e = classic_ulp(nt2, ref32, np.float32)
+    frac_bad = np.mean(e > 1.0)
+    assert 0.005 < frac_bad < 0.10, frac_bad
+
+
+def test_pack_neighbours_choose_the_reduction():
+    """NT2 picks the range reduction by an all-lanes test over the 8-lane pack
+    (trig_reduction.hpp:94-215): a small angle next to one in (pi/4, pi/2]
+    goes through the pi/2 'straight' reduction, which changes its bits."""
+    alone = O.map_f32("cos", np.full(8, 0.03, np.float32))[0]
+    mixed = O.map_f32("cos", np.array([0.03] * 7 + [1.2], np.float32))[0]
+    assert alone != mixed
+    assert alone == np.float32(np.cos(0.03))
+    # the pi/2 reduction leaves |xr| ~ 1.54, outside the polynomial's range
+    assert 1e-5 < abs(float(mixed) - np.cos(0.03)) < 1e-3
+    # the scalar tail evaluates each element alone
+    tail = O.map_f32("cos", np.array([0.5] * 8 + [0.03, 1.2], np.float32))
+    assert tail[8] == alone
+
+
+def test_f64_erfc_scalar_tail_differs_beyond_six():
+    # scalar f64 erfc returns 0 for x >= 6 (scalar/erfc.hpp:52-67); SIMD does not.
+    x = np.array([7.0] * 4 + [7.0], np.float64)
+    y = O.map_f64("erfc", x)
+    assert y[0] > 0 and y[4] == 0.0
+
+
+def test_fastnormcdf_cover_bound():
+    # modules/core/statistics/cover/scalar/fastnormcdf.cpp:25-35 checks 128
+    # random points of [-10, 10] against 2*eps; a dense sweep peaks at 2.5*eps.
+    x = np.linspace(-10, 10, 20001).astype(np.float32)
+    f = O.map_f32("fastnormcdf", x)
+    n = O.map_f32("normcdf", x)
+    d = np.abs(f - n)
+    eps = np.finfo(np.float32).eps
+    assert np.max(d) <= 2.5 * eps
+    assert np.mean(d <= 2 * eps) > 0.99
+
+
+def test_sincos_matches_sin_and_cos():
+    rng = np.random.default_rng(3)
+    for dt in (np.float32, np.float64):
+        x = rng.uniform(-1e4, 1e4, 1003).astype(dt)
+        s, c = O.sincos(x)
+        mapf = O.map_f32 if dt is np.float32 else O.map_f64
+        assert np.array_equal(s, mapf("sin", x)) and np.array_equal(c, mapf("cos", x))
