@@ -1,0 +1,165 @@
+/*
+ * nt2h.h — C ABI of the MI355X (gfx950) architecture site for NT2's table
+ * expression path.  Plain pointers and sizes only; every entry point returns
+ * NT2H_OK (0) or a negative NT2H_E* code and never throws.  Device pointers
+ * are owned by the caller unless they come from nt2h_malloc.  Calls are
+ * asynchronous on the given HIP stream (NULL = the default stream) unless
+ * stated otherwise; nt2h_last_error() returns a per-thread message.
+ *
+ * Reference interfaces replaced (paths relative to the reference tree):
+ *  - transform_(out, in, range):
+ *      modules/arch/shared_memory/include/nt2/core/functions/shared_memory/transform.hpp:26-36
+ *  - inner_fold_(out, in, neutral, bop, uop):
+ *      modules/arch/shared_memory/include/nt2/core/functions/shared_memory/inner_fold.hpp:24-38
+ *  - the site tag and its device buffer:
+ *      modules/arch/cuda/include/nt2/sdk/cuda/cuda.hpp:16-24,
+ *      modules/arch/cuda/include/nt2/sdk/cuda/memory/buffer.hpp:36-242
+ * The C++ front end (nt2_amd/include/nt2/table.hpp) lowers arbitrary
+ * expressions at compile time; the entry points below are the fixed
+ * workloads and runtime services behind it.
+ */
+#ifndef NT2H_H
+#define NT2H_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NT2H_OK 0
+#define NT2H_EINVAL (-1)     /* bad argument (null pointer, size, op code) */
+#define NT2H_EHIP (-2)       /* HIP runtime error */
+#define NT2H_ENOMEM (-3)     /* device allocation failed */
+#define NT2H_EBUDGET (-4)    /* request larger than the server's memory budget */
+#define NT2H_EAGAIN (-5)     /* not ready yet (poll) */
+#define NT2H_ECANCELLED (-6) /* request cancelled */
+#define NT2H_ESTATE (-7)     /* object in the wrong state */
+
+/* evaluation policy of floating-point kernels */
+#define NT2H_EXACT 0 /* NT2's arithmetic: fma = a*b+c with two roundings; bit-identical to the CPU evaluation */
+#define NT2H_FAST 1  /* NT2's fma call sites fused in hardware; within each function's ULP bound */
+
+/* elementwise operations for nt2h_map_* (NT2 functors exp_, log_, sqrt_, erfc_,
+ * normcdf_, fastnormcdf_, cos_, sin_, abs_) */
+#define NT2H_OP_EXP 0
+#define NT2H_OP_LOG 1
+#define NT2H_OP_SQRT 2
+#define NT2H_OP_ERFC 3
+#define NT2H_OP_NORMCDF 4
+#define NT2H_OP_FASTNORMCDF 5 /* single precision only, as in NT2 */
+#define NT2H_OP_COS 6
+#define NT2H_OP_SIN 7
+#define NT2H_OP_ABS 8
+
+/* Black-Scholes formula variants */
+#define NT2H_BS_TEXTBOOK 0 /* d1 = (log(S/X)+(r+v^2/2)T)/(v sqrt T), call and put, erfc-based normcdf */
+#define NT2H_BS_DEMO 1     /* demo/blackscholes/nt2/blackandscholes_nt2.cpp:55-59: call only, fastnormcdf */
+
+/* ---- runtime ----------------------------------------------------------- */
+int nt2h_abi_version(void);
+const char* nt2h_last_error(void);
+/* device properties: compute units, HBM bytes; name may be NULL */
+int nt2h_device_info(int* cus, size_t* hbm_bytes, char* name, size_t name_len);
+
+/* Device storage for tables: the HIP counterpart of cuda_buffer
+ * (modules/arch/cuda/include/nt2/sdk/cuda/memory/buffer.hpp:36-242). */
+int nt2h_malloc(void** dptr, size_t bytes);
+int nt2h_free(void* dptr);
+#define NT2H_H2D 1
+#define NT2H_D2H 2
+#define NT2H_D2D 3
+int nt2h_memcpy(void* dst, const void* src, size_t bytes, int kind, void* stream);
+int nt2h_stream_create(void** stream);
+int nt2h_stream_destroy(void* stream);
+int nt2h_stream_sync(void* stream);
+
+/* ---- seeded synthetic tables (counter-based SplitMix64) ---------------- */
+/* out[i] = (float)(lo + (hi-lo)*u_i), u_i = (splitmix64(seed, i) >> 11) * 2^-53 */
+int nt2h_fill_uniform_f32(float* out, size_t n, double lo, double hi, uint64_t seed, void* stream);
+int nt2h_fill_uniform_f64(double* out, size_t n, double lo, double hi, uint64_t seed, void* stream);
+/* standard normal via Box-Muller in double, rounded to float */
+int nt2h_fill_normal_f32(float* out, size_t n, uint64_t seed, void* stream);
+/* sigma-delta test sequence: nframes frames of npix = h*w uint8 pixels
+ * (column-major, frame k at frames + k*npix): seeded background U{16..200},
+ * nrect moving bright rectangles, uniform noise +-noise, saturated. */
+int nt2h_fill_frames_u8(uint8_t* frames, int h, int w, int nframes, int nrect, int noise,
+                        uint64_t seed, void* stream);
+
+/* ---- transform_: elementwise, NT2 pack semantics ----------------------- */
+/* y = op(x).  Packs of 8 floats / 4 doubles from index 0 as on NT2's AVX site,
+ * the last n%8 (n%4) elements with NT2's scalar kernels (trigonometric range
+ * reduction is chosen per pack, as in NT2). */
+int nt2h_map_f32(int op, size_t n, const float* x, float* y, int policy, void* stream);
+int nt2h_map_f64(int op, size_t n, const double* x, double* y, int policy, void* stream);
+/* [s, c] = sincos(x): modules/core/trigonometric/include/nt2/trigonometric/functions/generic/sincos.hpp:50-64 */
+int nt2h_sincos_f32(size_t n, const float* x, float* s, float* c, int policy, void* stream);
+int nt2h_sincos_f64(size_t n, const double* x, double* s, double* c, int policy, void* stream);
+/* dst = src with 16-byte streaming loads/stores; the HBM bandwidth reference
+ * (grid <= 0 picks one vector per thread).  No counterpart in the reference. */
+int nt2h_stream_copy(const void* src, void* dst, size_t bytes, int grid, void* stream);
+/* Y = a*X + Y: demo/saxpy/nt2/saxpy_nt2.cpp:39-42 */
+int nt2h_saxpy_f32(size_t n, float a, const float* x, float* y, void* stream);
+
+/* ---- inner_fold_: sum / globalsum / dot -------------------------------- */
+/* *out = sum(x) or sum(x.*y) (demo/sdot/nt2/sdot_nt2.cpp:35-38), written to
+ * device memory as double.  Products are rounded to float as in the NT2
+ * expression X*Y; partial sums are compensated FP32 (each addition's rounding
+ * error recovered with TwoSum and accumulated per lane, error-free pairwise
+ * combine across the wave), blocks are combined in FP64 in a fixed order, so
+ * results are deterministic. */
+int nt2h_sum_f32(size_t n, const float* x, double* out, void* stream);
+int nt2h_dot_f32(size_t n, const float* x, const float* y, double* out, void* stream);
+
+/* ---- fused Black-Scholes ---------------------------------------------- */
+/* call/put from tables S, X, T and scalars r, v; one pass, one kernel.
+ * demo mode writes call only (put may be NULL). */
+int nt2h_blackscholes_f32(size_t n, const float* S, const float* X, const float* T, float r,
+                          float v, float* call, float* put, int mode, int policy, void* stream);
+int nt2h_blackscholes_f64(size_t n, const double* S, const double* X, const double* T,
+                          double r, double v, double* call, double* put, int policy,
+                          void* stream);
+
+/* ---- sigma-delta motion detection (uint8, bit-exact) ------------------- */
+/* Runs frames 1..nframes-1 over state bkg/var/label (demo/sigmadelta/nt2/
+ * sigmadelta_nt2.cpp:37-58, driver :100-106) in one fused kernel that keeps
+ * the state in registers; N is the variance factor (mul = adds applied N-1
+ * times).  frame_counts (optional, nframes entries) receives the number of
+ * label==1 pixels after each frame (entry 0 is zeroed). */
+int nt2h_sigmadelta_u8(size_t npix, int nframes, const uint8_t* frames, size_t frame_stride,
+                       uint8_t* bkg, uint8_t* var, uint8_t* label, int N,
+                       uint32_t* frame_counts, void* stream);
+/* One frame step per launch (the plain NT2 schedule, for comparison). */
+int nt2h_sigmadelta_step_u8(size_t npix, const uint8_t* frame, uint8_t* bkg, uint8_t* var,
+                            uint8_t* label, int N, void* stream);
+
+/* ---- batched expression z = exp(-x.*x/2).*cos(y) + sqrt(abs(x)) ------- */
+int nt2h_expr4_f32(size_t n, const float* x, const float* y, float* z, int policy, void* stream);
+/* *out = sum(z) for one table, one launch (the plain schedule) */
+int nt2h_expr4_sum_f32(size_t n, const float* x, const float* y, double* out, int policy,
+                       void* stream);
+
+/* ---- batched evaluation: many uneven tables, one persistent launch ------ */
+/* No counterpart in the reference (NT2 runs one statement per transform_ /
+ * inner_fold_ call); batches the expression above over `count` tables.
+ * x, y, lens: host arrays of device pointers and lengths; the tables must
+ * stay allocated while the batch is used. */
+typedef struct nt2h_batch nt2h_batch;
+int nt2h_batch_create(nt2h_batch** batch, int count, const float* const* x, const float* const* y,
+                      const size_t* lens, int policy);
+/* tile = block*ept elements (ept a multiple of 8), wg_per_cu persistent workgroups per CU */
+int nt2h_batch_set_config(nt2h_batch* batch, int block, int ept, int wg_per_cu);
+int nt2h_batch_get_config(nt2h_batch* batch, int* block, int* ept, int* wg_per_cu);
+/* sums: device array of `count` doubles; replays the captured hipGraph on stream */
+int nt2h_batch_run(nt2h_batch* batch, double* sums, void* stream);
+/* the plain schedule: one nt2h_expr4_sum_f32 launch per table */
+int nt2h_batch_run_plain(nt2h_batch* batch, double* sums, void* stream);
+/* times the candidate configurations and keeps the fastest (synchronous) */
+int nt2h_batch_autotune(nt2h_batch* batch, double* sums, double* best_ms);
+int nt2h_batch_destroy(nt2h_batch* batch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NT2H_H */

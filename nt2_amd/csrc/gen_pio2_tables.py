@@ -9,7 +9,8 @@ oracle (oracle/pio2_tables.h); the layouts follow the fdlibm reduction that
 NT2 embeds in modules/core/trigonometric/include/nt2/trigonometric/functions/
 scalar/rem_pio2.hpp:63-117 (single) and :482-520 (double).
 
-Usage: python3 gen_pio2_tables.py <out.hpp> <namespace>
+Usage: python3 gen_pio2_tables.py <out.hpp> <namespace> [--device]
+With --device the arrays are emitted as __constant__ (HIP constant memory).
 """
 import struct
 import sys
@@ -86,6 +87,7 @@ def npio2_hw_double():
 
 def main():
     out_path, ns = sys.argv[1], sys.argv[2]
+    q = "static __constant__ const" if "--device" in sys.argv[3:] else "static const"
     t8 = two_over_pi_chunks(8, 198)
     t24 = two_over_pi_chunks(24, 66)
     p8 = pio2_chunks(8, 8, 11, "f")
@@ -105,22 +107,22 @@ def main():
         f.write("#pragma once\n#include <stdint.h>\n")
         f.write("namespace %s {\n" % ns)
         f.write("// 2/pi in 8-bit chunks (single-precision Payne-Hanek)\n")
-        f.write("static const int32_t two_over_pi_8[%d] = {\n%s\n};\n" %
+        f.write(q + " int32_t two_over_pi_8[%d] = {\n%s\n};\n" %
                 (len(t8), rows(t8, lambda v: "0x%02X" % v, 12)))
         f.write("// 2/pi in 24-bit chunks (double-precision Payne-Hanek)\n")
-        f.write("static const int32_t two_over_pi_24[%d] = {\n%s\n};\n" %
+        f.write(q + " int32_t two_over_pi_24[%d] = {\n%s\n};\n" %
                 (len(t24), rows(t24, lambda v: "0x%06X" % v, 6)))
         f.write("// pi/2 in 8-bit float chunks\n")
-        f.write("static const uint32_t pio2_8_bits[%d] = {\n%s\n};\n" %
+        f.write(q + " uint32_t pio2_8_bits[%d] = {\n%s\n};\n" %
                 (len(p8), rows([f32_bits(v) for v in p8], lambda v: "0x%08Xu" % v, 4)))
         f.write("// pi/2 in 24-bit double chunks\n")
-        f.write("static const uint64_t pio2_24_bits[%d] = {\n%s\n};\n" %
+        f.write(q + " uint64_t pio2_24_bits[%d] = {\n%s\n};\n" %
                 (len(p24), rows([f64_bits(v) for v in p24], lambda v: "0x%016Xull" % v, 2)))
         f.write("// float(n*pi/2) & 0xffffff00, n = 1..32\n")
-        f.write("static const int32_t npio2_hw_f[32] = {\n%s\n};\n" %
+        f.write(q + " int32_t npio2_hw_f[32] = {\n%s\n};\n" %
                 rows(hw32, lambda v: "0x%08X" % v, 6))
         f.write("// high word of double(n*pi/2), n = 1..32\n")
-        f.write("static const int32_t npio2_hw_d[32] = {\n%s\n};\n" %
+        f.write(q + " int32_t npio2_hw_d[32] = {\n%s\n};\n" %
                 rows(hw64, lambda v: "0x%08X" % v, 6))
         f.write("}  // namespace %s\n" % ns)
 

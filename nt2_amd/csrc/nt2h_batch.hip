@@ -1,0 +1,370 @@
+// Batched evaluation of many uneven tables in one persistent launch.
+//
+// Each table i holds x_i, y_i of len_i floats and the batch computes
+// s_i = sum(exp(-x.*x/2).*cos(y) + sqrt(abs(x))) for every table.  The
+// tables are cut into tiles of `tile` elements (a multiple of the 8-float
+// pack, so NT2's pack grouping restarts at each table start); a persistent
+// grid pulls tile indices from a device work queue, reduces each tile to a
+// compensated FP32 partial, and the block that completes a table's last tile
+// sums that table's partials in tile order in FP64 (deterministic).  The reset
+// kernel and the persistent kernel are captured once in a hipGraph and
+// replayed.  An autotuner picks block size, elements per thread and
+// workgroups per CU by timing the candidates.
+//
+// No counterpart in the reference: NT2 evaluates one expression per
+// statement; this is the batched form of the same
+// inner_fold_ over the same transform.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+#include "nt2/hip/math.hpp"
+#include "nt2h.h"
+
+#pragma clang fp contract(off)
+
+using nt2::hip::exact_policy;
+using nt2::hip::fast_policy;
+namespace dm = nt2::hip::dm;
+
+namespace nt2h {
+
+struct TableDesc {
+  const float* x;
+  const float* y;
+  uint64_t len;
+  uint32_t first_tile;
+  uint32_t ntiles;
+  uint32_t vec;  // both pointers 16-byte aligned
+  uint32_t pad;
+};
+
+template <class P, int G>
+__device__ __forceinline__ void expr4_g(const float* x, const float* y, float* z) {
+  _Pragma("clang fp contract(off)")
+  float c[G];
+  dm::trig_group_f<P, G>(dm::k_cos, y, c, nullptr);
+#pragma unroll
+  for (int l = 0; l < G; ++l) {
+    const float t = ((-x[l]) * x[l]) / 2.0f;
+    z[l] = dm::exp_f<P>(t) * c[l] + __builtin_sqrtf(__builtin_fabsf(x[l]));
+  }
+}
+
+__global__ void batch_reset_k(unsigned* queue, unsigned* done, int ntables) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *queue = 0u;
+  if (i < ntables) done[i] = 0u;
+}
+
+template <class P, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void batch_persistent_k(
+    const TableDesc* __restrict__ tables, const uint32_t* __restrict__ tile_table,
+    uint32_t ntiles, uint32_t tile, unsigned* queue, unsigned* done,
+    double* __restrict__ tile_partial, double* __restrict__ sums) {
+  __shared__ unsigned cur;
+  __shared__ double wsum[BLOCK / 64];
+  __shared__ bool last;
+  for (;;) {
+    if (threadIdx.x == 0) cur = atomicAdd(queue, 1u);
+    __syncthreads();
+    const unsigned t = cur;
+    __syncthreads();
+    if (t >= ntiles) break;  // uniform across the block: every wave exits
+    const uint32_t ti = tile_table[t];
+    const TableDesc d = tables[ti];
+    const uint64_t start = (uint64_t)(t - d.first_tile) * tile;
+    const uint64_t end = (start + tile < d.len) ? start + tile : d.len;
+    const uint64_t full_end = start + ((end - start) / 8) * 8;
+    float s = 0.0f, c = 0.0f;
+    for (uint64_t i = start + 8ull * threadIdx.x; i < full_end; i += 8ull * BLOCK) {
+      float a[8], b[8], z[8];
+      if (d.vec) {
+        const float4 a0 = *reinterpret_cast<const float4*>(d.x + i);
+        const float4 a1 = *reinterpret_cast<const float4*>(d.x + i + 4);
+        const float4 b0 = *reinterpret_cast<const float4*>(d.y + i);
+        const float4 b1 = *reinterpret_cast<const float4*>(d.y + i + 4);
+        a[0] = a0.x; a[1] = a0.y; a[2] = a0.z; a[3] = a0.w;
+        a[4] = a1.x; a[5] = a1.y; a[6] = a1.z; a[7] = a1.w;
+        b[0] = b0.x; b[1] = b0.y; b[2] = b0.z; b[3] = b0.w;
+        b[4] = b1.x; b[5] = b1.y; b[6] = b1.z; b[7] = b1.w;
+      } else {
+#pragma unroll
+        for (int l = 0; l < 8; ++l) { a[l] = d.x[i + l]; b[l] = d.y[i + l]; }
+      }
+      expr4_g<P, 8>(a, b, z);
+#pragma unroll
+      for (int l = 0; l < 8; ++l) comp_add(s, c, z[l]);
+    }
+    // the table's last n%8 elements use NT2's scalar path
+    for (uint64_t i = full_end + threadIdx.x; i < end; i += BLOCK) {
+      float a = d.x[i], b = d.y[i], z;
+      expr4_g<P, 1>(&a, &b, &z);
+      comp_add(s, c, z);
+    }
+    // block reduction (error-free wave merge, FP64 across waves in order)
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+      const float s2 = __shfl_xor(s, off, 64);
+      const float c2 = __shfl_xor(c, off, 64);
+      comp_merge(s, c, s2, c2);
+    }
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = (double)s + (double)c;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double v = 0.0;
+#pragma unroll
+      for (int w = 0; w < BLOCK / 64; ++w) v += wsum[w];
+      tile_partial[t] = v;
+      __threadfence();
+      const unsigned prev = atomicAdd(&done[ti], 1u);
+      last = (prev + 1 == d.ntiles);
+    }
+    __syncthreads();
+    if (last && threadIdx.x == 0) {
+      __threadfence();
+      double v = 0.0;
+      for (uint32_t k = 0; k < d.ntiles; ++k)
+        v += __builtin_nontemporal_load(&tile_partial[d.first_tile + k]);
+      sums[ti] = v;
+    }
+    __syncthreads();
+  }
+}
+
+struct BatchConfig {
+  int block;       // threads per workgroup
+  int ept;         // elements per thread per tile (multiple of 8)
+  int wg_per_cu;   // persistent workgroups per compute unit
+};
+
+struct Batch {
+  int count = 0;
+  int policy = NT2H_FAST;
+  std::vector<const float*> hx, hy;
+  std::vector<uint64_t> hlen;
+  BatchConfig cfg{256, 16, 8};
+  TableDesc* tables = nullptr;
+  uint32_t* tile_table = nullptr;
+  double* tile_partial = nullptr;
+  unsigned* queue = nullptr;
+  unsigned* done = nullptr;
+  uint32_t ntiles = 0;
+  size_t tile_cap = 0;
+  int cus = 256;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  double* graph_sums = nullptr;
+  hipStream_t graph_stream = nullptr;
+};
+
+static int layout(Batch* b) {
+  const uint32_t tile = (uint32_t)(b->cfg.block * b->cfg.ept);
+  std::vector<TableDesc> td(b->count);
+  std::vector<uint32_t> tt;
+  uint64_t total_tiles = 0;
+  for (int i = 0; i < b->count; ++i) total_tiles += (b->hlen[i] + tile - 1) / tile;
+  if (total_tiles >= 0xffffffffull) return fail(NT2H_EINVAL, "nt2h_batch: too many tiles");
+  tt.reserve(total_tiles);
+  for (int i = 0; i < b->count; ++i) {
+    const uint32_t nt = (uint32_t)((b->hlen[i] + tile - 1) / tile);
+    td[i] = TableDesc{b->hx[i], b->hy[i], b->hlen[i], (uint32_t)tt.size(), nt,
+                      (uint32_t)(aligned16(b->hx[i]) && aligned16(b->hy[i])), 0u};
+    for (uint32_t k = 0; k < nt; ++k) tt.push_back((uint32_t)i);
+  }
+  b->ntiles = (uint32_t)tt.size();
+  if (!b->tables) NT2H_CHECK(hipMalloc(&b->tables, sizeof(TableDesc) * b->count), "batch alloc");
+  if (!b->done) NT2H_CHECK(hipMalloc(&b->done, sizeof(unsigned) * b->count), "batch alloc");
+  if (!b->queue) NT2H_CHECK(hipMalloc(&b->queue, sizeof(unsigned)), "batch alloc");
+  if (b->tile_cap < tt.size()) {
+    if (b->tile_table) { NT2H_CHECK(hipFree(b->tile_table), "batch free"); b->tile_table = nullptr; }
+    if (b->tile_partial) { NT2H_CHECK(hipFree(b->tile_partial), "batch free"); b->tile_partial = nullptr; }
+    const size_t cap = std::max<size_t>(tt.size(), 1);
+    NT2H_CHECK(hipMalloc(&b->tile_table, sizeof(uint32_t) * cap), "batch alloc");
+    NT2H_CHECK(hipMalloc(&b->tile_partial, sizeof(double) * cap), "batch alloc");
+    b->tile_cap = cap;
+  }
+  NT2H_CHECK(hipMemcpy(b->tables, td.data(), sizeof(TableDesc) * b->count, hipMemcpyHostToDevice), "batch upload");
+  if (!tt.empty())
+    NT2H_CHECK(hipMemcpy(b->tile_table, tt.data(), sizeof(uint32_t) * tt.size(), hipMemcpyHostToDevice), "batch upload");
+  return NT2H_OK;
+}
+
+template <class P, int BLOCK>
+static void launch_persistent(Batch* b, double* sums, hipStream_t s) {
+  const unsigned grid = (unsigned)std::max(1, std::min<int>(b->cus * b->cfg.wg_per_cu, (int)b->ntiles));
+  hipLaunchKernelGGL((batch_persistent_k<P, BLOCK>), dim3(grid), dim3(BLOCK), 0, s, b->tables,
+                     b->tile_table, b->ntiles, (uint32_t)(BLOCK * b->cfg.ept), b->queue, b->done,
+                     b->tile_partial, sums);
+}
+
+static int enqueue(Batch* b, double* sums, hipStream_t s) {
+  const int rb = 256;
+  hipLaunchKernelGGL(batch_reset_k, dim3((b->count + rb - 1) / rb), dim3(rb), 0, s, b->queue, b->done, b->count);
+  const bool fast = b->policy == NT2H_FAST;
+  switch (b->cfg.block) {
+    case 128: fast ? launch_persistent<fast_policy, 128>(b, sums, s) : launch_persistent<exact_policy, 128>(b, sums, s); break;
+    case 256: fast ? launch_persistent<fast_policy, 256>(b, sums, s) : launch_persistent<exact_policy, 256>(b, sums, s); break;
+    case 512: fast ? launch_persistent<fast_policy, 512>(b, sums, s) : launch_persistent<exact_policy, 512>(b, sums, s); break;
+    default: return fail(NT2H_EINVAL, "nt2h_batch: unsupported block size");
+  }
+  NT2H_CHECK_LAUNCH("batch_persistent");
+  return NT2H_OK;
+}
+
+static int drop_graph(Batch* b) {
+  if (b->exec) { NT2H_CHECK(hipGraphExecDestroy(b->exec), "graph destroy"); b->exec = nullptr; }
+  if (b->graph) { NT2H_CHECK(hipGraphDestroy(b->graph), "graph destroy"); b->graph = nullptr; }
+  b->graph_sums = nullptr;
+  return NT2H_OK;
+}
+
+// capture reset + persistent launch once per (config, output buffer)
+static int ensure_graph(Batch* b, double* sums) {
+  if (b->exec && b->graph_sums == sums) return NT2H_OK;
+  int rc = drop_graph(b);
+  if (rc) return rc;
+  if (!b->graph_stream) NT2H_CHECK(hipStreamCreateWithFlags(&b->graph_stream, hipStreamNonBlocking), "graph stream");
+  NT2H_CHECK(hipStreamBeginCapture(b->graph_stream, hipStreamCaptureModeThreadLocal), "begin capture");
+  rc = enqueue(b, sums, b->graph_stream);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(b->graph_stream, &g);
+  if (rc) { if (g) (void)hipGraphDestroy(g); return rc; }
+  if (e != hipSuccess) return hip_fail(e, "end capture");
+  b->graph = g;
+  NT2H_CHECK(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0), "graph instantiate");
+  b->graph_sums = sums;
+  return NT2H_OK;
+}
+
+}  // namespace nt2h
+
+using namespace nt2h;
+
+extern "C" {
+
+/* Create a batch over `count` tables (host arrays of device pointers and
+ * lengths).  The tables must stay allocated while the batch is used. */
+int nt2h_batch_create(nt2h_batch** out, int count, const float* const* x, const float* const* y,
+                      const size_t* lens, int policy) {
+  if (!out || count <= 0 || !x || !y || !lens) return fail(NT2H_EINVAL, "nt2h_batch_create: bad argument");
+  if (policy != NT2H_EXACT && policy != NT2H_FAST) return fail(NT2H_EINVAL, "nt2h_batch_create: bad policy");
+  Batch* b = new Batch();
+  b->count = count;
+  b->policy = policy;
+  for (int i = 0; i < count; ++i) {
+    if ((!x[i] || !y[i]) && lens[i]) { delete b; return fail(NT2H_EINVAL, "nt2h_batch_create: null table"); }
+    b->hx.push_back(x[i]);
+    b->hy.push_back(y[i]);
+    b->hlen.push_back(lens[i]);
+  }
+  int dev = 0;
+  hipDeviceProp_t p;
+  if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess)
+    b->cus = p.multiProcessorCount;
+  int rc = layout(b);
+  if (rc) { delete b; return rc; }
+  *out = reinterpret_cast<nt2h_batch*>(b);
+  return NT2H_OK;
+}
+
+int nt2h_batch_set_config(nt2h_batch* hb, int block, int ept, int wg_per_cu) {
+  Batch* b = reinterpret_cast<Batch*>(hb);
+  if (!b || (block != 128 && block != 256 && block != 512) || ept < 8 || ept % 8 || ept > 256 ||
+      wg_per_cu < 1 || wg_per_cu > 32)
+    return fail(NT2H_EINVAL, "nt2h_batch_set_config: bad argument");
+  b->cfg = BatchConfig{block, ept, wg_per_cu};
+  int rc = drop_graph(b);
+  if (rc) return rc;
+  return layout(b);
+}
+
+int nt2h_batch_get_config(nt2h_batch* hb, int* block, int* ept, int* wg_per_cu) {
+  Batch* b = reinterpret_cast<Batch*>(hb);
+  if (!b) return fail(NT2H_EINVAL, "nt2h_batch_get_config: null batch");
+  if (block) *block = b->cfg.block;
+  if (ept) *ept = b->cfg.ept;
+  if (wg_per_cu) *wg_per_cu = b->cfg.wg_per_cu;
+  return NT2H_OK;
+}
+
+/* sums: device array of `count` doubles.  Replays the captured graph on
+ * `stream` (captured on a private stream the first time, and again when the
+ * configuration or the output buffer changes). */
+int nt2h_batch_run(nt2h_batch* hb, double* sums, void* stream) {
+  Batch* b = reinterpret_cast<Batch*>(hb);
+  if (!b || !sums) return fail(NT2H_EINVAL, "nt2h_batch_run: bad argument");
+  int rc = ensure_graph(b, sums);
+  if (rc) return rc;
+  NT2H_CHECK(hipGraphLaunch(b->exec, as_stream(stream)), "graph launch");
+  return NT2H_OK;
+}
+
+/* The plain schedule: one reduction launch per table. */
+int nt2h_batch_run_plain(nt2h_batch* hb, double* sums, void* stream) {
+  Batch* b = reinterpret_cast<Batch*>(hb);
+  if (!b || !sums) return fail(NT2H_EINVAL, "nt2h_batch_run_plain: bad argument");
+  for (int i = 0; i < b->count; ++i) {
+    int rc = nt2h_expr4_sum_f32(b->hlen[i], b->hx[i], b->hy[i], sums + i, b->policy, stream);
+    if (rc) return rc;
+  }
+  return NT2H_OK;
+}
+
+/* Time each candidate configuration (3 graph replays, HIP events) and keep
+ * the fastest; *best_ms (optional, host) receives its time per run. */
+int nt2h_batch_autotune(nt2h_batch* hb, double* sums, double* best_ms) {
+  Batch* b = reinterpret_cast<Batch*>(hb);
+  if (!b || !sums) return fail(NT2H_EINVAL, "nt2h_batch_autotune: bad argument");
+  const BatchConfig cands[] = {{256, 16, 4}, {256, 16, 8}, {256, 32, 8}, {256, 64, 4},
+                               {512, 16, 4}, {512, 32, 2}, {128, 32, 8}, {256, 8, 8}};
+  hipStream_t s;
+  NT2H_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "autotune stream");
+  hipEvent_t e0, e1;
+  NT2H_CHECK(hipEventCreate(&e0), "event");
+  NT2H_CHECK(hipEventCreate(&e1), "event");
+  float best = 1e30f;
+  BatchConfig bestc = b->cfg;
+  int rc = NT2H_OK;
+  for (const BatchConfig& c : cands) {
+    if ((rc = nt2h_batch_set_config(hb, c.block, c.ept, c.wg_per_cu))) break;
+    if ((rc = nt2h_batch_run(hb, sums, s))) break;  // warm-up, capture
+    NT2H_CHECK(hipEventRecord(e0, s), "event");
+    for (int r = 0; r < 3; ++r)
+      if ((rc = nt2h_batch_run(hb, sums, s))) break;
+    if (rc) break;
+    NT2H_CHECK(hipEventRecord(e1, s), "event");
+    NT2H_CHECK(hipEventSynchronize(e1), "event");
+    float ms = 0.0f;
+    NT2H_CHECK(hipEventElapsedTime(&ms, e0, e1), "event");
+    ms /= 3.0f;
+    if (ms < best) { best = ms; bestc = c; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipStreamDestroy(s);
+  if (rc) return rc;
+  rc = nt2h_batch_set_config(hb, bestc.block, bestc.ept, bestc.wg_per_cu);
+  if (best_ms) *best_ms = best;
+  return rc;
+}
+
+int nt2h_batch_destroy(nt2h_batch* hb) {
+  Batch* b = reinterpret_cast<Batch*>(hb);
+  if (!b) return NT2H_OK;
+  (void)hipDeviceSynchronize();
+  drop_graph(b);
+  if (b->graph_stream) (void)hipStreamDestroy(b->graph_stream);
+  (void)hipFree(b->tables);
+  (void)hipFree(b->tile_table);
+  (void)hipFree(b->tile_partial);
+  (void)hipFree(b->queue);
+  (void)hipFree(b->done);
+  delete b;
+  return NT2H_OK;
+}
+
+}  // extern "C"

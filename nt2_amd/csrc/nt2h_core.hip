@@ -1,0 +1,1010 @@
+// libnt2h core: the HIP architecture site's kernels for NT2's transform_ and
+// inner_fold_ over tables, the fused Black-Scholes and sigma-delta workloads,
+// the seeded synthetic-table generators and the runtime services of the C ABI.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+
+#include "common.hpp"
+#include "nt2/hip/math.hpp"
+#include "nt2h.h"
+
+#pragma clang fp contract(off)
+
+using nt2::hip::exact_policy;
+using nt2::hip::fast_policy;
+namespace dm = nt2::hip::dm;
+
+namespace nt2h {
+
+// ============================================================================
+// errors
+// ============================================================================
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+int fail(int code, const char* what) {
+  set_error(what);
+  return code;
+}
+int hip_fail(hipError_t e, const char* what) {
+  set_error(std::string(what) + ": " + hipGetErrorString(e));
+  return e == hipErrorOutOfMemory ? NT2H_ENOMEM : NT2H_EHIP;
+}
+
+// ============================================================================
+// reduction scratch, one per (device, stream)
+// ============================================================================
+static std::mutex g_scratch_mu;
+static std::map<std::pair<int, void*>, Scratch> g_scratch;
+
+int get_scratch(size_t nblocks, Scratch* out, hipStream_t s) {
+  int dev = 0;
+  NT2H_CHECK(hipGetDevice(&dev), "hipGetDevice");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  Scratch& sc = g_scratch[{dev, (void*)s}];
+  if (sc.cap < nblocks) {
+    if (sc.partial) {
+      NT2H_CHECK(hipStreamSynchronize(s), "scratch sync");
+      NT2H_CHECK(hipFree(sc.partial), "scratch free");
+      NT2H_CHECK(hipFree(sc.counter), "scratch free");
+      sc.partial = nullptr;
+      sc.counter = nullptr;
+    }
+    const size_t cap = nblocks < kMaxFoldBlocks ? kMaxFoldBlocks : nblocks;
+    NT2H_CHECK(hipMalloc(&sc.partial, cap * sizeof(double)), "scratch alloc");
+    NT2H_CHECK(hipMalloc(&sc.counter, sizeof(unsigned)), "scratch alloc");
+    NT2H_CHECK(hipMemsetAsync(sc.counter, 0, sizeof(unsigned), s), "scratch init");
+    NT2H_CHECK(hipStreamSynchronize(s), "scratch init");
+    sc.cap = cap;
+  }
+  *out = sc;
+  return NT2H_OK;
+}
+
+// ============================================================================
+// generators: SplitMix64 stream out_i = mix(seed + (i+1)*golden)
+// ============================================================================
+__host__ __device__ inline uint64_t splitmix64_at(uint64_t seed, uint64_t i) {
+  uint64_t z = seed + (i + 1) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__host__ __device__ inline double unit_at(uint64_t seed, uint64_t i) {
+  return (double)(splitmix64_at(seed, i) >> 11) * 0x1.0p-53;
+}
+
+template <class T>
+__global__ void fill_uniform_k(T* out, size_t n, double lo, double hi, uint64_t seed) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    out[i] = (T)(lo + (hi - lo) * unit_at(seed, i));
+}
+
+__global__ void fill_normal_k(float* out, size_t n, uint64_t seed) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    const uint64_t p = i >> 1;
+    const double u1 = unit_at(seed, 2 * p), u2 = unit_at(seed, 2 * p + 1);
+    const double r = sqrt(-2.0 * log(1.0 - u1));
+    const double a = 6.283185307179586 * u2;
+    out[i] = (float)((i & 1) ? r * sin(a) : r * cos(a));
+  }
+}
+
+struct Rect {
+  int x0, y0, w, h, vx, vy, val;
+};
+struct Rects {
+  Rect r[8];
+  int n;
+};
+
+__global__ void fill_frames_k(uint8_t* frames, int h, int w, int nframes, Rects rects, int noise,
+                              uint64_t seed) {
+  const size_t npix = (size_t)h * w;
+  const size_t total = npix * (size_t)nframes;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t idx = (size_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total; idx += stride) {
+    const int k = (int)(idx / npix);
+    const size_t p = idx - (size_t)k * npix;
+    const int i = (int)(p % h), j = (int)(p / h);  // column-major (row i, column j)
+    int v = 16 + (int)(splitmix64_at(seed, p) % 185u);
+    for (int q = 0; q < rects.n; ++q) {
+      const Rect& R = rects.r[q];
+      const int cx = (((R.x0 + R.vx * k) % w) + w) % w;
+      const int cy = (((R.y0 + R.vy * k) % h) + h) % h;
+      const int dx = ((j - cx) % w + w) % w, dy = ((i - cy) % h + h) % h;
+      if (dx < R.w && dy < R.h) v = R.val > v ? R.val : v;
+    }
+    if (noise > 0) {
+      const int nz = (int)(splitmix64_at(seed ^ 0xA5A5A5A5A5A5A5A5ull, idx) % (uint64_t)(2 * noise + 1)) - noise;
+      v += nz;
+    }
+    frames[idx] = (uint8_t)(v < 0 ? 0 : (v > 255 ? 255 : v));
+  }
+}
+
+// ============================================================================
+// transform_: elementwise maps with NT2 pack semantics.  Work item t < npacks
+// is the pack [W*t, W*t+W); items beyond are the scalar tail elements.
+// ============================================================================
+template <class P, int OP, int G>
+__device__ __forceinline__ void apply_f32(const float* x, float* y) {
+  if constexpr (OP == NT2H_OP_COS || OP == NT2H_OP_SIN) {
+    dm::trig_group_f<P, G>(OP == NT2H_OP_COS ? dm::k_cos : dm::k_sin, x, y, nullptr);
+  } else {
+#pragma unroll
+    for (int l = 0; l < G; ++l) {
+      const float a = x[l];
+      float r;
+      if constexpr (OP == NT2H_OP_EXP) r = dm::exp_f<P>(a);
+      else if constexpr (OP == NT2H_OP_LOG) r = dm::log_f<P>(a);
+      else if constexpr (OP == NT2H_OP_SQRT) r = __builtin_sqrtf(a);
+      else if constexpr (OP == NT2H_OP_ERFC) r = dm::erfc_f<P>(a);
+      else if constexpr (OP == NT2H_OP_NORMCDF) r = dm::normcdf_f<P>(a);
+      else if constexpr (OP == NT2H_OP_FASTNORMCDF) r = dm::fastnormcdf_f<P>(a);
+      else r = __builtin_fabsf(a);
+      y[l] = r;
+    }
+  }
+}
+
+template <class P, int OP, int G>
+__device__ __forceinline__ void apply_f64(const double* x, double* y) {
+  constexpr bool scalar = (G == 1);
+  if constexpr (OP == NT2H_OP_COS || OP == NT2H_OP_SIN) {
+    dm::trig_group_d<P, G>(OP == NT2H_OP_COS ? dm::k_cos : dm::k_sin, x, y, nullptr);
+  } else {
+#pragma unroll
+    for (int l = 0; l < G; ++l) {
+      const double a = x[l];
+      double r;
+      if constexpr (OP == NT2H_OP_EXP) r = dm::exp_d<P>(a);
+      else if constexpr (OP == NT2H_OP_LOG) r = dm::log_d<P>(a);
+      else if constexpr (OP == NT2H_OP_SQRT) r = __builtin_sqrt(a);
+      else if constexpr (OP == NT2H_OP_ERFC) r = dm::erfc_d<P, scalar>(a);
+      else if constexpr (OP == NT2H_OP_NORMCDF) r = dm::normcdf_d<P, scalar>(a);
+      else r = __builtin_fabs(a);
+      y[l] = r;
+    }
+  }
+}
+
+template <class P, int OP>
+__global__ __launch_bounds__(kBlock) void map_f32_k(const float* __restrict__ x,
+                                                    float* __restrict__ y, size_t n, int vec) {
+  const size_t npacks = n / 8;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < npacks) {
+    float a[8], r[8];
+    const size_t i = t * 8;
+    if (vec) {
+      const float4 v0 = *reinterpret_cast<const float4*>(x + i);
+      const float4 v1 = *reinterpret_cast<const float4*>(x + i + 4);
+      a[0] = v0.x; a[1] = v0.y; a[2] = v0.z; a[3] = v0.w;
+      a[4] = v1.x; a[5] = v1.y; a[6] = v1.z; a[7] = v1.w;
+    } else {
+#pragma unroll
+      for (int l = 0; l < 8; ++l) a[l] = x[i + l];
+    }
+    apply_f32<P, OP, 8>(a, r);
+    if (vec) {
+      *reinterpret_cast<float4*>(y + i) = make_float4(r[0], r[1], r[2], r[3]);
+      *reinterpret_cast<float4*>(y + i + 4) = make_float4(r[4], r[5], r[6], r[7]);
+    } else {
+#pragma unroll
+      for (int l = 0; l < 8; ++l) y[i + l] = r[l];
+    }
+  } else if (t < npacks + (n - npacks * 8)) {
+    const size_t i = npacks * 8 + (t - npacks);
+    float a = x[i], r;
+    apply_f32<P, OP, 1>(&a, &r);
+    y[i] = r;
+  }
+}
+
+template <class P, int OP>
+__global__ __launch_bounds__(kBlock) void map_f64_k(const double* __restrict__ x,
+                                                    double* __restrict__ y, size_t n, int vec) {
+  const size_t npacks = n / 4;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < npacks) {
+    double a[4], r[4];
+    const size_t i = t * 4;
+    if (vec) {
+      const double2 v0 = *reinterpret_cast<const double2*>(x + i);
+      const double2 v1 = *reinterpret_cast<const double2*>(x + i + 2);
+      a[0] = v0.x; a[1] = v0.y; a[2] = v1.x; a[3] = v1.y;
+    } else {
+#pragma unroll
+      for (int l = 0; l < 4; ++l) a[l] = x[i + l];
+    }
+    apply_f64<P, OP, 4>(a, r);
+    if (vec) {
+      *reinterpret_cast<double2*>(y + i) = make_double2(r[0], r[1]);
+      *reinterpret_cast<double2*>(y + i + 2) = make_double2(r[2], r[3]);
+    } else {
+#pragma unroll
+      for (int l = 0; l < 4; ++l) y[i + l] = r[l];
+    }
+  } else if (t < npacks + (n - npacks * 4)) {
+    const size_t i = npacks * 4 + (t - npacks);
+    double a = x[i], r;
+    apply_f64<P, OP, 1>(&a, &r);
+    y[i] = r;
+  }
+}
+
+template <class P>
+__global__ __launch_bounds__(kBlock) void sincos_f32_k(const float* __restrict__ x,
+                                                       float* __restrict__ s,
+                                                       float* __restrict__ c, size_t n) {
+  const size_t npacks = n / 8;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < npacks) {
+    float a[8], rs[8], rc[8];
+#pragma unroll
+    for (int l = 0; l < 8; ++l) a[l] = x[t * 8 + l];
+    dm::trig_group_f<P, 8>(dm::k_sincos, a, rs, rc);
+#pragma unroll
+    for (int l = 0; l < 8; ++l) { s[t * 8 + l] = rs[l]; c[t * 8 + l] = rc[l]; }
+  } else if (t < npacks + (n - npacks * 8)) {
+    const size_t i = npacks * 8 + (t - npacks);
+    float a = x[i], rs, rc;
+    dm::trig_group_f<P, 1>(dm::k_sincos, &a, &rs, &rc);
+    s[i] = rs;
+    c[i] = rc;
+  }
+}
+
+template <class P>
+__global__ __launch_bounds__(kBlock) void sincos_f64_k(const double* __restrict__ x,
+                                                       double* __restrict__ s,
+                                                       double* __restrict__ c, size_t n) {
+  const size_t npacks = n / 4;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < npacks) {
+    double a[4], rs[4], rc[4];
+#pragma unroll
+    for (int l = 0; l < 4; ++l) a[l] = x[t * 4 + l];
+    dm::trig_group_d<P, 4>(dm::k_sincos, a, rs, rc);
+#pragma unroll
+    for (int l = 0; l < 4; ++l) { s[t * 4 + l] = rs[l]; c[t * 4 + l] = rc[l]; }
+  } else if (t < npacks + (n - npacks * 4)) {
+    const size_t i = npacks * 4 + (t - npacks);
+    double a = x[i], rs, rc;
+    dm::trig_group_d<P, 1>(dm::k_sincos, &a, &rs, &rc);
+    s[i] = rs;
+    c[i] = rc;
+  }
+}
+
+// Y = a*X + Y, one float4 per work item, scalar tail
+__global__ __launch_bounds__(kBlock) void saxpy_k(size_t n, float a, const float* __restrict__ x,
+                                                  float* __restrict__ y, int vec) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t n4 = vec ? n / 4 : 0;
+  if (t < n4) {
+    const float4 xv = reinterpret_cast<const float4*>(x)[t];
+    float4 yv = reinterpret_cast<float4*>(y)[t];
+    yv.x = a * xv.x + yv.x;
+    yv.y = a * xv.y + yv.y;
+    yv.z = a * xv.z + yv.z;
+    yv.w = a * xv.w + yv.w;
+    reinterpret_cast<float4*>(y)[t] = yv;
+  } else {
+    const size_t i = n4 * 4 + (t - n4);
+    if (i < n) y[i] = a * x[i] + y[i];
+  }
+}
+
+// ============================================================================
+// HBM reference kernel: y = x (16-byte vectors, grid-stride, streaming
+// stores).  Used to measure the achievable HBM bandwidth on the box.
+// ============================================================================
+__global__ __launch_bounds__(kBlock) void stream_copy_k(const float4* __restrict__ x,
+                                                        float4* __restrict__ y, size_t n4) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
+    const float4 v = __builtin_nontemporal_load(x + i);
+    __builtin_nontemporal_store(v, y + i);
+  }
+}
+
+// ============================================================================
+// inner_fold_: sum(x) and sum(x.*y), compensated FP32 per lane
+// ============================================================================
+template <bool Dot>
+__global__ __launch_bounds__(kBlock) void fold_f32_k(const float* __restrict__ x,
+                                                     const float* __restrict__ y, size_t n,
+                                                     int vec, double* partial, unsigned* counter,
+                                                     double* out) {
+  float s = 0.0f, c = 0.0f;
+  const size_t gtid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  size_t done = 0;
+  if (vec) {
+    const size_t n4 = n / 4;
+    const float4* x4 = reinterpret_cast<const float4*>(x);
+    const float4* y4 = reinterpret_cast<const float4*>(y);
+    for (size_t i = gtid; i < n4; i += stride) {
+      const float4 a = x4[i];
+      if (Dot) {
+        const float4 b = y4[i];
+        comp_add(s, c, a.x * b.x);
+        comp_add(s, c, a.y * b.y);
+        comp_add(s, c, a.z * b.z);
+        comp_add(s, c, a.w * b.w);
+      } else {
+        comp_add(s, c, a.x);
+        comp_add(s, c, a.y);
+        comp_add(s, c, a.z);
+        comp_add(s, c, a.w);
+      }
+    }
+    done = n4 * 4;
+  }
+  for (size_t i = done + gtid; i < n; i += stride) comp_add(s, c, Dot ? x[i] * y[i] : x[i]);
+  const double b = block_sum_comp(s, c);
+  grid_combine(b, partial, counter, out);
+}
+
+// ============================================================================
+// fused Black-Scholes
+// ============================================================================
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void store_nt(f32x4* p, float4 v) {
+  __builtin_nontemporal_store(f32x4{v.x, v.y, v.z, v.w}, p);
+}
+__device__ __forceinline__ void store_nt(f64x2* p, f64x2 v) { __builtin_nontemporal_store(v, p); }
+// textbook: da = sqrt(T); vsT = v*da; d1 = (log(S/X) + drift*T)/vsT with
+// drift = fma(sqr(v), 0.5, r); d2 = d1 - vsT; e = X*exp(-r*T);
+// call = S*N(d1) - e*N(d2); put = e*N(-d2) - S*N(-d1), N = normcdf.
+// N(d) and N(-d) share one erfc evaluation: erfc(-c*d) and erfc(c*d) have the
+// same |argument|, NT2's erfc folds the sign afterwards (erfc.hpp:118-121).
+// demo: demo/blackscholes/nt2/blackandscholes_nt2.cpp:55-59.
+template <class P, int MODE>
+__device__ __forceinline__ void bs_one_f32(float S, float X, float T, float mr, float v,
+                                           float drift, float& call, float& put) {
+  _Pragma("clang fp contract(off)")
+  const float da = __builtin_sqrtf(T);
+  const float vsT = v * da;
+  const float e = X * dm::exp_f<P>(mr * T);
+  if constexpr (MODE == NT2H_BS_TEXTBOOK) {
+    const float d1 = (dm::log_f<P>(S / X) + drift * T) / vsT;
+    const float d2 = d1 - vsT;
+    const float c = NT2H_F(0x3f3504f3);
+    const float a1 = (-c) * d1, a2 = (-c) * d2;  // erfc arguments of N(d1), N(d2)
+    const float r1 = dm::erfc_abs_f<P>(__builtin_fabsf(a1));
+    const float r2 = dm::erfc_abs_f<P>(__builtin_fabsf(a2));
+    const float n1 = 0.5f * dm::erfc_fold_f(a1, r1), m1 = 0.5f * dm::erfc_fold_f(-a1, r1);
+    const float n2 = 0.5f * dm::erfc_fold_f(a2, r2), m2 = 0.5f * dm::erfc_fold_f(-a2, r2);
+    call = S * n1 - e * n2;
+    put = e * m2 - S * m1;
+  } else {
+    const float d1 = dm::log_f<P>(S / X) + (drift * T) / vsT;
+    const float d2 = d1 - v * da;
+    call = S * dm::fastnormcdf_f<P>(d1) - e * dm::fastnormcdf_f<P>(d2);
+    put = 0.0f;
+  }
+}
+
+template <class P, int MODE>
+__global__ __launch_bounds__(kBlock) void bs_f32_k(size_t n, const float* __restrict__ S,
+                                                   const float* __restrict__ X,
+                                                   const float* __restrict__ T, float r, float v,
+                                                   float* __restrict__ call,
+                                                   float* __restrict__ put, int vec) {
+  _Pragma("clang fp contract(off)")
+  const float drift = (v * v) * 0.5f + r;
+  const float mr = -r;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t n4 = vec ? n / 4 : 0;
+  if (t < n4) {
+    const float4 s4 = reinterpret_cast<const float4*>(S)[t];
+    const float4 x4 = reinterpret_cast<const float4*>(X)[t];
+    const float4 t4 = reinterpret_cast<const float4*>(T)[t];
+    float4 c4, p4;
+    bs_one_f32<P, MODE>(s4.x, x4.x, t4.x, mr, v, drift, c4.x, p4.x);
+    bs_one_f32<P, MODE>(s4.y, x4.y, t4.y, mr, v, drift, c4.y, p4.y);
+    bs_one_f32<P, MODE>(s4.z, x4.z, t4.z, mr, v, drift, c4.z, p4.z);
+    bs_one_f32<P, MODE>(s4.w, x4.w, t4.w, mr, v, drift, c4.w, p4.w);
+    store_nt(reinterpret_cast<f32x4*>(call) + t, c4);
+    if (MODE == NT2H_BS_TEXTBOOK) store_nt(reinterpret_cast<f32x4*>(put) + t, p4);
+  } else {
+    const size_t i = n4 * 4 + (t - n4);
+    if (i < n) {
+      float c, p;
+      bs_one_f32<P, MODE>(S[i], X[i], T[i], mr, v, drift, c, p);
+      call[i] = c;
+      if (MODE == NT2H_BS_TEXTBOOK) put[i] = p;
+    }
+  }
+}
+
+template <class P, bool Scalar>
+__device__ __forceinline__ void bs_one_f64(double S, double X, double T, double mr, double v,
+                                           double drift, double& call, double& put) {
+  _Pragma("clang fp contract(off)")
+  const double da = __builtin_sqrt(T);
+  const double vsT = v * da;
+  const double e = X * dm::exp_d<P>(mr * T);
+  const double d1 = (dm::log_d<P>(S / X) + drift * T) / vsT;
+  const double d2 = d1 - vsT;
+  call = S * dm::normcdf_d<P, Scalar>(d1) - e * dm::normcdf_d<P, Scalar>(d2);
+  put = e * dm::normcdf_d<P, Scalar>(-d2) - S * dm::normcdf_d<P, Scalar>(-d1);
+}
+
+// f64: one NT2 pack (4 options) per work item; the n%4 tail uses NT2's scalar erfc
+template <class P>
+__global__ __launch_bounds__(kBlock) void bs_f64_k(size_t n, const double* __restrict__ S,
+                                                   const double* __restrict__ X,
+                                                   const double* __restrict__ T, double r,
+                                                   double v, double* __restrict__ call,
+                                                   double* __restrict__ put, int vec) {
+  _Pragma("clang fp contract(off)")
+  const double drift = (v * v) * 0.5 + r;
+  const double mr = -r;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t npacks = n / 4;
+  if (t < npacks) {
+    const size_t i = t * 4;
+    double s[4], x[4], tt[4], c[4], p[4];
+    if (vec) {
+      const double2 a0 = reinterpret_cast<const double2*>(S + i)[0], a1 = reinterpret_cast<const double2*>(S + i)[1];
+      const double2 b0 = reinterpret_cast<const double2*>(X + i)[0], b1 = reinterpret_cast<const double2*>(X + i)[1];
+      const double2 c0 = reinterpret_cast<const double2*>(T + i)[0], c1 = reinterpret_cast<const double2*>(T + i)[1];
+      s[0] = a0.x; s[1] = a0.y; s[2] = a1.x; s[3] = a1.y;
+      x[0] = b0.x; x[1] = b0.y; x[2] = b1.x; x[3] = b1.y;
+      tt[0] = c0.x; tt[1] = c0.y; tt[2] = c1.x; tt[3] = c1.y;
+    } else {
+#pragma unroll
+      for (int l = 0; l < 4; ++l) { s[l] = S[i + l]; x[l] = X[i + l]; tt[l] = T[i + l]; }
+    }
+#pragma unroll
+    for (int l = 0; l < 4; ++l) bs_one_f64<P, false>(s[l], x[l], tt[l], mr, v, drift, c[l], p[l]);
+    if (vec) {
+      store_nt(reinterpret_cast<f64x2*>(call + i), f64x2{c[0], c[1]});
+      store_nt(reinterpret_cast<f64x2*>(call + i) + 1, f64x2{c[2], c[3]});
+      store_nt(reinterpret_cast<f64x2*>(put + i), f64x2{p[0], p[1]});
+      store_nt(reinterpret_cast<f64x2*>(put + i) + 1, f64x2{p[2], p[3]});
+    } else {
+#pragma unroll
+      for (int l = 0; l < 4; ++l) { call[i + l] = c[l]; put[i + l] = p[l]; }
+    }
+  } else if (t < npacks + (n - npacks * 4)) {
+    const size_t i = npacks * 4 + (t - npacks);
+    double c, p;
+    bs_one_f64<P, true>(S[i], X[i], T[i], mr, v, drift, c, p);
+    call[i] = c;
+    put[i] = p;
+  }
+}
+
+// ============================================================================
+// sigma-delta (uint8): one pixel per byte lane; state kept in registers
+// across frames.  Frame rule: demo/sigmadelta/nt2/sigmadelta_nt2.cpp:37-58.
+// ============================================================================
+__device__ __forceinline__ void sd_pixel(uint32_t f, uint32_t& b, uint32_t& v, uint32_t& lab,
+                                         int N) {
+  b = (b < f) ? b + 1u : ((b > f) ? b - 1u : b);            // selinc / seldec
+  const uint32_t d = (b > f ? b : f) - (b < f ? b : f);     // max - min
+  uint32_t m = d;
+  for (int t = 1; t < N; ++t) m = dm::adds_u8(m, d);        // adds(...adds(d,d)...)
+  if (d != 0u) v = (v < m) ? v + 1u : ((v > m) ? v - 1u : v);
+  lab = (d < v) ? 0u : 1u;                                  // if_zero_else_one(d < v)
+}
+
+__device__ __forceinline__ uint32_t wave_count(uint32_t c) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) c += __shfl_xor(c, off, 64);
+  return c;
+}
+
+// 16 pixels per thread (one 128-bit load per frame)
+__global__ __launch_bounds__(kBlock) void sigmadelta16_k(const uint8_t* __restrict__ frames,
+                                                         size_t stride, int nframes, size_t n16,
+                                                         uint8_t* __restrict__ bkg,
+                                                         uint8_t* __restrict__ var,
+                                                         uint8_t* __restrict__ label, int N,
+                                                         uint32_t* __restrict__ counts) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = t < n16;
+  uint32_t b[16], v[16], l[16];
+  if (active) {
+    const uint4 bb = reinterpret_cast<const uint4*>(bkg)[t];
+    const uint4 vv = reinterpret_cast<const uint4*>(var)[t];
+    const uint32_t bw[4] = {bb.x, bb.y, bb.z, bb.w}, vw[4] = {vv.x, vv.y, vv.z, vv.w};
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      b[q] = (bw[q >> 2] >> (8 * (q & 3))) & 0xffu;
+      v[q] = (vw[q >> 2] >> (8 * (q & 3))) & 0xffu;
+      l[q] = 0u;
+    }
+  }
+  for (int k = 1; k < nframes; ++k) {
+    uint32_t cnt = 0;
+    if (active) {
+      const uint4 ff = reinterpret_cast<const uint4*>(frames + (size_t)k * stride)[t];
+      const uint32_t fw[4] = {ff.x, ff.y, ff.z, ff.w};
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        sd_pixel((fw[q >> 2] >> (8 * (q & 3))) & 0xffu, b[q], v[q], l[q], N);
+        cnt += l[q];
+      }
+    }
+    if (counts) {
+      cnt = wave_count(cnt);
+      if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(&counts[k], cnt);
+    }
+  }
+  if (active) {
+    uint32_t bw[4] = {0, 0, 0, 0}, vw[4] = {0, 0, 0, 0}, lw[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      bw[q >> 2] |= b[q] << (8 * (q & 3));
+      vw[q >> 2] |= v[q] << (8 * (q & 3));
+      lw[q >> 2] |= l[q] << (8 * (q & 3));
+    }
+    reinterpret_cast<uint4*>(bkg)[t] = make_uint4(bw[0], bw[1], bw[2], bw[3]);
+    reinterpret_cast<uint4*>(var)[t] = make_uint4(vw[0], vw[1], vw[2], vw[3]);
+    reinterpret_cast<uint4*>(label)[t] = make_uint4(lw[0], lw[1], lw[2], lw[3]);
+  }
+}
+
+// one pixel per thread: pixels [p0, npix) (tails and unaligned buffers)
+__global__ __launch_bounds__(kBlock) void sigmadelta1_k(const uint8_t* __restrict__ frames,
+                                                        size_t stride, int nframes, size_t p0,
+                                                        size_t npix, uint8_t* __restrict__ bkg,
+                                                        uint8_t* __restrict__ var,
+                                                        uint8_t* __restrict__ label, int N,
+                                                        uint32_t* __restrict__ counts) {
+  const size_t p = p0 + (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool active = p < npix;
+  uint32_t b = 0, v = 0, l = 0;
+  if (active) { b = bkg[p]; v = var[p]; l = label[p]; }
+  for (int k = 1; k < nframes; ++k) {
+    uint32_t cnt = 0;
+    if (active) {
+      sd_pixel(frames[(size_t)k * stride + p], b, v, l, N);
+      cnt = l;
+    }
+    if (counts) {
+      cnt = wave_count(cnt);
+      if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(&counts[k], cnt);
+    }
+  }
+  if (active) { bkg[p] = (uint8_t)b; var[p] = (uint8_t)v; label[p] = (uint8_t)l; }
+}
+
+// ============================================================================
+// batched expression z = exp(-x.*x/2).*cos(y) + sqrt(abs(x)) (one table)
+// ============================================================================
+template <class P, int G>
+__device__ __forceinline__ void expr4_group(const float* x, const float* y, float* z) {
+  _Pragma("clang fp contract(off)")
+  float c[G];
+  dm::trig_group_f<P, G>(dm::k_cos, y, c, nullptr);
+#pragma unroll
+  for (int l = 0; l < G; ++l) {
+    const float t = ((-x[l]) * x[l]) / 2.0f;
+    z[l] = dm::exp_f<P>(t) * c[l] + __builtin_sqrtf(__builtin_fabsf(x[l]));
+  }
+}
+
+template <class P>
+__global__ __launch_bounds__(kBlock) void expr4_k(const float* __restrict__ x,
+                                                  const float* __restrict__ y,
+                                                  float* __restrict__ z, size_t n) {
+  const size_t npacks = n / 8;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < npacks) {
+    float a[8], b[8], r[8];
+#pragma unroll
+    for (int l = 0; l < 8; ++l) { a[l] = x[t * 8 + l]; b[l] = y[t * 8 + l]; }
+    expr4_group<P, 8>(a, b, r);
+#pragma unroll
+    for (int l = 0; l < 8; ++l) z[t * 8 + l] = r[l];
+  } else if (t < npacks + (n - npacks * 8)) {
+    const size_t i = npacks * 8 + (t - npacks);
+    float a = x[i], b = y[i], r;
+    expr4_group<P, 1>(&a, &b, &r);
+    z[i] = r;
+  }
+}
+
+template <class P>
+__global__ __launch_bounds__(kBlock) void expr4_sum_k(const float* __restrict__ x,
+                                                      const float* __restrict__ y, size_t n,
+                                                      double* partial, unsigned* counter,
+                                                      double* out) {
+  float s = 0.0f, c = 0.0f;
+  const size_t npacks = n / 8;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < npacks + (n - npacks * 8);
+       t += stride) {
+    if (t < npacks) {
+      float a[8], b[8], r[8];
+#pragma unroll
+      for (int l = 0; l < 8; ++l) { a[l] = x[t * 8 + l]; b[l] = y[t * 8 + l]; }
+      expr4_group<P, 8>(a, b, r);
+#pragma unroll
+      for (int l = 0; l < 8; ++l) comp_add(s, c, r[l]);
+    } else {
+      const size_t i = npacks * 8 + (t - npacks);
+      float a = x[i], b = y[i], r;
+      expr4_group<P, 1>(&a, &b, &r);
+      comp_add(s, c, r);
+    }
+  }
+  const double bsum = block_sum_comp(s, c);
+  grid_combine(bsum, partial, counter, out);
+}
+
+// ---- transform_ ---------------------------------------------------------
+template <class P>
+static int launch_map_f32(int op, size_t n, const float* x, float* y, hipStream_t s) {
+  const size_t items = n / 8 + n % 8;
+  const int vec = aligned16(x) && aligned16(y);
+  const dim3 g(grid_for(items)), b(kBlock);
+  switch (op) {
+    case NT2H_OP_EXP: hipLaunchKernelGGL((map_f32_k<P, NT2H_OP_EXP>), g, b, 0, s, x, y, n, vec); break;
+    case NT2H_OP_LOG: hipLaunchKernelGGL((map_f32_k<P, NT2H_OP_LOG>), g, b, 0, s, x, y, n, vec); break;
+    case NT2H_OP_SQRT: hipLaunchKernelGGL((map_f32_k<P, NT2H_OP_SQRT>), g, b, 0, s, x, y, n, vec); break;
+    case NT2H_OP_ERFC: hipLaunchKernelGGL((map_f32_k<P, NT2H_OP_ERFC>), g, b, 0, s, x, y, n, vec); break;
+    case NT2H_OP_NORMCDF: hipLaunchKernelGGL((map_f32_k<P, NT2H_OP_NORMCDF>), g, b, 0, s, x, y, n, vec); break;
+    case NT2H_OP_FASTNORMCDF: hipLaunchKernelGGL((map_f32_k<P, NT2H_OP_FASTNORMCDF>), g, b, 0, s, x, y, n, vec); break;
+    case NT2H_OP_COS: hipLaunchKernelGGL((map_f32_k<P, NT2H_OP_COS>), g, b, 0, s, x, y, n, vec); break;
+    case NT2H_OP_SIN: hipLaunchKernelGGL((map_f32_k<P, NT2H_OP_SIN>), g, b, 0, s, x, y, n, vec); break;
+    case NT2H_OP_ABS: hipLaunchKernelGGL((map_f32_k<P, NT2H_OP_ABS>), g, b, 0, s, x, y, n, vec); break;
+    default: return fail(NT2H_EINVAL, "nt2h_map_f32: unknown op");
+  }
+  NT2H_CHECK_LAUNCH("map_f32");
+  return NT2H_OK;
+}
+
+template <class P>
+static int launch_map_f64(int op, size_t n, const double* x, double* y, hipStream_t s) {
+  const size_t items = n / 4 + n % 4;
+  const int vec = aligned16(x) && aligned16(y);
+  const dim3 g(grid_for(items)), b(kBlock);
+  switch (op) {
+    case NT2H_OP_EXP: hipLaunchKernelGGL((map_f64_k<P, NT2H_OP_EXP>), g, b, 0, s, x, y, n, vec); break;
+    case NT2H_OP_LOG: hipLaunchKernelGGL((map_f64_k<P, NT2H_OP_LOG>), g, b, 0, s, x, y, n, vec); break;
+    case NT2H_OP_SQRT: hipLaunchKernelGGL((map_f64_k<P, NT2H_OP_SQRT>), g, b, 0, s, x, y, n, vec); break;
+    case NT2H_OP_ERFC: hipLaunchKernelGGL((map_f64_k<P, NT2H_OP_ERFC>), g, b, 0, s, x, y, n, vec); break;
+    case NT2H_OP_NORMCDF: hipLaunchKernelGGL((map_f64_k<P, NT2H_OP_NORMCDF>), g, b, 0, s, x, y, n, vec); break;
+    case NT2H_OP_COS: hipLaunchKernelGGL((map_f64_k<P, NT2H_OP_COS>), g, b, 0, s, x, y, n, vec); break;
+    case NT2H_OP_SIN: hipLaunchKernelGGL((map_f64_k<P, NT2H_OP_SIN>), g, b, 0, s, x, y, n, vec); break;
+    case NT2H_OP_ABS: hipLaunchKernelGGL((map_f64_k<P, NT2H_OP_ABS>), g, b, 0, s, x, y, n, vec); break;
+    default: return fail(NT2H_EINVAL, "nt2h_map_f64: unknown op (fastnormcdf is single precision only)");
+  }
+  NT2H_CHECK_LAUNCH("map_f64");
+  return NT2H_OK;
+}
+
+}  // namespace nt2h
+
+using namespace nt2h;
+
+// ============================================================================
+// C ABI
+// ============================================================================
+extern "C" {
+
+int nt2h_abi_version(void) { return 1; }
+
+const char* nt2h_last_error(void) { return g_last_error.c_str(); }
+
+int nt2h_device_info(int* cus, size_t* hbm_bytes, char* name, size_t name_len) {
+  int dev = 0;
+  NT2H_CHECK(hipGetDevice(&dev), "hipGetDevice");
+  hipDeviceProp_t p;
+  NT2H_CHECK(hipGetDeviceProperties(&p, dev), "hipGetDeviceProperties");
+  if (cus) *cus = p.multiProcessorCount;
+  if (hbm_bytes) *hbm_bytes = p.totalGlobalMem;
+  if (name && name_len) {
+    std::strncpy(name, p.gcnArchName, name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  return NT2H_OK;
+}
+
+int nt2h_malloc(void** dptr, size_t bytes) {
+  if (!dptr) return fail(NT2H_EINVAL, "nt2h_malloc: null out pointer");
+  *dptr = nullptr;
+  if (bytes == 0) return NT2H_OK;
+  NT2H_CHECK(hipMalloc(dptr, bytes), "hipMalloc");
+  return NT2H_OK;
+}
+
+int nt2h_free(void* dptr) {
+  if (!dptr) return NT2H_OK;
+  NT2H_CHECK(hipFree(dptr), "hipFree");
+  return NT2H_OK;
+}
+
+int nt2h_memcpy(void* dst, const void* src, size_t bytes, int kind, void* stream) {
+  if (bytes == 0) return NT2H_OK;
+  if (!dst || !src) return fail(NT2H_EINVAL, "nt2h_memcpy: null pointer");
+  hipMemcpyKind k;
+  switch (kind) {
+    case NT2H_H2D: k = hipMemcpyHostToDevice; break;
+    case NT2H_D2H: k = hipMemcpyDeviceToHost; break;
+    case NT2H_D2D: k = hipMemcpyDeviceToDevice; break;
+    default: return fail(NT2H_EINVAL, "nt2h_memcpy: bad kind");
+  }
+  NT2H_CHECK(hipMemcpyAsync(dst, src, bytes, k, as_stream(stream)), "hipMemcpyAsync");
+  return NT2H_OK;
+}
+
+int nt2h_stream_create(void** stream) {
+  if (!stream) return fail(NT2H_EINVAL, "nt2h_stream_create: null out pointer");
+  hipStream_t s;
+  NT2H_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+  *stream = s;
+  return NT2H_OK;
+}
+int nt2h_stream_destroy(void* stream) {
+  if (stream) NT2H_CHECK(hipStreamDestroy(as_stream(stream)), "hipStreamDestroy");
+  return NT2H_OK;
+}
+int nt2h_stream_sync(void* stream) {
+  NT2H_CHECK(hipStreamSynchronize(as_stream(stream)), "hipStreamSynchronize");
+  return NT2H_OK;
+}
+
+static unsigned gen_grid(size_t n) {
+  size_t g = (n + kBlock - 1) / kBlock;
+  return (unsigned)(g > 65536 ? 65536 : (g ? g : 1));
+}
+
+int nt2h_fill_uniform_f32(float* out, size_t n, double lo, double hi, uint64_t seed, void* stream) {
+  if (n == 0) return NT2H_OK;
+  if (!out) return fail(NT2H_EINVAL, "nt2h_fill_uniform_f32: null pointer");
+  hipLaunchKernelGGL(fill_uniform_k<float>, dim3(gen_grid(n)), dim3(kBlock), 0, as_stream(stream), out, n, lo, hi, seed);
+  NT2H_CHECK_LAUNCH("fill_uniform_f32");
+  return NT2H_OK;
+}
+int nt2h_fill_uniform_f64(double* out, size_t n, double lo, double hi, uint64_t seed, void* stream) {
+  if (n == 0) return NT2H_OK;
+  if (!out) return fail(NT2H_EINVAL, "nt2h_fill_uniform_f64: null pointer");
+  hipLaunchKernelGGL(fill_uniform_k<double>, dim3(gen_grid(n)), dim3(kBlock), 0, as_stream(stream), out, n, lo, hi, seed);
+  NT2H_CHECK_LAUNCH("fill_uniform_f64");
+  return NT2H_OK;
+}
+int nt2h_fill_normal_f32(float* out, size_t n, uint64_t seed, void* stream) {
+  if (n == 0) return NT2H_OK;
+  if (!out) return fail(NT2H_EINVAL, "nt2h_fill_normal_f32: null pointer");
+  hipLaunchKernelGGL(fill_normal_k, dim3(gen_grid(n)), dim3(kBlock), 0, as_stream(stream), out, n, seed);
+  NT2H_CHECK_LAUNCH("fill_normal_f32");
+  return NT2H_OK;
+}
+
+int nt2h_fill_frames_u8(uint8_t* frames, int h, int w, int nframes, int nrect, int noise,
+                        uint64_t seed, void* stream) {
+  if (!frames || h <= 0 || w <= 0 || nframes <= 0 || nrect < 0 || nrect > 8 || noise < 0 || noise > 64)
+    return fail(NT2H_EINVAL, "nt2h_fill_frames_u8: bad argument");
+  Rects R;
+  R.n = nrect;
+  for (int q = 0; q < nrect; ++q) {
+    auto u = [&](int j) { return splitmix64_at(seed ^ 0x5bd1e995ull, (uint64_t)(q * 16 + j)); };
+    Rect& r = R.r[q];
+    r.w = 48 + (int)(u(0) % 113u);
+    r.h = 48 + (int)(u(1) % 113u);
+    r.x0 = (int)(u(2) % (uint64_t)w);
+    r.y0 = (int)(u(3) % (uint64_t)h);
+    r.vx = (int)(u(4) % 17u) - 8;
+    r.vy = (int)(u(5) % 17u) - 8;
+    r.val = 230 + (int)(u(6) % 26u);
+  }
+  const size_t total = (size_t)h * w * nframes;
+  hipLaunchKernelGGL(fill_frames_k, dim3(gen_grid(total)), dim3(kBlock), 0, as_stream(stream), frames, h, w, nframes, R, noise, seed);
+  NT2H_CHECK_LAUNCH("fill_frames_u8");
+  return NT2H_OK;
+}
+
+int nt2h_map_f32(int op, size_t n, const float* x, float* y, int policy, void* stream) {
+  if (n == 0) return NT2H_OK;
+  if (!x || !y) return fail(NT2H_EINVAL, "nt2h_map_f32: null pointer");
+  if (policy == NT2H_EXACT) return launch_map_f32<exact_policy>(op, n, x, y, as_stream(stream));
+  if (policy == NT2H_FAST) return launch_map_f32<fast_policy>(op, n, x, y, as_stream(stream));
+  return fail(NT2H_EINVAL, "nt2h_map_f32: bad policy");
+}
+
+int nt2h_map_f64(int op, size_t n, const double* x, double* y, int policy, void* stream) {
+  if (n == 0) return NT2H_OK;
+  if (!x || !y) return fail(NT2H_EINVAL, "nt2h_map_f64: null pointer");
+  if (policy == NT2H_EXACT) return launch_map_f64<exact_policy>(op, n, x, y, as_stream(stream));
+  if (policy == NT2H_FAST) return launch_map_f64<fast_policy>(op, n, x, y, as_stream(stream));
+  return fail(NT2H_EINVAL, "nt2h_map_f64: bad policy");
+}
+
+int nt2h_sincos_f32(size_t n, const float* x, float* s, float* c, int policy, void* stream) {
+  if (n == 0) return NT2H_OK;
+  if (!x || !s || !c) return fail(NT2H_EINVAL, "nt2h_sincos_f32: null pointer");
+  const dim3 g(grid_for(n / 8 + n % 8)), b(kBlock);
+  if (policy == NT2H_EXACT) hipLaunchKernelGGL(sincos_f32_k<exact_policy>, g, b, 0, as_stream(stream), x, s, c, n);
+  else if (policy == NT2H_FAST) hipLaunchKernelGGL(sincos_f32_k<fast_policy>, g, b, 0, as_stream(stream), x, s, c, n);
+  else return fail(NT2H_EINVAL, "nt2h_sincos_f32: bad policy");
+  NT2H_CHECK_LAUNCH("sincos_f32");
+  return NT2H_OK;
+}
+
+int nt2h_sincos_f64(size_t n, const double* x, double* s, double* c, int policy, void* stream) {
+  if (n == 0) return NT2H_OK;
+  if (!x || !s || !c) return fail(NT2H_EINVAL, "nt2h_sincos_f64: null pointer");
+  const dim3 g(grid_for(n / 4 + n % 4)), b(kBlock);
+  if (policy == NT2H_EXACT) hipLaunchKernelGGL(sincos_f64_k<exact_policy>, g, b, 0, as_stream(stream), x, s, c, n);
+  else if (policy == NT2H_FAST) hipLaunchKernelGGL(sincos_f64_k<fast_policy>, g, b, 0, as_stream(stream), x, s, c, n);
+  else return fail(NT2H_EINVAL, "nt2h_sincos_f64: bad policy");
+  NT2H_CHECK_LAUNCH("sincos_f64");
+  return NT2H_OK;
+}
+
+int nt2h_stream_copy(const void* src, void* dst, size_t bytes, int grid, void* stream) {
+  if (bytes == 0) return NT2H_OK;
+  if (!src || !dst || !aligned16(src) || !aligned16(dst) || bytes % 16)
+    return fail(NT2H_EINVAL, "nt2h_stream_copy: needs 16-byte aligned buffers and size");
+  const size_t n4 = bytes / 16;
+  const unsigned g = grid > 0 ? (unsigned)grid : grid_for(n4);
+  hipLaunchKernelGGL(stream_copy_k, dim3(g), dim3(kBlock), 0, as_stream(stream),
+                     (const float4*)src, (float4*)dst, n4);
+  NT2H_CHECK_LAUNCH("stream_copy");
+  return NT2H_OK;
+}
+
+int nt2h_saxpy_f32(size_t n, float a, const float* x, float* y, void* stream) {
+  if (n == 0) return NT2H_OK;
+  if (!x || !y) return fail(NT2H_EINVAL, "nt2h_saxpy_f32: null pointer");
+  const int vec = aligned16(x) && aligned16(y);
+  const size_t items = vec ? n / 4 + n % 4 : n;
+  hipLaunchKernelGGL(saxpy_k, dim3(grid_for(items)), dim3(kBlock), 0, as_stream(stream), n, a, x, y, vec);
+  NT2H_CHECK_LAUNCH("saxpy_f32");
+  return NT2H_OK;
+}
+
+// ---- inner_fold_ ----------------------------------------------------------
+static unsigned fold_grid(size_t work_items) {
+  size_t g = (work_items + kBlock - 1) / kBlock;
+  if (g < 1) g = 1;
+  if (g > kMaxFoldBlocks) g = kMaxFoldBlocks;
+  return (unsigned)g;
+}
+
+static int launch_fold(size_t n, const float* x, const float* y, double* out, hipStream_t s) {
+  const int vec = aligned16(x) && (!y || aligned16(y));
+  const unsigned g = fold_grid(vec ? (n + 3) / 4 : n);
+  Scratch sc;
+  int rc = get_scratch(g, &sc, s);
+  if (rc) return rc;
+  if (y) hipLaunchKernelGGL(fold_f32_k<true>, dim3(g), dim3(kBlock), 0, s, x, y, n, vec, sc.partial, sc.counter, out);
+  else hipLaunchKernelGGL(fold_f32_k<false>, dim3(g), dim3(kBlock), 0, s, x, y, n, vec, sc.partial, sc.counter, out);
+  NT2H_CHECK_LAUNCH("fold_f32");
+  return NT2H_OK;
+}
+
+int nt2h_sum_f32(size_t n, const float* x, double* out, void* stream) {
+  if (!out || (n && !x)) return fail(NT2H_EINVAL, "nt2h_sum_f32: null pointer");
+  if (n == 0) return hipMemsetAsync(out, 0, sizeof(double), as_stream(stream)) == hipSuccess ? NT2H_OK : NT2H_EHIP;
+  return launch_fold(n, x, nullptr, out, as_stream(stream));
+}
+
+int nt2h_dot_f32(size_t n, const float* x, const float* y, double* out, void* stream) {
+  if (!out || (n && (!x || !y))) return fail(NT2H_EINVAL, "nt2h_dot_f32: null pointer");
+  if (n == 0) return hipMemsetAsync(out, 0, sizeof(double), as_stream(stream)) == hipSuccess ? NT2H_OK : NT2H_EHIP;
+  return launch_fold(n, x, y, out, as_stream(stream));
+}
+
+// ---- Black-Scholes ----------------------------------------------------------
+int nt2h_blackscholes_f32(size_t n, const float* S, const float* X, const float* T, float r,
+                          float v, float* call, float* put, int mode, int policy, void* stream) {
+  if (n == 0) return NT2H_OK;
+  if (!S || !X || !T || !call || (mode == NT2H_BS_TEXTBOOK && !put))
+    return fail(NT2H_EINVAL, "nt2h_blackscholes_f32: null pointer");
+  if (mode != NT2H_BS_TEXTBOOK && mode != NT2H_BS_DEMO)
+    return fail(NT2H_EINVAL, "nt2h_blackscholes_f32: bad mode");
+  if (policy != NT2H_EXACT && policy != NT2H_FAST)
+    return fail(NT2H_EINVAL, "nt2h_blackscholes_f32: bad policy");
+  const int vec = aligned16(S) && aligned16(X) && aligned16(T) && aligned16(call) &&
+                  (mode == NT2H_BS_DEMO || aligned16(put));
+  const size_t items = vec ? n / 4 + n % 4 : n;
+  const dim3 g(grid_for(items)), b(kBlock);
+  hipStream_t s = as_stream(stream);
+  if (policy == NT2H_EXACT) {
+    if (mode == NT2H_BS_TEXTBOOK) hipLaunchKernelGGL((bs_f32_k<exact_policy, NT2H_BS_TEXTBOOK>), g, b, 0, s, n, S, X, T, r, v, call, put, vec);
+    else hipLaunchKernelGGL((bs_f32_k<exact_policy, NT2H_BS_DEMO>), g, b, 0, s, n, S, X, T, r, v, call, put, vec);
+  } else {
+    if (mode == NT2H_BS_TEXTBOOK) hipLaunchKernelGGL((bs_f32_k<fast_policy, NT2H_BS_TEXTBOOK>), g, b, 0, s, n, S, X, T, r, v, call, put, vec);
+    else hipLaunchKernelGGL((bs_f32_k<fast_policy, NT2H_BS_DEMO>), g, b, 0, s, n, S, X, T, r, v, call, put, vec);
+  }
+  NT2H_CHECK_LAUNCH("blackscholes_f32");
+  return NT2H_OK;
+}
+
+int nt2h_blackscholes_f64(size_t n, const double* S, const double* X, const double* T,
+                          double r, double v, double* call, double* put, int policy,
+                          void* stream) {
+  if (n == 0) return NT2H_OK;
+  if (!S || !X || !T || !call || !put) return fail(NT2H_EINVAL, "nt2h_blackscholes_f64: null pointer");
+  const int vec = aligned16(S) && aligned16(X) && aligned16(T) && aligned16(call) && aligned16(put);
+  const dim3 g(grid_for(n / 4 + n % 4)), b(kBlock);
+  hipStream_t s = as_stream(stream);
+  if (policy == NT2H_EXACT) hipLaunchKernelGGL(bs_f64_k<exact_policy>, g, b, 0, s, n, S, X, T, r, v, call, put, vec);
+  else if (policy == NT2H_FAST) hipLaunchKernelGGL(bs_f64_k<fast_policy>, g, b, 0, s, n, S, X, T, r, v, call, put, vec);
+  else return fail(NT2H_EINVAL, "nt2h_blackscholes_f64: bad policy");
+  NT2H_CHECK_LAUNCH("blackscholes_f64");
+  return NT2H_OK;
+}
+
+// ---- sigma-delta --------------------------------------------------------------
+int nt2h_sigmadelta_u8(size_t npix, int nframes, const uint8_t* frames, size_t frame_stride,
+                       uint8_t* bkg, uint8_t* var, uint8_t* label, int N,
+                       uint32_t* frame_counts, void* stream) {
+  if (npix == 0) return NT2H_OK;
+  if (!frames || !bkg || !var || !label || nframes < 1 || N < 1 || frame_stride < npix)
+    return fail(NT2H_EINVAL, "nt2h_sigmadelta_u8: bad argument");
+  hipStream_t s = as_stream(stream);
+  if (frame_counts) NT2H_CHECK(hipMemsetAsync(frame_counts, 0, sizeof(uint32_t) * nframes, s), "memset counts");
+  const bool vec = aligned16(frames) && aligned16(bkg) && aligned16(var) && aligned16(label) &&
+                   (frame_stride % 16 == 0);
+  size_t p0 = 0;
+  if (vec) {
+    const size_t n16 = npix / 16;
+    if (n16) {
+      hipLaunchKernelGGL(sigmadelta16_k, dim3(grid_for(n16)), dim3(kBlock), 0, s, frames, frame_stride, nframes, n16, bkg, var, label, N, frame_counts);
+      NT2H_CHECK_LAUNCH("sigmadelta16");
+    }
+    p0 = n16 * 16;
+  }
+  if (p0 < npix) {
+    hipLaunchKernelGGL(sigmadelta1_k, dim3(grid_for(npix - p0)), dim3(kBlock), 0, s, frames, frame_stride, nframes, p0, npix, bkg, var, label, N, frame_counts);
+    NT2H_CHECK_LAUNCH("sigmadelta1");
+  }
+  return NT2H_OK;
+}
+
+int nt2h_sigmadelta_step_u8(size_t npix, const uint8_t* frame, uint8_t* bkg, uint8_t* var,
+                            uint8_t* label, int N, void* stream) {
+  if (npix == 0) return NT2H_OK;
+  if (!frame || !bkg || !var || !label || N < 1) return fail(NT2H_EINVAL, "nt2h_sigmadelta_step_u8: bad argument");
+  // a two-frame run whose frame 1 is `frame`: base pointer one stride before it
+  const uint8_t* base = frame - npix;
+  return nt2h_sigmadelta_u8(npix, 2, base, npix, bkg, var, label, N, nullptr, stream);
+}
+
+// ---- batched expression, single table ---------------------------------------
+int nt2h_expr4_f32(size_t n, const float* x, const float* y, float* z, int policy, void* stream) {
+  if (n == 0) return NT2H_OK;
+  if (!x || !y || !z) return fail(NT2H_EINVAL, "nt2h_expr4_f32: null pointer");
+  const dim3 g(grid_for(n / 8 + n % 8)), b(kBlock);
+  if (policy == NT2H_EXACT) hipLaunchKernelGGL(expr4_k<exact_policy>, g, b, 0, as_stream(stream), x, y, z, n);
+  else if (policy == NT2H_FAST) hipLaunchKernelGGL(expr4_k<fast_policy>, g, b, 0, as_stream(stream), x, y, z, n);
+  else return fail(NT2H_EINVAL, "nt2h_expr4_f32: bad policy");
+  NT2H_CHECK_LAUNCH("expr4_f32");
+  return NT2H_OK;
+}
+
+int nt2h_expr4_sum_f32(size_t n, const float* x, const float* y, double* out, int policy,
+                       void* stream) {
+  if (!out || (n && (!x || !y))) return fail(NT2H_EINVAL, "nt2h_expr4_sum_f32: null pointer");
+  hipStream_t s = as_stream(stream);
+  if (n == 0) return hipMemsetAsync(out, 0, sizeof(double), s) == hipSuccess ? NT2H_OK : NT2H_EHIP;
+  const unsigned g = fold_grid(n / 8 + n % 8);
+  Scratch sc;
+  int rc = get_scratch(g, &sc, s);
+  if (rc) return rc;
+  if (policy == NT2H_EXACT) hipLaunchKernelGGL(expr4_sum_k<exact_policy>, dim3(g), dim3(kBlock), 0, s, x, y, n, sc.partial, sc.counter, out);
+  else if (policy == NT2H_FAST) hipLaunchKernelGGL(expr4_sum_k<fast_policy>, dim3(g), dim3(kBlock), 0, s, x, y, n, sc.partial, sc.counter, out);
+  else return fail(NT2H_EINVAL, "nt2h_expr4_sum_f32: bad policy");
+  NT2H_CHECK_LAUNCH("expr4_sum_f32");
+  return NT2H_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,800 @@
+// Device math kernels of the HIP architecture site, for gfx950.
+//
+// Each function restates, for one lane, the Boost.SIMD / NT2 algorithm the
+// reference evaluates on CPU packs (file:line cited per function, paths
+// relative to the reference tree).  Two evaluation policies share the code:
+//
+//  * exact_policy: every NT2 fma(a,b,c) is a*b+c with two roundings, as NT2
+//    computes it on x86 AVX (boost/simd/arithmetic/functions/generic/fma.hpp:32),
+//    and no other operation is contracted.  Results are bit-identical to the
+//    CPU evaluation for the same inputs.
+//  * fast_policy: NT2's fma/fnms call sites (Horner steps included) use the
+//    hardware fused multiply-add; everything else is unchanged.  Results stay
+//    within each function's documented ULP bound.
+//
+// Every function body opens with NT2H_STRICT so that the compiler never fuses
+// an operation on its own, whatever -ffp-contract the including file uses.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "detail/pio2_tables.hpp"
+
+#define NT2H_FN __device__ __forceinline__
+#define NT2H_STRICT _Pragma("clang fp contract(off)")
+
+namespace nt2 { namespace hip {
+
+struct exact_policy {
+  static constexpr bool fused = false;
+};
+struct fast_policy {
+  static constexpr bool fused = true;
+};
+
+namespace dm {
+
+NT2H_FN uint32_t bits(float f) { return __builtin_bit_cast(uint32_t, f); }
+NT2H_FN uint64_t bits(double d) { return __builtin_bit_cast(uint64_t, d); }
+NT2H_FN float as_f(uint32_t b) { return __builtin_bit_cast(float, b); }
+NT2H_FN double as_d(uint64_t b) { return __builtin_bit_cast(double, b); }
+
+#define NT2H_F(b) __builtin_bit_cast(float, (uint32_t)(b##u))
+#define NT2H_D(b) __builtin_bit_cast(double, (uint64_t)(b##ull))
+
+// NT2 fma(a,b,c) and fnms(a,b,c) = c - a*b under the policy.
+template <class P> NT2H_FN float fma(float a, float b, float c) {
+  NT2H_STRICT
+  if constexpr (P::fused) return __builtin_fmaf(a, b, c);
+  else return a * b + c;
+}
+template <class P> NT2H_FN double fma(double a, double b, double c) {
+  NT2H_STRICT
+  if constexpr (P::fused) return __builtin_fma(a, b, c);
+  else return a * b + c;
+}
+template <class P> NT2H_FN float fnms(float a, float b, float c) {
+  NT2H_STRICT
+  if constexpr (P::fused) return __builtin_fmaf(-a, b, c);
+  else return c - a * b;
+}
+template <class P> NT2H_FN double fnms(double a, double b, double c) {
+  NT2H_STRICT
+  if constexpr (P::fused) return __builtin_fma(-a, b, c);
+  else return c - a * b;
+}
+
+// Static Horner, coefficients in decreasing powers; each step is NT2's
+// fma(x, acc, c): modules/core/polynomials/include/nt2/polynomials/functions/
+// scalar/impl/horner.hpp:13-45.
+template <class P, class T, int K>
+NT2H_FN T horner(const T (&c)[K], T x) {
+  NT2H_STRICT
+  T acc = c[0];
+#pragma unroll
+  for (int i = 1; i < K; ++i) acc = fma<P>(x, acc, c[i]);
+  return acc;
+}
+
+// ---------------------------------------------------------------------------
+// exp.  f32: modules/core/exponential/include/nt2/exponential/functions/scalar/
+// impl/expo/f_expo_reduction.hpp:58-79, expo_scale.hpp:26-35,
+// simd/common/impl/expo/expo_base.hpp:47-60.
+// ---------------------------------------------------------------------------
+template <class P>
+NT2H_FN float exp_f(float a0) {
+  NT2H_STRICT
+  constexpr float cExp[5] = {NT2H_F(0x3ab778cf), NT2H_F(0x3c098d8b), NT2H_F(0x3d2aa957),
+                             NT2H_F(0x3e2aa9a5), NT2H_F(0x3f000000)};
+  const float k = __builtin_rintf(NT2H_F(0x3fb8aa3b) * a0);
+  float x = fnms<P>(k, NT2H_F(0x3f318000), a0);
+  x = fnms<P>(k, NT2H_F(0xb95e8083), x);
+  const float y = horner<P>(cExp, x);
+  float r = fma<P>(y, x * x, x) + 1.0f;
+  const float kc = __builtin_fminf(__builtin_fmaxf(k, -200.0f), 200.0f);
+  r = r * as_f((uint32_t)((int32_t)kc + 127) << 23);
+  r = (a0 <= NT2H_F(0xc2b0c0a5)) ? 0.0f : r;
+  r = (a0 >= NT2H_F(0x42b0c0a5)) ? __builtin_inff() : r;
+  return (a0 != a0) ? a0 : r;
+}
+
+// f64: d_expo_reduction.hpp:57-84, simd/common/impl/expo/expo_base.hpp:31-42.
+template <class P>
+NT2H_FN double exp_d(double a0) {
+  NT2H_STRICT
+  constexpr double cExp[5] = {NT2H_D(0x3e66376972bea4d0), NT2H_D(0xbebbbd41c5d26bf1),
+                              NT2H_D(0x3f11566aaf25de2c), NT2H_D(0xbf66c16c16bebd93),
+                              NT2H_D(0x3fc555555555553e)};
+  const double k = __builtin_rint(NT2H_D(0x3ff71547652b82fe) * a0);
+  const double hi = fnms<P>(k, NT2H_D(0x3fe62e42fee00000), a0);
+  const double lo = k * NT2H_D(0x3dea39ef35793c76);
+  const double x = hi - lo;
+  const double t = x * x;
+  double c = fnms<P>(t, horner<P>(cExp, t), x);
+  c = 1.0 - ((lo - (x * c) / (2.0 - c)) - hi);
+  const double kc = __builtin_fmin(__builtin_fmax(k, -2000.0), 2000.0);
+  c = c * as_d((uint64_t)((int64_t)kc + 1023) << 52);
+  c = (a0 <= NT2H_D(0xc086232bdd7abcd2)) ? 0.0 : c;
+  c = (a0 >= NT2H_D(0x40862e42fefa39ef)) ? __builtin_inf() : c;
+  return (a0 != a0) ? a0 : c;
+}
+
+// ---------------------------------------------------------------------------
+// log.  f32: simd/common/impl/logs/f_log.hpp:101-125, :173-186 (finalize),
+// scalar/impl/logs/f_kernel.hpp:47-78; fast_frexp:
+// boost/simd/ieee/functions/simd/common/fast_frexp.hpp:33-40.
+// ---------------------------------------------------------------------------
+template <class P>
+NT2H_FN float log_f(float a0) {
+  NT2H_STRICT
+  constexpr float cLog[8] = {NT2H_F(0xbda5dff0), NT2H_F(0x3e0229f9), NT2H_F(0xbe04d6b7),
+                             NT2H_F(0x3e116e80), NT2H_F(0xbe2a6aa0), NT2H_F(0x3e4cd0a3),
+                             NT2H_F(0xbe800064), NT2H_F(0x3eaaaaa9)};
+  float z = a0, t = 0.0f;
+  const bool den = __builtin_fabsf(z) < NT2H_F(0x00800000);
+  z = den ? z * NT2H_F(0x4B000000) : z;
+  t = den ? NT2H_F(0xc17f1402) : t;
+  const uint32_t b = bits(z);
+  const int32_t e = (int32_t)((b & 0x7f800000u) >> 23) - 126;
+  const float m = as_f((b & ~0x7f800000u) | 0x3f000000u);
+  const bool lt = m < NT2H_F(0x3f3504f3);
+  const float fe = lt ? (float)e + (-1.0f) : (float)e;
+  const float x = (lt ? m + m : m) - 1.0f;
+  const float x2 = x * x;
+  float y = (horner<P>(cLog, x) * x) * x2;
+  y = fma<P>(fe, NT2H_F(0xb95e8083), y);
+  y = fma<P>(-0.5f, x2, y);
+  float r = fma<P>(NT2H_F(0x3f318000), fe, (x + y) + t);
+  r = (a0 < 0.0f || a0 != a0) ? __builtin_nanf("") : r;
+  r = (a0 == __builtin_inff()) ? a0 : r;
+  return (a0 == 0.0f) ? -__builtin_inff() : r;
+}
+
+// f64: simd/common/impl/logs/d_log.hpp:60-97.
+template <class P>
+NT2H_FN double log_d(double a0) {
+  NT2H_STRICT
+  constexpr double cT1[3] = {NT2H_D(0x3fc39a09d078c69f), NT2H_D(0x3fcc71c51d8e78af),
+                             NT2H_D(0x3fd999999997fa04)};
+  constexpr double cT2[4] = {NT2H_D(0x3fc2f112df3e5244), NT2H_D(0x3fc7466496cb03de),
+                             NT2H_D(0x3fd2492494229359), NT2H_D(0x3fe5555555555593)};
+  const uint64_t b = bits(a0);
+  int64_t k = (int64_t)((b & 0x7ff0000000000000ull) >> 52) - 1022;
+  const double x = as_d((b & ~0x7ff0000000000000ull) | 0x3fe0000000000000ull);
+  const bool lt = NT2H_D(0x3FE6A09E667F3BCD) > x;
+  k = k + (lt ? -1 : 0);
+  const double f = (x + (lt ? x : 0.0)) - 1.0;
+  const double dk = (double)k;
+  const double s = f / (2.0 + f);
+  const double z = s * s;
+  const double w = z * z;
+  const double t1 = w * horner<P>(cT1, w);
+  const double t2 = z * horner<P>(cT2, w);
+  const double R = t2 + t1;
+  const double hfsq = 0.5 * (f * f);
+  double r = dk * NT2H_D(0x3fe62e42fee00000) -
+             ((hfsq - (s * (hfsq + R) + dk * NT2H_D(0x3dea39ef35793c76))) - f);
+  r = (a0 < 0.0 || a0 != a0) ? __builtin_nan("") : r;
+  r = (a0 == __builtin_inf()) ? a0 : r;
+  return (a0 == 0.0) ? -__builtin_inf() : r;
+}
+
+// ---------------------------------------------------------------------------
+// erfc.  f32: modules/core/euler/include/nt2/euler/functions/simd/common/erfc.hpp:92-123
+// (the scalar form returns the same values), kernels erf_kernel.hpp:46-85.
+// erfc_abs_f returns erfc(|a|) before the sign fold, so erfc(a) and erfc(-a)
+// can share one evaluation.
+// ---------------------------------------------------------------------------
+template <class P>
+NT2H_FN float erfc_abs_f(float x) {
+  NT2H_STRICT
+  constexpr float c2[11] = {NT2H_F(0xc0283611), NT2H_F(0x40d67e3b), NT2H_F(0xc0cd1a85),
+                            NT2H_F(0x4050b063), NT2H_F(0xbfaea865), NT2H_F(0x3e2006f0),
+                            NT2H_F(0x3e1175c7), NT2H_F(0x3ec4ca6e), NT2H_F(0x3e243828),
+                            NT2H_F(0xbf918a62), NT2H_F(0x3f0a0e8b)};
+  constexpr float c3[9] = {NT2H_F(0x41aa8e55), NT2H_F(0x401b5680), NT2H_F(0xc010d956),
+                           NT2H_F(0x3f2cff3b), NT2H_F(0xc016c985), NT2H_F(0xbffc9284),
+                           NT2H_F(0xbfa11698), NT2H_F(0xbe036d7e), NT2H_F(0x3f7ffffe)};
+  const float z = x / (1.0f + x);
+  float r;
+  if (x < NT2H_F(0x3f2aaaab)) {
+    r = (1.0f - z) * horner<P>(c3, z);
+  } else {
+    r = exp_f<P>(-(x * x)) * horner<P>(c2, z - 0.4f);
+  }
+  return (x == __builtin_inff()) ? 0.0f : r;
+}
+NT2H_FN float erfc_fold_f(float a0, float r) {
+  NT2H_STRICT
+  return (a0 < 0.0f) ? 2.0f - r : r;
+}
+template <class P>
+NT2H_FN float erfc_f(float a0) {
+  return erfc_fold_f(a0, erfc_abs_f<P>(__builtin_fabsf(a0)));
+}
+
+// f64 SIMD: simd/common/erfc.hpp:46-89; scalar (pack tail): scalar/erfc.hpp:38-70.
+template <class P, bool Scalar>
+NT2H_FN double erfc_d(double a0) {
+  NT2H_STRICT
+  constexpr double e1p[5] = {NT2H_D(0x3f110512d5b20332), NT2H_D(0x3f53b7664358865a),
+                             NT2H_D(0x3fa4a59a4f02579c), NT2H_D(0x3fc16500f106c0a5),
+                             NT2H_D(0x3ff20dd750429b61)};
+  constexpr double e1q[5] = {NT2H_D(0x3f37ea4332348252), NT2H_D(0x3f8166f75999dbd1),
+                             NT2H_D(0x3fb64536ca92ea2f), NT2H_D(0x3fdd0a84eb1ca867),
+                             NT2H_D(0x3ff0000000000000)};
+  constexpr double e2p[7] = {0.0, NT2H_D(0x3f7cf4cfe0aacbb4), NT2H_D(0x3fb2488a6b5cb5e5),
+                             NT2H_D(0x3fd53dd7a67c7e9f), NT2H_D(0x3fec1986509e687b),
+                             NT2H_D(0x3ff54dfe9b258a60), NT2H_D(0x3feffffffbbb552b)};
+  constexpr double e2q[7] = {NT2H_D(0x3f89a996639b0d00), NT2H_D(0x3fc033c113a7deee),
+                             NT2H_D(0x3fe307622fcff772), NT2H_D(0x3ff9e677c2777c3c),
+                             NT2H_D(0x40053b1052dca8bd), NT2H_D(0x4003adeae79b9708),
+                             NT2H_D(0x3ff0000000000000)};
+  constexpr double e3p[7] = {0.0, NT2H_D(0x3f971d0907ea7a92), NT2H_D(0x3fc42210f88b9d43),
+                             NT2H_D(0x3fe29be1cff90d94), NT2H_D(0x3ff44744306832ae),
+                             NT2H_D(0x3ff9fa202deb88e5), NT2H_D(0x3fefff5a9e697ae2)};
+  constexpr double e3q[7] = {NT2H_D(0x3fa47bd61bbb3843), NT2H_D(0x3fd1d7ab774bb837),
+                             NT2H_D(0x3ff0cfd4cb6cde9f), NT2H_D(0x400315ffdfd5ce91),
+                             NT2H_D(0x400afd487397568f), NT2H_D(0x400602f24bf3fdb6),
+                             NT2H_D(0x3ff0000000000000)};
+  const double x = __builtin_fabs(a0), xx = x * x;
+  double r;
+  if (Scalar) {
+    if (a0 != a0) return a0;
+    r = 0.0;
+    if (x <= 0.0000000001) r = 1.0 - (x * 2.0) / __builtin_sqrt(NT2H_D(0x400921fb54442d18));
+    else if (x < 0.65) r = 1.0 - x * (horner<P>(e1p, xx) / horner<P>(e1q, xx));
+    else if (x < 2.2) r = exp_d<P>(-xx) * (horner<P>(e2p, x) / horner<P>(e2q, x));
+    else if (x < 6.0) r = exp_d<P>(-xx) * (horner<P>(e3p, x) / horner<P>(e3q, x));
+    return (a0 < 0.0) ? 2.0 - r : r;
+  }
+  if (x < 0.65) r = 1.0 - x * (horner<P>(e1p, xx) / horner<P>(e1q, xx));
+  else if (x < 2.2) r = exp_d<P>(-xx) * (horner<P>(e2p, x) / horner<P>(e2q, x));
+  else r = exp_d<P>(-xx) * (horner<P>(e3p, x) / horner<P>(e3q, x));
+  r = (x == __builtin_inf()) ? 0.0 : r;
+  return (a0 < 0.0) ? 2.0 - r : r;
+}
+
+// normcdf(x) = Half*erfc(-Sqrt_2o_2*x):
+// modules/core/statistics/include/nt2/statistics/functions/generic/normcdf.hpp:33
+template <class P>
+NT2H_FN float normcdf_f(float a0) {
+  NT2H_STRICT
+  return 0.5f * erfc_f<P>((-NT2H_F(0x3f3504f3)) * a0);
+}
+template <class P, bool Scalar>
+NT2H_FN double normcdf_d(double a0) {
+  NT2H_STRICT
+  return 0.5 * erfc_d<P, Scalar>((-NT2H_D(0x3FE6A09E667F3BCD)) * a0);
+}
+
+// fastnormcdf (single only): generic/fastnormcdf.hpp:39-54
+template <class P>
+NT2H_FN float fastnormcdf_f(float a0) {
+  NT2H_STRICT
+  constexpr float c[5] = {NT2H_F(0x3faa466f), NT2H_F(0xbfe91eea), NT2H_F(0x3fe40778),
+                          NT2H_F(0xbeb68f87), NT2H_F(0x3ea385fa)};
+  const float l = __builtin_fabsf(a0);
+  const float k = 1.0f / fma<P>(0.2316419f, l, 1.0f);
+  float w = horner<P>(c, k);
+  w = w * ((k * NT2H_F(0x3ecc422a)) * exp_f<P>((-(l * l)) * 0.5f));
+  return (a0 > 0.0f) ? 1.0f - w : w;
+}
+
+// ---------------------------------------------------------------------------
+// Trigonometric range reduction.
+// fdlibm kernel (single: 8-bit chunks, jk = 9; double: 24-bit chunks, jk = 4;
+// prec = 2) as NT2 embeds it:
+// modules/core/trigonometric/include/nt2/trigonometric/functions/scalar/rem_pio2.hpp:234-402, :690-905.
+// Rarely reached (|x| > 2^18*pi single, |x| > 2^19*pi/2 double); runs per lane.
+// ---------------------------------------------------------------------------
+template <class T, int B>
+__device__ __noinline__ int kernel_rem_pio2(const T* x, T* y, int e0, int nx) {
+  NT2H_STRICT
+  const int jk = (B == 8) ? 9 : 4;
+  const int jp = jk;
+  const T two_b = (T)(1 << B);
+  const T twon_b = (T)1 / two_b;
+  const int32_t full = 1 << B;
+  const int32_t maskall = full - 1;
+  int32_t jz, jx, jv, carry, n, iq[20], i, j, k, m, q0, ih;
+  T z, fw, f[20], fq[20], q[20], PIo2[(B == 8) ? 11 : 8];
+  for (i = 0; i < ((B == 8) ? 11 : 8); ++i)
+    PIo2[i] = (B == 8) ? (T)as_f(nt2_hip_tables::pio2_8_bits[i])
+                       : (T)as_d(nt2_hip_tables::pio2_24_bits[i]);
+  jx = nx - 1;
+  jv = (e0 - 3) / B;
+  if (jv < 0) jv = 0;
+  q0 = e0 - B * (jv + 1);
+  j = jv - jx;
+  m = jx + jk;
+  for (i = 0; i <= m; i++, j++) {
+    const int32_t d = (j < 0) ? 0 : ((B == 8) ? nt2_hip_tables::two_over_pi_8[j]
+                                              : nt2_hip_tables::two_over_pi_24[j]);
+    f[i] = (T)d;
+  }
+  for (i = 0; i <= jk; i++) {
+    for (j = 0, fw = 0; j <= jx; j++) fw += x[j] * f[jx + i - j];
+    q[i] = fw;
+  }
+  jz = jk;
+  for (;;) {
+    for (i = 0, j = jz, z = q[jz]; j > 0; i++, j--) {
+      fw = (T)((int32_t)(twon_b * z));
+      iq[i] = (int32_t)(z - two_b * fw);
+      z = q[j - 1] + fw;
+    }
+    if constexpr (B == 8) {
+      z = __builtin_ldexpf(z, q0);
+      z -= 8.0f * __builtin_floorf(z * 0.125f);
+    } else {
+      z = __builtin_ldexp(z, q0);
+      z -= 8.0 * __builtin_floor(z * 0.125);
+    }
+    n = (int32_t)z;
+    z -= (T)n;
+    ih = 0;
+    if (q0 > 0) {
+      i = (iq[jz - 1] >> (B - q0));
+      n += i;
+      iq[jz - 1] -= i << (B - q0);
+      ih = iq[jz - 1] >> (B - 1 - q0);
+    } else if (q0 == 0) {
+      // rem_pio2.hpp:290 shifts the single-precision chunk by 8, :791 the
+      // double one by 23; kept as NT2 computes it.
+      ih = iq[jz - 1] >> ((B == 8) ? 8 : 23);
+    } else if (z >= (T)0.5) {
+      ih = 2;
+    }
+    if (ih > 0) {
+      n += 1;
+      carry = 0;
+      for (i = 0; i < jz; i++) {
+        j = iq[i];
+        if (carry == 0) {
+          if (j != 0) { carry = 1; iq[i] = full - j; }
+        } else {
+          iq[i] = maskall - j;
+        }
+      }
+      if (q0 == 1) iq[jz - 1] &= (maskall >> 1);
+      else if (q0 == 2) iq[jz - 1] &= (maskall >> 2);
+      if (ih == 2) {
+        z = (T)1 - z;
+        if (carry != 0) z -= (T)__builtin_ldexp(1.0, q0);  // exact power of two
+      }
+    }
+    bool again = false;
+    if (z == (T)0) {
+      j = 0;
+      for (i = jz - 1; i >= jk; i--) j |= iq[i];
+      if (j == 0) {
+        for (k = 1; iq[jk - k] == 0; k++) {
+        }
+        for (i = jz + 1; i <= jz + k; i++) {
+          f[jx + i] = (T)((B == 8) ? nt2_hip_tables::two_over_pi_8[jv + i]
+                                   : nt2_hip_tables::two_over_pi_24[jv + i]);
+          for (j = 0, fw = 0; j <= jx; j++) fw += x[j] * f[jx + i - j];
+          q[i] = fw;
+        }
+        jz += k;
+        again = true;
+      }
+    }
+    if (!again) break;
+  }
+  if (z == (T)0) {
+    jz -= 1;
+    q0 -= B;
+    while (iq[jz] == 0) { jz--; q0 -= B; }
+  } else {
+    if constexpr (B == 8) z = __builtin_ldexpf(z, -q0);
+    else z = __builtin_ldexp(z, -q0);
+    if (z >= two_b) {
+      fw = (T)((int32_t)(twon_b * z));
+      iq[jz] = (int32_t)(z - two_b * fw);
+      jz += 1;
+      q0 += B;
+      iq[jz] = (int32_t)fw;
+    } else {
+      iq[jz] = (int32_t)z;
+    }
+  }
+  fw = (T)__builtin_ldexp(1.0, q0);
+  for (i = jz; i >= 0; i--) { q[i] = fw * (T)iq[i]; fw *= twon_b; }
+  for (i = jz; i >= 0; i--) {
+    for (fw = 0, k = 0; k <= jp && k <= jz - i; k++) fw += PIo2[k] * q[i + k];
+    fq[jz - i] = fw;
+  }
+  fw = 0;
+  for (i = jz; i >= 0; i--) fw += fq[i];
+  y[0] = (ih == 0) ? fw : -fw;
+  fw = fq[0] - fw;
+  for (i = 1; i <= jz; i++) fw += fq[i];
+  y[1] = (ih == 0) ? fw : -fw;
+  return n & 7;
+}
+
+// __ieee754_rem_pio2f (rem_pio2.hpp:114-229) with NT2's n&3 wrapper (:37-44).
+__device__ __noinline__ int rem_pio2f_fdlibm(float x, float* xr) {
+  NT2H_STRICT
+  const float two8 = 2.5600000000e+02f, invpio2 = NT2H_F(0x3f22f984),
+              pio2_1 = NT2H_F(0x3fc90f80), pio2_1t = NT2H_F(0x37354443),
+              pio2_2 = NT2H_F(0x37354400), pio2_2t = NT2H_F(0x2e85a308),
+              pio2_3 = NT2H_F(0x2e85a300), pio2_3t = NT2H_F(0x248d3132);
+  float y0, z, w, t, r, fn, tx[3];
+  int32_t e0, i, j, nx, n, ix, hx;
+  hx = (int32_t)bits(x);
+  ix = hx & 0x7fffffff;
+  if (ix <= 0x3f490fd8) { *xr = x; return 0; }
+  if (ix < 0x4016cbe4) {
+    if (hx > 0) {
+      z = x - pio2_1;
+      if ((ix & 0xfffffff0) != 0x3fc90fd0) { y0 = z - pio2_1t; }
+      else { z -= pio2_2; y0 = z - pio2_2t; }
+      *xr = y0; return 1;
+    }
+    z = x + pio2_1;
+    if ((ix & 0xfffffff0) != 0x3fc90fd0) { y0 = z + pio2_1t; }
+    else { z += pio2_2; y0 = z + pio2_2t; }
+    *xr = y0; return 3;
+  }
+  if (ix <= 0x43490f80) {
+    t = __builtin_fabsf(x);
+    n = (int32_t)(t * invpio2 + 0.5f);
+    fn = (float)n;
+    r = t - fn * pio2_1;
+    w = fn * pio2_1t;
+    if (n < 32 && (int32_t)(ix & 0xffffff00) != nt2_hip_tables::npio2_hw_f[n - 1]) {
+      y0 = r - w;
+    } else {
+      j = ix >> 23;
+      y0 = r - w;
+      uint32_t high = bits(y0);
+      i = j - ((high >> 23) & 0xff);
+      if (i > 8) {
+        t = r;
+        w = fn * pio2_2;
+        r = t - w;
+        w = fn * pio2_2t - ((t - r) - w);
+        y0 = r - w;
+        high = bits(y0);
+        i = j - ((high >> 23) & 0xff);
+        if (i > 25) {
+          t = r;
+          w = fn * pio2_3;
+          r = t - w;
+          w = fn * pio2_3t - ((t - r) - w);
+          y0 = r - w;
+        }
+      }
+    }
+    if (hx < 0) { *xr = -y0; return (-n) & 3; }
+    *xr = y0; return n & 3;
+  }
+  if (ix >= 0x7f800000) { *xr = x - x; return 0; }
+  e0 = (ix >> 23) - 134;
+  z = as_f((uint32_t)(ix - (int32_t)((uint32_t)e0 << 23)));
+  for (i = 0; i < 2; i++) { tx[i] = (float)((int32_t)(z)); z = (z - tx[i]) * two8; }
+  tx[2] = z;
+  nx = 3;
+  while (tx[nx - 1] == 0.0f) nx--;
+  float y[2];
+  n = kernel_rem_pio2<float, 8>(tx, y, e0, nx);
+  if (hx < 0) { *xr = -y[0]; return (-n) & 3; }
+  *xr = y[0];
+  return n & 3;
+}
+
+// __ieee754_rem_pio2 (double): rem_pio2.hpp:521-621, NT2 wrapper :414-427.
+__device__ __noinline__ int rem_pio2d_fdlibm(double x, double* xr) {
+  NT2H_STRICT
+  if (x == __builtin_inf()) { *xr = __builtin_nan(""); return 0; }
+  const double two24 = 1.67772160000000000000e+07, invpio2 = NT2H_D(0x3FE45F306DC9C883),
+               pio2_1 = NT2H_D(0x3FF921FB54400000), pio2_1t = NT2H_D(0x3DD0B4611A626331),
+               pio2_2 = NT2H_D(0x3DD0B4611A600000), pio2_2t = NT2H_D(0x3BA3198A2E037073),
+               pio2_3 = NT2H_D(0x3BA3198A2E000000), pio2_3t = NT2H_D(0x397B839A252049C1);
+  double z, w, t, r, fn, y0, tx[3];
+  int32_t e0, i, j, nx, n, ix, hx;
+  const uint64_t xb = bits(x);
+  hx = (int32_t)(xb >> 32);
+  ix = hx & 0x7fffffff;
+  if (ix <= 0x3fe921fb) { *xr = x; return 0; }
+  if (ix < 0x4002d97c) {
+    if (hx > 0) {
+      z = x - pio2_1;
+      if (ix != 0x3ff921fb) { y0 = z - pio2_1t; }
+      else { z -= pio2_2; y0 = z - pio2_2t; }
+      *xr = y0; return 1;
+    }
+    z = x + pio2_1;
+    if (ix != 0x3ff921fb) { y0 = z + pio2_1t; }
+    else { z += pio2_2; y0 = z + pio2_2t; }
+    *xr = y0; return 3;
+  }
+  if (ix <= 0x413921fb) {
+    t = __builtin_fabs(x);
+    n = (int32_t)(t * invpio2 + 0.5);
+    fn = (double)n;
+    r = t - fn * pio2_1;
+    w = fn * pio2_1t;
+    if ((n < 32) && (n > 0) && (ix != nt2_hip_tables::npio2_hw_d[n - 1])) {
+      y0 = r - w;
+    } else {
+      j = ix >> 20;
+      y0 = r - w;
+      uint32_t high = (uint32_t)(bits(y0) >> 32);
+      i = j - ((high >> 20) & 0x7ff);
+      if (i > 16) {
+        t = r;
+        w = fn * pio2_2;
+        r = t - w;
+        w = fn * pio2_2t - ((t - r) - w);
+        y0 = r - w;
+        high = (uint32_t)(bits(y0) >> 32);
+        i = j - ((high >> 20) & 0x7ff);
+        if (i > 49) {
+          t = r;
+          w = fn * pio2_3;
+          r = t - w;
+          w = fn * pio2_3t - ((t - r) - w);
+          y0 = r - w;
+        }
+      }
+    }
+    if (hx < 0) { *xr = -y0; return (-n) & 3; }
+    *xr = y0; return n & 3;
+  }
+  if (ix >= 0x7ff00000) { *xr = x - x; return 0; }
+  const uint32_t low = (uint32_t)xb;
+  e0 = (ix >> 20) - 1046;
+  z = as_d(((uint64_t)(uint32_t)(ix - (int32_t)((uint32_t)e0 << 20)) << 32) | low);
+  for (i = 0; i < 2; i++) { tx[i] = (double)((int32_t)(z)); z = (z - tx[i]) * two24; }
+  tx[2] = z;
+  nx = 3;
+  while (tx[nx - 1] == 0.0) nx--;
+  double y[2];
+  n = kernel_rem_pio2<double, 24>(tx, y, e0, nx);
+  if (hx < 0) { *xr = -y[0]; return (-n) & 3; }
+  *xr = y[0];
+  return n & 3;
+}
+
+NT2H_FN bool ngt(float a, float b) { return !(a > b); }
+NT2H_FN bool ngt(double a, double b) { return !(a > b); }
+
+// f64 reduction cascade over a group of G lanes (G = 4: one NT2 AVX pack;
+// G = 1: the scalar tail): scalar/impl/trigo/trig_reduction.hpp:94-215;
+// straight rem_pio2_straight.hpp:58-63, cephes rem_pio2_cephes.hpp:65-72,
+// medium rem_pio2_medium.hpp:68-83; beyond Medium_pi (2^18*pi) fdlibm per lane.
+template <int G>
+NT2H_FN void reduce_d(const double* x, double* xr, int* n) {
+  NT2H_STRICT
+  bool all4 = true, all2 = true, all20 = true, allm = true;
+#pragma unroll
+  for (int i = 0; i < G; ++i) {
+    all4 = all4 && ngt(x[i], NT2H_D(0x3fe921fb54442d18));
+    all2 = all2 && ngt(x[i], NT2H_D(0x3ff921fb54442d18));
+    all20 = all20 && ngt(x[i], NT2H_D(0x404f6a7a2955385e));
+    allm = allm && ngt(x[i], NT2H_D(0x412921fb54442d18));
+  }
+  if (all4) {
+#pragma unroll
+    for (int i = 0; i < G; ++i) { xr[i] = x[i]; n[i] = 0; }
+  } else if (all2) {
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+      double r = x[i] - NT2H_D(0x3FF921FB54400000);
+      r -= NT2H_D(0x3DD0B4611A600000);
+      r -= NT2H_D(0x3BA3198A2E000000);
+      xr[i] = r; n[i] = 1;
+    }
+  } else if (all20) {
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+      const double xi = __builtin_rint(x[i] * NT2H_D(0x3fe45f306dc9c883));
+      double r = x[i] - xi * NT2H_D(0x3FF921FB54400000);
+      r -= xi * NT2H_D(0x3DD0B4611A600000);
+      r -= xi * NT2H_D(0x3BA3198A2E000000);
+      xr[i] = r; n[i] = ((int)(int64_t)xi) & 3;
+    }
+  } else if (allm) {
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+      const double t = x[i];
+      const double fn = __builtin_rint(t * NT2H_D(0x3FE45F306DC9C883));
+      double r = t - fn * NT2H_D(0x3FF921FB54400000);
+      double w = fn * NT2H_D(0x3DD0B4611A626331);
+      double t2 = r;
+      w = fn * NT2H_D(0x3DD0B4611A600000);
+      r = t2 - w;
+      w = fn * NT2H_D(0x3BA3198A2E037073) - ((t2 - r) - w);
+      t2 = r;
+      w = fn * NT2H_D(0x3BA3198A2E000000);
+      r = t2 - w;
+      w = fn * NT2H_D(0x397B839A252049C1) - ((t2 - r) - w);
+      xr[i] = r - w;
+      n[i] = ((int)(int64_t)fn) & 3;
+    }
+  } else {
+    for (int i = 0; i < G; ++i) n[i] = rem_pio2d_fdlibm(x[i], &xr[i]);
+  }
+}
+
+// f32 reduction cascade over G lanes (G = 8: one NT2 AVX pack; 1: scalar
+// tail): trig_reduction.hpp:94-215; when every lane is within 2^18*pi the
+// pack is split into halves of 4 doubles that each run the f64 cascade
+// (use_conversion, :190-215); beyond that, fdlibm single per lane.
+template <int G>
+NT2H_FN void reduce_f(const float* x, float* xr, int* n) {
+  NT2H_STRICT
+  bool all4 = true, all2 = true, all20 = true, allm = true, alld = true;
+#pragma unroll
+  for (int i = 0; i < G; ++i) {
+    all4 = all4 && ngt(x[i], NT2H_F(0x3f490fdb));
+    all2 = all2 && ngt(x[i], NT2H_F(0x3fc90fdb));
+    all20 = all20 && ngt(x[i], NT2H_F(0x427b53d1));
+    allm = allm && ngt(x[i], NT2H_F(0x43490fdb));
+    alld = alld && ngt(x[i], NT2H_F(0x49490fdb));
+  }
+  if (all4) {
+#pragma unroll
+    for (int i = 0; i < G; ++i) { xr[i] = x[i]; n[i] = 0; }
+  } else if (all2) {
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+      float r = x[i] - NT2H_F(0x3fc90f80);
+      r -= NT2H_F(0x37354400);
+      r -= NT2H_F(0x2e85a300);
+      xr[i] = r; n[i] = 1;
+    }
+  } else if (all20) {
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+      const float xi = __builtin_rintf(x[i] * NT2H_F(0x3f22f983));
+      float r = x[i] - xi * NT2H_F(0x3fc90f80);
+      r -= xi * NT2H_F(0x37354400);
+      r -= xi * NT2H_F(0x2e85a300);
+      xr[i] = r; n[i] = ((int)xi) & 3;
+    }
+  } else if (allm) {
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+      const float t = x[i];
+      const float fn = __builtin_rintf(t * NT2H_F(0x3f22f984));
+      float r = t - fn * NT2H_F(0x3fc90f80);
+      float w = fn * NT2H_F(0x37354443);
+      float t2 = r;
+      w = fn * NT2H_F(0x37354400);
+      r = t2 - w;
+      w = fn * NT2H_F(0x2e85a308) - ((t2 - r) - w);
+      t2 = r;
+      w = fn * NT2H_F(0x2e85a300);
+      r = t2 - w;
+      w = fn * NT2H_F(0x248d3132) - ((t2 - r) - w);
+      xr[i] = r - w;
+      n[i] = ((int)fn) & 3;
+    }
+  } else if (alld) {
+    constexpr int H = (G >= 4) ? 4 : G;
+#pragma unroll
+    for (int h = 0; h < G; h += H) {
+      double ux[H], uxr[H];
+      int un[H];
+#pragma unroll
+      for (int i = 0; i < H; ++i) ux[i] = (double)x[h + i];
+      reduce_d<H>(ux, uxr, un);
+#pragma unroll
+      for (int i = 0; i < H; ++i) { xr[h + i] = (float)uxr[i]; n[h + i] = un[i]; }
+    }
+  } else {
+    for (int i = 0; i < G; ++i) n[i] = rem_pio2f_fdlibm(x[i], &xr[i]);
+  }
+}
+
+// evaluation polynomials: simd/common/impl/trigo/f_trig_evaluation.hpp:35-46,
+// d_trig_evaluation.hpp:33-50
+template <class P>
+NT2H_FN float cos_eval_f(float z) {
+  NT2H_STRICT
+  constexpr float c[3] = {NT2H_F(0x37ccf5ce), NT2H_F(0xbab60619), NT2H_F(0x3d2aaaa5)};
+  const float y = horner<P>(c, z);
+  return 1.0f + fma<P>(z, -0.5f, y * (z * z));
+}
+template <class P>
+NT2H_FN float sin_eval_f(float z, float x) {
+  NT2H_STRICT
+  constexpr float c[3] = {NT2H_F(0xb94ca1f9), NT2H_F(0x3c08839d), NT2H_F(0xbe2aaaa2)};
+  const float y1 = horner<P>(c, z);
+  return fma<P>(y1 * z, x, x);
+}
+template <class P>
+NT2H_FN double cos_eval_d(double z) {
+  NT2H_STRICT
+  constexpr double c[7] = {NT2H_D(0x3da8ff831ad9b219), NT2H_D(0xbe21eea7c1e514d4),
+                           NT2H_D(0x3e927e4f8e06d9a5), NT2H_D(0xbefa01a019ddbcd9),
+                           NT2H_D(0x3f56c16c16c15d47), NT2H_D(0xbfa5555555555551),
+                           NT2H_D(0x3fe0000000000000)};
+  return 1.0 - horner<P>(c, z) * z;
+}
+template <class P>
+NT2H_FN double sin_eval_d(double z, double x) {
+  NT2H_STRICT
+  constexpr double c[6] = {NT2H_D(0x3de5d8fd1fcf0ec1), NT2H_D(0xbe5ae5e5a9291691),
+                           NT2H_D(0x3ec71de3567d4896), NT2H_D(0xbf2a01a019bfdf03),
+                           NT2H_D(0x3f8111111110f7d0), NT2H_D(0xbfc5555555555548)};
+  const double y1 = horner<P>(c, z);
+  return fma<P>(y1 * z, x, x);
+}
+
+enum trig_kind { k_cos = 0, k_sin = 1, k_sincos = 2 };
+
+// cos / sin / sincos over a group of G lanes: SIMD trig_base
+// simd/common/impl/trigo/trig_base.hpp:59-71 (cos), :80-91 (sin), :140-156
+// (sincos); scalar trig_base scalar/impl/trigo/trig_base.hpp:126-207 (G == 1:
+// NaN for Inf/NaN inputs first).  For sincos, out0 = sin and out1 = cos.
+template <class P, int G>
+NT2H_FN void trig_group_f(int kind, const float* a0, float* out0, float* out1) {
+  NT2H_STRICT
+  float x[G], xr[G];
+  int n[G];
+#pragma unroll
+  for (int i = 0; i < G; ++i) x[i] = __builtin_fabsf(a0[i]);
+  reduce_f<G>(x, xr, n);
+#pragma unroll
+  for (int i = 0; i < G; ++i) {
+    const uint32_t swap = (uint32_t)n[i] & 1u;
+    const float z = xr[i] * xr[i];
+    const float se = sin_eval_f<P>(z, xr[i]);
+    const float ce = cos_eval_f<P>(z);
+    const uint32_t cos_sign = (swap ^ (((uint32_t)n[i] & 2u) >> 1)) << 31;
+    const uint32_t sin_sign = (((uint32_t)n[i] & 2u) << 30) ^ (bits(a0[i]) & 0x80000000u);
+    float c = as_f(bits(swap ? se : ce) ^ cos_sign);
+    float s = as_f(bits(swap ? ce : se) ^ sin_sign);
+    if (G == 1 && !__builtin_isfinite(a0[i])) { c = __builtin_nanf(""); s = c; }
+    if (kind == k_cos) out0[i] = c;
+    else if (kind == k_sin) out0[i] = s;
+    else { out0[i] = s; out1[i] = c; }
+  }
+}
+
+template <class P, int G>
+NT2H_FN void trig_group_d(int kind, const double* a0, double* out0, double* out1) {
+  NT2H_STRICT
+  double x[G], xr[G];
+  int n[G];
+#pragma unroll
+  for (int i = 0; i < G; ++i) x[i] = __builtin_fabs(a0[i]);
+  reduce_d<G>(x, xr, n);
+#pragma unroll
+  for (int i = 0; i < G; ++i) {
+    const uint64_t swap = (uint64_t)n[i] & 1u;
+    const double z = xr[i] * xr[i];
+    const double se = sin_eval_d<P>(z, xr[i]);
+    const double ce = cos_eval_d<P>(z);
+    const uint64_t cos_sign = (swap ^ (((uint64_t)n[i] & 2u) >> 1)) << 63;
+    const uint64_t sin_sign = (((uint64_t)n[i] & 2u) << 62) ^ (bits(a0[i]) & 0x8000000000000000ull);
+    double c = as_d(bits(swap ? se : ce) ^ cos_sign);
+    double s = as_d(bits(swap ? ce : se) ^ sin_sign);
+    if (G == 1 && !__builtin_isfinite(a0[i])) { c = __builtin_nan(""); s = c; }
+    if (kind == k_cos) out0[i] = c;
+    else if (kind == k_sin) out0[i] = s;
+    else { out0[i] = s; out1[i] = c; }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// uint8 saturated / selection ops (bit-exact):
+// adds: boost/simd/arithmetic/functions/scalar/adds.hpp:54-67 (min(255, a+b));
+// subs: max(0, a-b); selinc/seldec: boolean/functions/scalar/selinc.hpp:18-29,
+// seldec.hpp:18-29; if_zero_else_one: boolean/functions/scalar/if_zero_else_one.hpp:21-52.
+// ---------------------------------------------------------------------------
+NT2H_FN uint32_t adds_u8(uint32_t a, uint32_t b) {
+  const uint32_t s = a + b;
+  return s > 255u ? 255u : s;
+}
+NT2H_FN uint32_t subs_u8(uint32_t a, uint32_t b) { return a > b ? a - b : 0u; }
+
+}  // namespace dm
+}}  // namespace nt2::hip

@@ -1,0 +1,114 @@
+"""ctypes binding of libnt2h.so (the C ABI declared in include/nt2h.h)."""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB_NAME = "libnt2h.so"
+
+EXACT, FAST = 0, 1
+BS_TEXTBOOK, BS_DEMO = 0, 1
+OPS = {"exp": 0, "log": 1, "sqrt": 2, "erfc": 3, "normcdf": 4,
+       "fastnormcdf": 5, "cos": 6, "sin": 7, "abs": 8}
+H2D, D2H, D2D = 1, 2, 3
+
+_lib = None
+
+
+class Nt2hError(RuntimeError):
+    pass
+
+
+def lib_path():
+    return os.path.join(HERE, LIB_NAME)
+
+
+def lib():
+    """Load libnt2h.so; fail loudly when it is missing (there is no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = lib_path()
+    if not os.path.exists(path):
+        raise Nt2hError("libnt2h.so is not built (expected at %s); run "
+                        "`python -c 'import __graft_entry__ as g; g.build()'`" % path)
+    L = ctypes.CDLL(path)
+    P, S, I, F, D, U64 = (ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int,
+                          ctypes.c_float, ctypes.c_double, ctypes.c_uint64)
+    sig = {
+        "nt2h_abi_version": ([], I),
+        "nt2h_last_error": ([], ctypes.c_char_p),
+        "nt2h_device_info": ([P, P, P, S], I),
+        "nt2h_malloc": ([P, S], I),
+        "nt2h_free": ([P], I),
+        "nt2h_memcpy": ([P, P, S, I, P], I),
+        "nt2h_stream_create": ([P], I),
+        "nt2h_stream_destroy": ([P], I),
+        "nt2h_stream_sync": ([P], I),
+        "nt2h_fill_uniform_f32": ([P, S, D, D, U64, P], I),
+        "nt2h_fill_uniform_f64": ([P, S, D, D, U64, P], I),
+        "nt2h_fill_normal_f32": ([P, S, U64, P], I),
+        "nt2h_fill_frames_u8": ([P, I, I, I, I, I, U64, P], I),
+        "nt2h_map_f32": ([I, S, P, P, I, P], I),
+        "nt2h_map_f64": ([I, S, P, P, I, P], I),
+        "nt2h_sincos_f32": ([S, P, P, P, I, P], I),
+        "nt2h_sincos_f64": ([S, P, P, P, I, P], I),
+        "nt2h_saxpy_f32": ([S, F, P, P, P], I),
+        "nt2h_stream_copy": ([P, P, S, I, P], I),
+        "nt2h_sum_f32": ([S, P, P, P], I),
+        "nt2h_dot_f32": ([S, P, P, P, P], I),
+        "nt2h_blackscholes_f32": ([S, P, P, P, F, F, P, P, I, I, P], I),
+        "nt2h_blackscholes_f64": ([S, P, P, P, D, D, P, P, I, P], I),
+        "nt2h_sigmadelta_u8": ([S, I, P, S, P, P, P, I, P, P], I),
+        "nt2h_sigmadelta_step_u8": ([S, P, P, P, P, I, P], I),
+        "nt2h_expr4_f32": ([S, P, P, P, I, P], I),
+        "nt2h_expr4_sum_f32": ([S, P, P, P, I, P], I),
+    }
+    for name, (args, res) in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = L
+    _extend(L)
+    return L
+
+
+def _extend(L):
+    """Signatures of the batch and server entry points (optional symbols)."""
+    P, S, I, D = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_double
+    extra = {
+        "nt2h_batch_create": ([P, I, P, P, P, I], I),  # noqa: E501
+        "nt2h_batch_run": ([P, P, P], I),
+        "nt2h_batch_run_plain": ([P, P, P], I),
+        "nt2h_batch_set_config": ([P, I, I, I], I),
+        "nt2h_batch_autotune": ([P, P, P], I),
+        "nt2h_batch_get_config": ([P, P, P, P], I),
+        "nt2h_batch_destroy": ([P], I),
+    }
+    for name, (args, res) in extra.items():
+        if hasattr(L, name):
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+
+
+def check(rc, what=""):
+    if rc != 0:
+        msg = lib().nt2h_last_error()
+        raise Nt2hError("%s failed (%d): %s" % (what, rc, msg.decode() if msg else ""))
+    return rc
+
+
+def ptr(t):
+    """Device/host address of a torch tensor or numpy array (None -> NULL)."""
+    if t is None:
+        return None
+    if hasattr(t, "data_ptr"):
+        return ctypes.c_void_p(t.data_ptr())
+    return t.ctypes.data_as(ctypes.c_void_p)
+
+
+def stream_handle(stream=None):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)

@@ -1,0 +1,109 @@
+"""Python-level calls of the C ABI on torch device tensors (test and bench
+plumbing).  Every function launches on torch's current stream."""
+import ctypes
+
+import torch
+
+from .native import (BS_TEXTBOOK, EXACT, OPS, check, lib, ptr, stream_handle)
+
+
+def _s():
+    return stream_handle()
+
+
+def fill_uniform(n, lo, hi, seed, dtype=torch.float32, device="cuda"):
+    out = torch.empty(n, dtype=dtype, device=device)
+    f = lib().nt2h_fill_uniform_f32 if dtype == torch.float32 else lib().nt2h_fill_uniform_f64
+    check(f(ptr(out), n, lo, hi, seed, _s()), "fill_uniform")
+    return out
+
+
+def fill_normal(n, seed, device="cuda"):
+    out = torch.empty(n, dtype=torch.float32, device=device)
+    check(lib().nt2h_fill_normal_f32(ptr(out), n, seed, _s()), "fill_normal")
+    return out
+
+
+def fill_frames(h, w, nframes, nrect=8, noise=8, seed=3, device="cuda"):
+    out = torch.empty((nframes, h * w), dtype=torch.uint8, device=device)
+    check(lib().nt2h_fill_frames_u8(ptr(out), h, w, nframes, nrect, noise, seed, _s()), "fill_frames")
+    return out
+
+
+def map(op, x, policy=EXACT, out=None):
+    y = torch.empty_like(x) if out is None else out
+    f = lib().nt2h_map_f32 if x.dtype == torch.float32 else lib().nt2h_map_f64
+    check(f(OPS[op], x.numel(), ptr(x), ptr(y), policy, _s()), "map " + op)
+    return y
+
+
+def sincos(x, policy=EXACT):
+    s, c = torch.empty_like(x), torch.empty_like(x)
+    f = lib().nt2h_sincos_f32 if x.dtype == torch.float32 else lib().nt2h_sincos_f64
+    check(f(x.numel(), ptr(x), ptr(s), ptr(c), policy, _s()), "sincos")
+    return s, c
+
+
+def stream_copy(src, dst, grid=0):
+    check(lib().nt2h_stream_copy(ptr(src), ptr(dst), src.numel() * src.element_size(), grid, _s()),
+          "stream_copy")
+
+
+def saxpy(a, x, y):
+    check(lib().nt2h_saxpy_f32(x.numel(), ctypes.c_float(a), ptr(x), ptr(y), _s()), "saxpy")
+    return y
+
+
+def sum(x, out=None):
+    out = torch.empty(1, dtype=torch.float64, device=x.device) if out is None else out
+    check(lib().nt2h_sum_f32(x.numel(), ptr(x), ptr(out), _s()), "sum")
+    return out
+
+
+def dot(x, y, out=None):
+    out = torch.empty(1, dtype=torch.float64, device=x.device) if out is None else out
+    check(lib().nt2h_dot_f32(x.numel(), ptr(x), ptr(y), ptr(out), _s()), "dot")
+    return out
+
+
+def blackscholes(S, X, T, r, v, mode=BS_TEXTBOOK, policy=EXACT, call=None, put=None):
+    call = torch.empty_like(S) if call is None else call
+    if S.dtype == torch.float32:
+        if put is None and mode == BS_TEXTBOOK:
+            put = torch.empty_like(S)
+        check(lib().nt2h_blackscholes_f32(S.numel(), ptr(S), ptr(X), ptr(T), ctypes.c_float(r),
+                                          ctypes.c_float(v), ptr(call), ptr(put), mode, policy,
+                                          _s()), "blackscholes_f32")
+    else:
+        put = torch.empty_like(S) if put is None else put
+        check(lib().nt2h_blackscholes_f64(S.numel(), ptr(S), ptr(X), ptr(T), r, v, ptr(call),
+                                          ptr(put), policy, _s()), "blackscholes_f64")
+    return call, put
+
+
+def sigmadelta(frames, N, counts=True, bkg=None, var=None, label=None):
+    nf, npix = frames.shape
+    bkg = frames[0].clone() if bkg is None else bkg
+    var = torch.ones(npix, dtype=torch.uint8, device=frames.device) if var is None else var
+    label = torch.zeros(npix, dtype=torch.uint8, device=frames.device) if label is None else label
+    cnt = torch.zeros(nf, dtype=torch.int32, device=frames.device) if counts else None
+    check(lib().nt2h_sigmadelta_u8(npix, nf, ptr(frames), frames.stride(0), ptr(bkg), ptr(var),
+                                   ptr(label), N, ptr(cnt), _s()), "sigmadelta")
+    return bkg, var, label, cnt
+
+
+def sigmadelta_step(frame, bkg, var, label, N):
+    check(lib().nt2h_sigmadelta_step_u8(frame.numel(), ptr(frame), ptr(bkg), ptr(var), ptr(label),
+                                        N, _s()), "sigmadelta_step")
+
+
+def expr4(x, y, policy=EXACT):
+    z = torch.empty_like(x)
+    check(lib().nt2h_expr4_f32(x.numel(), ptr(x), ptr(y), ptr(z), policy, _s()), "expr4")
+    return z
+
+
+def expr4_sum(x, y, policy=EXACT, out=None):
+    out = torch.empty(1, dtype=torch.float64, device=x.device) if out is None else out
+    check(lib().nt2h_expr4_sum_f32(x.numel(), ptr(x), ptr(y), ptr(out), policy, _s()), "expr4_sum")
+    return out
