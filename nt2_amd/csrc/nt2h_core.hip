@@ -307,11 +307,12 @@ __global__ __launch_bounds__(kBlock) void saxpy_k(size_t n, float a, const float
 // HBM reference kernel: y = x (16-byte vectors, grid-stride, streaming
 // stores).  Used to measure the achievable HBM bandwidth on the box.
 // ============================================================================
-__global__ __launch_bounds__(kBlock) void stream_copy_k(const float4* __restrict__ x,
-                                                        float4* __restrict__ y, size_t n4) {
+typedef float v4f __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(kBlock) void stream_copy_k(const v4f* __restrict__ x,
+                                                        v4f* __restrict__ y, size_t n4) {
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) {
-    const float4 v = __builtin_nontemporal_load(x + i);
+    const v4f v = __builtin_nontemporal_load(x + i);
     __builtin_nontemporal_store(v, y + i);
   }
 }
@@ -855,7 +856,7 @@ int nt2h_stream_copy(const void* src, void* dst, size_t bytes, int grid, void* s
   const size_t n4 = bytes / 16;
   const unsigned g = grid > 0 ? (unsigned)grid : grid_for(n4);
   hipLaunchKernelGGL(stream_copy_k, dim3(g), dim3(kBlock), 0, as_stream(stream),
-                     (const float4*)src, (float4*)dst, n4);
+                     (const v4f*)src, (v4f*)dst, n4);
   NT2H_CHECK_LAUNCH("stream_copy");
   return NT2H_OK;
 }
