@@ -159,6 +159,58 @@ int nt2h_batch_run_plain(nt2h_batch* batch, double* sums, void* stream);
 int nt2h_batch_autotune(nt2h_batch* batch, double* sums, double* best_ms);
 int nt2h_batch_destroy(nt2h_batch* batch);
 
+/* ---- serving: Black-Scholes requests with continuous batching ---------- */
+/* No counterpart in the reference (NT2 evaluates synchronously on the calling
+ * thread).  Requests are admitted in arrival order against a device-memory
+ * budget (input + output tables), batched into shared launches while up to
+ * max_inflight launches run concurrently, and completed asynchronously. */
+#define NT2H_F32 0
+#define NT2H_F64 1
+#define NT2H_BACKEND_HIP 0  /* device memory, HIP streams */
+#define NT2H_BACKEND_HOST 1 /* host memory and CPU evaluation: scheduler tests without a GPU */
+typedef struct nt2h_server nt2h_server;
+typedef struct {
+  size_t budget_bytes; /* bytes of admitted requests' tables (5*n*sizeof(T) each) */
+  int max_inflight;    /* concurrent batched launches (streams), 1..16 */
+  int max_batch;       /* requests per launch; 1 with max_inflight 1 = plain schedule */
+  int policy;          /* NT2H_EXACT or NT2H_FAST */
+  int backend;         /* NT2H_BACKEND_HIP or NT2H_BACKEND_HOST */
+  int fail_alloc_every; /* test hook: fail every k-th output allocation (0 = never) */
+} nt2h_server_config;
+typedef struct {
+  int dtype;           /* NT2H_F32 or NT2H_F64 */
+  size_t n;            /* options */
+  const void* S;       /* resident input tables (device memory, caller-owned) */
+  const void* X;
+  const void* T;
+  double r, v;         /* rounded to float for f32 requests */
+  void* call;          /* NULL: the server allocates call/put within the budget */
+  void* put;
+  int64_t t_arrival_ns; /* steady-clock arrival stamp; 0 = time of submission */
+} nt2h_bs_request;
+typedef struct {
+  uint64_t completed, failed, cancelled, options, launches;
+  double mean_batch, p50_us, p99_us, mean_us, max_us, span_s, options_per_s;
+  size_t used_bytes, peak_bytes;
+} nt2h_server_stats;
+int nt2h_server_create(const nt2h_server_config* cfg, nt2h_server** server);
+/* NT2H_EINVAL for a malformed request, NT2H_EBUDGET if it can never fit */
+int nt2h_server_submit(nt2h_server* server, const nt2h_bs_request* req, uint64_t* id);
+/* NT2H_OK when done, NT2H_EAGAIN while queued or running, else the failure code */
+int nt2h_server_poll(nt2h_server* server, uint64_t id);
+int nt2h_server_wait(nt2h_server* server, uint64_t id, double timeout_s);
+int nt2h_server_result(nt2h_server* server, uint64_t id, void** call, void** put,
+                       int64_t* latency_ns);
+int nt2h_server_cancel(nt2h_server* server, uint64_t id); /* before launch only */
+int nt2h_server_release(nt2h_server* server, uint64_t id);
+int nt2h_server_drain(nt2h_server* server, double timeout_s);
+int nt2h_server_get_stats(nt2h_server* server, nt2h_server_stats* stats);
+int nt2h_server_reset_stats(nt2h_server* server);
+/* submit items[k] at t0 + arrival_s[k] (busy-wait pacing), release each on completion */
+int nt2h_server_run_trace(nt2h_server* server, int count, const nt2h_bs_request* items,
+                          const double* arrival_s, double timeout_s);
+int nt2h_server_destroy(nt2h_server* server);
+
 #ifdef __cplusplus
 }
 #endif

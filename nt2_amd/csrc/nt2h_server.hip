@@ -1,0 +1,668 @@
+// Serving engine for Black-Scholes pricing requests with continuous batching.
+//
+// Requests (f32 or f64 tables of S, X, T resident in device memory, scalars
+// r and v) arrive at any time.  The scheduler thread admits them in arrival
+// order against a device-memory budget (input tables plus the output tables
+// the server allocates), and whenever a launch slot is free it packs every
+// admitted request that is not yet running into one batched launch: the
+// kernel's blocks are (request, tile) pairs found by a binary search over the
+// batch's tile prefix, and f32 and f64 requests share the launch.  Up to
+// max_inflight batches run concurrently on separate streams, so requests that
+// arrive while others are on the GPU overlap with them.  A completion thread
+// waits on each batch's event in launch order and stamps the completion time;
+// latency = completion - arrival.  The plain schedule (one launch and one
+// wait per request, in arrival order) is the same engine with one slot and
+// batches of one.
+//
+// Every request is evaluated per option with the same device code as the
+// standalone kernel (bs_device.hpp) and the same NT2 pack grouping from the
+// request's first option, so its results are bit-identical alone or in a mix.
+//
+// No counterpart in the reference: NT2 evaluates statements synchronously on
+// the calling thread.  A host back end (config.backend = NT2H_BACKEND_HOST)
+// runs the same scheduler on the CPU with host memory, for testing admission,
+// budget, failure and cancellation handling without a GPU.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "bs_device.hpp"
+#include "common.hpp"
+#include "nt2h.h"
+
+#pragma clang fp contract(off)
+
+using nt2::hip::exact_policy;
+using nt2::hip::fast_policy;
+
+namespace nt2h {
+
+using clk = std::chrono::steady_clock;
+static inline int64_t now_ns() {
+  return std::chrono::duration_cast<std::chrono::nanoseconds>(clk::now().time_since_epoch()).count();
+}
+
+// ---------------------------------------------------------------------------
+// batched kernel
+// ---------------------------------------------------------------------------
+struct ReqDesc {
+  const void* S;
+  const void* X;
+  const void* T;
+  void* call;
+  void* put;
+  uint64_t n;
+  double r, v;
+  uint32_t tile0;  // first tile of this request in the batch
+  uint32_t f64;
+};
+
+constexpr int kSrvBlock = 256;
+constexpr int kSrvTile = kSrvBlock * 4;  // 4 options per thread
+
+template <class P>
+__global__ __launch_bounds__(kSrvBlock) void serve_bs_k(const ReqDesc* __restrict__ reqs, int nreq) {
+  _Pragma("clang fp contract(off)")
+  // find the request owning this tile: last r with reqs[r].tile0 <= blockIdx.x
+  int lo = 0, hi = nreq - 1;
+  const uint32_t b = blockIdx.x;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (reqs[mid].tile0 <= b) lo = mid; else hi = mid - 1;
+  }
+  const ReqDesc d = reqs[lo];
+  const uint64_t base = (uint64_t)(b - d.tile0) * kSrvTile;
+  if (!d.f64) {
+    const float* S = (const float*)d.S;
+    const float* X = (const float*)d.X;
+    const float* T = (const float*)d.T;
+    float* C = (float*)d.call;
+    float* Pt = (float*)d.put;
+    const float r = (float)d.r, v = (float)d.v;
+    const float drift = (v * v) * 0.5f + r, mr = -r;
+    for (uint64_t i = base + threadIdx.x; i < base + kSrvTile && i < d.n; i += kSrvBlock) {
+      float c, p;
+      bs_one_f32<P, NT2H_BS_TEXTBOOK>(S[i], X[i], T[i], mr, v, drift, c, p);
+      C[i] = c;
+      Pt[i] = p;
+    }
+  } else {
+    const double* S = (const double*)d.S;
+    const double* X = (const double*)d.X;
+    const double* T = (const double*)d.T;
+    double* C = (double*)d.call;
+    double* Pt = (double*)d.put;
+    const double r = d.r, v = d.v;
+    const double drift = (v * v) * 0.5 + r, mr = -r;
+    const uint64_t npacks = d.n / 4;
+    // one NT2 pack of 4 per thread; the request's last n%4 options use NT2's scalar erfc
+    const uint64_t pk = base / 4 + threadIdx.x;
+    if (pk < npacks) {
+#pragma unroll
+      for (int l = 0; l < 4; ++l) {
+        const uint64_t i = pk * 4 + l;
+        double c, p;
+        bs_one_f64<P, false>(S[i], X[i], T[i], mr, v, drift, c, p);
+        C[i] = c;
+        Pt[i] = p;
+      }
+    } else if (pk == npacks) {
+      for (uint64_t i = npacks * 4; i < d.n; ++i) {
+        double c, p;
+        bs_one_f64<P, true>(S[i], X[i], T[i], mr, v, drift, c, p);
+        C[i] = c;
+        Pt[i] = p;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host back end (CPU): textbook formula in the request's precision with the
+// C library; used to exercise the scheduler without a GPU
+// ---------------------------------------------------------------------------
+template <class T>
+static void host_bs(const ReqDesc& d) {
+  const T* S = (const T*)d.S;
+  const T* X = (const T*)d.X;
+  const T* Tt = (const T*)d.T;
+  T* C = (T*)d.call;
+  T* Pp = (T*)d.put;
+  const T r = (T)d.r, v = (T)d.v;
+  for (uint64_t i = 0; i < d.n; ++i) {
+    const T vst = v * std::sqrt(Tt[i]);
+    const T d1 = (std::log(S[i] / X[i]) + (v * v * (T)0.5 + r) * Tt[i]) / vst;
+    const T d2 = d1 - vst;
+    const T e = X[i] * std::exp(-r * Tt[i]);
+    const T s2 = (T)0.70710678118654752440;
+    C[i] = S[i] * (T)0.5 * std::erfc(-d1 * s2) - e * (T)0.5 * std::erfc(-d2 * s2);
+    Pp[i] = e * (T)0.5 * std::erfc(d2 * s2) - S[i] * (T)0.5 * std::erfc(d1 * s2);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// engine
+// ---------------------------------------------------------------------------
+enum ReqState { RS_PENDING = 0, RS_ADMITTED = 1, RS_RUNNING = 2, RS_DONE = 3, RS_FAILED = 4,
+                RS_CANCELLED = 5 };
+
+struct Request {
+  uint64_t id;
+  ReqDesc d;
+  size_t bytes;        // budget charge: inputs + outputs
+  bool own_out;        // outputs allocated by the server
+  int state;
+  int status;
+  int64_t t_arrival, t_admit, t_launch, t_done;
+  int slot;
+};
+
+struct Slot {
+  hipStream_t stream = nullptr;
+  hipEvent_t event = nullptr;
+  ReqDesc* d_desc = nullptr;   // device descriptors
+  ReqDesc* h_desc = nullptr;   // pinned staging
+  int cap = 0;
+  bool busy = false;
+  std::vector<uint64_t> reqs;
+  std::thread host_worker;
+};
+
+struct Server {
+  nt2h_server_config cfg;
+  std::mutex mu;
+  std::condition_variable cv_sched, cv_done, cv_slot;
+  std::deque<uint64_t> pending;            // arrival order
+  std::deque<uint64_t> admitted;           // admitted, not yet launched
+  std::map<uint64_t, std::unique_ptr<Request>> reqs;
+  std::deque<int> launch_order;            // slots in flight, launch order
+  std::vector<Slot> slots;
+  size_t used = 0, peak_used = 0;
+  uint64_t next_id = 1;
+  bool stop = false;
+  bool sched_exited = false;
+  std::thread sched, done_thr;
+  int alloc_count = 0;
+  // statistics
+  std::vector<double> lat_us;
+  uint64_t options_done = 0, launches = 0, batched_reqs = 0;
+  int64_t t_first_arrival = -1, t_last_done = -1;
+  uint64_t failed = 0, cancelled = 0;
+};
+
+static size_t elem(const ReqDesc& d) { return d.f64 ? 8 : 4; }
+
+static int srv_alloc(Server* s, Slot& sl, size_t bytes, void** p) {
+  ++s->alloc_count;
+  if (s->cfg.fail_alloc_every > 0 && s->alloc_count % s->cfg.fail_alloc_every == 0)
+    return NT2H_ENOMEM;
+  if (s->cfg.backend == NT2H_BACKEND_HOST) {
+    *p = std::malloc(bytes);
+    return *p ? NT2H_OK : NT2H_ENOMEM;
+  }
+  hipError_t e = hipMallocAsync(p, bytes, sl.stream);
+  return e == hipSuccess ? NT2H_OK : NT2H_ENOMEM;
+}
+
+static void srv_free(Server* s, Request* r) {
+  if (!r->own_out) return;
+  if (s->cfg.backend == NT2H_BACKEND_HOST) {
+    std::free(r->d.call);
+  } else {
+    hipStream_t st = (r->slot >= 0) ? s->slots[r->slot].stream : nullptr;
+    (void)hipFreeAsync(r->d.call, st);
+  }
+  r->d.call = r->d.put = nullptr;
+  r->own_out = false;
+}
+
+// launch every admitted request (up to max_batch) on a free slot; called with mu held
+static void launch_batch(Server* s, int si, std::unique_lock<std::mutex>& lk) {
+  Slot& sl = s->slots[si];
+  std::vector<Request*> batch;
+  while (!s->admitted.empty() && (int)batch.size() < s->cfg.max_batch) {
+    Request* r = s->reqs[s->admitted.front()].get();
+    s->admitted.pop_front();
+    if (r->state != RS_ADMITTED) continue;
+    batch.push_back(r);
+  }
+  if (batch.empty()) return;
+  // output tables (one allocation holds call and put)
+  std::vector<Request*> ok;
+  for (Request* r : batch) {
+    r->slot = si;
+    if (!r->d.call) {
+      void* p = nullptr;
+      const size_t ob = 2 * r->d.n * elem(r->d);
+      int rc = srv_alloc(s, sl, ob, &p);
+      if (rc) {
+        r->state = RS_FAILED;
+        r->status = rc;
+        r->t_done = now_ns();
+        s->used -= r->bytes;
+        ++s->failed;
+        continue;
+      }
+      r->d.call = p;
+      r->d.put = (char*)p + r->d.n * elem(r->d);
+      r->own_out = true;
+    }
+    ok.push_back(r);
+  }
+  s->cv_done.notify_all();
+  s->cv_slot.notify_all();
+  if (ok.empty()) return;
+  if ((int)ok.size() > sl.cap) {
+    if (s->cfg.backend == NT2H_BACKEND_HOST) {
+      delete[] sl.h_desc;
+      sl.h_desc = new ReqDesc[ok.size()];
+    } else {
+      if (sl.h_desc) (void)hipHostFree(sl.h_desc);
+      if (sl.d_desc) (void)hipFree(sl.d_desc);
+      (void)hipHostMalloc((void**)&sl.h_desc, sizeof(ReqDesc) * ok.size(), 0);
+      (void)hipMalloc((void**)&sl.d_desc, sizeof(ReqDesc) * ok.size());
+    }
+    sl.cap = (int)ok.size();
+  }
+  uint32_t tiles = 0;
+  const int64_t t = now_ns();
+  sl.reqs.clear();
+  for (size_t k = 0; k < ok.size(); ++k) {
+    Request* r = ok[k];
+    r->d.tile0 = tiles;
+    tiles += (uint32_t)((r->d.n + kSrvTile - 1) / kSrvTile);
+    sl.h_desc[k] = r->d;
+    r->state = RS_RUNNING;
+    r->t_launch = t;
+    sl.reqs.push_back(r->id);
+  }
+  sl.busy = true;
+  s->launch_order.push_back(si);
+  ++s->launches;
+  s->batched_reqs += ok.size();
+  const int nreq = (int)ok.size();
+  if (s->cfg.backend == NT2H_BACKEND_HOST) {
+    std::vector<ReqDesc> descs(sl.h_desc, sl.h_desc + nreq);
+    if (sl.host_worker.joinable()) sl.host_worker.join();
+    sl.host_worker = std::thread([descs]() {
+      for (const ReqDesc& d : descs) d.f64 ? host_bs<double>(d) : host_bs<float>(d);
+    });
+  } else {
+    lk.unlock();
+    (void)hipMemcpyAsync(sl.d_desc, sl.h_desc, sizeof(ReqDesc) * nreq, hipMemcpyHostToDevice, sl.stream);
+    if (s->cfg.policy == NT2H_EXACT)
+      hipLaunchKernelGGL(serve_bs_k<exact_policy>, dim3(tiles), dim3(kSrvBlock), 0, sl.stream, sl.d_desc, nreq);
+    else
+      hipLaunchKernelGGL(serve_bs_k<fast_policy>, dim3(tiles), dim3(kSrvBlock), 0, sl.stream, sl.d_desc, nreq);
+    (void)hipEventRecord(sl.event, sl.stream);
+    lk.lock();
+  }
+  s->cv_done.notify_all();
+}
+
+static void sched_loop(Server* s) {
+  std::unique_lock<std::mutex> lk(s->mu);
+  for (;;) {
+    // admission in arrival order against the budget
+    while (!s->pending.empty()) {
+      Request* r = s->reqs[s->pending.front()].get();
+      if (r->state == RS_CANCELLED) { s->pending.pop_front(); continue; }
+      if (s->used + r->bytes > s->cfg.budget_bytes) break;
+      s->pending.pop_front();
+      s->used += r->bytes;
+      s->peak_used = std::max(s->peak_used, s->used);
+      r->state = RS_ADMITTED;
+      r->t_admit = now_ns();
+      s->admitted.push_back(r->id);
+    }
+    int free_slot = -1;
+    for (int i = 0; i < (int)s->slots.size(); ++i)
+      if (!s->slots[i].busy) { free_slot = i; break; }
+    if (!s->admitted.empty() && free_slot >= 0) {
+      launch_batch(s, free_slot, lk);
+      continue;
+    }
+    if (s->stop && s->pending.empty() && s->admitted.empty()) break;
+    s->cv_sched.wait(lk);
+  }
+  s->sched_exited = true;
+  s->cv_done.notify_all();
+}
+
+static void done_loop(Server* s) {
+  std::unique_lock<std::mutex> lk(s->mu);
+  for (;;) {
+    while (s->launch_order.empty() && !s->sched_exited) s->cv_done.wait(lk);
+    if (s->launch_order.empty() && s->sched_exited) break;
+    const int si = s->launch_order.front();
+    Slot& sl = s->slots[si];
+    lk.unlock();
+    int status = NT2H_OK;
+    if (s->cfg.backend == NT2H_BACKEND_HOST) {
+      if (sl.host_worker.joinable()) sl.host_worker.join();
+    } else if (hipEventSynchronize(sl.event) != hipSuccess) {
+      status = NT2H_EHIP;
+    }
+    const int64_t t = now_ns();
+    lk.lock();
+    s->launch_order.pop_front();
+    for (uint64_t id : sl.reqs) {
+      Request* r = s->reqs[id].get();
+      r->state = status == NT2H_OK ? RS_DONE : RS_FAILED;
+      r->status = status;
+      r->t_done = t;
+      if (status == NT2H_OK) {
+        s->lat_us.push_back((r->t_done - r->t_arrival) * 1e-3);
+        s->options_done += r->d.n;
+        s->t_last_done = std::max(s->t_last_done, t);
+      } else {
+        ++s->failed;
+      }
+    }
+    sl.reqs.clear();
+    sl.busy = false;
+    s->cv_sched.notify_all();
+    s->cv_slot.notify_all();
+  }
+}
+
+static int validate(const nt2h_bs_request* q) {
+  if (!q) return NT2H_EINVAL;
+  if (q->n == 0 || !q->S || !q->X || !q->T) return NT2H_EINVAL;
+  if (q->dtype != NT2H_F32 && q->dtype != NT2H_F64) return NT2H_EINVAL;
+  if ((q->call == nullptr) != (q->put == nullptr)) return NT2H_EINVAL;
+  if (!(q->v > 0.0) || !std::isfinite(q->r) || !std::isfinite(q->v)) return NT2H_EINVAL;
+  return NT2H_OK;
+}
+
+}  // namespace nt2h
+
+using namespace nt2h;
+
+extern "C" {
+
+int nt2h_server_create(const nt2h_server_config* cfg, nt2h_server** out) {
+  if (!cfg || !out || cfg->budget_bytes == 0 || cfg->max_inflight < 1 || cfg->max_inflight > 16 ||
+      cfg->max_batch < 1 || (cfg->policy != NT2H_EXACT && cfg->policy != NT2H_FAST) ||
+      (cfg->backend != NT2H_BACKEND_HIP && cfg->backend != NT2H_BACKEND_HOST))
+    return fail(NT2H_EINVAL, "nt2h_server_create: bad configuration");
+  Server* s = new Server();
+  s->cfg = *cfg;
+  s->slots.resize(cfg->max_inflight);
+  if (cfg->backend == NT2H_BACKEND_HIP) {
+    for (Slot& sl : s->slots) {
+      if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
+          hipEventCreateWithFlags(&sl.event, hipEventDisableTiming) != hipSuccess) {
+        delete s;
+        return fail(NT2H_EHIP, "nt2h_server_create: stream/event creation failed");
+      }
+    }
+  }
+  s->sched = std::thread(sched_loop, s);
+  s->done_thr = std::thread(done_loop, s);
+  *out = reinterpret_cast<nt2h_server*>(s);
+  return NT2H_OK;
+}
+
+int nt2h_server_submit(nt2h_server* hs, const nt2h_bs_request* q, uint64_t* id) {
+  Server* s = reinterpret_cast<Server*>(hs);
+  if (!s || !id) return fail(NT2H_EINVAL, "nt2h_server_submit: null argument");
+  if (validate(q)) return fail(NT2H_EINVAL, "nt2h_server_submit: malformed request");
+  const size_t es = q->dtype == NT2H_F64 ? 8 : 4;
+  const size_t bytes = 5 * q->n * es;
+  if (bytes > s->cfg.budget_bytes) return fail(NT2H_EBUDGET, "nt2h_server_submit: request exceeds the memory budget");
+  auto r = std::make_unique<Request>();
+  r->d = ReqDesc{q->S, q->X, q->T, q->call, q->put, q->n, q->r, q->v, 0u,
+                 (uint32_t)(q->dtype == NT2H_F64)};
+  r->bytes = bytes;
+  r->own_out = false;
+  r->state = RS_PENDING;
+  r->status = NT2H_EAGAIN;
+  r->t_arrival = q->t_arrival_ns > 0 ? q->t_arrival_ns : now_ns();
+  r->t_admit = r->t_launch = r->t_done = 0;
+  r->slot = -1;
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->stop) return fail(NT2H_ESTATE, "nt2h_server_submit: server is shutting down");
+  r->id = s->next_id++;
+  *id = r->id;
+  if (s->t_first_arrival < 0) s->t_first_arrival = r->t_arrival;
+  s->pending.push_back(r->id);
+  s->reqs[r->id] = std::move(r);
+  s->cv_sched.notify_all();
+  return NT2H_OK;
+}
+
+/* 0 when done; NT2H_EAGAIN while pending or running; the failure code otherwise */
+int nt2h_server_poll(nt2h_server* hs, uint64_t id) {
+  Server* s = reinterpret_cast<Server*>(hs);
+  if (!s) return NT2H_EINVAL;
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) return fail(NT2H_EINVAL, "nt2h_server_poll: unknown request");
+  const Request* r = it->second.get();
+  if (r->state == RS_DONE) return NT2H_OK;
+  if (r->state == RS_FAILED) return r->status;
+  if (r->state == RS_CANCELLED) return NT2H_ECANCELLED;
+  return NT2H_EAGAIN;
+}
+
+int nt2h_server_wait(nt2h_server* hs, uint64_t id, double timeout_s) {
+  Server* s = reinterpret_cast<Server*>(hs);
+  if (!s) return NT2H_EINVAL;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) return fail(NT2H_EINVAL, "nt2h_server_wait: unknown request");
+  Request* r = it->second.get();
+  const auto deadline = clk::now() + std::chrono::duration<double>(timeout_s);
+  while (r->state < RS_DONE) {
+    if (s->cv_slot.wait_until(lk, deadline) == std::cv_status::timeout && r->state < RS_DONE)
+      return NT2H_EAGAIN;
+  }
+  if (r->state == RS_DONE) return NT2H_OK;
+  if (r->state == RS_CANCELLED) return NT2H_ECANCELLED;
+  return r->status;
+}
+
+/* outputs of a completed request (device pointers; owned by the server until released) */
+int nt2h_server_result(nt2h_server* hs, uint64_t id, void** call, void** put, int64_t* latency_ns) {
+  Server* s = reinterpret_cast<Server*>(hs);
+  if (!s) return NT2H_EINVAL;
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) return fail(NT2H_EINVAL, "nt2h_server_result: unknown request");
+  Request* r = it->second.get();
+  if (r->state != RS_DONE) return r->state == RS_FAILED ? r->status : NT2H_EAGAIN;
+  if (call) *call = r->d.call;
+  if (put) *put = r->d.put;
+  if (latency_ns) *latency_ns = r->t_done - r->t_arrival;
+  return NT2H_OK;
+}
+
+/* cancel a request that has not been launched yet */
+int nt2h_server_cancel(nt2h_server* hs, uint64_t id) {
+  Server* s = reinterpret_cast<Server*>(hs);
+  if (!s) return NT2H_EINVAL;
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) return fail(NT2H_EINVAL, "nt2h_server_cancel: unknown request");
+  Request* r = it->second.get();
+  if (r->state == RS_PENDING) {
+    r->state = RS_CANCELLED;
+    ++s->cancelled;
+  } else if (r->state == RS_ADMITTED) {
+    r->state = RS_CANCELLED;
+    s->used -= r->bytes;
+    ++s->cancelled;
+  } else {
+    return fail(NT2H_ESTATE, "nt2h_server_cancel: request already running or finished");
+  }
+  s->cv_sched.notify_all();
+  s->cv_slot.notify_all();
+  return NT2H_OK;
+}
+
+/* return the request's memory to the budget (and its outputs to the pool) */
+int nt2h_server_release(nt2h_server* hs, uint64_t id) {
+  Server* s = reinterpret_cast<Server*>(hs);
+  if (!s) return NT2H_EINVAL;
+  std::lock_guard<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(id);
+  if (it == s->reqs.end()) return fail(NT2H_EINVAL, "nt2h_server_release: unknown request");
+  Request* r = it->second.get();
+  if (r->state == RS_PENDING || r->state == RS_ADMITTED || r->state == RS_RUNNING)
+    return fail(NT2H_ESTATE, "nt2h_server_release: request not finished");
+  if (r->state == RS_DONE) {
+    srv_free(s, r);
+    s->used -= r->bytes;
+  }
+  s->reqs.erase(it);
+  s->cv_sched.notify_all();
+  return NT2H_OK;
+}
+
+/* block until every submitted request has finished (or failed/cancelled) */
+int nt2h_server_drain(nt2h_server* hs, double timeout_s) {
+  Server* s = reinterpret_cast<Server*>(hs);
+  if (!s) return NT2H_EINVAL;
+  std::unique_lock<std::mutex> lk(s->mu);
+  const auto deadline = clk::now() + std::chrono::duration<double>(timeout_s);
+  auto busy = [&]() {
+    for (auto& kv : s->reqs)
+      if (kv.second->state < RS_DONE) return true;
+    return false;
+  };
+  while (busy()) {
+    if (s->cv_slot.wait_until(lk, deadline) == std::cv_status::timeout && busy()) return NT2H_EAGAIN;
+  }
+  return NT2H_OK;
+}
+
+int nt2h_server_get_stats(nt2h_server* hs, nt2h_server_stats* st) {
+  Server* s = reinterpret_cast<Server*>(hs);
+  if (!s || !st) return NT2H_EINVAL;
+  std::lock_guard<std::mutex> lk(s->mu);
+  std::memset(st, 0, sizeof(*st));
+  std::vector<double> l = s->lat_us;
+  std::sort(l.begin(), l.end());
+  auto pct = [&](double q) {
+    if (l.empty()) return 0.0;
+    size_t k = (size_t)std::ceil(q * l.size()) - 1;
+    return l[std::min(k, l.size() - 1)];
+  };
+  st->completed = l.size();
+  st->failed = s->failed;
+  st->cancelled = s->cancelled;
+  st->options = s->options_done;
+  st->launches = s->launches;
+  st->mean_batch = s->launches ? (double)s->batched_reqs / s->launches : 0.0;
+  st->p50_us = pct(0.50);
+  st->p99_us = pct(0.99);
+  st->max_us = l.empty() ? 0.0 : l.back();
+  double sum = 0.0;
+  for (double v : l) sum += v;
+  st->mean_us = l.empty() ? 0.0 : sum / l.size();
+  const double span = (s->t_last_done > s->t_first_arrival && s->t_first_arrival >= 0)
+                          ? (s->t_last_done - s->t_first_arrival) * 1e-9 : 0.0;
+  st->span_s = span;
+  st->options_per_s = span > 0 ? s->options_done / span : 0.0;
+  st->used_bytes = s->used;
+  st->peak_bytes = s->peak_used;
+  return NT2H_OK;
+}
+
+/* reset latency/throughput statistics (requests already finished stay) */
+int nt2h_server_reset_stats(nt2h_server* hs) {
+  Server* s = reinterpret_cast<Server*>(hs);
+  if (!s) return NT2H_EINVAL;
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->lat_us.clear();
+  s->options_done = s->launches = s->batched_reqs = 0;
+  s->t_first_arrival = s->t_last_done = -1;
+  s->failed = s->cancelled = 0;
+  s->peak_used = s->used;
+  return NT2H_OK;
+}
+
+/* Replay a request trace: item k is submitted at t0 + arrival_s[k] (busy-wait
+ * pacing on the calling thread) and released as soon as it completes.  A
+ * closed loop of this call gives the served latency/throughput figures. */
+int nt2h_server_run_trace(nt2h_server* hs, int count, const nt2h_bs_request* items,
+                          const double* arrival_s, double timeout_s) {
+  Server* s = reinterpret_cast<Server*>(hs);
+  if (!s || count < 0 || (count && (!items || !arrival_s))) return fail(NT2H_EINVAL, "nt2h_server_run_trace: bad argument");
+  std::vector<uint64_t> ids;
+  ids.reserve(count);
+  const int64_t t0 = now_ns();
+  size_t next_release = 0;
+  for (int k = 0; k < count; ++k) {
+    const int64_t due = t0 + (int64_t)(arrival_s[k] * 1e9);
+    while (now_ns() < due) {
+      // release finished requests while waiting (their outputs leave the budget)
+      if (next_release < ids.size() && nt2h_server_poll(hs, ids[next_release]) != NT2H_EAGAIN) {
+        nt2h_server_release(hs, ids[next_release]);
+        ++next_release;
+      }
+    }
+    nt2h_bs_request q = items[k];
+    q.t_arrival_ns = due;
+    uint64_t id = 0;
+    int rc = nt2h_server_submit(hs, &q, &id);
+    if (rc) return rc;
+    ids.push_back(id);
+  }
+  for (; next_release < ids.size(); ++next_release) {
+    int rc = nt2h_server_wait(hs, ids[next_release], timeout_s);
+    if (rc == NT2H_EAGAIN) return fail(NT2H_EAGAIN, "nt2h_server_run_trace: timeout");
+    nt2h_server_release(hs, ids[next_release]);
+  }
+  return NT2H_OK;
+}
+
+int nt2h_server_destroy(nt2h_server* hs) {
+  Server* s = reinterpret_cast<Server*>(hs);
+  if (!s) return NT2H_OK;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+    for (auto& kv : s->reqs)
+      if (kv.second->state == RS_PENDING) kv.second->state = RS_CANCELLED;
+    s->cv_sched.notify_all();
+    s->cv_done.notify_all();
+  }
+  s->sched.join();
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->cv_done.notify_all();
+  }
+  s->done_thr.join();
+  for (Slot& sl : s->slots) {
+    if (sl.host_worker.joinable()) sl.host_worker.join();
+  }
+  for (auto& kv : s->reqs) srv_free(s, kv.second.get());
+  for (Slot& sl : s->slots) {
+    if (s->cfg.backend == NT2H_BACKEND_HOST) {
+      delete[] sl.h_desc;
+    } else {
+      if (sl.stream) (void)hipStreamSynchronize(sl.stream);
+      if (sl.h_desc) (void)hipHostFree(sl.h_desc);
+      if (sl.d_desc) (void)hipFree(sl.d_desc);
+      if (sl.event) (void)hipEventDestroy(sl.event);
+      if (sl.stream) (void)hipStreamDestroy(sl.stream);
+    }
+  }
+  delete s;
+  return NT2H_OK;
+}
+
+}  // extern "C"
