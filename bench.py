@@ -118,7 +118,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--policy", choices=["fast", "exact"], default="fast")
+    ap.add_argument("--policy", choices=["accurate", "fast", "exact"], default="accurate")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -129,7 +129,7 @@ def main():
     from nt2_amd import EXACT, FAST, ops
     import _oracle as O
 
-    policy = FAST if args.policy == "fast" else EXACT
+    policy = {"fast": FAST, "exact": EXACT, "accurate": 2}[args.policy]
     torch.cuda.set_device(0)
     n = N_OPTIONS
     S = ops.fill_uniform(n, 5.0, 30.0, 2 * 16 + 1)

@@ -40,6 +40,8 @@ extern "C" {
 /* evaluation policy of floating-point kernels */
 #define NT2H_EXACT 0 /* NT2's arithmetic: fma = a*b+c with two roundings; bit-identical to the CPU evaluation */
 #define NT2H_FAST 1  /* NT2's fma call sites fused in hardware; within each function's ULP bound */
+#define NT2H_ACCURATE 2 /* Black-Scholes f32 only: FP64 internal evaluation with this library's
+                           own approximations; within a few ulp of the exact price */
 
 /* elementwise operations for nt2h_map_* (NT2 functors exp_, log_, sqrt_, erfc_,
  * normcdf_, fastnormcdf_, cos_, sin_, abs_) */

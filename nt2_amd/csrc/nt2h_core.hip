@@ -10,6 +10,7 @@
 
 #include "bs_device.hpp"
 #include "common.hpp"
+#include "nt2/hip/bs_accurate.hpp"
 #include "nt2/hip/math.hpp"
 #include "nt2h.h"
 
@@ -365,6 +366,32 @@ __device__ __forceinline__ void store_nt(f32x4* p, float4 v) {
   __builtin_nontemporal_store(f32x4{v.x, v.y, v.z, v.w}, p);
 }
 __device__ __forceinline__ void store_nt(f64x2* p, f64x2 v) { __builtin_nontemporal_store(v, p); }
+// accurate policy: one float4 of options per thread, FP64 inside
+__global__ __launch_bounds__(kBlock) void bs_acc_f32_k(size_t n, const float* __restrict__ S,
+                                                       const float* __restrict__ X,
+                                                       const float* __restrict__ T,
+                                                       nt2::hip::bsacc::Params prm,
+                                                       float* __restrict__ call,
+                                                       float* __restrict__ put, int vec) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t n4 = vec ? n / 4 : 0;
+  if (t < n4) {
+    const f32x4 s4 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(S) + t);
+    const f32x4 x4 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(X) + t);
+    const f32x4 t4 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(T) + t);
+    float4 c4, p4;
+    nt2::hip::bsacc::bs_accurate_one(s4.x, x4.x, t4.x, prm, c4.x, p4.x);
+    nt2::hip::bsacc::bs_accurate_one(s4.y, x4.y, t4.y, prm, c4.y, p4.y);
+    nt2::hip::bsacc::bs_accurate_one(s4.z, x4.z, t4.z, prm, c4.z, p4.z);
+    nt2::hip::bsacc::bs_accurate_one(s4.w, x4.w, t4.w, prm, c4.w, p4.w);
+    store_nt(reinterpret_cast<f32x4*>(call) + t, c4);
+    store_nt(reinterpret_cast<f32x4*>(put) + t, p4);
+  } else {
+    const size_t i = n4 * 4 + (t - n4);
+    if (i < n) nt2::hip::bsacc::bs_accurate_one(S[i], X[i], T[i], prm, call[i], put[i]);
+  }
+}
+
 template <class P, int MODE>
 __global__ __launch_bounds__(kBlock) void bs_f32_k(size_t n, const float* __restrict__ S,
                                                    const float* __restrict__ X,
@@ -867,14 +894,18 @@ int nt2h_blackscholes_f32(size_t n, const float* S, const float* X, const float*
     return fail(NT2H_EINVAL, "nt2h_blackscholes_f32: null pointer");
   if (mode != NT2H_BS_TEXTBOOK && mode != NT2H_BS_DEMO)
     return fail(NT2H_EINVAL, "nt2h_blackscholes_f32: bad mode");
-  if (policy != NT2H_EXACT && policy != NT2H_FAST)
+  if (policy != NT2H_EXACT && policy != NT2H_FAST && policy != NT2H_ACCURATE)
     return fail(NT2H_EINVAL, "nt2h_blackscholes_f32: bad policy");
+  if (policy == NT2H_ACCURATE && mode != NT2H_BS_TEXTBOOK)
+    return fail(NT2H_EINVAL, "nt2h_blackscholes_f32: the accurate policy evaluates the textbook formula");
   const int vec = aligned16(S) && aligned16(X) && aligned16(T) && aligned16(call) &&
                   (mode == NT2H_BS_DEMO || aligned16(put));
   const size_t items = vec ? n / 4 + n % 4 : n;
   const dim3 g(grid_for(items)), b(kBlock);
   hipStream_t s = as_stream(stream);
-  if (policy == NT2H_EXACT) {
+  if (policy == NT2H_ACCURATE) {
+    hipLaunchKernelGGL(bs_acc_f32_k, g, b, 0, s, n, S, X, T, nt2::hip::bsacc::make_params(r, v), call, put, vec);
+  } else if (policy == NT2H_EXACT) {
     if (mode == NT2H_BS_TEXTBOOK) hipLaunchKernelGGL((bs_f32_k<exact_policy, NT2H_BS_TEXTBOOK>), g, b, 0, s, n, S, X, T, r, v, call, put, vec);
     else hipLaunchKernelGGL((bs_f32_k<exact_policy, NT2H_BS_DEMO>), g, b, 0, s, n, S, X, T, r, v, call, put, vec);
   } else {

@@ -288,7 +288,6 @@ static void launch_batch(Server* s, int si, std::unique_lock<std::mutex>& lk) {
     sl.reqs.push_back(r->id);
   }
   sl.busy = true;
-  s->launch_order.push_back(si);
   ++s->launches;
   s->batched_reqs += ok.size();
   const int nreq = (int)ok.size();
@@ -308,6 +307,8 @@ static void launch_batch(Server* s, int si, std::unique_lock<std::mutex>& lk) {
     (void)hipEventRecord(sl.event, sl.stream);
     lk.lock();
   }
+  // publish to the completion thread only once the batch's event is recorded
+  s->launch_order.push_back(si);
   s->cv_done.notify_all();
 }
 

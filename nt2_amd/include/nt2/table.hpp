@@ -1,0 +1,782 @@
+// nt2::table<T> and its expression DSL on the HIP architecture site (gfx950).
+//
+// User code written against NT2's table API compiles against this header with
+// hipcc: each statement `A = expression;`, `tie(A, B) = tie(e1, e2);`,
+// `globalsum(expression)`, `dot(a, b)` is lowered at compile time to ONE fused
+// kernel that evaluates the whole expression tree per 128-bit pack, keeping
+// every intermediate in registers.
+//
+// Semantics kept from NT2 (paths relative to the reference tree):
+//  * table<T>: owning contiguous storage, column-major, 1-based operator()
+//    (modules/core/container/table/include/nt2/core/container/table/table.hpp:26-121,
+//    modules/core/sdk/include/nt2/core/settings/index.hpp:29, storage_order.hpp:22);
+//  * assignment resizes the left-hand side to the right-hand side's extent and
+//    evaluates elementwise, in place when the LHS appears on the RHS
+//    (modules/core/base/include/nt2/dsl/functions/elementwise/run.hpp:21-36);
+//  * transform over packs of the AVX width from index 0, scalar tail
+//    (modules/core/base/include/nt2/core/functions/simd/transform.hpp:40-57), so
+//    pack-dependent kernels (the trigonometric range reduction) give NT2's bits;
+//  * tie reads every right-hand side at an index before writing any output
+//    (modules/core/container/table/include/nt2/core/functions/expr/tie.hpp:40-121);
+//  * sum/globalsum/dot fold the whole table (modules/core/reduction/include/nt2/
+//    core/functions/globalsum.hpp:24-88, reduction/functions/expr/dot.hpp:22-29),
+//    here with compensated FP32 partials and an FP64 cross-block combine.
+//
+// Storage is mirrored: a table keeps a host copy and a device copy with
+// validity flags, so host element access (setup loops) and device evaluation
+// can be mixed; the device copy is the HIP counterpart of cuda_buffer
+// (modules/arch/cuda/include/nt2/sdk/cuda/memory/buffer.hpp:36-242).
+//
+// Floating-point policy: NT2_HIP_POLICY (default nt2::hip::exact_policy: NT2's
+// two-rounding fma, bit-identical to the CPU evaluation; nt2::hip::fast_policy
+// fuses NT2's fma call sites).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <stdexcept>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "nt2/hip/math.hpp"
+
+#ifndef NT2_HIP_POLICY
+#define NT2_HIP_POLICY nt2::hip::exact_policy
+#endif
+
+namespace nt2 {
+
+// ---- architecture site tags ------------------------------------------------
+// NT2 stacks sites (modules/arch/cuda/include/nt2/sdk/cuda/cuda.hpp:16-24);
+// this header's evaluator is the hip_ site over the host site.
+namespace tag {
+struct cpu_ {};
+template <class Site> struct hip_ : Site { typedef Site parent; };
+}  // namespace tag
+typedef tag::hip_<tag::cpu_> default_site;
+
+namespace hip { namespace fe {
+
+inline void check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw std::runtime_error(std::string("nt2 hip site: ") + what + ": " + hipGetErrorString(e));
+}
+
+// pack width per element type: NT2's AVX cardinal for f32/f64, one 128-bit vector for bytes
+template <class T> struct pack_width { static constexpr int value = 16 / (int)sizeof(T) * (sizeof(T) >= 4 ? 2 : 1); };
+template <> struct pack_width<float> { static constexpr int value = 8; };
+template <> struct pack_width<double> { static constexpr int value = 4; };
+template <> struct pack_width<uint8_t> { static constexpr int value = 16; };
+template <> struct pack_width<bool> { static constexpr int value = 16; };
+
+}}  // namespace hip::fe
+
+// ---- extents ----------------------------------------------------------------
+struct of_size_ {
+  size_t d[2];
+  size_t numel() const { return d[0] * d[1]; }
+  bool operator==(const of_size_& o) const { return d[0] == o.d[0] && d[1] == o.d[1]; }
+};
+// extent of an expression: none for a scalar
+struct ext_t {
+  bool has;
+  of_size_ s;
+};
+inline ext_t merge_ext(const ext_t& x, const ext_t& y) {
+  if (x.has && y.has && !(x.s == y.s)) throw std::invalid_argument("nt2: non-conformant table sizes");
+  return x.has ? x : y;
+}
+namespace meta { template <class T> struct as_ {}; }
+inline of_size_ of_size(size_t n) { return of_size_{{n, 1}}; }
+inline of_size_ of_size(size_t h, size_t w) { return of_size_{{h, w}}; }
+
+template <class T> class table;
+
+// ---- expression node types ----------------------------------------------------
+namespace expr {
+
+struct is_expr_base {};
+
+// device views: trivially copyable, passed to the fused kernel by value
+template <class T> struct term_v { const T* p; };
+template <class T> struct scal_v { T v; };
+
+template <class T> struct terminal : is_expr_base {
+  typedef T value_type;
+  const table<T>* t;
+  ext_t ext() const;
+  void prepare() const;
+  term_v<T> view() const;
+};
+template <class T> struct scalar : is_expr_base {
+  typedef T value_type;
+  T v;
+  ext_t ext() const { return ext_t{false, of_size_{{0, 1}}}; }
+  void prepare() const {}
+  scal_v<T> view() const { return scal_v<T>{v}; }
+};
+
+template <class Tag, class A> struct node1 : is_expr_base {
+  typedef typename Tag::template result<typename A::value_type>::type value_type;
+  A a;
+  ext_t ext() const { return a.ext(); }
+  void prepare() const { a.prepare(); }
+  struct view_t { decltype(std::declval<A>().view()) a; };
+  view_t view() const { return view_t{a.view()}; }
+};
+template <class Tag, class A, class B> struct node2 : is_expr_base {
+  typedef typename Tag::template result<typename A::value_type, typename B::value_type>::type value_type;
+  A a;
+  B b;
+  ext_t ext() const { return merge_ext(a.ext(), b.ext()); }
+  void prepare() const { a.prepare(); b.prepare(); }
+  struct view_t { decltype(std::declval<A>().view()) a; decltype(std::declval<B>().view()) b; };
+  view_t view() const { return view_t{a.view(), b.view()}; }
+};
+template <class Tag, class A, class B, class C> struct node3 : is_expr_base {
+  typedef typename Tag::template result<typename A::value_type, typename B::value_type,
+                                        typename C::value_type>::type value_type;
+  A a;
+  B b;
+  C c;
+  ext_t ext() const { return merge_ext(merge_ext(a.ext(), b.ext()), c.ext()); }
+  void prepare() const { a.prepare(); b.prepare(); c.prepare(); }
+  struct view_t {
+    decltype(std::declval<A>().view()) a;
+    decltype(std::declval<B>().view()) b;
+    decltype(std::declval<C>().view()) c;
+  };
+  view_t view() const { return view_t{a.view(), b.view(), c.view()}; }
+};
+
+}  // namespace expr
+
+// ---- result-type helpers and tags ---------------------------------------------
+namespace tagfn {
+template <class X, class Y> struct common { typedef typename std::conditional<sizeof(X) >= sizeof(Y), X, Y>::type type; };
+template <class X> struct common<X, bool> { typedef X type; };
+template <class Y> struct common<bool, Y> { typedef Y type; };
+
+struct same1 { template <class A> struct result { typedef A type; }; };
+struct same2 { template <class A, class B> struct result { typedef typename common<A, B>::type type; }; };
+struct logical2 { template <class A, class B> struct result { typedef bool type; }; };
+struct sel3 { template <class C, class A, class B> struct result { typedef typename common<A, B>::type type; }; };
+struct same3 { template <class A, class B, class C> struct result { typedef typename common<typename common<A, B>::type, C>::type type; }; };
+struct selinc2 { template <class C, class A> struct result { typedef A type; }; };
+struct izeo1 { template <class C> struct result { typedef uint8_t type; }; };
+
+#define NT2H_TAG(name, base) struct name : base {};
+NT2H_TAG(plus_, same2) NT2H_TAG(minus_, same2) NT2H_TAG(multiplies_, same2) NT2H_TAG(divides_, same2)
+NT2H_TAG(max_, same2) NT2H_TAG(min_, same2) NT2H_TAG(adds_, same2) NT2H_TAG(subs_, same2)
+NT2H_TAG(lt_, logical2) NT2H_TAG(gt_, logical2) NT2H_TAG(le_, logical2) NT2H_TAG(ge_, logical2)
+NT2H_TAG(eq_, logical2) NT2H_TAG(neq_, logical2)
+NT2H_TAG(neg_, same1) NT2H_TAG(exp_, same1) NT2H_TAG(log_, same1) NT2H_TAG(sqrt_, same1)
+NT2H_TAG(erfc_, same1) NT2H_TAG(normcdf_, same1) NT2H_TAG(fastnormcdf_, same1) NT2H_TAG(cos_, same1)
+NT2H_TAG(sin_, same1) NT2H_TAG(abs_, same1) NT2H_TAG(sqr_, same1) NT2H_TAG(rec_, same1)
+NT2H_TAG(fma_, same3) NT2H_TAG(fnms_, same3) NT2H_TAG(if_else_, sel3)
+NT2H_TAG(selinc_, selinc2) NT2H_TAG(seldec_, selinc2) NT2H_TAG(if_zero_else_one_, izeo1)
+#undef NT2H_TAG
+}  // namespace tagfn
+
+// ---- device evaluation of packs -------------------------------------------------
+namespace hip { namespace fe {
+namespace dm = nt2::hip::dm;
+typedef NT2_HIP_POLICY P;
+
+template <int G, class T> struct pk { T v[G]; };
+
+template <class T, int G>
+__device__ __forceinline__ void load(const expr::term_v<T>& t, size_t i, T (&r)[G]) {
+#pragma unroll
+  for (int l = 0; l < G; ++l) r[l] = t.p[i + l];
+}
+template <class T, int G>
+__device__ __forceinline__ void load(const expr::scal_v<T>& s, size_t, T (&r)[G]) {
+#pragma unroll
+  for (int l = 0; l < G; ++l) r[l] = s.v;
+}
+
+template <class T> struct is_f { static constexpr bool value = std::is_floating_point<T>::value; };
+
+template <class Tag> struct apply1;
+#define NT2H_UN(TAG, EXPR_F, EXPR_D, EXPR_I)                                         \
+  template <> struct apply1<tagfn::TAG> {                                            \
+    template <int G, class T> __device__ __forceinline__ static void run(T (&a)[G], T (&r)[G]) { \
+      _Pragma("clang fp contract(off)")                                              \
+      _Pragma("unroll") for (int l = 0; l < G; ++l) {                                \
+        const T x = a[l];                                                            \
+        if constexpr (std::is_same<T, float>::value) { r[l] = (EXPR_F); }          \
+        else if constexpr (std::is_same<T, double>::value) { r[l] = (EXPR_D); }    \
+        else { r[l] = (T)(EXPR_I); }                                                 \
+      }                                                                              \
+    }                                                                                \
+  };
+NT2H_UN(neg_, -x, -x, -x)
+NT2H_UN(exp_, dm::exp_f<P>(x), dm::exp_d<P>(x), 0)
+NT2H_UN(log_, dm::log_f<P>(x), dm::log_d<P>(x), 0)
+NT2H_UN(sqrt_, __builtin_sqrtf(x), __builtin_sqrt(x), 0)
+NT2H_UN(erfc_, dm::erfc_f<P>(x), (G == 1 ? dm::erfc_d<P, true>(x) : dm::erfc_d<P, false>(x)), 0)
+NT2H_UN(normcdf_, dm::normcdf_f<P>(x), (G == 1 ? dm::normcdf_d<P, true>(x) : dm::normcdf_d<P, false>(x)), 0)
+NT2H_UN(fastnormcdf_, dm::fastnormcdf_f<P>(x), (double)dm::fastnormcdf_f<P>((float)x), 0)
+NT2H_UN(abs_, __builtin_fabsf(x), __builtin_fabs(x), x)
+NT2H_UN(sqr_, x * x, x * x, x * x)
+NT2H_UN(rec_, 1.0f / x, 1.0 / x, 0)
+#undef NT2H_UN
+
+template <> struct apply1<tagfn::cos_> {
+  template <int G, class T> __device__ __forceinline__ static void run(T (&a)[G], T (&r)[G]) {
+    if constexpr (std::is_same<T, float>::value) dm::trig_group_f<P, G>(dm::k_cos, a, r, nullptr);
+    else dm::trig_group_d<P, G>(dm::k_cos, a, r, nullptr);
+  }
+};
+template <> struct apply1<tagfn::sin_> {
+  template <int G, class T> __device__ __forceinline__ static void run(T (&a)[G], T (&r)[G]) {
+    if constexpr (std::is_same<T, float>::value) dm::trig_group_f<P, G>(dm::k_sin, a, r, nullptr);
+    else dm::trig_group_d<P, G>(dm::k_sin, a, r, nullptr);
+  }
+};
+
+template <class Tag> struct apply2;
+#define NT2H_BIN(TAG, EXPR)                                                           \
+  template <> struct apply2<tagfn::TAG> {                                             \
+    template <class R, class T, class U>                                              \
+    __device__ __forceinline__ static R one(T x, U y) {                               \
+      _Pragma("clang fp contract(off)")                                               \
+      return (R)(EXPR);                                                               \
+    }                                                                                 \
+  };
+NT2H_BIN(plus_, x + y)
+NT2H_BIN(minus_, x - y)
+NT2H_BIN(multiplies_, x * y)
+NT2H_BIN(divides_, x / y)
+NT2H_BIN(max_, x > y ? x : y)
+NT2H_BIN(min_, x < y ? x : y)
+NT2H_BIN(lt_, x < y)
+NT2H_BIN(gt_, x > y)
+NT2H_BIN(le_, x <= y)
+NT2H_BIN(ge_, x >= y)
+NT2H_BIN(eq_, x == y)
+NT2H_BIN(neq_, x != y)
+#undef NT2H_BIN
+template <> struct apply2<tagfn::adds_> {
+  template <class R, class T, class U> __device__ __forceinline__ static R one(T x, U y) {
+    if constexpr (std::is_same<R, uint8_t>::value) return (R)dm::adds_u8(x, y);
+    else return (R)(x + y);
+  }
+};
+template <> struct apply2<tagfn::subs_> {
+  template <class R, class T, class U> __device__ __forceinline__ static R one(T x, U y) {
+    if constexpr (std::is_same<R, uint8_t>::value) return (R)dm::subs_u8(x, y);
+    else return (R)(x - y);
+  }
+};
+template <> struct apply2<tagfn::selinc_> {  // selinc(c, a) = c ? a+1 : a (wrapping)
+  template <class R, class C, class A> __device__ __forceinline__ static R one(C c, A a) {
+    return c ? (R)(a + 1) : (R)a;
+  }
+};
+template <> struct apply2<tagfn::seldec_> {
+  template <class R, class C, class A> __device__ __forceinline__ static R one(C c, A a) {
+    return c ? (R)(a - 1) : (R)a;
+  }
+};
+
+}}  // namespace hip::fe
+
+// ---- user-facing table ------------------------------------------------------
+template <class T>
+class table {
+ public:
+  typedef T value_type;
+  table() : ext_{{0, 1}} {}
+  explicit table(of_size_ s) : ext_{{0, 1}} { resize(s); }
+  table(const table& o) : ext_{{0, 1}} { *this = o; }
+  template <class E, class = typename std::enable_if<std::is_base_of<expr::is_expr_base, E>::value>::type>
+  table(const E& e) : ext_{{0, 1}} { assign(e); }
+  ~table() { release(); }
+
+  table& operator=(const table& o) {
+    if (this == &o) return *this;
+    resize(o.ext_);
+    if (o.dev_valid_ && !o.host_valid_) {
+      ensure_dev_alloc();
+      hip::fe::check(hipMemcpy(dev_, o.dev_, bytes(), hipMemcpyDeviceToDevice), "copy");
+      dev_valid_ = true;
+      host_valid_ = false;
+    } else {
+      o.sync_host();
+      host_ = o.host_;
+      host_valid_ = true;
+      dev_valid_ = false;
+    }
+    return *this;
+  }
+  template <class E>
+  typename std::enable_if<std::is_base_of<expr::is_expr_base, E>::value, table&>::type
+  operator=(const E& e) {
+    assign(e);
+    return *this;
+  }
+
+  void resize(of_size_ s) {
+    if (s == ext_) return;
+    if (s.numel() != numel()) {
+      host_.assign(s.numel(), T());
+      if (dev_) { hip::fe::check(hipFree(dev_), "free"); dev_ = nullptr; }
+      host_valid_ = true;
+      dev_valid_ = false;
+    }
+    ext_ = s;
+  }
+  size_t numel() const { return ext_.numel(); }
+  of_size_ extent() const { return ext_; }
+  size_t size(int dim) const { return dim == 1 ? ext_.d[0] : (dim == 2 ? ext_.d[1] : 1); }
+
+  // 1-based, column-major host access (synchronizes with the device copy)
+  T& operator()(size_t i) {
+    sync_host();
+    dev_valid_ = false;
+    return host_[i - 1];
+  }
+  T operator()(size_t i) const {
+    sync_host();
+    return host_[i - 1];
+  }
+  T& operator()(size_t i, size_t j) { return (*this)((j - 1) * ext_.d[0] + i); }
+  T operator()(size_t i, size_t j) const { return (*this)((j - 1) * ext_.d[0] + i); }
+
+  // raw access
+  const T* device_data() const { sync_dev(); return dev_; }
+  T* device_data_for_write() {
+    ensure_dev_alloc();
+    sync_dev();
+    host_valid_ = false;
+    dev_valid_ = true;
+    return dev_;
+  }
+  const T* data() const { sync_host(); return host_.data(); }
+
+  void sync_host() const {
+    if (!host_valid_ && dev_valid_) {
+      host_.resize(numel());
+      hip::fe::check(hipMemcpy(host_.data(), dev_, bytes(), hipMemcpyDeviceToHost), "download");
+      host_valid_ = true;
+    }
+  }
+  void sync_dev() const {
+    if (numel() == 0) return;
+    ensure_dev_alloc();
+    if (!dev_valid_) {
+      hip::fe::check(hipMemcpy(dev_, host_.data(), bytes(), hipMemcpyHostToDevice), "upload");
+      dev_valid_ = true;
+    }
+  }
+
+  template <class E> void assign(const E& e);
+
+ private:
+  size_t bytes() const { return numel() * sizeof(T); }
+  void ensure_dev_alloc() const {
+    if (!dev_ && numel()) hip::fe::check(hipMalloc((void**)&dev_, bytes()), "alloc");
+  }
+  void release() {
+    if (dev_) (void)hipFree(dev_);
+    dev_ = nullptr;
+  }
+  of_size_ ext_;
+  mutable std::vector<T> host_;
+  mutable T* dev_ = nullptr;
+  mutable bool host_valid_ = true, dev_valid_ = false;
+};
+
+// ---- terminals ------------------------------------------------------------------
+namespace expr {
+template <class T> ext_t terminal<T>::ext() const { return ext_t{true, t->extent()}; }
+template <class T> void terminal<T>::prepare() const { t->sync_dev(); }
+template <class T> term_v<T> terminal<T>::view() const { return term_v<T>{t->device_data()}; }
+
+template <class X> struct wrap { typedef X type; static const X& get(const X& x) { return x; } };
+template <class T> struct wrap<table<T>> {
+  typedef terminal<T> type;
+  static terminal<T> get(const table<T>& t) { terminal<T> r; r.t = &t; return r; }
+};
+template <class X, class Hint, bool = std::is_arithmetic<X>::value> struct as_expr : wrap<X> {};
+template <class X, class Hint> struct as_expr<X, Hint, true> {
+  typedef scalar<Hint> type;
+  static scalar<Hint> get(X x) { scalar<Hint> s; s.v = (Hint)x; return s; }
+};
+template <class X> struct is_dsl {
+  static constexpr bool value = std::is_base_of<is_expr_base, X>::value;
+};
+template <class T> struct is_dsl<table<T>> { static constexpr bool value = true; };
+template <class X> struct vt { typedef typename X::value_type type; };
+template <class X, bool = std::is_arithmetic<X>::value> struct vt_or { typedef typename X::value_type type; };
+template <class X> struct vt_or<X, true> { typedef X type; };
+}  // namespace expr
+
+// ---- builders ---------------------------------------------------------------------
+#define NT2H_ENABLE1(A) typename std::enable_if<expr::is_dsl<A>::value, int>::type = 0
+#define NT2H_ENABLE2(A, B) typename std::enable_if<expr::is_dsl<A>::value || expr::is_dsl<B>::value, int>::type = 0
+
+template <class Tag, class A>
+expr::node1<Tag, typename expr::wrap<A>::type> make1(const A& a) {
+  expr::node1<Tag, typename expr::wrap<A>::type> n;
+  n.a = expr::wrap<A>::get(a);
+  return n;
+}
+template <class Tag, class A, class B>
+auto make2(const A& a, const B& b) {
+  typedef typename std::conditional<expr::is_dsl<A>::value, typename expr::vt_or<typename expr::wrap<A>::type>::type,
+                                    typename expr::vt_or<typename expr::wrap<B>::type>::type>::type hint;
+  typedef typename expr::as_expr<A, hint>::type EA;
+  typedef typename expr::as_expr<B, hint>::type EB;
+  expr::node2<Tag, EA, EB> n;
+  n.a = expr::as_expr<A, hint>::get(a);
+  n.b = expr::as_expr<B, hint>::get(b);
+  return n;
+}
+template <class Tag, class A, class B, class C>
+auto make3(const A& a, const B& b, const C& c) {
+  typedef typename std::conditional<
+      expr::is_dsl<B>::value, typename expr::vt_or<typename expr::wrap<B>::type>::type,
+      typename std::conditional<expr::is_dsl<C>::value, typename expr::vt_or<typename expr::wrap<C>::type>::type,
+                                typename expr::vt_or<typename expr::wrap<A>::type>::type>::type>::type hint;
+  typedef typename expr::as_expr<A, hint>::type EA;
+  typedef typename expr::as_expr<B, hint>::type EB;
+  typedef typename expr::as_expr<C, hint>::type EC;
+  expr::node3<Tag, EA, EB, EC> n;
+  n.a = expr::as_expr<A, hint>::get(a);
+  n.b = expr::as_expr<B, hint>::get(b);
+  n.c = expr::as_expr<C, hint>::get(c);
+  return n;
+}
+
+#define NT2H_OP2(op, Tag) \
+  template <class A, class B, NT2H_ENABLE2(A, B)> auto operator op(const A& a, const B& b) { return make2<tagfn::Tag>(a, b); }
+NT2H_OP2(+, plus_) NT2H_OP2(-, minus_) NT2H_OP2(*, multiplies_) NT2H_OP2(/, divides_)
+NT2H_OP2(<, lt_) NT2H_OP2(>, gt_) NT2H_OP2(<=, le_) NT2H_OP2(>=, ge_) NT2H_OP2(==, eq_) NT2H_OP2(!=, neq_)
+#undef NT2H_OP2
+template <class A, NT2H_ENABLE1(A)> auto operator-(const A& a) { return make1<tagfn::neg_>(a); }
+
+#define NT2H_FN1(name, Tag) \
+  template <class A, NT2H_ENABLE1(A)> auto name(const A& a) { return make1<tagfn::Tag>(a); }
+NT2H_FN1(exp, exp_) NT2H_FN1(log, log_) NT2H_FN1(sqrt, sqrt_) NT2H_FN1(erfc, erfc_)
+NT2H_FN1(normcdf, normcdf_) NT2H_FN1(fastnormcdf, fastnormcdf_) NT2H_FN1(cos, cos_)
+NT2H_FN1(sin, sin_) NT2H_FN1(abs, abs_) NT2H_FN1(sqr, sqr_) NT2H_FN1(rec, rec_)
+NT2H_FN1(if_zero_else_one, if_zero_else_one_)
+#undef NT2H_FN1
+#define NT2H_FN2(name, Tag) \
+  template <class A, class B, NT2H_ENABLE2(A, B)> auto name(const A& a, const B& b) { return make2<tagfn::Tag>(a, b); }
+NT2H_FN2(max, max_) NT2H_FN2(min, min_) NT2H_FN2(adds, adds_) NT2H_FN2(subs, subs_)
+NT2H_FN2(selinc, selinc_) NT2H_FN2(seldec, seldec_)
+#undef NT2H_FN2
+// fma(a,b,c) = a*b+c and fnms(a,b,c) = c-a*b, NT2's two-rounding forms
+template <class A, class B, class C,
+          typename std::enable_if<expr::is_dsl<A>::value || expr::is_dsl<B>::value || expr::is_dsl<C>::value, int>::type = 0>
+auto fma(const A& a, const B& b, const C& c) { return make3<tagfn::fma_>(a, b, c); }
+template <class A, class B, class C,
+          typename std::enable_if<expr::is_dsl<A>::value || expr::is_dsl<B>::value || expr::is_dsl<C>::value, int>::type = 0>
+auto fnms(const A& a, const B& b, const C& c) { return make3<tagfn::fnms_>(a, b, c); }
+template <class Cd, class A, class B,
+          typename std::enable_if<expr::is_dsl<Cd>::value || expr::is_dsl<A>::value || expr::is_dsl<B>::value, int>::type = 0>
+auto if_else(const Cd& c, const A& a, const B& b) { return make3<tagfn::if_else_>(c, a, b); }
+// scalar fma for host-side constants (NT2's scalar fma, two roundings)
+inline float fma(float a, float b, float c) { volatile float p = a * b; return p + c; }
+inline double fma(double a, double b, double c) { volatile double p = a * b; return p + c; }
+inline float sqr(float a) { return a * a; }
+inline double sqr(double a) { return a * a; }
+
+// ---- device evaluation -------------------------------------------------------------
+namespace hip { namespace fe {
+
+template <int G, class T>
+__device__ __forceinline__ void ev(const expr::term_v<T>& v, size_t i, T (&r)[G]) { load<T, G>(v, i, r); }
+template <int G, class T>
+__device__ __forceinline__ void ev(const expr::scal_v<T>& v, size_t i, T (&r)[G]) { load<T, G>(v, i, r); }
+
+template <class E> struct vview;  // value type of a view
+template <class T> struct vview<expr::term_v<T>> { typedef T type; };
+template <class T> struct vview<expr::scal_v<T>> { typedef T type; };
+
+// node views carry their tag and child views through their host type
+template <class Tag, class VA> struct n1v { VA a; };
+template <class Tag, class VA, class VB> struct n2v { VA a; VB b; };
+template <class Tag, class VA, class VB, class VC> struct n3v { VA a; VB b; VC c; };
+
+}}  // namespace hip::fe
+
+namespace expr {
+// host node -> device view (tagged)
+template <class X> struct dview;
+template <class T> struct dview<terminal<T>> {
+  typedef term_v<T> type;
+  static type get(const terminal<T>& t) { return t.view(); }
+};
+template <class T> struct dview<scalar<T>> {
+  typedef scal_v<T> type;
+  static type get(const scalar<T>& s) { return s.view(); }
+};
+template <class Tag, class A> struct dview<node1<Tag, A>> {
+  typedef hip::fe::n1v<Tag, typename dview<A>::type> type;
+  static type get(const node1<Tag, A>& n) { return type{dview<A>::get(n.a)}; }
+};
+template <class Tag, class A, class B> struct dview<node2<Tag, A, B>> {
+  typedef hip::fe::n2v<Tag, typename dview<A>::type, typename dview<B>::type> type;
+  static type get(const node2<Tag, A, B>& n) { return type{dview<A>::get(n.a), dview<B>::get(n.b)}; }
+};
+template <class Tag, class A, class B, class C> struct dview<node3<Tag, A, B, C>> {
+  typedef hip::fe::n3v<Tag, typename dview<A>::type, typename dview<B>::type, typename dview<C>::type> type;
+  static type get(const node3<Tag, A, B, C>& n) {
+    return type{dview<A>::get(n.a), dview<B>::get(n.b), dview<C>::get(n.c)};
+  }
+};
+}  // namespace expr
+
+namespace hip { namespace fe {
+
+template <class Tag, class VA> struct vview<n1v<Tag, VA>> {
+  typedef typename Tag::template result<typename vview<VA>::type>::type type;
+};
+template <class Tag, class VA, class VB> struct vview<n2v<Tag, VA, VB>> {
+  typedef typename Tag::template result<typename vview<VA>::type, typename vview<VB>::type>::type type;
+};
+template <class Tag, class VA, class VB, class VC> struct vview<n3v<Tag, VA, VB, VC>> {
+  typedef typename Tag::template result<typename vview<VA>::type, typename vview<VB>::type,
+                                        typename vview<VC>::type>::type type;
+};
+
+template <int G, class Tag, class VA, class R>
+__device__ __forceinline__ void ev(const n1v<Tag, VA>& v, size_t i, R (&r)[G]) {
+  typedef typename vview<VA>::type TA;
+  TA a[G];
+  ev<G>(v.a, i, a);
+  if constexpr (std::is_same<Tag, tagfn::if_zero_else_one_>::value) {
+#pragma unroll
+    for (int l = 0; l < G; ++l) r[l] = a[l] ? (R)0 : (R)1;
+  } else {
+    apply1<Tag>::template run<G, TA>(a, r);
+  }
+}
+
+template <int G, class Tag, class VA, class VB, class R>
+__device__ __forceinline__ void ev(const n2v<Tag, VA, VB>& v, size_t i, R (&r)[G]) {
+  typedef typename vview<VA>::type TA;
+  typedef typename vview<VB>::type TB;
+  TA a[G];
+  TB b[G];
+  ev<G>(v.a, i, a);
+  ev<G>(v.b, i, b);
+#pragma unroll
+  for (int l = 0; l < G; ++l) r[l] = apply2<Tag>::template one<R>(a[l], b[l]);
+}
+
+template <int G, class Tag, class VA, class VB, class VC, class R>
+__device__ __forceinline__ void ev(const n3v<Tag, VA, VB, VC>& v, size_t i, R (&r)[G]) {
+  _Pragma("clang fp contract(off)")
+  typedef typename vview<VA>::type TA;
+  typedef typename vview<VB>::type TB;
+  typedef typename vview<VC>::type TC;
+  TA a[G];
+  TB b[G];
+  TC c[G];
+  ev<G>(v.a, i, a);
+  ev<G>(v.b, i, b);
+  ev<G>(v.c, i, c);
+#pragma unroll
+  for (int l = 0; l < G; ++l) {
+    if constexpr (std::is_same<Tag, tagfn::if_else_>::value) r[l] = a[l] ? (R)b[l] : (R)c[l];
+    else if constexpr (std::is_same<Tag, tagfn::fma_>::value) r[l] = dm::fma<P>((R)a[l], (R)b[l], (R)c[l]);
+    else r[l] = dm::fnms<P>((R)a[l], (R)b[l], (R)c[l]);
+  }
+}
+
+// one output: packs of W from index 0, then the scalar tail
+template <class T, class V>
+__global__ __launch_bounds__(256) void transform_k(T* __restrict__ out, V v, size_t n) {
+  constexpr int W = pack_width<T>::value;
+  const size_t npacks = n / W;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < npacks) {
+    T r[W];
+    ev<W>(v, t * W, r);
+#pragma unroll
+    for (int l = 0; l < W; ++l) out[t * W + l] = r[l];
+  } else if (t < npacks + (n - npacks * W)) {
+    const size_t i = npacks * W + (t - npacks);
+    T r[1];
+    ev<1>(v, i, r);
+    out[i] = r[0];
+  }
+}
+
+// tie: every right-hand side of an index is evaluated before any output of
+// that index is written
+template <class T, class V1, class V2>
+__global__ __launch_bounds__(256) void transform2_k(T* __restrict__ o1, T* __restrict__ o2, V1 v1,
+                                                    V2 v2, size_t n) {
+  constexpr int W = pack_width<T>::value;
+  const size_t npacks = n / W;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < npacks) {
+    T r1[W], r2[W];
+    ev<W>(v1, t * W, r1);
+    ev<W>(v2, t * W, r2);
+#pragma unroll
+    for (int l = 0; l < W; ++l) { o1[t * W + l] = r1[l]; o2[t * W + l] = r2[l]; }
+  } else if (t < npacks + (n - npacks * W)) {
+    const size_t i = npacks * W + (t - npacks);
+    T r1[1], r2[1];
+    ev<1>(v1, i, r1);
+    ev<1>(v2, i, r2);
+    o1[i] = r1[0];
+    o2[i] = r2[0];
+  }
+}
+
+// fold: compensated FP32/FP64 partials per lane, error-free wave merge,
+// per-block FP64 partials combined in block order by a second launch
+template <class T>
+__device__ __forceinline__ void acc_add(T& s, T& c, T v) {
+  _Pragma("clang fp contract(off)")
+  const T t = s + v;
+  const T bp = t - s;
+  const T e = (s - (t - bp)) + (v - bp);
+  s = t;
+  c = c + e;
+}
+
+template <class T, class V>
+__global__ __launch_bounds__(256) void fold_k(V v, size_t n, double* __restrict__ partial) {
+  constexpr int W = pack_width<T>::value;
+  const size_t npacks = n / W;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  T s = 0, c = 0;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < npacks + (n - npacks * W); t += stride) {
+    if (t < npacks) {
+      T r[W];
+      ev<W>(v, t * W, r);
+#pragma unroll
+      for (int l = 0; l < W; ++l) acc_add(s, c, r[l]);
+    } else {
+      T r[1];
+      ev<1>(v, npacks * W + (t - npacks), r);
+      acc_add(s, c, r[0]);
+    }
+  }
+  double d = (double)s + (double)c;
+  __shared__ double red[256];
+  red[threadIdx.x] = d;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = 0.0;
+    for (int k = 0; k < 256; ++k) b += red[k];
+    partial[blockIdx.x] = b;
+  }
+}
+
+__global__ inline void fold_final_k(const double* __restrict__ partial, int nb, double* out) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double s = 0.0;
+    for (int k = 0; k < nb; ++k) s += partial[k];
+    *out = s;
+  }
+}
+
+inline unsigned grid(size_t items) {
+  size_t g = (items + 255) / 256;
+  return (unsigned)(g ? g : 1);
+}
+
+}}  // namespace hip::fe
+
+template <class T>
+template <class E>
+void table<T>::assign(const E& e) {
+  typedef typename expr::wrap<E>::type EX;
+  const EX& x = expr::wrap<E>::get(e);
+  const ext_t ex = x.ext();
+  if (!ex.has) throw std::invalid_argument("nt2: assignment of a scalar expression to a table");
+  const of_size_ s = ex.s;
+  x.prepare();  // terminals' device copies are current (this table included, for in-place)
+  resize(s);
+  T* out = device_data_for_write();
+  const size_t n = numel();
+  if (n == 0) return;
+  constexpr int W = hip::fe::pack_width<T>::value;
+  auto v = expr::dview<EX>::get(x);
+  hipLaunchKernelGGL((hip::fe::transform_k<T, decltype(v)>), dim3(hip::fe::grid(n / W + n % W)),
+                     dim3(256), 0, 0, out, v, n);
+  hip::fe::check(hipGetLastError(), "transform launch");
+}
+
+// globalsum(expr): fold of the whole expression, returned on the host
+template <class E, NT2H_ENABLE1(E)>
+typename expr::vt<typename expr::wrap<E>::type>::type globalsum(const E& e) {
+  typedef typename expr::wrap<E>::type EX;
+  typedef typename expr::vt<EX>::type T;
+  const EX& x = expr::wrap<E>::get(e);
+  const ext_t ex = x.ext();
+  const size_t n = ex.has ? ex.s.numel() : 1;
+  x.prepare();
+  constexpr int W = hip::fe::pack_width<T>::value;
+  const size_t items = n / W + n % W;
+  const unsigned nb = (unsigned)std::min<size_t>(1024, (items + 255) / 256 ? (items + 255) / 256 : 1);
+  double* buf = nullptr;
+  hip::fe::check(hipMalloc(&buf, sizeof(double) * (nb + 1)), "fold alloc");
+  auto v = expr::dview<EX>::get(x);
+  hipLaunchKernelGGL((hip::fe::fold_k<T, decltype(v)>), dim3(nb), dim3(256), 0, 0, v, n, buf);
+  hipLaunchKernelGGL(hip::fe::fold_final_k, dim3(1), dim3(64), 0, 0, buf, (int)nb, buf + nb);
+  double r = 0.0;
+  hip::fe::check(hipMemcpy(&r, buf + nb, sizeof(double), hipMemcpyDeviceToHost), "fold result");
+  hip::fe::check(hipFree(buf), "fold free");
+  return (T)r;
+}
+// sum of a vector table (NT2's sum along the first non-singleton dimension)
+template <class E, NT2H_ENABLE1(E)>
+auto sum(const E& e) { return globalsum(e); }
+// dot(a, b) = sum(a .* conj(b)): modules/core/reduction/include/nt2/reduction/functions/expr/dot.hpp:22-29
+template <class A, class B, NT2H_ENABLE2(A, B)>
+auto dot(const A& a, const B& b) { return globalsum(a * b); }
+
+// ---- tie ---------------------------------------------------------------------------
+template <class E1, class E2> struct tie2_rhs { E1 e1; E2 e2; };
+template <class T> struct tie2_lhs {
+  table<T>& a;
+  table<T>& b;
+  template <class E1, class E2> tie2_lhs& operator=(const tie2_rhs<E1, E2>& r) {
+    const ext_t x = merge_ext(r.e1.ext(), r.e2.ext());
+    if (!x.has) throw std::invalid_argument("nt2: tie of scalar expressions");
+    r.e1.prepare();
+    r.e2.prepare();
+    a.resize(x.s);
+    b.resize(x.s);
+    T* o1 = a.device_data_for_write();
+    T* o2 = b.device_data_for_write();
+    const size_t n = x.s.numel();
+    constexpr int W = hip::fe::pack_width<T>::value;
+    auto v1 = expr::dview<E1>::get(r.e1);
+    auto v2 = expr::dview<E2>::get(r.e2);
+    hipLaunchKernelGGL((hip::fe::transform2_k<T, decltype(v1), decltype(v2)>),
+                       dim3(hip::fe::grid(n / W + n % W)), dim3(256), 0, 0, o1, o2, v1, v2, n);
+    hip::fe::check(hipGetLastError(), "tie launch");
+    return *this;
+  }
+};
+template <class T> tie2_lhs<T> tie(table<T>& a, table<T>& b) { return tie2_lhs<T>{a, b}; }
+template <class E1, class E2>
+tie2_rhs<typename expr::wrap<E1>::type, typename expr::wrap<E2>::type> tie(const E1& e1, const E2& e2) {
+  return tie2_rhs<typename expr::wrap<E1>::type, typename expr::wrap<E2>::type>{
+      expr::wrap<E1>::get(e1), expr::wrap<E2>::get(e2)};
+}
+
+// ---- generators used by the demos ----------------------------------------------------
+template <class T> table<T> ones(of_size_ s, meta::as_<T>) {
+  table<T> t(s);
+  for (size_t i = 1; i <= s.numel(); ++i) t(i) = T(1);
+  return t;
+}
+template <class T> table<T> zeros(of_size_ s, meta::as_<T>) { return table<T>(s); }
+
+}  // namespace nt2

@@ -1,0 +1,131 @@
+// NT2 demo statements compiled unchanged against the HIP site (nt2/table.hpp)
+// and run on the GPU; every result is written to <outdir>/<name>.bin so the
+// test can compare it with the oracle.  The statement bodies are those of
+// demo/saxpy/nt2/saxpy_nt2.cpp:39-42, demo/sdot/nt2/sdot_nt2.cpp:35-38,
+// demo/blackscholes/nt2/blackandscholes_nt2.cpp:55-59 and
+// demo/sigmadelta/nt2/sigmadelta_nt2.cpp:37-58 (NT2's table API).
+#include <nt2/table.hpp>
+
+#include <cstdio>
+#include <cstdlib>
+#include <string>
+#include <vector>
+
+using namespace nt2;
+
+static std::string outdir;
+
+template <class T> static void dump(const char* name, const table<T>& t) {
+  FILE* f = std::fopen((outdir + "/" + name + ".bin").c_str(), "wb");
+  std::fwrite(t.data(), sizeof(T), t.numel(), f);
+  std::fclose(f);
+}
+template <class T> static void dump_scalar(const char* name, T v) {
+  FILE* f = std::fopen((outdir + "/" + name + ".bin").c_str(), "wb");
+  std::fwrite(&v, sizeof(T), 1, f);
+  std::fclose(f);
+}
+
+// simple seeded inputs (the test regenerates them with numpy)
+static uint64_t lcg(uint64_t& s) { s = s * 6364136223846793005ull + 1442695040888963407ull; return s >> 11; }
+static float unif(uint64_t& s, float lo, float hi) { return lo + (hi - lo) * (float)((double)lcg(s) * 0x1.0p-53); }
+
+template <class Pixel>
+void do_work(Pixel& bkg, const Pixel& fr, Pixel& var, Pixel& diff_img, Pixel& mul_img, Pixel& out) {
+  typename Pixel::value_type zero = 0;
+  bkg = nt2::selinc(bkg < fr, nt2::seldec(bkg > fr, bkg));
+  diff_img = nt2::max(bkg, fr) - nt2::min(bkg, fr);
+  mul_img = nt2::adds(nt2::adds(diff_img, diff_img), diff_img);
+  var = nt2::if_else(diff_img != zero, nt2::selinc(var < mul_img, nt2::seldec(var > mul_img, var)), var);
+  out = nt2::if_zero_else_one(diff_img < var);
+}
+
+int main(int argc, char** argv) {
+  outdir = argc > 1 ? argv[1] : ".";
+  const std::size_t size = argc > 2 ? std::strtoul(argv[2], nullptr, 10) : 100003;
+  uint64_t seed = 12345;
+
+  // ---- saxpy / sdot
+  table<float> X(of_size(size)), Y(of_size(size));
+  for (std::size_t i = 1; i <= size; ++i) { X(i) = unif(seed, -1, 1); Y(i) = unif(seed, -1, 1); }
+  dump("x", X);
+  dump("y", Y);
+  float alpha = 2.5f;
+  Y = alpha * X + Y;
+  dump("saxpy", Y);
+  float res = globalsum(X * Y);
+  dump_scalar("sdot", res);
+  dump_scalar("dot", nt2::dot(X, Y));
+
+  // ---- Black-Scholes, the demo's four statements
+  table<float> Sa(of_size(size)), Xa(of_size(size)), Ta(of_size(size)), ra(of_size(size)),
+      va(of_size(size)), R(of_size(size));
+  for (std::size_t i = 1; i <= size; ++i) {
+    Sa(i) = unif(seed, 5, 30);
+    Xa(i) = unif(seed, 1, 100);
+    Ta(i) = unif(seed, 0.25f, 10);
+    ra(i) = 0.02f;
+    va(i) = 0.30f;
+  }
+  dump("S", Sa);
+  dump("X", Xa);
+  dump("T", Ta);
+  {
+    typedef float T;
+    nt2::table<T> da = nt2::sqrt(Ta);
+    nt2::table<T> d1 = nt2::log(Sa / Xa) + (nt2::fma(nt2::sqr(va), T(0.5), ra) * Ta) / (va * da);
+    nt2::table<T> d2 = nt2::fnms(va, da, d1);
+    R = nt2::fnms(Xa * nt2::exp(-ra * Ta), nt2::fastnormcdf(d2), Sa * nt2::fastnormcdf(d1));
+  }
+  dump("bs_demo", R);
+
+  // ---- one fused statement: call and put with erfc-based normcdf, via tie
+  table<float> call, put;
+  {
+    const float r = 0.02f, v = 0.30f;
+    table<float> vsT = v * nt2::sqrt(Ta);
+    table<float> d1 = (nt2::log(Sa / Xa) + nt2::fma(nt2::sqr(v), 0.5f, r) * Ta) / vsT;
+    table<float> d2 = d1 - vsT;
+    table<float> e = Xa * nt2::exp(-r * Ta);
+    nt2::tie(call, put) = nt2::tie(Sa * nt2::normcdf(d1) - e * nt2::normcdf(d2),
+                                   e * nt2::normcdf(-d2) - Sa * nt2::normcdf(-d1));
+  }
+  dump("bs_call", call);
+  dump("bs_put", put);
+
+  // ---- trigonometric expression with packs (config-4 expression)
+  table<float> xn(of_size(size)), yn(of_size(size)), z(of_size(size));
+  for (std::size_t i = 1; i <= size; ++i) { xn(i) = unif(seed, -3, 3); yn(i) = unif(seed, -1e4, 1e4); }
+  dump("xn", xn);
+  dump("yn", yn);
+  z = nt2::exp(-xn * xn / 2.0f) * nt2::cos(yn) + nt2::sqrt(nt2::abs(xn));
+  dump("expr4", z);
+  dump_scalar("expr4_sum", nt2::globalsum(nt2::exp(-xn * xn / 2.0f) * nt2::cos(yn) + nt2::sqrt(nt2::abs(xn))));
+
+  // ---- sigma-delta: 8 frames of 37x29 pixels
+  const std::size_t h = 37, w = 29, nf = 8;
+  std::vector<table<uint8_t>> frames(nf);
+  for (std::size_t k = 0; k < nf; ++k) {
+    frames[k].resize(of_size(h, w));
+    for (std::size_t j = 1; j <= w; ++j)
+      for (std::size_t i = 1; i <= h; ++i) frames[k](i, j) = (uint8_t)(lcg(seed) & 0xff);
+  }
+  {
+    table<uint8_t> all(of_size(h * w * nf));
+    for (std::size_t k = 0; k < nf; ++k)
+      for (std::size_t p = 1; p <= h * w; ++p) all(k * h * w + p) = frames[k](p);
+    dump("frames", all);
+  }
+  table<uint8_t> image_variance = ones(of_size(h, w), meta::as_<uint8_t>());
+  table<uint8_t> background_img(of_size(h, w)), diff_img, mul_img;
+  table<uint8_t> binary_label = zeros(of_size(h, w), meta::as_<uint8_t>());
+  for (std::size_t i = 1; i <= h; ++i)
+    for (std::size_t j = 1; j <= w; ++j) background_img(i, j) = frames[0](i, j);
+  for (std::size_t k = 1; k < nf; ++k)
+    do_work(background_img, frames[k], image_variance, diff_img, mul_img, binary_label);
+  dump("sd_bkg", background_img);
+  dump("sd_var", image_variance);
+  dump("sd_label", binary_label);
+  std::printf("frontend demos done: %zu elements\n", size);
+  return 0;
+}

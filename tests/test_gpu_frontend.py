@@ -1,0 +1,80 @@
+"""The C++ front end (nt2/table.hpp): NT2 demo statements compiled unchanged
+with hipcc, run on the GPU, compared with the oracle on the same inputs."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():
+    pytest.skip("no GPU", allow_module_level=True)
+
+import _oracle as O  # noqa: E402
+from _gpu import first_diff, same_bits, say  # noqa: E402
+
+EXE = os.path.join(os.path.dirname(__file__), "cpp", "bin", "frontend_demos")
+
+
+@pytest.fixture(scope="module")
+def outputs(tmp_path_factory):
+    if not os.path.exists(EXE):
+        pytest.fail("front-end demo program not built (run the build)")
+    d = str(tmp_path_factory.mktemp("fe"))
+    say("frontend_demos")
+    p = subprocess.run([EXE, d, "100003"], capture_output=True, text=True, timeout=240)
+    assert p.returncode == 0, p.stdout + p.stderr
+
+    def rd(name, dt):
+        return np.fromfile(os.path.join(d, name + ".bin"), dtype=dt)
+    return rd
+
+
+def test_saxpy_statement(outputs):
+    x, y0, y = outputs("x", np.float32), outputs("y", np.float32), outputs("saxpy", np.float32)
+    assert same_bits(y, O.saxpy(2.5, x, y0))
+
+
+def test_sdot_and_dot_statements(outputs):
+    x, y = outputs("x", np.float32), outputs("saxpy", np.float32)
+    p = (x * y).astype(np.float32)
+    ex = O.sum_f32_exact(p)
+    mag = O.sum_f32_exact(np.abs(p))
+    for name in ("sdot", "dot"):
+        r = float(outputs(name, np.float32)[0])
+        # result rounded to float once, from a sum accurate to ~2^-40 of the magnitude
+        assert abs(r - ex) <= abs(float(np.float32(ex)) - ex) + 2 ** -40 * mag, (name, r, ex)
+
+
+def test_blackscholes_demo_statements(outputs):
+    S, X, T = (outputs(n, np.float32) for n in ("S", "X", "T"))
+    R = outputs("bs_demo", np.float32)
+    want, _ = O.bs_f32(S, X, T, 0.02, 0.30, mode=1)
+    assert same_bits(R, want), first_diff(R, want)
+
+
+def test_blackscholes_tie_statement(outputs):
+    S, X, T = (outputs(n, np.float32) for n in ("S", "X", "T"))
+    c, p = outputs("bs_call", np.float32), outputs("bs_put", np.float32)
+    wc, wp = O.bs_f32(S, X, T, 0.02, 0.30, mode=0)
+    assert same_bits(c, wc), first_diff(c, wc)
+    assert same_bits(p, wp), first_diff(p, wp)
+
+
+def test_trig_expression_and_globalsum(outputs):
+    xn, yn, z = outputs("xn", np.float32), outputs("yn", np.float32), outputs("expr4", np.float32)
+    want = O.expr4(xn, yn)
+    assert same_bits(z, want), first_diff(z, want)
+    s = float(outputs("expr4_sum", np.float32)[0])
+    ex = O.sum_f32_exact(want)
+    assert abs(s - ex) <= abs(float(np.float32(ex)) - ex) + 2 ** -40 * O.sum_f32_exact(np.abs(want))
+
+
+def test_sigmadelta_statements(outputs):
+    frames = outputs("frames", np.uint8).reshape(8, 37 * 29)
+    b, v, l, _ = O.sigmadelta(frames, 3)  # the demo multiplies by 3 with adds
+    assert np.array_equal(outputs("sd_bkg", np.uint8), b)
+    assert np.array_equal(outputs("sd_var", np.uint8), v)
+    assert np.array_equal(outputs("sd_label", np.uint8), l)

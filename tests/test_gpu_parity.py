@@ -134,7 +134,10 @@ def test_map_f32_fast_within_bound(op):
     if op == "fastnormcdf":
         ok = np.abs(got.astype(np.float64) - truth) <= 3 * np.finfo(np.float32).eps
     if op in ("cos", "sin"):
+        # packs reduced by NT2's pi/2 step evaluate the polynomials near |xr| = pi/2,
+        # where terms of magnitude 1 cancel: compare those absolutely
         ok |= np.abs(got.astype(np.float64) - truth) <= 2 * np.finfo(np.float32).eps
+        ok |= np.abs(got.astype(np.float64) - exact) <= 4 * np.finfo(np.float32).eps
     assert ok.all(), (x[~ok][:4], got[~ok][:4], truth[~ok][:4], err[~ok][:4])
 
 
@@ -340,3 +343,20 @@ def test_batch_persistent_matches_plain_and_oracle():
         assert float(rel.max()) <= 2 ** -38
     finally:
         lib().nt2h_batch_destroy(b)
+
+
+def test_blackscholes_accurate_within_4ulp():
+    """NT2H_ACCURATE: every option of a 2^22 sample within 4 ulp of the FP64
+    evaluation of the same formula on the same f32 inputs (r, v promoted from
+    their f32 values); results below the normal range within 4 denormal steps."""
+    from nt2_amd.native import lib
+    n = (1 << 22) + 3
+    say("bs accurate n=%d" % n)
+    S, X, T = _bs_inputs(n)
+    c, p = ops.blackscholes(S, X, T, 0.02, 0.30, policy=2)
+    s_, x_, t_ = (a.cpu().numpy() for a in (S, X, T))
+    rc, rp = O.bs_fp64ref(s_, x_, t_, float(np.float32(0.02)), float(np.float32(0.30)))
+    for got, ref in ((c.cpu().numpy(), rc), (p.cpu().numpy(), rp)):
+        r32 = ref.astype(np.float32).astype(np.float64)
+        e = classic_ulp(got, r32, np.float32)
+        assert e.max() <= 4.0, (float(e.max()), int(np.argmax(e)))
