@@ -119,6 +119,9 @@ int nt2h_dot_f32(size_t n, const float* x, const float* y, double* out, void* st
  * demo mode writes call only (put may be NULL). */
 int nt2h_blackscholes_f32(size_t n, const float* S, const float* X, const float* T, float r,
                           float v, float* call, float* put, int mode, int policy, void* stream);
+/* workgroups of the accurate-policy kernel's grid-stride launch (default 16384);
+ * a tuning hook with no counterpart in the reference */
+int nt2h_bs_set_grid(int workgroups);
 int nt2h_blackscholes_f64(size_t n, const double* S, const double* X, const double* T,
                           double r, double v, double* call, double* put, int policy,
                           void* stream);

@@ -3,6 +3,7 @@
 // the seeded synthetic-table generators and the runtime services of the C ABI.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -366,30 +367,35 @@ __device__ __forceinline__ void store_nt(f32x4* p, float4 v) {
   __builtin_nontemporal_store(f32x4{v.x, v.y, v.z, v.w}, p);
 }
 __device__ __forceinline__ void store_nt(f64x2* p, f64x2 v) { __builtin_nontemporal_store(v, p); }
-// accurate policy: one float4 of options per thread, FP64 inside
-__global__ __launch_bounds__(kBlock) void bs_acc_f32_k(size_t n, const float* __restrict__ S,
+// accurate policy: grid-stride over float4s of options (the LDS tables are
+// loaded once per workgroup), FP64 inside.  I is the float4 index type: 32-bit
+// when the tables hold fewer than 2^33 options, which lets the loads and
+// stores address from a scalar base plus one 32-bit offset.
+template <class I>
+__global__ __launch_bounds__(kBlock) void bs_acc_f32_k(I n4, size_t n, const float* __restrict__ S,
                                                        const float* __restrict__ X,
                                                        const float* __restrict__ T,
                                                        nt2::hip::bsacc::Params prm,
                                                        float* __restrict__ call,
-                                                       float* __restrict__ put, int vec) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t n4 = vec ? n / 4 : 0;
-  if (t < n4) {
+                                                       float* __restrict__ put) {
+  namespace ba = nt2::hip::bsacc;
+  __shared__ double tab[(ba::kTailDeg + 1) * ba::kTailN];
+  __shared__ double ex2[32];
+  ba::load_tables(tab, ex2);
+  const I stride = (I)gridDim.x * blockDim.x;
+  for (I t = (I)blockIdx.x * blockDim.x + threadIdx.x; t < n4; t += stride) {
     const f32x4 s4 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(S) + t);
     const f32x4 x4 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(X) + t);
     const f32x4 t4 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(T) + t);
-    float4 c4, p4;
-    nt2::hip::bsacc::bs_accurate_one(s4.x, x4.x, t4.x, prm, c4.x, p4.x);
-    nt2::hip::bsacc::bs_accurate_one(s4.y, x4.y, t4.y, prm, c4.y, p4.y);
-    nt2::hip::bsacc::bs_accurate_one(s4.z, x4.z, t4.z, prm, c4.z, p4.z);
-    nt2::hip::bsacc::bs_accurate_one(s4.w, x4.w, t4.w, prm, c4.w, p4.w);
-    store_nt(reinterpret_cast<f32x4*>(call) + t, c4);
-    store_nt(reinterpret_cast<f32x4*>(put) + t, p4);
-  } else {
-    const size_t i = n4 * 4 + (t - n4);
-    if (i < n) nt2::hip::bsacc::bs_accurate_one(S[i], X[i], T[i], prm, call[i], put[i]);
+    float c[4], p[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) ba::bs_accurate(s4[k], x4[k], t4[k], prm, tab, ex2, c[k], p[k]);
+    __builtin_nontemporal_store(f32x4{c[0], c[1], c[2], c[3]}, reinterpret_cast<f32x4*>(call) + t);
+    __builtin_nontemporal_store(f32x4{p[0], p[1], p[2], p[3]}, reinterpret_cast<f32x4*>(put) + t);
   }
+  const size_t sstride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)n4 * 4 + (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += sstride)
+    ba::bs_accurate(S[i], X[i], T[i], prm, tab, ex2, call[i], put[i]);
 }
 
 template <class P, int MODE>
@@ -887,6 +893,15 @@ int nt2h_dot_f32(size_t n, const float* x, const float* y, double* out, void* st
 }
 
 // ---- Black-Scholes ----------------------------------------------------------
+static unsigned g_acc_grid = 256 * 64;  // workgroups of the grid-stride accurate kernel
+
+/* tuning hook: workgroups launched by the accurate Black-Scholes kernel */
+int nt2h_bs_set_grid(int workgroups) {
+  if (workgroups < 1) return fail(NT2H_EINVAL, "nt2h_bs_set_grid: bad count");
+  g_acc_grid = (unsigned)workgroups;
+  return NT2H_OK;
+}
+
 int nt2h_blackscholes_f32(size_t n, const float* S, const float* X, const float* T, float r,
                           float v, float* call, float* put, int mode, int policy, void* stream) {
   if (n == 0) return NT2H_OK;
@@ -904,7 +919,13 @@ int nt2h_blackscholes_f32(size_t n, const float* S, const float* X, const float*
   const dim3 g(grid_for(items)), b(kBlock);
   hipStream_t s = as_stream(stream);
   if (policy == NT2H_ACCURATE) {
-    hipLaunchKernelGGL(bs_acc_f32_k, g, b, 0, s, n, S, X, T, nt2::hip::bsacc::make_params(r, v), call, put, vec);
+    const size_t n4 = vec ? n / 4 : 0;
+    const unsigned ga = std::min<unsigned>(grid_for(vec ? n4 + n % 4 : n), g_acc_grid);
+    const auto prm = nt2::hip::bsacc::make_params(r, v);
+    if (n4 < (size_t(1) << 31))
+      hipLaunchKernelGGL(bs_acc_f32_k<uint32_t>, dim3(ga), b, 0, s, (uint32_t)n4, n, S, X, T, prm, call, put);
+    else
+      hipLaunchKernelGGL(bs_acc_f32_k<size_t>, dim3(ga), b, 0, s, n4, n, S, X, T, prm, call, put);
   } else if (policy == NT2H_EXACT) {
     if (mode == NT2H_BS_TEXTBOOK) hipLaunchKernelGGL((bs_f32_k<exact_policy, NT2H_BS_TEXTBOOK>), g, b, 0, s, n, S, X, T, r, v, call, put, vec);
     else hipLaunchKernelGGL((bs_f32_k<exact_policy, NT2H_BS_DEMO>), g, b, 0, s, n, S, X, T, r, v, call, put, vec);

@@ -1,130 +1,139 @@
 // Accurate single-precision Black-Scholes for gfx950 (NT2H_ACCURATE policy).
 //
 // Same textbook formula as the NT2 evaluation (call and put from log, sqrt,
-// exp and normcdf), evaluated with this library's own FP64 approximations:
-// on MI355X a non-packed FP32 VALU op and an FP64 one issue at the same rate,
-// so carrying the whole option in FP64 costs no more instructions than FP32,
-// and it is what keeps deep out-of-the-money prices (where S*N(d1) and
-// X*exp(-rT)*N(d2) cancel by up to ~170x over the benchmark distribution)
-// within a few ulp of the exact price after the final rounding to float.
+// exp and normcdf), evaluated with this library's own approximations so that
+// deep out-of-the-money prices, where S*N(d1) and X*exp(-rT)*N(d2) cancel by
+// up to ~170x over the benchmark distribution, stay within about one ulp of
+// the exact price after the final rounding to float.
 //
-//   1/sqrt(T): v_rsq_f64 + one Newton step;  S/X: v_rcp_f64 + one Newton step
-//   log(q):  q = m 2^e, m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s),
-//            s = (m-1)/(m+1), series to s^11 (|s| <= 0.172: < 2e-11)
-//   exp(w):  w = k ln2 + r, |r| <= ln2/2, degree-10 fit (1.6e-15), ldexp
-//   N(d):    N(-|d|) = exp(-d^2/2) erfcx(|d|/sqrt2)/2 with erfcx a degree-12
-//            polynomial in t = A*C/(C+z) + B (4.3e-11 relative on [0, 12]);
-//            exp(-d2^2/2) = exp(-d1^2/2) * (S/X) * exp(rT) (exact identity)
-//   exp(rT): Taylor degree 8 on [0, 0.2]
-// Error budget of the f32 result: final rounding 0.5 ulp plus ~0.25 ulp from
-// erfcx amplified by the cancellation, the rest below 1e-9 relative.
+// Stationarity.  With d2 = d1 - tau formed exactly (tau = v sqrt(T)), the
+// derivative dC/dd1 = S phi(d1) - F phi(d1 - tau) vanishes at the true d1, so
+// an error delta in d1 moves the call (and the put) only by
+// -S phi(d1) tau delta^2 / 2: a relative ~a^2 delta^2 / 2 for a deep
+// out-of-the-money option at a = |d1|.  d1 is therefore taken from the
+// single-precision hardware rcp/log2/rsq (delta ~ 1e-6, below 1e-9 of the
+// price), while tau, d2, F = X exp(-rT) and both normcdf are FP64:
+//   exp(w)   = 2^(k/32) e^r, |r| <= ln2/64: 2^(j/32) from a 32-entry LDS
+//              table, e^r of degree 3 (7e-11), 2^(k>>5) by ldexp;
+//   N(-|d|)  = exp(-d^2/2) * erfcx(|d|/sqrt2)/2 with the second factor a
+//              piecewise polynomial (32 intervals, degree 4, 9e-11) in
+//              y = C/(C+|d|); the two tails share one reciprocal.
+// Both LDS tables hold 32 doubles per array, so a wave's gathers are free of
+// bank conflicts.  On MI355X an FP64 FMA issues at the rate of an FP32 one,
+// which is what makes the FP64 evaluation affordable.
+// Error budget of the f32 result: final rounding 0.5 ulp plus ~170 x 1.6e-10
+// from the approximations at the worst cancellation (0.4 ulp).
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <cstdint>
 
 #include "detail/bs_accurate_coeffs.hpp"
 
 namespace nt2 { namespace hip { namespace bsacc {
 
+constexpr double kRound52 = 0x1.8p52;  // adding it rounds |x| < 2^51 to an integer
+constexpr int kTabDoubles = (kTailDeg + 1) * kTailN + 32;  // LDS per workgroup (1536 B)
+
 struct Params {
-  double r, v, mu, inv_v;  // r, sigma, r + sigma^2/2, 1/sigma (from the f32 scalars)
+  double r, v;          // rate and volatility, promoted from the f32 scalars
+  float mu_v, ln2_v;    // (r + v^2/2)/v and ln2/v in single precision, for the d1 estimate
 };
 
 __host__ __device__ inline Params make_params(float r, float v) {
   Params p;
   p.r = (double)r;
   p.v = (double)v;
-  p.mu = p.r + 0.5 * p.v * p.v;
-  p.inv_v = 1.0 / p.v;
+  p.mu_v = (float)((p.r + 0.5 * p.v * p.v) / p.v);
+  p.ln2_v = (float)(0.69314718055994530942 / p.v);
   return p;
 }
 
-__device__ __forceinline__ double rcp_nr(double a) {
-  double y = __builtin_amdgcn_rcp(a);
-  const double e = __builtin_fma(-a, y, 1.0);
-  return __builtin_fma(y, e, y);
+// 2^(k>>5) * 2^((k&31)/32) * e^r0, k the low word of sh = rint(...) + kRound52
+__device__ __forceinline__ double exp_core(double sh, double r0, const double* __restrict__ ex2) {
+  const int k = (int)(uint32_t)__builtin_bit_cast(uint64_t, sh);
+  double p = __builtin_fma(r0, kExpR[3], kExpR[2]);
+  p = __builtin_fma(p, r0, kExpR[1]);
+  p = __builtin_fma(p, r0, kExpR[0]);
+  return __builtin_amdgcn_ldexp(ex2[k & 31] * p, k >> 5);
 }
 
-template <int K>
-__device__ __forceinline__ double horner_d(const double (&c)[K], double x) {
-  double acc = c[0];
-#pragma unroll
-  for (int i = 1; i < K; ++i) acc = __builtin_fma(acc, x, c[i]);
-  return acc;
+// exp(w) for w in [-700, 700]
+__device__ __forceinline__ double exp_w(double w, const double* __restrict__ ex2) {
+  const double sh = __builtin_fma(w, 46.166241308446828, kRound52);  // w * 32/ln2
+  const double kd = sh - kRound52;
+  return exp_core(sh, __builtin_fma(kd, -0.693147180559945309417 / 32.0, w), ex2);
 }
 
-// exp(w) for w in [-745, 0]
-__device__ __forceinline__ double exp_neg(double w) {
-  const double k = __builtin_rint(w * 1.4426950408889634);
-  double r = __builtin_fma(-k, 6.93147180369123816490e-01, w);  // ln2 hi (32 bits)
-  r = __builtin_fma(-k, 1.90821492927058770002e-10, r);         // ln2 lo
-  const double p = horner_d(kExp, r);
-  return __builtin_amdgcn_ldexp(p, (int)k);
+// exp(-q/2) for q in [0, 1400]
+__device__ __forceinline__ double exp_mhalf(double q, const double* __restrict__ ex2) {
+  const double sh = __builtin_fma(q, -23.083120654223414, kRound52);  // -q/2 * 32/ln2
+  const double kd = sh - kRound52;
+  return exp_core(sh, __builtin_fma(q, -0.5, kd * (-0.693147180559945309417 / 32.0)), ex2);
 }
 
-// log(q) for normal positive q
-__device__ __forceinline__ double log_pos(double q) {
-  const uint64_t b = __builtin_bit_cast(uint64_t, q);
-  int e = (int)((b >> 52) & 0x7ff) - 1023;
-  double m = __builtin_bit_cast(double, (b & 0x000fffffffffffffull) | 0x3ff0000000000000ull);
-  const bool big = m > 1.4142135623730951;
-  m = big ? 0.5 * m : m;
-  e = big ? e + 1 : e;
-  const double f = m - 1.0;
-  const double s = f * rcp_nr(2.0 + f);
-  const double s2 = s * s;
-  // 2 atanh(s) = 2s + s*s2*(2/3 + s2*(2/5 + s2*(2/7 + s2*(2/9 + s2*2/11))))
-  double p = __builtin_fma(s2, 2.0 / 11.0, 2.0 / 9.0);
-  p = __builtin_fma(p, s2, 2.0 / 7.0);
-  p = __builtin_fma(p, s2, 2.0 / 5.0);
-  p = __builtin_fma(p, s2, 2.0 / 3.0);
-  const double lm = __builtin_fma(s * s2, p, 2.0 * s);
-  return __builtin_fma((double)e, 6.93147180559945309417e-01, lm);
+// erfcx(.)/2 polynomial of interval j at u in [0, 1)
+__device__ __forceinline__ double tail_poly(const double* __restrict__ c, double u) {
+  double g = __builtin_fma(c[4 * kTailN], u, c[3 * kTailN]);
+  g = __builtin_fma(g, u, c[2 * kTailN]);
+  g = __builtin_fma(g, u, c[1 * kTailN]);
+  return __builtin_fma(g, u, c[0]);
 }
 
-__device__ __forceinline__ void bs_accurate_one(float Sf, float Kf, float Tf, const Params& P,
-                                                float& call, float& put) {
-  const double S = Sf, K = Kf, T = Tf;
-  // 1/sqrt(T), sqrt(T)
-  double y = __builtin_amdgcn_rsq(T);
-  const double h = T * y;
-  y = __builtin_fma(0.5 * y, __builtin_fma(-h, y, 1.0), y);
-  const double tau = P.v * (T * y);
-  const double inv_tau = y * P.inv_v;
-  // q = S/K, log q
-  const double q = S * rcp_nr(K);
-  const double L = log_pos(q);
-  const double d1 = __builtin_fma(P.mu, T, L) * inv_tau;
+// Q1 = N(-|d1|), Q2 = N(-|d2|); |d| is clamped at kTailDmax (tails below 1e-63
+// there).  A NaN d gives a finite Q: the callers carry NaN through S and F.
+__device__ __forceinline__ void tails(double d1, double d2, const double* __restrict__ tab,
+                                      const double* __restrict__ ex2, double& Q1, double& Q2) {
+  static_assert(kTailDeg == 4, "tail_poly is written for degree 4");
+  const double c1 = __builtin_fmin(__builtin_fabs(d1), kTailDmax);
+  const double c2 = __builtin_fmin(__builtin_fabs(d2), kTailDmax);
+  const double a1 = kTailC + c1, a2 = kTailC + c2;
+  const double pr = a1 * a2;
+  double ra = __builtin_amdgcn_rcp(pr);
+  ra = __builtin_fma(ra, __builtin_fma(-pr, ra, 1.0), ra);  // one Newton step: 1/(a1 a2)
+  const double cra = (kTailC * kTailA) * ra;
+  const double x1 = __builtin_fma(a2, cra, kTailB);  // A C/(C + |d1|) + B in [0, 32)
+  const double x2 = __builtin_fma(a1, cra, kTailB);
+  const int j1 = (int)x1, j2 = (int)x2;
+  Q1 = exp_mhalf(c1 * c1, ex2) * tail_poly(tab + j1, __builtin_amdgcn_fract(x1));
+  Q2 = exp_mhalf(c2 * c2, ex2) * tail_poly(tab + j2, __builtin_amdgcn_fract(x2));
+}
+
+__device__ __forceinline__ void bs_accurate(float Sf, float Kf, float Tf, const Params& P,
+                                            const double* __restrict__ tab,
+                                            const double* __restrict__ ex2, float& call,
+                                            float& put) {
+  // single-precision estimate of d1 = (ln(S/K) + (r + v^2/2) T) / (v sqrt(T))
+  const float yf = __builtin_amdgcn_rsqf(Tf);
+  const float qf = Sf * __builtin_amdgcn_rcpf(Kf);
+  const float d1f = __builtin_fmaf(__builtin_amdgcn_logf(qf), P.ln2_v, P.mu_v * Tf) * yf;
+  // FP64: tau = v sqrt(T) by one Newton step from the single-precision rsq
+  const double T = Tf, y = yf;
+  const double yT = T * y;
+  const double h = __builtin_fma(-yT, yT, T);
+  const double tau = __builtin_fma((0.5 * P.v) * y, h, P.v * yT);
+  const double d1 = d1f;
   const double d2 = d1 - tau;
-  // E = exp(rT) (rT in [0, 0.2]), DF = 1/E, F = K*DF
-  const double x = P.r * T;
-  double E = __builtin_fma(x, 1.0 / 40320.0, 1.0 / 5040.0);
-  E = __builtin_fma(E, x, 1.0 / 720.0);
-  E = __builtin_fma(E, x, 1.0 / 120.0);
-  E = __builtin_fma(E, x, 1.0 / 24.0);
-  E = __builtin_fma(E, x, 1.0 / 6.0);
-  E = __builtin_fma(E, x, 0.5);
-  E = __builtin_fma(E, x, 1.0);
-  E = __builtin_fma(E, x, 1.0);
-  const double F = K * rcp_nr(E);
-  // Gaussians
-  const double e1 = exp_neg(-0.5 * d1 * d1);
-  const double e2 = e1 * q * E;
-  // erfcx(|d|/sqrt2) for both, one reciprocal for the two changes of variable
-  const double z1 = __builtin_fmin(__builtin_fabs(d1) * 0.70710678118654752440, kErfcxZmax);
-  const double z2 = __builtin_fmin(__builtin_fabs(d2) * 0.70710678118654752440, kErfcxZmax);
-  const double a1 = kErfcxC + z1, a2 = kErfcxC + z2;
-  const double ra = rcp_nr(a1 * a2);
-  const double t1 = __builtin_fma(kErfcxA * kErfcxC, a2 * ra, kErfcxB);
-  const double t2 = __builtin_fma(kErfcxA * kErfcxC, a1 * ra, kErfcxB);
-  const double Q1 = 0.5 * e1 * horner_d(kErfcx, t1);  // N(-|d1|)
-  const double Q2 = 0.5 * e2 * horner_d(kErfcx, t2);  // N(-|d2|)
-  const double N1 = d1 < 0.0 ? Q1 : 1.0 - Q1, M1 = d1 < 0.0 ? 1.0 - Q1 : Q1;  // N(d1), N(-d1)
-  const double N2 = d2 < 0.0 ? Q2 : 1.0 - Q2, M2 = d2 < 0.0 ? 1.0 - Q2 : Q2;
-  const double C = S * N1 - F * N2;
-  const double Pp = F * M2 - S * M1;
-  call = (float)__builtin_fmax(C, 0.0);
-  put = (float)__builtin_fmax(Pp, 0.0);
+  const double F = (double)Kf * exp_w(__builtin_fmax(-P.r * T, -700.0), ex2);
+  double Q1, Q2;
+  tails(d1, d2, tab, ex2, Q1, Q2);
+  // N(d) = b - s Q and N(-d) = (1 - b) + s Q with s = sign(d), b = (1 + s)/2
+  const double s1 = __builtin_copysign(1.0, d1), s2 = __builtin_copysign(1.0, d2);
+  const double N1 = __builtin_fma(-s1, Q1, __builtin_fma(0.5, s1, 0.5));
+  const double M1 = __builtin_fma(s1, Q1, __builtin_fma(-0.5, s1, 0.5));
+  const double N2 = __builtin_fma(-s2, Q2, __builtin_fma(0.5, s2, 0.5));
+  const double M2 = __builtin_fma(s2, Q2, __builtin_fma(-0.5, s2, 0.5));
+  const double S = Sf;
+  call = (float)__builtin_fma(S, N1, -(F * N2));
+  put = (float)__builtin_fma(F, M2, -(S * M1));
+}
+
+// workgroup-wide copy of the tables into LDS (coefficient-major tail table, then 2^(j/32))
+__device__ __forceinline__ void load_tables(double* tab, double* ex2) {
+  for (int i = threadIdx.x; i < (kTailDeg + 1) * kTailN; i += blockDim.x) tab[i] = kTail[i];
+  for (int i = threadIdx.x; i < 32; i += blockDim.x) ex2[i] = kExp2j[i];
+  __syncthreads();
 }
 
 }}}  // namespace nt2::hip::bsacc
