@@ -119,7 +119,7 @@ int nt2h_dot_f32(size_t n, const float* x, const float* y, double* out, void* st
  * demo mode writes call only (put may be NULL). */
 int nt2h_blackscholes_f32(size_t n, const float* S, const float* X, const float* T, float r,
                           float v, float* call, float* put, int mode, int policy, void* stream);
-/* workgroups of the accurate-policy kernel's grid-stride launch (default 16384);
+/* workgroups of the accurate-policy kernel's grid-stride launch (default 8192);
  * a tuning hook with no counterpart in the reference */
 int nt2h_bs_set_grid(int workgroups);
 int nt2h_blackscholes_f64(size_t n, const double* S, const double* X, const double* T,
@@ -178,7 +178,7 @@ typedef struct {
   size_t budget_bytes; /* bytes of admitted requests' tables (5*n*sizeof(T) each) */
   int max_inflight;    /* concurrent batched launches (streams), 1..16 */
   int max_batch;       /* requests per launch; 1 with max_inflight 1 = plain schedule */
-  int policy;          /* NT2H_EXACT or NT2H_FAST */
+  int policy;          /* NT2H_EXACT, NT2H_FAST, or NT2H_ACCURATE (f32 requests; f64 ones exact) */
   int backend;         /* NT2H_BACKEND_HIP or NT2H_BACKEND_HOST */
   int fail_alloc_every; /* test hook: fail every k-th output allocation (0 = never) */
 } nt2h_server_config;

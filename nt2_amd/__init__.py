@@ -8,5 +8,5 @@ touch the GPU; calling anything without the built library raises.
 """
 from . import native  # noqa: F401
 from .native import (  # noqa: F401
-    EXACT, FAST, OPS, BS_TEXTBOOK, BS_DEMO, Nt2hError, lib, lib_path,
+    EXACT, FAST, ACCURATE, OPS, BS_TEXTBOOK, BS_DEMO, Nt2hError, lib, lib_path,
 )

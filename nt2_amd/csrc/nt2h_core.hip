@@ -893,7 +893,7 @@ int nt2h_dot_f32(size_t n, const float* x, const float* y, double* out, void* st
 }
 
 // ---- Black-Scholes ----------------------------------------------------------
-static unsigned g_acc_grid = 256 * 64;  // workgroups of the grid-stride accurate kernel
+static unsigned g_acc_grid = 256 * 32;  // workgroups of the grid-stride accurate kernel (32 per CU)
 
 /* tuning hook: workgroups launched by the accurate Black-Scholes kernel */
 int nt2h_bs_set_grid(int workgroups) {

@@ -38,6 +38,7 @@
 #include <vector>
 
 #include "bs_device.hpp"
+#include "nt2/hip/bs_accurate.hpp"
 #include "common.hpp"
 #include "nt2h.h"
 
@@ -70,57 +71,89 @@ struct ReqDesc {
 
 constexpr int kSrvBlock = 256;
 constexpr int kSrvTile = kSrvBlock * 4;  // 4 options per thread
+constexpr uint32_t kSrvGridMax = 256 * 32;  // workgroups of a batch launch (32 per CU)
 
-template <class P>
-__global__ __launch_bounds__(kSrvBlock) void serve_bs_k(const ReqDesc* __restrict__ reqs, int nreq) {
+typedef float srv_f32x4 __attribute__((ext_vector_type(4)));
+
+// One launch prices every request of a batch.  The batch's options are cut in
+// tiles of kSrvTile (a request's tiles are consecutive, starting at tile0);
+// workgroup k walks the contiguous tile range [k tpb, (k+1) tpb), so the
+// owning request is found by one binary search and then advances linearly.
+// P: the NT2 policy of the f64 requests and (ACC false) of the f32 ones;
+// ACC: f32 requests use the accurate FP64-internal evaluation.
+template <class P, bool ACC>
+__global__ __launch_bounds__(kSrvBlock) void serve_bs_k(const ReqDesc* __restrict__ reqs, int nreq,
+                                                      uint32_t ntiles, uint32_t tpb) {
   _Pragma("clang fp contract(off)")
-  // find the request owning this tile: last r with reqs[r].tile0 <= blockIdx.x
-  int lo = 0, hi = nreq - 1;
-  const uint32_t b = blockIdx.x;
+  namespace ba = nt2::hip::bsacc;
+  __shared__ double tab[ACC ? (ba::kTailDeg + 1) * ba::kTailN : 1];
+  __shared__ double ex2[ACC ? 32 : 1];
+  if (ACC) ba::load_tables(tab, ex2);
+  const uint32_t b0 = blockIdx.x * tpb;
+  const uint32_t b1 = min(b0 + tpb, ntiles);
+  if (b0 >= b1) return;
+  int lo = 0, hi = nreq - 1;  // last request with tile0 <= b0
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (reqs[mid].tile0 <= b) lo = mid; else hi = mid - 1;
+    if (reqs[mid].tile0 <= b0) lo = mid; else hi = mid - 1;
   }
-  const ReqDesc d = reqs[lo];
-  const uint64_t base = (uint64_t)(b - d.tile0) * kSrvTile;
-  if (!d.f64) {
-    const float* S = (const float*)d.S;
-    const float* X = (const float*)d.X;
-    const float* T = (const float*)d.T;
-    float* C = (float*)d.call;
-    float* Pt = (float*)d.put;
-    const float r = (float)d.r, v = (float)d.v;
-    const float drift = (v * v) * 0.5f + r, mr = -r;
-    for (uint64_t i = base + threadIdx.x; i < base + kSrvTile && i < d.n; i += kSrvBlock) {
-      float c, p;
-      bs_one_f32<P, NT2H_BS_TEXTBOOK>(S[i], X[i], T[i], mr, v, drift, c, p);
-      C[i] = c;
-      Pt[i] = p;
-    }
-  } else {
-    const double* S = (const double*)d.S;
-    const double* X = (const double*)d.X;
-    const double* T = (const double*)d.T;
-    double* C = (double*)d.call;
-    double* Pt = (double*)d.put;
-    const double r = d.r, v = d.v;
-    const double drift = (v * v) * 0.5 + r, mr = -r;
-    const uint64_t npacks = d.n / 4;
-    // one NT2 pack of 4 per thread; the request's last n%4 options use NT2's scalar erfc
-    const uint64_t pk = base / 4 + threadIdx.x;
-    if (pk < npacks) {
+  for (uint32_t b = b0; b < b1; ++b) {
+    while (lo + 1 < nreq && reqs[lo + 1].tile0 <= b) ++lo;
+    const ReqDesc& d = reqs[lo];
+    const uint64_t base = (uint64_t)(b - d.tile0) * kSrvTile;
+    if (!d.f64) {
+      const float* S = (const float*)d.S;
+      const float* X = (const float*)d.X;
+      const float* T = (const float*)d.T;
+      float* C = (float*)d.call;
+      float* Pt = (float*)d.put;
+      const float r = (float)d.r, v = (float)d.v;
+      const bool vec = ((reinterpret_cast<uintptr_t>(S) | reinterpret_cast<uintptr_t>(X) |
+                         reinterpret_cast<uintptr_t>(T) | reinterpret_cast<uintptr_t>(C) |
+                         reinterpret_cast<uintptr_t>(Pt)) & 15) == 0;
+      const uint64_t i4 = base + 4 * threadIdx.x;
+      if (ACC) {
+        const ba::Params prm = ba::make_params(r, v);
+        if (vec && i4 + 4 <= d.n) {
+          const srv_f32x4 s4 = __builtin_nontemporal_load(reinterpret_cast<const srv_f32x4*>(S + i4));
+          const srv_f32x4 x4 = __builtin_nontemporal_load(reinterpret_cast<const srv_f32x4*>(X + i4));
+          const srv_f32x4 t4 = __builtin_nontemporal_load(reinterpret_cast<const srv_f32x4*>(T + i4));
+          float c[4], p[4];
 #pragma unroll
-      for (int l = 0; l < 4; ++l) {
-        const uint64_t i = pk * 4 + l;
-        double c, p;
-        bs_one_f64<P, false>(S[i], X[i], T[i], mr, v, drift, c, p);
-        C[i] = c;
-        Pt[i] = p;
+          for (int k = 0; k < 4; ++k) ba::bs_accurate(s4[k], x4[k], t4[k], prm, tab, ex2, c[k], p[k]);
+          __builtin_nontemporal_store(srv_f32x4{c[0], c[1], c[2], c[3]}, reinterpret_cast<srv_f32x4*>(C + i4));
+          __builtin_nontemporal_store(srv_f32x4{p[0], p[1], p[2], p[3]}, reinterpret_cast<srv_f32x4*>(Pt + i4));
+        } else if (vec) {  // the request's last, partial float4: this thread's options only
+          for (uint64_t i = i4; i < i4 + 4 && i < d.n; ++i)
+            ba::bs_accurate(S[i], X[i], T[i], prm, tab, ex2, C[i], Pt[i]);
+        } else {
+          for (uint64_t i = base + threadIdx.x; i < base + kSrvTile && i < d.n; i += kSrvBlock)
+            ba::bs_accurate(S[i], X[i], T[i], prm, tab, ex2, C[i], Pt[i]);
+        }
+      } else {
+        const float drift = (v * v) * 0.5f + r, mr = -r;
+        for (uint64_t i = base + threadIdx.x; i < base + kSrvTile && i < d.n; i += kSrvBlock) {
+          float c, p;
+          bs_one_f32<P, NT2H_BS_TEXTBOOK>(S[i], X[i], T[i], mr, v, drift, c, p);
+          C[i] = c;
+          Pt[i] = p;
+        }
       }
-    } else if (pk == npacks) {
-      for (uint64_t i = npacks * 4; i < d.n; ++i) {
+    } else {
+      const double* S = (const double*)d.S;
+      const double* X = (const double*)d.X;
+      const double* T = (const double*)d.T;
+      double* C = (double*)d.call;
+      double* Pt = (double*)d.put;
+      const double r = d.r, v = d.v;
+      const double drift = (v * v) * 0.5 + r, mr = -r;
+      // NT2 packs of 4 from the request's first option; its last n%4 options
+      // take NT2's scalar erfc
+      const uint64_t npk = d.n / 4 * 4;
+      for (uint64_t i = base + threadIdx.x; i < base + kSrvTile && i < d.n; i += kSrvBlock) {
         double c, p;
-        bs_one_f64<P, true>(S[i], X[i], T[i], mr, v, drift, c, p);
+        if (i < npk) bs_one_f64<P, false>(S[i], X[i], T[i], mr, v, drift, c, p);
+        else bs_one_f64<P, true>(S[i], X[i], T[i], mr, v, drift, c, p);
         C[i] = c;
         Pt[i] = p;
       }
@@ -157,11 +190,20 @@ static void host_bs(const ReqDesc& d) {
 enum ReqState { RS_PENDING = 0, RS_ADMITTED = 1, RS_RUNNING = 2, RS_DONE = 3, RS_FAILED = 4,
                 RS_CANCELLED = 5 };
 
+// One device allocation holds the outputs of every request of a batch that
+// did not bring its own; it is returned when the last of them is released.
+struct OutBlock {
+  void* base = nullptr;
+  int refs = 0;
+  int slot = -1;
+};
+
 struct Request {
   uint64_t id;
   ReqDesc d;
   size_t bytes;        // budget charge: inputs + outputs
-  bool own_out;        // outputs allocated by the server
+  bool own_out;        // outputs allocated by the server (in blk)
+  OutBlock* blk = nullptr;
   int state;
   int status;
   int64_t t_arrival, t_admit, t_launch, t_done;
@@ -194,6 +236,8 @@ struct Server {
   bool sched_exited = false;
   std::thread sched, done_thr;
   int alloc_count = 0;
+  hipMemPool_t pool = nullptr;  // the server's own stream-ordered pool: it keeps up to the
+                                // budget cached, so steady-state admissions do not reach the driver
   // statistics
   std::vector<double> lat_us;
   uint64_t options_done = 0, launches = 0, batched_reqs = 0;
@@ -211,18 +255,20 @@ static int srv_alloc(Server* s, Slot& sl, size_t bytes, void** p) {
     *p = std::malloc(bytes);
     return *p ? NT2H_OK : NT2H_ENOMEM;
   }
-  hipError_t e = hipMallocAsync(p, bytes, sl.stream);
+  hipError_t e = s->pool ? hipMallocFromPoolAsync(p, bytes, s->pool, sl.stream)
+                         : hipMallocAsync(p, bytes, sl.stream);
   return e == hipSuccess ? NT2H_OK : NT2H_ENOMEM;
 }
 
 static void srv_free(Server* s, Request* r) {
   if (!r->own_out) return;
-  if (s->cfg.backend == NT2H_BACKEND_HOST) {
-    std::free(r->d.call);
-  } else {
-    hipStream_t st = (r->slot >= 0) ? s->slots[r->slot].stream : nullptr;
-    (void)hipFreeAsync(r->d.call, st);
+  OutBlock* b = r->blk;
+  if (--b->refs == 0) {
+    if (s->cfg.backend == NT2H_BACKEND_HOST) std::free(b->base);
+    else (void)hipFreeAsync(b->base, s->slots[b->slot].stream);
+    delete b;
   }
+  r->blk = nullptr;
   r->d.call = r->d.put = nullptr;
   r->own_out = false;
 }
@@ -238,25 +284,40 @@ static void launch_batch(Server* s, int si, std::unique_lock<std::mutex>& lk) {
     batch.push_back(r);
   }
   if (batch.empty()) return;
-  // output tables (one allocation holds call and put)
-  std::vector<Request*> ok;
+  // output tables: one allocation for the batch, 256-byte aligned call and put per request
+  auto out_bytes = [](const Request* r) { return (r->d.n * elem(r->d) + 255) / 256 * 256; };
+  size_t total = 0;
+  int owners = 0;
   for (Request* r : batch) {
     r->slot = si;
+    if (!r->d.call) { total += 2 * out_bytes(r); ++owners; }
+  }
+  void* base = nullptr;
+  const int arc = owners ? srv_alloc(s, sl, total, &base) : NT2H_OK;
+  OutBlock* blk = nullptr;
+  if (owners && arc == NT2H_OK) {
+    blk = new OutBlock();
+    blk->base = base;
+    blk->refs = owners;
+    blk->slot = si;
+  }
+  std::vector<Request*> ok;
+  size_t off = 0;
+  for (Request* r : batch) {
     if (!r->d.call) {
-      void* p = nullptr;
-      const size_t ob = 2 * r->d.n * elem(r->d);
-      int rc = srv_alloc(s, sl, ob, &p);
-      if (rc) {
+      if (arc) {
         r->state = RS_FAILED;
-        r->status = rc;
+        r->status = arc;
         r->t_done = now_ns();
         s->used -= r->bytes;
         ++s->failed;
         continue;
       }
-      r->d.call = p;
-      r->d.put = (char*)p + r->d.n * elem(r->d);
+      r->d.call = (char*)base + off;
+      r->d.put = (char*)base + off + out_bytes(r);
+      off += 2 * out_bytes(r);
       r->own_out = true;
+      r->blk = blk;
     }
     ok.push_back(r);
   }
@@ -300,10 +361,14 @@ static void launch_batch(Server* s, int si, std::unique_lock<std::mutex>& lk) {
   } else {
     lk.unlock();
     (void)hipMemcpyAsync(sl.d_desc, sl.h_desc, sizeof(ReqDesc) * nreq, hipMemcpyHostToDevice, sl.stream);
+    const uint32_t grid = std::min<uint32_t>(tiles, kSrvGridMax);
+    const uint32_t tpb = (tiles + grid - 1) / grid;
     if (s->cfg.policy == NT2H_EXACT)
-      hipLaunchKernelGGL(serve_bs_k<exact_policy>, dim3(tiles), dim3(kSrvBlock), 0, sl.stream, sl.d_desc, nreq);
+      hipLaunchKernelGGL((serve_bs_k<exact_policy, false>), dim3(grid), dim3(kSrvBlock), 0, sl.stream, sl.d_desc, nreq, tiles, tpb);
+    else if (s->cfg.policy == NT2H_FAST)
+      hipLaunchKernelGGL((serve_bs_k<fast_policy, false>), dim3(grid), dim3(kSrvBlock), 0, sl.stream, sl.d_desc, nreq, tiles, tpb);
     else
-      hipLaunchKernelGGL(serve_bs_k<fast_policy>, dim3(tiles), dim3(kSrvBlock), 0, sl.stream, sl.d_desc, nreq);
+      hipLaunchKernelGGL((serve_bs_k<exact_policy, true>), dim3(grid), dim3(kSrvBlock), 0, sl.stream, sl.d_desc, nreq, tiles, tpb);
     (void)hipEventRecord(sl.event, sl.stream);
     lk.lock();
   }
@@ -395,7 +460,7 @@ extern "C" {
 
 int nt2h_server_create(const nt2h_server_config* cfg, nt2h_server** out) {
   if (!cfg || !out || cfg->budget_bytes == 0 || cfg->max_inflight < 1 || cfg->max_inflight > 16 ||
-      cfg->max_batch < 1 || (cfg->policy != NT2H_EXACT && cfg->policy != NT2H_FAST) ||
+      cfg->max_batch < 1 || (cfg->policy != NT2H_EXACT && cfg->policy != NT2H_FAST && cfg->policy != NT2H_ACCURATE) ||
       (cfg->backend != NT2H_BACKEND_HIP && cfg->backend != NT2H_BACKEND_HOST))
     return fail(NT2H_EINVAL, "nt2h_server_create: bad configuration");
   Server* s = new Server();
@@ -408,6 +473,19 @@ int nt2h_server_create(const nt2h_server_config* cfg, nt2h_server** out) {
         delete s;
         return fail(NT2H_EHIP, "nt2h_server_create: stream/event creation failed");
       }
+    }
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    hipMemPoolProps props = {};
+    props.allocType = hipMemAllocationTypePinned;
+    props.location.type = hipMemLocationTypeDevice;
+    props.location.id = dev;
+    if (hipMemPoolCreate(&s->pool, &props) == hipSuccess) {
+      uint64_t keep = cfg->budget_bytes;
+      (void)hipMemPoolSetAttribute(s->pool, hipMemPoolAttrReleaseThreshold, &keep);
+    } else {
+      s->pool = nullptr;  // fall back to the device's default pool
+      (void)hipGetLastError();
     }
   }
   s->sched = std::thread(sched_loop, s);
@@ -661,6 +739,9 @@ int nt2h_server_destroy(nt2h_server* hs) {
       if (sl.event) (void)hipEventDestroy(sl.event);
       if (sl.stream) (void)hipStreamDestroy(sl.stream);
     }
+  }
+  if (s->pool) (void)hipMemPoolDestroy(s->pool);  // every free above completed on its slot stream
+  {
   }
   delete s;
   return NT2H_OK;

@@ -6,7 +6,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB_NAME = "libnt2h.so"
 
-EXACT, FAST = 0, 1
+EXACT, FAST, ACCURATE = 0, 1, 2
 BS_TEXTBOOK, BS_DEMO = 0, 1
 OPS = {"exp": 0, "log": 1, "sqrt": 2, "erfc": 3, "normcdf": 4,
        "fastnormcdf": 5, "cos": 6, "sin": 7, "abs": 8}

@@ -14,9 +14,8 @@ import numpy as np
 import torch
 
 from . import ops
-from .native import EXACT, FAST
+from .native import ACCURATE, EXACT, FAST
 
-ACCURATE = 2
 
 
 def _copy():
@@ -58,31 +57,14 @@ def sd(reps=3):
 
 
 def batch(reps=2, count=4096, seed=4):
-    import ctypes
-    from .native import check, lib, ptr, stream_handle
-    rng = np.random.default_rng(seed)
-    lens = np.floor(np.exp(rng.uniform(np.log(1e3), np.log(1e6), count))).astype(np.int64)
-    total = int(lens.sum())
-    x = ops.fill_normal(total, 41)
-    y = ops.fill_uniform(total, -1e4, 1e4, 42)
-    offs = np.concatenate([[0], np.cumsum(lens)[:-1]])
-    # 16-byte aligned sub-tables: round each offset up to a multiple of 4 floats
-    offs = (offs + 3) // 4 * 4
-    need = int(offs[-1] + lens[-1])
-    if need > total:
-        x = torch.cat([x, ops.fill_normal(need - total, 43)])
-        y = torch.cat([y, ops.fill_uniform(need - total, -1e4, 1e4, 44)])
-    XP = (ctypes.c_void_p * count)(*[x.data_ptr() + 4 * int(o) for o in offs])
-    YP = (ctypes.c_void_p * count)(*[y.data_ptr() + 4 * int(o) for o in offs])
-    LN = (ctypes.c_size_t * count)(*[int(n) for n in lens])
-    b = ctypes.c_void_p()
-    check(lib().nt2h_batch_create(ctypes.byref(b), count, XP, YP, LN, FAST), "batch_create")
-    sums = torch.zeros(count, dtype=torch.float64, device="cuda")
-    for _ in range(reps):
-        check(lib().nt2h_batch_run(b, ptr(sums), stream_handle()), "batch_run")
-        check(lib().nt2h_batch_run_plain(b, ptr(sums), stream_handle()), "batch_plain")
-    torch.cuda.synchronize()
-    lib().nt2h_batch_destroy(b)
+    from .batch import Batch, new_sums, uneven_tables
+    x, y, offs, lens = uneven_tables(count, seed=seed)
+    sums = new_sums(count)
+    with Batch(x, y, offs, lens, FAST) as b:
+        for _ in range(reps):
+            b.run(sums)
+            b.run_plain(sums)
+        torch.cuda.synchronize()
     _copy()
 
 
