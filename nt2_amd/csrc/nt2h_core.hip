@@ -497,55 +497,134 @@ __device__ __forceinline__ uint32_t wave_count(uint32_t c) {
   return c;
 }
 
-// 16 pixels per thread (one 128-bit load per frame)
+// 16 pixels per thread (one 128-bit load per frame), two pixels per 32-bit
+// register as 16-bit halves so that every step is one packed integer op per
+// pixel pair.  Per pair (values 0..255 in signed 16-bit lanes):
+//   b = min(b+1, max(b-1, f))            selinc(b<f, seldec(b>f, b))
+//   d = max(b,f) - min(b,f)
+//   m = min(N d, 255)                    adds applied N-1 times
+//   v = min(v+nz, max(v-nz, m)), nz = min(d,1)   (no change when d == 0)
+//   l = (d + 256 - v) >> 8               if_zero_else_one(d < v)
+// The frame loop loads 4 frames ahead.  Per-frame label counts: a wave sum
+// per frame into LDS, the workgroup's sum per frame to partials[k][block],
+// and sd_counts_k adds the partials of each frame in block order.
+typedef short sd_i16x2 __attribute__((ext_vector_type(2)));
+typedef unsigned short sd_u16x2 __attribute__((ext_vector_type(2)));
+constexpr int kSdChunk = 256;  // frames per launch (LDS holds their wave counts)
+
+__device__ __forceinline__ sd_i16x2 sd_lo(uint32_t w) {  // bytes 0,1 -> 16-bit lanes
+  return __builtin_bit_cast(sd_i16x2, __builtin_amdgcn_perm(0u, w, 0x0c010c00u));
+}
+__device__ __forceinline__ sd_i16x2 sd_hi(uint32_t w) {  // bytes 2,3 -> 16-bit lanes
+  return __builtin_bit_cast(sd_i16x2, __builtin_amdgcn_perm(0u, w, 0x0c030c02u));
+}
+__device__ __forceinline__ uint32_t sd_pack(sd_i16x2 lo, sd_i16x2 hi) {  // low bytes of 4 lanes
+  return __builtin_amdgcn_perm(__builtin_bit_cast(uint32_t, hi), __builtin_bit_cast(uint32_t, lo), 0x06040200u);
+}
+__device__ __forceinline__ sd_i16x2 sd_pair(sd_i16x2 f, sd_i16x2& b, sd_i16x2& v, sd_u16x2 n) {
+  const sd_i16x2 one = {1, 1};
+  b = __builtin_elementwise_min(b + one, __builtin_elementwise_max(b - one, f));
+  const sd_i16x2 d = __builtin_elementwise_max(b, f) - __builtin_elementwise_min(b, f);
+  const sd_u16x2 mu = __builtin_elementwise_min(__builtin_bit_cast(sd_u16x2, d) * n, (sd_u16x2){255, 255});
+  const sd_i16x2 m = __builtin_bit_cast(sd_i16x2, mu);
+  const sd_i16x2 nz = __builtin_elementwise_min(d, one);
+  v = __builtin_elementwise_min(v + nz, __builtin_elementwise_max(v - nz, m));
+  // d >= v  <=>  (d + 256 - v) >> 8 == 1, for d, v in [0, 255]
+  return __builtin_bit_cast(sd_i16x2, __builtin_bit_cast(sd_u16x2, d + (sd_i16x2){256, 256} - v) >> (sd_u16x2){8, 8});
+}
+
 __global__ __launch_bounds__(kBlock) void sigmadelta16_k(const uint8_t* __restrict__ frames,
-                                                         size_t stride, int nframes, size_t n16,
+                                                         size_t stride, int k0, int k1, size_t n16,
                                                          uint8_t* __restrict__ bkg,
                                                          uint8_t* __restrict__ var,
                                                          uint8_t* __restrict__ label, int N,
-                                                         uint32_t* __restrict__ counts) {
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+                                                         uint32_t* __restrict__ partials) {
+  __shared__ uint32_t wc[kSdChunk][kBlock / 64];
+  // XCD-aware tiles: workgroup b runs on XCD b % 8, and each XCD takes a
+  // contiguous eighth of the image, so that its L2 sees contiguous spans of
+  // every frame (the grid is a multiple of 8)
+  const unsigned per = gridDim.x / 8;
+  const size_t tile = (size_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
+  const size_t t = tile * blockDim.x + threadIdx.x;
   const bool active = t < n16;
-  uint32_t b[16], v[16], l[16];
-  if (active) {
-    const uint4 bb = reinterpret_cast<const uint4*>(bkg)[t];
-    const uint4 vv = reinterpret_cast<const uint4*>(var)[t];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const unsigned short nn = (unsigned short)(N < 255 ? N : 255);
+  const sd_u16x2 n = {nn, nn};
+  sd_i16x2 b[8], v[8], l[8];
+  {
+    uint4 bb = make_uint4(0, 0, 0, 0), vv = make_uint4(0, 0, 0, 0);
+    if (active) {
+      bb = reinterpret_cast<const uint4*>(bkg)[t];
+      vv = reinterpret_cast<const uint4*>(var)[t];
+    }
     const uint32_t bw[4] = {bb.x, bb.y, bb.z, bb.w}, vw[4] = {vv.x, vv.y, vv.z, vv.w};
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      b[q] = (bw[q >> 2] >> (8 * (q & 3))) & 0xffu;
-      v[q] = (vw[q >> 2] >> (8 * (q & 3))) & 0xffu;
-      l[q] = 0u;
+    for (int q = 0; q < 4; ++q) {
+      b[2 * q] = sd_lo(bw[q]);
+      b[2 * q + 1] = sd_hi(bw[q]);
+      v[2 * q] = sd_lo(vw[q]);
+      v[2 * q + 1] = sd_hi(vw[q]);
+      l[2 * q] = l[2 * q + 1] = (sd_i16x2){0, 0};
     }
   }
-  for (int k = 1; k < nframes; ++k) {
-    uint32_t cnt = 0;
-    if (active) {
-      const uint4 ff = reinterpret_cast<const uint4*>(frames + (size_t)k * stride)[t];
-      const uint32_t fw[4] = {ff.x, ff.y, ff.z, ff.w};
+  for (int k = k0; k < k1; k += 4) {
+    uint4 ff[4];
 #pragma unroll
-      for (int q = 0; q < 16; ++q) {
-        sd_pixel((fw[q >> 2] >> (8 * (q & 3))) & 0xffu, b[q], v[q], l[q], N);
-        cnt += l[q];
+    for (int j = 0; j < 4; ++j)  // clamped indices: every load in bounds and issued unconditionally
+      ff[j] = reinterpret_cast<const uint4*>(frames + (size_t)(k + j < k1 ? k + j : k1 - 1) * stride)[active ? t : 0];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      if (k + j >= k1) break;
+      const uint32_t fw[4] = {ff[j].x, ff[j].y, ff[j].z, ff[j].w};
+      sd_i16x2 acc = {0, 0};
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        l[2 * q] = sd_pair(sd_lo(fw[q]), b[2 * q], v[2 * q], n);
+        l[2 * q + 1] = sd_pair(sd_hi(fw[q]), b[2 * q + 1], v[2 * q + 1], n);
+        acc += l[2 * q] + l[2 * q + 1];
       }
-    }
-    if (counts) {
+      uint32_t cnt = active ? (uint32_t)(acc.x + acc.y) : 0u;
       cnt = wave_count(cnt);
-      if ((threadIdx.x & 63) == 0 && cnt) atomicAdd(&counts[k], cnt);
+      if (lane == 0) wc[k + j - k0][wid] = cnt;
     }
   }
   if (active) {
-    uint32_t bw[4] = {0, 0, 0, 0}, vw[4] = {0, 0, 0, 0}, lw[4] = {0, 0, 0, 0};
+    uint32_t bw[4], vw[4], lw[4];
 #pragma unroll
-    for (int q = 0; q < 16; ++q) {
-      bw[q >> 2] |= b[q] << (8 * (q & 3));
-      vw[q >> 2] |= v[q] << (8 * (q & 3));
-      lw[q >> 2] |= l[q] << (8 * (q & 3));
+    for (int q = 0; q < 4; ++q) {
+      bw[q] = sd_pack(b[2 * q], b[2 * q + 1]);
+      vw[q] = sd_pack(v[2 * q], v[2 * q + 1]);
+      lw[q] = sd_pack(l[2 * q], l[2 * q + 1]);
     }
     reinterpret_cast<uint4*>(bkg)[t] = make_uint4(bw[0], bw[1], bw[2], bw[3]);
     reinterpret_cast<uint4*>(var)[t] = make_uint4(vw[0], vw[1], vw[2], vw[3]);
     reinterpret_cast<uint4*>(label)[t] = make_uint4(lw[0], lw[1], lw[2], lw[3]);
   }
+  if (partials) {
+    __syncthreads();
+    for (int i = threadIdx.x; i < k1 - k0; i += blockDim.x) {
+      uint32_t c = 0;
+#pragma unroll
+      for (int w = 0; w < kBlock / 64; ++w) c += wc[i][w];
+      partials[(size_t)i * gridDim.x + blockIdx.x] = c;
+    }
+  }
+}
+
+// counts[k0 + i] += sum over blocks of partials[i][block], blocks in order
+__global__ __launch_bounds__(kBlock) void sd_counts_k(const uint32_t* __restrict__ partials,
+                                                      int nblocks, int nk,
+                                                      uint32_t* __restrict__ counts) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nk) return;
+  uint32_t c = 0;
+  for (int b = 0; b < nblocks; ++b) c += partials[(size_t)i * nblocks + b];
+  counts[i] += c;
+}
+
+__global__ __launch_bounds__(kBlock) void zero_u32_k(uint32_t* __restrict__ p, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
 }
 
 // one pixel per thread: pixels [p0, npix) (tails and unaligned buffers)
@@ -960,15 +1039,36 @@ int nt2h_sigmadelta_u8(size_t npix, int nframes, const uint8_t* frames, size_t f
   if (!frames || !bkg || !var || !label || nframes < 1 || N < 1 || frame_stride < npix)
     return fail(NT2H_EINVAL, "nt2h_sigmadelta_u8: bad argument");
   hipStream_t s = as_stream(stream);
-  if (frame_counts) NT2H_CHECK(hipMemsetAsync(frame_counts, 0, sizeof(uint32_t) * nframes, s), "memset counts");
+  // counts are accumulated by the kernels below: start from zero (a fill
+  // kernel, so that the sequence can be captured in a graph)
+  if (frame_counts) {
+    hipLaunchKernelGGL(zero_u32_k, dim3((nframes + kBlock - 1) / kBlock), dim3(kBlock), 0, s, frame_counts, nframes);
+    NT2H_CHECK_LAUNCH("sigmadelta zero counts");
+  }
   const bool vec = aligned16(frames) && aligned16(bkg) && aligned16(var) && aligned16(label) &&
                    (frame_stride % 16 == 0);
   size_t p0 = 0;
   if (vec) {
     const size_t n16 = npix / 16;
     if (n16) {
-      hipLaunchKernelGGL(sigmadelta16_k, dim3(grid_for(n16)), dim3(kBlock), 0, s, frames, frame_stride, nframes, n16, bkg, var, label, N, frame_counts);
-      NT2H_CHECK_LAUNCH("sigmadelta16");
+      const unsigned grid = (grid_for(n16) + 7) / 8 * 8;  // a multiple of the 8 XCDs
+      Scratch sc;
+      if (frame_counts) {
+        const size_t need = (size_t)kSdChunk * grid;  // uint32 partials of one chunk
+        if (int e = get_scratch((need + 1) / 2, &sc, s)) return e;
+      }
+      uint32_t* part = frame_counts ? reinterpret_cast<uint32_t*>(sc.partial) : nullptr;
+      for (int k0 = 1; k0 < nframes; k0 += kSdChunk) {
+        const int k1 = std::min(nframes, k0 + kSdChunk);
+        hipLaunchKernelGGL(sigmadelta16_k, dim3(grid), dim3(kBlock), 0, s, frames, frame_stride, k0, k1, n16,
+                           bkg, var, label, N, part);
+        NT2H_CHECK_LAUNCH("sigmadelta16");
+        if (part) {
+          hipLaunchKernelGGL(sd_counts_k, dim3((k1 - k0 + kBlock - 1) / kBlock), dim3(kBlock), 0, s, part,
+                             (int)grid, k1 - k0, frame_counts + k0);
+          NT2H_CHECK_LAUNCH("sigmadelta counts");
+        }
+      }
     }
     p0 = n16 * 16;
   }
