@@ -157,8 +157,12 @@ def leg_sigmadelta(torch, ops):
             "frames_per_s": 64 / (ms * 1e-3), "frame_gbs": 64.0 * npix / (ms * 1e-3) / 1e9}
 
 
-def leg_batch(torch):
-    from nt2_amd import FAST
+def leg_batch(torch, O):
+    """4096 uneven tables (config shape); persistent graph launch vs one
+    launch per table, and the sums' error on 32 sampled tables: against the
+    exact sum of the f32 terms the oracle evaluates (summation error) and
+    against the FP64 evaluation of the expression (total error)."""
+    from nt2_amd import EXACT, FAST
     from nt2_amd.batch import Batch, new_sums, uneven_tables
     count = 4096
     x, y, offs, lens = uneven_tables(count, seed=4)
@@ -168,20 +172,42 @@ def leg_batch(torch):
         cfg = b.config()
         ms = time_steps(torch, lambda: b.run(sums), 5, 1)
         plain_ms = time_steps(torch, lambda: b.run_plain(sums), 3, 1)
+        b.run(sums)
+        torch.cuda.synchronize()
+    fast = sums.cpu().numpy()
+    with Batch(x, y, offs, lens, EXACT) as b:  # exact policy: the oracle's f32 terms
+        b.run(sums)
+        torch.cuda.synchronize()
+    exact_pol = sums.cpu().numpy()
+    pick = np.random.default_rng(9).choice(count, 32, replace=False)
+    e_sum, e_tot = [], []
+    for i in pick:
+        o, n = int(offs[i]), int(lens[i])
+        xs, ys = x[o:o + n].cpu().numpy(), y[o:o + n].cpu().numpy()
+        exact = O.expr4_sum_fp64(xs, ys)  # exact sum of the oracle's f32 terms
+        xd, yd = xs.astype(np.float64), ys.astype(np.float64)
+        fp64 = float(np.sum(np.exp(-xd * xd / 2) * np.cos(yd) + np.sqrt(np.abs(xd))))
+        e_sum.append(abs(exact_pol[i] - exact) / abs(exact))
+        e_tot.append(abs(fast[i] - fp64) / abs(fp64))
     total = int(lens.sum())
     del x, y
     torch.cuda.empty_cache()
     return {"tables": count, "elements": total, "persistent_ms": ms, "plain_ms": plain_ms,
             "speedup_vs_plain": plain_ms / ms, "elements_per_s": total / (ms * 1e-3),
-            "config": {"block": cfg[0], "elements_per_thread": cfg[1], "wg_per_cu": cfg[2]}}
+            "config": {"block": cfg[0], "elements_per_thread": cfg[1], "wg_per_cu": cfg[2]},
+            "summation_error_exact_policy": {"max_rel": float(max(e_sum)), "mean_rel": float(np.mean(e_sum)),
+                                             "vs": "exact sum of the oracle's f32 terms"},
+            "total_error_fast_policy": {"max_rel": float(max(e_tot)), "mean_rel": float(np.mean(e_tot)),
+                                        "vs": "FP64 evaluation of the expression on the f32 inputs"},
+            "error_sample": "32 seeded tables"}
 
 
 def leg_served(torch, ops):
     """Served Black-Scholes: 10,000 requests with sizes log-uniform in
     [2^10, 2^22] options, 80% f32 / 20% f64, windows of resident HBM tables,
     64 GB admission budget, accurate policy for f32 (exact for f64).
-    Capacity: every request present at t=0.  Latency: Poisson arrivals at 0.7x
-    the served capacity.  Plain baseline: one launch + one synchronisation per
+    Capacity: every request present at t=0.  Latency: Poisson arrivals at 0.5x
+    and 0.7x the served capacity.  Plain baseline: one launch + one synchronisation per
     request in arrival order on one stream (same kernels' standalone forms)."""
     from nt2_amd import ACCURATE, EXACT
     from nt2_amd.native import lib
@@ -248,19 +274,21 @@ def leg_served(torch, ops):
     served_run(zero[:])  # warm: pool growth, code paths
     served = served_run(zero)
     plain = plain_run(zero)
-    rate = 0.7 * count / served["span_s"]
-    arrivals = list(np.cumsum(rng.exponential(1.0 / rate, count)))
-    served_l = served_run(arrivals)
-    plain_l = plain_run(arrivals)
     keep = ("p50_us", "p99_us", "mean_us", "max_us", "options_per_s", "launches", "mean_batch", "span_s")
     pick = lambda d: {k: d[k] for k in keep if k in d}
+    loads = {}
+    for frac in (0.5, 0.7):
+        rate = frac * count / served["span_s"]
+        arrivals = list(np.cumsum(rng.exponential(1.0 / rate, count)))
+        served_l = served_run(arrivals)
+        plain_l = plain_run(arrivals)
+        loads["%.1f" % frac] = {"requests_per_s": rate, "served": pick(served_l), "plain": pick(plain_l),
+                                "p99_plain_over_served": plain_l["p99_us"] / served_l["p99_us"]}
     return {"requests": count, "options": options, "f64_fraction": float(f64.mean()),
             "sizes": "log-uniform [2^10, 2^22]", "policy": "accurate (f32), exact (f64)",
             "capacity": {"served": pick(served), "plain": pick(plain),
                          "served_vs_plain": served["options_per_s"] / plain["options_per_s"]},
-            "offered_load": {"requests_per_s": rate, "fraction_of_served_capacity": 0.7,
-                             "served": pick(served_l), "plain": pick(plain_l),
-                             "p99_plain_over_served": plain_l["p99_us"] / served_l["p99_us"]}}
+            "offered_load": {"fraction_of_served_capacity": loads}}
 
 
 def main():
@@ -318,7 +346,7 @@ def main():
     if not args.no_extra:
         for name, fn in (("saxpy_sdot_2p20", lambda: leg_saxpy_sdot(torch, ops, O)),
                          ("sigmadelta_64x1920x1080", lambda: leg_sigmadelta(torch, ops)),
-                         ("batched_4096_tables", lambda: leg_batch(torch)),
+                         ("batched_4096_tables", lambda: leg_batch(torch, O)),
                          ("served_blackscholes", lambda: leg_served(torch, ops))):
             t0 = time.perf_counter()
             try:

@@ -691,6 +691,41 @@ inline unsigned grid(size_t items) {
 
 }}  // namespace hip::fe
 
+// ---- lowering entry points -------------------------------------------------------
+// What an NT2 site override calls: transform_(out, in, range) over the whole
+// table (range = {0, numel}) and inner_fold_ of a whole expression.  Each
+// expression type instantiates its own fused kernel; the terminals' device
+// copies must be current (expr.prepare()).
+namespace hip {
+template <class T, class EX>
+void transform(T* out, const EX& x, size_t n, hipStream_t s = nullptr) {
+  if (n == 0) return;
+  constexpr int W = fe::pack_width<T>::value;
+  auto v = expr::dview<EX>::get(x);
+  hipLaunchKernelGGL((fe::transform_k<T, decltype(v)>), dim3(fe::grid(n / W + n % W)), dim3(256), 0, s,
+                     out, v, n);
+  fe::check(hipGetLastError(), "transform launch");
+}
+
+// FP64 value of the compensated fold of x's n elements (synchronises s)
+template <class T, class EX>
+double fold(const EX& x, size_t n, hipStream_t s = nullptr) {
+  constexpr int W = fe::pack_width<T>::value;
+  const size_t items = n / W + n % W;
+  const unsigned nb = (unsigned)std::min<size_t>(1024, (items + 255) / 256 ? (items + 255) / 256 : 1);
+  double* buf = nullptr;
+  fe::check(hipMalloc(&buf, sizeof(double) * (nb + 1)), "fold alloc");
+  auto v = expr::dview<EX>::get(x);
+  hipLaunchKernelGGL((fe::fold_k<T, decltype(v)>), dim3(nb), dim3(256), 0, s, v, n, buf);
+  hipLaunchKernelGGL(fe::fold_final_k, dim3(1), dim3(64), 0, s, buf, (int)nb, buf + nb);
+  double r = 0.0;
+  fe::check(hipMemcpyAsync(&r, buf + nb, sizeof(double), hipMemcpyDeviceToHost, s), "fold result");
+  fe::check(hipStreamSynchronize(s), "fold sync");
+  fe::check(hipFree(buf), "fold free");
+  return r;
+}
+}  // namespace hip
+
 template <class T>
 template <class E>
 void table<T>::assign(const E& e) {
@@ -702,13 +737,7 @@ void table<T>::assign(const E& e) {
   x.prepare();  // terminals' device copies are current (this table included, for in-place)
   resize(s);
   T* out = device_data_for_write();
-  const size_t n = numel();
-  if (n == 0) return;
-  constexpr int W = hip::fe::pack_width<T>::value;
-  auto v = expr::dview<EX>::get(x);
-  hipLaunchKernelGGL((hip::fe::transform_k<T, decltype(v)>), dim3(hip::fe::grid(n / W + n % W)),
-                     dim3(256), 0, 0, out, v, n);
-  hip::fe::check(hipGetLastError(), "transform launch");
+  hip::transform(out, x, numel());
 }
 
 // globalsum(expr): fold of the whole expression, returned on the host
@@ -720,18 +749,7 @@ typename expr::vt<typename expr::wrap<E>::type>::type globalsum(const E& e) {
   const ext_t ex = x.ext();
   const size_t n = ex.has ? ex.s.numel() : 1;
   x.prepare();
-  constexpr int W = hip::fe::pack_width<T>::value;
-  const size_t items = n / W + n % W;
-  const unsigned nb = (unsigned)std::min<size_t>(1024, (items + 255) / 256 ? (items + 255) / 256 : 1);
-  double* buf = nullptr;
-  hip::fe::check(hipMalloc(&buf, sizeof(double) * (nb + 1)), "fold alloc");
-  auto v = expr::dview<EX>::get(x);
-  hipLaunchKernelGGL((hip::fe::fold_k<T, decltype(v)>), dim3(nb), dim3(256), 0, 0, v, n, buf);
-  hipLaunchKernelGGL(hip::fe::fold_final_k, dim3(1), dim3(64), 0, 0, buf, (int)nb, buf + nb);
-  double r = 0.0;
-  hip::fe::check(hipMemcpy(&r, buf + nb, sizeof(double), hipMemcpyDeviceToHost), "fold result");
-  hip::fe::check(hipFree(buf), "fold free");
-  return (T)r;
+  return (T)hip::fold<T>(x, n);
 }
 // sum of a vector table (NT2's sum along the first non-singleton dimension)
 template <class E, NT2H_ENABLE1(E)>
