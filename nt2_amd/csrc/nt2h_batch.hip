@@ -6,8 +6,10 @@
 // pack, so NT2's pack grouping restarts at each table start); a persistent
 // grid pulls tile indices from a device work queue, reduces each tile to a
 // compensated FP32 partial, and the block that completes a table's last tile
-// sums that table's partials in tile order in FP64 (deterministic).  The reset
-// kernel and the persistent kernel are captured once in a hipGraph and
+// sums that table's partials in tile order in FP64 (deterministic).  Tiles
+// with arguments that need the out-of-line trig reduction are listed and
+// redone by a second kernel, so that the persistent one holds no calls.  The
+// reset, persistent and slow-tile kernels are captured once in a hipGraph and
 // replayed.  An autotuner picks block size, elements per thread and
 // workgroups per CU by timing the candidates.
 //
@@ -42,11 +44,11 @@ struct TableDesc {
   uint32_t pad;
 };
 
-template <class P, int G>
-__device__ __forceinline__ void expr4_g(const float* x, const float* y, float* z) {
+template <class P, int G, bool BAIL = false>
+__device__ __forceinline__ void expr4_g(const float* x, const float* y, float* z, bool* bail = nullptr) {
   _Pragma("clang fp contract(off)")
   float c[G];
-  dm::trig_group_f<P, G>(dm::k_cos, y, c, nullptr);
+  dm::trig_group_f<P, G, BAIL>(dm::k_cos, y, c, nullptr, bail);
 #pragma unroll
   for (int l = 0; l < G; ++l) {
     const float t = ((-x[l]) * x[l]) / 2.0f;
@@ -54,22 +56,92 @@ __device__ __forceinline__ void expr4_g(const float* x, const float* y, float* z
   }
 }
 
-__global__ void batch_reset_k(unsigned* queue, unsigned* done, int ntables) {
+__global__ void batch_reset_k(unsigned* state, unsigned* done, int ntables) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) *queue = 0u;
+  if (i < 2) state[i] = 0u;
   if (i < ntables) done[i] = 0u;
 }
 
+// one thread's compensated sum over its elements of a tile: 8-lane packs,
+// then the table's last n%8 elements through NT2's scalar path
+template <class P, int BLOCK, bool BAIL>
+__device__ __forceinline__ void tile_sum(const TableDesc& d, uint64_t start, uint64_t full_end, uint64_t end,
+                                         float& s, float& c, bool* bail) {
+  for (uint64_t i = start + 8ull * threadIdx.x; i < full_end; i += 8ull * BLOCK) {
+    float a[8], b[8], z[8];
+    if (d.vec) {
+      const float4 a0 = *reinterpret_cast<const float4*>(d.x + i);
+      const float4 a1 = *reinterpret_cast<const float4*>(d.x + i + 4);
+      const float4 b0 = *reinterpret_cast<const float4*>(d.y + i);
+      const float4 b1 = *reinterpret_cast<const float4*>(d.y + i + 4);
+      a[0] = a0.x; a[1] = a0.y; a[2] = a0.z; a[3] = a0.w;
+      a[4] = a1.x; a[5] = a1.y; a[6] = a1.z; a[7] = a1.w;
+      b[0] = b0.x; b[1] = b0.y; b[2] = b0.z; b[3] = b0.w;
+      b[4] = b1.x; b[5] = b1.y; b[6] = b1.z; b[7] = b1.w;
+    } else {
+#pragma unroll
+      for (int l = 0; l < 8; ++l) { a[l] = d.x[i + l]; b[l] = d.y[i + l]; }
+    }
+    expr4_g<P, 8, BAIL>(a, b, z, bail);
+#pragma unroll
+    for (int l = 0; l < 8; ++l) comp_add(s, c, z[l]);
+  }
+  for (uint64_t i = full_end + threadIdx.x; i < end; i += BLOCK) {
+    float a = d.x[i], b = d.y[i], z;
+    expr4_g<P, 1, BAIL>(&a, &b, &z, bail);
+    comp_add(s, c, z);
+  }
+}
+
+// the tile's FP64 partial; the block that brings a table's done count to its
+// tile count sums that table's partials in tile order
+template <int BLOCK>
+__device__ __forceinline__ void finish_tile(float s, float c, unsigned t, uint32_t ti, uint32_t ntiles_of_table,
+                                            uint32_t first_tile, unsigned* done, double* __restrict__ tile_partial,
+                                            double* __restrict__ sums, double* wsum, bool* last) {
+  // block reduction (error-free wave merge, FP64 across waves in order)
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    const float s2 = __shfl_xor(s, off, 64);
+    const float c2 = __shfl_xor(c, off, 64);
+    comp_merge(s, c, s2, c2);
+  }
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = (double)s + (double)c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double v = 0.0;
+#pragma unroll
+    for (int w = 0; w < BLOCK / 64; ++w) v += wsum[w];
+    tile_partial[t] = v;
+    __threadfence();
+    const unsigned prev = atomicAdd(&done[ti], 1u);
+    *last = (prev + 1 == ntiles_of_table);
+  }
+  __syncthreads();
+  if (*last && threadIdx.x == 0) {
+    __threadfence();
+    double v = 0.0;
+    for (uint32_t k = 0; k < ntiles_of_table; ++k)
+      v += __builtin_nontemporal_load(&tile_partial[first_tile + k]);
+    sums[ti] = v;
+  }
+  __syncthreads();
+}
+
+// state[0]: the work queue head, state[1]: the number of tiles handed to the
+// slow kernel.  The persistent kernel holds no calls: a tile with an argument
+// that needs the out-of-line fdlibm reduction (beyond 2^18*pi) is listed in
+// `slow` and left to batch_slow_k, which runs after it.
 template <class P, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void batch_persistent_k(
     const TableDesc* __restrict__ tables, const uint32_t* __restrict__ tile_table,
-    uint32_t ntiles, uint32_t tile, unsigned* queue, unsigned* done,
+    uint32_t ntiles, uint32_t tile, unsigned* state, unsigned* done, uint32_t* __restrict__ slow,
     double* __restrict__ tile_partial, double* __restrict__ sums) {
   __shared__ unsigned cur;
   __shared__ double wsum[BLOCK / 64];
   __shared__ bool last;
   for (;;) {
-    if (threadIdx.x == 0) cur = atomicAdd(queue, 1u);
+    if (threadIdx.x == 0) cur = atomicAdd(&state[0], 1u);
     __syncthreads();
     const unsigned t = cur;
     __syncthreads();
@@ -80,58 +152,40 @@ __global__ __launch_bounds__(BLOCK) void batch_persistent_k(
     const uint64_t end = (start + tile < d.len) ? start + tile : d.len;
     const uint64_t full_end = start + ((end - start) / 8) * 8;
     float s = 0.0f, c = 0.0f;
-    for (uint64_t i = start + 8ull * threadIdx.x; i < full_end; i += 8ull * BLOCK) {
-      float a[8], b[8], z[8];
-      if (d.vec) {
-        const float4 a0 = *reinterpret_cast<const float4*>(d.x + i);
-        const float4 a1 = *reinterpret_cast<const float4*>(d.x + i + 4);
-        const float4 b0 = *reinterpret_cast<const float4*>(d.y + i);
-        const float4 b1 = *reinterpret_cast<const float4*>(d.y + i + 4);
-        a[0] = a0.x; a[1] = a0.y; a[2] = a0.z; a[3] = a0.w;
-        a[4] = a1.x; a[5] = a1.y; a[6] = a1.z; a[7] = a1.w;
-        b[0] = b0.x; b[1] = b0.y; b[2] = b0.z; b[3] = b0.w;
-        b[4] = b1.x; b[5] = b1.y; b[6] = b1.z; b[7] = b1.w;
-      } else {
-#pragma unroll
-        for (int l = 0; l < 8; ++l) { a[l] = d.x[i + l]; b[l] = d.y[i + l]; }
-      }
-      expr4_g<P, 8>(a, b, z);
-#pragma unroll
-      for (int l = 0; l < 8; ++l) comp_add(s, c, z[l]);
+    bool bail = false;
+    tile_sum<P, BLOCK, true>(d, start, full_end, end, s, c, &bail);
+    if (__syncthreads_or(bail)) {  // uniform: the whole tile goes to the slow kernel
+      if (threadIdx.x == 0) slow[atomicAdd(&state[1], 1u)] = t;
+      // A barrier between this thread-0 block and the one at the loop head:
+      // without it the optimizer threads the two `threadIdx.x == 0` tests
+      // into one path and makes the loop head's barrier an inner loop that
+      // lanes 1..63 of wave 0 re-enter while lane 0 waits outside it.
+      __syncthreads();
+      continue;
     }
-    // the table's last n%8 elements use NT2's scalar path
-    for (uint64_t i = full_end + threadIdx.x; i < end; i += BLOCK) {
-      float a = d.x[i], b = d.y[i], z;
-      expr4_g<P, 1>(&a, &b, &z);
-      comp_add(s, c, z);
-    }
-    // block reduction (error-free wave merge, FP64 across waves in order)
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-      const float s2 = __shfl_xor(s, off, 64);
-      const float c2 = __shfl_xor(c, off, 64);
-      comp_merge(s, c, s2, c2);
-    }
-    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = (double)s + (double)c;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double v = 0.0;
-#pragma unroll
-      for (int w = 0; w < BLOCK / 64; ++w) v += wsum[w];
-      tile_partial[t] = v;
-      __threadfence();
-      const unsigned prev = atomicAdd(&done[ti], 1u);
-      last = (prev + 1 == d.ntiles);
-    }
-    __syncthreads();
-    if (last && threadIdx.x == 0) {
-      __threadfence();
-      double v = 0.0;
-      for (uint32_t k = 0; k < d.ntiles; ++k)
-        v += __builtin_nontemporal_load(&tile_partial[d.first_tile + k]);
-      sums[ti] = v;
-    }
-    __syncthreads();
+    finish_tile<BLOCK>(s, c, t, ti, d.ntiles, d.first_tile, done, tile_partial, sums, wsum, &last);
+  }
+}
+
+// the tiles listed by batch_persistent_k, with the out-of-line reduction
+template <class P, int BLOCK>
+__global__ __launch_bounds__(BLOCK) void batch_slow_k(
+    const TableDesc* __restrict__ tables, const uint32_t* __restrict__ tile_table,
+    uint32_t tile, const unsigned* state, unsigned* done, const uint32_t* __restrict__ slow,
+    double* __restrict__ tile_partial, double* __restrict__ sums) {
+  __shared__ double wsum[BLOCK / 64];
+  __shared__ bool last;
+  const unsigned nslow = state[1];
+  for (unsigned u = blockIdx.x; u < nslow; u += gridDim.x) {  // uniform across the block
+    const unsigned t = slow[u];
+    const uint32_t ti = tile_table[t];
+    const TableDesc d = tables[ti];
+    const uint64_t start = (uint64_t)(t - d.first_tile) * tile;
+    const uint64_t end = (start + tile < d.len) ? start + tile : d.len;
+    const uint64_t full_end = start + ((end - start) / 8) * 8;
+    float s = 0.0f, c = 0.0f;
+    tile_sum<P, BLOCK, false>(d, start, full_end, end, s, c, nullptr);
+    finish_tile<BLOCK>(s, c, t, ti, d.ntiles, d.first_tile, done, tile_partial, sums, wsum, &last);
   }
 }
 
@@ -150,7 +204,8 @@ struct Batch {
   TableDesc* tables = nullptr;
   uint32_t* tile_table = nullptr;
   double* tile_partial = nullptr;
-  unsigned* queue = nullptr;
+  uint32_t* slow = nullptr;     // tiles left to the slow kernel
+  unsigned* state = nullptr;    // queue head, slow tile count
   unsigned* done = nullptr;
   uint32_t ntiles = 0;
   size_t tile_cap = 0;
@@ -178,13 +233,15 @@ static int layout(Batch* b) {
   b->ntiles = (uint32_t)tt.size();
   if (!b->tables) NT2H_CHECK(hipMalloc(&b->tables, sizeof(TableDesc) * b->count), "batch alloc");
   if (!b->done) NT2H_CHECK(hipMalloc(&b->done, sizeof(unsigned) * b->count), "batch alloc");
-  if (!b->queue) NT2H_CHECK(hipMalloc(&b->queue, sizeof(unsigned)), "batch alloc");
+  if (!b->state) NT2H_CHECK(hipMalloc(&b->state, 2 * sizeof(unsigned)), "batch alloc");
   if (b->tile_cap < tt.size()) {
     if (b->tile_table) { NT2H_CHECK(hipFree(b->tile_table), "batch free"); b->tile_table = nullptr; }
     if (b->tile_partial) { NT2H_CHECK(hipFree(b->tile_partial), "batch free"); b->tile_partial = nullptr; }
+    if (b->slow) { NT2H_CHECK(hipFree(b->slow), "batch free"); b->slow = nullptr; }
     const size_t cap = std::max<size_t>(tt.size(), 1);
     NT2H_CHECK(hipMalloc(&b->tile_table, sizeof(uint32_t) * cap), "batch alloc");
     NT2H_CHECK(hipMalloc(&b->tile_partial, sizeof(double) * cap), "batch alloc");
+    NT2H_CHECK(hipMalloc(&b->slow, sizeof(uint32_t) * cap), "batch alloc");
     b->tile_cap = cap;
   }
   NT2H_CHECK(hipMemcpy(b->tables, td.data(), sizeof(TableDesc) * b->count, hipMemcpyHostToDevice), "batch upload");
@@ -196,14 +253,17 @@ static int layout(Batch* b) {
 template <class P, int BLOCK>
 static void launch_persistent(Batch* b, double* sums, hipStream_t s) {
   const unsigned grid = (unsigned)std::max(1, std::min<int>(b->cus * b->cfg.wg_per_cu, (int)b->ntiles));
+  const uint32_t tile = (uint32_t)(BLOCK * b->cfg.ept);
   hipLaunchKernelGGL((batch_persistent_k<P, BLOCK>), dim3(grid), dim3(BLOCK), 0, s, b->tables,
-                     b->tile_table, b->ntiles, (uint32_t)(BLOCK * b->cfg.ept), b->queue, b->done,
-                     b->tile_partial, sums);
+                     b->tile_table, b->ntiles, tile, b->state, b->done, b->slow, b->tile_partial, sums);
+  const unsigned sgrid = (unsigned)std::max(1, std::min<int>(b->cus, (int)b->ntiles));
+  hipLaunchKernelGGL((batch_slow_k<P, BLOCK>), dim3(sgrid), dim3(BLOCK), 0, s, b->tables, b->tile_table,
+                     tile, b->state, b->done, b->slow, b->tile_partial, sums);
 }
 
 static int enqueue(Batch* b, double* sums, hipStream_t s) {
   const int rb = 256;
-  hipLaunchKernelGGL(batch_reset_k, dim3((b->count + rb - 1) / rb), dim3(rb), 0, s, b->queue, b->done, b->count);
+  hipLaunchKernelGGL(batch_reset_k, dim3((b->count + rb - 1) / rb), dim3(rb), 0, s, b->state, b->done, b->count);
   const bool fast = b->policy == NT2H_FAST;
   switch (b->cfg.block) {
     case 128: fast ? launch_persistent<fast_policy, 128>(b, sums, s) : launch_persistent<exact_policy, 128>(b, sums, s); break;
@@ -361,7 +421,8 @@ int nt2h_batch_destroy(nt2h_batch* hb) {
   (void)hipFree(b->tables);
   (void)hipFree(b->tile_table);
   (void)hipFree(b->tile_partial);
-  (void)hipFree(b->queue);
+  (void)hipFree(b->slow);
+  (void)hipFree(b->state);
   (void)hipFree(b->done);
   delete b;
   return NT2H_OK;

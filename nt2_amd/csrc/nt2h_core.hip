@@ -611,15 +611,24 @@ __global__ __launch_bounds__(kBlock) void sigmadelta16_k(const uint8_t* __restri
   }
 }
 
-// counts[k0 + i] += sum over blocks of partials[i][block], blocks in order
+// counts[k0 + i] += sum over blocks of partials[i][block]: one workgroup per
+// frame (integer sums, so the result does not depend on the order)
 __global__ __launch_bounds__(kBlock) void sd_counts_k(const uint32_t* __restrict__ partials,
-                                                      int nblocks, int nk,
-                                                      uint32_t* __restrict__ counts) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nk) return;
+                                                      int nblocks, uint32_t* __restrict__ counts) {
+  __shared__ uint32_t ws[kBlock / 64];
+  const uint32_t* row = partials + (size_t)blockIdx.x * nblocks;
   uint32_t c = 0;
-  for (int b = 0; b < nblocks; ++b) c += partials[(size_t)i * nblocks + b];
-  counts[i] += c;
+  for (int b = threadIdx.x; b < nblocks; b += kBlock) c += row[b];
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) c += __shfl_xor(c, off, 64);
+  if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / 64; ++w) t += ws[w];
+    counts[blockIdx.x] += t;
+  }
 }
 
 __global__ __launch_bounds__(kBlock) void zero_u32_k(uint32_t* __restrict__ p, int n) {
@@ -1064,8 +1073,8 @@ int nt2h_sigmadelta_u8(size_t npix, int nframes, const uint8_t* frames, size_t f
                            bkg, var, label, N, part);
         NT2H_CHECK_LAUNCH("sigmadelta16");
         if (part) {
-          hipLaunchKernelGGL(sd_counts_k, dim3((k1 - k0 + kBlock - 1) / kBlock), dim3(kBlock), 0, s, part,
-                             (int)grid, k1 - k0, frame_counts + k0);
+          hipLaunchKernelGGL(sd_counts_k, dim3(k1 - k0), dim3(kBlock), 0, s, part, (int)grid,
+                             frame_counts + k0);
           NT2H_CHECK_LAUNCH("sigmadelta counts");
         }
       }

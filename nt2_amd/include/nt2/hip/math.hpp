@@ -564,12 +564,25 @@ __device__ __noinline__ int rem_pio2d_fdlibm(double x, double* xr) {
 NT2H_FN bool ngt(float a, float b) { return !(a > b); }
 NT2H_FN bool ngt(double a, double b) { return !(a > b); }
 
+// The per-lane fdlibm reductions of a whole group behind ONE out-of-line call:
+// the path is rare (arguments beyond 2^18*pi), and a single call site keeps
+// the call's register saves and stack traffic out of the hot loops that
+// inline the reduction cascade.
+__device__ __noinline__ void huge_d(const double* x, double* xr, int* n, int g) {
+  for (int i = 0; i < g; ++i) n[i] = rem_pio2d_fdlibm(x[i], &xr[i]);
+}
+__device__ __noinline__ void huge_f(const float* x, float* xr, int* n, int g) {
+  for (int i = 0; i < g; ++i) n[i] = rem_pio2f_fdlibm(x[i], &xr[i]);
+}
+
 // f64 reduction cascade over a group of G lanes (G = 4: one NT2 AVX pack;
 // G = 1: the scalar tail): scalar/impl/trigo/trig_reduction.hpp:94-215;
 // straight rem_pio2_straight.hpp:58-63, cephes rem_pio2_cephes.hpp:65-72,
 // medium rem_pio2_medium.hpp:68-83; beyond Medium_pi (2^18*pi) fdlibm per lane.
-template <int G>
-NT2H_FN void reduce_d(const double* x, double* xr, int* n) {
+// BAIL: instead of the out-of-line fdlibm path, set *bail and return zeros
+// (the caller redoes its work with BAIL false); keeps every call out of hot loops.
+template <int G, bool BAIL = false>
+NT2H_FN void reduce_d(const double* x, double* xr, int* n, bool* bail = nullptr) {
   NT2H_STRICT
   bool all4 = true, all2 = true, all20 = true, allm = true;
 #pragma unroll
@@ -617,8 +630,12 @@ NT2H_FN void reduce_d(const double* x, double* xr, int* n) {
       xr[i] = r - w;
       n[i] = ((int)(int64_t)fn) & 3;
     }
+  } else if constexpr (BAIL) {
+    *bail = true;
+#pragma unroll
+    for (int i = 0; i < G; ++i) { xr[i] = 0.0; n[i] = 0; }
   } else {
-    for (int i = 0; i < G; ++i) n[i] = rem_pio2d_fdlibm(x[i], &xr[i]);
+    huge_d(x, xr, n, G);
   }
 }
 
@@ -626,8 +643,8 @@ NT2H_FN void reduce_d(const double* x, double* xr, int* n) {
 // tail): trig_reduction.hpp:94-215; when every lane is within 2^18*pi the
 // pack is split into halves of 4 doubles that each run the f64 cascade
 // (use_conversion, :190-215); beyond that, fdlibm single per lane.
-template <int G>
-NT2H_FN void reduce_f(const float* x, float* xr, int* n) {
+template <int G, bool BAIL = false>
+NT2H_FN void reduce_f(const float* x, float* xr, int* n, bool* bail = nullptr) {
   NT2H_STRICT
   bool all4 = true, all2 = true, all20 = true, allm = true, alld = true;
 #pragma unroll
@@ -684,12 +701,16 @@ NT2H_FN void reduce_f(const float* x, float* xr, int* n) {
       int un[H];
 #pragma unroll
       for (int i = 0; i < H; ++i) ux[i] = (double)x[h + i];
-      reduce_d<H>(ux, uxr, un);
+      reduce_d<H, BAIL>(ux, uxr, un, bail);
 #pragma unroll
       for (int i = 0; i < H; ++i) { xr[h + i] = (float)uxr[i]; n[h + i] = un[i]; }
     }
+  } else if constexpr (BAIL) {
+    *bail = true;
+#pragma unroll
+    for (int i = 0; i < G; ++i) { xr[i] = 0.0f; n[i] = 0; }
   } else {
-    for (int i = 0; i < G; ++i) n[i] = rem_pio2f_fdlibm(x[i], &xr[i]);
+    huge_f(x, xr, n, G);
   }
 }
 
@@ -734,14 +755,14 @@ enum trig_kind { k_cos = 0, k_sin = 1, k_sincos = 2 };
 // simd/common/impl/trigo/trig_base.hpp:59-71 (cos), :80-91 (sin), :140-156
 // (sincos); scalar trig_base scalar/impl/trigo/trig_base.hpp:126-207 (G == 1:
 // NaN for Inf/NaN inputs first).  For sincos, out0 = sin and out1 = cos.
-template <class P, int G>
-NT2H_FN void trig_group_f(int kind, const float* a0, float* out0, float* out1) {
+template <class P, int G, bool BAIL = false>
+NT2H_FN void trig_group_f(int kind, const float* a0, float* out0, float* out1, bool* bail = nullptr) {
   NT2H_STRICT
   float x[G], xr[G];
   int n[G];
 #pragma unroll
   for (int i = 0; i < G; ++i) x[i] = __builtin_fabsf(a0[i]);
-  reduce_f<G>(x, xr, n);
+  reduce_f<G, BAIL>(x, xr, n, bail);
 #pragma unroll
   for (int i = 0; i < G; ++i) {
     const uint32_t swap = (uint32_t)n[i] & 1u;

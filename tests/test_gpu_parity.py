@@ -298,10 +298,13 @@ def test_expr4_exact_bitwise(n):
     assert abs(s - ex) <= 2 ** -40 * O.sum_f32_exact(np.abs(want)) + 1e-30
 
 
-def test_batch_persistent_matches_plain_and_oracle():
+@pytest.mark.parametrize("huge", [False, True])
+def test_batch_persistent_matches_plain_and_oracle(huge):
     """Config-4 shape at reduced count: uneven tables in one persistent
     hipGraph launch; every per-table sum equals the plain one-launch-per-table
-    result up to summation order and the oracle's z within 2^-40."""
+    result up to summation order and the oracle's z within 2^-40.  With
+    `huge`, every fourth table holds cos arguments up to 1e7 (beyond 2^18*pi),
+    so that its tiles take the slow-tile kernel."""
     import ctypes
     from nt2_amd.native import check, lib, ptr, stream_handle
     rng = np.random.default_rng(4)
@@ -309,8 +312,9 @@ def test_batch_persistent_matches_plain_and_oracle():
     lens = np.floor(np.exp(rng.uniform(np.log(1e3), np.log(1e5), count))).astype(np.int64)
     lens[0], lens[1] = 1, 7  # ragged: pure scalar tails
     xs = [ops.fill_normal(int(n), 1000 + i) for i, n in enumerate(lens)]
-    ys = [ops.fill_uniform(int(n), -1e4, 1e4, 2000 + i) for i, n in enumerate(lens)]
-    say("batch count=%d total=%d" % (count, int(lens.sum())))
+    ymax = [1e7 if huge and i % 4 == 1 else 1e4 for i in range(count)]
+    ys = [ops.fill_uniform(int(n), -ymax[i], ymax[i], 2000 + i) for i, n in enumerate(lens)]
+    say("batch count=%d total=%d huge=%s" % (count, int(lens.sum()), huge))
     XP = (ctypes.c_void_p * count)(*[t.data_ptr() for t in xs])
     YP = (ctypes.c_void_p * count)(*[t.data_ptr() for t in ys])
     LN = (ctypes.c_size_t * count)(*[int(n) for n in lens])
