@@ -14,6 +14,9 @@
 * exp(r) on |r| <= ln2/64 (the reduced argument of exp(w) = 2^(k/32) e^r):
   degree-3 fit for relative error.
 * 2^(j/32), j = 0..31, correctly rounded.
+* exp(w) on |w| <= SMALL_W (the discount factor exp(-rT) for usual rates and
+  maturities): degree-SMALL_DEG fit for relative error, so that the common
+  case needs no table lookup and no scaling.
 
 Usage: python3 gen_bs_coeffs.py <out.hpp>
 """
@@ -29,6 +32,8 @@ DEG = 4
 DMAX = 16.97                      # |d| clamp (= 12 sqrt2 - 0.0005): N(-16.97) < 1e-63
 C = 2.5 * float(mpmath.sqrt(2))   # y = C / (C + |d|)
 EXP_DEG = 3
+SMALL_W = 0.25
+SMALL_DEG = 7
 
 
 def d_of_x(x, A, B):
@@ -68,10 +73,18 @@ def main():
     et = np.array([float(mpmath.exp(mpmath.mpf(x))) for x in xt])
     eworst = float(np.max(np.abs(np.polynomial.polynomial.polyval(xt, ce) / et - 1)))
     exp2 = [float(mpmath.mpf(2) ** (mpmath.mpf(j) / 32)) for j in range(32)]
+    xs = SMALL_W * np.cos(np.pi * (np.arange(200) + 0.5) / 200)
+    sv = np.array([float(mpmath.exp(mpmath.mpf(x))) for x in xs])
+    V = np.vander(xs, SMALL_DEG + 1, increasing=True) / sv[:, None]
+    cs, *_ = np.linalg.lstsq(V, np.ones_like(xs), rcond=None)
+    xt = np.linspace(-SMALL_W, SMALL_W, 4001)
+    st = np.array([float(mpmath.exp(mpmath.mpf(x))) for x in xt])
+    sworst = float(np.max(np.abs(np.polynomial.polynomial.polyval(xt, cs) / st - 1)))
     with open(out, "w") as f:
         f.write("// Generated by nt2_amd/csrc/gen_bs_coeffs.py. Do not edit.\n")
         f.write("// erfcx(|d|/sqrt2)/2 on |d| <= %g: max relative error %.3g;\n" % (DMAX, worst))
-        f.write("// exp(r) on |r| <= ln2/64: max relative error %.3g\n" % eworst)
+        f.write("// exp(r) on |r| <= ln2/64: max relative error %.3g;\n" % eworst)
+        f.write("// exp(w) on |w| <= %g: max relative error %.3g\n" % (SMALL_W, sworst))
         f.write("#pragma once\nnamespace nt2 { namespace hip { namespace bsacc {\n")
         f.write("constexpr int kTailN = %d;      // intervals\n" % NINT)
         f.write("constexpr int kTailDeg = %d;    // polynomial degree\n" % DEG)
@@ -91,8 +104,12 @@ def main():
         f.write("constexpr double kExp2j[32] = {\n")
         for v in exp2:
             f.write("    %r,\n" % v)
-        f.write("};\n}}}  // namespace nt2::hip::bsacc\n")
-    print("tail max rel err %.3g, exp max rel err %.3g" % (worst, eworst))
+        f.write("};\n")
+        f.write("// exp(w) ~ sum_i kExpS[i] w^i for |w| <= kExpSmall\n")
+        f.write("constexpr double kExpSmall = %r;\n" % SMALL_W)
+        f.write("constexpr double kExpS[%d] = {%s};\n" % (SMALL_DEG + 1, ", ".join(repr(float(v)) for v in cs)))
+        f.write("}}}  // namespace nt2::hip::bsacc\n")
+    print("tail max rel err %.3g, exp max rel err %.3g, small exp max rel err %.3g" % (worst, eworst, sworst))
 
 
 if __name__ == "__main__":

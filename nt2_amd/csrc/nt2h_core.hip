@@ -388,8 +388,7 @@ __global__ __launch_bounds__(kBlock) void bs_acc_f32_k(I n4, size_t n, const flo
     const f32x4 x4 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(X) + t);
     const f32x4 t4 = __builtin_nontemporal_load(reinterpret_cast<const f32x4*>(T) + t);
     float c[4], p[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) ba::bs_accurate(s4[k], x4[k], t4[k], prm, tab, ex2, c[k], p[k]);
+    ba::bs_accurate4(s4, x4, t4, prm, tab, ex2, c, p);
     __builtin_nontemporal_store(f32x4{c[0], c[1], c[2], c[3]}, reinterpret_cast<f32x4*>(call) + t);
     __builtin_nontemporal_store(f32x4{p[0], p[1], p[2], p[3]}, reinterpret_cast<f32x4*>(put) + t);
   }

@@ -75,20 +75,24 @@ constexpr uint32_t kSrvGridMax = 256 * 32;  // workgroups of a batch launch (32 
 
 typedef float srv_f32x4 __attribute__((ext_vector_type(4)));
 
-// One launch prices every request of a batch.  The batch's options are cut in
-// tiles of kSrvTile (a request's tiles are consecutive, starting at tile0);
+// One launch prices every request of one precision in a batch (a batch is
+// launched as its f32 group then its f64 group, so that each kernel holds the
+// registers of its own evaluation only).  The group's options are cut in tiles
+// of kSrvTile (a request's tiles are consecutive, starting at tile0);
 // workgroup k walks the contiguous tile range [k tpb, (k+1) tpb), so the
 // owning request is found by one binary search and then advances linearly.
 // P: the NT2 policy of the f64 requests and (ACC false) of the f32 ones;
-// ACC: f32 requests use the accurate FP64-internal evaluation.
-template <class P, bool ACC>
+// ACC: f32 requests use the accurate FP64-internal evaluation; F64: the
+// group's precision.
+template <class P, bool ACC, bool F64>
 __global__ __launch_bounds__(kSrvBlock) void serve_bs_k(const ReqDesc* __restrict__ reqs, int nreq,
                                                       uint32_t ntiles, uint32_t tpb) {
   _Pragma("clang fp contract(off)")
   namespace ba = nt2::hip::bsacc;
-  __shared__ double tab[ACC ? (ba::kTailDeg + 1) * ba::kTailN : 1];
-  __shared__ double ex2[ACC ? 32 : 1];
-  if (ACC) ba::load_tables(tab, ex2);
+  constexpr bool kTables = ACC && !F64;
+  __shared__ double tab[kTables ? (ba::kTailDeg + 1) * ba::kTailN : 1];
+  __shared__ double ex2[kTables ? 32 : 1];
+  if (kTables) ba::load_tables(tab, ex2);
   const uint32_t b0 = blockIdx.x * tpb;
   const uint32_t b1 = min(b0 + tpb, ntiles);
   if (b0 >= b1) return;
@@ -101,7 +105,7 @@ __global__ __launch_bounds__(kSrvBlock) void serve_bs_k(const ReqDesc* __restric
     while (lo + 1 < nreq && reqs[lo + 1].tile0 <= b) ++lo;
     const ReqDesc& d = reqs[lo];
     const uint64_t base = (uint64_t)(b - d.tile0) * kSrvTile;
-    if (!d.f64) {
+    if constexpr (!F64) {
       const float* S = (const float*)d.S;
       const float* X = (const float*)d.X;
       const float* T = (const float*)d.T;
@@ -112,15 +116,14 @@ __global__ __launch_bounds__(kSrvBlock) void serve_bs_k(const ReqDesc* __restric
                          reinterpret_cast<uintptr_t>(T) | reinterpret_cast<uintptr_t>(C) |
                          reinterpret_cast<uintptr_t>(Pt)) & 15) == 0;
       const uint64_t i4 = base + 4 * threadIdx.x;
-      if (ACC) {
+      if constexpr (ACC) {
         const ba::Params prm = ba::make_params(r, v);
         if (vec && i4 + 4 <= d.n) {
           const srv_f32x4 s4 = __builtin_nontemporal_load(reinterpret_cast<const srv_f32x4*>(S + i4));
           const srv_f32x4 x4 = __builtin_nontemporal_load(reinterpret_cast<const srv_f32x4*>(X + i4));
           const srv_f32x4 t4 = __builtin_nontemporal_load(reinterpret_cast<const srv_f32x4*>(T + i4));
           float c[4], p[4];
-#pragma unroll
-          for (int k = 0; k < 4; ++k) ba::bs_accurate(s4[k], x4[k], t4[k], prm, tab, ex2, c[k], p[k]);
+          ba::bs_accurate4(s4, x4, t4, prm, tab, ex2, c, p);
           __builtin_nontemporal_store(srv_f32x4{c[0], c[1], c[2], c[3]}, reinterpret_cast<srv_f32x4*>(C + i4));
           __builtin_nontemporal_store(srv_f32x4{p[0], p[1], p[2], p[3]}, reinterpret_cast<srv_f32x4*>(Pt + i4));
         } else if (vec) {  // the request's last, partial float4: this thread's options only
@@ -217,8 +220,10 @@ struct Slot {
   ReqDesc* h_desc = nullptr;   // pinned staging
   int cap = 0;
   bool busy = false;
+  bool launched = false;       // a batch is recorded on the stream, awaiting completion
   std::vector<uint64_t> reqs;
   std::thread host_worker;
+  std::thread done_thr;        // this slot's completion thread
 };
 
 struct Server {
@@ -228,13 +233,12 @@ struct Server {
   std::deque<uint64_t> pending;            // arrival order
   std::deque<uint64_t> admitted;           // admitted, not yet launched
   std::map<uint64_t, std::unique_ptr<Request>> reqs;
-  std::deque<int> launch_order;            // slots in flight, launch order
   std::vector<Slot> slots;
   size_t used = 0, peak_used = 0;
   uint64_t next_id = 1;
   bool stop = false;
   bool sched_exited = false;
-  std::thread sched, done_thr;
+  std::thread sched;
   int alloc_count = 0;
   hipMemPool_t pool = nullptr;  // the server's own stream-ordered pool: it keeps up to the
                                 // budget cached, so steady-state admissions do not reach the driver
@@ -271,6 +275,15 @@ static void srv_free(Server* s, Request* r) {
   r->blk = nullptr;
   r->d.call = r->d.put = nullptr;
   r->own_out = false;
+}
+
+// one precision group of a batch: nr descriptors (device) covering `tiles` tiles
+template <class P, bool ACC>
+static void launch_group(bool f64, const ReqDesc* dd, int nr, uint32_t tiles, hipStream_t st) {
+  const uint32_t grid = std::min<uint32_t>(tiles, kSrvGridMax);
+  const uint32_t tpb = (tiles + grid - 1) / grid;
+  if (f64) hipLaunchKernelGGL((serve_bs_k<P, ACC, true>), dim3(grid), dim3(kSrvBlock), 0, st, dd, nr, tiles, tpb);
+  else hipLaunchKernelGGL((serve_bs_k<P, ACC, false>), dim3(grid), dim3(kSrvBlock), 0, st, dd, nr, tiles, tpb);
 }
 
 // launch every admitted request (up to max_batch) on a free slot; called with mu held
@@ -336,13 +349,19 @@ static void launch_batch(Server* s, int si, std::unique_lock<std::mutex>& lk) {
     }
     sl.cap = (int)ok.size();
   }
-  uint32_t tiles = 0;
+  // the f32 group first, then the f64 group, each in arrival order; tiles are
+  // numbered within a group
+  std::stable_partition(ok.begin(), ok.end(), [](const Request* r) { return !r->d.f64; });
+  uint32_t tiles[2] = {0u, 0u};
+  int ngroup[2] = {0, 0};
   const int64_t t = now_ns();
   sl.reqs.clear();
   for (size_t k = 0; k < ok.size(); ++k) {
     Request* r = ok[k];
-    r->d.tile0 = tiles;
-    tiles += (uint32_t)((r->d.n + kSrvTile - 1) / kSrvTile);
+    const int g = r->d.f64 ? 1 : 0;
+    r->d.tile0 = tiles[g];
+    tiles[g] += (uint32_t)((r->d.n + kSrvTile - 1) / kSrvTile);
+    ++ngroup[g];
     sl.h_desc[k] = r->d;
     r->state = RS_RUNNING;
     r->t_launch = t;
@@ -361,19 +380,18 @@ static void launch_batch(Server* s, int si, std::unique_lock<std::mutex>& lk) {
   } else {
     lk.unlock();
     (void)hipMemcpyAsync(sl.d_desc, sl.h_desc, sizeof(ReqDesc) * nreq, hipMemcpyHostToDevice, sl.stream);
-    const uint32_t grid = std::min<uint32_t>(tiles, kSrvGridMax);
-    const uint32_t tpb = (tiles + grid - 1) / grid;
-    if (s->cfg.policy == NT2H_EXACT)
-      hipLaunchKernelGGL((serve_bs_k<exact_policy, false>), dim3(grid), dim3(kSrvBlock), 0, sl.stream, sl.d_desc, nreq, tiles, tpb);
-    else if (s->cfg.policy == NT2H_FAST)
-      hipLaunchKernelGGL((serve_bs_k<fast_policy, false>), dim3(grid), dim3(kSrvBlock), 0, sl.stream, sl.d_desc, nreq, tiles, tpb);
-    else
-      hipLaunchKernelGGL((serve_bs_k<exact_policy, true>), dim3(grid), dim3(kSrvBlock), 0, sl.stream, sl.d_desc, nreq, tiles, tpb);
+    for (int g = 0; g < 2; ++g) {
+      if (!ngroup[g]) continue;
+      const ReqDesc* dd = sl.d_desc + (g ? ngroup[0] : 0);
+      if (s->cfg.policy == NT2H_EXACT) launch_group<exact_policy, false>(g == 1, dd, ngroup[g], tiles[g], sl.stream);
+      else if (s->cfg.policy == NT2H_FAST) launch_group<fast_policy, false>(g == 1, dd, ngroup[g], tiles[g], sl.stream);
+      else launch_group<exact_policy, true>(g == 1, dd, ngroup[g], tiles[g], sl.stream);
+    }
     (void)hipEventRecord(sl.event, sl.stream);
     lk.lock();
   }
-  // publish to the completion thread only once the batch's event is recorded
-  s->launch_order.push_back(si);
+  // publish to the slot's completion thread only once the batch's event is recorded
+  sl.launched = true;
   s->cv_done.notify_all();
 }
 
@@ -406,13 +424,14 @@ static void sched_loop(Server* s) {
   s->cv_done.notify_all();
 }
 
-static void done_loop(Server* s) {
+// One completion thread per slot, so that a long batch on one slot never
+// holds back the completion (and the reuse) of a shorter one on another.
+static void done_loop(Server* s, int si) {
+  Slot& sl = s->slots[si];
   std::unique_lock<std::mutex> lk(s->mu);
   for (;;) {
-    while (s->launch_order.empty() && !s->sched_exited) s->cv_done.wait(lk);
-    if (s->launch_order.empty() && s->sched_exited) break;
-    const int si = s->launch_order.front();
-    Slot& sl = s->slots[si];
+    while (!sl.launched && !s->sched_exited) s->cv_done.wait(lk);
+    if (!sl.launched && s->sched_exited) break;
     lk.unlock();
     int status = NT2H_OK;
     if (s->cfg.backend == NT2H_BACKEND_HOST) {
@@ -422,7 +441,7 @@ static void done_loop(Server* s) {
     }
     const int64_t t = now_ns();
     lk.lock();
-    s->launch_order.pop_front();
+    sl.launched = false;
     for (uint64_t id : sl.reqs) {
       Request* r = s->reqs[id].get();
       r->state = status == NT2H_OK ? RS_DONE : RS_FAILED;
@@ -489,7 +508,7 @@ int nt2h_server_create(const nt2h_server_config* cfg, nt2h_server** out) {
     }
   }
   s->sched = std::thread(sched_loop, s);
-  s->done_thr = std::thread(done_loop, s);
+  for (int i = 0; i < (int)s->slots.size(); ++i) s->slots[i].done_thr = std::thread(done_loop, s, i);
   *out = reinterpret_cast<nt2h_server*>(s);
   return NT2H_OK;
 }
@@ -724,8 +743,8 @@ int nt2h_server_destroy(nt2h_server* hs) {
     std::lock_guard<std::mutex> lk(s->mu);
     s->cv_done.notify_all();
   }
-  s->done_thr.join();
   for (Slot& sl : s->slots) {
+    if (sl.done_thr.joinable()) sl.done_thr.join();
     if (sl.host_worker.joinable()) sl.host_worker.join();
   }
   for (auto& kv : s->reqs) srv_free(s, kv.second.get());
@@ -741,8 +760,6 @@ int nt2h_server_destroy(nt2h_server* hs) {
     }
   }
   if (s->pool) (void)hipMemPoolDestroy(s->pool);  // every free above completed on its slot stream
-  {
-  }
   delete s;
   return NT2H_OK;
 }

@@ -14,12 +14,18 @@
 // single-precision hardware rcp/log2/rsq (delta ~ 1e-6, below 1e-9 of the
 // price), while tau, d2, F = X exp(-rT) and both normcdf are FP64:
 //   exp(w)   = 2^(k/32) e^r, |r| <= ln2/64: 2^(j/32) from a 32-entry LDS
-//              table, e^r of degree 3 (7e-11), 2^(k>>5) by ldexp;
+//              table, e^r of degree 3 (7e-11), 2^(k>>5) by ldexp; the
+//              discount factor exp(-rT) for |rT| <= 1/4 (every usual rate
+//              and maturity) by one degree-7 polynomial instead (3e-12);
 //   N(-|d|)  = exp(-d^2/2) * erfcx(|d|/sqrt2)/2 with the second factor a
 //              piecewise polynomial (32 intervals, degree 4, 9e-11) in
 //              y = C/(C+|d|); the two tails share one reciprocal.
 // Both LDS tables hold 32 doubles per array, so a wave's gathers are free of
-// bank conflicts.  On MI355X an FP64 FMA issues at the rate of an FP32 one,
+// bank conflicts.
+// Outputs.  Only the out-of-the-money side is evaluated from the tails (the
+// call when d1 < 0, else the put); the other price follows from put-call
+// parity, C - P = S - F, which adds a term of the same sign (d1 < 0 implies
+// F > S) or, for d1 >= 0 > d2, cancels by at most a factor of ~2.  On MI355X an FP64 FMA issues at the rate of an FP32 one,
 // which is what makes the FP64 evaluation affordable.
 // Error budget of the f32 result: final rounding 0.5 ulp plus ~170 x 1.6e-10
 // from the approximations at the worst cancellation (0.4 ulp).
@@ -66,6 +72,38 @@ __device__ __forceinline__ double exp_w(double w, const double* __restrict__ ex2
   return exp_core(sh, __builtin_fma(kd, -0.693147180559945309417 / 32.0, w), ex2);
 }
 
+// exp(w) for |w| <= kExpSmall
+__device__ __forceinline__ double exp_small(double w) {
+  double p = __builtin_fma(kExpS[7], w, kExpS[6]);
+#pragma unroll
+  for (int i = 5; i >= 0; --i) p = __builtin_fma(p, w, kExpS[i]);
+  return p;
+}
+
+// exp(w), by the small-argument polynomial or the table path: a function of
+// w alone, so an option's price never depends on its neighbours.  A NaN w
+// gives NaN.
+__device__ __forceinline__ double discount(double w, const double* __restrict__ ex2) {
+  if (__builtin_fabs(w) <= kExpSmall) return exp_small(w);
+  return exp_w(w < -700.0 ? -700.0 : w, ex2);
+}
+
+// four discount factors, each equal to discount(w[k]); when every argument
+// of the wave is small (the common case) the table path is skipped (a
+// wave-uniform branch: it changes the work, not the bits)
+__device__ __forceinline__ void discount4(const double (&w)[4], const double* __restrict__ ex2, double (&E)[4]) {
+  bool small = true;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) small = small && __builtin_fabs(w[k]) <= kExpSmall;
+  if (__all(small)) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) E[k] = exp_small(w[k]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) E[k] = discount(w[k], ex2);
+  }
+}
+
 // exp(-q/2) for q in [0, 1400]
 __device__ __forceinline__ double exp_mhalf(double q, const double* __restrict__ ex2) {
   const double sh = __builtin_fma(q, -23.083120654223414, kRound52);  // -q/2 * 32/ln2
@@ -100,10 +138,13 @@ __device__ __forceinline__ void tails(double d1, double d2, const double* __rest
   Q2 = exp_mhalf(c2 * c2, ex2) * tail_poly(tab + j2, __builtin_amdgcn_fract(x2));
 }
 
-__device__ __forceinline__ void bs_accurate(float Sf, float Kf, float Tf, const Params& P,
-                                            const double* __restrict__ tab,
-                                            const double* __restrict__ ex2, float& call,
-                                            float& put) {
+// the discount argument w = -rT of one option (exp(w) is its discount factor)
+__device__ __forceinline__ double disc_arg(float Tf, const Params& P) { return -P.r * (double)Tf; }
+
+// call and put of one option given its discount factor E = exp(-rT)
+__device__ __forceinline__ void bs_price(float Sf, float Kf, float Tf, double E, const Params& P,
+                                         const double* __restrict__ tab, const double* __restrict__ ex2,
+                                         float& call, float& put) {
   // single-precision estimate of d1 = (ln(S/K) + (r + v^2/2) T) / (v sqrt(T))
   const float yf = __builtin_amdgcn_rsqf(Tf);
   const float qf = Sf * __builtin_amdgcn_rcpf(Kf);
@@ -115,18 +156,41 @@ __device__ __forceinline__ void bs_accurate(float Sf, float Kf, float Tf, const 
   const double tau = __builtin_fma((0.5 * P.v) * y, h, P.v * yT);
   const double d1 = d1f;
   const double d2 = d1 - tau;
-  const double F = (double)Kf * exp_w(__builtin_fmax(-P.r * T, -700.0), ex2);
+  const double F = (double)Kf * E;
   double Q1, Q2;
   tails(d1, d2, tab, ex2, Q1, Q2);
-  // N(d) = b - s Q and N(-d) = (1 - b) + s Q with s = sign(d), b = (1 + s)/2
-  const double s1 = __builtin_copysign(1.0, d1), s2 = __builtin_copysign(1.0, d2);
-  const double N1 = __builtin_fma(-s1, Q1, __builtin_fma(0.5, s1, 0.5));
-  const double M1 = __builtin_fma(s1, Q1, __builtin_fma(-0.5, s1, 0.5));
-  const double N2 = __builtin_fma(-s2, Q2, __builtin_fma(0.5, s2, 0.5));
-  const double M2 = __builtin_fma(s2, Q2, __builtin_fma(-0.5, s2, 0.5));
+  // With Q = N(-|d|): d1 < 0 (so d2 < 0): call = S Q1 - F Q2;
+  // d1 >= 0 > d2: put = F (1 - Q2) - S Q1; d2 >= 0: put = F Q2 - S Q1.
+  // D = S Q1 - F W is the call, or minus the put; the other side is
+  // D + (F - S) (put = call - S + F), or minus it.
   const double S = Sf;
-  call = (float)__builtin_fma(S, N1, -(F * N2));
-  put = (float)__builtin_fma(F, M2, -(S * M1));
+  const bool call_otm = d1 < 0.0;
+  const double W = (!call_otm && d2 < 0.0) ? 1.0 - Q2 : Q2;
+  const double D = __builtin_fma(S, Q1, -(F * W));
+  const double G = D + (F - S);
+  const float fd = (float)D, fg = (float)G;
+  call = call_otm ? fd : -fg;
+  put = call_otm ? fg : -fd;
+}
+
+__device__ __forceinline__ void bs_accurate(float Sf, float Kf, float Tf, const Params& P,
+                                            const double* __restrict__ tab,
+                                            const double* __restrict__ ex2, float& call,
+                                            float& put) {
+  bs_price(Sf, Kf, Tf, discount(disc_arg(Tf, P), ex2), P, tab, ex2, call, put);
+}
+
+// four options (one float4 of each table)
+template <class V4>
+__device__ __forceinline__ void bs_accurate4(const V4& s4, const V4& x4, const V4& t4, const Params& P,
+                                             const double* __restrict__ tab, const double* __restrict__ ex2,
+                                             float (&call)[4], float (&put)[4]) {
+  double w[4], E[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) w[k] = disc_arg(t4[k], P);
+  discount4(w, ex2, E);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) bs_price(s4[k], x4[k], t4[k], E[k], P, tab, ex2, call[k], put[k]);
 }
 
 // workgroup-wide copy of the tables into LDS (coefficient-major tail table, then 2^(j/32))

@@ -349,18 +349,37 @@ def test_batch_persistent_matches_plain_and_oracle(huge):
         lib().nt2h_batch_destroy(b)
 
 
-def test_blackscholes_accurate_within_4ulp():
+@pytest.mark.parametrize("r,v,tmax", [(0.02, 0.30, 10.0), (0.09, 0.30, 10.0), (-0.012, 0.45, 30.0)])
+def test_blackscholes_accurate_within_4ulp(r, v, tmax):
     """NT2H_ACCURATE: every option of a 2^22 sample within 4 ulp of the FP64
     evaluation of the same formula on the same f32 inputs (r, v promoted from
-    their f32 values); results below the normal range within 4 denormal steps."""
-    from nt2_amd.native import lib
+    their f32 values); results below the normal range within 4 denormal steps.
+    The benchmark parameters, then rates and maturities whose |rT| crosses the
+    discount polynomial's range (1/4) inside waves."""
     n = (1 << 22) + 3
-    say("bs accurate n=%d" % n)
-    S, X, T = _bs_inputs(n)
-    c, p = ops.blackscholes(S, X, T, 0.02, 0.30, policy=2)
+    say("bs accurate n=%d r=%g v=%g tmax=%g" % (n, r, v, tmax))
+    S, X, _ = _bs_inputs(n)
+    T = ops.fill_uniform(n, 0.25, tmax, 2 * 16 + 3)
+    c, p = ops.blackscholes(S, X, T, r, v, policy=2)
     s_, x_, t_ = (a.cpu().numpy() for a in (S, X, T))
-    rc, rp = O.bs_fp64ref(s_, x_, t_, float(np.float32(0.02)), float(np.float32(0.30)))
+    rc, rp = O.bs_fp64ref(s_, x_, t_, float(np.float32(r)), float(np.float32(v)))
     for got, ref in ((c.cpu().numpy(), rc), (p.cpu().numpy(), rp)):
         r32 = ref.astype(np.float32).astype(np.float64)
         e = classic_ulp(got, r32, np.float32)
         assert e.max() <= 4.0, (float(e.max()), int(np.argmax(e)))
+
+
+def test_blackscholes_accurate_neighbour_independent():
+    """An option's accurate price depends on its own inputs only: pricing a
+    permuted copy of a table whose discount arguments straddle the
+    polynomial's range gives the permuted prices, bit for bit."""
+    n = (1 << 18) + 1
+    say("bs accurate permutation n=%d" % n)
+    S, X, _ = _bs_inputs(n)
+    T = ops.fill_uniform(n, 0.25, 10.0, 77)
+    c, p = ops.blackscholes(S, X, T, 0.05, 0.30, policy=2)
+    perm = torch.from_numpy(np.random.default_rng(9).permutation(n)).cuda()
+    c2, p2 = ops.blackscholes(S[perm].contiguous(), X[perm].contiguous(), T[perm].contiguous(), 0.05, 0.30,
+                              policy=2)
+    assert same_bits(c2.cpu().numpy(), c[perm].cpu().numpy())
+    assert same_bits(p2.cpu().numpy(), p[perm].cpu().numpy())
