@@ -114,6 +114,26 @@ int nt2h_saxpy_f32(size_t n, float a, const float* x, float* y, void* stream);
 int nt2h_sum_f32(size_t n, const float* x, double* out, void* stream);
 int nt2h_dot_f32(size_t n, const float* x, const float* y, double* out, void* stream);
 
+/* ---- sum(a, dim) and cumsum(a, dim) over an N-D table -------------------- */
+/* x: device table of extent ext[0..3] (host array; column-major, trailing
+ * sizes 1 for fewer dimensions); dim >= 1 (beyond the last dimension: a
+ * dimension of size 1).  sum writes the table of extent ext with ext[dim-1]
+ * replaced by 1; cumsum writes a table of extent ext (out may equal x).
+ * policy NT2H_EXACT: NT2's order in the table's precision, bit-identical to
+ * NT2's CPU result (inner fold: pack accumulators, horizontal sum, scalar
+ * tail; outer fold and scans: sequential along the dimension); NT2H_FAST:
+ * parallel chunked forms with compensated / FP64 accumulation, each output
+ * rounded once.
+ * sum replaces NT2's reduction run_assign_ (modules/core/reduction/include/nt2/dsl/functions/reduction/run.hpp:30-93)
+ * with inner_fold_ (modules/core/reduction/include/nt2/core/functions/simd/inner_fold.hpp:44-67)
+ * and outer_fold_ (modules/core/reduction/include/nt2/core/functions/simd/outer_fold.hpp:46-76);
+ * cumsum replaces inner_scan_ / outer_scan_
+ * (modules/core/swar/include/nt2/core/functions/scalar/inner_scan.hpp:36-55, scalar/outer_scan.hpp:34-55). */
+int nt2h_sum_dim_f32(const float* x, const size_t* ext, int dim, float* out, int policy, void* stream);
+int nt2h_sum_dim_f64(const double* x, const size_t* ext, int dim, double* out, int policy, void* stream);
+int nt2h_cumsum_dim_f32(const float* x, const size_t* ext, int dim, float* out, int policy, void* stream);
+int nt2h_cumsum_dim_f64(const double* x, const size_t* ext, int dim, double* out, int policy, void* stream);
+
 /* ---- fused Black-Scholes ---------------------------------------------- */
 /* call/put from tables S, X, T and scalars r, v; one pass, one kernel.
  * demo mode writes call only (put may be NULL). */

@@ -64,6 +64,10 @@ def lib():
         "nt2h_sigmadelta_step_u8": ([S, P, P, P, P, I, P], I),
         "nt2h_expr4_f32": ([S, P, P, P, I, P], I),
         "nt2h_expr4_sum_f32": ([S, P, P, P, I, P], I),
+        "nt2h_sum_dim_f32": ([P, P, I, P, I, P], I),
+        "nt2h_sum_dim_f64": ([P, P, I, P, I, P], I),
+        "nt2h_cumsum_dim_f32": ([P, P, I, P, I, P], I),
+        "nt2h_cumsum_dim_f64": ([P, P, I, P, I, P], I),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)

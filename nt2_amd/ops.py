@@ -107,3 +107,37 @@ def expr4_sum(x, y, policy=EXACT, out=None):
     out = torch.empty(1, dtype=torch.float64, device=x.device) if out is None else out
     check(lib().nt2h_expr4_sum_f32(x.numel(), ptr(x), ptr(y), ptr(out), policy, _s()), "expr4_sum")
     return out
+
+
+def _ext4(shape):
+    """NT2 extent (column-major d0..d3) of a table given as a sequence of at
+    most 4 sizes; trailing sizes are 1."""
+    e = list(shape) + [1] * (4 - len(shape))
+    if len(e) != 4:
+        raise ValueError("at most 4 dimensions")
+    arr = (ctypes.c_size_t * 4)(*e)
+    return arr, e
+
+
+def sum_dim(x, ext, dim, policy=EXACT, out=None):
+    """NT2's sum(a, dim) of the device table x (flat, column-major, extent
+    `ext`); returns the flat result of extent ext with ext[dim-1] = 1."""
+    e4, e = _ext4(ext)
+    if dim <= 4:
+        e[dim - 1] = 1
+    n = 1
+    for v in e:
+        n *= v
+    out = torch.empty(n, dtype=x.dtype, device=x.device) if out is None else out
+    f = lib().nt2h_sum_dim_f32 if x.dtype == torch.float32 else lib().nt2h_sum_dim_f64
+    check(f(ptr(x), ctypes.cast(e4, ctypes.c_void_p), dim, ptr(out), policy, _s()), "sum_dim")
+    return out
+
+
+def cumsum_dim(x, ext, dim, policy=EXACT, out=None):
+    """NT2's cumsum(a, dim) of the device table x (flat, column-major)."""
+    e4, _ = _ext4(ext)
+    out = torch.empty_like(x) if out is None else out
+    f = lib().nt2h_cumsum_dim_f32 if x.dtype == torch.float32 else lib().nt2h_cumsum_dim_f64
+    check(f(ptr(x), ctypes.cast(e4, ctypes.c_void_p), dim, ptr(out), policy, _s()), "cumsum_dim")
+    return out

@@ -830,6 +830,74 @@ float fold_sum_f32(size_t n, Elem elem) {
   return s;
 }
 
+// sum(a, dim) and cumsum(a, dim) in NT2's order.  The reduction dispatch
+// (modules/core/reduction/include/nt2/dsl/functions/reduction/run.hpp:50-90):
+// with lo_dim the first non-singleton dimension, red <= lo_dim folds the
+// table reshaped to (size of red, product of the dimensions after red) with
+// inner_fold (SIMD: pack accumulators from each column's start, the AVX
+// horizontal sum, scalar tail; simd/inner_fold.hpp:44-67); otherwise it folds
+// (product before red, size of red, product after red) with outer_fold, a
+// sequential sum over the reduced dimension per output
+// (details/outer_fold_step.hpp:33-60).  Scans are sequential per line
+// (modules/core/swar/include/nt2/core/functions/scalar/inner_scan.hpp:36-55,
+// outer_scan.hpp:34-55, details/scan_step.hpp:30-44).
+struct Lines {
+  size_t inner, len, outer;
+  bool inner_fold;
+};
+Lines lines_of(const size_t* ext, int dim) {
+  Lines L{1, 1, 1, false};
+  size_t lo_dim = 1;
+  for (int k = 0; k < 4; ++k)
+    if (ext[k] != 1) { lo_dim = (size_t)k + 1; break; }
+  for (int k = 0; k < 4; ++k) {
+    if (k < dim - 1) L.inner *= ext[k];
+    else if (k == dim - 1) L.len = ext[k];
+    else L.outer *= ext[k];
+  }
+  L.inner_fold = (size_t)dim <= lo_dim;
+  return L;
+}
+template <class T, int N>
+void sum_dim(const T* x, const size_t* ext, int dim, T* out) {
+  const Lines L = lines_of(ext, dim);
+  if (L.inner_fold) {
+    for (size_t o = 0; o < L.outer; ++o) {
+      const T* c = x + o * L.len;
+      T acc[N];
+      for (int l = 0; l < N; ++l) acc[l] = T(0);
+      const size_t nb = L.len / N;
+      for (size_t p = 0; p < nb; ++p)
+        for (int l = 0; l < N; ++l) acc[l] = acc[l] + c[p * N + l];
+      T r;
+      if constexpr (N == 8) r = hsum8(acc);
+      else r = hsum4(acc);
+      for (size_t i = nb * N; i < L.len; ++i) r = r + c[i];
+      out[o] = r;
+    }
+  } else {
+    for (size_t o = 0; o < L.outer; ++o)
+      for (size_t i = 0; i < L.inner; ++i) {
+        T acc = T(0);
+        for (size_t m = 0; m < L.len; ++m) acc = acc + x[i + L.inner * (m + L.len * o)];
+        out[i + L.inner * o] = acc;
+      }
+  }
+}
+template <class T>
+void cumsum_dim(const T* x, const size_t* ext, int dim, T* out) {
+  const Lines L = lines_of(ext, dim);
+  for (size_t o = 0; o < L.outer; ++o)
+    for (size_t i = 0; i < L.inner; ++i) {
+      T acc = T(0);
+      for (size_t m = 0; m < L.len; ++m) {
+        const size_t idx = i + L.inner * (m + L.len * o);
+        acc = acc + x[idx];
+        out[idx] = acc;
+      }
+    }
+}
+
 }  // namespace
 
 // ===========================================================================
@@ -1184,6 +1252,28 @@ double oracle_expr4_sum_fp64(const float* x, const float* y, size_t n) {
     lo += e;
   }
   return hi + lo;
+}
+
+// sum(a, dim) / cumsum(a, dim) of a column-major table of extent ext[0..3]
+int oracle_sum_dim_f32(const float* x, const size_t* ext, int dim, float* out) {
+  if (dim < 1) return -1;
+  sum_dim<float, 8>(x, ext, dim, out);
+  return 0;
+}
+int oracle_sum_dim_f64(const double* x, const size_t* ext, int dim, double* out) {
+  if (dim < 1) return -1;
+  sum_dim<double, 4>(x, ext, dim, out);
+  return 0;
+}
+int oracle_cumsum_dim_f32(const float* x, const size_t* ext, int dim, float* out) {
+  if (dim < 1) return -1;
+  cumsum_dim<float>(x, ext, dim, out);
+  return 0;
+}
+int oracle_cumsum_dim_f64(const double* x, const size_t* ext, int dim, double* out) {
+  if (dim < 1) return -1;
+  cumsum_dim<double>(x, ext, dim, out);
+  return 0;
 }
 
 }  // extern "C"

@@ -49,6 +49,8 @@ def lib():
         _lib.oracle_expr4_sum_f32.restype = ctypes.c_float
         _lib.oracle_expr4_sum_fp64.argtypes = [P, P, S]
         _lib.oracle_expr4_sum_fp64.restype = ctypes.c_double
+        for f in ("oracle_sum_dim_f32", "oracle_sum_dim_f64", "oracle_cumsum_dim_f32", "oracle_cumsum_dim_f64"):
+            getattr(_lib, f).argtypes = [P, P, ctypes.c_int, P]
         _lib.oracle_set_threads.argtypes = [ctypes.c_int]
         _lib.oracle_get_threads.restype = ctypes.c_int
     return _lib
@@ -188,3 +190,31 @@ def expr4_sum_fp64(x, y):
     x = np.ascontiguousarray(x, dtype=np.float32)
     y = np.ascontiguousarray(y, dtype=np.float32)
     return lib().oracle_expr4_sum_fp64(_p(x), _p(y), x.size)
+
+
+def _ext4(ext):
+    e = list(ext) + [1] * (4 - len(ext))
+    return np.array(e, dtype=np.uint64)
+
+
+def sum_dim(x, ext, dim):
+    """NT2's sum(a, dim) of the flat column-major table x of extent ext."""
+    x = np.ascontiguousarray(x)
+    e = _ext4(ext)
+    oe = e.copy()
+    if dim <= 4:
+        oe[dim - 1] = 1
+    out = np.empty(int(np.prod(oe)), dtype=x.dtype)
+    f = lib().oracle_sum_dim_f32 if x.dtype == np.float32 else lib().oracle_sum_dim_f64
+    assert f(_p(x), _p(e), dim, _p(out)) == 0
+    return out
+
+
+def cumsum_dim(x, ext, dim):
+    """NT2's cumsum(a, dim) of the flat column-major table x of extent ext."""
+    x = np.ascontiguousarray(x)
+    e = _ext4(ext)
+    out = np.empty_like(x)
+    f = lib().oracle_cumsum_dim_f32 if x.dtype == np.float32 else lib().oracle_cumsum_dim_f64
+    assert f(_p(x), _p(e), dim, _p(out)) == 0
+    return out

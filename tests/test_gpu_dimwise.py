@@ -1,0 +1,105 @@
+"""sum(a, dim) and cumsum(a, dim) on the GPU through the C ABI
+(nt2h_sum_dim_*, nt2h_cumsum_dim_*) against the oracle.
+
+Exact policy: bit-identical to the oracle's restatement of NT2's order for
+every shape and dimension (inner fold with pack accumulators, sequential outer
+folds and scans).  Fast policy: within the error of one final rounding plus
+FP64 accumulation of the exact sum (math.fsum), deterministic run to run.
+"""
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+import _oracle as O  # noqa: E402
+from _gpu import same_bits, say  # noqa: E402
+from nt2_amd import EXACT, FAST, ops  # noqa: E402
+
+SHAPES = [(37,), (1000,), (5, 7), (16, 3), (3, 16), (1, 33), (33, 1), (64, 100), (4, 5, 6), (2, 3, 4, 5),
+          (17, 1, 9), (0, 4), (4, 0), (9, 2, 1, 3)]
+
+
+def _dev(x):
+    return torch.from_numpy(x).cuda()
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("shape", SHAPES)
+def test_exact_matches_oracle(shape, dtype):
+    rng = np.random.default_rng(sum(shape) + 7 * len(shape))
+    n = int(np.prod(shape))
+    x = rng.uniform(-1, 1, n).astype(dtype) * np.where(rng.random(n) < 0.1, 1e6, 1).astype(dtype)
+    xd = _dev(x)
+    for dim in range(1, 6):
+        say("dimwise exact", shape, dtype.__name__, "dim=%d" % dim)
+        s = ops.sum_dim(xd, shape, dim, EXACT).cpu().numpy()
+        assert same_bits(s, O.sum_dim(x, shape, dim)), (shape, dim)
+        c = ops.cumsum_dim(xd, shape, dim, EXACT).cpu().numpy()
+        assert same_bits(c, O.cumsum_dim(x, shape, dim)), (shape, dim)
+
+
+def test_exact_nt2_kats():
+    # modules/core/swar/unit/table/cumsum.cpp and modules/core/reduction/unit/table/sum.cpp
+    kat = np.arange(1, 31, dtype=np.float64)
+    c1 = ops.cumsum_dim(_dev(kat), (3, 5, 2), 1, EXACT).cpu().numpy()
+    c2 = ops.cumsum_dim(_dev(kat), (3, 5, 2), 2, EXACT).cpu().numpy()
+    assert list(c1[:6]) == [1, 3, 6, 4, 9, 15] and c1[-1] == 87
+    assert list(c2[:6]) == [1, 2, 3, 5, 7, 9] and c2[-1] == 120
+    ones = _dev(np.ones(20, dtype=np.float64))
+    assert list(ops.sum_dim(ones, (5, 4), 1, EXACT).cpu().numpy()) == [5] * 4
+    assert list(ops.sum_dim(ones, (5, 4), 2, EXACT).cpu().numpy()) == [4] * 5
+    col = np.array([2.0 ** 24, 1, 1, 1, 1, 1, 1, 1], dtype=np.float32)
+    assert ops.sum_dim(_dev(col), (8,), 1, EXACT).item() == 2.0 ** 24 + 6
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("shape,dim", [((1 << 20,), 1), ((3, 100000), 2), ((20000, 7), 1), ((1, 300000), 2),
+                                       ((50000, 3), 1), ((8, 8, 4096), 3), ((3,), 1)])
+def test_fast_within_bounds(shape, dim, dtype):
+    rng = np.random.default_rng(len(shape) + dim)
+    n = int(np.prod(shape))
+    x = rng.uniform(-1, 1, n).astype(dtype)
+    say("dimwise fast", shape, dtype.__name__, "dim=%d" % dim)
+    xd = _dev(x)
+    s1 = ops.sum_dim(xd, shape, dim, FAST)
+    s2 = ops.sum_dim(xd, shape, dim, FAST)
+    assert torch.equal(s1, s2)  # deterministic
+    a = x.reshape(shape, order="F").astype(np.float64)
+    ax = dim - 1
+    ref = np.apply_along_axis(lambda v: math.fsum(v), ax, a).ravel(order="F")
+    got = s1.cpu().numpy().astype(np.float64)
+    eps = np.finfo(dtype).eps
+    mag = np.sum(np.abs(a), axis=ax).ravel(order="F")
+    # one rounding of the result plus the accumulation error of compensated sums
+    assert np.all(np.abs(got - ref) <= eps * np.abs(ref) + 1e-12 * mag + 1e-300)
+    c = ops.cumsum_dim(xd, shape, dim, FAST)
+    cref = np.cumsum(a, axis=ax).ravel(order="F")
+    cmag = np.cumsum(np.abs(a), axis=ax).ravel(order="F")
+    err = np.abs(c.cpu().numpy().astype(np.float64) - cref)
+    tol = (eps * np.abs(cref) + 1e-12 * cmag) if dtype == np.float32 else (eps * np.abs(cref) + shape[ax] * eps * cmag)
+    assert np.all(err <= tol + 1e-300)
+
+
+def test_cumsum_in_place():
+    x = np.random.default_rng(3).uniform(-1, 1, 3 * 20000).astype(np.float32)
+    for pol, shape, dim in ((EXACT, (3, 20000), 2), (FAST, (20000, 3), 1), (FAST, (3, 20000), 2)):
+        xd = _dev(x.copy())
+        want = ops.cumsum_dim(_dev(x), shape, dim, pol)
+        ops.cumsum_dim(xd, shape, dim, pol, out=xd)
+        assert torch.equal(xd, want)
+
+
+def test_argument_errors():
+    from nt2_amd.native import Nt2hError
+    xd = _dev(np.ones(8, dtype=np.float32))
+    with pytest.raises(Nt2hError):
+        ops.sum_dim(xd, (8,), 0, EXACT)
+    with pytest.raises(Nt2hError):
+        ops.sum_dim(xd, (8,), 1, 7)
