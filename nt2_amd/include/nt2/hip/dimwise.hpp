@@ -32,7 +32,10 @@
 //
 // A source is any type with
 //   template <int G> __device__ void load(size_t i, T (&r)[G]) const;
-// giving the G elements from linear index i (a pointer, or a fused expression).
+// giving the G elements from linear index i (a pointer, or a fused expression),
+// and a constant `pack_free`: true when loading elements one at a time gives
+// the values a pack load gives (false for expressions whose functions decide
+// per pack, such as NT2's trigonometric range reduction).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -46,6 +49,7 @@ template <> struct pw<float> { static constexpr int value = 8; };
 template <> struct pw<double> { static constexpr int value = 4; };
 
 template <class T> struct ptr_src {
+  static constexpr bool pack_free = true;
   const T* p;
   template <int G> __device__ __forceinline__ void load(size_t i, T (&r)[G]) const {
 #pragma unroll
@@ -101,6 +105,43 @@ __global__ __launch_bounds__(256) void inner_fold_exact_k(Src src, size_t len, s
     s = s + v[0];
   }
   out[col] = s;
+}
+
+// inner fold of pack-free sources: W consecutive threads per column, thread l
+// accumulating lane l of every pack (the same chain as the pack accumulator's
+// lane), then NT2's horizontal sum and the tail in the column's first thread
+template <class T, class Src>
+__global__ __launch_bounds__(256) void inner_fold_lanes_k(Src src, size_t len, size_t ncol, T* __restrict__ out) {
+  _Pragma("clang fp contract(off)")
+  constexpr int W = pw<T>::value;
+  const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t col = g / W;
+  const int lane = (int)(g % W);
+  const bool live = col < ncol;
+  T acc = T(0);
+  if (live) {
+    const size_t k = col * len;
+    const size_t nb = len / W;
+    for (size_t p = 0; p < nb; ++p) {
+      T v[1];
+      src.template load<1>(k + p * W + lane, v);
+      acc = acc + v[0];
+    }
+  }
+  T a[W];
+  const int first = (int)(threadIdx.x & 63) - lane;  // the column's first lane in the wave
+#pragma unroll
+  for (int l = 0; l < W; ++l) a[l] = __shfl(acc, first + l, 64);
+  if (live && lane == 0) {
+    const size_t k = col * len;
+    T s = hsum(a);
+    for (size_t i = len / W * W; i < len; ++i) {
+      T v[1];
+      src.template load<1>(k + i, v);
+      s = s + v[0];
+    }
+    out[col] = s;
+  }
 }
 
 // outer fold: per outer index o, items are the W-packs of the inner positions
@@ -328,7 +369,11 @@ void launch_sum(const Src& src, const lines& L, bool fast, double* scratch, T* o
   const size_t nout = L.inner * L.outer;
   if (nout == 0) return;
   if (!fast) {
-    if (L.inner == 1) {
+    if (L.inner == 1 && Src::pack_free) {
+      constexpr int W = pw<T>::value;
+      hipLaunchKernelGGL((inner_fold_lanes_k<T, Src>), dim3(blocks_for(L.outer * W)), dim3(256), 0, s, src, L.len,
+                         L.outer, out);
+    } else if (L.inner == 1) {
       hipLaunchKernelGGL((inner_fold_exact_k<T, Src>), dim3(blocks_for(L.outer)), dim3(256), 0, s, src, L.len,
                          L.outer, out);
     } else {

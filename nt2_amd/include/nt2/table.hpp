@@ -18,9 +18,13 @@
 //    pack-dependent kernels (the trigonometric range reduction) give NT2's bits;
 //  * tie reads every right-hand side at an index before writing any output
 //    (modules/core/container/table/include/nt2/core/functions/expr/tie.hpp:40-121);
-//  * sum/globalsum/dot fold the whole table (modules/core/reduction/include/nt2/
+//  * globalsum/dot fold the whole table (modules/core/reduction/include/nt2/
 //    core/functions/globalsum.hpp:24-88, reduction/functions/expr/dot.hpp:22-29),
-//    here with compensated FP32 partials and an FP64 cross-block combine.
+//    here with compensated FP32 partials and an FP64 cross-block combine;
+//  * sum(a[, dim]) and cumsum(a[, dim]) reduce or scan along a dimension
+//    (default: the first non-singleton one) in NT2's order under the exact
+//    policy (modules/core/reduction/include/nt2/dsl/functions/reduction/run.hpp:30-93),
+//    in the parallel fast order under the fast policy (nt2/hip/dimwise.hpp).
 //
 // Storage is mirrored: a table keeps a host copy and a device copy with
 // validity flags, so host element access (setup loops) and device evaluation
@@ -41,6 +45,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "nt2/hip/dimwise.hpp"
 #include "nt2/hip/math.hpp"
 
 #ifndef NT2_HIP_POLICY
@@ -74,10 +79,13 @@ template <> struct pack_width<bool> { static constexpr int value = 16; };
 }}  // namespace hip::fe
 
 // ---- extents ----------------------------------------------------------------
+// up to NT2_MAX_DIMENSIONS = 4 dimensions, trailing sizes 1
 struct of_size_ {
-  size_t d[2];
-  size_t numel() const { return d[0] * d[1]; }
-  bool operator==(const of_size_& o) const { return d[0] == o.d[0] && d[1] == o.d[1]; }
+  size_t d[4];
+  size_t numel() const { return d[0] * d[1] * d[2] * d[3]; }
+  bool operator==(const of_size_& o) const {
+    return d[0] == o.d[0] && d[1] == o.d[1] && d[2] == o.d[2] && d[3] == o.d[3];
+  }
 };
 // extent of an expression: none for a scalar
 struct ext_t {
@@ -89,8 +97,16 @@ inline ext_t merge_ext(const ext_t& x, const ext_t& y) {
   return x.has ? x : y;
 }
 namespace meta { template <class T> struct as_ {}; }
-inline of_size_ of_size(size_t n) { return of_size_{{n, 1}}; }
-inline of_size_ of_size(size_t h, size_t w) { return of_size_{{h, w}}; }
+inline of_size_ of_size(size_t n) { return of_size_{{n, 1, 1, 1}}; }
+inline of_size_ of_size(size_t h, size_t w) { return of_size_{{h, w, 1, 1}}; }
+inline of_size_ of_size(size_t a, size_t b, size_t c) { return of_size_{{a, b, c, 1}}; }
+inline of_size_ of_size(size_t a, size_t b, size_t c, size_t e) { return of_size_{{a, b, c, e}}; }
+// first dimension of size other than 1 (1 when there is none)
+inline int firstnonsingleton(const of_size_& s) {
+  for (int k = 0; k < 4; ++k)
+    if (s.d[k] != 1) return k + 1;
+  return 1;
+}
 
 template <class T> class table;
 
@@ -113,7 +129,7 @@ template <class T> struct terminal : is_expr_base {
 template <class T> struct scalar : is_expr_base {
   typedef T value_type;
   T v;
-  ext_t ext() const { return ext_t{false, of_size_{{0, 1}}}; }
+  ext_t ext() const { return ext_t{false, of_size_{{0, 1, 1, 1}}}; }
   void prepare() const {}
   scal_v<T> view() const { return scal_v<T>{v}; }
 };
@@ -290,11 +306,11 @@ template <class T>
 class table {
  public:
   typedef T value_type;
-  table() : ext_{{0, 1}} {}
-  explicit table(of_size_ s) : ext_{{0, 1}} { resize(s); }
-  table(const table& o) : ext_{{0, 1}} { *this = o; }
+  table() : ext_{{0, 1, 1, 1}} {}
+  explicit table(of_size_ s) : ext_{{0, 1, 1, 1}} { resize(s); }
+  table(const table& o) : ext_{{0, 1, 1, 1}} { *this = o; }
   template <class E, class = typename std::enable_if<std::is_base_of<expr::is_expr_base, E>::value>::type>
-  table(const E& e) : ext_{{0, 1}} { assign(e); }
+  table(const E& e) : ext_{{0, 1, 1, 1}} { assign(e); }
   ~table() { release(); }
 
   table& operator=(const table& o) {
@@ -332,7 +348,7 @@ class table {
   }
   size_t numel() const { return ext_.numel(); }
   of_size_ extent() const { return ext_; }
-  size_t size(int dim) const { return dim == 1 ? ext_.d[0] : (dim == 2 ? ext_.d[1] : 1); }
+  size_t size(int dim) const { return dim >= 1 && dim <= 4 ? ext_.d[dim - 1] : 1; }
 
   // 1-based, column-major host access (synchronizes with the device copy)
   T& operator()(size_t i) {
@@ -346,6 +362,20 @@ class table {
   }
   T& operator()(size_t i, size_t j) { return (*this)((j - 1) * ext_.d[0] + i); }
   T operator()(size_t i, size_t j) const { return (*this)((j - 1) * ext_.d[0] + i); }
+  T& operator()(size_t i, size_t j, size_t k) { return (*this)(((k - 1) * ext_.d[1] + (j - 1)) * ext_.d[0] + i); }
+  T operator()(size_t i, size_t j, size_t k) const {
+    return (*this)(((k - 1) * ext_.d[1] + (j - 1)) * ext_.d[0] + i);
+  }
+  // the value of a one-element table (a reduction to a scalar: T s = sum(x))
+  operator T() const {
+    if (numel() != 1) throw std::invalid_argument("nt2: table of more than one element used as a scalar");
+    return (*this)(1);
+  }
+  // same elements, new extent of the same number of elements
+  void reshape_to(of_size_ s) {
+    if (s.numel() != numel()) throw std::invalid_argument("nt2: reshape changes the number of elements");
+    ext_ = s;
+  }
 
   // raw access
   const T* device_data() const { sync_dev(); return dev_; }
@@ -751,9 +781,126 @@ typename expr::vt<typename expr::wrap<E>::type>::type globalsum(const E& e) {
   x.prepare();
   return (T)hip::fold<T>(x, n);
 }
-// sum of a vector table (NT2's sum along the first non-singleton dimension)
+// ---- reductions and scans along a dimension ------------------------------------------
+namespace hip { namespace fe {
+// an expression as a source of the dimension-wise kernels
+template <class T, class V, bool PF> struct expr_src {
+  static constexpr bool pack_free = PF;
+  V v;
+  template <int G> __device__ __forceinline__ void load(size_t i, T (&r)[G]) const { ev<G>(v, i, r); }
+};
+}}  // namespace hip::fe
+namespace expr {
+// false when a function of the expression decides per pack (NT2's
+// trigonometric range reduction; the f64 erfc regions)
+template <class X> struct pack_free { static constexpr bool value = true; };
+template <class Tag, class A> struct pack_free<node1<Tag, A>> {
+  static constexpr bool value = pack_free<A>::value && !std::is_same<Tag, tagfn::cos_>::value &&
+                                !std::is_same<Tag, tagfn::sin_>::value &&
+                                !((std::is_same<Tag, tagfn::erfc_>::value || std::is_same<Tag, tagfn::normcdf_>::value) &&
+                                  std::is_same<typename A::value_type, double>::value);
+};
+template <class Tag, class A, class B> struct pack_free<node2<Tag, A, B>> {
+  static constexpr bool value = pack_free<A>::value && pack_free<B>::value;
+};
+template <class Tag, class A, class B, class C> struct pack_free<node3<Tag, A, B, C>> {
+  static constexpr bool value = pack_free<A>::value && pack_free<B>::value && pack_free<C>::value;
+};
+}  // namespace expr
+namespace hip {
+// sum (scan false) or cumsum (scan true) of x along dim into a new table
+template <class T, class EX>
+table<T> dimwise(const EX& x, int dim, bool scan, hipStream_t s = nullptr) {
+  static_assert(std::is_same<T, float>::value || std::is_same<T, double>::value,
+                "sum/cumsum along a dimension: float or double tables");
+  if (dim < 1) throw std::invalid_argument("nt2: dimension must be >= 1");
+  const ext_t ex = x.ext();
+  if (!ex.has) throw std::invalid_argument("nt2: reduction of a scalar expression");
+  x.prepare();
+  of_size_ os = ex.s;
+  if (!scan && dim <= 4) os.d[dim - 1] = 1;
+  table<T> r(os);
+  if (r.numel() == 0) return r;
+  const size_t e4[4] = {ex.s.d[0], ex.s.d[1], ex.s.d[2], ex.s.d[3]};
+  const dw::lines L = dw::lines_of(e4, dim);
+  const bool fast = NT2_HIP_POLICY::fused;
+  double* scratch = nullptr;
+  if (const size_t need = dw::scratch_doubles(L, fast)) fe::check(hipMalloc(&scratch, need * sizeof(double)), "scratch");
+  auto v = expr::dview<EX>::get(x);
+  const fe::expr_src<T, decltype(v), expr::pack_free<EX>::value> src{v};
+  T* out = r.device_data_for_write();
+  if (scan) dw::launch_scan<T>(src, L, fast, scratch, out, s);
+  else dw::launch_sum<T>(src, L, fast, scratch, out, s);
+  fe::check(hipGetLastError(), "dimwise launch");
+  if (scratch) {
+    fe::check(hipStreamSynchronize(s), "dimwise sync");
+    fe::check(hipFree(scratch), "scratch free");
+  }
+  return r;
+}
+}  // namespace hip
+
+// sum(a, dim): NT2's reduction along dim (a table of a's extent with dim of size 1)
 template <class E, NT2H_ENABLE1(E)>
-auto sum(const E& e) { return globalsum(e); }
+auto sum(const E& e, int dim) {
+  typedef typename expr::wrap<E>::type EX;
+  typedef typename expr::vt<EX>::type T;
+  return hip::dimwise<T>(expr::wrap<E>::get(e), dim, false);
+}
+// sum(a): along the first non-singleton dimension; a one-element result
+// converts to a scalar (T s = sum(x) for a vector x)
+template <class E, NT2H_ENABLE1(E)>
+auto sum(const E& e) {
+  typedef typename expr::wrap<E>::type EX;
+  const EX& x = expr::wrap<E>::get(e);
+  const ext_t ex = x.ext();
+  return sum(e, ex.has ? firstnonsingleton(ex.s) : 1);
+}
+// cumsum(a, dim) and cumsum(a): running sums along a dimension
+// (modules/core/swar/include/nt2/swar/functions/container/cumsum.hpp)
+template <class E, NT2H_ENABLE1(E)>
+auto cumsum(const E& e, int dim) {
+  typedef typename expr::wrap<E>::type EX;
+  typedef typename expr::vt<EX>::type T;
+  return hip::dimwise<T>(expr::wrap<E>::get(e), dim, true);
+}
+template <class E, NT2H_ENABLE1(E)>
+auto cumsum(const E& e) {
+  typedef typename expr::wrap<E>::type EX;
+  const EX& x = expr::wrap<E>::get(e);
+  const ext_t ex = x.ext();
+  return cumsum(e, ex.has ? firstnonsingleton(ex.s) : 1);
+}
+
+// ---- reshape and colon ---------------------------------------------------------------
+// reshape(a, sizes): the elements of a (evaluated if an expression) with a new
+// extent of the same number of elements
+template <class E, NT2H_ENABLE1(E)>
+auto reshape(const E& e, of_size_ s) {
+  typedef typename expr::wrap<E>::type EX;
+  typedef typename expr::vt<EX>::type T;
+  table<T> r;
+  r = expr::wrap<E>::get(e);
+  r.reshape_to(s);
+  return r;
+}
+template <class E, NT2H_ENABLE1(E)> auto reshape(const E& e, size_t a, size_t b) { return reshape(e, of_size(a, b)); }
+template <class E, NT2H_ENABLE1(E)> auto reshape(const E& e, size_t a, size_t b, size_t c) {
+  return reshape(e, of_size(a, b, c));
+}
+template <class E, NT2H_ENABLE1(E)> auto reshape(const E& e, size_t a, size_t b, size_t c, size_t d) {
+  return reshape(e, of_size(a, b, c, d));
+}
+// _(a, b): the row vector a, a+1, ..., b (NT2's colon)
+struct colon_ {
+  template <class T> table<T> operator()(T a, T b) const {
+    const size_t n = b >= a ? (size_t)(b - a) + 1 : 0;
+    table<T> r(of_size(1, n));
+    for (size_t i = 0; i < n; ++i) r(i + 1) = (T)(a + (T)i);
+    return r;
+  }
+};
+static const colon_ _ = {};
 // dot(a, b) = sum(a .* conj(b)): modules/core/reduction/include/nt2/reduction/functions/expr/dot.hpp:22-29
 template <class A, class B, NT2H_ENABLE2(A, B)>
 auto dot(const A& a, const B& b) { return globalsum(a * b); }

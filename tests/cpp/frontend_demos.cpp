@@ -126,6 +126,40 @@ int main(int argc, char** argv) {
   dump("sd_bkg", background_img);
   dump("sd_var", image_variance);
   dump("sd_label", binary_label);
+
+  // ---- sum / cumsum along dimensions (NT2's unit tests
+  // modules/core/reduction/unit/table/sum.cpp and modules/core/swar/unit/table/cumsum.cpp)
+  table<double> a = reshape(_(1.0, 30.0), 3, 5, 2);
+  table<double> b = cumsum(a);
+  dump("cumsum_inner", b);
+  b = cumsum(a, 2);
+  dump("cumsum_outer", b);
+  table<double> r = sum(a);
+  dump("sum_a", r);
+  r = sum(a, 2);
+  dump("sum_a2", r);
+  r = sum(a, 3);
+  dump("sum_a3", r);
+  r = sum(a, 4);
+  dump("sum_a4", r);
+  const std::size_t mh = 256, mw = 129;
+  table<float> M(of_size(mh, mw));
+  for (std::size_t i = 1; i <= mh * mw; ++i) M(i) = unif(seed, -1e4f, 1e4f);
+  dump("M", M);
+  table<float> s1 = sum(M, 1), s2 = sum(M, 2), c1 = cumsum(M), c2 = cumsum(M, 2);
+  dump("M_sum1", s1);
+  dump("M_sum2", s2);
+  dump("M_cumsum1", c1);
+  dump("M_cumsum2", c2);
+  s1 = sum(nt2::cos(M), 1);  // cos decides its range reduction per pack
+  s2 = sum(nt2::cos(M), 2);
+  dump("M_sumcos1", s1);
+  dump("M_sumcos2", s2);
+  s1 = sum(M * 2.0f + 1.0f);  // first non-singleton dimension: 1
+  dump("M_sumexpr", s1);
+  table<float> v = M(1) * ones(of_size(1000), meta::as_<float>());
+  const float vs = sum(v);  // a one-element result used as a scalar
+  dump_scalar("v_sum", vs);
   std::printf("frontend demos done: %zu elements\n", size);
   return 0;
 }

@@ -78,3 +78,38 @@ def test_sigmadelta_statements(outputs):
     assert np.array_equal(outputs("sd_bkg", np.uint8), b)
     assert np.array_equal(outputs("sd_var", np.uint8), v)
     assert np.array_equal(outputs("sd_label", np.uint8), l)
+
+
+def test_dimwise_kats(outputs):
+    """sum and cumsum along dimensions of reshape(_(1, 30), 3, 5, 2): NT2's
+    unit-test answers (modules/core/swar/unit/table/cumsum.cpp,
+    modules/core/reduction/unit/table/sum.cpp)."""
+    a = np.arange(1, 31, dtype=np.float64)
+    assert list(outputs("cumsum_inner", np.float64)) == list(O.cumsum_dim(a, (3, 5, 2), 1))
+    assert list(outputs("cumsum_outer", np.float64)[:6]) == [1, 2, 3, 5, 7, 9]
+    assert list(outputs("cumsum_outer", np.float64)) == list(O.cumsum_dim(a, (3, 5, 2), 2))
+    assert list(outputs("sum_a", np.float64)) == list(O.sum_dim(a, (3, 5, 2), 1))
+    assert list(outputs("sum_a2", np.float64)) == list(O.sum_dim(a, (3, 5, 2), 2))
+    assert list(outputs("sum_a3", np.float64)) == list(O.sum_dim(a, (3, 5, 2), 3))
+    assert list(outputs("sum_a4", np.float64)) == list(a)  # a singleton dimension: a copy
+
+
+def test_dimwise_statements_bitwise(outputs):
+    """Front-end sum/cumsum (exact policy) equal the oracle's NT2-order
+    restatement bit for bit, expressions included: cos evaluated per pack of
+    8 along the columns (inner fold) or the rows (outer fold) as NT2 does."""
+    M = outputs("M", np.float32)
+    ext = (256, 129)
+    assert same_bits(outputs("M_sum1", np.float32), O.sum_dim(M, ext, 1))
+    assert same_bits(outputs("M_sum2", np.float32), O.sum_dim(M, ext, 2))
+    assert same_bits(outputs("M_cumsum1", np.float32), O.cumsum_dim(M, ext, 1))
+    assert same_bits(outputs("M_cumsum2", np.float32), O.cumsum_dim(M, ext, 2))
+    # 256 rows: the columns' packs and the rows' packs (256 a multiple of 16)
+    # coincide with the table's packs from index 0, so map then fold restates it
+    c = O.map_f32("cos", M)
+    assert same_bits(outputs("M_sumcos1", np.float32), O.sum_dim(c, ext, 1))
+    assert same_bits(outputs("M_sumcos2", np.float32), O.sum_dim(c, ext, 2))
+    e = (M * np.float32(2.0)).astype(np.float32) + np.float32(1.0)
+    assert same_bits(outputs("M_sumexpr", np.float32), O.sum_dim(e.astype(np.float32), ext, 1))
+    v = np.full(1000, M[0], dtype=np.float32)
+    assert outputs("v_sum", np.float32)[0] == O.sum_dim(v, (1000,), 1)[0]
