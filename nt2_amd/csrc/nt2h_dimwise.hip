@@ -30,7 +30,7 @@ static int run_dim(const char* name, bool scan, const T* x, const size_t* ext, i
   const bool fast = policy == NT2H_FAST;
   hipStream_t s = as_stream(stream);
   double* scratch = nullptr;
-  const size_t need = dw::scratch_doubles(L, fast);
+  const size_t need = dw::scratch_doubles(L, fast, scan);
   if (need) {
     Scratch sc;
     if (int rc = get_scratch(need, &sc, s)) return rc;

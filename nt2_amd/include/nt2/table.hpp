@@ -825,7 +825,7 @@ table<T> dimwise(const EX& x, int dim, bool scan, hipStream_t s = nullptr) {
   const dw::lines L = dw::lines_of(e4, dim);
   const bool fast = NT2_HIP_POLICY::fused;
   double* scratch = nullptr;
-  if (const size_t need = dw::scratch_doubles(L, fast)) fe::check(hipMalloc(&scratch, need * sizeof(double)), "scratch");
+  if (const size_t need = dw::scratch_doubles(L, fast, scan)) fe::check(hipMalloc(&scratch, need * sizeof(double)), "scratch");
   auto v = expr::dview<EX>::get(x);
   const fe::expr_src<T, decltype(v), expr::pack_free<EX>::value> src{v};
   T* out = r.device_data_for_write();

@@ -61,7 +61,8 @@ def test_exact_nt2_kats():
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("shape,dim", [((1 << 20,), 1), ((3, 100000), 2), ((20000, 7), 1), ((1, 300000), 2),
-                                       ((50000, 3), 1), ((8, 8, 4096), 3), ((3,), 1)])
+                                       ((50000, 3), 1), ((8, 8, 4096), 3), ((3,), 1), ((100, 5000), 1),
+                                       ((300, 70000), 1), ((70000, 300), 2)])
 def test_fast_within_bounds(shape, dim, dtype):
     rng = np.random.default_rng(len(shape) + dim)
     n = int(np.prod(shape))
@@ -89,7 +90,8 @@ def test_fast_within_bounds(shape, dim, dtype):
 
 def test_cumsum_in_place():
     x = np.random.default_rng(3).uniform(-1, 1, 3 * 20000).astype(np.float32)
-    for pol, shape, dim in ((EXACT, (3, 20000), 2), (FAST, (20000, 3), 1), (FAST, (3, 20000), 2)):
+    for pol, shape, dim in ((EXACT, (3, 20000), 2), (EXACT, (20000, 3), 1), (FAST, (20000, 3), 1),
+                            (FAST, (3, 20000), 2), (FAST, (300, 200), 1)):
         xd = _dev(x.copy())
         want = ops.cumsum_dim(_dev(x), shape, dim, pol)
         ops.cumsum_dim(xd, shape, dim, pol, out=xd)
