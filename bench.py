@@ -202,6 +202,47 @@ def leg_batch(torch, O):
             "error_sample": "32 seeded tables"}
 
 
+def leg_dimwise(torch, ops, O):
+    """sum(a, dim) and cumsum(a, dim) of an 8192 x 8192 f32 table (256 MiB),
+    dimensions 1 and 2, exact (NT2's order) and fast policies; algorithmic
+    bytes: 4 per element for a sum, 8 for a scan.  Exact results are checked
+    bit for bit against the oracle, fast ones against an FP64 sum, and the
+    oracle's time (NT2's order on one host thread) is reported beside them."""
+    from nt2_amd import EXACT, FAST
+    n = 8192
+    ext = (n, n)
+    x = ops.fill_uniform(n * n, -1.0, 1.0, 81)
+    out = torch.empty(n * n, dtype=torch.float32, device="cuda")
+    xh = x.cpu().numpy()
+    res = {"table": "8192 x 8192 f32"}
+    for dim in (1, 2):
+        for pname, pol in (("exact", EXACT), ("fast", FAST)):
+            ms_s = time_steps(torch, lambda: ops.sum_dim(x, ext, dim, pol), 10, 2)
+            ms_c = time_steps(torch, lambda: ops.cumsum_dim(x, ext, dim, pol, out=out), 10, 2)
+            res["dim%d_%s" % (dim, pname)] = {
+                "sum_ms": ms_s, "sum_gbs": 4.0 * n * n / (ms_s * 1e-3) / 1e9,
+                "cumsum_ms": ms_c, "cumsum_gbs": 8.0 * n * n / (ms_c * 1e-3) / 1e9}
+        t0 = time.perf_counter()
+        want_s = O.sum_dim(xh, ext, dim)
+        cpu_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        want_c = O.cumsum_dim(xh, ext, dim)
+        cpu_c = time.perf_counter() - t0
+        got_s = ops.sum_dim(x, ext, dim, EXACT).cpu().numpy()
+        got_c = ops.cumsum_dim(x, ext, dim, EXACT).cpu().numpy()
+        a = xh.reshape(ext, order="F").astype(np.float64)
+        ref = a.sum(axis=dim - 1)
+        fast_s = ops.sum_dim(x, ext, dim, FAST).cpu().numpy().astype(np.float64)
+        res["dim%d_check" % dim] = {
+            "exact_bitwise_vs_oracle": bool(np.array_equal(got_s.view(np.uint32), want_s.view(np.uint32)) and
+                                            np.array_equal(got_c.view(np.uint32), want_c.view(np.uint32))),
+            "fast_sum_max_rel_vs_fp64": float(np.max(np.abs(fast_s - ref) / np.maximum(np.abs(ref), 1e-30))),
+            "oracle_sum_ms_1_thread": cpu_s * 1e3, "oracle_cumsum_ms_1_thread": cpu_c * 1e3}
+    del x, out
+    torch.cuda.empty_cache()
+    return res
+
+
 def leg_served(torch, ops):
     """Served Black-Scholes: 10,000 requests with sizes log-uniform in
     [2^10, 2^22] options, 80% f32 / 20% f64, windows of resident HBM tables,
@@ -353,6 +394,7 @@ def main():
         for name, fn in (("saxpy_sdot_2p20", lambda: leg_saxpy_sdot(torch, ops, O)),
                          ("sigmadelta_64x1920x1080", lambda: leg_sigmadelta(torch, ops)),
                          ("batched_4096_tables", lambda: leg_batch(torch, O)),
+                         ("dimwise_8192x8192", lambda: leg_dimwise(torch, ops, O)),
                          ("served_blackscholes", lambda: leg_served(torch, ops))):
             t0 = time.perf_counter()
             try:

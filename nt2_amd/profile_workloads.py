@@ -4,6 +4,7 @@
     python3 -m nt2_amd.profile_workloads fold    # saxpy + dot 2^20 and 2^26
     python3 -m nt2_amd.profile_workloads sd      # sigma-delta 64 x 1920x1080
     python3 -m nt2_amd.profile_workloads batch   # 4096 uneven tables, persistent (512x32, 2/CU) vs plain
+    python3 -m nt2_amd.profile_workloads dimwise # sum / cumsum of 8192 x 8192 f32, dims 1 and 2, both policies
 
 Each run also launches a 1 GiB streaming copy: the calibration kernel that
 turns FETCH_SIZE / WRITE_SIZE counter values into bytes.
@@ -69,8 +70,20 @@ def batch(reps=2, count=4096, seed=4):
     _copy()
 
 
+def dimwise(reps=2):
+    n = 8192
+    x = ops.fill_uniform(n * n, -1.0, 1.0, 81)
+    out = torch.empty_like(x)
+    for _ in range(reps):
+        for dim in (1, 2):
+            for pol in (EXACT, FAST):
+                ops.sum_dim(x, (n, n), dim, pol)
+                ops.cumsum_dim(x, (n, n), dim, pol, out=out)
+    _copy()
+
+
 if __name__ == "__main__":
     what = sys.argv[1] if len(sys.argv) > 1 else "bs"
-    {"bs": bs, "fold": fold, "sd": sd, "batch": batch}[what]()
+    {"bs": bs, "fold": fold, "sd": sd, "batch": batch, "dimwise": dimwise}[what]()
     torch.cuda.synchronize()
     print("profile_workloads %s done" % what, flush=True)

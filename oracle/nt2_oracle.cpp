@@ -876,26 +876,29 @@ void sum_dim(const T* x, const size_t* ext, int dim, T* out) {
       out[o] = r;
     }
   } else {
-    for (size_t o = 0; o < L.outer; ++o)
-      for (size_t i = 0; i < L.inner; ++i) {
-        T acc = T(0);
-        for (size_t m = 0; m < L.len; ++m) acc = acc + x[i + L.inner * (m + L.len * o)];
-        out[i + L.inner * o] = acc;
+    // NT2 runs this loop nest with m outside and i inside (details/outer_fold_step.hpp:51-59):
+    // each output is still summed in order over m
+    for (size_t o = 0; o < L.outer; ++o) {
+      T* acc = out + L.inner * o;
+      for (size_t i = 0; i < L.inner; ++i) acc[i] = T(0);
+      for (size_t m = 0; m < L.len; ++m) {
+        const T* row = x + L.inner * (m + L.len * o);
+        for (size_t i = 0; i < L.inner; ++i) acc[i] = acc[i] + row[i];
       }
+    }
   }
 }
 template <class T>
 void cumsum_dim(const T* x, const size_t* ext, int dim, T* out) {
   const Lines L = lines_of(ext, dim);
-  for (size_t o = 0; o < L.outer; ++o)
-    for (size_t i = 0; i < L.inner; ++i) {
-      T acc = T(0);
-      for (size_t m = 0; m < L.len; ++m) {
-        const size_t idx = i + L.inner * (m + L.len * o);
-        acc = acc + x[idx];
-        out[idx] = acc;
+  for (size_t o = 0; o < L.outer; ++o) {
+    const size_t b = L.inner * L.len * o;
+    for (size_t m = 0; m < L.len; ++m)  // each line summed in order over m
+      for (size_t i = 0; i < L.inner; ++i) {
+        const size_t idx = b + i + L.inner * m;
+        out[idx] = (m ? out[idx - L.inner] : T(0)) + x[idx];
       }
-    }
+  }
 }
 
 }  // namespace
