@@ -54,6 +54,16 @@ extern "C" {
 #define NT2H_OP_COS 6
 #define NT2H_OP_SIN 7
 #define NT2H_OP_ABS 8
+#define NT2H_OP_EXP2 9
+#define NT2H_OP_EXP10 10
+#define NT2H_OP_LOG2 11
+#define NT2H_OP_LOG10 12
+#define NT2H_OP_FLOOR 13
+#define NT2H_OP_CEIL 14
+#define NT2H_OP_TRUNC 15
+#define NT2H_OP_ROUND 16      /* half away from zero */
+#define NT2H_OP_ROUND2EVEN 17 /* half to even */
+#define NT2H_OP_SIGN 18
 
 /* Black-Scholes formula variants */
 #define NT2H_BS_TEXTBOOK 0 /* d1 = (log(S/X)+(r+v^2/2)T)/(v sqrt T), call and put, erfc-based normcdf */

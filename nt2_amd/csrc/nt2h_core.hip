@@ -151,6 +151,16 @@ __device__ __forceinline__ void apply_f32(const float* x, float* y) {
       else if constexpr (OP == NT2H_OP_ERFC) r = dm::erfc_f<P>(a);
       else if constexpr (OP == NT2H_OP_NORMCDF) r = dm::normcdf_f<P>(a);
       else if constexpr (OP == NT2H_OP_FASTNORMCDF) r = dm::fastnormcdf_f<P>(a);
+      else if constexpr (OP == NT2H_OP_EXP2) r = dm::exp2_f<P>(a);
+      else if constexpr (OP == NT2H_OP_EXP10) r = dm::exp10_f<P>(a);
+      else if constexpr (OP == NT2H_OP_LOG2) r = dm::log2_f<P>(a);
+      else if constexpr (OP == NT2H_OP_LOG10) r = dm::log10_f<P>(a);
+      else if constexpr (OP == NT2H_OP_FLOOR) r = __builtin_floorf(a);
+      else if constexpr (OP == NT2H_OP_CEIL) r = __builtin_ceilf(a);
+      else if constexpr (OP == NT2H_OP_TRUNC) r = __builtin_truncf(a);
+      else if constexpr (OP == NT2H_OP_ROUND) r = dm::round_nt2(a);
+      else if constexpr (OP == NT2H_OP_ROUND2EVEN) r = __builtin_rintf(a);
+      else if constexpr (OP == NT2H_OP_SIGN) r = dm::sign_nt2(a);
       else r = __builtin_fabsf(a);
       y[l] = r;
     }
@@ -172,6 +182,16 @@ __device__ __forceinline__ void apply_f64(const double* x, double* y) {
       else if constexpr (OP == NT2H_OP_SQRT) r = __builtin_sqrt(a);
       else if constexpr (OP == NT2H_OP_ERFC) r = dm::erfc_d<P, scalar>(a);
       else if constexpr (OP == NT2H_OP_NORMCDF) r = dm::normcdf_d<P, scalar>(a);
+      else if constexpr (OP == NT2H_OP_EXP2) r = dm::exp2_d<P>(a);
+      else if constexpr (OP == NT2H_OP_EXP10) r = dm::exp10_d<P>(a);
+      else if constexpr (OP == NT2H_OP_LOG2) r = dm::log2_d<P>(a);
+      else if constexpr (OP == NT2H_OP_LOG10) r = dm::log10_d<P>(a);
+      else if constexpr (OP == NT2H_OP_FLOOR) r = __builtin_floor(a);
+      else if constexpr (OP == NT2H_OP_CEIL) r = __builtin_ceil(a);
+      else if constexpr (OP == NT2H_OP_TRUNC) r = __builtin_trunc(a);
+      else if constexpr (OP == NT2H_OP_ROUND) r = dm::round_nt2(a);
+      else if constexpr (OP == NT2H_OP_ROUND2EVEN) r = __builtin_rint(a);
+      else if constexpr (OP == NT2H_OP_SIGN) r = dm::sign_nt2(a);
       else r = __builtin_fabs(a);
       y[l] = r;
     }
@@ -740,6 +760,11 @@ static int launch_map_f32(int op, size_t n, const float* x, float* y, hipStream_
     case NT2H_OP_COS: hipLaunchKernelGGL((map_f32_k<P, NT2H_OP_COS>), g, b, 0, s, x, y, n, vec); break;
     case NT2H_OP_SIN: hipLaunchKernelGGL((map_f32_k<P, NT2H_OP_SIN>), g, b, 0, s, x, y, n, vec); break;
     case NT2H_OP_ABS: hipLaunchKernelGGL((map_f32_k<P, NT2H_OP_ABS>), g, b, 0, s, x, y, n, vec); break;
+#define NT2H_MAP32(OPC) case OPC: hipLaunchKernelGGL((map_f32_k<P, OPC>), g, b, 0, s, x, y, n, vec); break;
+    NT2H_MAP32(NT2H_OP_EXP2) NT2H_MAP32(NT2H_OP_EXP10) NT2H_MAP32(NT2H_OP_LOG2) NT2H_MAP32(NT2H_OP_LOG10)
+    NT2H_MAP32(NT2H_OP_FLOOR) NT2H_MAP32(NT2H_OP_CEIL) NT2H_MAP32(NT2H_OP_TRUNC) NT2H_MAP32(NT2H_OP_ROUND)
+    NT2H_MAP32(NT2H_OP_ROUND2EVEN) NT2H_MAP32(NT2H_OP_SIGN)
+#undef NT2H_MAP32
     default: return fail(NT2H_EINVAL, "nt2h_map_f32: unknown op");
   }
   NT2H_CHECK_LAUNCH("map_f32");
@@ -760,6 +785,11 @@ static int launch_map_f64(int op, size_t n, const double* x, double* y, hipStrea
     case NT2H_OP_COS: hipLaunchKernelGGL((map_f64_k<P, NT2H_OP_COS>), g, b, 0, s, x, y, n, vec); break;
     case NT2H_OP_SIN: hipLaunchKernelGGL((map_f64_k<P, NT2H_OP_SIN>), g, b, 0, s, x, y, n, vec); break;
     case NT2H_OP_ABS: hipLaunchKernelGGL((map_f64_k<P, NT2H_OP_ABS>), g, b, 0, s, x, y, n, vec); break;
+#define NT2H_MAP64(OPC) case OPC: hipLaunchKernelGGL((map_f64_k<P, OPC>), g, b, 0, s, x, y, n, vec); break;
+    NT2H_MAP64(NT2H_OP_EXP2) NT2H_MAP64(NT2H_OP_EXP10) NT2H_MAP64(NT2H_OP_LOG2) NT2H_MAP64(NT2H_OP_LOG10)
+    NT2H_MAP64(NT2H_OP_FLOOR) NT2H_MAP64(NT2H_OP_CEIL) NT2H_MAP64(NT2H_OP_TRUNC) NT2H_MAP64(NT2H_OP_ROUND)
+    NT2H_MAP64(NT2H_OP_ROUND2EVEN) NT2H_MAP64(NT2H_OP_SIGN)
+#undef NT2H_MAP64
     default: return fail(NT2H_EINVAL, "nt2h_map_f64: unknown op (fastnormcdf is single precision only)");
   }
   NT2H_CHECK_LAUNCH("map_f64");

@@ -181,6 +181,208 @@ NT2H_FN double log_d(double a0) {
 }
 
 // ---------------------------------------------------------------------------
+// exp2, exp10 (NT2's expa with the two_tag / ten_tag reductions):
+// f32 scalar/impl/expo/f_expo_reduction.hpp:90-167, f64 d_expo_reduction.hpp:87-166,
+// both finished as simd/common/impl/expo/expo_base.hpp:31-61 (the scale of
+// expo_scale.hpp:26-35, zero at or below Minlog*, Inf at or above Maxlog*).
+// ---------------------------------------------------------------------------
+NT2H_FN float expa_finish_f(float a0, float r, float k, float minlog, float maxlog) {
+  NT2H_STRICT
+  const float kc = __builtin_fminf(__builtin_fmaxf(k, -200.0f), 200.0f);
+  r = r * as_f((uint32_t)((int32_t)kc + 127) << 23);
+  r = (a0 <= minlog) ? 0.0f : r;
+  r = (a0 >= maxlog) ? __builtin_inff() : r;
+  return (a0 != a0) ? a0 : r;
+}
+NT2H_FN double expa_finish_d(double a0, double c, double k, double minlog, double maxlog) {
+  NT2H_STRICT
+  const double kc = __builtin_fmin(__builtin_fmax(k, -2000.0), 2000.0);
+  c = c * as_d((uint64_t)((int64_t)kc + 1023) << 52);
+  c = (a0 <= minlog) ? 0.0 : c;
+  c = (a0 >= maxlog) ? __builtin_inf() : c;
+  return (a0 != a0) ? a0 : c;
+}
+template <class P>
+NT2H_FN float exp2_f(float a0) {
+  NT2H_STRICT
+  constexpr float c[5] = {NT2H_F(0x3922c8c4), NT2H_F(0x3ab01218), NT2H_F(0x3c1d9422),
+                          NT2H_F(0x3d6356eb), NT2H_F(0x3e75fdf1)};
+  const float k = __builtin_rintf(a0);
+  const float x = a0 - k;
+  const float y = horner<P>(c, x);
+  const float r = fma<P>(y, x * x, x * NT2H_F(0x3f317218)) + 1.0f;
+  return expa_finish_f(a0, r, k, NT2H_F(0xc2fe0000), NT2H_F(0x42fe0000));
+}
+template <class P>
+NT2H_FN float exp10_f(float a0) {
+  NT2H_STRICT
+  constexpr float c[6] = {NT2H_F(0x3e54dff1), NT2H_F(0x3f0aacef), NT2H_F(0x3f95eb4c),
+                          NT2H_F(0x400237da), NT2H_F(0x4029a926), NT2H_F(0x40135d8e)};
+  const float k = __builtin_rintf(NT2H_F(0x40549a78) * a0);
+  float x = fnms<P>(k, NT2H_F(0x3e9a0000), a0);
+  x = x - k * NT2H_F(0x39826a14);
+  const float r = horner<P>(c, x) * x + 1.0f;
+  return expa_finish_f(a0, r, k, NT2H_F(0xc2179999), NT2H_F(0x4218ec59));
+}
+template <class P>
+NT2H_FN double exp2_d(double a0) {
+  NT2H_STRICT
+  constexpr double cExp[5] = {NT2H_D(0x3e66376972bea4d0), NT2H_D(0xbebbbd41c5d26bf1),
+                              NT2H_D(0x3f11566aaf25de2c), NT2H_D(0xbf66c16c16bebd93),
+                              NT2H_D(0x3fc555555555553e)};
+  const double k = __builtin_rint(a0);
+  const double x = (a0 - k) * NT2H_D(0x3fe62e42fefa39ef);
+  const double t = x * x;
+  double c = fnms<P>(t, horner<P>(cExp, t), x);
+  c = 1.0 - ((-(x * c) / (2.0 - c)) - x);
+  return expa_finish_d(a0, c, k, NT2H_D(0xc08ff00000000000), NT2H_D(0x408ff80000000000));
+}
+template <class P>
+NT2H_FN double exp10_d(double a0) {
+  NT2H_STRICT
+  constexpr double cP[4] = {NT2H_D(0x3fa4fd75f3062dd4), NT2H_D(0x40277d9474c55934),
+                            NT2H_D(0x40796b7a050349e4), NT2H_D(0x40a2b4798e134a01)};
+  constexpr double cQ[4] = {NT2H_D(0x3ff0000000000000), NT2H_D(0x405545fdce51ca08),
+                            NT2H_D(0x4093e05eefd67782), NT2H_D(0x40a03f37650df6e2)};
+  const double k = __builtin_rint(NT2H_D(0x400a934f0979a372) * a0);
+  double x = fnms<P>(k, NT2H_D(0x3fd3440000000000), a0);
+  x = fnms<P>(k, NT2H_D(0x3ed3509f79fef312), x);
+  const double xx = x * x;
+  const double px = x * horner<P>(cP, xx);
+  const double x2 = px / (horner<P>(cQ, xx) - px);
+  const double c = 1.0 + (x2 + x2);
+  return expa_finish_d(a0, c, k, NT2H_D(0xc0734413509f79fe), NT2H_D(0x40734413509f79fe));
+}
+
+// ---------------------------------------------------------------------------
+// log2, log10.  f32 simd/common/impl/logs/f_log.hpp:128-174 on the kernel of
+// scalar/impl/logs/f_kernel.hpp:47-78 (denormals prescaled by 2^23 with the
+// matching constant added); f64 simd/common/impl/logs/d_log.hpp:58-110.
+// The results are finished like log (NaN below 0 and for NaN, Inf at Inf,
+// -Inf at 0).
+// ---------------------------------------------------------------------------
+template <class P>
+NT2H_FN void log_kernel_f(float z, float& fe, float& x, float& x2, float& y) {
+  NT2H_STRICT
+  constexpr float cLog[8] = {NT2H_F(0xbda5dff0), NT2H_F(0x3e0229f9), NT2H_F(0xbe04d6b7),
+                             NT2H_F(0x3e116e80), NT2H_F(0xbe2a6aa0), NT2H_F(0x3e4cd0a3),
+                             NT2H_F(0xbe800064), NT2H_F(0x3eaaaaa9)};
+  const uint32_t b = bits(z);
+  const int32_t e = (int32_t)((b & 0x7f800000u) >> 23) - 126;
+  const float m = as_f((b & ~0x7f800000u) | 0x3f000000u);
+  const bool lt = m < NT2H_F(0x3f3504f3);
+  fe = lt ? (float)e + (-1.0f) : (float)e;
+  x = (lt ? m + m : m) - 1.0f;
+  x2 = x * x;
+  y = (horner<P>(cLog, x) * x) * x2;
+}
+NT2H_FN float log_finish_f(float a0, float r) {
+  r = (a0 < 0.0f || a0 != a0) ? __builtin_nanf("") : r;
+  r = (a0 == __builtin_inff()) ? a0 : r;
+  return (a0 == 0.0f) ? -__builtin_inff() : r;
+}
+template <class P>
+NT2H_FN float log2_f(float a0) {
+  NT2H_STRICT
+  const bool den = __builtin_fabsf(a0) < NT2H_F(0x00800000);
+  const float z0 = den ? a0 * NT2H_F(0x4B000000) : a0;
+  const float t = den ? NT2H_F(0xc1b80000) : 0.0f;
+  float fe, x, x2, y;
+  log_kernel_f<P>(z0, fe, x, x2, y);
+  y = fma<P>(-0.5f, x2, y);
+  const float z = fma<P>(x, NT2H_F(0x3ee2a8ed), y * NT2H_F(0x3ee2a8ed));
+  return log_finish_f(a0, (((z + y) + x) + fe) + t);
+}
+template <class P>
+NT2H_FN float log10_f(float a0) {
+  NT2H_STRICT
+  const bool den = __builtin_fabsf(a0) < NT2H_F(0x00800000);
+  const float z0 = den ? a0 * NT2H_F(0x4B000000) : a0;
+  const float t = den ? NT2H_F(0xc0dd8ede) : 0.0f;
+  float fe, x, x2, y;
+  log_kernel_f<P>(z0, fe, x, x2, y);
+  y = fma<P>(x2, -0.5f, y);
+  float z = (x + y) * NT2H_F(0x3a37b152);
+  z = fma<P>(y, NT2H_F(0x3ede0000), z);
+  z = fma<P>(x, NT2H_F(0x3ede0000), z);
+  z = fma<P>(fe, NT2H_F(0x3e9a0000), z);
+  return log_finish_f(a0, fma<P>(fe, NT2H_F(0x39826a14), z + t));
+}
+// f64: the kernel of log_d, then -(hfsq - s (hfsq + R) - f) scaled
+template <class P>
+NT2H_FN double log_poly_d(double a0, double& dk) {
+  NT2H_STRICT
+  constexpr double cT1[3] = {NT2H_D(0x3fc39a09d078c69f), NT2H_D(0x3fcc71c51d8e78af),
+                             NT2H_D(0x3fd999999997fa04)};
+  constexpr double cT2[4] = {NT2H_D(0x3fc2f112df3e5244), NT2H_D(0x3fc7466496cb03de),
+                             NT2H_D(0x3fd2492494229359), NT2H_D(0x3fe5555555555593)};
+  const uint64_t b = bits(a0);
+  int64_t k = (int64_t)((b & 0x7ff0000000000000ull) >> 52) - 1022;
+  const double x = as_d((b & ~0x7ff0000000000000ull) | 0x3fe0000000000000ull);
+  const bool lt = NT2H_D(0x3FE6A09E667F3BCD) > x;
+  k = k + (lt ? -1 : 0);
+  const double f = (x + (lt ? x : 0.0)) - 1.0;
+  dk = (double)k;
+  const double s = f / (2.0 + f);
+  const double z = s * s;
+  const double w = z * z;
+  const double t1 = w * horner<P>(cT1, w);
+  const double t2 = z * horner<P>(cT2, w);
+  const double R = t2 + t1;
+  const double hfsq = 0.5 * (f * f);
+  return -((hfsq - (s * (hfsq + R))) - f);
+}
+NT2H_FN double log_finish_d(double a0, double r) {
+  r = (a0 < 0.0 || a0 != a0) ? __builtin_nan("") : r;
+  r = (a0 == __builtin_inf()) ? a0 : r;
+  return (a0 == 0.0) ? -__builtin_inf() : r;
+}
+template <class P>
+NT2H_FN double log2_d(double a0) {
+  NT2H_STRICT
+  double dk;
+  const double u = log_poly_d<P>(a0, dk);
+  return log_finish_d(a0, u * NT2H_D(0x3ff71547652b82fe) + dk);
+}
+template <class P>
+NT2H_FN double log10_d(double a0) {
+  NT2H_STRICT
+  double dk;
+  const double u = log_poly_d<P>(a0, dk);
+  return log_finish_d(a0, u * NT2H_D(0x3fdbcb7b1526e50d) + dk * NT2H_D(0x3fd34413509f79ff));
+}
+
+// ---------------------------------------------------------------------------
+// rounding and sign.  floor, ceil, trunc, round2even are the IEEE operations;
+// round is half away from zero as boost/simd/arithmetic/functions/simd/common/round.hpp:45-52
+// computes it (c = ceil(|a|), c - 1 when c - 1/2 > |a|, the sign of a);
+// sign is (a > 0) - (a < 0), NaN for NaN
+// (boost/simd/ieee/functions/simd/common/sign.hpp:55-66).
+// ---------------------------------------------------------------------------
+NT2H_FN float round_nt2(float a0) {
+  NT2H_STRICT
+  const float v = __builtin_fabsf(a0);
+  const float c = __builtin_ceilf(v);
+  const float r = (c - 0.5f > v) ? c - 1.0f : c;
+  return __builtin_copysignf(r, a0);
+}
+NT2H_FN double round_nt2(double a0) {
+  NT2H_STRICT
+  const double v = __builtin_fabs(a0);
+  const double c = __builtin_ceil(v);
+  const double r = (c - 0.5 > v) ? c - 1.0 : c;
+  return __builtin_copysign(r, a0);
+}
+NT2H_FN float sign_nt2(float a0) {
+  const float r = (float)(a0 > 0.0f) - (float)(a0 < 0.0f);
+  return (a0 != a0) ? as_f(0xffffffffu) : r;
+}
+NT2H_FN double sign_nt2(double a0) {
+  const double r = (double)(a0 > 0.0) - (double)(a0 < 0.0);
+  return (a0 != a0) ? as_d(0xffffffffffffffffull) : r;
+}
+
+// ---------------------------------------------------------------------------
 // erfc.  f32: modules/core/euler/include/nt2/euler/functions/simd/common/erfc.hpp:92-123
 // (the scalar form returns the same values), kernels erf_kernel.hpp:46-85.
 // erfc_abs_f returns erfc(|a|) before the sign fold, so erfc(a) and erfc(-a)

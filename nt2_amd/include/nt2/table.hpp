@@ -191,6 +191,9 @@ NT2H_TAG(eq_, logical2) NT2H_TAG(neq_, logical2)
 NT2H_TAG(neg_, same1) NT2H_TAG(exp_, same1) NT2H_TAG(log_, same1) NT2H_TAG(sqrt_, same1)
 NT2H_TAG(erfc_, same1) NT2H_TAG(normcdf_, same1) NT2H_TAG(fastnormcdf_, same1) NT2H_TAG(cos_, same1)
 NT2H_TAG(sin_, same1) NT2H_TAG(abs_, same1) NT2H_TAG(sqr_, same1) NT2H_TAG(rec_, same1)
+NT2H_TAG(exp2_, same1) NT2H_TAG(exp10_, same1) NT2H_TAG(log2_, same1) NT2H_TAG(log10_, same1)
+NT2H_TAG(floor_, same1) NT2H_TAG(ceil_, same1) NT2H_TAG(trunc_, same1) NT2H_TAG(round_, same1)
+NT2H_TAG(round2even_, same1) NT2H_TAG(sign_, same1)
 NT2H_TAG(fma_, same3) NT2H_TAG(fnms_, same3) NT2H_TAG(if_else_, sel3)
 NT2H_TAG(selinc_, selinc2) NT2H_TAG(seldec_, selinc2) NT2H_TAG(if_zero_else_one_, izeo1)
 #undef NT2H_TAG
@@ -216,6 +219,13 @@ __device__ __forceinline__ void load(const expr::scal_v<T>& s, size_t, T (&r)[G]
 
 template <class T> struct is_f { static constexpr bool value = std::is_floating_point<T>::value; };
 
+// Functions with no integer form here fail to compile on integer tables
+// instead of producing a value.
+template <class T> __device__ __forceinline__ T no_integer_form() {
+  static_assert(sizeof(T) == 0, "this function is defined for floating-point tables only");
+  return T();
+}
+
 template <class Tag> struct apply1;
 #define NT2H_UN(TAG, EXPR_F, EXPR_D, EXPR_I)                                         \
   template <> struct apply1<tagfn::TAG> {                                            \
@@ -230,15 +240,29 @@ template <class Tag> struct apply1;
     }                                                                                \
   };
 NT2H_UN(neg_, -x, -x, -x)
-NT2H_UN(exp_, dm::exp_f<P>(x), dm::exp_d<P>(x), 0)
-NT2H_UN(log_, dm::log_f<P>(x), dm::log_d<P>(x), 0)
-NT2H_UN(sqrt_, __builtin_sqrtf(x), __builtin_sqrt(x), 0)
-NT2H_UN(erfc_, dm::erfc_f<P>(x), (G == 1 ? dm::erfc_d<P, true>(x) : dm::erfc_d<P, false>(x)), 0)
-NT2H_UN(normcdf_, dm::normcdf_f<P>(x), (G == 1 ? dm::normcdf_d<P, true>(x) : dm::normcdf_d<P, false>(x)), 0)
-NT2H_UN(fastnormcdf_, dm::fastnormcdf_f<P>(x), (double)dm::fastnormcdf_f<P>((float)x), 0)
+NT2H_UN(exp_, dm::exp_f<P>(x), dm::exp_d<P>(x), no_integer_form<T>())
+NT2H_UN(log_, dm::log_f<P>(x), dm::log_d<P>(x), no_integer_form<T>())
+NT2H_UN(sqrt_, __builtin_sqrtf(x), __builtin_sqrt(x), no_integer_form<T>())
+NT2H_UN(erfc_, dm::erfc_f<P>(x), (G == 1 ? dm::erfc_d<P, true>(x) : dm::erfc_d<P, false>(x)), no_integer_form<T>())
+NT2H_UN(normcdf_, dm::normcdf_f<P>(x), (G == 1 ? dm::normcdf_d<P, true>(x) : dm::normcdf_d<P, false>(x)),
+        no_integer_form<T>())
+NT2H_UN(fastnormcdf_, dm::fastnormcdf_f<P>(x), (double)dm::fastnormcdf_f<P>((float)x), no_integer_form<T>())
 NT2H_UN(abs_, __builtin_fabsf(x), __builtin_fabs(x), x)
 NT2H_UN(sqr_, x * x, x * x, x * x)
-NT2H_UN(rec_, 1.0f / x, 1.0 / x, 0)
+NT2H_UN(rec_, 1.0f / x, 1.0 / x, no_integer_form<T>())
+// exp2, exp10, log2, log10: NT2's expa / logarithm kernels (math.hpp)
+NT2H_UN(exp2_, dm::exp2_f<P>(x), dm::exp2_d<P>(x), no_integer_form<T>())
+NT2H_UN(exp10_, dm::exp10_f<P>(x), dm::exp10_d<P>(x), no_integer_form<T>())
+NT2H_UN(log2_, dm::log2_f<P>(x), dm::log2_d<P>(x), no_integer_form<T>())
+NT2H_UN(log10_, dm::log10_f<P>(x), dm::log10_d<P>(x), no_integer_form<T>())
+// rounding is the identity on integers (boost/simd/arithmetic/functions/scalar/
+// round.hpp:26-34 and the floor/ceil/trunc/round2even integer forms)
+NT2H_UN(floor_, __builtin_floorf(x), __builtin_floor(x), x)
+NT2H_UN(ceil_, __builtin_ceilf(x), __builtin_ceil(x), x)
+NT2H_UN(trunc_, __builtin_truncf(x), __builtin_trunc(x), x)
+NT2H_UN(round_, dm::round_nt2(x), dm::round_nt2(x), x)
+NT2H_UN(round2even_, __builtin_rintf(x), __builtin_rint(x), x)
+NT2H_UN(sign_, dm::sign_nt2(x), dm::sign_nt2(x), (x > T(0)) - (x < T(0)))
 #undef NT2H_UN
 
 template <> struct apply1<tagfn::cos_> {
@@ -495,6 +519,9 @@ template <class A, NT2H_ENABLE1(A)> auto operator-(const A& a) { return make1<ta
 NT2H_FN1(exp, exp_) NT2H_FN1(log, log_) NT2H_FN1(sqrt, sqrt_) NT2H_FN1(erfc, erfc_)
 NT2H_FN1(normcdf, normcdf_) NT2H_FN1(fastnormcdf, fastnormcdf_) NT2H_FN1(cos, cos_)
 NT2H_FN1(sin, sin_) NT2H_FN1(abs, abs_) NT2H_FN1(sqr, sqr_) NT2H_FN1(rec, rec_)
+NT2H_FN1(exp2, exp2_) NT2H_FN1(exp10, exp10_) NT2H_FN1(log2, log2_) NT2H_FN1(log10, log10_)
+NT2H_FN1(floor, floor_) NT2H_FN1(ceil, ceil_) NT2H_FN1(trunc, trunc_) NT2H_FN1(round, round_)
+NT2H_FN1(round2even, round2even_) NT2H_FN1(sign, sign_)
 NT2H_FN1(if_zero_else_one, if_zero_else_one_)
 #undef NT2H_FN1
 #define NT2H_FN2(name, Tag) \

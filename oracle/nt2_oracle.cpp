@@ -238,6 +238,164 @@ inline double log_d(double a0) {
 }
 
 // ===========================================================================
+// exp2, exp10, log2, log10
+// ===========================================================================
+// NT2's expa with the two_tag and ten_tag reductions.  f32:
+// scalar/impl/expo/f_expo_reduction.hpp:88-121 (two: k = round2even(a0),
+// x = a0 - k, 1 + (y x^2 + x Log_2) with y a degree-4 Remez polynomial) and
+// :123-165 (ten: k = round2even(Invlog10_2 a0), x = a0 - k Log10_2hi - k Log10_2lo,
+// 1 + P(x) x); limits Minlog2/Maxlog2, Minlog10/Maxlog10 (constants/minlog2.hpp:29,
+// maxlog2.hpp:29, minlog10.hpp:29, maxlog10.hpp:29).  Special values and scale as
+// exp above (expo_base.hpp:47-60; scalar expo_base.hpp:56-68 gives the same values).
+inline float pow2k_scale_f(float a0, float r, float k, float lo, float hi) {
+  float kc = std::fmin(std::fmax(k, -200.0f), 200.0f);
+  r = r * as_f((uint32_t)((int32_t)kc + 127) << 23);
+  if (a0 <= lo) r = 0.0f;
+  if (a0 >= hi) r = INFINITY;
+  return (a0 != a0) ? a0 : r;
+}
+inline double pow2k_scale_d(double a0, double c, double k, double lo, double hi) {
+  double kc = std::fmin(std::fmax(k, -2000.0), 2000.0);
+  c = c * as_d((uint64_t)((int64_t)kc + 1023) << 52);
+  if (a0 <= lo) c = 0.0;
+  if (a0 >= hi) c = (double)INFINITY;
+  return (a0 != a0) ? a0 : c;
+}
+const float fExp2C[5] = {F32(0x3922c8c4), F32(0x3ab01218), F32(0x3c1d9422),
+                         F32(0x3d6356eb), F32(0x3e75fdf1)};
+const float fExp10C[6] = {F32(0x3e54dff1), F32(0x3f0aacef), F32(0x3f95eb4c),
+                          F32(0x400237da), F32(0x4029a926), F32(0x40135d8e)};
+inline float exp2_f(float a0) {
+  float k = r2e(a0);
+  float x = a0 - k;
+  float y = horner(fExp2C, x);
+  float r = (y * (x * x) + x * F32(0x3f317218)) + 1.0f;
+  return pow2k_scale_f(a0, r, k, F32(0xc2fe0000), F32(0x42fe0000));
+}
+inline float exp10_f(float a0) {
+  float k = r2e(F32(0x40549a78) * a0);
+  float x = a0 - k * F32(0x3e9a0000);
+  x = x - k * F32(0x39826a14);
+  float r = horner(fExp10C, x) * x + 1.0f;
+  return pow2k_scale_f(a0, r, k, F32(0xc2179999), F32(0x4218ec59));
+}
+// f64: scalar/impl/expo/d_expo_reduction.hpp:85-123 (two: x = (a0 - k) Log_2, the
+// exp approximation and finalize 1 - ((-(x c)/(2 - c)) - x)) and :125-170 (ten:
+// the rational P/Q form, 1 + 2 x P/(Q - x P)); expo_base.hpp:31-42.
+const double dExp10P[4] = {F64(0x3fa4fd75f3062dd4), F64(0x40277d9474c55934),
+                           F64(0x40796b7a050349e4), F64(0x40a2b4798e134a01)};
+const double dExp10Q[4] = {F64(0x3ff0000000000000), F64(0x405545fdce51ca08),
+                           F64(0x4093e05eefd67782), F64(0x40a03f37650df6e2)};
+inline double exp2_d(double a0) {
+  double k = r2e(a0);
+  double x = (a0 - k) * F64(0x3fe62e42fefa39ef);
+  double t = x * x;
+  double c = x - t * horner(dExpC, t);
+  c = 1.0 - ((-(x * c) / (2.0 - c)) - x);
+  return pow2k_scale_d(a0, c, k, F64(0xc08ff00000000000), F64(0x408ff80000000000));
+}
+inline double exp10_d(double a0) {
+  double k = r2e(F64(0x400a934f0979a372) * a0);
+  double x = a0 - k * F64(0x3fd3440000000000);
+  x = x - k * F64(0x3ed3509f79fef312);
+  double xx = x * x;
+  double px = x * horner(dExp10P, xx);
+  double x2 = px / (horner(dExp10Q, xx) - px);
+  return pow2k_scale_d(a0, 1.0 + (x2 + x2), k, F64(0xc0734413509f79fe), F64(0x40734413509f79fe));
+}
+// f32 log2 / log10: simd/common/impl/logs/f_log.hpp:127-171 on the f_kernel.hpp
+// kernel (the same lines as log_f up to y); denormals are scaled by 2^23 with
+// Mlog2two2nmb = -23 / Mlog10two2nmb = -23 log10(2) added
+// (constants/mlog2two2nmb.hpp:29, mlog10two2nmb.hpp:29).  The scalar variant
+// (scalar/impl/logs/f_log.hpp:83-153) computes the same expressions.
+inline void log_kernel_f(float a0, float tden, float& fe, float& x, float& x2, float& y, float& t) {
+  float z = a0;
+  t = 0.0f;
+  if (std::fabs(z) < fSmallest) { z = z * fTwotonmb; t = tden; }
+  uint32_t b = bits(z);
+  int32_t e = (int32_t)((b & 0x7f800000u) >> 23) - 126;
+  float m = as_f((b & ~0x7f800000u) | 0x3f000000u);
+  bool lt = m < fSqrt_2o_2;
+  fe = lt ? (float)e + (-1.0f) : (float)e;
+  x = (lt ? m + m : m) - 1.0f;
+  x2 = x * x;
+  y = (horner(fLogC, x) * x) * x2;
+}
+inline float log_special_f(float a0, float r) {
+  if (a0 < 0.0f || a0 != a0) r = NAN;
+  if (a0 == INFINITY) r = a0;
+  if (a0 == 0.0f) r = -INFINITY;
+  return r;
+}
+inline float log2_f(float a0) {
+  float fe, x, x2, y, t;
+  log_kernel_f(a0, F32(0xc1b80000), fe, x, x2, y, t);
+  y = -0.5f * x2 + y;
+  const float l2em1 = F32(0x3ee2a8ed);  // log2(e) - 1
+  float z = x * l2em1 + y * l2em1;
+  return log_special_f(a0, (((z + y) + x) + fe) + t);
+}
+inline float log10_f(float a0) {
+  float fe, x, x2, y, t;
+  log_kernel_f(a0, F32(0xc0dd8ede), fe, x, x2, y, t);
+  y = -0.5f * x2 + y;                                  // amul(y, Mhalf, x2)
+  float z = (x + y) * F32(0x3a37b152);                 // Log10_elo
+  z = y * F32(0x3ede0000) + z;                         // Log10_ehi
+  z = x * F32(0x3ede0000) + z;
+  z = fe * F32(0x3e9a0000) + z;                        // Log10_2hi
+  return log_special_f(a0, fe * F32(0x39826a14) + (z + t));  // Log10_2lo
+}
+// f64 log2 / log10: simd/common/impl/logs/d_log.hpp:101-115 (and scalar
+// d_log.hpp:107-137, the same expressions): the log kernel, then
+// -((hfsq - s (hfsq + R)) - f) times Invlog_2 plus k, or times Invlog_10 plus
+// k Log_2olog_10.
+inline double log_frac_d(double a0, double& dk) {
+  uint64_t b = bits(a0);
+  int64_t k = (int64_t)((b & 0x7ff0000000000000ull) >> 52) - 1022;
+  double x = as_d((b & ~0x7ff0000000000000ull) | 0x3fe0000000000000ull);
+  bool lt = dSqrt_2o_2 > x;
+  k = k + (lt ? -1 : 0);
+  double f = (x + (lt ? x : 0.0)) - 1.0;
+  dk = (double)k;
+  double s = f / (2.0 + f);
+  double z = s * s;
+  double w = z * z;
+  double R = z * horner(dLogT2, w) + w * horner(dLogT1, w);
+  double hfsq = 0.5 * (f * f);
+  return -((hfsq - s * (hfsq + R)) - f);
+}
+inline double log_special_d(double a0, double r) {
+  if (a0 < 0.0 || a0 != a0) r = NAN;
+  if (a0 == (double)INFINITY) r = a0;
+  if (a0 == 0.0) r = -(double)INFINITY;
+  return r;
+}
+inline double log2_d(double a0) {
+  double dk;
+  double u = log_frac_d(a0, dk);
+  return log_special_d(a0, u * F64(0x3ff71547652b82fe) + dk);
+}
+inline double log10_d(double a0) {
+  double dk;
+  double u = log_frac_d(a0, dk);
+  return log_special_d(a0, u * F64(0x3fdbcb7b1526e50d) + dk * F64(0x3fd34413509f79ff));
+}
+
+// ===========================================================================
+// rounding and sign
+// ===========================================================================
+// floor / ceil / trunc / round2even are the IEEE operations (SSE4.1/AVX
+// roundps with the matching mode).  round: half away from zero,
+// modules/boost/simd/base/include/boost/simd/arithmetic/functions/scalar/round.hpp:30-44
+// (::roundf / ::round, or the ceil-based form, which agree for every input).
+// sign: boost/simd/ieee/functions/simd/common/sign.hpp:55-66, (a > 0) - (a < 0),
+// NaN for NaN.
+template <class T> inline T sign_of(T a) {
+  if (a != a) return a;
+  return (T)(a > T(0)) - (T)(a < T(0));
+}
+
+// ===========================================================================
 // erfc, normcdf, fastnormcdf
 // ===========================================================================
 // f32: modules/core/euler/include/nt2/euler/functions/simd/common/erfc.hpp:92-123
@@ -910,7 +1068,9 @@ extern "C" {
 
 enum {
   OR_EXP = 0, OR_LOG = 1, OR_SQRT = 2, OR_ERFC = 3, OR_NORMCDF = 4,
-  OR_FASTNORMCDF = 5, OR_COS = 6, OR_SIN = 7, OR_ABS = 8
+  OR_FASTNORMCDF = 5, OR_COS = 6, OR_SIN = 7, OR_ABS = 8, OR_EXP2 = 9, OR_EXP10 = 10,
+  OR_LOG2 = 11, OR_LOG10 = 12, OR_FLOOR = 13, OR_CEIL = 14, OR_TRUNC = 15, OR_ROUND = 16,
+  OR_ROUND2EVEN = 17, OR_SIGN = 18, OR_LAST = OR_SIGN
 };
 
 int oracle_abi_version(void) { return 1; }
@@ -927,7 +1087,7 @@ int oracle_get_threads(void) { return omp_get_max_threads(); }
 
 // y = f(x) over a table of n floats, NT2 transform semantics
 int oracle_map_f32(int op, const float* x, float* y, size_t n) {
-  if (op < 0 || op > OR_ABS) return -1;
+  if (op < 0 || op > OR_LAST) return -1;
   nt2_transform(n, 8, sizeof(float), [&](size_t i, bool pack) {
     const size_t w = pack ? 8 : 1;
     switch (op) {
@@ -938,6 +1098,16 @@ int oracle_map_f32(int op, const float* x, float* y, size_t n) {
       case OR_NORMCDF: for (size_t l = 0; l < w; ++l) y[i + l] = normcdf_f(x[i + l]); break;
       case OR_FASTNORMCDF: for (size_t l = 0; l < w; ++l) y[i + l] = fastnormcdf_f(x[i + l]); break;
       case OR_ABS: for (size_t l = 0; l < w; ++l) y[i + l] = std::fabs(x[i + l]); break;
+      case OR_EXP2: for (size_t l = 0; l < w; ++l) y[i + l] = exp2_f(x[i + l]); break;
+      case OR_EXP10: for (size_t l = 0; l < w; ++l) y[i + l] = exp10_f(x[i + l]); break;
+      case OR_LOG2: for (size_t l = 0; l < w; ++l) y[i + l] = log2_f(x[i + l]); break;
+      case OR_LOG10: for (size_t l = 0; l < w; ++l) y[i + l] = log10_f(x[i + l]); break;
+      case OR_FLOOR: for (size_t l = 0; l < w; ++l) y[i + l] = std::floor(x[i + l]); break;
+      case OR_CEIL: for (size_t l = 0; l < w; ++l) y[i + l] = std::ceil(x[i + l]); break;
+      case OR_TRUNC: for (size_t l = 0; l < w; ++l) y[i + l] = std::trunc(x[i + l]); break;
+      case OR_ROUND: for (size_t l = 0; l < w; ++l) y[i + l] = std::round(x[i + l]); break;
+      case OR_ROUND2EVEN: for (size_t l = 0; l < w; ++l) y[i + l] = std::nearbyint(x[i + l]); break;
+      case OR_SIGN: for (size_t l = 0; l < w; ++l) y[i + l] = sign_of(x[i + l]); break;
       case OR_COS:
       case OR_SIN:
         if (pack) trig_group_f<8>(op == OR_COS ? 0 : 1, x + i, y + i, nullptr);
@@ -949,7 +1119,7 @@ int oracle_map_f32(int op, const float* x, float* y, size_t n) {
 }
 
 int oracle_map_f64(int op, const double* x, double* y, size_t n) {
-  if (op < 0 || op > OR_ABS || op == OR_FASTNORMCDF) return -1;
+  if (op < 0 || op > OR_LAST || op == OR_FASTNORMCDF) return -1;
   nt2_transform(n, 4, sizeof(double), [&](size_t i, bool pack) {
     const size_t w = pack ? 4 : 1;
     switch (op) {
@@ -959,6 +1129,16 @@ int oracle_map_f64(int op, const double* x, double* y, size_t n) {
       case OR_ERFC: for (size_t l = 0; l < w; ++l) y[i + l] = erfc_d(x[i + l], !pack); break;
       case OR_NORMCDF: for (size_t l = 0; l < w; ++l) y[i + l] = normcdf_d(x[i + l], !pack); break;
       case OR_ABS: for (size_t l = 0; l < w; ++l) y[i + l] = std::fabs(x[i + l]); break;
+      case OR_EXP2: for (size_t l = 0; l < w; ++l) y[i + l] = exp2_d(x[i + l]); break;
+      case OR_EXP10: for (size_t l = 0; l < w; ++l) y[i + l] = exp10_d(x[i + l]); break;
+      case OR_LOG2: for (size_t l = 0; l < w; ++l) y[i + l] = log2_d(x[i + l]); break;
+      case OR_LOG10: for (size_t l = 0; l < w; ++l) y[i + l] = log10_d(x[i + l]); break;
+      case OR_FLOOR: for (size_t l = 0; l < w; ++l) y[i + l] = std::floor(x[i + l]); break;
+      case OR_CEIL: for (size_t l = 0; l < w; ++l) y[i + l] = std::ceil(x[i + l]); break;
+      case OR_TRUNC: for (size_t l = 0; l < w; ++l) y[i + l] = std::trunc(x[i + l]); break;
+      case OR_ROUND: for (size_t l = 0; l < w; ++l) y[i + l] = std::round(x[i + l]); break;
+      case OR_ROUND2EVEN: for (size_t l = 0; l < w; ++l) y[i + l] = std::nearbyint(x[i + l]); break;
+      case OR_SIGN: for (size_t l = 0; l < w; ++l) y[i + l] = sign_of(x[i + l]); break;
       case OR_COS:
       case OR_SIN:
         if (pack) trig_group_d<4>(op == OR_COS ? 0 : 1, x + i, y + i, nullptr);

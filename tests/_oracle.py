@@ -10,7 +10,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
 
 OPS = {"exp": 0, "log": 1, "sqrt": 2, "erfc": 3, "normcdf": 4,
-       "fastnormcdf": 5, "cos": 6, "sin": 7, "abs": 8}
+       "fastnormcdf": 5, "cos": 6, "sin": 7, "abs": 8, "exp2": 9, "exp10": 10, "log2": 11,
+       "log10": 12, "floor": 13, "ceil": 14, "trunc": 15, "round": 16, "round2even": 17, "sign": 18}
 
 _lib = None
 

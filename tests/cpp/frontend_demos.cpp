@@ -160,6 +160,17 @@ int main(int argc, char** argv) {
   table<float> v = M(1) * ones(of_size(1000), meta::as_<float>());
   const float vs = sum(v);  // a one-element result used as a scalar
   dump_scalar("v_sum", vs);
+
+  // ---- exponentials, logarithms, rounding and sign in one statement each
+  table<float> ew(of_size(size));
+  ew = nt2::round(xn * 10.0f) + nt2::log2(nt2::abs(yn) + 1.0f) - nt2::exp10(xn / 8.0f);
+  dump("ew_f32", ew);
+  ew = nt2::round2even(yn / 16.0f) * nt2::sign(xn) + nt2::exp2(nt2::floor(xn) + nt2::ceil(-xn) / 4.0f);
+  dump("ew2_f32", ew);
+  table<double> dm(of_size(mh * mw)), dr;
+  for (std::size_t i = 1; i <= mh * mw; ++i) dm(i) = (double)M(i) / 100.0;
+  dr = nt2::trunc(dm) * nt2::exp2(nt2::ceil(dm / 50.0)) + nt2::sign(dm) * nt2::log10(nt2::abs(dm) + 1.0);
+  dump("ew_f64", dr);
   std::printf("frontend demos done: %zu elements\n", size);
   return 0;
 }

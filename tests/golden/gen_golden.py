@@ -2,8 +2,8 @@
 """Writes the golden fixtures under tests/golden/ (run once; outputs committed).
 
 * math_f32.npz / math_f64.npz — inputs and correctly rounded outputs of exp,
-  log, erfc, normcdf, cos and sin, computed with mpmath at 60 significant
-  digits.  Inputs: edge values, the NT2 reduction-range boundaries, and seeded
+  log, erfc, normcdf, cos, sin, exp2, exp10, log2 and log10, computed with
+  mpmath at 60 significant digits.  Inputs: edge values, the NT2 reduction-range boundaries, and seeded
   random points spread over each function's domain (including huge
   arguments for the Payne-Hanek path of cos/sin).
 * bs_golden.npz — Black-Scholes call/put in mpmath for 512 options drawn
@@ -60,6 +60,11 @@ FUNCS = {
     "normcdf": lambda x: mpmath.ncdf(x),
     "cos": lambda x: mpmath.cos(x),
     "sin": lambda x: mpmath.sin(x),
+    # appended after the first six so their seeded inputs stay as committed
+    "exp2": lambda x: mpmath.power(2, x),
+    "exp10": lambda x: mpmath.power(10, x),
+    "log2": lambda x: mpmath.log(x, 2) if x > 0 else (mpmath.mpf("-inf") if x == 0 else mpmath.nan),
+    "log10": lambda x: mpmath.log10(x) if x > 0 else (mpmath.mpf("-inf") if x == 0 else mpmath.nan),
 }
 
 
@@ -71,7 +76,12 @@ def inputs(name, dtype, rng):
         lim = 88.0 if dtype is np.float32 else 708.0
         pts += list(rng.uniform(-lim, lim, 1200)) + list(rng.uniform(-1, 1, 300))
         pts += [0.0, 1.0, -1.0, 0.5, -0.5, lim, -lim, 1e-8, -1e-8]
-    elif name == "log":
+    elif name in ("exp2", "exp10"):
+        # the normal range of the result (NT2 flushes what lies below 2^-126 / 2^-1022)
+        lim = (126.0 if dtype is np.float32 else 1021.0) / (1.0 if name == "exp2" else np.log2(10.0))
+        pts += list(rng.uniform(-lim, lim, 1200)) + list(rng.uniform(-1, 1, 300))
+        pts += [0.0, 1.0, -1.0, 0.5, -0.5, 2.0, 3.0, 10.0, -10.0, 1e-8, -1e-8]
+    elif name in ("log", "log2", "log10"):
         e = rng.uniform(-120 if dtype is np.float32 else -1000, 120 if dtype is np.float32 else 1000, 1200)
         pts += list(np.exp2(e)) + list(rng.uniform(0.5, 2.0, 300))
         pts += [1.0, 2.0, 0.5, np.e, 10.0, 0.7071, 1.4142, 1e-30]
@@ -175,6 +185,34 @@ KATS = [
     ("cos", "pi/2", "0", 0.5, "modules/core/trigonometric/unit/simd/cos.cpp:58"),
     ("cos", "pi/4", "sqrt(2)/2", 0.5, "modules/core/trigonometric/unit/simd/cos.cpp:59"),
     ("cos", "0", "1", 0.5, "modules/core/trigonometric/unit/simd/cos.cpp:60"),
+    # modules/core/exponential/unit/simd/exp2.cpp:46-52
+    ("exp2", "inf", "inf", 0, "modules/core/exponential/unit/simd/exp2.cpp:46"),
+    ("exp2", "-inf", "0", 0, "modules/core/exponential/unit/simd/exp2.cpp:47"),
+    ("exp2", "nan", "nan", 0, "modules/core/exponential/unit/simd/exp2.cpp:48"),
+    ("exp2", "-1", "0.5", 0, "modules/core/exponential/unit/simd/exp2.cpp:50"),
+    ("exp2", "1", "2", 0, "modules/core/exponential/unit/simd/exp2.cpp:51"),
+    ("exp2", "0", "1", 0, "modules/core/exponential/unit/simd/exp2.cpp:52"),
+    # modules/core/exponential/unit/simd/exp10.cpp:46-52
+    ("exp10", "inf", "inf", 0, "modules/core/exponential/unit/simd/exp10.cpp:46"),
+    ("exp10", "-inf", "0", 0, "modules/core/exponential/unit/simd/exp10.cpp:47"),
+    ("exp10", "nan", "nan", 0, "modules/core/exponential/unit/simd/exp10.cpp:48"),
+    ("exp10", "-1", "0.1", 0, "modules/core/exponential/unit/simd/exp10.cpp:50"),
+    ("exp10", "1", "10", 0, "modules/core/exponential/unit/simd/exp10.cpp:51"),
+    ("exp10", "0", "1", 0, "modules/core/exponential/unit/simd/exp10.cpp:52"),
+    # modules/core/exponential/unit/simd/log2.cpp:43-49
+    ("log2", "inf", "inf", 0, "modules/core/exponential/unit/simd/log2.cpp:43"),
+    ("log2", "-inf", "nan", 0, "modules/core/exponential/unit/simd/log2.cpp:44"),
+    ("log2", "nan", "nan", 0, "modules/core/exponential/unit/simd/log2.cpp:45"),
+    ("log2", "-1", "nan", 0, "modules/core/exponential/unit/simd/log2.cpp:46"),
+    ("log2", "0", "-inf", 0, "modules/core/exponential/unit/simd/log2.cpp:47"),
+    ("log2", "1", "0", 0, "modules/core/exponential/unit/simd/log2.cpp:49"),
+    # modules/core/exponential/unit/simd/log10.cpp:46-52
+    ("log10", "inf", "inf", 0, "modules/core/exponential/unit/simd/log10.cpp:46"),
+    ("log10", "-inf", "nan", 0, "modules/core/exponential/unit/simd/log10.cpp:47"),
+    ("log10", "nan", "nan", 0, "modules/core/exponential/unit/simd/log10.cpp:48"),
+    ("log10", "-1", "nan", 0, "modules/core/exponential/unit/simd/log10.cpp:49"),
+    ("log10", "0", "-inf", 0, "modules/core/exponential/unit/simd/log10.cpp:50"),
+    ("log10", "1", "0", 0, "modules/core/exponential/unit/simd/log10.cpp:52"),
 ]
 
 

@@ -26,6 +26,19 @@ def test_header_declares_the_path():
     assert len(names) >= 40
 
 
+def test_op_codes_agree_with_the_bindings():
+    """nt2h_map_f32/f64 op codes: the header's NT2H_OP_* values are the ones the
+    Python bindings and the oracle bindings send."""
+    import sys
+    sys.path.insert(0, ROOT)
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from nt2_amd.native import OPS
+    import _oracle
+    ops = {m.group(1).lower(): int(m.group(2))
+           for m in re.finditer(r"#define NT2H_OP_(\w+) (\d+)", open(HEADER).read())}
+    assert ops == OPS == _oracle.OPS
+
+
 @pytest.mark.skipif(not os.path.exists(LIB), reason="libnt2h.so not built (run __graft_entry__.build())")
 def test_library_exports_every_declared_symbol():
     lib = ctypes.CDLL(LIB)

@@ -113,3 +113,24 @@ def test_dimwise_statements_bitwise(outputs):
     assert same_bits(outputs("M_sumexpr", np.float32), O.sum_dim(e.astype(np.float32), ext, 1))
     v = np.full(1000, M[0], dtype=np.float32)
     assert outputs("v_sum", np.float32)[0] == O.sum_dim(v, (1000,), 1)[0]
+
+
+def test_exponential_rounding_statements(outputs):
+    """exp2, exp10, log2, log10, floor, ceil, trunc, round, round2even and sign
+    inside fused statements: every function is lane-independent, so the oracle
+    maps each one over the whole table and combines in float32/float64."""
+    f32 = np.float32
+    xn, yn = outputs("xn", f32), outputs("yn", f32)
+    a = O.map_f32("round", xn * f32(10)) + O.map_f32("log2", np.abs(yn) + f32(1))
+    want = a - O.map_f32("exp10", xn / f32(8))
+    got = outputs("ew_f32", f32)
+    assert same_bits(got, want), first_diff(got, want)
+    b = O.map_f32("round2even", yn / f32(16)) * O.map_f32("sign", xn)
+    want = b + O.map_f32("exp2", O.map_f32("floor", xn) + O.map_f32("ceil", -xn) / f32(4))
+    got = outputs("ew2_f32", f32)
+    assert same_bits(got, want), first_diff(got, want)
+    d = outputs("M", f32).astype(np.float64) / 100.0
+    want = (O.map_f64("trunc", d) * O.map_f64("exp2", O.map_f64("ceil", d / 50.0)) +
+            O.map_f64("sign", d) * O.map_f64("log10", np.abs(d) + 1.0))
+    got = outputs("ew_f64", np.float64)
+    assert same_bits(got, want), first_diff(got, want)

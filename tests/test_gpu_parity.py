@@ -18,15 +18,31 @@ from _ulp import classic_ulp  # noqa: E402
 from nt2_amd import EXACT, FAST, ops  # noqa: E402
 
 G = os.path.join(os.path.dirname(__file__), "golden")
-F32_OPS = ["exp", "log", "sqrt", "erfc", "normcdf", "fastnormcdf", "cos", "sin", "abs"]
-F64_OPS = ["exp", "log", "sqrt", "erfc", "normcdf", "cos", "sin", "abs"]
+ROUNDING = ["floor", "ceil", "trunc", "round", "round2even", "sign"]
+F32_OPS = ["exp", "log", "sqrt", "erfc", "normcdf", "fastnormcdf", "cos", "sin", "abs",
+           "exp2", "exp10", "log2", "log10"] + ROUNDING
+F64_OPS = ["exp", "log", "sqrt", "erfc", "normcdf", "cos", "sin", "abs",
+           "exp2", "exp10", "log2", "log10"] + ROUNDING
+
+
+def _rounding_inputs(rng, n, dt):
+    k = n // 4
+    ties = rng.integers(-(1 << 20), 1 << 20, k) + 0.5
+    big = rng.uniform(-1, 1, k) * np.exp2(rng.uniform(0, 60, k))
+    return np.concatenate([rng.normal(0, 10, n - 3 * k), ties, -ties, big]).astype(dt)
 
 
 def _inputs_f32(op, rng, n):
     if op in ("exp",):
         return rng.uniform(-100, 100, n).astype(np.float32)
-    if op in ("log", "sqrt"):
+    if op in ("log", "sqrt", "log2", "log10"):
         return np.exp2(rng.uniform(-140, 127, n)).astype(np.float32)
+    if op == "exp2":
+        return rng.uniform(-150, 150, n).astype(np.float32)
+    if op == "exp10":
+        return rng.uniform(-45, 45, n).astype(np.float32)
+    if op in ROUNDING:
+        return _rounding_inputs(rng, n, np.float32)
     if op in ("erfc", "normcdf", "fastnormcdf"):
         return rng.uniform(-12, 12, n).astype(np.float32)
     if op in ("cos", "sin"):
@@ -40,8 +56,14 @@ def _inputs_f32(op, rng, n):
 def _inputs_f64(op, rng, n):
     if op == "exp":
         return rng.uniform(-750, 750, n)
-    if op in ("log", "sqrt"):
+    if op in ("log", "sqrt", "log2", "log10"):
         return np.exp2(rng.uniform(-1000, 1000, n))
+    if op == "exp2":
+        return rng.uniform(-1100, 1100, n)
+    if op == "exp10":
+        return rng.uniform(-330, 330, n)
+    if op in ROUNDING:
+        return _rounding_inputs(rng, n, np.float64)
     if op in ("erfc", "normcdf"):
         return rng.uniform(-30, 30, n)
     if op in ("cos", "sin"):
@@ -90,7 +112,11 @@ def _truth_f32(op, x):
     xd = x.astype(np.float64)
     f = {"exp": np.exp, "log": lambda v: np.log(np.where(v >= 0, v, np.nan)), "sqrt": np.sqrt,
          "erfc": sp.erfc, "normcdf": sp.ndtr, "fastnormcdf": sp.ndtr, "cos": np.cos,
-         "sin": np.sin, "abs": np.abs}[op]
+         "sin": np.sin, "abs": np.abs, "exp2": np.exp2, "exp10": lambda v: 10.0 ** v,
+         "log2": lambda v: np.log2(np.where(v >= 0, v, np.nan)),
+         "log10": lambda v: np.log10(np.where(v >= 0, v, np.nan)),
+         "floor": np.floor, "ceil": np.ceil, "trunc": np.trunc, "round2even": np.rint, "sign": np.sign,
+         "round": lambda v: np.sign(v) * np.floor(np.abs(v) + 0.5)}[op]
     with np.errstate(all="ignore"):
         return f(xd)
 
@@ -99,6 +125,12 @@ def _fast_bound(op, x, y):
     ax = np.abs(x.astype(np.float64))
     if op in ("exp", "log"):
         return np.full(x.shape, 1.0)
+    if op in ("exp2", "exp10", "log2"):
+        return np.full(x.shape, 1.5)
+    if op == "log10":
+        return np.full(x.shape, 2.0)
+    if op in ROUNDING:
+        return np.zeros(x.shape)
     if op in ("sqrt", "abs"):
         return np.full(x.shape, 0.5)
     if op == "erfc":

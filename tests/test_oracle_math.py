@@ -46,8 +46,13 @@ def _bound(name, x, y, dt):
     """Classic-ULP bounds per function and region (NT2's documented bounds,
     plus the exp(-x^2) conditioning of erfc tails)."""
     ax = np.abs(x.astype(np.float64))
-    if name in ("exp", "log"):
+    if name in ("exp", "log", "exp2", "exp10", "log2"):
         return np.full(x.shape, 1.0)
+    if name == "log10":
+        # Log10_2hi*k is added before Log10_2lo*k (f_log.hpp:163-169), so the
+        # large term is rounded twice: up to 1.6 ulp from the exact value in
+        # f32, 2 ulp from the rounded one in f64
+        return np.full(x.shape, 1.0 if dt is np.float32 else 2.0)
     if name == "erfc":
         if dt is np.float32:
             # erf_kernel.hpp:75 documents 2 ulp on [0, 2/3] (measured here: 4
@@ -70,10 +75,16 @@ def _bound(name, x, y, dt):
 
 
 @pytest.mark.parametrize("dt,tag", [(np.float32, "f32"), (np.float64, "f64")])
-@pytest.mark.parametrize("name", ["exp", "log", "erfc", "normcdf", "cos", "sin"])
+@pytest.mark.parametrize("name", ["exp", "log", "erfc", "normcdf", "cos", "sin", "exp2", "exp10", "log2", "log10"])
 def test_golden_mpmath(dt, tag, name):
     g = np.load(os.path.join(G, "math_%s.npz" % tag))
     x, y = g[name + "_x"], g[name + "_y"]
+    if name in ("exp2", "exp10"):
+        # at or below Minlog2 / Minlog10 NT2 returns 0 (constants/minlog2.hpp:29,
+        # minlog10.hpp:29), also where the result is still normal; checked
+        # separately in test_expa_limits
+        keep = x > MINLOG[name][dt]
+        x, y = x[keep], y[keep]
     mapf = O.map_f32 if dt is np.float32 else O.map_f64
     W = 8 if dt is np.float32 else 4
     if name in ("cos", "sin") and dt is np.float32:
@@ -98,6 +109,59 @@ def test_golden_mpmath(dt, tag, name):
         assert (ae[near] <= 4 * np.finfo(dt).eps).all()
 
 
+MINLOG = {"exp2": {np.float32: np.float32(-127.0), np.float64: -1022.0},
+          "exp10": {np.float32: np.uint32(0xc2179999).view(np.float32),
+                    np.float64: np.uint64(0xc0734413509f79fe).view(np.float64)}}
+MAXLOG = {"exp2": {np.float32: np.float32(127.0), np.float64: 1023.0},
+          "exp10": {np.float32: np.uint32(0x4218ec59).view(np.float32),
+                    np.float64: np.uint64(0x40734413509f79fe).view(np.float64)}}
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("name", ["exp2", "exp10"])
+def test_expa_limits(dt, name):
+    """expo_base.hpp:56-58: 0 at or below Minlog*, Inf at or above Maxlog*,
+    NaN through; just inside the limits the result is finite (just above
+    Minlog2 = -127 the f32 scale 2^k with k = -127 is itself 0)."""
+    mapf = O.map_f32 if dt is np.float32 else O.map_f64
+    lo, hi = MINLOG[name][dt], MAXLOG[name][dt]
+    x = np.array([lo, np.nextafter(lo, dt(-np.inf)), -np.inf, hi, np.nextafter(hi, dt(np.inf)), np.inf,
+                  np.nextafter(lo, dt(0)), np.nextafter(hi, dt(0)), np.nan], dtype=dt)
+    got = mapf(name, x)
+    assert list(got[:3]) == [0, 0, 0] and list(got[3:6]) == [np.inf] * 3
+    assert np.isnan(got[8])
+    if name == "exp10" and dt is np.float64:
+        # Minlog10 / Maxlog10 are +-log10(2^1024), so just inside them k
+        # reaches -+1024 and the scale 2^k (expo_scale.hpp:30-33: (k + 1023)
+        # shifted into the exponent field) becomes -Inf / Inf
+        assert got[6] == -np.inf and got[7] == np.inf
+    else:
+        assert np.all(np.isfinite(got[6:8])) and got[6] >= 0 and got[7] > 0
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_exp2_log2_closed_forms(dt):
+    """exp2(k) = 2^k exactly for integers k inside the limits (the reduced
+    argument is 0 and the polynomial returns 1), log2(2^k) = k exactly for
+    normal powers of two (x = 0 after the frexp), log10(10^k) = k and
+    exp10(k) = 10^k for the powers of ten that are exact in the format."""
+    mapf = O.map_f32 if dt is np.float32 else O.map_f64
+    W = 8 if dt is np.float32 else 4
+    emin, emax = (-126, 127) if dt is np.float32 else (-1021, 1023)
+    k = np.arange(emin, emax)
+    p = np.ldexp(np.ones(k.size), k).astype(dt)
+    assert np.array_equal(_iso(mapf, "exp2", k.astype(dt), W), p)
+    assert np.array_equal(_iso(mapf, "log2", p, W), k.astype(dt))
+    if dt is np.float32:  # denormals are prescaled by 2^23 (f_log.hpp:130-133)
+        kd = np.arange(-149, -126)
+        assert np.array_equal(_iso(mapf, "log2", np.ldexp(np.ones(kd.size), kd).astype(dt), W), kd.astype(dt))
+    t = np.arange(0, 11 if dt is np.float32 else 23)
+    tens = np.array([10 ** int(i) for i in t], dtype=np.float64).astype(dt)
+    assert np.array_equal(_iso(mapf, "exp10", t.astype(dt), W), tens)
+    l10 = _iso(mapf, "log10", tens, W)
+    assert np.all(np.abs(l10 - t) <= 2 * np.finfo(dt).eps * np.maximum(t, 1))
+
+
 def _uniform_bits(lo, hi, n, rng):
     a = np.float32(lo).view(np.int32)
     b = np.float32(hi).view(np.int32)
@@ -115,6 +179,22 @@ def test_exp_f32_matches_nt2_published_histogram():
     exact = np.mean(got == ref)
     assert abs(exact - 0.9898) < 0.004, exact
     assert np.all(classic_ulp(got, ref.astype(np.float64), np.float32) <= 1.0)
+
+
+@pytest.mark.parametrize("name,lo,hi,share", [("exp2", -127.0, 127.0, 0.9896), ("exp10", -37.9, 38.2308, 0.9915)])
+def test_exp2_exp10_f32_match_nt2_published_histograms(name, lo, hi, share):
+    # f_expo_reduction.hpp:104-108 (exp2 on [-127, 127]: 98.96% within 0 ULPs,
+    # the rest within 0.5) and :150-154 (exp10 on [-37.9, 38.2308]: 99.15%)
+    rng = np.random.default_rng(13)
+    n = 300000
+    x = np.concatenate([_uniform_bits(0.0, hi, n, rng), -_uniform_bits(0.0, -lo, n, rng)])
+    got = O.map_f32(name, x)
+    ref = np.exp2(x.astype(np.float64)) if name == "exp2" else 10.0 ** x.astype(np.float64)
+    r32 = ref.astype(np.float32)
+    ok = np.isfinite(r32) & (np.abs(r32) >= np.finfo(np.float32).tiny)
+    exact = np.mean(got[ok] == r32[ok])
+    assert abs(exact - share) < 0.002, exact
+    assert np.all(classic_ulp(got[ok], ref[ok], np.float32) <= 1.5)
 
 
 def test_log_f32_histogram():
