@@ -144,6 +144,27 @@ int nt2h_sum_dim_f64(const double* x, const size_t* ext, int dim, double* out, i
 int nt2h_cumsum_dim_f32(const float* x, const size_t* ext, int dim, float* out, int policy, void* stream);
 int nt2h_cumsum_dim_f64(const double* x, const size_t* ext, int dim, double* out, int policy, void* stream);
 
+/* ---- image demos ------------------------------------------------------ */
+/* grey = 0.3 r + 0.59 g + 0.11 b over n pixels (planar float tables):
+ * replaces the statement of demo/rgb2grey/nt2/rgb2grey_nt2.cpp:42-45. */
+int nt2h_rgb2grey_f32(size_t n, const float* r, const float* g, const float* b, float* grey, int policy,
+                      void* stream);
+/* y = 0.299 r + 0.587 g + 0.114 b; u = 0.492 (b - y); v = 0.877 (r - y), one
+ * pass: replaces rgb2yuv_work, demo/rgb2yuv/nt2/rgb2yuv_nt2.cpp:28-35. */
+int nt2h_rgb2yuv_f32(size_t n, const float* r, const float* g, const float* b, float* y, float* u, float* v,
+                     int policy, void* stream);
+/* out[p] = hi for p = n - 1, else p*step + lo with step = (hi - lo)/(n - 1):
+ * replaces linspace(lo, hi, n),
+ * modules/core/generative/include/nt2/core/functions/details/linspace.hpp:22-48. */
+int nt2h_linspace_f32(size_t n, float lo, float hi, float* out, int policy, void* stream);
+int nt2h_linspace_f64(size_t n, double lo, double hi, double* out, int policy, void* stream);
+/* out = arrayfun(step, a, b): escape counts of z = z^2 + (a + ib) from z = 0,
+ * iterated per pack of 8 until no lane of the pack has |z|^2 < 4 or max_iter
+ * iterations (256 in the demo); replaces demo/mandelbrot/nt2/mandelbrot_nt2.cpp:35-70
+ * and :100-103.  max_iter must be at least 1. */
+int nt2h_mandelbrot_f32(size_t n, const float* a, const float* b, int32_t* out, int max_iter, int policy,
+                        void* stream);
+
 /* ---- fused Black-Scholes ---------------------------------------------- */
 /* call/put from tables S, X, T and scalars r, v; one pass, one kernel.
  * demo mode writes call only (put may be NULL). */

@@ -963,6 +963,231 @@ tie2_rhs<typename expr::wrap<E1>::type, typename expr::wrap<E2>::type> tie(const
       expr::wrap<E1>::get(e1), expr::wrap<E2>::get(e2)};
 }
 
+// ---- packs for arrayfun -------------------------------------------------------------
+// arrayfun(f, a, b) calls f once per pack of NT2's width (8 floats) and once
+// per element of the tail with nt2::pack<T, 1>, as NT2's transform calls a
+// functor on native packs and then on scalars.  A functor decides per pack
+// with any()/all(); its operator() must be callable on the device
+// (__host__ __device__).  Pack arithmetic rounds every operation (no
+// contraction), as on AVX without FMA.
+template <class T, int N> struct pack {
+  T v[N];
+  __host__ __device__ pack() {}
+  __host__ __device__ pack(T x) {
+    for (int l = 0; l < N; ++l) v[l] = x;
+  }
+  __host__ __device__ T& operator[](int l) { return v[l]; }
+  __host__ __device__ const T& operator[](int l) const { return v[l]; }
+};
+#define NT2H_PACK_OP(OP, R)                                                                      \
+  template <class T, int N>                                                                      \
+  __host__ __device__ pack<R, N> operator OP(const pack<T, N>& a, const pack<T, N>& b) {         \
+    _Pragma("clang fp contract(off)") pack<R, N> r;                                              \
+    for (int l = 0; l < N; ++l) r.v[l] = a.v[l] OP b.v[l];                                       \
+    return r;                                                                                    \
+  }                                                                                              \
+  template <class T, int N, class S, class = typename std::enable_if<std::is_arithmetic<S>::value>::type> \
+  __host__ __device__ pack<R, N> operator OP(const pack<T, N>& a, S b) {                         \
+    return a OP pack<T, N>((T)b);                                                                \
+  }                                                                                              \
+  template <class T, int N, class S, class = typename std::enable_if<std::is_arithmetic<S>::value>::type> \
+  __host__ __device__ pack<R, N> operator OP(S a, const pack<T, N>& b) {                         \
+    return pack<T, N>((T)a) OP b;                                                                \
+  }
+NT2H_PACK_OP(+, T) NT2H_PACK_OP(-, T) NT2H_PACK_OP(*, T) NT2H_PACK_OP(/, T)
+NT2H_PACK_OP(<, bool) NT2H_PACK_OP(>, bool) NT2H_PACK_OP(<=, bool) NT2H_PACK_OP(>=, bool)
+NT2H_PACK_OP(==, bool) NT2H_PACK_OP(!=, bool)
+#undef NT2H_PACK_OP
+template <class T, int N> __host__ __device__ pack<T, N> operator-(const pack<T, N>& a) {
+  pack<T, N> r;
+  for (int l = 0; l < N; ++l) r.v[l] = -a.v[l];
+  return r;
+}
+// any / all lanes of a mask (modules/boost/simd/base/include/boost/simd/reduction/functions/any.hpp)
+template <int N> __host__ __device__ bool any(const pack<bool, N>& m) {
+  bool r = false;
+  for (int l = 0; l < N; ++l) r = r || m.v[l];
+  return r;
+}
+template <int N> __host__ __device__ bool all(const pack<bool, N>& m) {
+  bool r = true;
+  for (int l = 0; l < N; ++l) r = r && m.v[l];
+  return r;
+}
+__host__ __device__ inline bool any(bool m) { return m; }
+__host__ __device__ inline bool all(bool m) { return m; }
+// seladd(c, a, b) = c ? a + b : a, lane by lane
+// (modules/boost/simd/base/include/boost/simd/boolean/functions/seladd.hpp)
+template <class T, int N>
+__host__ __device__ pack<T, N> seladd(const pack<bool, N>& c, const pack<T, N>& a, const pack<T, N>& b) {
+  pack<T, N> r;
+  for (int l = 0; l < N; ++l) r.v[l] = c.v[l] ? (T)(a.v[l] + b.v[l]) : a.v[l];
+  return r;
+}
+template <class T> __host__ __device__ T seladd(bool c, T a, T b) { return c ? (T)(a + b) : a; }
+template <class T> __host__ __device__ T Zero() { return T(0); }
+template <class T> __host__ __device__ T One() { return T(1); }
+namespace meta {
+template <class T> struct scalar_of { typedef T type; };
+template <class T, int N> struct scalar_of<pack<T, N>> { typedef T type; };
+template <class T> struct as_integer {
+  typedef typename std::conditional<sizeof(T) == 8, int64_t,
+          typename std::conditional<sizeof(T) == 4, int32_t,
+          typename std::conditional<sizeof(T) == 2, int16_t, int8_t>::type>::type>::type type;
+};
+template <class T, int N> struct as_integer<pack<T, N>> { typedef pack<typename as_integer<T>::type, N> type; };
+template <class T> struct as_logical { typedef bool type; };
+template <class T, int N> struct as_logical<pack<T, N>> { typedef pack<bool, N> type; };
+}  // namespace meta
+
+// ---- linspace, expand_to, colvect, arrayfun ---------------------------------------------
+namespace hip { namespace fe {
+template <class T> struct lin_v { size_t n; T lo, step, hi; };
+template <class VA> struct bcast_v { VA a; size_t to[4]; size_t from[4]; };
+template <class F, class VA, class VB> struct afun2_v { F f; VA a; VB b; };
+template <class T> struct vview<lin_v<T>> { typedef T type; };
+template <class VA> struct vview<bcast_v<VA>> { typedef typename vview<VA>::type type; };
+template <class F, class VA, class VB> struct vview<afun2_v<F, VA, VB>> {
+  typedef typename meta::scalar_of<decltype(std::declval<const F&>()(
+      std::declval<const nt2::pack<typename vview<VA>::type, 8>&>(),
+      std::declval<const nt2::pack<typename vview<VB>::type, 8>&>()))>::type type;
+};
+
+// linspace: hi at the last index, p*step + lo elsewhere
+// (modules/core/generative/include/nt2/core/functions/details/linspace.hpp:33-45)
+template <int G, class T>
+__device__ __forceinline__ void ev(const lin_v<T>& v, size_t i, T (&r)[G]) {
+#pragma unroll
+  for (int l = 0; l < G; ++l) {
+    const size_t p = i + l;
+    r[l] = (p == v.n - 1) ? v.hi : dm::fma<P>((T)p, v.step, v.lo);
+  }
+}
+// expand_to: each output index reads the source at its coordinates, 0 along
+// the source's singleton dimensions (one element at a time)
+template <int G, class VA, class R>
+__device__ __forceinline__ void ev(const bcast_v<VA>& v, size_t i, R (&r)[G]) {
+#pragma unroll
+  for (int l = 0; l < G; ++l) {
+    size_t k = i + l, src = 0, stride = 1;
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+      const size_t c = k % v.to[d];
+      k /= v.to[d];
+      src += (v.from[d] == 1 ? 0 : c) * stride;
+      stride *= v.from[d];
+    }
+    R t[1];
+    ev<1>(v.a, src, t);
+    r[l] = t[0];
+  }
+}
+// arrayfun: the functor sees the whole pack
+template <int G, class F, class VA, class VB, class R>
+__device__ __forceinline__ void ev(const afun2_v<F, VA, VB>& v, size_t i, R (&r)[G]) {
+  nt2::pack<typename vview<VA>::type, G> a;
+  nt2::pack<typename vview<VB>::type, G> b;
+  ev<G>(v.a, i, a.v);
+  ev<G>(v.b, i, b.v);
+  const auto res = v.f(a, b);
+#pragma unroll
+  for (int l = 0; l < G; ++l) r[l] = (R)res[l];
+}
+}}  // namespace hip::fe
+
+namespace expr {
+template <class T> struct linspace_node : is_expr_base {
+  typedef T value_type;
+  size_t n;
+  T lo, hi;
+  ext_t ext() const { return ext_t{true, of_size(1, n)}; }
+  void prepare() const {}
+};
+template <class A> struct expand_node : is_expr_base {
+  typedef typename A::value_type value_type;
+  A a;
+  of_size_ to;
+  ext_t ext() const { return ext_t{true, to}; }
+  void prepare() const { a.prepare(); }
+};
+template <class F, class A, class B> struct arrayfun_node : is_expr_base {
+  typedef typename hip::fe::vview<hip::fe::afun2_v<F, typename dview<A>::type, typename dview<B>::type>>::type
+      value_type;
+  F f;
+  A a;
+  B b;
+  ext_t ext() const { return merge_ext(a.ext(), b.ext()); }
+  void prepare() const { a.prepare(); b.prepare(); }
+};
+template <class T> struct dview<linspace_node<T>> {
+  typedef hip::fe::lin_v<T> type;
+  static type get(const linspace_node<T>& x) {
+    return type{x.n, x.lo, (T)((x.hi - x.lo) / (T)(x.n - 1)), x.hi};
+  }
+};
+template <class A> struct dview<expand_node<A>> {
+  typedef hip::fe::bcast_v<typename dview<A>::type> type;
+  static type get(const expand_node<A>& x) {
+    type v;
+    v.a = dview<A>::get(x.a);
+    const ext_t e = x.a.ext();
+    for (int d = 0; d < 4; ++d) {
+      v.to[d] = x.to.d[d];
+      v.from[d] = e.has ? e.s.d[d] : 1;
+    }
+    return v;
+  }
+};
+template <class F, class A, class B> struct dview<arrayfun_node<F, A, B>> {
+  typedef hip::fe::afun2_v<F, typename dview<A>::type, typename dview<B>::type> type;
+  static type get(const arrayfun_node<F, A, B>& x) { return type{x.f, dview<A>::get(x.a), dview<B>::get(x.b)}; }
+};
+template <class A> struct pack_free<expand_node<A>> { static constexpr bool value = pack_free<A>::value; };
+template <class F, class A, class B> struct pack_free<arrayfun_node<F, A, B>> { static constexpr bool value = false; };
+}  // namespace expr
+
+// linspace(a, b, n): n points from a to b in a 1 x n row (100 by default)
+template <class T> expr::linspace_node<T> linspace(T a, T b, size_t n = 100) {
+  expr::linspace_node<T> x;
+  x.n = n;
+  x.lo = a;
+  x.hi = b;
+  return x;
+}
+// expand_to(a, sizes): a broadcast along its singleton dimensions to the new
+// extent (every other dimension must match)
+template <class E, NT2H_ENABLE1(E)>
+auto expand_to(const E& e, of_size_ s) {
+  typedef typename expr::wrap<E>::type EX;
+  expr::expand_node<EX> x;
+  x.a = expr::wrap<E>::get(e);
+  x.to = s;
+  const ext_t from = x.a.ext();
+  for (int d = 0; d < 4 && from.has; ++d)
+    if (from.s.d[d] != 1 && from.s.d[d] != s.d[d])
+      throw std::invalid_argument("nt2: expand_to of a non-singleton dimension to another size");
+  return x;
+}
+template <class E, NT2H_ENABLE1(E)> auto expand_to(const E& e, size_t h, size_t w) { return expand_to(e, of_size(h, w)); }
+// colvect(a): the elements of a as one column
+template <class E, NT2H_ENABLE1(E)>
+auto colvect(const E& e) {
+  typedef typename expr::wrap<E>::type EX;
+  const ext_t x = expr::wrap<E>::get(e).ext();
+  return reshape(e, of_size(x.has ? x.s.numel() : 1));
+}
+// arrayfun(f, a, b): f applied to a and b pack by pack (tail element by element)
+template <class F, class A, class B, NT2H_ENABLE2(A, B)>
+auto arrayfun(const F& f, const A& a, const B& b) {
+  typedef typename expr::wrap<A>::type EA;
+  typedef typename expr::wrap<B>::type EB;
+  expr::arrayfun_node<F, EA, EB> x;
+  x.f = f;
+  x.a = expr::wrap<A>::get(a);
+  x.b = expr::wrap<B>::get(b);
+  return x;
+}
+
 // ---- generators used by the demos ----------------------------------------------------
 template <class T> table<T> ones(of_size_ s, meta::as_<T>) {
   table<T> t(s);

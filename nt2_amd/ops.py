@@ -141,3 +141,32 @@ def cumsum_dim(x, ext, dim, policy=EXACT, out=None):
     f = lib().nt2h_cumsum_dim_f32 if x.dtype == torch.float32 else lib().nt2h_cumsum_dim_f64
     check(f(ptr(x), ctypes.cast(e4, ctypes.c_void_p), dim, ptr(out), policy, _s()), "cumsum_dim")
     return out
+
+
+def rgb2grey(r, g, b, policy=EXACT, out=None):
+    """grey = 0.3 r + 0.59 g + 0.11 b over planar float32 device tables."""
+    out = torch.empty_like(r) if out is None else out
+    check(lib().nt2h_rgb2grey_f32(r.numel(), ptr(r), ptr(g), ptr(b), ptr(out), policy, _s()), "rgb2grey")
+    return out
+
+
+def rgb2yuv(r, g, b, policy=EXACT):
+    y, u, v = torch.empty_like(r), torch.empty_like(r), torch.empty_like(r)
+    check(lib().nt2h_rgb2yuv_f32(r.numel(), ptr(r), ptr(g), ptr(b), ptr(y), ptr(u), ptr(v), policy, _s()),
+          "rgb2yuv")
+    return y, u, v
+
+
+def linspace(lo, hi, n, dtype=torch.float32, policy=EXACT, device="cuda"):
+    out = torch.empty(n, dtype=dtype, device=device)
+    f = lib().nt2h_linspace_f32 if dtype == torch.float32 else lib().nt2h_linspace_f64
+    check(f(n, lo, hi, ptr(out), policy, _s()), "linspace")
+    return out
+
+
+def mandelbrot(a, b, max_iter=256, policy=EXACT, out=None):
+    """Escape counts (int32) of the points a + ib, NT2's pack-uniform loop."""
+    out = torch.empty(a.numel(), dtype=torch.int32, device=a.device) if out is None else out
+    check(lib().nt2h_mandelbrot_f32(a.numel(), ptr(a), ptr(b), ptr(out), int(max_iter), policy, _s()),
+          "mandelbrot")
+    return out

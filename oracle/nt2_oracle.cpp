@@ -1459,4 +1459,73 @@ int oracle_cumsum_dim_f64(const double* x, const size_t* ext, int dim, double* o
   return 0;
 }
 
+
+// ---------------------------------------------------------------------------
+// The image demos.  rgb2grey: gr = 0.3 r + 0.59 g + 0.11 b in float
+// (demo/rgb2grey/nt2/rgb2grey_nt2.cpp:42-45); rgb2yuv: y, then u and v from the
+// stored y (demo/rgb2yuv/nt2/rgb2yuv_nt2.cpp:30-35).  Each product and sum
+// rounds once (AVX without FMA).
+// ---------------------------------------------------------------------------
+int oracle_rgb2grey_f32(size_t n, const float* r, const float* g, const float* b, float* gr) {
+  const float cr = 0.3f, cg = 0.59f, cb = 0.11f;
+  for (size_t i = 0; i < n; ++i) gr[i] = (cr * r[i] + cg * g[i]) + cb * b[i];
+  return 0;
+}
+int oracle_rgb2yuv_f32(size_t n, const float* r, const float* g, const float* b, float* y, float* u,
+                       float* v) {
+  for (size_t i = 0; i < n; ++i) {
+    const float yy = (0.299f * r[i] + 0.587f * g[i]) + 0.114f * b[i];
+    y[i] = yy;
+    u[i] = 0.492f * (b[i] - yy);
+    v[i] = 0.877f * (r[i] - yy);
+  }
+  return 0;
+}
+
+// linspace(l, u, n): element p (0-based) is u for p = n - 1, else p*step + l
+// with step = (u - l)/(n - 1), two roundings
+// (modules/core/generative/include/nt2/core/functions/details/linspace.hpp:22-48).
+extern "C++" template <class T> void linspace_t(size_t n, T l, T u, T* out) {
+  const T step = (u - l) / T(n - 1);
+  for (size_t p = 0; p < n; ++p) out[p] = (p == n - 1) ? u : T(p) * step + l;
+}
+int oracle_linspace_f32(size_t n, float l, float u, float* out) { linspace_t(n, l, u, out); return 0; }
+int oracle_linspace_f64(size_t n, double l, double u, double* out) { linspace_t(n, l, u, out); return 0; }
+
+// mandelbrot: C = arrayfun(step, A, B) (demo/mandelbrot/nt2/mandelbrot_nt2.cpp:35-70,
+// :100-103).  The step runs on a whole pack: z = z^2 + c for every lane, a lane
+// counts an iteration while its |z|^2 before the update is below 4, and the
+// pack stops when no lane counted or after max_iter iterations (256 in the
+// demo).  Packs of 8 from index 0 (8 floats, 8 int32 on AVX), then one lane
+// at a time for the tail.
+extern "C++" template <int W>
+void mandel_pack(const float* a, const float* b, int32_t* c, int max_iter) {
+  float x[W], y[W];
+  int32_t it[W];
+  for (int l = 0; l < W; ++l) { x[l] = 0.0f; y[l] = 0.0f; it[l] = 0; }
+  int i = 0;
+  bool any;
+  do {
+    any = false;
+    for (int l = 0; l < W; ++l) {
+      const float x2 = x[l] * x[l], y2 = y[l] * y[l], xy = (2.0f * x[l]) * y[l];
+      x[l] = (x2 - y2) + a[l];
+      y[l] = xy + b[l];
+      const bool in = (x2 + y2) < 4.0f;
+      it[l] += in ? 1 : 0;
+      any = any || in;
+    }
+    ++i;
+  } while (any && i < max_iter);
+  for (int l = 0; l < W; ++l) c[l] = it[l];
+}
+int oracle_mandelbrot_f32(size_t n, const float* a, const float* b, int32_t* c, int max_iter) {
+  if (max_iter < 1) return -1;
+  nt2_transform(n, 8, sizeof(float), [&](size_t i, bool pack) {
+    if (pack) mandel_pack<8>(a + i, b + i, c + i, max_iter);
+    else mandel_pack<1>(a + i, b + i, c + i, max_iter);
+  });
+  return 0;
+}
+
 }  // extern "C"

@@ -52,6 +52,11 @@ def lib():
         _lib.oracle_expr4_sum_fp64.restype = ctypes.c_double
         for f in ("oracle_sum_dim_f32", "oracle_sum_dim_f64", "oracle_cumsum_dim_f32", "oracle_cumsum_dim_f64"):
             getattr(_lib, f).argtypes = [P, P, ctypes.c_int, P]
+        _lib.oracle_rgb2grey_f32.argtypes = [S, P, P, P, P]
+        _lib.oracle_rgb2yuv_f32.argtypes = [S, P, P, P, P, P, P]
+        _lib.oracle_linspace_f32.argtypes = [S, ctypes.c_float, ctypes.c_float, P]
+        _lib.oracle_linspace_f64.argtypes = [S, ctypes.c_double, ctypes.c_double, P]
+        _lib.oracle_mandelbrot_f32.argtypes = [S, P, P, P, ctypes.c_int]
         _lib.oracle_set_threads.argtypes = [ctypes.c_int]
         _lib.oracle_get_threads.restype = ctypes.c_int
     return _lib
@@ -219,3 +224,33 @@ def cumsum_dim(x, ext, dim):
     f = lib().oracle_cumsum_dim_f32 if x.dtype == np.float32 else lib().oracle_cumsum_dim_f64
     assert f(_p(x), _p(e), dim, _p(out)) == 0
     return out
+
+
+def rgb2grey(r, g, b):
+    """gr = 0.3 r + 0.59 g + 0.11 b (float32, each operation rounded)."""
+    r, g, b = (np.ascontiguousarray(v, dtype=np.float32) for v in (r, g, b))
+    out = np.empty_like(r)
+    assert lib().oracle_rgb2grey_f32(r.size, _p(r), _p(g), _p(b), _p(out)) == 0
+    return out
+
+
+def rgb2yuv(r, g, b):
+    r, g, b = (np.ascontiguousarray(v, dtype=np.float32) for v in (r, g, b))
+    y, u, v = np.empty_like(r), np.empty_like(r), np.empty_like(r)
+    assert lib().oracle_rgb2yuv_f32(r.size, _p(r), _p(g), _p(b), _p(y), _p(u), _p(v)) == 0
+    return y, u, v
+
+
+def linspace(lo, hi, n, dtype=np.float32):
+    out = np.empty(n, dtype=dtype)
+    f = lib().oracle_linspace_f32 if dtype == np.float32 else lib().oracle_linspace_f64
+    assert f(n, lo, hi, _p(out)) == 0
+    return out
+
+
+def mandelbrot(a, b, max_iter=256):
+    """C = arrayfun(step, A, B) over flat float32 tables (NT2's packs of 8)."""
+    a, b = (np.ascontiguousarray(v, dtype=np.float32) for v in (a, b))
+    c = np.empty(a.size, dtype=np.int32)
+    assert lib().oracle_mandelbrot_f32(a.size, _p(a), _p(b), _p(c), int(max_iter)) == 0
+    return c

@@ -2,8 +2,10 @@
 // and run on the GPU; every result is written to <outdir>/<name>.bin so the
 // test can compare it with the oracle.  The statement bodies are those of
 // demo/saxpy/nt2/saxpy_nt2.cpp:39-42, demo/sdot/nt2/sdot_nt2.cpp:35-38,
-// demo/blackscholes/nt2/blackandscholes_nt2.cpp:55-59 and
-// demo/sigmadelta/nt2/sigmadelta_nt2.cpp:37-58 (NT2's table API).
+// demo/blackscholes/nt2/blackandscholes_nt2.cpp:55-59,
+// demo/sigmadelta/nt2/sigmadelta_nt2.cpp:37-58, demo/rgb2grey/nt2/rgb2grey_nt2.cpp:44,
+// demo/rgb2yuv/nt2/rgb2yuv_nt2.cpp:30-35 and demo/mandelbrot/nt2/mandelbrot_nt2.cpp:86-88,
+// :102 (NT2's table API).
 #include <nt2/table.hpp>
 
 #include <cstdio>
@@ -39,6 +41,31 @@ void do_work(Pixel& bkg, const Pixel& fr, Pixel& var, Pixel& diff_img, Pixel& mu
   var = nt2::if_else(diff_img != zero, nt2::selinc(var < mul_img, nt2::seldec(var > mul_img, var)), var);
   out = nt2::if_zero_else_one(diff_img < var);
 }
+
+// Escape-time count of z = z^2 + c from z = 0, one call per pack: a lane
+// counts the iterations whose |z|^2 (before the update) is below 4, and the
+// pack goes on while any lane still counts, at most `limit` times.
+struct escape_count {
+  int limit;
+  template <class V>
+  __host__ __device__ typename nt2::meta::as_integer<V>::type operator()(const V& cr, const V& ci) const {
+    typedef typename nt2::meta::as_integer<V>::type count_t;
+    typedef typename nt2::meta::scalar_of<V>::type real_t;
+    count_t count = nt2::Zero<count_t>();
+    V zr = nt2::Zero<V>(), zi = nt2::Zero<V>();
+    int k = 0;
+    bool again;
+    do {
+      const V rr = zr * zr, ii = zi * zi, ri = real_t(2) * zr * zi;
+      const auto inside = (rr + ii) < real_t(4);
+      zr = rr - ii + cr;
+      zi = ri + ci;
+      count = nt2::seladd(inside, count, nt2::One<count_t>());
+      again = nt2::any(inside);
+    } while (again && ++k < limit);
+    return count;
+  }
+};
 
 int main(int argc, char** argv) {
   outdir = argc > 1 ? argv[1] : ".";
@@ -171,6 +198,43 @@ int main(int argc, char** argv) {
   for (std::size_t i = 1; i <= mh * mw; ++i) dm(i) = (double)M(i) / 100.0;
   dr = nt2::trunc(dm) * nt2::exp2(nt2::ceil(dm / 50.0)) + nt2::sign(dm) * nt2::log10(nt2::abs(dm) + 1.0);
   dump("ew_f64", dr);
+
+  // ---- rgb2grey and rgb2yuv on a 96 x 131 image: the demos' ramp and random planes
+  const std::size_t ih = 96, iw = 131;
+  table<float> pr(of_size(ih, iw)), pg(of_size(ih, iw)), pb(of_size(ih, iw)), grey;
+  for (std::size_t i = 1; i <= ih * iw; ++i) pr(i) = pg(i) = pb(i) = float(i - 1);
+  grey = float(0.3) * pr + float(0.59) * pg + float(0.11) * pb;
+  dump("grey_ramp", grey);
+  for (std::size_t i = 1; i <= ih * iw; ++i) {
+    pr(i) = unif(seed, 0, 255);
+    pg(i) = unif(seed, 0, 255);
+    pb(i) = unif(seed, 0, 255);
+  }
+  dump("pr", pr);
+  dump("pg", pg);
+  dump("pb", pb);
+  grey = float(0.3) * pr + float(0.59) * pg + float(0.11) * pb;
+  dump("grey", grey);
+  table<float> yy, uu, vv;
+  yy = float(0.299f) * pr + float(0.587f) * pg + float(0.114f) * pb;
+  uu = float(0.492f) * (pb - yy);
+  vv = float(0.877f) * (pr - yy);
+  dump("yuv_y", yy);
+  dump("yuv_u", uu);
+  dump("yuv_v", vv);
+
+  // ---- mandelbrot: a 203 x 203 grid of [-1.5, 1.5]^2 (203^2 is not a multiple of 8)
+  const std::size_t mn = 203;
+  table<float> MA, MB;
+  MA = expand_to(linspace(-1.5f, 1.5f, mn), mn, mn);
+  MB = expand_to(colvect(linspace(-1.5f, 1.5f, mn)), mn, mn);
+  table<int> MC;
+  MC = arrayfun(escape_count{256}, MA, MB);
+  dump("mandel_a", MA);
+  dump("mandel_b", MB);
+  dump("mandel", MC);
+  table<float> lin = linspace(0.0f, 1.0f);
+  dump("linspace100", lin);
   std::printf("frontend demos done: %zu elements\n", size);
   return 0;
 }

@@ -134,3 +134,32 @@ def test_exponential_rounding_statements(outputs):
             O.map_f64("sign", d) * O.map_f64("log10", np.abs(d) + 1.0))
     got = outputs("ew_f64", np.float64)
     assert same_bits(got, want), first_diff(got, want)
+
+
+def test_image_demo_statements(outputs):
+    """rgb2grey (demo/rgb2grey/nt2/rgb2grey_nt2.cpp:44) on the demo's ramp and
+    on random planes, rgb2yuv's three statements (demo/rgb2yuv/nt2/rgb2yuv_nt2.cpp:30-35)."""
+    f32 = np.float32
+    ramp = np.arange(96 * 131, dtype=f32)
+    assert same_bits(outputs("grey_ramp", f32), O.rgb2grey(ramp, ramp, ramp))
+    r, g, b = outputs("pr", f32), outputs("pg", f32), outputs("pb", f32)
+    assert same_bits(outputs("grey", f32), O.rgb2grey(r, g, b))
+    y, u, v = O.rgb2yuv(r, g, b)
+    assert same_bits(outputs("yuv_y", f32), y)
+    assert same_bits(outputs("yuv_u", f32), u)
+    assert same_bits(outputs("yuv_v", f32), v)
+
+
+def test_mandelbrot_demo_statements(outputs):
+    """A = expand_to(linspace(...)), B = expand_to(colvect(linspace(...))) and
+    C = arrayfun(step, A, B) with a functor deciding per pack of 8
+    (demo/mandelbrot/nt2/mandelbrot_nt2.cpp:86-88, :102)."""
+    n = 203
+    lin = O.linspace(-1.5, 1.5, n)
+    A, B = outputs("mandel_a", np.float32), outputs("mandel_b", np.float32)
+    assert same_bits(A, np.repeat(lin, n))  # A(i, j) = lin(j)
+    assert same_bits(B, np.tile(lin, n))    # B(i, j) = lin(i)
+    C = outputs("mandel", np.int32)
+    want = O.mandelbrot(A, B, 256)
+    assert np.array_equal(C, want), first_diff(C, want)
+    assert same_bits(outputs("linspace100", np.float32), O.linspace(0.0, 1.0, 100))
