@@ -243,6 +243,64 @@ def leg_dimwise(torch, ops, O):
     return res
 
 
+def leg_images(torch, ops, O):
+    """The image demos.  rgb2grey and rgb2yuv on an 8192 x 8192 image (planar
+    f32, 2^26 pixels): algorithmic bytes 16 and 24 per pixel (three planes
+    read, one or three written).  mandelbrot on the demo's largest grid,
+    1600 x 1600 points of [-1.5, 1.5]^2 with at most 256 iterations: the
+    measure is pixels per second and counted iterations per second.  Exact
+    results are checked bit for bit against the oracle, whose time is
+    reported beside them."""
+    from nt2_amd import EXACT, FAST
+    n = 8192 * 8192
+    r, g, b = (ops.fill_uniform(n, 0.0, 255.0, 90 + k) for k in range(3))
+    grey = torch.empty(n, dtype=torch.float32, device="cuda")
+    res = {"image": "8192 x 8192 f32 planes"}
+    for pname, pol in (("exact", EXACT), ("fast", FAST)):
+        ms = time_steps(torch, lambda: ops.rgb2grey(r, g, b, pol, out=grey), 20, 3)
+        res["rgb2grey_" + pname] = {"ms": ms, "gbs": 16.0 * n / (ms * 1e-3) / 1e9}
+    outs = [torch.empty(n, dtype=torch.float32, device="cuda") for _ in range(3)]
+    from nt2_amd.native import check, lib, ptr
+
+    def yuv(pol):
+        check(lib().nt2h_rgb2yuv_f32(n, ptr(r), ptr(g), ptr(b), ptr(outs[0]), ptr(outs[1]), ptr(outs[2]), pol,
+                                     None), "rgb2yuv")
+    for pname, pol in (("exact", EXACT), ("fast", FAST)):
+        ms = time_steps(torch, lambda: yuv(pol), 20, 3)
+        res["rgb2yuv_" + pname] = {"ms": ms, "gbs": 24.0 * n / (ms * 1e-3) / 1e9}
+    k = 1 << 22  # bit check on the first 2^22 pixels (the kernel is position-independent)
+    rh, gh, bh = (t[:k].cpu().numpy() for t in (r, g, b))
+    ops.rgb2grey(r, g, b, EXACT, out=grey)
+    yuv(EXACT)
+    torch.cuda.synchronize()
+    wy, wu, wv = O.rgb2yuv(rh, gh, bh)
+    res["images_check"] = {"exact_bitwise_vs_oracle_first_2^22": bool(
+        np.array_equal(grey[:k].cpu().numpy().view(np.uint32), O.rgb2grey(rh, gh, bh).view(np.uint32)) and
+        all(np.array_equal(o[:k].cpu().numpy().view(np.uint32), w.view(np.uint32))
+            for o, w in zip(outs, (wy, wu, wv))))}
+    del r, g, b, grey, outs
+    torch.cuda.empty_cache()
+    m = 1600
+    la = ops.linspace(-1.5, 1.5, m)
+    A = la.repeat_interleave(m)      # A(i, j) = linspace(j)
+    B = la.repeat(m)                 # B(i, j) = linspace(i)
+    C = torch.empty(m * m, dtype=torch.int32, device="cuda")
+    for pname, pol in (("exact", EXACT), ("fast", FAST)):
+        ms = time_steps(torch, lambda: ops.mandelbrot(A, B, 256, pol, out=C), 20, 3)
+        its = int(C.to(torch.int64).sum().item())
+        res["mandelbrot_" + pname] = {"ms": ms, "gpix_s": m * m / (ms * 1e-3) / 1e9,
+                                      "counted_iterations_per_s": its / (ms * 1e-3)}
+    ops.mandelbrot(A, B, 256, EXACT, out=C)
+    Ah, Bh = A.cpu().numpy(), B.cpu().numpy()
+    t0 = time.perf_counter()
+    want = O.mandelbrot(Ah, Bh, 256)
+    cpu = time.perf_counter() - t0
+    res["mandelbrot_check"] = {"grid": "1600 x 1600, 256 iterations",
+                               "exact_equal_vs_oracle": bool(np.array_equal(C.cpu().numpy(), want)),
+                               "oracle_ms": cpu * 1e3, "oracle_threads": int(O.lib().oracle_get_threads())}
+    return res
+
+
 def leg_served(torch, ops):
     """Served Black-Scholes: 10,000 requests with sizes log-uniform in
     [2^10, 2^22] options, 80% f32 / 20% f64, windows of resident HBM tables,
@@ -395,6 +453,7 @@ def main():
                          ("sigmadelta_64x1920x1080", lambda: leg_sigmadelta(torch, ops)),
                          ("batched_4096_tables", lambda: leg_batch(torch, O)),
                          ("dimwise_8192x8192", lambda: leg_dimwise(torch, ops, O)),
+                         ("images_8192x8192_mandelbrot_1600x1600", lambda: leg_images(torch, ops, O)),
                          ("served_blackscholes", lambda: leg_served(torch, ops))):
             t0 = time.perf_counter()
             try:

@@ -5,6 +5,7 @@
     python3 -m nt2_amd.profile_workloads sd      # sigma-delta 64 x 1920x1080
     python3 -m nt2_amd.profile_workloads batch   # 4096 uneven tables, persistent (512x32, 2/CU) vs plain
     python3 -m nt2_amd.profile_workloads dimwise # sum / cumsum of 8192 x 8192 f32, dims 1 and 2, both policies
+    python3 -m nt2_amd.profile_workloads images  # rgb2grey / rgb2yuv 8192 x 8192, mandelbrot 1600 x 1600
 
 Each run also launches a 1 GiB streaming copy: the calibration kernel that
 turns FETCH_SIZE / WRITE_SIZE counter values into bytes.
@@ -82,8 +83,26 @@ def dimwise(reps=2):
     _copy()
 
 
+def images(reps=3):
+    n = 8192 * 8192
+    r, g, b = (ops.fill_uniform(n, 0.0, 255.0, 90 + k) for k in range(3))
+    grey = torch.empty_like(r)
+    y, u, v = torch.empty_like(r), torch.empty_like(r), torch.empty_like(r)
+    m = 1600
+    la = ops.linspace(-1.5, 1.5, m)
+    A, B = la.repeat_interleave(m), la.repeat(m)
+    C = torch.empty(m * m, dtype=torch.int32, device="cuda")
+    from .native import check, lib, ptr
+    for _ in range(reps):
+        for pol in (EXACT, FAST):
+            ops.rgb2grey(r, g, b, pol, out=grey)
+            check(lib().nt2h_rgb2yuv_f32(n, ptr(r), ptr(g), ptr(b), ptr(y), ptr(u), ptr(v), pol, None), "rgb2yuv")
+            ops.mandelbrot(A, B, 256, pol, out=C)
+    _copy()
+
+
 if __name__ == "__main__":
     what = sys.argv[1] if len(sys.argv) > 1 else "bs"
-    {"bs": bs, "fold": fold, "sd": sd, "batch": batch, "dimwise": dimwise}[what]()
+    {"bs": bs, "fold": fold, "sd": sd, "batch": batch, "dimwise": dimwise, "images": images}[what]()
     torch.cuda.synchronize()
     print("profile_workloads %s done" % what, flush=True)
