@@ -14,7 +14,8 @@
 // single-precision hardware rcp/log2/rsq (delta ~ 1e-6, below 1e-9 of the
 // price), while tau, d2, F = X exp(-rT) and both normcdf are FP64:
 //   exp(w)   = 2^(k/32) e^r, |r| <= ln2/64: 2^(j/32) from a 32-entry LDS
-//              table, e^r of degree 3 (7e-11), 2^(k>>5) by ldexp; the
+//              table, e^r of degree 3 (7e-11, evaluated in units of ln2/32
+//              as a monic cubic), 2^(k>>5) by ldexp; the
 //              discount factor exp(-rT) for |rT| <= 1/4 (every usual rate
 //              and maturity) by one degree-7 polynomial instead (3e-12);
 //   N(-|d|)  = exp(-d^2/2) * erfcx(|d|/sqrt2)/2 with the second factor a
@@ -56,20 +57,36 @@ __host__ __device__ inline Params make_params(float r, float v) {
   return p;
 }
 
-// 2^(k>>5) * 2^((k&31)/32) * e^r0, k the low word of sh = rint(...) + kRound52
-__device__ __forceinline__ double exp_core(double sh, double r0, const double* __restrict__ ex2) {
+// The table path works in units of ln2/32: with t = w*32/ln2 and k = rint(t),
+// exp(w) = 2^(k>>5) * 2^((k&31)/32) * e^(r L), r = t - k in [-1/2, 1/2],
+// L = ln2/32.  r comes from one fma (t - k with t's product unrounded), and
+// e^(r L) = kE3 * (((r + kA2) r + kA1) r + kA0): kExpR rescaled to r and made
+// monic, its leading coefficient kE3 folded into the LDS table of 2^(j/32).
+// The first Horner step then reads one constant, and the reduction needs no
+// separate multiply.
+constexpr double kExpL = 0.693147180559945309417 / 32.0;
+constexpr double kE1 = kExpR[1] * kExpL;
+constexpr double kE2 = kExpR[2] * kExpL * kExpL;
+constexpr double kE3 = kExpR[3] * kExpL * kExpL * kExpL;
+constexpr double kA0 = kExpR[0] / kE3, kA1 = kE1 / kE3, kA2 = kE2 / kE3;
+constexpr double k32oLn2 = 46.166241308446828;    // 32/ln2
+constexpr double kM16oLn2 = -23.083120654223414;  // -16/ln2: exp(-q/2) = 2^(q * -16/ln2 / 32)
+
+// 2^(k>>5) * ex2[k&31] * m(r), k the low word of sh = rint(t) + kRound52 and
+// ex2[j] = 2^(j/32) kE3
+__device__ __forceinline__ double exp_core(double sh, double r, const double* __restrict__ ex2) {
   const int k = (int)(uint32_t)__builtin_bit_cast(uint64_t, sh);
-  double p = __builtin_fma(r0, kExpR[3], kExpR[2]);
-  p = __builtin_fma(p, r0, kExpR[1]);
-  p = __builtin_fma(p, r0, kExpR[0]);
+  double p = r + kA2;
+  p = __builtin_fma(p, r, kA1);
+  p = __builtin_fma(p, r, kA0);
   return __builtin_amdgcn_ldexp(ex2[k & 31] * p, k >> 5);
 }
 
 // exp(w) for w in [-700, 700]
 __device__ __forceinline__ double exp_w(double w, const double* __restrict__ ex2) {
-  const double sh = __builtin_fma(w, 46.166241308446828, kRound52);  // w * 32/ln2
+  const double sh = __builtin_fma(w, k32oLn2, kRound52);
   const double kd = sh - kRound52;
-  return exp_core(sh, __builtin_fma(kd, -0.693147180559945309417 / 32.0, w), ex2);
+  return exp_core(sh, __builtin_fma(w, k32oLn2, -kd), ex2);
 }
 
 // exp(w) for |w| <= kExpSmall
@@ -104,11 +121,12 @@ __device__ __forceinline__ void discount4(const double (&w)[4], const double* __
   }
 }
 
-// exp(-q/2) for q in [0, 1400]
+// exp(-q/2) for q in [0, 1400] (|r|'s error from the rounded constant: below
+// 1400 * 23.1 * 2^-53 = 3.6e-12 units, 8e-14 relative)
 __device__ __forceinline__ double exp_mhalf(double q, const double* __restrict__ ex2) {
-  const double sh = __builtin_fma(q, -23.083120654223414, kRound52);  // -q/2 * 32/ln2
+  const double sh = __builtin_fma(q, kM16oLn2, kRound52);
   const double kd = sh - kRound52;
-  return exp_core(sh, __builtin_fma(q, -0.5, kd * (-0.693147180559945309417 / 32.0)), ex2);
+  return exp_core(sh, __builtin_fma(q, kM16oLn2, -kd), ex2);
 }
 
 // erfcx(.)/2 polynomial of interval j at u in [0, 1)
@@ -193,10 +211,10 @@ __device__ __forceinline__ void bs_accurate4(const V4& s4, const V4& x4, const V
   for (int k = 0; k < 4; ++k) bs_price(s4[k], x4[k], t4[k], E[k], P, tab, ex2, call[k], put[k]);
 }
 
-// workgroup-wide copy of the tables into LDS (coefficient-major tail table, then 2^(j/32))
+// workgroup-wide copy of the tables into LDS (coefficient-major tail table, then 2^(j/32) kE3)
 __device__ __forceinline__ void load_tables(double* tab, double* ex2) {
   for (int i = threadIdx.x; i < (kTailDeg + 1) * kTailN; i += blockDim.x) tab[i] = kTail[i];
-  for (int i = threadIdx.x; i < 32; i += blockDim.x) ex2[i] = kExp2j[i];
+  for (int i = threadIdx.x; i < 32; i += blockDim.x) ex2[i] = kExp2j[i] * kE3;
   __syncthreads();
 }
 
