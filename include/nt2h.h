@@ -149,6 +149,12 @@ int nt2h_cumsum_dim_f64(const double* x, const size_t* ext, int dim, double* out
  * replaces the statement of demo/rgb2grey/nt2/rgb2grey_nt2.cpp:42-45. */
 int nt2h_rgb2grey_f32(size_t n, const float* r, const float* g, const float* b, float* grey, int policy,
                       void* stream);
+/* uint8 planes: grey = min(255, (77 r + 151 g + 28 b + 128) >> 8), the same
+ * weights in 8.8 fixed point, rounded and saturated (no counterpart in the
+ * reference, whose demo is float only: an integer variant of
+ * demo/rgb2grey/nt2/rgb2grey_nt2.cpp:42-45). */
+int nt2h_rgb2grey_u8(size_t n, const uint8_t* r, const uint8_t* g, const uint8_t* b, uint8_t* grey,
+                     void* stream);
 /* y = 0.299 r + 0.587 g + 0.114 b; u = 0.492 (b - y); v = 0.877 (r - y), one
  * pass: replaces rgb2yuv_work, demo/rgb2yuv/nt2/rgb2yuv_nt2.cpp:28-35. */
 int nt2h_rgb2yuv_f32(size_t n, const float* r, const float* g, const float* b, float* y, float* u, float* v,

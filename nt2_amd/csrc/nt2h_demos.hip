@@ -88,6 +88,42 @@ __global__ __launch_bounds__(kBlock) void rgb2yuv_k(size_t n, const float* __res
   }
 }
 
+// uint8 grey: the demo's weights 0.3, 0.59, 0.11 in 8.8 fixed point (77, 151,
+// 28; they sum to 256), rounded and saturated:
+// grey = min(255, (77 r + 151 g + 28 b + 128) >> 8).  Four pixels per 32-bit
+// word: the even and the odd bytes are spread into 16-bit lanes, where every
+// product and sum stays below 2^16.
+__device__ __forceinline__ uint32_t grey4_u8(uint32_t r, uint32_t g, uint32_t b) {
+  const uint32_t m = 0x00ff00ffu;
+  const uint32_t ev = ((r & m) * 77u + (g & m) * 151u + (b & m) * 28u + 0x00800080u) >> 8;
+  const uint32_t od = (((r >> 8) & m) * 77u + ((g >> 8) & m) * 151u + ((b >> 8) & m) * 28u + 0x00800080u) >> 8;
+  return (ev & m) | ((od & m) << 8);
+}
+__device__ __forceinline__ uint8_t grey1_u8(uint32_t r, uint32_t g, uint32_t b) {
+  const uint32_t v = (77u * r + 151u * g + 28u * b + 128u) >> 8;
+  return (uint8_t)(v > 255u ? 255u : v);
+}
+typedef uint32_t v4u __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(kBlock) void rgb2grey_u8_k(size_t n, const uint8_t* __restrict__ r,
+                                                        const uint8_t* __restrict__ g,
+                                                        const uint8_t* __restrict__ b,
+                                                        uint8_t* __restrict__ gr, int vec) {
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t n16 = vec ? n / 16 : 0;
+  if (t < n16) {
+    const v4u rv = __builtin_nontemporal_load(reinterpret_cast<const v4u*>(r) + t);
+    const v4u gv = __builtin_nontemporal_load(reinterpret_cast<const v4u*>(g) + t);
+    const v4u bv = __builtin_nontemporal_load(reinterpret_cast<const v4u*>(b) + t);
+    v4u o;
+#pragma unroll
+    for (int l = 0; l < 4; ++l) o[l] = grey4_u8(rv[l], gv[l], bv[l]);
+    __builtin_nontemporal_store(o, reinterpret_cast<v4u*>(gr) + t);
+  } else {
+    const size_t i = n16 * 16 + (t - n16);
+    if (i < n) gr[i] = grey1_u8(r[i], g[i], b[i]);
+  }
+}
+
 // linspace(l, u, n): u at the last index, p*step + l elsewhere
 // (modules/core/generative/include/nt2/core/functions/details/linspace.hpp:33-45)
 template <class P, class T>
@@ -181,6 +217,17 @@ int nt2h_rgb2grey_f32(size_t n, const float* r, const float* g, const float* b, 
   if (policy == NT2H_EXACT) return launch_rgb2grey<exact_policy>(n, r, g, b, grey, as_stream(stream));
   if (policy == NT2H_FAST) return launch_rgb2grey<fast_policy>(n, r, g, b, grey, as_stream(stream));
   return fail(NT2H_EINVAL, "nt2h_rgb2grey_f32: bad policy");
+}
+
+int nt2h_rgb2grey_u8(size_t n, const uint8_t* r, const uint8_t* g, const uint8_t* b, uint8_t* grey,
+                     void* stream) {
+  if (n == 0) return NT2H_OK;
+  if (!r || !g || !b || !grey) return fail(NT2H_EINVAL, "nt2h_rgb2grey_u8: null pointer");
+  const int vec = aligned16(r) && aligned16(g) && aligned16(b) && aligned16(grey);
+  const size_t items = vec ? n / 16 + n % 16 : n;
+  hipLaunchKernelGGL(rgb2grey_u8_k, dim3(grid_for(items)), dim3(kBlock), 0, as_stream(stream), n, r, g, b, grey, vec);
+  NT2H_CHECK_LAUNCH("rgb2grey_u8");
+  return NT2H_OK;
 }
 
 int nt2h_rgb2yuv_f32(size_t n, const float* r, const float* g, const float* b, float* y, float* u, float* v,

@@ -403,9 +403,11 @@ class table {
 
   // raw access
   const T* device_data() const { sync_dev(); return dev_; }
+  // device buffer for a kernel that overwrites every element: the host copy
+  // is not uploaded first (an in-place statement has already made this
+  // table's device copy current as one of its own terminals)
   T* device_data_for_write() {
     ensure_dev_alloc();
-    sync_dev();
     host_valid_ = false;
     dev_valid_ = true;
     return dev_;

@@ -71,6 +71,7 @@ def lib():
         "nt2h_cumsum_dim_f64": ([P, P, I, P, I, P], I),
         "nt2h_rgb2grey_f32": ([S, P, P, P, P, I, P], I),
         "nt2h_rgb2yuv_f32": ([S, P, P, P, P, P, P, I, P], I),
+        "nt2h_rgb2grey_u8": ([S, P, P, P, P, P], I),
         "nt2h_linspace_f32": ([S, F, F, P, I, P], I),
         "nt2h_linspace_f64": ([S, D, D, P, I, P], I),
         "nt2h_mandelbrot_f32": ([S, P, P, P, I, I, P], I),

@@ -150,6 +150,13 @@ def rgb2grey(r, g, b, policy=EXACT, out=None):
     return out
 
 
+def rgb2grey_u8(r, g, b, out=None):
+    """grey = min(255, (77 r + 151 g + 28 b + 128) >> 8) over uint8 device planes."""
+    out = torch.empty_like(r) if out is None else out
+    check(lib().nt2h_rgb2grey_u8(r.numel(), ptr(r), ptr(g), ptr(b), ptr(out), _s()), "rgb2grey_u8")
+    return out
+
+
 def rgb2yuv(r, g, b, policy=EXACT):
     y, u, v = torch.empty_like(r), torch.empty_like(r), torch.empty_like(r)
     check(lib().nt2h_rgb2yuv_f32(r.numel(), ptr(r), ptr(g), ptr(b), ptr(y), ptr(u), ptr(v), policy, _s()),

@@ -1471,6 +1471,14 @@ int oracle_rgb2grey_f32(size_t n, const float* r, const float* g, const float* b
   for (size_t i = 0; i < n; ++i) gr[i] = (cr * r[i] + cg * g[i]) + cb * b[i];
   return 0;
 }
+// uint8 variant: the weights in 8.8 fixed point, rounded, saturated
+int oracle_rgb2grey_u8(size_t n, const uint8_t* r, const uint8_t* g, const uint8_t* b, uint8_t* gr) {
+  for (size_t i = 0; i < n; ++i) {
+    const unsigned v = (77u * r[i] + 151u * g[i] + 28u * b[i] + 128u) >> 8;
+    gr[i] = (uint8_t)(v > 255u ? 255u : v);
+  }
+  return 0;
+}
 int oracle_rgb2yuv_f32(size_t n, const float* r, const float* g, const float* b, float* y, float* u,
                        float* v) {
   for (size_t i = 0; i < n; ++i) {

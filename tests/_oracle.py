@@ -54,6 +54,7 @@ def lib():
             getattr(_lib, f).argtypes = [P, P, ctypes.c_int, P]
         _lib.oracle_rgb2grey_f32.argtypes = [S, P, P, P, P]
         _lib.oracle_rgb2yuv_f32.argtypes = [S, P, P, P, P, P, P]
+        _lib.oracle_rgb2grey_u8.argtypes = [S, P, P, P, P]
         _lib.oracle_linspace_f32.argtypes = [S, ctypes.c_float, ctypes.c_float, P]
         _lib.oracle_linspace_f64.argtypes = [S, ctypes.c_double, ctypes.c_double, P]
         _lib.oracle_mandelbrot_f32.argtypes = [S, P, P, P, ctypes.c_int]
@@ -231,6 +232,14 @@ def rgb2grey(r, g, b):
     r, g, b = (np.ascontiguousarray(v, dtype=np.float32) for v in (r, g, b))
     out = np.empty_like(r)
     assert lib().oracle_rgb2grey_f32(r.size, _p(r), _p(g), _p(b), _p(out)) == 0
+    return out
+
+
+def rgb2grey_u8(r, g, b):
+    """min(255, (77 r + 151 g + 28 b + 128) >> 8) over uint8 planes."""
+    r, g, b = (np.ascontiguousarray(v, dtype=np.uint8) for v in (r, g, b))
+    out = np.empty_like(r)
+    assert lib().oracle_rgb2grey_u8(r.size, _p(r), _p(g), _p(b), _p(out)) == 0
     return out
 
 

@@ -235,6 +235,33 @@ int main(int argc, char** argv) {
   dump("mandel", MC);
   table<float> lin = linspace(0.0f, 1.0f);
   dump("linspace100", lin);
+
+  // ---- device storage: host and device copies stay coherent across statements
+  {
+    const std::size_t m = 100003;
+    table<float> h(of_size(m));
+    for (std::size_t i = 1; i <= m; ++i) h(i) = float(i);   // host writes
+    table<float> d2 = h * 2.0f;                               // uploads h, result on the device
+    table<float> probe(of_size(8));
+    probe(1) = d2(m);                                         // downloads d2: 2m
+    h(7) = -1.0f;                                             // host write after h went to the device
+    table<float> d3 = h + 0.0f;                               // must see -1 at 7 (re-upload)
+    probe(2) = d3(7);
+    probe(3) = d3(m);
+    table<float> cp = d3;                                     // device-to-device copy
+    cp(1) = 42.0f;                                            // the copy's host write
+    cp = cp + 1.0f;                                           // in place: 43 at 1
+    probe(4) = cp(1);
+    probe(5) = d3(1);                                         // the source is unchanged: 1
+    d3 = d3 * 3.0f;                                           // in place on the device, host stale
+    probe(6) = d3(m);                                         // 3m
+    h.resize(of_size(2 * m));                                 // new size: zeros
+    table<float> d4 = h + 1.0f;
+    probe(7) = globalsum(d4);                                 // 2m
+    probe(8) = d4(2 * m);
+    dump("coherence", probe);
+    dump("coherence_cp", cp);
+  }
   std::printf("frontend demos done: %zu elements\n", size);
   return 0;
 }

@@ -106,3 +106,17 @@ def test_mandelbrot_fast_close_to_exact():
     assert same > 0.95
     assert np.mean(fa[ex <= 2] == ex[ex <= 2]) > 0.999
     assert np.mean(fa[ex == 256] == 256) > 0.99
+
+
+@pytest.mark.parametrize("offset", [0, 5])
+def test_rgb2grey_u8_every_colour(offset):
+    """All 2^24 colours as one image (plus a misaligned start): equal to the
+    oracle byte for byte."""
+    v = np.arange(256, dtype=np.uint8)
+    planes = [a.ravel() for a in np.meshgrid(v, v, v, indexing="ij")]
+    planes = [np.concatenate([np.zeros(offset, np.uint8), p, v[:11]]) for p in planes]  # ragged tail
+    say("rgb2grey_u8 offset=%d" % offset)
+    d = [_dev(p)[offset:] for p in planes]
+    got = ops.rgb2grey_u8(*d).cpu().numpy()
+    want = O.rgb2grey_u8(*(p[offset:] for p in planes))
+    assert np.array_equal(got, want), first_diff(got, want)

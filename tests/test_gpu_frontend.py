@@ -163,3 +163,18 @@ def test_mandelbrot_demo_statements(outputs):
     want = O.mandelbrot(A, B, 256)
     assert np.array_equal(C, want), first_diff(C, want)
     assert same_bits(outputs("linspace100", np.float32), O.linspace(0.0, 1.0, 100))
+
+
+def test_device_storage_coherence(outputs):
+    """Tables keep a host and a device copy with validity flags: host writes
+    after device use are uploaded (all elements, not a prefix), device results
+    are downloaded on host access, copies are independent, in-place
+    statements read the current device copy, resizing starts from zeros."""
+    m = 100003
+    p = outputs("coherence", np.float32)
+    assert list(p) == [2.0 * m, -1.0, float(m), 43.0, 1.0, 3.0 * m, 2.0 * m, 1.0]
+    cp = outputs("coherence_cp", np.float32)
+    want = np.arange(1, m + 1, dtype=np.float32) + np.float32(1)
+    want[6] = np.float32(0.0)  # -1 + 1
+    want[0] = np.float32(43.0)
+    assert np.array_equal(cp, want)

@@ -150,3 +150,19 @@ def test_mandelbrot_pack_semantics():
     # a tail lane is a pack of one
     n = 8 * 5 + 3
     assert np.array_equal(O.mandelbrot(a[:n], b[:n])[40:], _alone(a[40:n], b[40:n]))
+
+
+def test_rgb2grey_u8_every_colour():
+    """All 2^24 colours: within one grey level of the float formula (the 8.8
+    weights differ from 0.3, 0.59, 0.11 by at most 0.2/256 each), grey input
+    returns itself, the primaries give their weights."""
+    v = np.arange(256, dtype=np.uint8)
+    r, g, b = (a.ravel() for a in np.meshgrid(v, v, v, indexing="ij"))
+    got = O.rgb2grey_u8(r, g, b).astype(np.int64)
+    f = 0.3 * r + 0.59 * g + 0.11 * b
+    assert np.all(np.abs(got - f) <= 1.0)
+    assert np.array_equal(O.rgb2grey_u8(v, v, v), v)
+    z = np.zeros(1, np.uint8)
+    full = np.full(1, 255, np.uint8)
+    assert O.rgb2grey_u8(full, z, z)[0] == 77 and O.rgb2grey_u8(z, full, z)[0] == 150
+    assert O.rgb2grey_u8(z, z, full)[0] == 28
