@@ -246,7 +246,8 @@ def leg_dimwise(torch, ops, O):
 def leg_images(torch, ops, O):
     """The image demos.  rgb2grey and rgb2yuv on an 8192 x 8192 image (planar
     f32, 2^26 pixels): algorithmic bytes 16 and 24 per pixel (three planes
-    read, one or three written).  mandelbrot on the demo's largest grid,
+    read, one or three written); the uint8 rgb2grey on uint8 planes: 4 bytes
+    per pixel.  mandelbrot on the demo's largest grid,
     1600 x 1600 points of [-1.5, 1.5]^2 with at most 256 iterations: the
     measure is pixels per second and counted iterations per second.  Exact
     results are checked bit for bit against the oracle, whose time is
@@ -268,6 +269,14 @@ def leg_images(torch, ops, O):
     for pname, pol in (("exact", EXACT), ("fast", FAST)):
         ms = time_steps(torch, lambda: yuv(pol), 20, 3)
         res["rgb2yuv_" + pname] = {"ms": ms, "gbs": 24.0 * n / (ms * 1e-3) / 1e9}
+    r8, g8, b8 = (torch.randint(0, 256, (n,), dtype=torch.uint8, device="cuda", generator=None) for _ in range(3))
+    grey8 = torch.empty_like(r8)
+    ms = time_steps(torch, lambda: ops.rgb2grey_u8(r8, g8, b8, out=grey8), 20, 3)
+    res["rgb2grey_u8"] = {"ms": ms, "gbs": 4.0 * n / (ms * 1e-3) / 1e9}
+    kk = 1 << 22
+    res["rgb2grey_u8_check"] = {"equal_vs_oracle_first_2^22": bool(np.array_equal(
+        grey8[:kk].cpu().numpy(), O.rgb2grey_u8(r8[:kk].cpu().numpy(), g8[:kk].cpu().numpy(), b8[:kk].cpu().numpy())))}
+    del r8, g8, b8, grey8
     k = 1 << 22  # bit check on the first 2^22 pixels (the kernel is position-independent)
     rh, gh, bh = (t[:k].cpu().numpy() for t in (r, g, b))
     ops.rgb2grey(r, g, b, EXACT, out=grey)

@@ -93,7 +93,10 @@ def images(reps=3):
     A, B = la.repeat_interleave(m), la.repeat(m)
     C = torch.empty(m * m, dtype=torch.int32, device="cuda")
     from .native import check, lib, ptr
+    r8, g8, b8 = (torch.randint(0, 256, (n,), dtype=torch.uint8, device="cuda") for _ in range(3))
+    grey8 = torch.empty_like(r8)
     for _ in range(reps):
+        ops.rgb2grey_u8(r8, g8, b8, out=grey8)
         for pol in (EXACT, FAST):
             ops.rgb2grey(r, g, b, pol, out=grey)
             check(lib().nt2h_rgb2yuv_f32(n, ptr(r), ptr(g), ptr(b), ptr(y), ptr(u), ptr(v), pol, None), "rgb2yuv")
