@@ -64,6 +64,9 @@ extern "C" {
 #define NT2H_OP_ROUND 16      /* half away from zero */
 #define NT2H_OP_ROUND2EVEN 17 /* half to even */
 #define NT2H_OP_SIGN 18
+#define NT2H_OP_EXPM1 19
+#define NT2H_OP_LOG1P 20
+#define NT2H_OP_TANH 21
 
 /* Black-Scholes formula variants */
 #define NT2H_BS_TEXTBOOK 0 /* d1 = (log(S/X)+(r+v^2/2)T)/(v sqrt T), call and put, erfc-based normcdf */

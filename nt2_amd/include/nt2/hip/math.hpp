@@ -383,6 +383,120 @@ NT2H_FN double sign_nt2(double a0) {
 }
 
 // ---------------------------------------------------------------------------
+// expm1: modules/core/exponential/include/nt2/exponential/functions/details/
+// expm1_kernel.hpp:26-50 (f32) and :53-85 (f64), with
+// simd/common/expm1.hpp:29-37 (-1 below Logeps, Inf above Maxlog).  The scale
+// factors 2^-k and 2^k are built in the exponent field as NT2 does
+// (shl(Maxexponent - k, Nbmantissabits), fast_ldexp), including its values at
+// the top of the range (k = 128 / 1024 makes 2^-k -Inf and the result Inf).
+// k is clamped before the integer conversion: lanes where that matters are
+// replaced by the -1 / Inf / NaN selects.
+// ---------------------------------------------------------------------------
+template <class P>
+NT2H_FN float expm1_f(float a0) {
+  NT2H_STRICT
+  constexpr float c[3] = {NT2H_F(0x3acf6db4), NT2H_F(0xbd08887f), NT2H_F(0x3f800000)};
+  const float k = __builtin_rintf(NT2H_F(0x3fb8aa3b) * a0);
+  float x = fnms<P>(k, NT2H_F(0x3f318000), a0);
+  x = fnms<P>(k, NT2H_F(0xb95e8083), x);
+  const float hx = x * 0.5f;
+  const float hxs = x * hx;
+  const float r1 = horner<P>(c, hxs);
+  const float t = fnms<P>(r1, hx, 3.0f);
+  float e = hxs * ((r1 - t) / (6.0f - x * t));
+  e = fma<P>(x, e, -hxs);
+  const int32_t ik = (k == k) ? (int32_t)__builtin_fminf(__builtin_fmaxf(k, -200.0f), 200.0f) : 0;
+  const float two2mk = as_f((uint32_t)(127 - ik) << 23);
+  const float y = (1.0f - two2mk) - (e - x);
+  float r = y * as_f((uint32_t)(ik + 127) << 23);
+  r = (a0 > NT2H_F(0x42b0c0a5)) ? __builtin_inff() : r;
+  return (a0 < NT2H_F(0xc17f1402)) ? -1.0f : r;
+}
+template <class P>
+NT2H_FN double expm1_d(double a0) {
+  NT2H_STRICT
+  constexpr double c[6] = {NT2H_D(0xBE8AFDB76E09C32D), NT2H_D(0x3ED0CFCA86E65239),
+                           NT2H_D(0xBF14CE199EAADBB7), NT2H_D(0x3F5A01A019FE5585),
+                           NT2H_D(0xBFA11111111110F4), NT2H_D(0x3ff0000000000000)};
+  const double k = __builtin_rint(NT2H_D(0x3ff71547652b82fe) * a0);
+  const double hi = fnms<P>(k, NT2H_D(0x3fe62e42fee00000), a0);
+  const double lo = k * NT2H_D(0x3dea39ef35793c76);
+  const double x = hi - lo;
+  const double hxs = (x * x) * 0.5;
+  const double r1 = horner<P>(c, hxs);
+  const double t = 3.0 - (r1 * 0.5) * x;
+  double e = hxs * ((r1 - t) / (6.0 - x * t));
+  const double cc = (hi - x) - lo;
+  e = (x * (e - cc) - cc) - hxs;
+  const int64_t ik = (k == k) ? (int64_t)__builtin_fmin(__builtin_fmax(k, -2000.0), 2000.0) : 0;
+  const double two2mk = as_d((uint64_t)(1023 - ik) << 52);
+  const double ct1 = (1.0 - two2mk) - (e - x);
+  const double ct2 = 1.0 + (x - (e + two2mk));
+  const double y = (k < 20.0) ? ct1 : ct2;
+  double r = y * as_d((uint64_t)(ik + 1023) << 52);
+  r = (a0 > NT2H_D(0x40862e42fefa39ef)) ? __builtin_inf() : r;
+  return (a0 < NT2H_D(0xc04205966f2b4f19)) ? -1.0 : r;
+}
+
+// log1p: simd/common/log1p.hpp:26-37, u = 1 + a0 and log(u) corrected by
+// (a0 - (u - 1))/u where u != 0, Inf at Inf (the scalar form,
+// scalar/log1p.hpp:27-37, returns the same values)
+template <class P>
+NT2H_FN float log1p_f(float a0) {
+  NT2H_STRICT
+  const float u = 1.0f + a0;
+  const float l = log_f<P>(u);
+  const float r = (u != 0.0f) ? l + (a0 - (u - 1.0f)) / u : l;
+  return (u == __builtin_inff()) ? u : r;
+}
+template <class P>
+NT2H_FN double log1p_d(double a0) {
+  NT2H_STRICT
+  const double u = 1.0 + a0;
+  const double l = log_d<P>(u);
+  const double r = (u != 0.0) ? l + (a0 - (u - 1.0)) / u : l;
+  return (u == __builtin_inf()) ? u : r;
+}
+
+// tanh: modules/core/hyperbolic/include/nt2/hyperbolic/functions/simd/common/tanh.hpp:27-45:
+// |a0| < 5/8 by the Cephes kernel (details/tanh_kernel.hpp:24-36 f32
+// polynomial, :44-71 f64 rational) on |a0|, else 1 - 2/(1 + exp(2|a0|)); the
+// sign of a0 is xored in.  The choice is per lane, so a lane's bits do not
+// depend on its pack.
+template <class P>
+NT2H_FN float tanh_f(float a0) {
+  NT2H_STRICT
+  constexpr float c[5] = {NT2H_F(0xbbbaf0ea), NT2H_F(0x3ca9134e), NT2H_F(0xbd5c1e2d),
+                          NT2H_F(0x3e088393), NT2H_F(0xbeaaaa99)};
+  const float x = __builtin_fabsf(a0);
+  float r;
+  if (x < 0.625f) {
+    const float x2 = x * x;
+    r = fma<P>(horner<P>(c, x2) * x2, x, x);
+  } else {
+    r = fma<P>(-2.0f, 1.0f / (1.0f + exp_f<P>(x + x)), 1.0f);
+  }
+  return as_f(bits(r) ^ (bits(a0) & 0x80000000u));
+}
+template <class P>
+NT2H_FN double tanh_d(double a0) {
+  NT2H_STRICT
+  constexpr double p[3] = {NT2H_D(0xbfeedc5baafd6f4b), NT2H_D(0xc058d26a0e26682d),
+                           NT2H_D(0xc0993ac030580563)};
+  constexpr double q[4] = {NT2H_D(0x3FF0000000000000), NT2H_D(0x405c33f28a581b86),
+                           NT2H_D(0x40a176fa0e5535fa), NT2H_D(0x40b2ec102442040c)};
+  const double x = __builtin_fabs(a0);
+  double r;
+  if (x < 0.625) {
+    const double x2 = x * x;
+    r = fma<P>((x2 * horner<P>(p, x2)) / horner<P>(q, x2), x, x);
+  } else {
+    r = fma<P>(-2.0, 1.0 / (1.0 + exp_d<P>(x + x)), 1.0);
+  }
+  return as_d(bits(r) ^ (bits(a0) & 0x8000000000000000ull));
+}
+
+// ---------------------------------------------------------------------------
 // erfc.  f32: modules/core/euler/include/nt2/euler/functions/simd/common/erfc.hpp:92-123
 // (the scalar form returns the same values), kernels erf_kernel.hpp:46-85.
 // erfc_abs_f returns erfc(|a|) before the sign fold, so erfc(a) and erfc(-a)

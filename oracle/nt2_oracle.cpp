@@ -382,6 +382,92 @@ inline double log10_d(double a0) {
 }
 
 // ===========================================================================
+// expm1, log1p, tanh
+// ===========================================================================
+// expm1: modules/core/exponential/include/nt2/exponential/functions/details/
+// expm1_kernel.hpp:26-50 (f32) and :53-85 (f64); simd/common/expm1.hpp:29-37
+// (-1 below Logeps = constants/logeps.hpp:30, Inf above Maxlog).  2^-k and the
+// final 2^k are the exponent-field constructions shl(Maxexponent - k, 23/52)
+// and fast_ldexp (boost/simd/ieee/functions/simd/common/fast_ldexp.hpp).
+inline float expm1_f(float a0) {
+  const float cc[3] = {F32(0x3acf6db4), F32(0xbd08887f), F32(0x3f800000)};
+  float k = r2e(fInvlog_2 * a0);
+  float x = a0 - k * fLog_2hi;
+  x = x - k * fLog_2lo;
+  float hx = x * 0.5f;
+  float hxs = x * hx;
+  float r1 = horner(cc, hxs);
+  float t = 3.0f - r1 * hx;
+  float e = hxs * ((r1 - t) / (6.0f - x * t));
+  e = x * e - hxs;
+  int32_t ik = (k != k) ? 0 : (int32_t)std::fmin(std::fmax(k, -200.0f), 200.0f);
+  float two2mk = as_f((uint32_t)(127 - ik) << 23);
+  float y = (1.0f - two2mk) - (e - x);
+  float r = y * as_f((uint32_t)(ik + 127) << 23);
+  if (a0 > fMaxlog) r = INFINITY;
+  if (a0 < F32(0xc17f1402)) r = -1.0f;
+  return r;
+}
+inline double expm1_d(double a0) {
+  const double cc[6] = {F64(0xBE8AFDB76E09C32D), F64(0x3ED0CFCA86E65239), F64(0xBF14CE199EAADBB7),
+                        F64(0x3F5A01A019FE5585), F64(0xBFA11111111110F4), F64(0x3ff0000000000000)};
+  double k = r2e(dInvlog_2 * a0);
+  double hi = a0 - k * dLog_2hi;
+  double lo = k * dLog_2lo;
+  double x = hi - lo;
+  double hxs = (x * x) * 0.5;
+  double r1 = horner(cc, hxs);
+  double t = 3.0 - (r1 * 0.5) * x;
+  double e = hxs * ((r1 - t) / (6.0 - x * t));
+  double c = (hi - x) - lo;
+  e = (x * (e - c) - c) - hxs;
+  int64_t ik = (k != k) ? 0 : (int64_t)std::fmin(std::fmax(k, -2000.0), 2000.0);
+  double two2mk = as_d((uint64_t)(1023 - ik) << 52);
+  double ct1 = (1.0 - two2mk) - (e - x);
+  double ct2 = 1.0 + (x - (e + two2mk));
+  double y = (k < 20.0) ? ct1 : ct2;
+  double r = y * as_d((uint64_t)(ik + 1023) << 52);
+  if (a0 > dMaxlog) r = (double)INFINITY;
+  if (a0 < F64(0xc04205966f2b4f19)) r = -1.0;
+  return r;
+}
+// log1p: simd/common/log1p.hpp:26-37 (scalar/log1p.hpp:27-37 gives the same values)
+template <class T, class L> inline T log1p_t(T a0, L logf) {
+  T u = T(1) + a0;
+  T l = logf(u);
+  T r = (u != T(0)) ? l + (a0 - (u - T(1))) / u : l;
+  return (u == (T)INFINITY) ? u : r;
+}
+inline float log1p_f(float a0) { return log1p_t(a0, log_f); }
+inline double log1p_d(double a0) { return log1p_t(a0, log_d); }
+// tanh: modules/core/hyperbolic/include/nt2/hyperbolic/functions/simd/common/tanh.hpp:27-45,
+// details/tanh_kernel.hpp:24-36 (f32) and :44-71 (f64)
+inline float tanh_f(float a0) {
+  const float cc[5] = {F32(0xbbbaf0ea), F32(0x3ca9134e), F32(0xbd5c1e2d), F32(0x3e088393), F32(0xbeaaaa99)};
+  float x = std::fabs(a0), r;
+  if (x < 0.625f) {
+    float x2 = x * x;
+    r = (horner(cc, x2) * x2) * x + x;
+  } else {
+    r = -2.0f * (1.0f / (1.0f + exp_f(x + x))) + 1.0f;
+  }
+  return as_f(bits(r) ^ (bits(a0) & 0x80000000u));
+}
+inline double tanh_d(double a0) {
+  const double p[3] = {F64(0xbfeedc5baafd6f4b), F64(0xc058d26a0e26682d), F64(0xc0993ac030580563)};
+  const double q[4] = {F64(0x3FF0000000000000), F64(0x405c33f28a581b86), F64(0x40a176fa0e5535fa),
+                       F64(0x40b2ec102442040c)};
+  double x = std::fabs(a0), r;
+  if (x < 0.625) {
+    double x2 = x * x;
+    r = ((x2 * horner(p, x2)) / horner(q, x2)) * x + x;
+  } else {
+    r = -2.0 * (1.0 / (1.0 + exp_d(x + x))) + 1.0;
+  }
+  return as_d(bits(r) ^ (bits(a0) & 0x8000000000000000ull));
+}
+
+// ===========================================================================
 // rounding and sign
 // ===========================================================================
 // floor / ceil / trunc / round2even are the IEEE operations (SSE4.1/AVX
@@ -1070,7 +1156,7 @@ enum {
   OR_EXP = 0, OR_LOG = 1, OR_SQRT = 2, OR_ERFC = 3, OR_NORMCDF = 4,
   OR_FASTNORMCDF = 5, OR_COS = 6, OR_SIN = 7, OR_ABS = 8, OR_EXP2 = 9, OR_EXP10 = 10,
   OR_LOG2 = 11, OR_LOG10 = 12, OR_FLOOR = 13, OR_CEIL = 14, OR_TRUNC = 15, OR_ROUND = 16,
-  OR_ROUND2EVEN = 17, OR_SIGN = 18, OR_LAST = OR_SIGN
+  OR_ROUND2EVEN = 17, OR_SIGN = 18, OR_EXPM1 = 19, OR_LOG1P = 20, OR_TANH = 21, OR_LAST = OR_TANH
 };
 
 int oracle_abi_version(void) { return 1; }
@@ -1108,6 +1194,9 @@ int oracle_map_f32(int op, const float* x, float* y, size_t n) {
       case OR_ROUND: for (size_t l = 0; l < w; ++l) y[i + l] = std::round(x[i + l]); break;
       case OR_ROUND2EVEN: for (size_t l = 0; l < w; ++l) y[i + l] = std::nearbyint(x[i + l]); break;
       case OR_SIGN: for (size_t l = 0; l < w; ++l) y[i + l] = sign_of(x[i + l]); break;
+      case OR_EXPM1: for (size_t l = 0; l < w; ++l) y[i + l] = expm1_f(x[i + l]); break;
+      case OR_LOG1P: for (size_t l = 0; l < w; ++l) y[i + l] = log1p_f(x[i + l]); break;
+      case OR_TANH: for (size_t l = 0; l < w; ++l) y[i + l] = tanh_f(x[i + l]); break;
       case OR_COS:
       case OR_SIN:
         if (pack) trig_group_f<8>(op == OR_COS ? 0 : 1, x + i, y + i, nullptr);
@@ -1139,6 +1228,9 @@ int oracle_map_f64(int op, const double* x, double* y, size_t n) {
       case OR_ROUND: for (size_t l = 0; l < w; ++l) y[i + l] = std::round(x[i + l]); break;
       case OR_ROUND2EVEN: for (size_t l = 0; l < w; ++l) y[i + l] = std::nearbyint(x[i + l]); break;
       case OR_SIGN: for (size_t l = 0; l < w; ++l) y[i + l] = sign_of(x[i + l]); break;
+      case OR_EXPM1: for (size_t l = 0; l < w; ++l) y[i + l] = expm1_d(x[i + l]); break;
+      case OR_LOG1P: for (size_t l = 0; l < w; ++l) y[i + l] = log1p_d(x[i + l]); break;
+      case OR_TANH: for (size_t l = 0; l < w; ++l) y[i + l] = tanh_d(x[i + l]); break;
       case OR_COS:
       case OR_SIN:
         if (pack) trig_group_d<4>(op == OR_COS ? 0 : 1, x + i, y + i, nullptr);

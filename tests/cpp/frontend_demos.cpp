@@ -194,6 +194,8 @@ int main(int argc, char** argv) {
   dump("ew_f32", ew);
   ew = nt2::round2even(yn / 16.0f) * nt2::sign(xn) + nt2::exp2(nt2::floor(xn) + nt2::ceil(-xn) / 4.0f);
   dump("ew2_f32", ew);
+  ew = nt2::tanh(xn) * nt2::expm1(xn / 4.0f) + nt2::log1p(nt2::abs(yn));
+  dump("ew3_f32", ew);
   table<double> dm(of_size(mh * mw)), dr;
   for (std::size_t i = 1; i <= mh * mw; ++i) dm(i) = (double)M(i) / 100.0;
   dr = nt2::trunc(dm) * nt2::exp2(nt2::ceil(dm / 50.0)) + nt2::sign(dm) * nt2::log10(nt2::abs(dm) + 1.0);

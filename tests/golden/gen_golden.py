@@ -65,6 +65,9 @@ FUNCS = {
     "exp10": lambda x: mpmath.power(10, x),
     "log2": lambda x: mpmath.log(x, 2) if x > 0 else (mpmath.mpf("-inf") if x == 0 else mpmath.nan),
     "log10": lambda x: mpmath.log10(x) if x > 0 else (mpmath.mpf("-inf") if x == 0 else mpmath.nan),
+    "expm1": lambda x: mpmath.expm1(x),
+    "log1p": lambda x: mpmath.log1p(x) if x > -1 else (mpmath.mpf("-inf") if x == -1 else mpmath.nan),
+    "tanh": lambda x: mpmath.tanh(x),
 }
 
 
@@ -81,6 +84,20 @@ def inputs(name, dtype, rng):
         lim = (126.0 if dtype is np.float32 else 1021.0) / (1.0 if name == "exp2" else np.log2(10.0))
         pts += list(rng.uniform(-lim, lim, 1200)) + list(rng.uniform(-1, 1, 300))
         pts += [0.0, 1.0, -1.0, 0.5, -0.5, 2.0, 3.0, 10.0, -10.0, 1e-8, -1e-8]
+    elif name == "expm1":
+        lim = 88.0 if dtype is np.float32 else 709.0
+        pts += list(rng.uniform(-20, lim, 800)) + list(rng.uniform(-1, 1, 400))
+        pts += list(np.exp2(rng.uniform(-60, -1, 200)) * rng.choice([-1.0, 1.0], 200))
+        pts += [0.0, 1.0, -1.0, 0.5, -0.5, 0.34657359, -0.34657359, 1e-8, -1e-8, 10.0]
+    elif name == "log1p":
+        pts += list(rng.uniform(-0.99, 1, 600)) + list(np.exp2(rng.uniform(-60, 100, 600)))
+        pts += list(-np.exp2(rng.uniform(-60, -1, 200)))
+        pts += [0.0, 1.0, -0.5, 1e-8, -1e-8, np.e - 1, 1e10]
+    elif name == "tanh":
+        pts += list(rng.uniform(-0.625, 0.625, 500)) + list(rng.uniform(-10, 10, 600))
+        pts += list(np.exp2(rng.uniform(-60, -2, 200)) * rng.choice([-1.0, 1.0], 200))
+        pts += list(rng.uniform(10, 25, 100))
+        pts += [0.0, 0.625, -0.625, 0.6249, 1.0, -1.0, 5.0, 20.0]
     elif name in ("log", "log2", "log10"):
         e = rng.uniform(-120 if dtype is np.float32 else -1000, 120 if dtype is np.float32 else 1000, 1200)
         pts += list(np.exp2(e)) + list(rng.uniform(0.5, 2.0, 300))
@@ -213,6 +230,29 @@ KATS = [
     ("log10", "-1", "nan", 0, "modules/core/exponential/unit/simd/log10.cpp:49"),
     ("log10", "0", "-inf", 0, "modules/core/exponential/unit/simd/log10.cpp:50"),
     ("log10", "1", "0", 0, "modules/core/exponential/unit/simd/log10.cpp:52"),
+    # modules/core/exponential/unit/simd/expm1.cpp:44-50
+    ("expm1", "inf", "inf", 1.0, "modules/core/exponential/unit/simd/expm1.cpp:44"),
+    ("expm1", "-inf", "-1", 1.0, "modules/core/exponential/unit/simd/expm1.cpp:45"),
+    ("expm1", "nan", "nan", 1.0, "modules/core/exponential/unit/simd/expm1.cpp:46"),
+    ("expm1", "-1", "1/e-1", 1.0, "modules/core/exponential/unit/simd/expm1.cpp:48"),
+    ("expm1", "1", "e-1", 1.0, "modules/core/exponential/unit/simd/expm1.cpp:49"),
+    ("expm1", "0", "0", 1.0, "modules/core/exponential/unit/simd/expm1.cpp:50"),
+    # modules/core/exponential/unit/simd/log1p.cpp:46-54
+    ("log1p", "inf", "inf", 0.5, "modules/core/exponential/unit/simd/log1p.cpp:46"),
+    ("log1p", "-inf", "nan", 0.5, "modules/core/exponential/unit/simd/log1p.cpp:47"),
+    ("log1p", "nan", "nan", 0.5, "modules/core/exponential/unit/simd/log1p.cpp:48"),
+    ("log1p", "-1", "-inf", 0.5, "modules/core/exponential/unit/simd/log1p.cpp:49"),
+    ("log1p", "eps", "eps", 0.5, "modules/core/exponential/unit/simd/log1p.cpp:51"),
+    ("log1p", "1", "ln2", 0.5, "modules/core/exponential/unit/simd/log1p.cpp:52"),
+    ("log1p", "0", "0", 0.5, "modules/core/exponential/unit/simd/log1p.cpp:53"),
+    ("log1p", "tiny", "tiny", 0.5, "modules/core/exponential/unit/simd/log1p.cpp:54"),
+    # modules/core/hyperbolic/unit/simd/tanh.cpp:43-49
+    ("tanh", "inf", "1", 0.5, "modules/core/hyperbolic/unit/simd/tanh.cpp:43"),
+    ("tanh", "-inf", "-1", 0.5, "modules/core/hyperbolic/unit/simd/tanh.cpp:44"),
+    ("tanh", "nan", "nan", 0.5, "modules/core/hyperbolic/unit/simd/tanh.cpp:45"),
+    ("tanh", "-1", "-0.7615941559557649", 0.5, "modules/core/hyperbolic/unit/simd/tanh.cpp:47"),
+    ("tanh", "1", "0.7615941559557649", 0.5, "modules/core/hyperbolic/unit/simd/tanh.cpp:48"),
+    ("tanh", "0", "0", 0.5, "modules/core/hyperbolic/unit/simd/tanh.cpp:49"),
 ]
 
 

@@ -129,6 +129,9 @@ def test_exponential_rounding_statements(outputs):
     want = b + O.map_f32("exp2", O.map_f32("floor", xn) + O.map_f32("ceil", -xn) / f32(4))
     got = outputs("ew2_f32", f32)
     assert same_bits(got, want), first_diff(got, want)
+    want = (O.map_f32("tanh", xn) * O.map_f32("expm1", xn / f32(4))) + O.map_f32("log1p", np.abs(yn))
+    got = outputs("ew3_f32", f32)
+    assert same_bits(got, want), first_diff(got, want)
     d = outputs("M", f32).astype(np.float64) / 100.0
     want = (O.map_f64("trunc", d) * O.map_f64("exp2", O.map_f64("ceil", d / 50.0)) +
             O.map_f64("sign", d) * O.map_f64("log10", np.abs(d) + 1.0))

@@ -18,7 +18,12 @@ G = os.path.join(os.path.dirname(__file__), "golden")
 def _val(s, dt):
     table = {"inf": np.inf, "-inf": -np.inf, "nan": np.nan, "e": np.e, "1/e": 1 / np.e,
              "pi": np.pi, "-pi": -np.pi, "pi/2": np.pi / 2, "-pi/2": -np.pi / 2,
-             "pi/4": np.pi / 4, "-pi/4": -np.pi / 4, "sqrt(2)/2": np.sqrt(2) / 2, "-0": -0.0}
+             "pi/4": np.pi / 4, "-pi/4": -np.pi / 4, "sqrt(2)/2": np.sqrt(2) / 2, "-0": -0.0,
+             "ln2": np.log(2.0), "eps": np.finfo(dt).eps, "tiny": np.finfo(dt).tiny}
+    if s == "e-1":  # NT2 forms these in the target precision: Exp_1<T>() - One<T>()
+        return dt(np.e) - dt(1)
+    if s == "1/e-1":
+        return dt(1) / dt(np.e) - dt(1)
     return dt(table[s]) if s in table else dt(float(s))
 
 
@@ -48,6 +53,8 @@ def _bound(name, x, y, dt):
     ax = np.abs(x.astype(np.float64))
     if name in ("exp", "log", "exp2", "exp10", "log2"):
         return np.full(x.shape, 1.0)
+    if name in ("expm1", "log1p", "tanh"):
+        return np.full(x.shape, BOUND3[name][dt])
     if name == "log10":
         # Log10_2hi*k is added before Log10_2lo*k (f_log.hpp:163-169), so the
         # large term is rounded twice: up to 1.6 ulp from the exact value in
@@ -75,10 +82,16 @@ def _bound(name, x, y, dt):
 
 
 @pytest.mark.parametrize("dt,tag", [(np.float32, "f32"), (np.float64, "f64")])
-@pytest.mark.parametrize("name", ["exp", "log", "erfc", "normcdf", "cos", "sin", "exp2", "exp10", "log2", "log10"])
+@pytest.mark.parametrize("name", ["exp", "log", "erfc", "normcdf", "cos", "sin", "exp2", "exp10", "log2", "log10",
+                                  "expm1", "log1p", "tanh"])
 def test_golden_mpmath(dt, tag, name):
     g = np.load(os.path.join(G, "math_%s.npz" % tag))
     x, y = g[name + "_x"], g[name + "_y"]
+    if name == "expm1":
+        # above 88.38 (f32) / 709.44 (f64) k reaches 128 / 1024 and NT2's
+        # 2^-k construction gives Inf (checked in test_expm1_tanh_log1p_structure)
+        keep = x < (88.37 if dt is np.float32 else 709.43)
+        x, y = x[keep], y[keep]
     if name in ("exp2", "exp10"):
         # at or below Minlog2 / Minlog10 NT2 returns 0 (constants/minlog2.hpp:29,
         # minlog10.hpp:29), also where the result is still normal; checked
@@ -108,6 +121,11 @@ def test_golden_mpmath(dt, tag, name):
         ae = np.abs(got.astype(np.float64) - y.astype(np.float64))
         assert (ae[near] <= 4 * np.finfo(dt).eps).all()
 
+
+# classic-ulp bounds against the correctly rounded value (measured maxima:
+# the constructions round several times)
+BOUND3 = {"expm1": {np.float32: 2.0, np.float64: 1.0}, "log1p": {np.float32: 1.0, np.float64: 1.0},
+          "tanh": {np.float32: 1.0, np.float64: 1.0}}
 
 MINLOG = {"exp2": {np.float32: np.float32(-127.0), np.float64: -1022.0},
           "exp10": {np.float32: np.uint32(0xc2179999).view(np.float32),
@@ -277,3 +295,35 @@ def test_sincos_matches_sin_and_cos():
         s, c = O.sincos(x)
         mapf = O.map_f32 if dt is np.float32 else O.map_f64
         assert np.array_equal(s, mapf("sin", x)) and np.array_equal(c, mapf("cos", x))
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_expm1_tanh_log1p_structure(dt):
+    """tanh is odd bit for bit and saturates at 1; expm1 agrees with exp - 1
+    where no cancellation occurs and is -1 below Logeps; log1p(x) = x for tiny
+    x and inverts expm1 within the roundings; NT2's Inf of expm1 at the top of
+    its range (k = 128 / 1024)."""
+    mapf = O.map_f32 if dt is np.float32 else O.map_f64
+    W = 8 if dt is np.float32 else 4
+    rng = np.random.default_rng(51)
+    x = np.concatenate([rng.uniform(-12, 12, 4000), np.exp2(rng.uniform(-40, 3, 4000))]).astype(dt)
+    t = _iso(mapf, "tanh", x, W)
+    tn = _iso(mapf, "tanh", -x, W)
+    assert np.array_equal(t.view(np.uint32 if dt is np.float32 else np.uint64) ^ (1 << (31 if dt is np.float32 else 63)),
+                          tn.view(np.uint32 if dt is np.float32 else np.uint64))
+    assert np.all(np.abs(t) <= 1)
+    big = np.array([20.0, 50.0, 1e6], dtype=dt) if dt is np.float32 else np.array([25.0, 50.0, 1e6], dtype=dt)
+    assert np.all(_iso(mapf, "tanh", big, W) == 1)
+    e1 = _iso(mapf, "expm1", x[x > 1].astype(dt), W).astype(np.float64)
+    ex = _iso(mapf, "exp", x[x > 1].astype(dt), W).astype(np.float64) - 1.0
+    assert np.all(np.abs(e1 - ex) <= 4 * np.spacing(np.abs(ex).astype(dt)).astype(np.float64))
+    lo = np.uint32(0xc17f1402).view(np.float32) if dt is np.float32 else np.uint64(0xc04205966f2b4f19).view(np.float64)
+    assert np.all(_iso(mapf, "expm1", np.array([lo * 1.01, -1e30, -np.inf], dtype=dt), W) == -1)
+    top = np.array([88.5, 88.7], dtype=dt) if dt is np.float32 else np.array([709.5, 709.7], dtype=dt)
+    assert np.all(np.isinf(_iso(mapf, "expm1", top, W)))
+    # below eps, x^2/2 is under half an ulp of x
+    tiny = np.exp2(rng.uniform(-60, -26, 100) if dt is np.float32 else rng.uniform(-300, -55, 100)).astype(dt)
+    assert np.array_equal(_iso(mapf, "log1p", tiny, W), tiny)
+    u = rng.uniform(-0.9, 5, 2000).astype(dt)
+    back = _iso(mapf, "log1p", _iso(mapf, "expm1", u, W), W).astype(np.float64)
+    assert np.all(np.abs(back - u) <= 8 * np.spacing(np.maximum(np.abs(u), np.finfo(dt).tiny)).astype(np.float64) + 4 * np.finfo(dt).eps)
