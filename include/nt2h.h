@@ -67,6 +67,8 @@ extern "C" {
 #define NT2H_OP_EXPM1 19
 #define NT2H_OP_LOG1P 20
 #define NT2H_OP_TANH 21
+#define NT2H_OP_SINH 22
+#define NT2H_OP_COSH 23
 
 /* Black-Scholes formula variants */
 #define NT2H_BS_TEXTBOOK 0 /* d1 = (log(S/X)+(r+v^2/2)T)/(v sqrt T), call and put, erfc-based normcdf */

@@ -496,6 +496,65 @@ NT2H_FN double tanh_d(double a0) {
   return as_d(bits(r) ^ (bits(a0) & 0x8000000000000000ull));
 }
 
+// sinh: modules/core/hyperbolic/include/nt2/hyperbolic/functions/simd/common/sinh.hpp:27-47
+// (|a0| < 1: details/sinh_kernel.hpp:25-37 f32 polynomial, :44-66 f64
+// rational; above: (e - 1/e)/2 with e = exp(|a0|), or (e/2) e with
+// e = exp(|a0|/2) above Maxlog - Log_2), sign xored in; the scalar form
+// (scalar/sinh.hpp:27-56) gives the same values.
+// cosh: generic/cosh.hpp:27-45, average(e, 1/e) = e/2 + (1/e)/2, or (e/2) e.
+template <class P>
+NT2H_FN float sinh_f(float a0) {
+  NT2H_STRICT
+  constexpr float c[4] = {NT2H_F(0x39559e2f), NT2H_F(0x3c087bbe), NT2H_F(0x3e2aaacc), NT2H_F(0x3f800000)};
+  const float x = __builtin_fabsf(a0);
+  float r;
+  if (x < 1.0f) {
+    r = horner<P>(c, x * x) * x;
+  } else {
+    const bool big = x > NT2H_F(0x42b0c0a5) - NT2H_F(0x3f317218);
+    const float e = exp_f<P>(x * (big ? 0.5f : 1.0f));
+    const float h = 0.5f * e;
+    r = big ? h * e : h - 0.5f * (1.0f / e);
+  }
+  return as_f(bits(r) ^ (bits(a0) & 0x80000000u));
+}
+template <class P>
+NT2H_FN double sinh_d(double a0) {
+  NT2H_STRICT
+  constexpr double p[4] = {NT2H_D(0xbfe9435fe8bb3cd6), NT2H_D(0xc064773a398ff4fe),
+                           NT2H_D(0xc0c694b8c71d6182), NT2H_D(0xc115782bdbf6ab05)};
+  constexpr double q[4] = {NT2H_D(0x3ff0000000000000), NT2H_D(0xc0715b6096e96484),
+                           NT2H_D(0x40e1a7ba7ed72245), NT2H_D(0xc1401a20e4f90044)};
+  const double x = __builtin_fabs(a0);
+  double r;
+  if (x < 1.0) {
+    const double x2 = x * x;
+    r = fma<P>(x, (x2 * horner<P>(p, x2)) / horner<P>(q, x2), x);
+  } else {
+    const bool big = x > NT2H_D(0x40862e42fefa39ef) - NT2H_D(0x3fe62e42fefa39ef);
+    const double e = exp_d<P>(x * (big ? 0.5 : 1.0));
+    const double h = 0.5 * e;
+    r = big ? h * e : h - 0.5 * (1.0 / e);
+  }
+  return as_d(bits(r) ^ (bits(a0) & 0x8000000000000000ull));
+}
+template <class P>
+NT2H_FN float cosh_f(float a0) {
+  NT2H_STRICT
+  const float x = __builtin_fabsf(a0);
+  const bool big = x > NT2H_F(0x42b0c0a5) - NT2H_F(0x3f317218);
+  const float e = exp_f<P>(x * (big ? 0.5f : 1.0f));
+  return big ? (0.5f * e) * e : fma<P>(e, 0.5f, (1.0f / e) * 0.5f);
+}
+template <class P>
+NT2H_FN double cosh_d(double a0) {
+  NT2H_STRICT
+  const double x = __builtin_fabs(a0);
+  const bool big = x > NT2H_D(0x40862e42fefa39ef) - NT2H_D(0x3fe62e42fefa39ef);
+  const double e = exp_d<P>(x * (big ? 0.5 : 1.0));
+  return big ? (0.5 * e) * e : fma<P>(e, 0.5, (1.0 / e) * 0.5);
+}
+
 // ---------------------------------------------------------------------------
 // erfc.  f32: modules/core/euler/include/nt2/euler/functions/simd/common/erfc.hpp:92-123
 // (the scalar form returns the same values), kernels erf_kernel.hpp:46-85.

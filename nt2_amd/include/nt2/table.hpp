@@ -194,7 +194,7 @@ NT2H_TAG(sin_, same1) NT2H_TAG(abs_, same1) NT2H_TAG(sqr_, same1) NT2H_TAG(rec_,
 NT2H_TAG(exp2_, same1) NT2H_TAG(exp10_, same1) NT2H_TAG(log2_, same1) NT2H_TAG(log10_, same1)
 NT2H_TAG(floor_, same1) NT2H_TAG(ceil_, same1) NT2H_TAG(trunc_, same1) NT2H_TAG(round_, same1)
 NT2H_TAG(round2even_, same1) NT2H_TAG(sign_, same1) NT2H_TAG(expm1_, same1) NT2H_TAG(log1p_, same1)
-NT2H_TAG(tanh_, same1)
+NT2H_TAG(tanh_, same1) NT2H_TAG(sinh_, same1) NT2H_TAG(cosh_, same1)
 NT2H_TAG(fma_, same3) NT2H_TAG(fnms_, same3) NT2H_TAG(if_else_, sel3)
 NT2H_TAG(selinc_, selinc2) NT2H_TAG(seldec_, selinc2) NT2H_TAG(if_zero_else_one_, izeo1)
 #undef NT2H_TAG
@@ -267,6 +267,8 @@ NT2H_UN(sign_, dm::sign_nt2(x), dm::sign_nt2(x), (x > T(0)) - (x < T(0)))
 NT2H_UN(expm1_, dm::expm1_f<P>(x), dm::expm1_d<P>(x), no_integer_form<T>())
 NT2H_UN(log1p_, dm::log1p_f<P>(x), dm::log1p_d<P>(x), no_integer_form<T>())
 NT2H_UN(tanh_, dm::tanh_f<P>(x), dm::tanh_d<P>(x), no_integer_form<T>())
+NT2H_UN(sinh_, dm::sinh_f<P>(x), dm::sinh_d<P>(x), no_integer_form<T>())
+NT2H_UN(cosh_, dm::cosh_f<P>(x), dm::cosh_d<P>(x), no_integer_form<T>())
 #undef NT2H_UN
 
 template <> struct apply1<tagfn::cos_> {
@@ -528,7 +530,8 @@ NT2H_FN1(sin, sin_) NT2H_FN1(abs, abs_) NT2H_FN1(sqr, sqr_) NT2H_FN1(rec, rec_)
 NT2H_FN1(exp2, exp2_) NT2H_FN1(exp10, exp10_) NT2H_FN1(log2, log2_) NT2H_FN1(log10, log10_)
 NT2H_FN1(floor, floor_) NT2H_FN1(ceil, ceil_) NT2H_FN1(trunc, trunc_) NT2H_FN1(round, round_)
 NT2H_FN1(round2even, round2even_) NT2H_FN1(sign, sign_)
-NT2H_FN1(expm1, expm1_) NT2H_FN1(log1p, log1p_) NT2H_FN1(tanh, tanh_)
+NT2H_FN1(expm1, expm1_) NT2H_FN1(log1p, log1p_) NT2H_FN1(tanh, tanh_) NT2H_FN1(sinh, sinh_)
+NT2H_FN1(cosh, cosh_)
 NT2H_FN1(if_zero_else_one, if_zero_else_one_)
 #undef NT2H_FN1
 #define NT2H_FN2(name, Tag) \

@@ -467,6 +467,52 @@ inline double tanh_d(double a0) {
   return as_d(bits(r) ^ (bits(a0) & 0x8000000000000000ull));
 }
 
+// sinh: modules/core/hyperbolic/include/nt2/hyperbolic/functions/simd/common/sinh.hpp:27-47,
+// details/sinh_kernel.hpp:25-66; cosh: generic/cosh.hpp:27-45 with
+// boost/simd average(a, b) = fma(a, 1/2, b/2) (arithmetic/functions/generic/average.hpp)
+inline float sinh_f(float a0) {
+  const float cc[4] = {F32(0x39559e2f), F32(0x3c087bbe), F32(0x3e2aaacc), F32(0x3f800000)};
+  float x = std::fabs(a0), r;
+  if (x < 1.0f) {
+    r = horner(cc, x * x) * x;
+  } else {
+    bool big = x > fMaxlog - F32(0x3f317218);
+    float e = exp_f(x * (big ? 0.5f : 1.0f));
+    float h = 0.5f * e;
+    r = big ? h * e : h - 0.5f * (1.0f / e);
+  }
+  return as_f(bits(r) ^ (bits(a0) & 0x80000000u));
+}
+inline double sinh_d(double a0) {
+  const double p[4] = {F64(0xbfe9435fe8bb3cd6), F64(0xc064773a398ff4fe), F64(0xc0c694b8c71d6182),
+                       F64(0xc115782bdbf6ab05)};
+  const double q[4] = {F64(0x3ff0000000000000), F64(0xc0715b6096e96484), F64(0x40e1a7ba7ed72245),
+                       F64(0xc1401a20e4f90044)};
+  double x = std::fabs(a0), r;
+  if (x < 1.0) {
+    double x2 = x * x;
+    r = x * ((x2 * horner(p, x2)) / horner(q, x2)) + x;
+  } else {
+    bool big = x > dMaxlog - F64(0x3fe62e42fefa39ef);
+    double e = exp_d(x * (big ? 0.5 : 1.0));
+    double h = 0.5 * e;
+    r = big ? h * e : h - 0.5 * (1.0 / e);
+  }
+  return as_d(bits(r) ^ (bits(a0) & 0x8000000000000000ull));
+}
+inline float cosh_f(float a0) {
+  float x = std::fabs(a0);
+  bool big = x > fMaxlog - F32(0x3f317218);
+  float e = exp_f(x * (big ? 0.5f : 1.0f));
+  return big ? (0.5f * e) * e : e * 0.5f + (1.0f / e) * 0.5f;
+}
+inline double cosh_d(double a0) {
+  double x = std::fabs(a0);
+  bool big = x > dMaxlog - F64(0x3fe62e42fefa39ef);
+  double e = exp_d(x * (big ? 0.5 : 1.0));
+  return big ? (0.5 * e) * e : e * 0.5 + (1.0 / e) * 0.5;
+}
+
 // ===========================================================================
 // rounding and sign
 // ===========================================================================
@@ -1156,7 +1202,8 @@ enum {
   OR_EXP = 0, OR_LOG = 1, OR_SQRT = 2, OR_ERFC = 3, OR_NORMCDF = 4,
   OR_FASTNORMCDF = 5, OR_COS = 6, OR_SIN = 7, OR_ABS = 8, OR_EXP2 = 9, OR_EXP10 = 10,
   OR_LOG2 = 11, OR_LOG10 = 12, OR_FLOOR = 13, OR_CEIL = 14, OR_TRUNC = 15, OR_ROUND = 16,
-  OR_ROUND2EVEN = 17, OR_SIGN = 18, OR_EXPM1 = 19, OR_LOG1P = 20, OR_TANH = 21, OR_LAST = OR_TANH
+  OR_ROUND2EVEN = 17, OR_SIGN = 18, OR_EXPM1 = 19, OR_LOG1P = 20, OR_TANH = 21, OR_SINH = 22,
+  OR_COSH = 23, OR_LAST = OR_COSH
 };
 
 int oracle_abi_version(void) { return 1; }
@@ -1197,6 +1244,8 @@ int oracle_map_f32(int op, const float* x, float* y, size_t n) {
       case OR_EXPM1: for (size_t l = 0; l < w; ++l) y[i + l] = expm1_f(x[i + l]); break;
       case OR_LOG1P: for (size_t l = 0; l < w; ++l) y[i + l] = log1p_f(x[i + l]); break;
       case OR_TANH: for (size_t l = 0; l < w; ++l) y[i + l] = tanh_f(x[i + l]); break;
+      case OR_SINH: for (size_t l = 0; l < w; ++l) y[i + l] = sinh_f(x[i + l]); break;
+      case OR_COSH: for (size_t l = 0; l < w; ++l) y[i + l] = cosh_f(x[i + l]); break;
       case OR_COS:
       case OR_SIN:
         if (pack) trig_group_f<8>(op == OR_COS ? 0 : 1, x + i, y + i, nullptr);
@@ -1231,6 +1280,8 @@ int oracle_map_f64(int op, const double* x, double* y, size_t n) {
       case OR_EXPM1: for (size_t l = 0; l < w; ++l) y[i + l] = expm1_d(x[i + l]); break;
       case OR_LOG1P: for (size_t l = 0; l < w; ++l) y[i + l] = log1p_d(x[i + l]); break;
       case OR_TANH: for (size_t l = 0; l < w; ++l) y[i + l] = tanh_d(x[i + l]); break;
+      case OR_SINH: for (size_t l = 0; l < w; ++l) y[i + l] = sinh_d(x[i + l]); break;
+      case OR_COSH: for (size_t l = 0; l < w; ++l) y[i + l] = cosh_d(x[i + l]); break;
       case OR_COS:
       case OR_SIN:
         if (pack) trig_group_d<4>(op == OR_COS ? 0 : 1, x + i, y + i, nullptr);

@@ -68,6 +68,8 @@ FUNCS = {
     "expm1": lambda x: mpmath.expm1(x),
     "log1p": lambda x: mpmath.log1p(x) if x > -1 else (mpmath.mpf("-inf") if x == -1 else mpmath.nan),
     "tanh": lambda x: mpmath.tanh(x),
+    "sinh": lambda x: mpmath.sinh(x),
+    "cosh": lambda x: mpmath.cosh(x),
 }
 
 
@@ -98,6 +100,11 @@ def inputs(name, dtype, rng):
         pts += list(np.exp2(rng.uniform(-60, -2, 200)) * rng.choice([-1.0, 1.0], 200))
         pts += list(rng.uniform(10, 25, 100))
         pts += [0.0, 0.625, -0.625, 0.6249, 1.0, -1.0, 5.0, 20.0]
+    elif name in ("sinh", "cosh"):
+        lim = 89.0 if dtype is np.float32 else 710.0
+        pts += list(rng.uniform(-1, 1, 400)) + list(rng.uniform(-lim, lim, 800))
+        pts += list(np.exp2(rng.uniform(-60, -1, 200)) * rng.choice([-1.0, 1.0], 200))
+        pts += [0.0, 1.0, -1.0, 0.999, 1.001, 20.0, -20.0, lim - 1.0, -(lim - 1.0)]
     elif name in ("log", "log2", "log10"):
         e = rng.uniform(-120 if dtype is np.float32 else -1000, 120 if dtype is np.float32 else 1000, 1200)
         pts += list(np.exp2(e)) + list(rng.uniform(0.5, 2.0, 300))
@@ -253,6 +260,15 @@ KATS = [
     ("tanh", "-1", "-0.7615941559557649", 0.5, "modules/core/hyperbolic/unit/simd/tanh.cpp:47"),
     ("tanh", "1", "0.7615941559557649", 0.5, "modules/core/hyperbolic/unit/simd/tanh.cpp:48"),
     ("tanh", "0", "0", 0.5, "modules/core/hyperbolic/unit/simd/tanh.cpp:49"),
+    # modules/core/hyperbolic/unit/simd/sinh.cpp:43-47 and cosh.cpp:42-46
+    ("sinh", "inf", "inf", 0.5, "modules/core/hyperbolic/unit/simd/sinh.cpp:43"),
+    ("sinh", "-inf", "-inf", 0.5, "modules/core/hyperbolic/unit/simd/sinh.cpp:44"),
+    ("sinh", "nan", "nan", 0.5, "modules/core/hyperbolic/unit/simd/sinh.cpp:45"),
+    ("sinh", "0", "0", 0.5, "modules/core/hyperbolic/unit/simd/sinh.cpp:47"),
+    ("cosh", "inf", "inf", 0.5, "modules/core/hyperbolic/unit/simd/cosh.cpp:42"),
+    ("cosh", "-inf", "inf", 0.5, "modules/core/hyperbolic/unit/simd/cosh.cpp:43"),
+    ("cosh", "nan", "nan", 0.5, "modules/core/hyperbolic/unit/simd/cosh.cpp:44"),
+    ("cosh", "0", "1", 0.5, "modules/core/hyperbolic/unit/simd/cosh.cpp:46"),
 ]
 
 

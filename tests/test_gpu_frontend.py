@@ -137,6 +137,9 @@ def test_exponential_rounding_statements(outputs):
             O.map_f64("sign", d) * O.map_f64("log10", np.abs(d) + 1.0))
     got = outputs("ew_f64", np.float64)
     assert same_bits(got, want), first_diff(got, want)
+    want = O.map_f64("sinh", d / 20.0) - O.map_f64("cosh", d / 30.0)
+    got = outputs("ew2_f64", np.float64)
+    assert same_bits(got, want), first_diff(got, want)
 
 
 def test_image_demo_statements(outputs):

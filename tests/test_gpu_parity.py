@@ -20,9 +20,9 @@ from nt2_amd import EXACT, FAST, ops  # noqa: E402
 G = os.path.join(os.path.dirname(__file__), "golden")
 ROUNDING = ["floor", "ceil", "trunc", "round", "round2even", "sign"]
 F32_OPS = ["exp", "log", "sqrt", "erfc", "normcdf", "fastnormcdf", "cos", "sin", "abs",
-           "exp2", "exp10", "log2", "log10"] + ROUNDING + ["expm1", "log1p", "tanh"]
+           "exp2", "exp10", "log2", "log10"] + ROUNDING + ["expm1", "log1p", "tanh", "sinh", "cosh"]
 F64_OPS = ["exp", "log", "sqrt", "erfc", "normcdf", "cos", "sin", "abs",
-           "exp2", "exp10", "log2", "log10"] + ROUNDING + ["expm1", "log1p", "tanh"]
+           "exp2", "exp10", "log2", "log10"] + ROUNDING + ["expm1", "log1p", "tanh", "sinh", "cosh"]
 
 
 def _mixed(rng, n, lo, hi, dt):
@@ -57,6 +57,8 @@ def _inputs_f32(op, rng, n):
                                np.exp2(rng.uniform(-20, 120, n - n // 2)).astype(np.float32)])
     if op == "tanh":
         return _mixed(rng, n, -12.0, 12.0, np.float32)
+    if op in ("sinh", "cosh"):
+        return _mixed(rng, n, -90.0, 90.0, np.float32)
     if op in ("erfc", "normcdf", "fastnormcdf"):
         return rng.uniform(-12, 12, n).astype(np.float32)
     if op in ("cos", "sin"):
@@ -84,6 +86,8 @@ def _inputs_f64(op, rng, n):
         return np.concatenate([_mixed(rng, n // 2, -1.0, 2.0, np.float64), np.exp2(rng.uniform(-20, 1000, n - n // 2))])
     if op == "tanh":
         return _mixed(rng, n, -25.0, 25.0, np.float64)
+    if op in ("sinh", "cosh"):
+        return _mixed(rng, n, -711.0, 711.0, np.float64)
     if op in ("erfc", "normcdf"):
         return rng.uniform(-30, 30, n)
     if op in ("cos", "sin"):
@@ -137,7 +141,7 @@ def _truth_f32(op, x):
          "log10": lambda v: np.log10(np.where(v >= 0, v, np.nan)),
          "floor": np.floor, "ceil": np.ceil, "trunc": np.trunc, "round2even": np.rint, "sign": np.sign,
          "round": lambda v: np.sign(v) * np.floor(np.abs(v) + 0.5),
-         "expm1": np.expm1, "log1p": lambda v: np.log1p(np.where(v >= -1, v, np.nan)), "tanh": np.tanh}[op]
+         "expm1": np.expm1, "log1p": lambda v: np.log1p(np.where(v >= -1, v, np.nan)), "tanh": np.tanh, "sinh": np.sinh, "cosh": np.cosh}[op]
     with np.errstate(all="ignore"):
         return f(xd)
 
@@ -150,7 +154,7 @@ def _fast_bound(op, x, y):
         return np.full(x.shape, 1.5)
     if op == "log10":
         return np.full(x.shape, 2.0)
-    if op in ("expm1", "log1p", "tanh"):
+    if op in ("expm1", "log1p", "tanh", "sinh", "cosh"):
         return np.full(x.shape, 2.5)
     if op in ROUNDING:
         return np.zeros(x.shape)

@@ -53,7 +53,7 @@ def _bound(name, x, y, dt):
     ax = np.abs(x.astype(np.float64))
     if name in ("exp", "log", "exp2", "exp10", "log2"):
         return np.full(x.shape, 1.0)
-    if name in ("expm1", "log1p", "tanh"):
+    if name in ("expm1", "log1p", "tanh", "sinh", "cosh"):
         return np.full(x.shape, BOUND3[name][dt])
     if name == "log10":
         # Log10_2hi*k is added before Log10_2lo*k (f_log.hpp:163-169), so the
@@ -83,7 +83,7 @@ def _bound(name, x, y, dt):
 
 @pytest.mark.parametrize("dt,tag", [(np.float32, "f32"), (np.float64, "f64")])
 @pytest.mark.parametrize("name", ["exp", "log", "erfc", "normcdf", "cos", "sin", "exp2", "exp10", "log2", "log10",
-                                  "expm1", "log1p", "tanh"])
+                                  "expm1", "log1p", "tanh", "sinh", "cosh"])
 def test_golden_mpmath(dt, tag, name):
     g = np.load(os.path.join(G, "math_%s.npz" % tag))
     x, y = g[name + "_x"], g[name + "_y"]
@@ -125,7 +125,8 @@ def test_golden_mpmath(dt, tag, name):
 # classic-ulp bounds against the correctly rounded value (measured maxima:
 # the constructions round several times)
 BOUND3 = {"expm1": {np.float32: 2.0, np.float64: 1.0}, "log1p": {np.float32: 1.0, np.float64: 1.0},
-          "tanh": {np.float32: 1.0, np.float64: 1.0}}
+          "tanh": {np.float32: 1.0, np.float64: 1.0}, "sinh": {np.float32: 2.0, np.float64: 1.0},
+          "cosh": {np.float32: 1.0, np.float64: 1.0}}
 
 MINLOG = {"exp2": {np.float32: np.float32(-127.0), np.float64: -1022.0},
           "exp10": {np.float32: np.uint32(0xc2179999).view(np.float32),
@@ -327,3 +328,27 @@ def test_expm1_tanh_log1p_structure(dt):
     u = rng.uniform(-0.9, 5, 2000).astype(dt)
     back = _iso(mapf, "log1p", _iso(mapf, "expm1", u, W), W).astype(np.float64)
     assert np.all(np.abs(back - u) <= 8 * np.spacing(np.maximum(np.abs(u), np.finfo(dt).tiny)).astype(np.float64) + 4 * np.finfo(dt).eps)
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_sinh_cosh_structure(dt):
+    """sinh odd and cosh even bit for bit; cosh^2 - sinh^2 = 1 within the
+    roundings where neither overflows; the large-argument form (e/2) e takes
+    over above Maxlog - Log_2 and overflows only where the true value does."""
+    mapf = O.map_f32 if dt is np.float32 else O.map_f64
+    W = 8 if dt is np.float32 else 4
+    rng = np.random.default_rng(52)
+    x = np.concatenate([rng.uniform(-1, 1, 3000), rng.uniform(-20, 20, 3000)]).astype(dt)
+    u = np.uint32 if dt is np.float32 else np.uint64
+    sb = u(1) << u(31 if dt is np.float32 else 63)
+    assert np.array_equal(_iso(mapf, "sinh", x, W).view(u) ^ sb, _iso(mapf, "sinh", -x, W).view(u))
+    assert np.array_equal(_iso(mapf, "cosh", x, W), _iso(mapf, "cosh", -x, W))
+    s_, c_ = _iso(mapf, "sinh", x, W).astype(np.float64), _iso(mapf, "cosh", x, W).astype(np.float64)
+    small = np.abs(x) < 8
+    eps = np.finfo(dt).eps
+    assert np.all(np.abs((c_ - s_) * (c_ + s_) - 1)[small] <= 8 * eps * (c_ * c_)[small])
+    ml = np.uint32(0x42b0c0a5).view(np.float32) if dt is np.float32 else np.uint64(0x40862e42fefa39ef).view(np.float64)
+    # Maxlog is 88.376 (f32) and 709.78 (f64); cosh overflows above ln(2 max) = 89.41 / 710.48
+    top = np.array([ml - dt(0.5), ml + dt(0.5), ml + dt(1.5 if dt is np.float32 else 1.0)], dtype=dt)
+    ch = _iso(mapf, "cosh", top, W)
+    assert np.all(np.isfinite(ch[:2])) and np.isinf(ch[2])
