@@ -69,6 +69,7 @@ extern "C" {
 #define NT2H_OP_TANH 21
 #define NT2H_OP_SINH 22
 #define NT2H_OP_COSH 23
+#define NT2H_OP_ATAN 24
 
 /* Black-Scholes formula variants */
 #define NT2H_BS_TEXTBOOK 0 /* d1 = (log(S/X)+(r+v^2/2)T)/(v sqrt T), call and put, erfc-based normcdf */
@@ -110,6 +111,13 @@ int nt2h_fill_frames_u8(uint8_t* frames, int h, int w, int nframes, int nrect, i
  * reduction is chosen per pack, as in NT2). */
 int nt2h_map_f32(int op, size_t n, const float* x, float* y, int policy, void* stream);
 int nt2h_map_f64(int op, size_t n, const double* x, double* y, int policy, void* stream);
+/* out = f(x, y) elementwise for the binary functions below, NT2's pack
+ * schedule as nt2h_map_*: replaces transform_ over a binary node such as
+ * atan2(a, b) (modules/core/trigonometric/include/nt2/trigonometric/functions/
+ * simd/common/atan2.hpp:30-54, scalar/atan2.hpp:29-45 for the pack tail). */
+#define NT2H_OP2_ATAN2 0
+int nt2h_map2_f32(int op, size_t n, const float* x, const float* y, float* out, int policy, void* stream);
+int nt2h_map2_f64(int op, size_t n, const double* x, const double* y, double* out, int policy, void* stream);
 /* [s, c] = sincos(x): modules/core/trigonometric/include/nt2/trigonometric/functions/generic/sincos.hpp:50-64 */
 int nt2h_sincos_f32(size_t n, const float* x, float* s, float* c, int policy, void* stream);
 int nt2h_sincos_f64(size_t n, const double* x, double* s, double* c, int policy, void* stream);

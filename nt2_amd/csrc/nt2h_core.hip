@@ -138,6 +138,7 @@ __global__ void fill_frames_k(uint8_t* frames, int h, int w, int nframes, Rects 
 // ============================================================================
 template <class P, int OP, int G>
 __device__ __forceinline__ void apply_f32(const float* x, float* y) {
+  constexpr bool scalar = (G == 1);
   if constexpr (OP == NT2H_OP_COS || OP == NT2H_OP_SIN) {
     dm::trig_group_f<P, G>(OP == NT2H_OP_COS ? dm::k_cos : dm::k_sin, x, y, nullptr);
   } else {
@@ -166,6 +167,7 @@ __device__ __forceinline__ void apply_f32(const float* x, float* y) {
       else if constexpr (OP == NT2H_OP_TANH) r = dm::tanh_f<P>(a);
       else if constexpr (OP == NT2H_OP_SINH) r = dm::sinh_f<P>(a);
       else if constexpr (OP == NT2H_OP_COSH) r = dm::cosh_f<P>(a);
+      else if constexpr (OP == NT2H_OP_ATAN) r = dm::atan_f<P, scalar>(a);
       else r = __builtin_fabsf(a);
       y[l] = r;
     }
@@ -202,6 +204,7 @@ __device__ __forceinline__ void apply_f64(const double* x, double* y) {
       else if constexpr (OP == NT2H_OP_TANH) r = dm::tanh_d<P>(a);
       else if constexpr (OP == NT2H_OP_SINH) r = dm::sinh_d<P>(a);
       else if constexpr (OP == NT2H_OP_COSH) r = dm::cosh_d<P>(a);
+      else if constexpr (OP == NT2H_OP_ATAN) r = dm::atan_d<P, scalar>(a);
       else r = __builtin_fabs(a);
       y[l] = r;
     }
@@ -270,6 +273,32 @@ __global__ __launch_bounds__(kBlock) void map_f64_k(const double* __restrict__ x
     double a = x[i], r;
     apply_f64<P, OP, 1>(&a, &r);
     y[i] = r;
+  }
+}
+
+// binary maps: out = f(x, y), packs of W from index 0 (f(pack) form), then
+// the scalar tail (NT2's scalar form)
+template <class P, int OP, class T, int G>
+__device__ __forceinline__ T apply2_one(T a, T b) {
+  constexpr bool scalar = (G == 1);
+  if constexpr (sizeof(T) == 4) return dm::atan2_f<P, scalar>(a, b);
+  else return dm::atan2_d<P, scalar>(a, b);
+}
+template <class P, int OP, class T, int W>
+__global__ __launch_bounds__(kBlock) void map2_k(const T* __restrict__ x, const T* __restrict__ y,
+                                                 T* __restrict__ out, size_t n) {
+  const size_t npacks = n / W;
+  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < npacks) {
+    const size_t i = t * W;
+    T r[W];
+#pragma unroll
+    for (int l = 0; l < W; ++l) r[l] = apply2_one<P, OP, T, W>(x[i + l], y[i + l]);
+#pragma unroll
+    for (int l = 0; l < W; ++l) out[i + l] = r[l];
+  } else if (t < npacks + (n - npacks * W)) {
+    const size_t i = npacks * W + (t - npacks);
+    out[i] = apply2_one<P, OP, T, 1>(x[i], y[i]);
   }
 }
 
@@ -775,7 +804,7 @@ static int launch_map_f32(int op, size_t n, const float* x, float* y, hipStream_
     NT2H_MAP32(NT2H_OP_FLOOR) NT2H_MAP32(NT2H_OP_CEIL) NT2H_MAP32(NT2H_OP_TRUNC) NT2H_MAP32(NT2H_OP_ROUND)
     NT2H_MAP32(NT2H_OP_ROUND2EVEN) NT2H_MAP32(NT2H_OP_SIGN)
     NT2H_MAP32(NT2H_OP_EXPM1) NT2H_MAP32(NT2H_OP_LOG1P) NT2H_MAP32(NT2H_OP_TANH)
-    NT2H_MAP32(NT2H_OP_SINH) NT2H_MAP32(NT2H_OP_COSH)
+    NT2H_MAP32(NT2H_OP_SINH) NT2H_MAP32(NT2H_OP_COSH) NT2H_MAP32(NT2H_OP_ATAN)
 #undef NT2H_MAP32
     default: return fail(NT2H_EINVAL, "nt2h_map_f32: unknown op");
   }
@@ -802,7 +831,7 @@ static int launch_map_f64(int op, size_t n, const double* x, double* y, hipStrea
     NT2H_MAP64(NT2H_OP_FLOOR) NT2H_MAP64(NT2H_OP_CEIL) NT2H_MAP64(NT2H_OP_TRUNC) NT2H_MAP64(NT2H_OP_ROUND)
     NT2H_MAP64(NT2H_OP_ROUND2EVEN) NT2H_MAP64(NT2H_OP_SIGN)
     NT2H_MAP64(NT2H_OP_EXPM1) NT2H_MAP64(NT2H_OP_LOG1P) NT2H_MAP64(NT2H_OP_TANH)
-    NT2H_MAP64(NT2H_OP_SINH) NT2H_MAP64(NT2H_OP_COSH)
+    NT2H_MAP64(NT2H_OP_SINH) NT2H_MAP64(NT2H_OP_COSH) NT2H_MAP64(NT2H_OP_ATAN)
 #undef NT2H_MAP64
     default: return fail(NT2H_EINVAL, "nt2h_map_f64: unknown op (fastnormcdf is single precision only)");
   }
@@ -945,6 +974,30 @@ int nt2h_map_f64(int op, size_t n, const double* x, double* y, int policy, void*
   if (policy == NT2H_EXACT) return launch_map_f64<exact_policy>(op, n, x, y, as_stream(stream));
   if (policy == NT2H_FAST) return launch_map_f64<fast_policy>(op, n, x, y, as_stream(stream));
   return fail(NT2H_EINVAL, "nt2h_map_f64: bad policy");
+}
+
+extern "C++" template <class T, int W>
+int launch_map2(const char* name, int op, size_t n, const T* x, const T* y, T* out, int policy,
+                       void* stream) {
+  if (n == 0) return NT2H_OK;
+  if (!x || !y || !out) return fail(NT2H_EINVAL, name);
+  if (op != NT2H_OP2_ATAN2) return fail(NT2H_EINVAL, name);
+  const dim3 g(grid_for(n / W + n % W)), b(kBlock);
+  if (policy == NT2H_EXACT)
+    hipLaunchKernelGGL((map2_k<exact_policy, NT2H_OP2_ATAN2, T, W>), g, b, 0, as_stream(stream), x, y, out, n);
+  else if (policy == NT2H_FAST)
+    hipLaunchKernelGGL((map2_k<fast_policy, NT2H_OP2_ATAN2, T, W>), g, b, 0, as_stream(stream), x, y, out, n);
+  else
+    return fail(NT2H_EINVAL, name);
+  NT2H_CHECK_LAUNCH(name);
+  return NT2H_OK;
+}
+
+int nt2h_map2_f32(int op, size_t n, const float* x, const float* y, float* out, int policy, void* stream) {
+  return launch_map2<float, 8>("nt2h_map2_f32", op, n, x, y, out, policy, stream);
+}
+int nt2h_map2_f64(int op, size_t n, const double* x, const double* y, double* out, int policy, void* stream) {
+  return launch_map2<double, 4>("nt2h_map2_f64", op, n, x, y, out, policy, stream);
 }
 
 int nt2h_sincos_f32(size_t n, const float* x, float* s, float* c, int policy, void* stream) {

@@ -555,6 +555,125 @@ NT2H_FN double cosh_d(double a0) {
   return big ? (0.5 * e) * e : fma<P>(e, 0.5, (1.0 / e) * 0.5);
 }
 
+// atan: modules/core/trigonometric/include/nt2/trigonometric/functions/
+// simd/common/impl/invtrig/f_invtrig.hpp:98-133 and d_invtrig.hpp:128-166 (packs):
+// |a0| below tan(pi/8) directly, up to tan(3pi/8) by (x-1)/(x+1) + pi/4, above
+// by -1/x + pi/2, the pi/4 and pi/2 low parts added to the polynomial's
+// result; sign xored in.  The scalar forms that NT2 uses for a pack tail
+// (scalar/impl/invtrig/f_invtrig.hpp:92-128, d_invtrig.hpp:112-159) take the
+// boundaries as strict and, for f64, switch to (x-1)/(x+1) only above 0.66:
+// Scalar selects them.
+template <class P, bool Scalar>
+NT2H_FN float atan_f(float a0) {
+  NT2H_STRICT
+  constexpr float c[4] = {NT2H_F(0x3da4f0d1), NT2H_F(0xbe0e1b85), NT2H_F(0x3e4c925f), NT2H_F(0xbeaaaa2a)};
+  const float t3 = NT2H_F(0x401a827a), t1 = NT2H_F(0x3ed413cd);
+  const float pio2 = NT2H_F(0x3fc90fdb), pio4 = NT2H_F(0x3f490fdb);
+  const float pio2lo = NT2H_F(0xb33bbd2e), pio4lo = NT2H_F(0xb2bbbd2e);
+  const float x = __builtin_fabsf(a0);
+  float r;
+  if constexpr (Scalar) {
+    if (a0 == 0.0f) {
+      r = 0.0f;
+    } else if (x == __builtin_inff()) {
+      r = pio2;
+    } else {
+      float y = 0.0f, more = 0.0f, xx = x;
+      if (x > t3) {
+        y = pio2; more = pio2lo; xx = -(1.0f / x);
+      } else if (x > t1) {
+        y = pio4; more = pio4lo; xx = (x - 1.0f) / (x + 1.0f);
+      }
+      const float z = xx * xx;
+      const float z1 = fma<P>(xx, horner<P>(c, z) * z, xx);
+      r = y + (z1 + more);
+    }
+  } else {
+    const bool f1 = x < t3;
+    const bool f2 = (x >= t1) && f1;
+    const float yy = f2 ? pio4 : (f1 ? 0.0f : pio2);
+    const float xx = f2 ? (x - 1.0f) / (x + 1.0f) : (f1 ? x : -(1.0f / x));
+    const float z = xx * xx;
+    float z1 = fma<P>(xx, horner<P>(c, z) * z, xx);
+    z1 = f2 ? z1 + pio4lo : z1;
+    z1 = f1 ? z1 : z1 + pio2lo;
+    r = yy + z1;
+  }
+  return as_f(bits(r) ^ (bits(a0) & 0x80000000u));
+}
+template <class P, bool Scalar>
+NT2H_FN double atan_d(double a0) {
+  NT2H_STRICT
+  constexpr double p[5] = {NT2H_D(0xbfec007fa1f72594), NT2H_D(0xc03028545b6b807a),
+                           NT2H_D(0xc052c08c36880273), NT2H_D(0xc05eb8bf2d05ba25),
+                           NT2H_D(0xc0503669fd28ec8e)};
+  constexpr double q[6] = {NT2H_D(0x3ff0000000000000), NT2H_D(0x4038dbc45b14603c),
+                           NT2H_D(0x4064a0dd43b8fa25), NT2H_D(0x407b0e18d2e2be3b),
+                           NT2H_D(0x407e563f13b049ea), NT2H_D(0x4068519efbbd62ec)};
+  const double t3 = NT2H_D(0x4003504f333f9de6), t1 = NT2H_D(0x3fda827999fcef31);
+  const double pio2 = NT2H_D(0x3ff921fb54442d18), pio4 = NT2H_D(0x3fe921fb54442d18);
+  const double pio2lo = NT2H_D(0x3c91a62633145c07), pio4lo = NT2H_D(0x3c81a62633145c07);
+  const double x = __builtin_fabs(a0);
+  double r;
+  if constexpr (Scalar) {
+    if (a0 == 0.0) {
+      r = 0.0;
+    } else if (x == __builtin_inf()) {
+      r = pio2;
+    } else {
+      double y, flag = (x > t3) ? 1.0 : 0.0, xx = x;
+      if (x > t3) {
+        y = pio2; xx = -(1.0 / x);
+      } else if (x <= NT2H_D(0x3fe51eb851eb851f)) {
+        y = 0.0;
+      } else {
+        y = pio4; flag = 0.5; xx = (x - 1.0) / (x + 1.0);
+      }
+      const double z = xx * xx;
+      double zz = (z * horner<P>(p, z)) / horner<P>(q, z);
+      zz = fma<P>(xx, zz, xx);
+      zz = zz + flag * pio2lo;
+      r = y + zz;
+    }
+  } else {
+    const bool f1 = x < t3;
+    const bool f2 = (x >= t1) && f1;
+    const double yy = f2 ? pio4 : (f1 ? 0.0 : pio2);
+    const double xx = f2 ? (x - 1.0) / (x + 1.0) : (f1 ? x : -(1.0 / x));
+    const double z = xx * xx;
+    double zz = (z * horner<P>(p, z)) / horner<P>(q, z);
+    zz = fma<P>(xx, zz, xx);
+    zz = f2 ? zz + pio4lo : zz;
+    zz = zz + (f1 ? 0.0 : pio2lo);
+    r = yy + zz;
+  }
+  return as_d(bits(r) ^ (bits(a0) & 0x8000000000000000ull));
+}
+
+// atan2(a0, a1): simd/common/atan2.hpp:30-54 (packs) and scalar/atan2.hpp:29-45:
+// both infinite -> the unit signs; z = atan(|a0/a1|) by the kernel above,
+// pi - z when a1 <= 0, the sign of a0 (signnz: -0 counts as negative);
+// a0 = +-0 -> pi when a1 < 0, else +0 (both forms drop the sign of a0
+// there); NaN in either -> NaN.
+template <class P, bool Scalar, class T>
+NT2H_FN T atan2_t(T a0, T a1) {
+  NT2H_STRICT
+  const T pi = sizeof(T) == 4 ? (T)NT2H_F(0x40490fdb) : (T)NT2H_D(0x400921fb54442d18);
+  if (a0 != a0 || a1 != a1) return a0 + a1;
+  if (__builtin_isinf(a0) && __builtin_isinf(a1)) {
+    a0 = __builtin_copysign((T)1, a0);
+    a1 = __builtin_copysign((T)1, a1);
+  }
+  T z;
+  if constexpr (sizeof(T) == 4) z = atan_f<P, Scalar>(__builtin_fabsf((float)(a0 / a1)));
+  else z = atan_d<P, Scalar>(__builtin_fabs((double)(a0 / a1)));
+  z = (a1 > (T)0) ? z : pi - z;
+  z = __builtin_signbit(a0) ? -z : z;
+  return (a0 == (T)0) ? ((a1 < (T)0) ? pi : (T)0) : z;
+}
+template <class P, bool Scalar> NT2H_FN float atan2_f(float a0, float a1) { return atan2_t<P, Scalar, float>(a0, a1); }
+template <class P, bool Scalar> NT2H_FN double atan2_d(double a0, double a1) { return atan2_t<P, Scalar, double>(a0, a1); }
+
 // ---------------------------------------------------------------------------
 // erfc.  f32: modules/core/euler/include/nt2/euler/functions/simd/common/erfc.hpp:92-123
 // (the scalar form returns the same values), kernels erf_kernel.hpp:46-85.

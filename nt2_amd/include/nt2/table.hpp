@@ -194,7 +194,8 @@ NT2H_TAG(sin_, same1) NT2H_TAG(abs_, same1) NT2H_TAG(sqr_, same1) NT2H_TAG(rec_,
 NT2H_TAG(exp2_, same1) NT2H_TAG(exp10_, same1) NT2H_TAG(log2_, same1) NT2H_TAG(log10_, same1)
 NT2H_TAG(floor_, same1) NT2H_TAG(ceil_, same1) NT2H_TAG(trunc_, same1) NT2H_TAG(round_, same1)
 NT2H_TAG(round2even_, same1) NT2H_TAG(sign_, same1) NT2H_TAG(expm1_, same1) NT2H_TAG(log1p_, same1)
-NT2H_TAG(tanh_, same1) NT2H_TAG(sinh_, same1) NT2H_TAG(cosh_, same1)
+NT2H_TAG(tanh_, same1) NT2H_TAG(sinh_, same1) NT2H_TAG(cosh_, same1) NT2H_TAG(atan_, same1)
+NT2H_TAG(atan2_, same2)
 NT2H_TAG(fma_, same3) NT2H_TAG(fnms_, same3) NT2H_TAG(if_else_, sel3)
 NT2H_TAG(selinc_, selinc2) NT2H_TAG(seldec_, selinc2) NT2H_TAG(if_zero_else_one_, izeo1)
 #undef NT2H_TAG
@@ -269,6 +270,9 @@ NT2H_UN(log1p_, dm::log1p_f<P>(x), dm::log1p_d<P>(x), no_integer_form<T>())
 NT2H_UN(tanh_, dm::tanh_f<P>(x), dm::tanh_d<P>(x), no_integer_form<T>())
 NT2H_UN(sinh_, dm::sinh_f<P>(x), dm::sinh_d<P>(x), no_integer_form<T>())
 NT2H_UN(cosh_, dm::cosh_f<P>(x), dm::cosh_d<P>(x), no_integer_form<T>())
+// atan: the pack form for packs, NT2's scalar form for the tail (G == 1)
+NT2H_UN(atan_, (G == 1 ? dm::atan_f<P, true>(x) : dm::atan_f<P, false>(x)),
+        (G == 1 ? dm::atan_d<P, true>(x) : dm::atan_d<P, false>(x)), no_integer_form<T>())
 #undef NT2H_UN
 
 template <> struct apply1<tagfn::cos_> {
@@ -531,13 +535,13 @@ NT2H_FN1(exp2, exp2_) NT2H_FN1(exp10, exp10_) NT2H_FN1(log2, log2_) NT2H_FN1(log
 NT2H_FN1(floor, floor_) NT2H_FN1(ceil, ceil_) NT2H_FN1(trunc, trunc_) NT2H_FN1(round, round_)
 NT2H_FN1(round2even, round2even_) NT2H_FN1(sign, sign_)
 NT2H_FN1(expm1, expm1_) NT2H_FN1(log1p, log1p_) NT2H_FN1(tanh, tanh_) NT2H_FN1(sinh, sinh_)
-NT2H_FN1(cosh, cosh_)
+NT2H_FN1(cosh, cosh_) NT2H_FN1(atan, atan_)
 NT2H_FN1(if_zero_else_one, if_zero_else_one_)
 #undef NT2H_FN1
 #define NT2H_FN2(name, Tag) \
   template <class A, class B, NT2H_ENABLE2(A, B)> auto name(const A& a, const B& b) { return make2<tagfn::Tag>(a, b); }
 NT2H_FN2(max, max_) NT2H_FN2(min, min_) NT2H_FN2(adds, adds_) NT2H_FN2(subs, subs_)
-NT2H_FN2(selinc, selinc_) NT2H_FN2(seldec, seldec_)
+NT2H_FN2(selinc, selinc_) NT2H_FN2(seldec, seldec_) NT2H_FN2(atan2, atan2_)
 #undef NT2H_FN2
 // fma(a,b,c) = a*b+c and fnms(a,b,c) = c-a*b, NT2's two-rounding forms
 template <class A, class B, class C,
@@ -635,8 +639,19 @@ __device__ __forceinline__ void ev(const n2v<Tag, VA, VB>& v, size_t i, R (&r)[G
   TB b[G];
   ev<G>(v.a, i, a);
   ev<G>(v.b, i, b);
+  if constexpr (std::is_same<Tag, tagfn::atan2_>::value) {
+    static_assert(std::is_floating_point<R>::value, "atan2 is defined for floating-point tables only");
 #pragma unroll
-  for (int l = 0; l < G; ++l) r[l] = apply2<Tag>::template one<R>(a[l], b[l]);
+    for (int l = 0; l < G; ++l) {
+      if constexpr (std::is_same<R, float>::value)
+        r[l] = G == 1 ? dm::atan2_f<P, true>((float)a[l], (float)b[l]) : dm::atan2_f<P, false>((float)a[l], (float)b[l]);
+      else
+        r[l] = G == 1 ? dm::atan2_d<P, true>((double)a[l], (double)b[l]) : dm::atan2_d<P, false>((double)a[l], (double)b[l]);
+    }
+  } else {
+#pragma unroll
+    for (int l = 0; l < G; ++l) r[l] = apply2<Tag>::template one<R>(a[l], b[l]);
+  }
 }
 
 template <int G, class Tag, class VA, class VB, class VC, class R>

@@ -8,10 +8,11 @@ LIB_NAME = "libnt2h.so"
 
 EXACT, FAST, ACCURATE = 0, 1, 2
 BS_TEXTBOOK, BS_DEMO = 0, 1
+OPS2 = {"atan2": 0}
 OPS = {"exp": 0, "log": 1, "sqrt": 2, "erfc": 3, "normcdf": 4,
        "fastnormcdf": 5, "cos": 6, "sin": 7, "abs": 8, "exp2": 9, "exp10": 10, "log2": 11,
        "log10": 12, "floor": 13, "ceil": 14, "trunc": 15, "round": 16, "round2even": 17, "sign": 18,
-       "expm1": 19, "log1p": 20, "tanh": 21, "sinh": 22, "cosh": 23}
+       "expm1": 19, "log1p": 20, "tanh": 21, "sinh": 22, "cosh": 23, "atan": 24}
 H2D, D2H, D2D = 1, 2, 3
 
 _lib = None
@@ -73,6 +74,8 @@ def lib():
         "nt2h_rgb2grey_f32": ([S, P, P, P, P, I, P], I),
         "nt2h_rgb2yuv_f32": ([S, P, P, P, P, P, P, I, P], I),
         "nt2h_rgb2grey_u8": ([S, P, P, P, P, P], I),
+        "nt2h_map2_f32": ([I, S, P, P, P, I, P], I),
+        "nt2h_map2_f64": ([I, S, P, P, P, I, P], I),
         "nt2h_linspace_f32": ([S, F, F, P, I, P], I),
         "nt2h_linspace_f64": ([S, D, D, P, I, P], I),
         "nt2h_mandelbrot_f32": ([S, P, P, P, I, I, P], I),

@@ -4,7 +4,7 @@ import ctypes
 
 import torch
 
-from .native import (BS_TEXTBOOK, EXACT, OPS, check, lib, ptr, stream_handle)
+from .native import (BS_TEXTBOOK, EXACT, OPS, OPS2, check, lib, ptr, stream_handle)
 
 
 def _s():
@@ -47,6 +47,14 @@ def sincos(x, policy=EXACT):
 def stream_copy(src, dst, grid=0):
     check(lib().nt2h_stream_copy(ptr(src), ptr(dst), src.numel() * src.element_size(), grid, _s()),
           "stream_copy")
+
+
+def map2(op, x, y, policy=EXACT, out=None):
+    """out = f(x, y) elementwise for a binary function (atan2)."""
+    out = torch.empty_like(x) if out is None else out
+    f = lib().nt2h_map2_f32 if x.dtype == torch.float32 else lib().nt2h_map2_f64
+    check(f(OPS2[op], x.numel(), ptr(x), ptr(y), ptr(out), policy, _s()), "map2 " + op)
+    return out
 
 
 def saxpy(a, x, y):

@@ -513,6 +513,89 @@ inline double cosh_d(double a0) {
   return big ? (0.5 * e) * e : e * 0.5 + (1.0 / e) * 0.5;
 }
 
+// atan: simd/common/impl/invtrig/f_invtrig.hpp:98-133, d_invtrig.hpp:128-166 (packs);
+// scalar/impl/invtrig/f_invtrig.hpp:92-128, d_invtrig.hpp:112-159 (pack tail)
+// (modules/core/trigonometric/include/nt2/trigonometric/functions/...)
+inline float atan_f(float a0, bool scalar) {
+  const float cc[4] = {F32(0x3da4f0d1), F32(0xbe0e1b85), F32(0x3e4c925f), F32(0xbeaaaa2a)};
+  const float t3 = F32(0x401a827a), t1 = F32(0x3ed413cd), pio2 = F32(0x3fc90fdb), pio4 = F32(0x3f490fdb);
+  const float pio2lo = F32(0xb33bbd2e), pio4lo = F32(0xb2bbbd2e);
+  float x = std::fabs(a0), r;
+  if (scalar) {
+    if (a0 == 0.0f) r = 0.0f;
+    else if (std::isinf(a0)) r = pio2;
+    else {
+      float y = 0.0f, more = 0.0f, xx = x;
+      if (x > t3) { y = pio2; more = pio2lo; xx = -(1.0f / x); }
+      else if (x > t1) { y = pio4; more = pio4lo; xx = (x - 1.0f) / (x + 1.0f); }
+      float z = xx * xx;
+      float z1 = xx * (horner(cc, z) * z) + xx;
+      r = y + (z1 + more);
+    }
+  } else {
+    bool f1 = x < t3, f2 = (x >= t1) && f1;
+    float yy = f2 ? pio4 : (f1 ? 0.0f : pio2);
+    float xx = f2 ? (x - 1.0f) / (x + 1.0f) : (f1 ? x : -(1.0f / x));
+    float z = xx * xx;
+    float z1 = xx * (horner(cc, z) * z) + xx;
+    if (f2) z1 = z1 + pio4lo;
+    if (!f1) z1 = z1 + pio2lo;
+    r = yy + z1;
+  }
+  return as_f(bits(r) ^ (bits(a0) & 0x80000000u));
+}
+inline double atan_d(double a0, bool scalar) {
+  const double p[5] = {F64(0xbfec007fa1f72594), F64(0xc03028545b6b807a), F64(0xc052c08c36880273),
+                       F64(0xc05eb8bf2d05ba25), F64(0xc0503669fd28ec8e)};
+  const double q[6] = {F64(0x3ff0000000000000), F64(0x4038dbc45b14603c), F64(0x4064a0dd43b8fa25),
+                       F64(0x407b0e18d2e2be3b), F64(0x407e563f13b049ea), F64(0x4068519efbbd62ec)};
+  const double t3 = F64(0x4003504f333f9de6), t1 = F64(0x3fda827999fcef31);
+  const double pio2 = F64(0x3ff921fb54442d18), pio4 = F64(0x3fe921fb54442d18);
+  const double pio2lo = F64(0x3c91a62633145c07), pio4lo = F64(0x3c81a62633145c07);
+  double x = std::fabs(a0), r;
+  if (scalar) {
+    if (a0 == 0.0) r = 0.0;
+    else if (std::isinf(a0)) r = pio2;
+    else {
+      double y, flag = (x > t3) ? 1.0 : 0.0, xx = x;
+      if (x > t3) { y = pio2; xx = -(1.0 / x); }
+      else if (x <= F64(0x3fe51eb851eb851f)) y = 0.0;
+      else { y = pio4; flag = 0.5; xx = (x - 1.0) / (x + 1.0); }
+      double z = xx * xx;
+      double zz = (z * horner(p, z)) / horner(q, z);
+      zz = xx * zz + xx;
+      zz = zz + flag * pio2lo;
+      r = y + zz;
+    }
+  } else {
+    bool f1 = x < t3, f2 = (x >= t1) && f1;
+    double yy = f2 ? pio4 : (f1 ? 0.0 : pio2);
+    double xx = f2 ? (x - 1.0) / (x + 1.0) : (f1 ? x : -(1.0 / x));
+    double z = xx * xx;
+    double zz = (z * horner(p, z)) / horner(q, z);
+    zz = xx * zz + xx;
+    if (f2) zz = zz + pio4lo;
+    zz = zz + (f1 ? 0.0 : pio2lo);
+    r = yy + zz;
+  }
+  return as_d(bits(r) ^ (bits(a0) & 0x8000000000000000ull));
+}
+// atan2: simd/common/atan2.hpp:30-54, scalar/atan2.hpp:29-45
+template <class T, class K> inline T atan2_t(T a0, T a1, T pi, K kernel) {
+  if (a0 != a0 || a1 != a1) return a0 + a1;
+  if (std::isinf(a0) && std::isinf(a1)) { a0 = std::copysign(T(1), a0); a1 = std::copysign(T(1), a1); }
+  T z = kernel(std::fabs(a0 / a1));
+  z = (a1 > T(0)) ? z : pi - z;
+  if (std::signbit(a0)) z = -z;
+  return (a0 == T(0)) ? ((a1 < T(0)) ? pi : T(0)) : z;
+}
+inline float atan2_f(float a0, float a1, bool scalar) {
+  return atan2_t(a0, a1, F32(0x40490fdb), [&](float q) { return atan_f(q, scalar); });
+}
+inline double atan2_d(double a0, double a1, bool scalar) {
+  return atan2_t(a0, a1, F64(0x400921fb54442d18), [&](double q) { return atan_d(q, scalar); });
+}
+
 // ===========================================================================
 // rounding and sign
 // ===========================================================================
@@ -1203,7 +1286,7 @@ enum {
   OR_FASTNORMCDF = 5, OR_COS = 6, OR_SIN = 7, OR_ABS = 8, OR_EXP2 = 9, OR_EXP10 = 10,
   OR_LOG2 = 11, OR_LOG10 = 12, OR_FLOOR = 13, OR_CEIL = 14, OR_TRUNC = 15, OR_ROUND = 16,
   OR_ROUND2EVEN = 17, OR_SIGN = 18, OR_EXPM1 = 19, OR_LOG1P = 20, OR_TANH = 21, OR_SINH = 22,
-  OR_COSH = 23, OR_LAST = OR_COSH
+  OR_COSH = 23, OR_ATAN = 24, OR_LAST = OR_ATAN
 };
 
 int oracle_abi_version(void) { return 1; }
@@ -1246,6 +1329,7 @@ int oracle_map_f32(int op, const float* x, float* y, size_t n) {
       case OR_TANH: for (size_t l = 0; l < w; ++l) y[i + l] = tanh_f(x[i + l]); break;
       case OR_SINH: for (size_t l = 0; l < w; ++l) y[i + l] = sinh_f(x[i + l]); break;
       case OR_COSH: for (size_t l = 0; l < w; ++l) y[i + l] = cosh_f(x[i + l]); break;
+      case OR_ATAN: for (size_t l = 0; l < w; ++l) y[i + l] = atan_f(x[i + l], !pack); break;
       case OR_COS:
       case OR_SIN:
         if (pack) trig_group_f<8>(op == OR_COS ? 0 : 1, x + i, y + i, nullptr);
@@ -1282,12 +1366,31 @@ int oracle_map_f64(int op, const double* x, double* y, size_t n) {
       case OR_TANH: for (size_t l = 0; l < w; ++l) y[i + l] = tanh_d(x[i + l]); break;
       case OR_SINH: for (size_t l = 0; l < w; ++l) y[i + l] = sinh_d(x[i + l]); break;
       case OR_COSH: for (size_t l = 0; l < w; ++l) y[i + l] = cosh_d(x[i + l]); break;
+      case OR_ATAN: for (size_t l = 0; l < w; ++l) y[i + l] = atan_d(x[i + l], !pack); break;
       case OR_COS:
       case OR_SIN:
         if (pack) trig_group_d<4>(op == OR_COS ? 0 : 1, x + i, y + i, nullptr);
         else trig_group_d<1>(op == OR_COS ? 0 : 1, x + i, y + i, nullptr);
         break;
     }
+  });
+  return 0;
+}
+
+// out = atan2(x, y) over tables of n elements, NT2 transform semantics
+int oracle_map2_f32(int op, const float* x, const float* y, float* out, size_t n) {
+  if (op != 0) return -1;
+  nt2_transform(n, 8, sizeof(float), [&](size_t i, bool pack) {
+    const size_t w = pack ? 8 : 1;
+    for (size_t l = 0; l < w; ++l) out[i + l] = atan2_f(x[i + l], y[i + l], !pack);
+  });
+  return 0;
+}
+int oracle_map2_f64(int op, const double* x, const double* y, double* out, size_t n) {
+  if (op != 0) return -1;
+  nt2_transform(n, 4, sizeof(double), [&](size_t i, bool pack) {
+    const size_t w = pack ? 4 : 1;
+    for (size_t l = 0; l < w; ++l) out[i + l] = atan2_d(x[i + l], y[i + l], !pack);
   });
   return 0;
 }

@@ -12,7 +12,7 @@ LIB_PATH = os.path.join(ROOT, "oracle", "liboracle.so")
 OPS = {"exp": 0, "log": 1, "sqrt": 2, "erfc": 3, "normcdf": 4,
        "fastnormcdf": 5, "cos": 6, "sin": 7, "abs": 8, "exp2": 9, "exp10": 10, "log2": 11,
        "log10": 12, "floor": 13, "ceil": 14, "trunc": 15, "round": 16, "round2even": 17, "sign": 18,
-       "expm1": 19, "log1p": 20, "tanh": 21, "sinh": 22, "cosh": 23}
+       "expm1": 19, "log1p": 20, "tanh": 21, "sinh": 22, "cosh": 23, "atan": 24}
 
 _lib = None
 
@@ -56,6 +56,8 @@ def lib():
         _lib.oracle_rgb2grey_f32.argtypes = [S, P, P, P, P]
         _lib.oracle_rgb2yuv_f32.argtypes = [S, P, P, P, P, P, P]
         _lib.oracle_rgb2grey_u8.argtypes = [S, P, P, P, P]
+        _lib.oracle_map2_f32.argtypes = [ctypes.c_int, P, P, P, S]
+        _lib.oracle_map2_f64.argtypes = [ctypes.c_int, P, P, P, S]
         _lib.oracle_linspace_f32.argtypes = [S, ctypes.c_float, ctypes.c_float, P]
         _lib.oracle_linspace_f64.argtypes = [S, ctypes.c_double, ctypes.c_double, P]
         _lib.oracle_mandelbrot_f32.argtypes = [S, P, P, P, ctypes.c_int]
@@ -264,3 +266,13 @@ def mandelbrot(a, b, max_iter=256):
     c = np.empty(a.size, dtype=np.int32)
     assert lib().oracle_mandelbrot_f32(a.size, _p(a), _p(b), _p(c), int(max_iter)) == 0
     return c
+
+
+def atan2(y, x):
+    """NT2's atan2(y, x) over flat tables (packs of 8 / 4, scalar tail)."""
+    y = np.ascontiguousarray(y)
+    x = np.ascontiguousarray(x, dtype=y.dtype)
+    out = np.empty_like(y)
+    f = lib().oracle_map2_f32 if y.dtype == np.float32 else lib().oracle_map2_f64
+    assert f(0, _p(y), _p(x), _p(out), y.size) == 0
+    return out

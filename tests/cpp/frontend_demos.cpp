@@ -196,6 +196,15 @@ int main(int argc, char** argv) {
   dump("ew2_f32", ew);
   ew = nt2::tanh(xn) * nt2::expm1(xn / 4.0f) + nt2::log1p(nt2::abs(yn));
   dump("ew3_f32", ew);
+  table<float> at(of_size(size - 3));  // a ragged tail of NT2's scalar forms
+  for (std::size_t i = 1; i <= size - 3; ++i) at(i) = xn(i);
+  table<float> bt(of_size(size - 3));
+  for (std::size_t i = 1; i <= size - 3; ++i) bt(i) = yn(i) / 1000.0f;
+  table<float> at2;
+  at2 = nt2::atan(at * 3.0f) + nt2::atan2(at, bt);
+  dump("at_a", at);
+  dump("at_b", bt);
+  dump("atan_f32", at2);
   table<double> dm(of_size(mh * mw)), dr;
   for (std::size_t i = 1; i <= mh * mw; ++i) dm(i) = (double)M(i) / 100.0;
   dr = nt2::trunc(dm) * nt2::exp2(nt2::ceil(dm / 50.0)) + nt2::sign(dm) * nt2::log10(nt2::abs(dm) + 1.0);

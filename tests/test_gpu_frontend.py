@@ -132,6 +132,10 @@ def test_exponential_rounding_statements(outputs):
     want = (O.map_f32("tanh", xn) * O.map_f32("expm1", xn / f32(4))) + O.map_f32("log1p", np.abs(yn))
     got = outputs("ew3_f32", f32)
     assert same_bits(got, want), first_diff(got, want)
+    a, b = outputs("at_a", f32), outputs("at_b", f32)
+    want = O.map_f32("atan", a * f32(3)) + O.atan2(a, b)
+    got = outputs("atan_f32", f32)
+    assert same_bits(got, want), first_diff(got, want)
     d = outputs("M", f32).astype(np.float64) / 100.0
     want = (O.map_f64("trunc", d) * O.map_f64("exp2", O.map_f64("ceil", d / 50.0)) +
             O.map_f64("sign", d) * O.map_f64("log10", np.abs(d) + 1.0))

@@ -20,9 +20,9 @@ from nt2_amd import EXACT, FAST, ops  # noqa: E402
 G = os.path.join(os.path.dirname(__file__), "golden")
 ROUNDING = ["floor", "ceil", "trunc", "round", "round2even", "sign"]
 F32_OPS = ["exp", "log", "sqrt", "erfc", "normcdf", "fastnormcdf", "cos", "sin", "abs",
-           "exp2", "exp10", "log2", "log10"] + ROUNDING + ["expm1", "log1p", "tanh", "sinh", "cosh"]
+           "exp2", "exp10", "log2", "log10"] + ROUNDING + ["expm1", "log1p", "tanh", "sinh", "cosh", "atan"]
 F64_OPS = ["exp", "log", "sqrt", "erfc", "normcdf", "cos", "sin", "abs",
-           "exp2", "exp10", "log2", "log10"] + ROUNDING + ["expm1", "log1p", "tanh", "sinh", "cosh"]
+           "exp2", "exp10", "log2", "log10"] + ROUNDING + ["expm1", "log1p", "tanh", "sinh", "cosh", "atan"]
 
 
 def _mixed(rng, n, lo, hi, dt):
@@ -59,6 +59,10 @@ def _inputs_f32(op, rng, n):
         return _mixed(rng, n, -12.0, 12.0, np.float32)
     if op in ("sinh", "cosh"):
         return _mixed(rng, n, -90.0, 90.0, np.float32)
+    if op == "atan":
+        return np.concatenate([_mixed(rng, n // 2, -3.0, 3.0, np.float32),
+                               (np.exp2(rng.uniform(-20, 120, n - n // 2)) * rng.choice([-1, 1], n - n // 2)).astype(np.float32),
+                               np.float32([0.41421357, 0.41421354, 2.4142137, 2.4142134, 0.66])])
     if op in ("erfc", "normcdf", "fastnormcdf"):
         return rng.uniform(-12, 12, n).astype(np.float32)
     if op in ("cos", "sin"):
@@ -88,6 +92,10 @@ def _inputs_f64(op, rng, n):
         return _mixed(rng, n, -25.0, 25.0, np.float64)
     if op in ("sinh", "cosh"):
         return _mixed(rng, n, -711.0, 711.0, np.float64)
+    if op == "atan":
+        return np.concatenate([_mixed(rng, n // 2, -3.0, 3.0, np.float64),
+                               np.exp2(rng.uniform(-20, 1000, n - n // 2)) * rng.choice([-1, 1], n - n // 2),
+                               [0.41421356237309503, 2.414213562373095, 0.66, 0.6600000000000001]])
     if op in ("erfc", "normcdf"):
         return rng.uniform(-30, 30, n)
     if op in ("cos", "sin"):
@@ -141,7 +149,7 @@ def _truth_f32(op, x):
          "log10": lambda v: np.log10(np.where(v >= 0, v, np.nan)),
          "floor": np.floor, "ceil": np.ceil, "trunc": np.trunc, "round2even": np.rint, "sign": np.sign,
          "round": lambda v: np.sign(v) * np.floor(np.abs(v) + 0.5),
-         "expm1": np.expm1, "log1p": lambda v: np.log1p(np.where(v >= -1, v, np.nan)), "tanh": np.tanh, "sinh": np.sinh, "cosh": np.cosh}[op]
+         "expm1": np.expm1, "log1p": lambda v: np.log1p(np.where(v >= -1, v, np.nan)), "tanh": np.tanh, "sinh": np.sinh, "cosh": np.cosh, "atan": np.arctan}[op]
     with np.errstate(all="ignore"):
         return f(xd)
 
@@ -154,7 +162,7 @@ def _fast_bound(op, x, y):
         return np.full(x.shape, 1.5)
     if op == "log10":
         return np.full(x.shape, 2.0)
-    if op in ("expm1", "log1p", "tanh", "sinh", "cosh"):
+    if op in ("expm1", "log1p", "tanh", "sinh", "cosh", "atan"):
         return np.full(x.shape, 2.5)
     if op in ROUNDING:
         return np.zeros(x.shape)
@@ -442,3 +450,23 @@ def test_blackscholes_accurate_neighbour_independent():
                               policy=2)
     assert same_bits(c2.cpu().numpy(), c[perm].cpu().numpy())
     assert same_bits(p2.cpu().numpy(), p[perm].cpu().numpy())
+
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("n", [100003, 8 * 4096, 13, 1])
+def test_atan2_exact_bitwise(dt, n):
+    """atan2 over the four quadrants, the axes, zeros of both signs,
+    infinities and NaN, with a ragged tail that takes NT2's scalar form."""
+    rng = np.random.default_rng(700 + n)
+    y = rng.uniform(-10, 10, n)
+    x = rng.uniform(-10, 10, n)
+    sp = np.array([0.0, -0.0, 1.0, -1.0, np.inf, -np.inf, np.nan, 0.5, 1e-30, 1e30])
+    k = min(n, sp.size)
+    y[:k] = sp[:k]
+    x[-k:] = np.roll(sp, 3)[:k]
+    y, x = y.astype(dt), x.astype(dt)
+    say("atan2", dt.__name__, "n=%d" % n)
+    got = ops.map2("atan2", torch.from_numpy(y).cuda(), torch.from_numpy(x).cuda(), EXACT).cpu().numpy()
+    want = O.atan2(y, x)
+    assert same_bits(got, want), first_diff(got, want)

@@ -352,3 +352,71 @@ def test_sinh_cosh_structure(dt):
     top = np.array([ml - dt(0.5), ml + dt(0.5), ml + dt(1.5 if dt is np.float32 else 1.0)], dtype=dt)
     ch = _iso(mapf, "cosh", top, W)
     assert np.all(np.isfinite(ch[:2])) and np.isinf(ch[2])
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_atan_and_atan2(dt):
+    """atan: NT2's KATs (modules/core/trigonometric/unit/simd/atan.cpp),
+    within 1 ulp of mpmath on every region, odd bit for bit; the pack form
+    and NT2's scalar form (the pack tail) agree except at the region
+    boundaries they treat differently; atan2 of the four quadrants against
+    mpmath, its special values as NT2 defines them."""
+    mapf = O.map_f32 if dt is np.float32 else O.map_f64
+    W = 8 if dt is np.float32 else 4
+    rng = np.random.default_rng(53)
+    x = np.concatenate([rng.uniform(-1, 1, 2000), rng.uniform(-5, 5, 2000), np.exp2(rng.uniform(-30, 60, 2000)),
+                        [0.4142, 0.66, 2.4142, 1.0, 1e30]]).astype(dt)
+    packs = _iso(mapf, "atan", x, W)
+    truth = np.array([float(mpmath.atan(mpmath.mpf(float(v)))) for v in x])
+    # f32: up to 2 ulp from the rounded value on (tan(pi/8), 0.53), where
+    # pi/4 + atan((x-1)/(x+1)) cancels; f64 within 1
+    bound = 2.5 if dt is np.float32 else 1.0
+    assert np.all(classic_ulp(packs, truth, dt) <= bound)
+    u = np.uint32 if dt is np.float32 else np.uint64
+    sb = u(1) << u(31 if dt is np.float32 else 63)
+    assert np.array_equal(_iso(mapf, "atan", -x, W).view(u), packs.view(u) ^ sb)
+    tail = mapf("atan", x[:1])  # a one-element table is all tail
+    assert classic_ulp(tail, truth[:1], dt)[0] <= bound
+    sc = np.array([mapf("atan", np.array([v], dtype=dt))[0] for v in x[:300]], dtype=dt)
+    assert np.all(classic_ulp(sc, truth[:300], dt) <= bound)
+    kats = {0.0: 0.0, np.inf: np.pi / 2, -np.inf: -np.pi / 2, 1.0: np.pi / 4, -1.0: -np.pi / 4}
+    for k, v in kats.items():
+        assert classic_ulp(_iso(mapf, "atan", np.array([k], dt), W), np.array([v]), dt)[0] <= 0.5 + 1e-9
+    assert np.isnan(_iso(mapf, "atan", np.array([np.nan], dt), W)[0])
+    # atan2 over the four quadrants and the axes
+    yy = rng.uniform(-10, 10, 3000).astype(dt)
+    xx = rng.uniform(-10, 10, 3000).astype(dt)
+    got = O.atan2(yy, xx)
+    truth = np.array([float(mpmath.atan2(mpmath.mpf(float(a)), mpmath.mpf(float(b)))) for a, b in zip(yy, xx)])
+    assert np.all(classic_ulp(got, truth, dt) <= bound + 1.0)
+    pi = dt(np.pi)
+    sp = O.atan2(np.array([0.0, -0.0, 0.0, 1.0, -1.0, np.inf, np.inf, -np.inf, np.nan, 1.0], dt),
+                 np.array([1.0, 1.0, -1.0, 0.0, 0.0, np.inf, -np.inf, np.inf, 1.0, np.nan], dt))
+    assert list(sp[:3]) == [0.0, 0.0, pi]  # a0 = +-0: pi when a1 < 0, else +0
+    assert classic_ulp(sp[3:8], np.array([np.pi / 2, -np.pi / 2, np.pi / 4, 3 * np.pi / 4, -np.pi / 4]), dt).max() <= 1
+    assert np.isnan(sp[8]) and np.isnan(sp[9])
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_atan_atan2_nt2_kats(dt):
+    """modules/core/trigonometric/unit/simd/atan.cpp:49-57 and atan2.cpp:50-62,
+    with their tolerance (0.5 in NT2's ulpdist)."""
+    W = 8 if dt is np.float32 else 4
+    pi, pio2, pio4 = dt(np.pi), dt(np.pi / 2), dt(np.pi / 4)
+    t34 = dt(3 * np.pi / 4)
+    atan_kats = [(np.inf, pio2), (-np.inf, -pio2), (0.5, dt(4.636476090008061e-01)),
+                 (-0.5, dt(-4.636476090008061e-01)), (-1.0, -pio4), (1.0, pio4), (0.0, dt(0))]
+    x = np.array([a for a, _ in atan_kats], dtype=dt)
+    got = _iso(O.map_f32 if dt is np.float32 else O.map_f64, "atan", x, W)
+    for g, (_, e) in zip(got, atan_kats):
+        assert nt2_ulpdist(dt(g), dt(e)) <= 0.5, (g, e)
+    assert np.isnan(_iso(O.map_f32 if dt is np.float32 else O.map_f64, "atan", np.array([np.nan], dt), W)[0])
+    a2 = [(np.inf, np.inf, pio4), (np.inf, 1.0, pio2), (-np.inf, -np.inf, -t34), (-np.inf, 1.0, -pio2),
+          (1.0, np.inf, dt(0)), (1.0, -np.inf, pi), (0.5, 0.5, pio4), (-0.5, -0.5, -t34), (-1.0, -1.0, -t34)]
+    y = np.repeat(np.array([a for a, _, _ in a2], dtype=dt), W)
+    xx = np.repeat(np.array([b for _, b, _ in a2], dtype=dt), W)
+    got = O.atan2(y, xx)[::W]
+    for g, (_, _, e) in zip(got, a2):
+        assert nt2_ulpdist(dt(g), dt(e)) <= 0.5, (g, e)
+    nans = O.atan2(np.array([np.nan, np.nan, 0.0], dt), np.array([np.nan, 0.0, np.nan], dt))
+    assert np.all(np.isnan(nans))
