@@ -192,6 +192,10 @@ int nt2h_blackscholes_f32(size_t n, const float* S, const float* X, const float*
 /* workgroups of the accurate-policy kernel's grid-stride launch (default 8192);
  * a tuning hook with no counterpart in the reference */
 int nt2h_bs_set_grid(int workgroups);
+/* pixels per thread of nt2h_sigmadelta_u8's main kernel: 16 (default), 8 or
+ * 4; results do not depend on it (no counterpart in the reference: a tuning
+ * hook) */
+int nt2h_sigmadelta_set_config(int pixels_per_thread);
 int nt2h_blackscholes_f64(size_t n, const double* S, const double* X, const double* T,
                           double r, double v, double* call, double* put, int policy,
                           void* stream);

@@ -591,12 +591,21 @@ __device__ __forceinline__ sd_i16x2 sd_pair(sd_i16x2 f, sd_i16x2& b, sd_i16x2& v
   return __builtin_bit_cast(sd_i16x2, __builtin_bit_cast(sd_u16x2, d + (sd_i16x2){256, 256} - v) >> (sd_u16x2){8, 8});
 }
 
-__global__ __launch_bounds__(kBlock) void sigmadelta16_k(const uint8_t* __restrict__ frames,
-                                                         size_t stride, int k0, int k1, size_t n16,
-                                                         uint8_t* __restrict__ bkg,
-                                                         uint8_t* __restrict__ var,
-                                                         uint8_t* __restrict__ label, int N,
-                                                         uint32_t* __restrict__ partials) {
+// NW 32-bit words (4 pixels each) per thread: 16 pixels (NW = 4) keeps the
+// most work per load, 8 or 4 pixels run more waves per SIMD.
+template <int NW> struct sd_vec;
+template <> struct sd_vec<4> { typedef uint4 type; static __device__ void words(const uint4& a, uint32_t (&w)[4]) { w[0] = a.x; w[1] = a.y; w[2] = a.z; w[3] = a.w; } static __device__ uint4 make(const uint32_t (&w)[4]) { return make_uint4(w[0], w[1], w[2], w[3]); } };
+template <> struct sd_vec<2> { typedef uint2 type; static __device__ void words(const uint2& a, uint32_t (&w)[2]) { w[0] = a.x; w[1] = a.y; } static __device__ uint2 make(const uint32_t (&w)[2]) { return make_uint2(w[0], w[1]); } };
+template <> struct sd_vec<1> { typedef uint32_t type; static __device__ void words(const uint32_t& a, uint32_t (&w)[1]) { w[0] = a; } static __device__ uint32_t make(const uint32_t (&w)[1]) { return w[0]; } };
+
+template <int NW>
+__global__ __launch_bounds__(kBlock) void sigmadelta_k(const uint8_t* __restrict__ frames,
+                                                       size_t stride, int k0, int k1, size_t n16,
+                                                       uint8_t* __restrict__ bkg,
+                                                       uint8_t* __restrict__ var,
+                                                       uint8_t* __restrict__ label, int N,
+                                                       uint32_t* __restrict__ partials) {
+  typedef typename sd_vec<NW>::type V;
   __shared__ uint32_t wc[kSdChunk][kBlock / 64];
   // XCD-aware tiles: workgroup b runs on XCD b % 8, and each XCD takes a
   // contiguous eighth of the image, so that its L2 sees contiguous spans of
@@ -608,16 +617,17 @@ __global__ __launch_bounds__(kBlock) void sigmadelta16_k(const uint8_t* __restri
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   const unsigned short nn = (unsigned short)(N < 255 ? N : 255);
   const sd_u16x2 n = {nn, nn};
-  sd_i16x2 b[8], v[8], l[8];
+  sd_i16x2 b[2 * NW], v[2 * NW], l[2 * NW];
   {
-    uint4 bb = make_uint4(0, 0, 0, 0), vv = make_uint4(0, 0, 0, 0);
-    if (active) {
-      bb = reinterpret_cast<const uint4*>(bkg)[t];
-      vv = reinterpret_cast<const uint4*>(var)[t];
-    }
-    const uint32_t bw[4] = {bb.x, bb.y, bb.z, bb.w}, vw[4] = {vv.x, vv.y, vv.z, vv.w};
+    uint32_t bw[NW], vw[NW];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < NW; ++q) bw[q] = vw[q] = 0u;
+    if (active) {
+      sd_vec<NW>::words(reinterpret_cast<const V*>(bkg)[t], bw);
+      sd_vec<NW>::words(reinterpret_cast<const V*>(var)[t], vw);
+    }
+#pragma unroll
+    for (int q = 0; q < NW; ++q) {
       b[2 * q] = sd_lo(bw[q]);
       b[2 * q + 1] = sd_hi(bw[q]);
       v[2 * q] = sd_lo(vw[q]);
@@ -626,17 +636,18 @@ __global__ __launch_bounds__(kBlock) void sigmadelta16_k(const uint8_t* __restri
     }
   }
   for (int k = k0; k < k1; k += 4) {
-    uint4 ff[4];
+    V ff[4];
 #pragma unroll
     for (int j = 0; j < 4; ++j)  // clamped indices: every load in bounds and issued unconditionally
-      ff[j] = reinterpret_cast<const uint4*>(frames + (size_t)(k + j < k1 ? k + j : k1 - 1) * stride)[active ? t : 0];
+      ff[j] = reinterpret_cast<const V*>(frames + (size_t)(k + j < k1 ? k + j : k1 - 1) * stride)[active ? t : 0];
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       if (k + j >= k1) break;
-      const uint32_t fw[4] = {ff[j].x, ff[j].y, ff[j].z, ff[j].w};
+      uint32_t fw[NW];
+      sd_vec<NW>::words(ff[j], fw);
       sd_i16x2 acc = {0, 0};
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
+      for (int q = 0; q < NW; ++q) {
         l[2 * q] = sd_pair(sd_lo(fw[q]), b[2 * q], v[2 * q], n);
         l[2 * q + 1] = sd_pair(sd_hi(fw[q]), b[2 * q + 1], v[2 * q + 1], n);
         acc += l[2 * q] + l[2 * q + 1];
@@ -647,16 +658,16 @@ __global__ __launch_bounds__(kBlock) void sigmadelta16_k(const uint8_t* __restri
     }
   }
   if (active) {
-    uint32_t bw[4], vw[4], lw[4];
+    uint32_t bw[NW], vw[NW], lw[NW];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < NW; ++q) {
       bw[q] = sd_pack(b[2 * q], b[2 * q + 1]);
       vw[q] = sd_pack(v[2 * q], v[2 * q + 1]);
       lw[q] = sd_pack(l[2 * q], l[2 * q + 1]);
     }
-    reinterpret_cast<uint4*>(bkg)[t] = make_uint4(bw[0], bw[1], bw[2], bw[3]);
-    reinterpret_cast<uint4*>(var)[t] = make_uint4(vw[0], vw[1], vw[2], vw[3]);
-    reinterpret_cast<uint4*>(label)[t] = make_uint4(lw[0], lw[1], lw[2], lw[3]);
+    reinterpret_cast<V*>(bkg)[t] = sd_vec<NW>::make(bw);
+    reinterpret_cast<V*>(var)[t] = sd_vec<NW>::make(vw);
+    reinterpret_cast<V*>(label)[t] = sd_vec<NW>::make(lw);
   }
   if (partials) {
     __syncthreads();
@@ -1137,6 +1148,14 @@ int nt2h_blackscholes_f64(size_t n, const double* S, const double* X, const doub
 }
 
 // ---- sigma-delta --------------------------------------------------------------
+static int g_sd_pixels = 16;
+int nt2h_sigmadelta_set_config(int pixels_per_thread) {
+  if (pixels_per_thread != 16 && pixels_per_thread != 8 && pixels_per_thread != 4)
+    return fail(NT2H_EINVAL, "nt2h_sigmadelta_set_config: 16, 8 or 4 pixels per thread");
+  g_sd_pixels = pixels_per_thread;
+  return NT2H_OK;
+}
+
 int nt2h_sigmadelta_u8(size_t npix, int nframes, const uint8_t* frames, size_t frame_stride,
                        uint8_t* bkg, uint8_t* var, uint8_t* label, int N,
                        uint32_t* frame_counts, void* stream) {
@@ -1150,11 +1169,13 @@ int nt2h_sigmadelta_u8(size_t npix, int nframes, const uint8_t* frames, size_t f
     hipLaunchKernelGGL(zero_u32_k, dim3((nframes + kBlock - 1) / kBlock), dim3(kBlock), 0, s, frame_counts, nframes);
     NT2H_CHECK_LAUNCH("sigmadelta zero counts");
   }
-  const bool vec = aligned16(frames) && aligned16(bkg) && aligned16(var) && aligned16(label) &&
-                   (frame_stride % 16 == 0);
+  const int px = g_sd_pixels;  // pixels per thread: 16, 8 or 4
+  const bool vec = ((reinterpret_cast<uintptr_t>(frames) | reinterpret_cast<uintptr_t>(bkg) |
+                     reinterpret_cast<uintptr_t>(var) | reinterpret_cast<uintptr_t>(label) | frame_stride) %
+                    (size_t)px) == 0;
   size_t p0 = 0;
   if (vec) {
-    const size_t n16 = npix / 16;
+    const size_t n16 = npix / px;
     if (n16) {
       const unsigned grid = (grid_for(n16) + 7) / 8 * 8;  // a multiple of the 8 XCDs
       Scratch sc;
@@ -1165,9 +1186,16 @@ int nt2h_sigmadelta_u8(size_t npix, int nframes, const uint8_t* frames, size_t f
       uint32_t* part = frame_counts ? reinterpret_cast<uint32_t*>(sc.partial) : nullptr;
       for (int k0 = 1; k0 < nframes; k0 += kSdChunk) {
         const int k1 = std::min(nframes, k0 + kSdChunk);
-        hipLaunchKernelGGL(sigmadelta16_k, dim3(grid), dim3(kBlock), 0, s, frames, frame_stride, k0, k1, n16,
-                           bkg, var, label, N, part);
-        NT2H_CHECK_LAUNCH("sigmadelta16");
+        if (px == 16)
+          hipLaunchKernelGGL(sigmadelta_k<4>, dim3(grid), dim3(kBlock), 0, s, frames, frame_stride, k0, k1, n16,
+                             bkg, var, label, N, part);
+        else if (px == 8)
+          hipLaunchKernelGGL(sigmadelta_k<2>, dim3(grid), dim3(kBlock), 0, s, frames, frame_stride, k0, k1, n16,
+                             bkg, var, label, N, part);
+        else
+          hipLaunchKernelGGL(sigmadelta_k<1>, dim3(grid), dim3(kBlock), 0, s, frames, frame_stride, k0, k1, n16,
+                             bkg, var, label, N, part);
+        NT2H_CHECK_LAUNCH("sigmadelta");
         if (part) {
           hipLaunchKernelGGL(sd_counts_k, dim3(k1 - k0), dim3(kBlock), 0, s, part, (int)grid,
                              frame_counts + k0);
@@ -1175,7 +1203,7 @@ int nt2h_sigmadelta_u8(size_t npix, int nframes, const uint8_t* frames, size_t f
         }
       }
     }
-    p0 = n16 * 16;
+    p0 = n16 * px;
   }
   if (p0 < npix) {
     hipLaunchKernelGGL(sigmadelta1_k, dim3(grid_for(npix - p0)), dim3(kBlock), 0, s, frames, frame_stride, nframes, p0, npix, bkg, var, label, N, frame_counts);

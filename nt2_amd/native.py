@@ -63,6 +63,7 @@ def lib():
         "nt2h_blackscholes_f32": ([S, P, P, P, F, F, P, P, I, I, P], I),
         "nt2h_blackscholes_f64": ([S, P, P, P, D, D, P, P, I, P], I),
         "nt2h_bs_set_grid": ([I], I),
+        "nt2h_sigmadelta_set_config": ([I], I),
         "nt2h_sigmadelta_u8": ([S, I, P, S, P, P, P, I, P, P], I),
         "nt2h_sigmadelta_step_u8": ([S, P, P, P, P, I, P], I),
         "nt2h_expr4_f32": ([S, P, P, P, I, P], I),
