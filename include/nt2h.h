@@ -70,6 +70,7 @@ extern "C" {
 #define NT2H_OP_SINH 22
 #define NT2H_OP_COSH 23
 #define NT2H_OP_ATAN 24
+#define NT2H_OP_TAN 25
 
 /* Black-Scholes formula variants */
 #define NT2H_BS_TEXTBOOK 0 /* d1 = (log(S/X)+(r+v^2/2)T)/(v sqrt T), call and put, erfc-based normcdf */

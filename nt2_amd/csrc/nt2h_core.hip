@@ -139,8 +139,9 @@ __global__ void fill_frames_k(uint8_t* frames, int h, int w, int nframes, Rects 
 template <class P, int OP, int G>
 __device__ __forceinline__ void apply_f32(const float* x, float* y) {
   constexpr bool scalar = (G == 1);
-  if constexpr (OP == NT2H_OP_COS || OP == NT2H_OP_SIN) {
-    dm::trig_group_f<P, G>(OP == NT2H_OP_COS ? dm::k_cos : dm::k_sin, x, y, nullptr);
+  if constexpr (OP == NT2H_OP_COS || OP == NT2H_OP_SIN || OP == NT2H_OP_TAN) {
+    dm::trig_group_f<P, G>(OP == NT2H_OP_COS ? dm::k_cos : (OP == NT2H_OP_SIN ? dm::k_sin : dm::k_tan), x, y,
+                           nullptr);
   } else {
 #pragma unroll
     for (int l = 0; l < G; ++l) {
@@ -177,8 +178,9 @@ __device__ __forceinline__ void apply_f32(const float* x, float* y) {
 template <class P, int OP, int G>
 __device__ __forceinline__ void apply_f64(const double* x, double* y) {
   constexpr bool scalar = (G == 1);
-  if constexpr (OP == NT2H_OP_COS || OP == NT2H_OP_SIN) {
-    dm::trig_group_d<P, G>(OP == NT2H_OP_COS ? dm::k_cos : dm::k_sin, x, y, nullptr);
+  if constexpr (OP == NT2H_OP_COS || OP == NT2H_OP_SIN || OP == NT2H_OP_TAN) {
+    dm::trig_group_d<P, G>(OP == NT2H_OP_COS ? dm::k_cos : (OP == NT2H_OP_SIN ? dm::k_sin : dm::k_tan), x, y,
+                           nullptr);
   } else {
 #pragma unroll
     for (int l = 0; l < G; ++l) {
@@ -815,7 +817,7 @@ static int launch_map_f32(int op, size_t n, const float* x, float* y, hipStream_
     NT2H_MAP32(NT2H_OP_FLOOR) NT2H_MAP32(NT2H_OP_CEIL) NT2H_MAP32(NT2H_OP_TRUNC) NT2H_MAP32(NT2H_OP_ROUND)
     NT2H_MAP32(NT2H_OP_ROUND2EVEN) NT2H_MAP32(NT2H_OP_SIGN)
     NT2H_MAP32(NT2H_OP_EXPM1) NT2H_MAP32(NT2H_OP_LOG1P) NT2H_MAP32(NT2H_OP_TANH)
-    NT2H_MAP32(NT2H_OP_SINH) NT2H_MAP32(NT2H_OP_COSH) NT2H_MAP32(NT2H_OP_ATAN)
+    NT2H_MAP32(NT2H_OP_SINH) NT2H_MAP32(NT2H_OP_COSH) NT2H_MAP32(NT2H_OP_ATAN) NT2H_MAP32(NT2H_OP_TAN)
 #undef NT2H_MAP32
     default: return fail(NT2H_EINVAL, "nt2h_map_f32: unknown op");
   }
@@ -842,7 +844,7 @@ static int launch_map_f64(int op, size_t n, const double* x, double* y, hipStrea
     NT2H_MAP64(NT2H_OP_FLOOR) NT2H_MAP64(NT2H_OP_CEIL) NT2H_MAP64(NT2H_OP_TRUNC) NT2H_MAP64(NT2H_OP_ROUND)
     NT2H_MAP64(NT2H_OP_ROUND2EVEN) NT2H_MAP64(NT2H_OP_SIGN)
     NT2H_MAP64(NT2H_OP_EXPM1) NT2H_MAP64(NT2H_OP_LOG1P) NT2H_MAP64(NT2H_OP_TANH)
-    NT2H_MAP64(NT2H_OP_SINH) NT2H_MAP64(NT2H_OP_COSH) NT2H_MAP64(NT2H_OP_ATAN)
+    NT2H_MAP64(NT2H_OP_SINH) NT2H_MAP64(NT2H_OP_COSH) NT2H_MAP64(NT2H_OP_ATAN) NT2H_MAP64(NT2H_OP_TAN)
 #undef NT2H_MAP64
     default: return fail(NT2H_EINVAL, "nt2h_map_f64: unknown op (fastnormcdf is single precision only)");
   }

@@ -1243,7 +1243,32 @@ NT2H_FN double sin_eval_d(double z, double x) {
   return fma<P>(y1 * z, x, x);
 }
 
-enum trig_kind { k_cos = 0, k_sin = 1, k_sincos = 2 };
+enum trig_kind { k_cos = 0, k_sin = 1, k_sincos = 2, k_tan = 3 };
+
+// tan on the reduced argument: f_trig_evaluation.hpp:47-64 (f32 polynomial in
+// z^2, z + z^3 P) and d_trig_evaluation.hpp:58-78 (f64 rational); n odd gives
+// -1/y (simd/common/impl/trigo/trig_base.hpp:115-124)
+template <class P>
+NT2H_FN float tan_eval_f(float z, int n) {
+  NT2H_STRICT
+  constexpr float c[6] = {NT2H_F(0x3c19c53b), NT2H_F(0x3b4c779c), NT2H_F(0x3cc821b5),
+                          NT2H_F(0x3d5ac5c9), NT2H_F(0x3e0896dd), NT2H_F(0x3eaaaa6f)};
+  const float zz = z * z;
+  const float y = ((horner<P>(c, zz) * zz) * z) + z;
+  return (n & 1) ? -(1.0f / y) : y;
+}
+template <class P>
+NT2H_FN double tan_eval_d(double z, int n) {
+  NT2H_STRICT
+  constexpr double num[3] = {NT2H_D(0xc0c992d8d24f3f38), NT2H_D(0x413199eca5fc9ddd),
+                             NT2H_D(0xc1711fead3299176)};
+  constexpr double den[5] = {NT2H_D(0x3ff0000000000000), NT2H_D(0x40cab8a5eeb36572),
+                             NT2H_D(0xc13427bc582abc96), NT2H_D(0x4177d98fc2ead8ef),
+                             NT2H_D(0xc189afe03cbe5a31)};
+  const double zz = z * z;
+  const double y = z + z * (zz * (horner<P>(num, zz) / horner<P>(den, zz)));
+  return (n & 1) ? -(1.0 / y) : y;
+}
 
 // cos / sin / sincos over a group of G lanes: SIMD trig_base
 // simd/common/impl/trigo/trig_base.hpp:59-71 (cos), :80-91 (sin), :140-156
@@ -1267,6 +1292,15 @@ NT2H_FN void trig_group_f(int kind, const float* a0, float* out0, float* out1, b
     const uint32_t sin_sign = (((uint32_t)n[i] & 2u) << 30) ^ (bits(a0[i]) & 0x80000000u);
     float c = as_f(bits(swap ? se : ce) ^ cos_sign);
     float s = as_f(bits(swap ? ce : se) ^ sin_sign);
+    if (kind == k_tan) {
+      // scalar form (scalar/impl/trigo/trig_base.hpp:165-176): NaN for
+      // Inf/NaN, a0 itself for +-0
+      float t = as_f(bits(tan_eval_f<P>(xr[i], n[i])) ^ (bits(a0[i]) & 0x80000000u));
+      if (G == 1 && !__builtin_isfinite(a0[i])) t = __builtin_nanf("");
+      else if (G == 1 && a0[i] == 0.0f) t = a0[i];
+      out0[i] = t;
+      continue;
+    }
     if (G == 1 && !__builtin_isfinite(a0[i])) { c = __builtin_nanf(""); s = c; }
     if (kind == k_cos) out0[i] = c;
     else if (kind == k_sin) out0[i] = s;
@@ -1292,6 +1326,13 @@ NT2H_FN void trig_group_d(int kind, const double* a0, double* out0, double* out1
     const uint64_t sin_sign = (((uint64_t)n[i] & 2u) << 62) ^ (bits(a0[i]) & 0x8000000000000000ull);
     double c = as_d(bits(swap ? se : ce) ^ cos_sign);
     double s = as_d(bits(swap ? ce : se) ^ sin_sign);
+    if (kind == k_tan) {
+      double t = as_d(bits(tan_eval_d<P>(xr[i], n[i])) ^ (bits(a0[i]) & 0x8000000000000000ull));
+      if (G == 1 && !__builtin_isfinite(a0[i])) t = __builtin_nan("");
+      else if (G == 1 && a0[i] == 0.0) t = a0[i];
+      out0[i] = t;
+      continue;
+    }
     if (G == 1 && !__builtin_isfinite(a0[i])) { c = __builtin_nan(""); s = c; }
     if (kind == k_cos) out0[i] = c;
     else if (kind == k_sin) out0[i] = s;

@@ -195,7 +195,7 @@ NT2H_TAG(exp2_, same1) NT2H_TAG(exp10_, same1) NT2H_TAG(log2_, same1) NT2H_TAG(l
 NT2H_TAG(floor_, same1) NT2H_TAG(ceil_, same1) NT2H_TAG(trunc_, same1) NT2H_TAG(round_, same1)
 NT2H_TAG(round2even_, same1) NT2H_TAG(sign_, same1) NT2H_TAG(expm1_, same1) NT2H_TAG(log1p_, same1)
 NT2H_TAG(tanh_, same1) NT2H_TAG(sinh_, same1) NT2H_TAG(cosh_, same1) NT2H_TAG(atan_, same1)
-NT2H_TAG(atan2_, same2)
+NT2H_TAG(atan2_, same2) NT2H_TAG(tan_, same1)
 NT2H_TAG(fma_, same3) NT2H_TAG(fnms_, same3) NT2H_TAG(if_else_, sel3)
 NT2H_TAG(selinc_, selinc2) NT2H_TAG(seldec_, selinc2) NT2H_TAG(if_zero_else_one_, izeo1)
 #undef NT2H_TAG
@@ -279,6 +279,12 @@ template <> struct apply1<tagfn::cos_> {
   template <int G, class T> __device__ __forceinline__ static void run(T (&a)[G], T (&r)[G]) {
     if constexpr (std::is_same<T, float>::value) dm::trig_group_f<P, G>(dm::k_cos, a, r, nullptr);
     else dm::trig_group_d<P, G>(dm::k_cos, a, r, nullptr);
+  }
+};
+template <> struct apply1<tagfn::tan_> {
+  template <int G, class T> __device__ __forceinline__ static void run(T (&a)[G], T (&r)[G]) {
+    if constexpr (std::is_same<T, float>::value) dm::trig_group_f<P, G>(dm::k_tan, a, r, nullptr);
+    else dm::trig_group_d<P, G>(dm::k_tan, a, r, nullptr);
   }
 };
 template <> struct apply1<tagfn::sin_> {
@@ -535,7 +541,7 @@ NT2H_FN1(exp2, exp2_) NT2H_FN1(exp10, exp10_) NT2H_FN1(log2, log2_) NT2H_FN1(log
 NT2H_FN1(floor, floor_) NT2H_FN1(ceil, ceil_) NT2H_FN1(trunc, trunc_) NT2H_FN1(round, round_)
 NT2H_FN1(round2even, round2even_) NT2H_FN1(sign, sign_)
 NT2H_FN1(expm1, expm1_) NT2H_FN1(log1p, log1p_) NT2H_FN1(tanh, tanh_) NT2H_FN1(sinh, sinh_)
-NT2H_FN1(cosh, cosh_) NT2H_FN1(atan, atan_)
+NT2H_FN1(cosh, cosh_) NT2H_FN1(atan, atan_) NT2H_FN1(tan, tan_)
 NT2H_FN1(if_zero_else_one, if_zero_else_one_)
 #undef NT2H_FN1
 #define NT2H_FN2(name, Tag) \
@@ -848,12 +854,13 @@ namespace expr {
 template <class X> struct pack_free { static constexpr bool value = true; };
 template <class Tag, class A> struct pack_free<node1<Tag, A>> {
   static constexpr bool value = pack_free<A>::value && !std::is_same<Tag, tagfn::cos_>::value &&
-                                !std::is_same<Tag, tagfn::sin_>::value &&
+                                !std::is_same<Tag, tagfn::sin_>::value && !std::is_same<Tag, tagfn::tan_>::value &&
+                                !std::is_same<Tag, tagfn::atan_>::value &&
                                 !((std::is_same<Tag, tagfn::erfc_>::value || std::is_same<Tag, tagfn::normcdf_>::value) &&
                                   std::is_same<typename A::value_type, double>::value);
 };
 template <class Tag, class A, class B> struct pack_free<node2<Tag, A, B>> {
-  static constexpr bool value = pack_free<A>::value && pack_free<B>::value;
+  static constexpr bool value = pack_free<A>::value && pack_free<B>::value && !std::is_same<Tag, tagfn::atan2_>::value;
 };
 template <class Tag, class A, class B, class C> struct pack_free<node3<Tag, A, B, C>> {
   static constexpr bool value = pack_free<A>::value && pack_free<B>::value && pack_free<C>::value;

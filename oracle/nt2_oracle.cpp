@@ -88,6 +88,9 @@ const float fPio_4 = F32(0x3f490fdb), fPio_2 = F32(0x3fc90fdb),
             fPio2_1t = F32(0x37354443), fPio2_2t = F32(0x2e85a308),
             fPio2_3t = F32(0x248d3132), fTwoopi = F32(0x3f22f983),
             fInvpio_2 = F32(0x3f22f984);
+// f_trig_evaluation.hpp:47-64 (tan)
+const float fTanC[6] = {F32(0x3c19c53b), F32(0x3b4c779c), F32(0x3cc821b5), F32(0x3d5ac5c9),
+                        F32(0x3e0896dd), F32(0x3eaaaa6f)};
 // f_trig_evaluation.hpp:35-46
 const float fCosC[3] = {F32(0x37ccf5ce), F32(0xbab60619), F32(0x3d2aaaa5)};
 const float fSinC[3] = {F32(0xb94ca1f9), F32(0x3c08839d), F32(0xbe2aaaa2)};
@@ -139,6 +142,10 @@ const double dPio_4 = F64(0x3fe921fb54442d18), dPio_2 = F64(0x3ff921fb54442d18),
              dPio2_2t = F64(0x3BA3198A2E037073), dPio2_3t = F64(0x397B839A252049C1),
              dTwoopi = F64(0x3fe45f306dc9c883), dInvpio_2 = F64(0x3FE45F306DC9C883);
 const double dDmpi = (double)F32(0x49490fdb);
+// d_trig_evaluation.hpp:58-70 (tan)
+const double dTanN[3] = {F64(0xc0c992d8d24f3f38), F64(0x413199eca5fc9ddd), F64(0xc1711fead3299176)};
+const double dTanD[5] = {F64(0x3ff0000000000000), F64(0x40cab8a5eeb36572), F64(0xc13427bc582abc96),
+                         F64(0x4177d98fc2ead8ef), F64(0xc189afe03cbe5a31)};
 // d_trig_evaluation.hpp:33-50
 const double dCosC[7] = {F64(0x3da8ff831ad9b219), F64(0xbe21eea7c1e514d4),
                          F64(0x3e927e4f8e06d9a5), F64(0xbefa01a019ddbcd9),
@@ -1108,6 +1115,16 @@ void trig_group_f(int kind, const float* a0, float* out0, float* out1) {
     const uint32_t sin_sign = (((uint32_t)n[i] & 2u) << 30) ^ (bits(a0[i]) & 0x80000000u);
     float c = as_f(bits(swap ? se : ce) ^ cos_sign);
     float s = as_f(bits(swap ? ce : se) ^ sin_sign);
+    if (kind == 3) {  // tan: f_trig_evaluation.hpp:47-64, trig_base.hpp:115-124 (scalar :165-176)
+      const float zz = xr[i] * xr[i];
+      float y = ((horner(fTanC, zz) * zz) * xr[i]) + xr[i];
+      if (n[i] & 1) y = -(1.0f / y);
+      float t = as_f(bits(y) ^ (bits(a0[i]) & 0x80000000u));
+      if (G == 1 && !std::isfinite(a0[i])) t = NAN;
+      else if (G == 1 && a0[i] == 0.0f) t = a0[i];
+      out0[i] = t;
+      continue;
+    }
     if (G == 1 && !std::isfinite(a0[i])) { c = NAN; s = NAN; }
     if (kind == 0) out0[i] = c;
     else if (kind == 1) out0[i] = s;
@@ -1130,6 +1147,16 @@ void trig_group_d(int kind, const double* a0, double* out0, double* out1) {
     const uint64_t sin_sign = (((uint64_t)n[i] & 2u) << 62) ^ (bits(a0[i]) & 0x8000000000000000ull);
     double c = as_d(bits(swap ? se : ce) ^ cos_sign);
     double s = as_d(bits(swap ? ce : se) ^ sin_sign);
+    if (kind == 3) {  // tan: d_trig_evaluation.hpp:58-78
+      const double zz = xr[i] * xr[i];
+      double y = xr[i] + xr[i] * (zz * (horner(dTanN, zz) / horner(dTanD, zz)));
+      if (n[i] & 1) y = -(1.0 / y);
+      double t = as_d(bits(y) ^ (bits(a0[i]) & 0x8000000000000000ull));
+      if (G == 1 && !std::isfinite(a0[i])) t = NAN;
+      else if (G == 1 && a0[i] == 0.0) t = a0[i];
+      out0[i] = t;
+      continue;
+    }
     if (G == 1 && !std::isfinite(a0[i])) { c = NAN; s = NAN; }
     if (kind == 0) out0[i] = c;
     else if (kind == 1) out0[i] = s;
@@ -1286,7 +1313,7 @@ enum {
   OR_FASTNORMCDF = 5, OR_COS = 6, OR_SIN = 7, OR_ABS = 8, OR_EXP2 = 9, OR_EXP10 = 10,
   OR_LOG2 = 11, OR_LOG10 = 12, OR_FLOOR = 13, OR_CEIL = 14, OR_TRUNC = 15, OR_ROUND = 16,
   OR_ROUND2EVEN = 17, OR_SIGN = 18, OR_EXPM1 = 19, OR_LOG1P = 20, OR_TANH = 21, OR_SINH = 22,
-  OR_COSH = 23, OR_ATAN = 24, OR_LAST = OR_ATAN
+  OR_COSH = 23, OR_ATAN = 24, OR_TAN = 25, OR_LAST = OR_TAN
 };
 
 int oracle_abi_version(void) { return 1; }
@@ -1332,8 +1359,9 @@ int oracle_map_f32(int op, const float* x, float* y, size_t n) {
       case OR_ATAN: for (size_t l = 0; l < w; ++l) y[i + l] = atan_f(x[i + l], !pack); break;
       case OR_COS:
       case OR_SIN:
-        if (pack) trig_group_f<8>(op == OR_COS ? 0 : 1, x + i, y + i, nullptr);
-        else trig_group_f<1>(op == OR_COS ? 0 : 1, x + i, y + i, nullptr);
+      case OR_TAN:
+        if (pack) trig_group_f<8>(op == OR_COS ? 0 : (op == OR_SIN ? 1 : 3), x + i, y + i, nullptr);
+        else trig_group_f<1>(op == OR_COS ? 0 : (op == OR_SIN ? 1 : 3), x + i, y + i, nullptr);
         break;
     }
   });
@@ -1369,8 +1397,9 @@ int oracle_map_f64(int op, const double* x, double* y, size_t n) {
       case OR_ATAN: for (size_t l = 0; l < w; ++l) y[i + l] = atan_d(x[i + l], !pack); break;
       case OR_COS:
       case OR_SIN:
-        if (pack) trig_group_d<4>(op == OR_COS ? 0 : 1, x + i, y + i, nullptr);
-        else trig_group_d<1>(op == OR_COS ? 0 : 1, x + i, y + i, nullptr);
+      case OR_TAN:
+        if (pack) trig_group_d<4>(op == OR_COS ? 0 : (op == OR_SIN ? 1 : 3), x + i, y + i, nullptr);
+        else trig_group_d<1>(op == OR_COS ? 0 : (op == OR_SIN ? 1 : 3), x + i, y + i, nullptr);
         break;
     }
   });

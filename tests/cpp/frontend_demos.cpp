@@ -205,12 +205,16 @@ int main(int argc, char** argv) {
   dump("at_a", at);
   dump("at_b", bt);
   dump("atan_f32", at2);
+  at2 = nt2::tan(at) * 0.5f - nt2::tan(bt * 100.0f);
+  dump("tan_f32", at2);
   table<double> dm(of_size(mh * mw)), dr;
   for (std::size_t i = 1; i <= mh * mw; ++i) dm(i) = (double)M(i) / 100.0;
   dr = nt2::trunc(dm) * nt2::exp2(nt2::ceil(dm / 50.0)) + nt2::sign(dm) * nt2::log10(nt2::abs(dm) + 1.0);
   dump("ew_f64", dr);
   dr = nt2::sinh(dm / 20.0) - nt2::cosh(dm / 30.0);
   dump("ew2_f64", dr);
+  dr = nt2::tan(dm / 7.0) + nt2::atan(dm);
+  dump("tan_f64", dr);
 
   // ---- rgb2grey and rgb2yuv on a 96 x 131 image: the demos' ramp and random planes
   const std::size_t ih = 96, iw = 131;

@@ -70,6 +70,7 @@ FUNCS = {
     "tanh": lambda x: mpmath.tanh(x),
     "sinh": lambda x: mpmath.sinh(x),
     "cosh": lambda x: mpmath.cosh(x),
+    "tan": lambda x: mpmath.tan(x),
 }
 
 
@@ -117,6 +118,11 @@ def inputs(name, dtype, rng):
         pts += [0.0, 0.5, 1.0, 2.0, 2.0 / 3.0, 0.65, 2.2, 6.0, -0.5, -1.0, -2.0]
         if name == "normcdf":
             pts += list(rng.uniform(-14, -6, 150))
+    elif name == "tan":
+        pts += list(rng.uniform(-np.pi / 4, np.pi / 4, 300)) + list(rng.uniform(-np.pi / 2, np.pi / 2, 300))
+        pts += list(rng.uniform(-20 * np.pi, 20 * np.pi, 300)) + list(rng.uniform(-1e4, 1e4, 300))
+        pts += list(np.exp2(rng.uniform(20, 100, 200)) * rng.choice([-1.0, 1.0], 200))
+        pts += [0.0, 1.0, -1.0, 0.5, np.pi / 4, 1.5, -1.5, 100.0, 1e5]
     else:  # cos, sin
         pts += list(rng.uniform(-np.pi / 4, np.pi / 4, 150))
         pts += list(rng.uniform(-np.pi / 2, np.pi / 2, 150))
@@ -269,6 +275,15 @@ KATS = [
     ("cosh", "-inf", "inf", 0.5, "modules/core/hyperbolic/unit/simd/cosh.cpp:43"),
     ("cosh", "nan", "nan", 0.5, "modules/core/hyperbolic/unit/simd/cosh.cpp:44"),
     ("cosh", "0", "1", 0.5, "modules/core/hyperbolic/unit/simd/cosh.cpp:46"),
+    # modules/core/trigonometric/unit/simd/tan.cpp:45-53
+    ("tan", "inf", "nan", 0.5, "modules/core/trigonometric/unit/simd/tan.cpp:45"),
+    ("tan", "-inf", "nan", 0.5, "modules/core/trigonometric/unit/simd/tan.cpp:46"),
+    ("tan", "nan", "nan", 0.5, "modules/core/trigonometric/unit/simd/tan.cpp:47"),
+    ("tan", "-pi", "0", 0.75, "modules/core/trigonometric/unit/simd/tan.cpp:49"),
+    ("tan", "-pi/4", "-1", 1.0, "modules/core/trigonometric/unit/simd/tan.cpp:50"),
+    ("tan", "pi", "0", 0.75, "modules/core/trigonometric/unit/simd/tan.cpp:51"),
+    ("tan", "pi/4", "1", 1.0, "modules/core/trigonometric/unit/simd/tan.cpp:52"),
+    ("tan", "0", "0", 0.5, "modules/core/trigonometric/unit/simd/tan.cpp:53"),
 ]
 
 

@@ -20,9 +20,9 @@ from nt2_amd import EXACT, FAST, ops  # noqa: E402
 G = os.path.join(os.path.dirname(__file__), "golden")
 ROUNDING = ["floor", "ceil", "trunc", "round", "round2even", "sign"]
 F32_OPS = ["exp", "log", "sqrt", "erfc", "normcdf", "fastnormcdf", "cos", "sin", "abs",
-           "exp2", "exp10", "log2", "log10"] + ROUNDING + ["expm1", "log1p", "tanh", "sinh", "cosh", "atan"]
+           "exp2", "exp10", "log2", "log10"] + ROUNDING + ["expm1", "log1p", "tanh", "sinh", "cosh", "atan", "tan"]
 F64_OPS = ["exp", "log", "sqrt", "erfc", "normcdf", "cos", "sin", "abs",
-           "exp2", "exp10", "log2", "log10"] + ROUNDING + ["expm1", "log1p", "tanh", "sinh", "cosh", "atan"]
+           "exp2", "exp10", "log2", "log10"] + ROUNDING + ["expm1", "log1p", "tanh", "sinh", "cosh", "atan", "tan"]
 
 
 def _mixed(rng, n, lo, hi, dt):
@@ -65,7 +65,7 @@ def _inputs_f32(op, rng, n):
                                np.float32([0.41421357, 0.41421354, 2.4142137, 2.4142134, 0.66])])
     if op in ("erfc", "normcdf", "fastnormcdf"):
         return rng.uniform(-12, 12, n).astype(np.float32)
-    if op in ("cos", "sin"):
+    if op in ("cos", "sin", "tan"):
         parts = [rng.uniform(-1, 1, n // 4), rng.uniform(-60, 60, n // 4),
                  rng.uniform(-1e4, 1e4, n // 4),
                  np.exp2(rng.uniform(0, 127, n - 3 * (n // 4))) * rng.choice([-1, 1], n - 3 * (n // 4))]
@@ -98,7 +98,7 @@ def _inputs_f64(op, rng, n):
                                [0.41421356237309503, 2.414213562373095, 0.66, 0.6600000000000001]])
     if op in ("erfc", "normcdf"):
         return rng.uniform(-30, 30, n)
-    if op in ("cos", "sin"):
+    if op in ("cos", "sin", "tan"):
         parts = [rng.uniform(-1, 1, n // 4), rng.uniform(-60, 60, n // 4),
                  rng.uniform(-1e6, 1e6, n // 4),
                  np.exp2(rng.uniform(0, 1000, n - 3 * (n // 4))) * rng.choice([-1, 1], n - 3 * (n // 4))]
@@ -149,7 +149,8 @@ def _truth_f32(op, x):
          "log10": lambda v: np.log10(np.where(v >= 0, v, np.nan)),
          "floor": np.floor, "ceil": np.ceil, "trunc": np.trunc, "round2even": np.rint, "sign": np.sign,
          "round": lambda v: np.sign(v) * np.floor(np.abs(v) + 0.5),
-         "expm1": np.expm1, "log1p": lambda v: np.log1p(np.where(v >= -1, v, np.nan)), "tanh": np.tanh, "sinh": np.sinh, "cosh": np.cosh, "atan": np.arctan}[op]
+         "expm1": np.expm1, "log1p": lambda v: np.log1p(np.where(v >= -1, v, np.nan)), "tanh": np.tanh, "sinh": np.sinh, "cosh": np.cosh, "atan": np.arctan,
+         "tan": np.tan}[op]
     with np.errstate(all="ignore"):
         return f(xd)
 
@@ -183,7 +184,7 @@ def test_map_f32_fast_within_bound(op):
     same bounds the oracle is pinned to), and a few ulp of the exact policy."""
     rng = np.random.default_rng(300 + F32_OPS.index(op))
     x = _inputs_f32(op, rng, 40000)
-    if op in ("cos", "sin"):
+    if op in ("cos", "sin", "tan"):
         x = x[np.abs(x) < 823549]  # beyond: fdlibm path, identical in both policies
         x = x[np.abs(x) < 1e4]
     say("map_f32 fast", op, "n=%d" % x.size)
@@ -205,6 +206,10 @@ def test_map_f32_fast_within_bound(op):
         # where terms of magnitude 1 cancel: compare those absolutely
         ok |= np.abs(got.astype(np.float64) - truth) <= 2 * np.finfo(np.float32).eps
         ok |= np.abs(got.astype(np.float64) - exact) <= 4 * np.finfo(np.float32).eps
+    if op == "tan":
+        # the same reduction differences, carried through d tan / dx = 1 + tan^2
+        e64 = exact.astype(np.float64)
+        ok |= np.abs(got.astype(np.float64) - e64) <= 4 * np.finfo(np.float32).eps * (1 + e64 * e64)
     assert ok.all(), (x[~ok][:4], got[~ok][:4], truth[~ok][:4], err[~ok][:4])
 
 

@@ -55,6 +55,10 @@ def _bound(name, x, y, dt):
         return np.full(x.shape, 1.0)
     if name in ("expm1", "log1p", "tanh", "sinh", "cosh"):
         return np.full(x.shape, BOUND3[name][dt])
+    if name == "tan":
+        # 2 ulp in f32 / 1 in f64 away from the poles; the -1/y branch of an
+        # argument within a few ulp of (k + 1/2) pi amplifies the reduction's error
+        return np.where(np.abs(y.astype(np.float64)) < 1e3, 2.0 if dt is np.float32 else 2.0, np.inf)
     if name == "log10":
         # Log10_2hi*k is added before Log10_2lo*k (f_log.hpp:163-169), so the
         # large term is rounded twice: up to 1.6 ulp from the exact value in
@@ -83,7 +87,7 @@ def _bound(name, x, y, dt):
 
 @pytest.mark.parametrize("dt,tag", [(np.float32, "f32"), (np.float64, "f64")])
 @pytest.mark.parametrize("name", ["exp", "log", "erfc", "normcdf", "cos", "sin", "exp2", "exp10", "log2", "log10",
-                                  "expm1", "log1p", "tanh", "sinh", "cosh"])
+                                  "expm1", "log1p", "tanh", "sinh", "cosh", "tan"])
 def test_golden_mpmath(dt, tag, name):
     g = np.load(os.path.join(G, "math_%s.npz" % tag))
     x, y = g[name + "_x"], g[name + "_y"]
@@ -100,7 +104,7 @@ def test_golden_mpmath(dt, tag, name):
         x, y = x[keep], y[keep]
     mapf = O.map_f32 if dt is np.float32 else O.map_f64
     W = 8 if dt is np.float32 else 4
-    if name in ("cos", "sin") and dt is np.float32:
+    if name in ("cos", "sin", "tan") and dt is np.float32:
         O.set_ph_fix(True)  # NT2's Payne-Hanek shift is tested separately
     try:
         got = _iso(mapf, name, x, W)
@@ -420,3 +424,28 @@ def test_atan_atan2_nt2_kats(dt):
         assert nt2_ulpdist(dt(g), dt(e)) <= 0.5, (g, e)
     nans = O.atan2(np.array([np.nan, np.nan, 0.0], dt), np.array([np.nan, 0.0, np.nan], dt))
     assert np.all(np.isnan(nans))
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_tan_structure(dt):
+    """tan is odd bit for bit (the sign of the argument is applied last:
+    modules/core/trigonometric/include/nt2/trigonometric/functions/simd/common/impl/trigo/trig_base.hpp:115-125),
+    and agrees with sin/cos of the same pack to
+    a few ulp away from the poles and zeros of tan."""
+    mapf = O.map_f32 if dt is np.float32 else O.map_f64
+    rng = np.random.default_rng(31)
+    for lo, hi in ((0.0, 0.78), (0.78, 20.0), (20.0, 1e5)):
+        x = rng.uniform(lo, hi, 4096).astype(dt)
+        t, tm = mapf("tan", x), mapf("tan", -x)
+        assert np.array_equal(t.view(np.uint32 if dt is np.float32 else np.uint64),
+                              (-tm).view(np.uint32 if dt is np.float32 else np.uint64))
+        q = mapf("sin", x).astype(np.float64) / mapf("cos", x).astype(np.float64)
+        ok = (np.abs(q) > 1e-3) & (np.abs(q) < 1e3)
+        assert np.all(classic_ulp(t[ok], q[ok], dt) <= 4.0)
+    # a pack holding a NaN or an Inf keeps its finite lanes; -0 stays -0 in the tail
+    x = np.array([0.5, np.nan, 1.0, np.inf, -2.0, 3.0, 4.0, 5.0, -0.0], dtype=dt)
+    t = mapf("tan", x)
+    assert np.isnan(t[1]) and np.isnan(t[3])
+    fin = np.isfinite(x[:8])
+    assert np.all(classic_ulp(t[:8][fin], np.tan(x[:8][fin].astype(np.float64)), dt) <= 2.0)
+    assert t[8] == 0.0 and np.signbit(t[8])
