@@ -44,7 +44,7 @@ extern "C" {
                            own approximations; within a few ulp of the exact price */
 
 /* elementwise operations for nt2h_map_* (NT2 functors exp_, log_, sqrt_, erfc_,
- * normcdf_, fastnormcdf_, cos_, sin_, abs_) */
+ * normcdf_, fastnormcdf_, cos_, sin_, abs_, exp2_ ... atan_, tan_) */
 #define NT2H_OP_EXP 0
 #define NT2H_OP_LOG 1
 #define NT2H_OP_SQRT 2
