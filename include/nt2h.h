@@ -44,7 +44,7 @@ extern "C" {
                            own approximations; within a few ulp of the exact price */
 
 /* elementwise operations for nt2h_map_* (NT2 functors exp_, log_, sqrt_, erfc_,
- * normcdf_, fastnormcdf_, cos_, sin_, abs_, exp2_ ... atan_, tan_) */
+ * normcdf_, fastnormcdf_, cos_, sin_, abs_, exp2_ ... atan_, tan_, asin_, acos_) */
 #define NT2H_OP_EXP 0
 #define NT2H_OP_LOG 1
 #define NT2H_OP_SQRT 2
@@ -71,6 +71,8 @@ extern "C" {
 #define NT2H_OP_COSH 23
 #define NT2H_OP_ATAN 24
 #define NT2H_OP_TAN 25
+#define NT2H_OP_ASIN 26
+#define NT2H_OP_ACOS 27
 
 /* Black-Scholes formula variants */
 #define NT2H_BS_TEXTBOOK 0 /* d1 = (log(S/X)+(r+v^2/2)T)/(v sqrt T), call and put, erfc-based normcdf */

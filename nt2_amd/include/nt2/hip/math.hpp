@@ -674,6 +674,110 @@ NT2H_FN T atan2_t(T a0, T a1) {
 template <class P, bool Scalar> NT2H_FN float atan2_f(float a0, float a1) { return atan2_t<P, Scalar, float>(a0, a1); }
 template <class P, bool Scalar> NT2H_FN double atan2_d(double a0, double a1) { return atan2_t<P, Scalar, double>(a0, a1); }
 
+// asin, acos (f32): simd/common/impl/invtrig/f_invtrig.hpp:55-97.  |x| up to
+// 1/2 by x + x^3 p(x^2); above by pi/2 - 2 asin(sqrt((1-|x|)/2)); acos from
+// asin of sqrt((1-|x|)/2) or of x.  NT2's scalar forms
+// (scalar/impl/invtrig/f_invtrig.hpp:49-90) give the same values, so a pack's
+// tail needs no separate form.
+template <class P>
+NT2H_FN float asin_f(float a0) {
+  NT2H_STRICT
+  constexpr float c[5] = {NT2H_F(0x3d2cb352), NT2H_F(0x3cc617e3), NT2H_F(0x3d3a3ec7),
+                          NT2H_F(0x3d9980f6), NT2H_F(0x3e2aaae4)};
+  float x = __builtin_fabsf(a0);
+  const bool big = x > 0.5f;
+  const float z = big ? 0.5f * (1.0f - x) : x * x;
+  if (big) x = __builtin_sqrtf(z);
+  float z1 = fma<P>(horner<P>(c, z), z * x, x);
+  if (big) z1 = NT2H_F(0x3fc90fdb) - (z1 + z1);
+  return as_f(bits(z1) ^ (bits(a0) & 0x80000000u));
+}
+template <class P>
+NT2H_FN float acos_f(float a0) {
+  NT2H_STRICT
+  float x = __builtin_fabsf(a0);
+  const bool big = x > 0.5f;
+  x = big ? __builtin_sqrtf(fma<P>(-0.5f, x, 0.5f)) : a0;
+  x = asin_f<P>(x);
+  if (big) x = x + x;
+  if (a0 < -0.5f) x = NT2H_F(0x40490fdb) - x;
+  return big ? x : NT2H_F(0x3fc90fdb) - x;
+}
+
+// asin, acos (f64).  Packs: simd/common/impl/invtrig/d_invtrig.hpp:57-127,
+// Cephes' rational forms, |x| above 0.625 through sqrt(2(1-|x|)) with the low
+// part of pi/2 subtracted, acos below -1/2 as pi - 2 asin(sqrt((1-|x|)/2)).
+// The scalar forms that NT2 uses for a pack's tail
+// (scalar/impl/invtrig/d_invtrig.hpp:46-111) add that low part instead and
+// take acos for x <= 1/2 as (pi/4 - asin(x) + pi/4lo) + pi/4: Scalar selects
+// them.
+template <class P, bool Scalar>
+NT2H_FN double asin_d(double a0) {
+  NT2H_STRICT
+  constexpr double rr[5] = {NT2H_D(0x3f684fc3988e9f08), NT2H_D(0xbfe2079259f9290f), NT2H_D(0x401bdff5baf33e6a),
+                            NT2H_D(0xc03991aaac01ab68), NT2H_D(0x403c896240f3081d)};
+  constexpr double ss[5] = {NT2H_D(0x3ff0000000000000), NT2H_D(0xc035f2a2b6bf5d8c), NT2H_D(0x40626219af6a7f42),
+                            NT2H_D(0xc077fe08959063ee), NT2H_D(0x40756709b0b644be)};
+  constexpr double pp[6] = {NT2H_D(0x3f716b9b0bd48ad3), NT2H_D(0xbfe34341333e5c16), NT2H_D(0x4015c74b178a2dd9),
+                            NT2H_D(0xc0304331de27907b), NT2H_D(0x40339007da779259), NT2H_D(0xc020656c06ceafd5)};
+  constexpr double qq[6] = {NT2H_D(0x3ff0000000000000), NT2H_D(0xc02d7b590b5e0eab), NT2H_D(0x40519fc025fe9054),
+                            NT2H_D(0xc06265bb6d3576d7), NT2H_D(0x4061705684ffbf9d), NT2H_D(0xc04898220a3607ac)};
+  const double pio4 = NT2H_D(0x3fe921fb54442d18), pio2lo = NT2H_D(0x3c91a62633145c07);
+  const double sqrteps = NT2H_D(0x3e50000000000000), ct1 = NT2H_D(0x3fe4000000000000);
+  const double x = __builtin_fabs(a0);
+  double r;
+  if constexpr (Scalar) {
+    if (x > 1.0) return __builtin_nan("");
+    if (x < sqrteps) return a0;
+    if (x > ct1) {
+      double zz = 1.0 - x;
+      const double vp = (zz * horner<P>(rr, zz)) / horner<P>(ss, zz);
+      zz = __builtin_sqrt(zz + zz);
+      const double z = pio4 - zz;
+      r = (z - fma<P>(zz, vp, pio2lo)) + pio4;
+    } else {
+      const double zz = x * x;
+      const double z = (zz * horner<P>(pp, zz)) / horner<P>(qq, zz);
+      r = x * z + x;
+    }
+  } else {
+    if (x > 1.0) return __builtin_nan("");
+    if (x < sqrteps) {
+      r = x;
+    } else if (x > ct1) {
+      double zz1 = 1.0 - x;
+      const double vp = (zz1 * horner<P>(rr, zz1)) / horner<P>(ss, zz1);
+      zz1 = __builtin_sqrt(zz1 + zz1);
+      const double z = pio4 - zz1;
+      r = (z - fma<P>(zz1, vp, -pio2lo)) + pio4;  // NT2's fms(zz1, vp, pio2lo)
+    } else {
+      const double zz2 = x * x;
+      const double z = (zz2 * horner<P>(pp, zz2)) / horner<P>(qq, zz2);
+      r = x * z + x;
+    }
+  }
+  return as_d(bits(r) ^ (bits(a0) & 0x8000000000000000ull));
+}
+template <class P, bool Scalar>
+NT2H_FN double acos_d(double a0) {
+  NT2H_STRICT
+  const double pi = NT2H_D(0x400921fb54442d18), pio2 = NT2H_D(0x3ff921fb54442d18);
+  const double pio4 = NT2H_D(0x3fe921fb54442d18), pio4lo = NT2H_D(0x3c81a62633145c07);
+  if constexpr (Scalar) {
+    if (__builtin_fabs(a0) > 1.0) return __builtin_nan("");
+    if (a0 > 0.5) return 2.0 * asin_d<P, true>(__builtin_sqrt(fma<P>(-0.5, a0, 0.5)));
+    return ((pio4 - asin_d<P, true>(a0)) + pio4lo) + pio4;
+  } else {
+    double x = __builtin_fabs(a0);
+    const bool big = x > 0.5;
+    x = big ? __builtin_sqrt(fma<P>(-0.5, x, 0.5)) : a0;
+    x = asin_d<P, false>(x);
+    if (big) x = x + x;
+    if (a0 < -0.5) x = pi - x;
+    return big ? x : pio2 - x;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // erfc.  f32: modules/core/euler/include/nt2/euler/functions/simd/common/erfc.hpp:92-123
 // (the scalar form returns the same values), kernels erf_kernel.hpp:46-85.

@@ -195,7 +195,7 @@ NT2H_TAG(exp2_, same1) NT2H_TAG(exp10_, same1) NT2H_TAG(log2_, same1) NT2H_TAG(l
 NT2H_TAG(floor_, same1) NT2H_TAG(ceil_, same1) NT2H_TAG(trunc_, same1) NT2H_TAG(round_, same1)
 NT2H_TAG(round2even_, same1) NT2H_TAG(sign_, same1) NT2H_TAG(expm1_, same1) NT2H_TAG(log1p_, same1)
 NT2H_TAG(tanh_, same1) NT2H_TAG(sinh_, same1) NT2H_TAG(cosh_, same1) NT2H_TAG(atan_, same1)
-NT2H_TAG(atan2_, same2) NT2H_TAG(tan_, same1)
+NT2H_TAG(atan2_, same2) NT2H_TAG(tan_, same1) NT2H_TAG(asin_, same1) NT2H_TAG(acos_, same1)
 NT2H_TAG(fma_, same3) NT2H_TAG(fnms_, same3) NT2H_TAG(if_else_, sel3)
 NT2H_TAG(selinc_, selinc2) NT2H_TAG(seldec_, selinc2) NT2H_TAG(if_zero_else_one_, izeo1)
 #undef NT2H_TAG
@@ -273,6 +273,9 @@ NT2H_UN(cosh_, dm::cosh_f<P>(x), dm::cosh_d<P>(x), no_integer_form<T>())
 // atan: the pack form for packs, NT2's scalar form for the tail (G == 1)
 NT2H_UN(atan_, (G == 1 ? dm::atan_f<P, true>(x) : dm::atan_f<P, false>(x)),
         (G == 1 ? dm::atan_d<P, true>(x) : dm::atan_d<P, false>(x)), no_integer_form<T>())
+// asin, acos: one f32 form; f64 packs and tails differ (G == 1 is the tail)
+NT2H_UN(asin_, dm::asin_f<P>(x), (G == 1 ? dm::asin_d<P, true>(x) : dm::asin_d<P, false>(x)), no_integer_form<T>())
+NT2H_UN(acos_, dm::acos_f<P>(x), (G == 1 ? dm::acos_d<P, true>(x) : dm::acos_d<P, false>(x)), no_integer_form<T>())
 #undef NT2H_UN
 
 template <> struct apply1<tagfn::cos_> {
@@ -541,7 +544,7 @@ NT2H_FN1(exp2, exp2_) NT2H_FN1(exp10, exp10_) NT2H_FN1(log2, log2_) NT2H_FN1(log
 NT2H_FN1(floor, floor_) NT2H_FN1(ceil, ceil_) NT2H_FN1(trunc, trunc_) NT2H_FN1(round, round_)
 NT2H_FN1(round2even, round2even_) NT2H_FN1(sign, sign_)
 NT2H_FN1(expm1, expm1_) NT2H_FN1(log1p, log1p_) NT2H_FN1(tanh, tanh_) NT2H_FN1(sinh, sinh_)
-NT2H_FN1(cosh, cosh_) NT2H_FN1(atan, atan_) NT2H_FN1(tan, tan_)
+NT2H_FN1(cosh, cosh_) NT2H_FN1(atan, atan_) NT2H_FN1(tan, tan_) NT2H_FN1(asin, asin_) NT2H_FN1(acos, acos_)
 NT2H_FN1(if_zero_else_one, if_zero_else_one_)
 #undef NT2H_FN1
 #define NT2H_FN2(name, Tag) \
@@ -850,13 +853,15 @@ template <class T, class V, bool PF> struct expr_src {
 }}  // namespace hip::fe
 namespace expr {
 // false when a function of the expression decides per pack (NT2's
-// trigonometric range reduction; the f64 erfc regions)
+// trigonometric range reduction) or has a separate scalar form for a pack's
+// tail (atan; the f64 erfc, normcdf, asin and acos)
 template <class X> struct pack_free { static constexpr bool value = true; };
 template <class Tag, class A> struct pack_free<node1<Tag, A>> {
   static constexpr bool value = pack_free<A>::value && !std::is_same<Tag, tagfn::cos_>::value &&
                                 !std::is_same<Tag, tagfn::sin_>::value && !std::is_same<Tag, tagfn::tan_>::value &&
                                 !std::is_same<Tag, tagfn::atan_>::value &&
-                                !((std::is_same<Tag, tagfn::erfc_>::value || std::is_same<Tag, tagfn::normcdf_>::value) &&
+                                !((std::is_same<Tag, tagfn::erfc_>::value || std::is_same<Tag, tagfn::normcdf_>::value ||
+                                   std::is_same<Tag, tagfn::asin_>::value || std::is_same<Tag, tagfn::acos_>::value) &&
                                   std::is_same<typename A::value_type, double>::value);
 };
 template <class Tag, class A, class B> struct pack_free<node2<Tag, A, B>> {

@@ -23,6 +23,7 @@
 #include <cstdint>
 #include <cstring>
 #include <algorithm>
+#include <limits>
 #include <omp.h>
 #include <unistd.h>
 
@@ -587,6 +588,99 @@ inline double atan_d(double a0, bool scalar) {
   }
   return as_d(bits(r) ^ (bits(a0) & 0x8000000000000000ull));
 }
+// asin, acos (f32): simd/common/impl/invtrig/f_invtrig.hpp:55-97; the scalar
+// forms of scalar/impl/invtrig/f_invtrig.hpp:49-90 give the same values (the
+// early return below 1e-4 is what the polynomial rounds to; x > 1 is NaN both
+// ways), so one restatement serves packs and tails.
+inline float asin_f(float a0) {
+  const float c[5] = {F32(0x3d2cb352), F32(0x3cc617e3), F32(0x3d3a3ec7), F32(0x3d9980f6), F32(0x3e2aaae4)};
+  float x = std::fabs(a0);
+  const bool big = x > 0.5f;
+  const float z = big ? 0.5f * (1.0f - x) : x * x;
+  if (big) x = std::sqrt(z);
+  float z1 = horner(c, z);
+  z1 = z1 * (z * x) + x;
+  if (big) z1 = F32(0x3fc90fdb) - (z1 + z1);
+  return as_f(bits(z1) ^ (bits(a0) & 0x80000000u));
+}
+inline float acos_f(float a0) {
+  float x = std::fabs(a0);
+  const bool big = x > 0.5f;
+  x = big ? std::sqrt(-0.5f * x + 0.5f) : a0;
+  x = asin_f(x);
+  if (big) x = x + x;
+  if (a0 < -0.5f) x = F32(0x40490fdb) - x;
+  return big ? x : F32(0x3fc90fdb) - x;
+}
+// asin, acos (f64): simd/common/impl/invtrig/d_invtrig.hpp:57-127 (packs: both
+// branches evaluated, the low part of pi/2 subtracted, acos through
+// pi - 2 asin(sqrt((1-|x|)/2)) below -1/2); scalar/impl/invtrig/d_invtrig.hpp:46-111
+// (pack tail: the low part of pi/2 added at :71, acos below 1/2 as
+// (pi/4 - asin(x) + pi/4lo) + pi/4).
+inline double asin_d(double a0, bool scalar) {
+  const double rr[5] = {F64(0x3f684fc3988e9f08), F64(0xbfe2079259f9290f), F64(0x401bdff5baf33e6a),
+                        F64(0xc03991aaac01ab68), F64(0x403c896240f3081d)};
+  const double ss[5] = {F64(0x3ff0000000000000), F64(0xc035f2a2b6bf5d8c), F64(0x40626219af6a7f42),
+                        F64(0xc077fe08959063ee), F64(0x40756709b0b644be)};
+  const double pp[6] = {F64(0x3f716b9b0bd48ad3), F64(0xbfe34341333e5c16), F64(0x4015c74b178a2dd9),
+                        F64(0xc0304331de27907b), F64(0x40339007da779259), F64(0xc020656c06ceafd5)};
+  const double qq[6] = {F64(0x3ff0000000000000), F64(0xc02d7b590b5e0eab), F64(0x40519fc025fe9054),
+                        F64(0xc06265bb6d3576d7), F64(0x4061705684ffbf9d), F64(0xc04898220a3607ac)};
+  const double pio4 = F64(0x3fe921fb54442d18), pio2lo = F64(0x3c91a62633145c07);
+  const double sqrteps = F64(0x3e50000000000000), ct1 = F64(0x3fe4000000000000);
+  const double x = std::fabs(a0);
+  double r;
+  if (scalar) {
+    if (x > 1.0) return std::numeric_limits<double>::quiet_NaN();
+    if (x < sqrteps) return a0;
+    if (x > ct1) {
+      double zz = 1.0 - x;
+      const double vp = (zz * horner(rr, zz)) / horner(ss, zz);
+      zz = std::sqrt(zz + zz);
+      double z = pio4 - zz;
+      zz = zz * vp + pio2lo;
+      z = z - zz;
+      r = z + pio4;
+    } else {
+      const double zz = x * x;
+      const double z = (zz * horner(pp, zz)) / horner(qq, zz);
+      r = x * z + x;
+    }
+  } else {
+    double zz1 = 1.0 - x;
+    const double vp = (zz1 * horner(rr, zz1)) / horner(ss, zz1);
+    zz1 = std::sqrt(zz1 + zz1);
+    double z = pio4 - zz1;
+    zz1 = zz1 * vp - pio2lo;
+    z = z - zz1;
+    zz1 = z + pio4;
+    const double zz2a = a0 * a0;
+    z = (zz2a * horner(pp, zz2a)) / horner(qq, zz2a);
+    const double zz2 = x * z + x;
+    if (x > 1.0) return std::numeric_limits<double>::quiet_NaN();
+    r = x < sqrteps ? x : (x > ct1 ? zz1 : zz2);
+  }
+  return as_d(bits(r) ^ (bits(a0) & 0x8000000000000000ull));
+}
+inline double acos_d(double a0, bool scalar) {
+  const double pi = F64(0x400921fb54442d18), pio2 = F64(0x3ff921fb54442d18);
+  const double pio4 = F64(0x3fe921fb54442d18), pio4lo = F64(0x3c81a62633145c07);
+  if (scalar) {
+    if (std::fabs(a0) > 1.0) return std::numeric_limits<double>::quiet_NaN();
+    if (a0 > 0.5) return 2.0 * asin_d(std::sqrt(-0.5 * a0 + 0.5), true);
+    double z = pio4 - asin_d(a0, true);
+    z = z + pio4lo;
+    return z + pio4;
+  }
+  double x = std::fabs(a0);
+  const bool big = x > 0.5;
+  x = big ? std::sqrt(-0.5 * x + 0.5) : a0;
+  x = asin_d(x, false);
+  if (big) x = x + x;
+  if (a0 < -0.5) x = pi - x;
+  return big ? x : pio2 - x;
+}
+
 // atan2: simd/common/atan2.hpp:30-54, scalar/atan2.hpp:29-45
 template <class T, class K> inline T atan2_t(T a0, T a1, T pi, K kernel) {
   if (a0 != a0 || a1 != a1) return a0 + a1;
@@ -1313,7 +1407,8 @@ enum {
   OR_FASTNORMCDF = 5, OR_COS = 6, OR_SIN = 7, OR_ABS = 8, OR_EXP2 = 9, OR_EXP10 = 10,
   OR_LOG2 = 11, OR_LOG10 = 12, OR_FLOOR = 13, OR_CEIL = 14, OR_TRUNC = 15, OR_ROUND = 16,
   OR_ROUND2EVEN = 17, OR_SIGN = 18, OR_EXPM1 = 19, OR_LOG1P = 20, OR_TANH = 21, OR_SINH = 22,
-  OR_COSH = 23, OR_ATAN = 24, OR_TAN = 25, OR_LAST = OR_TAN
+  OR_COSH = 23, OR_ATAN = 24, OR_TAN = 25, OR_ASIN = 26, OR_ACOS = 27,
+  OR_LAST = OR_ACOS
 };
 
 int oracle_abi_version(void) { return 1; }
@@ -1357,6 +1452,8 @@ int oracle_map_f32(int op, const float* x, float* y, size_t n) {
       case OR_SINH: for (size_t l = 0; l < w; ++l) y[i + l] = sinh_f(x[i + l]); break;
       case OR_COSH: for (size_t l = 0; l < w; ++l) y[i + l] = cosh_f(x[i + l]); break;
       case OR_ATAN: for (size_t l = 0; l < w; ++l) y[i + l] = atan_f(x[i + l], !pack); break;
+      case OR_ASIN: for (size_t l = 0; l < w; ++l) y[i + l] = asin_f(x[i + l]); break;
+      case OR_ACOS: for (size_t l = 0; l < w; ++l) y[i + l] = acos_f(x[i + l]); break;
       case OR_COS:
       case OR_SIN:
       case OR_TAN:
@@ -1395,6 +1492,8 @@ int oracle_map_f64(int op, const double* x, double* y, size_t n) {
       case OR_SINH: for (size_t l = 0; l < w; ++l) y[i + l] = sinh_d(x[i + l]); break;
       case OR_COSH: for (size_t l = 0; l < w; ++l) y[i + l] = cosh_d(x[i + l]); break;
       case OR_ATAN: for (size_t l = 0; l < w; ++l) y[i + l] = atan_d(x[i + l], !pack); break;
+      case OR_ASIN: for (size_t l = 0; l < w; ++l) y[i + l] = asin_d(x[i + l], !pack); break;
+      case OR_ACOS: for (size_t l = 0; l < w; ++l) y[i + l] = acos_d(x[i + l], !pack); break;
       case OR_COS:
       case OR_SIN:
       case OR_TAN:

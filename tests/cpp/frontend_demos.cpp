@@ -207,6 +207,8 @@ int main(int argc, char** argv) {
   dump("atan_f32", at2);
   at2 = nt2::tan(at) * 0.5f - nt2::tan(bt * 100.0f);
   dump("tan_f32", at2);
+  at2 = nt2::asin(at / 3.0f) * nt2::acos(bt / 10.0f);
+  dump("asin_f32", at2);
   table<double> dm(of_size(mh * mw)), dr;
   for (std::size_t i = 1; i <= mh * mw; ++i) dm(i) = (double)M(i) / 100.0;
   dr = nt2::trunc(dm) * nt2::exp2(nt2::ceil(dm / 50.0)) + nt2::sign(dm) * nt2::log10(nt2::abs(dm) + 1.0);
@@ -215,6 +217,11 @@ int main(int argc, char** argv) {
   dump("ew2_f64", dr);
   dr = nt2::tan(dm / 7.0) + nt2::atan(dm);
   dump("tan_f64", dr);
+  table<double> dq(of_size(1001)), dq2;  // a ragged tail of NT2's scalar forms
+  for (std::size_t i = 1; i <= 1001; ++i) dq(i) = dm(i) / 100.0;
+  dq2 = nt2::asin(dq) - nt2::acos(dq * 0.9);
+  dump("dq", dq);
+  dump("asin_f64", dq2);
 
   // ---- rgb2grey and rgb2yuv on a 96 x 131 image: the demos' ramp and random planes
   const std::size_t ih = 96, iw = 131;

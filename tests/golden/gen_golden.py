@@ -71,6 +71,8 @@ FUNCS = {
     "sinh": lambda x: mpmath.sinh(x),
     "cosh": lambda x: mpmath.cosh(x),
     "tan": lambda x: mpmath.tan(x),
+    "asin": lambda x: mpmath.asin(x) if abs(x) <= 1 else mpmath.nan,
+    "acos": lambda x: mpmath.acos(x) if abs(x) <= 1 else mpmath.nan,
 }
 
 
@@ -123,6 +125,11 @@ def inputs(name, dtype, rng):
         pts += list(rng.uniform(-20 * np.pi, 20 * np.pi, 300)) + list(rng.uniform(-1e4, 1e4, 300))
         pts += list(np.exp2(rng.uniform(20, 100, 200)) * rng.choice([-1.0, 1.0], 200))
         pts += [0.0, 1.0, -1.0, 0.5, np.pi / 4, 1.5, -1.5, 100.0, 1e5]
+    elif name in ("asin", "acos"):
+        pts += list(rng.uniform(-1, 1, 800)) + list(rng.uniform(-0.7, 0.7, 300))
+        pts += list((1 - np.exp2(rng.uniform(-50, -1, 200))) * rng.choice([-1.0, 1.0], 200))
+        pts += list(np.exp2(rng.uniform(-60, -2, 200)) * rng.choice([-1.0, 1.0], 200))
+        pts += [0.0, 0.5, -0.5, 0.625, -0.625, 1.0, -1.0, 0.9999, 1e-4, 1.5e-8, 0.25, 0.75]
     else:  # cos, sin
         pts += list(rng.uniform(-np.pi / 4, np.pi / 4, 150))
         pts += list(rng.uniform(-np.pi / 2, np.pi / 2, 150))
@@ -284,6 +291,23 @@ KATS = [
     ("tan", "pi", "0", 0.75, "modules/core/trigonometric/unit/simd/tan.cpp:51"),
     ("tan", "pi/4", "1", 1.0, "modules/core/trigonometric/unit/simd/tan.cpp:52"),
     ("tan", "0", "0", 0.5, "modules/core/trigonometric/unit/simd/tan.cpp:53"),
+    # modules/core/trigonometric/unit/simd/asin.cpp:49-57 and acos.cpp:49-57
+    ("asin", "inf", "nan", 0.5, "modules/core/trigonometric/unit/simd/asin.cpp:49"),
+    ("asin", "-inf", "nan", 0.5, "modules/core/trigonometric/unit/simd/asin.cpp:50"),
+    ("asin", "nan", "nan", 0.5, "modules/core/trigonometric/unit/simd/asin.cpp:51"),
+    ("asin", "0.5", "pi/6", 0.5, "modules/core/trigonometric/unit/simd/asin.cpp:53"),
+    ("asin", "-0.5", "-pi/6", 0.5, "modules/core/trigonometric/unit/simd/asin.cpp:54"),
+    ("asin", "-1", "-pi/2", 0.5, "modules/core/trigonometric/unit/simd/asin.cpp:55"),
+    ("asin", "1", "pi/2", 0.5, "modules/core/trigonometric/unit/simd/asin.cpp:56"),
+    ("asin", "0", "0", 0.5, "modules/core/trigonometric/unit/simd/asin.cpp:57"),
+    ("acos", "inf", "nan", 0.0, "modules/core/trigonometric/unit/simd/acos.cpp:49"),
+    ("acos", "-inf", "nan", 0.0, "modules/core/trigonometric/unit/simd/acos.cpp:50"),
+    ("acos", "nan", "nan", 0.0, "modules/core/trigonometric/unit/simd/acos.cpp:51"),
+    ("acos", "0.5", "pi/3", 0.5, "modules/core/trigonometric/unit/simd/acos.cpp:53"),
+    ("acos", "-0.5", "2pi/3", 0.5, "modules/core/trigonometric/unit/simd/acos.cpp:54"),
+    ("acos", "-1", "pi", 0.0, "modules/core/trigonometric/unit/simd/acos.cpp:55"),
+    ("acos", "1", "0", 0.0, "modules/core/trigonometric/unit/simd/acos.cpp:56"),
+    ("acos", "0", "pi/2", 0.0, "modules/core/trigonometric/unit/simd/acos.cpp:57"),
 ]
 
 

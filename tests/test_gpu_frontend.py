@@ -139,6 +139,9 @@ def test_exponential_rounding_statements(outputs):
     want = O.map_f32("tan", a) * f32(0.5) - O.map_f32("tan", b * f32(100))
     got = outputs("tan_f32", f32)
     assert same_bits(got, want), first_diff(got, want)
+    want = O.map_f32("asin", a / f32(3)) * O.map_f32("acos", b / f32(10))
+    got = outputs("asin_f32", f32)
+    assert same_bits(got, want), first_diff(got, want)
     d = outputs("M", f32).astype(np.float64) / 100.0
     want = (O.map_f64("trunc", d) * O.map_f64("exp2", O.map_f64("ceil", d / 50.0)) +
             O.map_f64("sign", d) * O.map_f64("log10", np.abs(d) + 1.0))
@@ -149,6 +152,11 @@ def test_exponential_rounding_statements(outputs):
     assert same_bits(got, want), first_diff(got, want)
     want = O.map_f64("tan", d / 7.0) + O.map_f64("atan", d)
     got = outputs("tan_f64", np.float64)
+    assert same_bits(got, want), first_diff(got, want)
+    q = outputs("dq", np.float64)
+    assert q.size == 1001
+    want = O.map_f64("asin", q) - O.map_f64("acos", q * 0.9)
+    got = outputs("asin_f64", np.float64)
     assert same_bits(got, want), first_diff(got, want)
 
 

@@ -18,7 +18,8 @@ G = os.path.join(os.path.dirname(__file__), "golden")
 def _val(s, dt):
     table = {"inf": np.inf, "-inf": -np.inf, "nan": np.nan, "e": np.e, "1/e": 1 / np.e,
              "pi": np.pi, "-pi": -np.pi, "pi/2": np.pi / 2, "-pi/2": -np.pi / 2,
-             "pi/4": np.pi / 4, "-pi/4": -np.pi / 4, "sqrt(2)/2": np.sqrt(2) / 2, "-0": -0.0,
+             "pi/4": np.pi / 4, "-pi/4": -np.pi / 4, "pi/6": np.pi / 6, "-pi/6": -np.pi / 6,
+             "pi/3": np.pi / 3, "2pi/3": 2 * np.pi / 3, "sqrt(2)/2": np.sqrt(2) / 2, "-0": -0.0,
              "ln2": np.log(2.0), "eps": np.finfo(dt).eps, "tiny": np.finfo(dt).tiny}
     if s == "e-1":  # NT2 forms these in the target precision: Exp_1<T>() - One<T>()
         return dt(np.e) - dt(1)
@@ -53,7 +54,7 @@ def _bound(name, x, y, dt):
     ax = np.abs(x.astype(np.float64))
     if name in ("exp", "log", "exp2", "exp10", "log2"):
         return np.full(x.shape, 1.0)
-    if name in ("expm1", "log1p", "tanh", "sinh", "cosh"):
+    if name in ("expm1", "log1p", "tanh", "sinh", "cosh", "asin", "acos"):
         return np.full(x.shape, BOUND3[name][dt])
     if name == "tan":
         # 2 ulp in f32 / 1 in f64 away from the poles; the -1/y branch of an
@@ -87,7 +88,7 @@ def _bound(name, x, y, dt):
 
 @pytest.mark.parametrize("dt,tag", [(np.float32, "f32"), (np.float64, "f64")])
 @pytest.mark.parametrize("name", ["exp", "log", "erfc", "normcdf", "cos", "sin", "exp2", "exp10", "log2", "log10",
-                                  "expm1", "log1p", "tanh", "sinh", "cosh", "tan"])
+                                  "expm1", "log1p", "tanh", "sinh", "cosh", "tan", "asin", "acos"])
 def test_golden_mpmath(dt, tag, name):
     g = np.load(os.path.join(G, "math_%s.npz" % tag))
     x, y = g[name + "_x"], g[name + "_y"]
@@ -130,7 +131,10 @@ def test_golden_mpmath(dt, tag, name):
 # the constructions round several times)
 BOUND3 = {"expm1": {np.float32: 2.0, np.float64: 1.0}, "log1p": {np.float32: 1.0, np.float64: 1.0},
           "tanh": {np.float32: 1.0, np.float64: 1.0}, "sinh": {np.float32: 2.0, np.float64: 1.0},
-          "cosh": {np.float32: 1.0, np.float64: 1.0}}
+          "cosh": {np.float32: 1.0, np.float64: 1.0},
+          # asin f32: pi/2 - 2 asin(sqrt((1-x)/2)) above 1/2 rounds pi/2 and the
+          # doubling; NT2's unit test allows 0.5 ulpdist at the special values only
+          "asin": {np.float32: 2.0, np.float64: 1.0}, "acos": {np.float32: 1.0, np.float64: 1.0}}
 
 MINLOG = {"exp2": {np.float32: np.float32(-127.0), np.float64: -1022.0},
           "exp10": {np.float32: np.uint32(0xc2179999).view(np.float32),
@@ -449,3 +453,52 @@ def test_tan_structure(dt):
     fin = np.isfinite(x[:8])
     assert np.all(classic_ulp(t[:8][fin], np.tan(x[:8][fin].astype(np.float64)), dt) <= 2.0)
     assert t[8] == 0.0 and np.signbit(t[8])
+
+
+def test_acos_f32_matches_nt2_published_histogram():
+    # simd/common/impl/invtrig/f_invtrig.hpp:85-89: over every float in [-1, 1]
+    # (2130706432 values) 92.38% exact, 7.62% within 0.5 NT2 ulp (one ulp)
+    rng = np.random.default_rng(17)
+    n = 600000
+    x = np.concatenate([_uniform_bits(0.0, 1.0, n, rng), -_uniform_bits(0.0, 1.0, n, rng)])
+    got = O.map_f32("acos", x)
+    ref = np.arccos(x.astype(np.float64)).astype(np.float32)
+    exact = np.mean(got == ref)
+    assert abs(exact - 0.9238) < 0.003, exact
+    assert np.all(classic_ulp(got, ref.astype(np.float64), np.float32) <= 1.0)
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_asin_acos_structure(dt):
+    """asin is odd bit for bit; asin(x) + acos(x) = pi/2 within an ulp of
+    pi/2; acos(-x) = pi - acos(x) within rounding; |x| > 1 gives NaN.  In f64
+    the scalar form that NT2 uses for a pack's tail adds the low part of pi/2
+    where the pack form subtracts it (scalar/impl/invtrig/d_invtrig.hpp:71 and
+    simd/common/impl/invtrig/d_invtrig.hpp:80): above 0.625 the tail's asin is
+    never above the pack's, is below it on a large share of inputs, and is at
+    most two ulp away (2 pi/2lo = 1.22e-16 after the roundings); at or below 0.625 the two forms agree bit for bit."""
+    mapf = O.map_f32 if dt is np.float32 else O.map_f64
+    W = 8 if dt is np.float32 else 4
+    rng = np.random.default_rng(41)
+    x = rng.uniform(-1, 1, 20000).astype(dt)
+    a, am = _iso(mapf, "asin", x, W), _iso(mapf, "asin", -x, W)
+    assert np.array_equal(a, -am)
+    c, cm = _iso(mapf, "acos", x, W), _iso(mapf, "acos", -x, W)
+    eps = np.finfo(dt).eps
+    s = a.astype(np.float64) + c.astype(np.float64)
+    assert np.all(np.abs(s - np.pi / 2) <= 2 * eps)
+    assert np.all(np.abs(cm.astype(np.float64) - (np.pi - c.astype(np.float64))) <= 4 * eps)
+    bad = np.array([1.0000001, -1.5, 2.0, np.inf, -np.inf, np.nan], dtype=dt)
+    assert np.all(np.isnan(_iso(mapf, "asin", bad, W))) and np.all(np.isnan(_iso(mapf, "acos", bad, W)))
+    if dt is np.float64:
+        hi = rng.uniform(0.625, 1.0, 20000)
+        lo = rng.uniform(-0.625, 0.625, 20000)
+        tail = np.array([mapf("asin", np.array([v]))[0] for v in hi])
+        pack = _iso(mapf, "asin", hi, W)
+        assert np.all(tail <= pack) and np.mean(tail < pack) > 0.2
+        assert np.all(classic_ulp(tail, pack, dt) <= 2.0)
+        tl = np.array([mapf("asin", np.array([v]))[0] for v in lo[:3000]])
+        assert np.array_equal(tl, _iso(mapf, "asin", lo[:3000], W))
+        # the tail's acos: (pi/4 - asin(x) + pi/4lo) + pi/4, within an ulp of the pack's
+        tc = np.array([mapf("acos", np.array([v]))[0] for v in lo[:3000]])
+        assert np.all(classic_ulp(tc, _iso(mapf, "acos", lo[:3000], W), dt) <= 1.0)
