@@ -117,8 +117,12 @@ int nt2h_map_f64(int op, size_t n, const double* x, double* y, int policy, void*
 /* out = f(x, y) elementwise for the binary functions below, NT2's pack
  * schedule as nt2h_map_*: replaces transform_ over a binary node such as
  * atan2(a, b) (modules/core/trigonometric/include/nt2/trigonometric/functions/
- * simd/common/atan2.hpp:30-54, scalar/atan2.hpp:29-45 for the pack tail). */
+ * simd/common/atan2.hpp:30-54, scalar/atan2.hpp:29-45 for the pack tail) or
+ * pow(a, b) (modules/core/exponential/include/nt2/exponential/functions/
+ * simd/common/pow.hpp:38-46, pow_abs.hpp:65-133; scalar/pow.hpp:30-36,
+ * pow_abs.hpp:51-111 for the pack tail). */
 #define NT2H_OP2_ATAN2 0
+#define NT2H_OP2_POW 1
 int nt2h_map2_f32(int op, size_t n, const float* x, const float* y, float* out, int policy, void* stream);
 int nt2h_map2_f64(int op, size_t n, const double* x, const double* y, double* out, int policy, void* stream);
 /* [s, c] = sincos(x): modules/core/trigonometric/include/nt2/trigonometric/functions/generic/sincos.hpp:50-64 */

@@ -287,7 +287,8 @@ __global__ __launch_bounds__(kBlock) void map_f64_k(const double* __restrict__ x
 template <class P, int OP, class T, int G>
 __device__ __forceinline__ T apply2_one(T a, T b) {
   constexpr bool scalar = (G == 1);
-  if constexpr (sizeof(T) == 4) return dm::atan2_f<P, scalar>(a, b);
+  if constexpr (OP == NT2H_OP2_POW) return dm::pow_t<P, scalar, T>(a, b);
+  else if constexpr (sizeof(T) == 4) return dm::atan2_f<P, scalar>(a, b);
   else return dm::atan2_d<P, scalar>(a, b);
 }
 template <class P, int OP, class T, int W>
@@ -1000,14 +1001,17 @@ int launch_map2(const char* name, int op, size_t n, const T* x, const T* y, T* o
                        void* stream) {
   if (n == 0) return NT2H_OK;
   if (!x || !y || !out) return fail(NT2H_EINVAL, name);
-  if (op != NT2H_OP2_ATAN2) return fail(NT2H_EINVAL, name);
+  if (op != NT2H_OP2_ATAN2 && op != NT2H_OP2_POW) return fail(NT2H_EINVAL, name);
+  if (policy != NT2H_EXACT && policy != NT2H_FAST) return fail(NT2H_EINVAL, name);
   const dim3 g(grid_for(n / W + n % W)), b(kBlock);
-  if (policy == NT2H_EXACT)
-    hipLaunchKernelGGL((map2_k<exact_policy, NT2H_OP2_ATAN2, T, W>), g, b, 0, as_stream(stream), x, y, out, n);
-  else if (policy == NT2H_FAST)
-    hipLaunchKernelGGL((map2_k<fast_policy, NT2H_OP2_ATAN2, T, W>), g, b, 0, as_stream(stream), x, y, out, n);
-  else
-    return fail(NT2H_EINVAL, name);
+  hipStream_t s = as_stream(stream);
+  if (op == NT2H_OP2_ATAN2) {
+    if (policy == NT2H_EXACT) hipLaunchKernelGGL((map2_k<exact_policy, NT2H_OP2_ATAN2, T, W>), g, b, 0, s, x, y, out, n);
+    else hipLaunchKernelGGL((map2_k<fast_policy, NT2H_OP2_ATAN2, T, W>), g, b, 0, s, x, y, out, n);
+  } else {
+    if (policy == NT2H_EXACT) hipLaunchKernelGGL((map2_k<exact_policy, NT2H_OP2_POW, T, W>), g, b, 0, s, x, y, out, n);
+    else hipLaunchKernelGGL((map2_k<fast_policy, NT2H_OP2_POW, T, W>), g, b, 0, s, x, y, out, n);
+  }
   NT2H_CHECK_LAUNCH(name);
   return NT2H_OK;
 }

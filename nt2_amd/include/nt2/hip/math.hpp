@@ -674,6 +674,200 @@ NT2H_FN T atan2_t(T a0, T a1) {
 template <class P, bool Scalar> NT2H_FN float atan2_f(float a0, float a1) { return atan2_t<P, Scalar, float>(a0, a1); }
 template <class P, bool Scalar> NT2H_FN double atan2_d(double a0, double a1) { return atan2_t<P, Scalar, double>(a0, a1); }
 
+// pow(a0, a1): modules/core/exponential/include/nt2/exponential/functions/
+// simd/common/pow.hpp:38-46, pow_abs.hpp:65-133 (packs), scalar/pow.hpp:30-36,
+// pow_abs.hpp:51-111 (pack tail), details/pow_kernel.hpp:30-228: Cephes' pow.
+// log2 of the significand against 2^(-i/16) (a 17-entry table with a 9-entry
+// continuation), the product with a1 split into sixteenths, 2^fraction by a
+// polynomial, ldexp as boost.simd does it (two products).  Differences of the
+// forms: at a1 = +Inf the tail gives Inf for a NaN base (packs 0), at an
+// infinite base the tail gives Inf for a NaN exponent (packs 1).
+template <class T> struct powk;
+template <> struct powk<float> {
+  using I = int;
+  using U = unsigned;
+  static constexpr int nbm = 23, maxe = 127, mine = -126, lime = 128;
+  static constexpr unsigned A[17] = {0x3f800000, 0x3f75257d, 0x3f6ac0c7, 0x3f60ccdf, 0x3f5744fd, 0x3f4e248c,
+                                     0x3f45672a, 0x3f3d08a4, 0x3f3504f3, 0x3f2d583f, 0x3f25fed7, 0x3f1ef532,
+                                     0x3f1837f0, 0x3f11c3d3, 0x3f0b95c2, 0x3f05aac3, 0x3f000000};
+  static constexpr unsigned B[9] = {0x00000000, 0xb1c116de, 0xb254a58a, 0x318aa837, 0x324fe77a,
+                                    0xb2ac9d5e, 0x32a31b71, 0xb1e0aba1, 0x00000000};
+  NT2H_FN static float a(int i) { return as_f(A[i]); }
+  NT2H_FN static float b(int i) { return as_f(B[i]); }
+  template <class P> NT2H_FN static float pow1(float x, float z) {
+    NT2H_STRICT
+    constexpr float c[4] = {NT2H_F(0xbe2a61b2), NT2H_F(0x3e4d2fa3), NT2H_F(0xbe800015), NT2H_F(0x3eaaaaa3)};
+    return (z * x) * horner<P>(c, x);
+  }
+  template <class P> NT2H_FN static float pow2(float x) {
+    constexpr float c[4] = {NT2H_F(0x3c1a49bc), NT2H_F(0x3d634d38), NT2H_F(0x3e75fde1), NT2H_F(0x3f317218)};
+    return horner<P>(c, x);
+  }
+  NT2H_FN static float log2em1() { return NT2H_F(0x3ee2a8ed); }
+  NT2H_FN static float large() { return NT2H_F(0x44ffe000); }  // 2047 sixteenths
+  NT2H_FN static float low() { return NT2H_F(0xc5160000); }    // -2400 sixteenths
+  NT2H_FN static float tiny() { return NT2H_F(0x00800000); }
+  NT2H_FN static float from_bits(U u) { return as_f(u); }
+  NT2H_FN static I toint(float x) { return (x > -2147483648.0f && x < 2147483648.0f) ? (I)x : (I)0x80000000u; }
+  NT2H_FN static float frexp(float x, I& e) { return __builtin_frexpf(x, &e); }
+};
+template <> struct powk<double> {
+  using I = long long;
+  using U = unsigned long long;
+  static constexpr int nbm = 52, maxe = 1023, mine = -1022, lime = 1024;
+  static constexpr unsigned long long A[17] = {
+      0x3ff0000000000000, 0x3feea4afa2a490da, 0x3fed5818dcfba487, 0x3fec199bdd85529c, 0x3feae89f995ad3ad,
+      0x3fe9c49182a3f090, 0x3fe8ace5422aa0db, 0x3fe7a11473eb0187, 0x3fe6a09e667f3bcd, 0x3fe5ab07dd485429,
+      0x3fe4bfdad5362a27, 0x3fe3dea64c123422, 0x3fe306fe0a31b715, 0x3fe2387a6e756238, 0x3fe172b83c7d517b,
+      0x3fe0b5586cf9890f, 0x3fe0000000000000};
+  static constexpr unsigned long long B[9] = {0x0000000000000000, 0x3c72ed02d75b3707, 0x3c87a1cd345dcc81,
+                                              0x3c86e9f156864b27, 0xbc8bdd3413b26456, 0x3c6d4397afec42e2,
+                                              0x3c76f46ad23182e4, 0xbc719041b9d78a76, 0x0000000000000000};
+  NT2H_FN static double a(int i) { return as_d(A[i]); }
+  NT2H_FN static double b(int i) { return as_d(B[i]); }
+  template <class P> NT2H_FN static double pow1(double x, double z) {
+    NT2H_STRICT
+    constexpr double p[4] = {NT2H_D(0x3fdfdb997f5b5cf0), NT2H_D(0x400dddefea9edf15), NT2H_D(0x401eccbd7f78eb6f),
+                             NT2H_D(0x4012aa83b65c9b74)};
+    constexpr double q[5] = {NT2H_D(0x3ff0000000000000), NT2H_D(0x4022aab49b20914e), NT2H_D(0x403bffff41c1c9f5),
+                             NT2H_D(0x4040ccbc1b176402), NT2H_D(0x402bffc5918ae92e)};
+    return x * ((z * horner<P>(p, x)) / horner<P>(q, x));
+  }
+  template <class P> NT2H_FN static double pow2(double x) {
+    constexpr double c[7] = {NT2H_D(0x3eef6307d7f2937f), NT2H_D(0x3f242fbe60fc9259), NT2H_D(0x3f55d87ec84aef1d),
+                             NT2H_D(0x3f83b2ab6ef133b7), NT2H_D(0x3fac6b08d7041a92), NT2H_D(0x3fcebfbdff82c56d),
+                             NT2H_D(0x3fe62e42fefa39ef)};
+    return horner<P>(c, x);
+  }
+  NT2H_FN static double log2em1() { return NT2H_D(0x3fdc551d94ae0bf8); }
+  NT2H_FN static double large() { return NT2H_D(0x40cfff8000000000); }  // 16383 sixteenths
+  NT2H_FN static double low() { return NT2H_D(0xc0d0c7c000000000); }    // -17183 sixteenths
+  NT2H_FN static double tiny() { return NT2H_D(0x0010000000000000); }
+  NT2H_FN static double from_bits(U u) { return as_d(u); }
+  NT2H_FN static I toint(double x) {
+    return (x > -9223372036854775808.0 && x < 9223372036854775808.0) ? (I)x : (I)0x8000000000000000ull;
+  }
+  NT2H_FN static double frexp(double x, I& e) {
+    int ei;
+    const double m = __builtin_frexp(x, &ei);
+    e = ei;
+    return m;
+  }
+};
+NT2H_FN float pw_floor(float x) { return __builtin_floorf(x); }
+NT2H_FN double pw_floor(double x) { return __builtin_floor(x); }
+NT2H_FN float pw_trunc(float x) { return __builtin_truncf(x); }
+NT2H_FN double pw_trunc(double x) { return __builtin_trunc(x); }
+NT2H_FN float pw_abs(float x) { return __builtin_fabsf(x); }
+NT2H_FN double pw_abs(double x) { return __builtin_fabs(x); }
+template <class T> NT2H_FN T pw_reduc(T x) { return T(0.0625) * pw_floor(T(16) * x); }
+template <class T> NT2H_FN bool pw_flint(T x) { return __builtin_isfinite(x) && (x - pw_trunc(x)) == T(0); }
+template <class T> NT2H_FN bool pw_odd(T x) {
+  NT2H_STRICT
+  return pw_flint((x - T(1)) * T(0.5));
+}
+// boost.simd's ldexp: 2^e built in the exponent field, below Minexponent by
+// a second product with 2^Minexponent, at Limitexponent by a second product with 2
+template <class T> NT2H_FN T pw_ldexp(T a, typename powk<T>::I e) {
+  NT2H_STRICT
+  using K = powk<T>;
+  using U = typename K::U;
+  T f = T(1);
+  if (e < K::mine) { e -= K::mine; f = K::tiny(); }
+  if (e == K::lime) { f = f + T(1); e -= 1; }
+  return (a * K::from_bits((U)(e + K::maxe) << K::nbm)) * f;
+}
+// the arithmetic shared by both forms; w receives the result's exponent in sixteenths
+template <class P, class T> NT2H_FN T pow_core(T ax, T a1, T& wout) {
+  NT2H_STRICT
+  using K = powk<T>;
+  using I = typename K::I;
+  I e = 0;
+  T x = ax;
+  if (ax != T(0) && __builtin_isfinite(ax)) x = K::frexp(ax, e);
+  else x = ax + T(0.5);  // boost.simd's frexp for Inf and NaN (0 is replaced below)
+  int i = 1;
+  if (x <= K::a(9)) i = 9;
+  if (x <= K::a(i + 4)) i += 4;
+  if (x <= K::a(i + 2)) i += 2;
+  if (x >= K::a(1)) i = -1;
+  i = i + 1;
+  const T tmp = K::a(i);
+  x = x - tmp;
+  x = x - K::b(i >> 1);
+  x = x / tmp;
+  T z = x * x;
+  T w = K::template pow1<P>(x, z);
+  w = fma<P>(T(-0.5), z, w);
+  w = fma<P>(K::log2em1(), w, w);
+  z = fma<P>(K::log2em1(), x, w);
+  z = z + x;
+  w = fma<P>((T)(-i), T(0.0625), (T)e);
+  const T ya = pw_reduc(a1);
+  const T yb = a1 - ya;
+  T W = fma<P>(z, a1, w * yb);
+  T Wa = pw_reduc(W);
+  T Wb = W - Wa;
+  W = fma<P>(w, ya, Wa);
+  Wa = pw_reduc(W);
+  const T u = W - Wa;
+  W = Wb + u;
+  Wb = pw_reduc(W);
+  w = T(16) * (Wa + Wb);
+  wout = w;
+  e = K::toint(w);
+  Wb = W - Wb;
+  if (Wb > T(0)) { e = e + 1; Wb = Wb - T(0.0625); }
+  z = K::template pow2<P>(Wb) * Wb;
+  const I ii = e / 16 + (e > 0 ? 1 : 0);
+  e = ii * 16 - e;
+  // e lies in [0, 16] wherever the result is kept; elsewhere read entry 0
+  const T wt = K::a((e >= 0 && e <= 16) ? (int)e : 0);
+  z = fma<P>(wt, z, wt);
+  return pw_ldexp(z, ii);
+}
+template <class P, bool Scalar, class T> NT2H_FN T pow_t(T a0, T a1) {
+  NT2H_STRICT
+  using K = powk<T>;
+  const T inf = (T)__builtin_inf(), nan = (T)__builtin_nan("");
+  const T x = pw_abs(a0);
+  const bool ltz = a0 < T(0);
+  T z;
+  if constexpr (Scalar) {
+    if (ltz && !pw_flint(a1)) return nan;
+    if (x == T(1)) z = x;
+    else if (x == T(0)) z = (a1 == T(0)) ? T(1) : T(0);
+    else if (x == a1 && a1 == inf) z = inf;
+    else if (x == inf && a1 == -inf) z = T(0);
+    else if (a1 == inf) z = (x < T(1)) ? T(0) : inf;
+    else if (a1 == -inf) z = (x > T(1)) ? T(0) : inf;
+    else if (x == inf) z = (a1 < T(0)) ? T(0) : ((a1 == T(0)) ? T(1) : inf);
+    else if (a0 != a0 || a1 != a1) z = nan;
+    else {
+      T w;
+      z = pow_core<P>(x, a1, w);
+      if (w > K::large()) z = inf;
+      else if (w < K::low()) z = T(0);
+    }
+  } else {
+    T w;
+    z = pow_core<P>(x, a1, w);
+    const bool gtax1 = x > T(1);
+    if (a1 == inf) z = gtax1 ? inf : T(0);
+    if (a1 == -inf) z = gtax1 ? T(0) : inf;
+    if (x == inf) z = (a1 > T(0)) ? inf : ((a1 < T(0)) ? T(0) : T(1));
+    if (w < K::low()) z = T(0);
+    if (w > K::large()) z = inf;
+    if (x == T(1)) z = x;
+    if (a0 == T(0)) z = (a1 == T(0)) ? T(1) : T(0);
+  }
+  if (ltz && pw_odd(a1)) z = -z;
+  if (ltz && !pw_flint(a1)) z = nan;
+  return z;
+}
+template <class P, bool Scalar> NT2H_FN float pow_f(float a0, float a1) { return pow_t<P, Scalar, float>(a0, a1); }
+template <class P, bool Scalar> NT2H_FN double pow_d(double a0, double a1) { return pow_t<P, Scalar, double>(a0, a1); }
+
 // asin, acos (f32): simd/common/impl/invtrig/f_invtrig.hpp:55-97.  |x| up to
 // 1/2 by x + x^3 p(x^2); above by pi/2 - 2 asin(sqrt((1-|x|)/2)); acos from
 // asin of sqrt((1-|x|)/2) or of x.  NT2's scalar forms

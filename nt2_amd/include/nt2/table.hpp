@@ -196,6 +196,7 @@ NT2H_TAG(floor_, same1) NT2H_TAG(ceil_, same1) NT2H_TAG(trunc_, same1) NT2H_TAG(
 NT2H_TAG(round2even_, same1) NT2H_TAG(sign_, same1) NT2H_TAG(expm1_, same1) NT2H_TAG(log1p_, same1)
 NT2H_TAG(tanh_, same1) NT2H_TAG(sinh_, same1) NT2H_TAG(cosh_, same1) NT2H_TAG(atan_, same1)
 NT2H_TAG(atan2_, same2) NT2H_TAG(tan_, same1) NT2H_TAG(asin_, same1) NT2H_TAG(acos_, same1)
+NT2H_TAG(pow_, same2)
 NT2H_TAG(fma_, same3) NT2H_TAG(fnms_, same3) NT2H_TAG(if_else_, sel3)
 NT2H_TAG(selinc_, selinc2) NT2H_TAG(seldec_, selinc2) NT2H_TAG(if_zero_else_one_, izeo1)
 #undef NT2H_TAG
@@ -552,6 +553,13 @@ NT2H_FN1(if_zero_else_one, if_zero_else_one_)
 NT2H_FN2(max, max_) NT2H_FN2(min, min_) NT2H_FN2(adds, adds_) NT2H_FN2(subs, subs_)
 NT2H_FN2(selinc, selinc_) NT2H_FN2(seldec, seldec_) NT2H_FN2(atan2, atan2_)
 #undef NT2H_FN2
+// pow(a, b) with floating-point operands (NT2's Cephes pow).  NT2 dispatches an
+// integral exponent to repeated squaring (simd/common/pow.hpp:49-72), which this
+// front end does not provide: such a call does not compile.
+template <class A, class B, NT2H_ENABLE2(A, B)> auto pow(const A& a, const B& b) {
+  static_assert(!std::is_integral<B>::value, "nt2::pow with an integral exponent (repeated squaring) is not provided");
+  return make2<tagfn::pow_>(a, b);
+}
 // fma(a,b,c) = a*b+c and fnms(a,b,c) = c-a*b, NT2's two-rounding forms
 template <class A, class B, class C,
           typename std::enable_if<expr::is_dsl<A>::value || expr::is_dsl<B>::value || expr::is_dsl<C>::value, int>::type = 0>
@@ -648,7 +656,16 @@ __device__ __forceinline__ void ev(const n2v<Tag, VA, VB>& v, size_t i, R (&r)[G
   TB b[G];
   ev<G>(v.a, i, a);
   ev<G>(v.b, i, b);
-  if constexpr (std::is_same<Tag, tagfn::atan2_>::value) {
+  if constexpr (std::is_same<Tag, tagfn::pow_>::value) {
+    static_assert(std::is_floating_point<R>::value, "pow is defined for floating-point tables only");
+#pragma unroll
+    for (int l = 0; l < G; ++l) {
+      if constexpr (std::is_same<R, float>::value)
+        r[l] = G == 1 ? dm::pow_f<P, true>((float)a[l], (float)b[l]) : dm::pow_f<P, false>((float)a[l], (float)b[l]);
+      else
+        r[l] = G == 1 ? dm::pow_d<P, true>((double)a[l], (double)b[l]) : dm::pow_d<P, false>((double)a[l], (double)b[l]);
+    }
+  } else if constexpr (std::is_same<Tag, tagfn::atan2_>::value) {
     static_assert(std::is_floating_point<R>::value, "atan2 is defined for floating-point tables only");
 #pragma unroll
     for (int l = 0; l < G; ++l) {
@@ -865,7 +882,8 @@ template <class Tag, class A> struct pack_free<node1<Tag, A>> {
                                   std::is_same<typename A::value_type, double>::value);
 };
 template <class Tag, class A, class B> struct pack_free<node2<Tag, A, B>> {
-  static constexpr bool value = pack_free<A>::value && pack_free<B>::value && !std::is_same<Tag, tagfn::atan2_>::value;
+  static constexpr bool value = pack_free<A>::value && pack_free<B>::value && !std::is_same<Tag, tagfn::atan2_>::value &&
+                                !std::is_same<Tag, tagfn::pow_>::value;
 };
 template <class Tag, class A, class B, class C> struct pack_free<node3<Tag, A, B, C>> {
   static constexpr bool value = pack_free<A>::value && pack_free<B>::value && pack_free<C>::value;

@@ -8,7 +8,7 @@ LIB_NAME = "libnt2h.so"
 
 EXACT, FAST, ACCURATE = 0, 1, 2
 BS_TEXTBOOK, BS_DEMO = 0, 1
-OPS2 = {"atan2": 0}
+OPS2 = {"atan2": 0, "pow": 1}
 OPS = {"exp": 0, "log": 1, "sqrt": 2, "erfc": 3, "normcdf": 4,
        "fastnormcdf": 5, "cos": 6, "sin": 7, "abs": 8, "exp2": 9, "exp10": 10, "log2": 11,
        "log10": 12, "floor": 13, "ceil": 14, "trunc": 15, "round": 16, "round2even": 17, "sign": 18,

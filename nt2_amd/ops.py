@@ -50,8 +50,12 @@ def stream_copy(src, dst, grid=0):
 
 
 def map2(op, x, y, policy=EXACT, out=None):
-    """out = f(x, y) elementwise for a binary function (atan2)."""
+    """out = f(x, y) elementwise for a binary function (atan2, pow)."""
+    if y.numel() != x.numel() or y.dtype != x.dtype or not (x.is_contiguous() and y.is_contiguous()):
+        raise ValueError("map2 %s: x and y must be contiguous tables of one size and type" % op)
     out = torch.empty_like(x) if out is None else out
+    if out.numel() != x.numel() or out.dtype != x.dtype or not out.is_contiguous():
+        raise ValueError("map2 %s: out must be a contiguous table like x" % op)
     f = lib().nt2h_map2_f32 if x.dtype == torch.float32 else lib().nt2h_map2_f64
     check(f(OPS2[op], x.numel(), ptr(x), ptr(y), ptr(out), policy, _s()), "map2 " + op)
     return out

@@ -681,6 +681,199 @@ inline double acos_d(double a0, bool scalar) {
   return big ? x : pio2 - x;
 }
 
+// pow(a0, a1): modules/core/exponential/include/nt2/exponential/functions/
+// simd/common/pow.hpp:38-46 and pow_abs.hpp:65-133 (packs), scalar/pow.hpp:30-36
+// and pow_abs.hpp:51-111 (pack tail), kernels details/pow_kernel.hpp:30-228.
+// Cephes' pow: log2 of the significand against a 17-entry table of 2^(-i/16),
+// the product with a1 split in sixteenths, 2^(fraction) by a polynomial.
+// Helpers restate boost.simd: frexp (ieee/functions/simd/common/frexp.hpp:52-75:
+// Inf and NaN give exponent 0 and mantissa a0 + 0.5), ldexp
+// (ieee/functions/simd/common/ldexp.hpp:57-71: two products, the second by
+// 2^Minexponent or 2), toint (cvtt: INT_MIN when out of range or NaN),
+// is_flint (x - trunc(x) == 0, false for Inf/NaN), is_odd(x) =
+// is_flint((x - 1) / 2) (predicates/functions/simd/common/{is_odd,is_even,is_flint}.hpp).
+template <class T> struct powk;
+template <> struct powk<float> {
+  using I = int32_t;
+  using U = uint32_t;
+  static constexpr int nbm = 23, maxe = 127, mine = -126, lime = 128;
+  static float A(int i) {
+    static const uint32_t t[17] = {0x3f800000, 0x3f75257d, 0x3f6ac0c7, 0x3f60ccdf, 0x3f5744fd, 0x3f4e248c,
+                                   0x3f45672a, 0x3f3d08a4, 0x3f3504f3, 0x3f2d583f, 0x3f25fed7, 0x3f1ef532,
+                                   0x3f1837f0, 0x3f11c3d3, 0x3f0b95c2, 0x3f05aac3, 0x3f000000};
+    return as_f(t[i]);
+  }
+  static float B(int i) {
+    static const uint32_t t[9] = {0x00000000, 0xb1c116de, 0xb254a58a, 0x318aa837, 0x324fe77a,
+                                  0xb2ac9d5e, 0x32a31b71, 0xb1e0aba1, 0x00000000};
+    return as_f(t[i]);
+  }
+  static float pow1(float x, float z) {
+    const float c[4] = {F32(0xbe2a61b2), F32(0x3e4d2fa3), F32(0xbe800015), F32(0x3eaaaaa3)};
+    return (z * x) * horner(c, x);
+  }
+  static float pow2(float x) {
+    const float c[4] = {F32(0x3c1a49bc), F32(0x3d634d38), F32(0x3e75fde1), F32(0x3f317218)};
+    return horner(c, x);
+  }
+  static float log2em1() { return F32(0x3ee2a8ed); }
+  static float large() { return F32(0x44ffe000); }
+  static float low() { return F32(0xc5160000); }
+  static float tiny() { return F32(0x00800000); }
+  static float bits2(U u) { return as_f(u); }
+};
+template <> struct powk<double> {
+  using I = int64_t;
+  using U = uint64_t;
+  static constexpr int nbm = 52, maxe = 1023, mine = -1022, lime = 1024;
+  static double A(int i) {
+    static const uint64_t t[17] = {
+        0x3ff0000000000000, 0x3feea4afa2a490da, 0x3fed5818dcfba487, 0x3fec199bdd85529c, 0x3feae89f995ad3ad,
+        0x3fe9c49182a3f090, 0x3fe8ace5422aa0db, 0x3fe7a11473eb0187, 0x3fe6a09e667f3bcd, 0x3fe5ab07dd485429,
+        0x3fe4bfdad5362a27, 0x3fe3dea64c123422, 0x3fe306fe0a31b715, 0x3fe2387a6e756238, 0x3fe172b83c7d517b,
+        0x3fe0b5586cf9890f, 0x3fe0000000000000};
+    return as_d(t[i]);
+  }
+  static double B(int i) {
+    static const uint64_t t[9] = {0x0000000000000000, 0x3c72ed02d75b3707, 0x3c87a1cd345dcc81,
+                                  0x3c86e9f156864b27, 0xbc8bdd3413b26456, 0x3c6d4397afec42e2,
+                                  0x3c76f46ad23182e4, 0xbc719041b9d78a76, 0x0000000000000000};
+    return as_d(t[i]);
+  }
+  static double pow1(double x, double z) {
+    const double p[4] = {F64(0x3fdfdb997f5b5cf0), F64(0x400dddefea9edf15), F64(0x401eccbd7f78eb6f),
+                         F64(0x4012aa83b65c9b74)};
+    const double q[5] = {F64(0x3ff0000000000000), F64(0x4022aab49b20914e), F64(0x403bffff41c1c9f5),
+                         F64(0x4040ccbc1b176402), F64(0x402bffc5918ae92e)};
+    return x * ((z * horner(p, x)) / horner(q, x));
+  }
+  static double pow2(double x) {
+    const double c[7] = {F64(0x3eef6307d7f2937f), F64(0x3f242fbe60fc9259), F64(0x3f55d87ec84aef1d),
+                         F64(0x3f83b2ab6ef133b7), F64(0x3fac6b08d7041a92), F64(0x3fcebfbdff82c56d),
+                         F64(0x3fe62e42fefa39ef)};
+    return horner(c, x);
+  }
+  static double log2em1() { return F64(0x3fdc551d94ae0bf8); }
+  static double large() { return F64(0x40cfff8000000000); }
+  static double low() { return F64(0xc0d0c7c000000000); }
+  static double tiny() { return F64(0x0010000000000000); }
+  static double bits2(U u) { return as_d(u); }
+};
+template <class T> inline typename powk<T>::I pw_toint(T x) {
+  using I = typename powk<T>::I;
+  const T lim = (T)std::numeric_limits<I>::max();
+  if (!(x > -lim - 1 && x < lim)) return std::numeric_limits<I>::min();
+  return (I)x;
+}
+template <class T> inline T pw_frexp(T a, typename powk<T>::I& e) {
+  if (a == 0) { e = 0; return a; }
+  if (!std::isfinite(a)) { e = 0; return a + T(0.5); }
+  int ee;
+  const T m = std::frexp(a, &ee);
+  e = ee;
+  return m;
+}
+template <class T> inline T pw_ldexp(T a, typename powk<T>::I e) {
+  using K = powk<T>;
+  using U = typename K::U;
+  T f = T(1);
+  if (e < K::mine) { e -= K::mine; f = K::tiny(); }
+  if (e == K::lime) { f = f + T(1); e -= 1; }
+  const U b = (U)(e + K::maxe) << K::nbm;
+  return (a * K::bits2(b)) * f;
+}
+template <class T> inline bool pw_flint(T x) { return std::isfinite(x) && (x - std::trunc(x)) == T(0); }
+template <class T> inline bool pw_odd(T x) { return pw_flint((x - T(1)) * T(0.5)); }
+template <class T> inline T pw_reduc(T x) { return T(0.0625) * std::floor(T(16) * x); }
+
+// the arithmetic common to both forms: sets w (sixteenths of the exponent)
+// and returns the unclamped result
+template <class T> inline T pow_core(T ax, T a1, T& wout) {
+  using K = powk<T>;
+  using I = typename K::I;
+  I e;
+  T x = pw_frexp(ax, e);
+  // select: the table entry just above the significand (pow_kernel.hpp:88-102)
+  I i = 1;
+  if (x <= K::A(9)) i = 9;
+  if (x <= K::A(i + 4)) i += 4;
+  if (x <= K::A(i + 2)) i += 2;
+  if (x >= K::A(1)) i = -1;
+  i = i + 1;
+  const T tmp = K::A((int)i);
+  x = x - tmp;
+  x = x - K::B((int)(i >> 1));
+  x = x / tmp;
+  T z = x * x;
+  T w = K::pow1(x, z);
+  w = T(-0.5) * z + w;
+  w = K::log2em1() * w + w;
+  z = K::log2em1() * x + w;
+  z = z + x;
+  w = (T)(-i) * T(0.0625) + (T)e;
+  const T ya = pw_reduc(a1);
+  const T yb = a1 - ya;
+  T W = z * a1 + w * yb;
+  T Wa = pw_reduc(W);
+  T Wb = W - Wa;
+  W = w * ya + Wa;
+  Wa = pw_reduc(W);
+  const T u = W - Wa;
+  W = Wb + u;
+  Wb = pw_reduc(W);
+  w = T(16) * (Wa + Wb);
+  wout = w;
+  e = pw_toint(w);
+  Wb = W - Wb;
+  if (Wb > T(0)) { e = e + 1; Wb = Wb - T(0.0625); }
+  z = K::pow2(Wb) * Wb;
+  // i = e/16 + (e > 0): the pack form (the tail's e >= 0 gives the same value)
+  i = e / 16 + (e > 0 ? 1 : 0);
+  e = i * 16 - e;
+  const T wt = K::A((e >= 0 && e <= 16) ? (int)e : 0);  // other e only in lanes replaced below
+  z = wt * z + wt;
+  return pw_ldexp(z, i);
+}
+template <class T> inline T pow_abs_t(T a0, T a1, bool scalar) {
+  using K = powk<T>;
+  const T inf = std::numeric_limits<T>::infinity();
+  const T x = std::fabs(a0);
+  if (scalar) {
+    if (x == T(1)) return x;
+    if (x == T(0)) return a1 == T(0) ? T(1) : T(0);
+    if (x == a1 && a1 == inf) return inf;
+    if (x == inf && a1 == -inf) return T(0);
+    if (a1 == inf) return (x < T(1)) ? T(0) : inf;
+    if (a1 == -inf) return (x > T(1)) ? T(0) : inf;
+    if (x == inf) return (a1 < T(0)) ? T(0) : ((a1 == T(0)) ? T(1) : inf);
+    if (std::isnan(a0) || std::isnan(a1)) return std::numeric_limits<T>::quiet_NaN();
+    T w;
+    const T z = pow_core(x, a1, w);
+    if (w > K::large()) return inf;
+    if (w < K::low()) return T(0);
+    return z;
+  }
+  T w;
+  T z = pow_core(x, a1, w);
+  const bool gtax1 = x > T(1);
+  if (a1 == inf) z = gtax1 ? inf : T(0);
+  if (a1 == -inf) z = gtax1 ? T(0) : inf;
+  if (x == inf) z = (a1 > T(0)) ? inf : ((a1 < T(0)) ? T(0) : T(1));
+  if (w < K::low()) z = T(0);
+  if (w > K::large()) z = inf;
+  if (x == T(1)) z = x;
+  if (a0 == T(0)) z = (a1 == T(0)) ? T(1) : T(0);
+  return z;
+}
+template <class T> inline T pow_t(T a0, T a1, bool scalar) {
+  const bool ltz = a0 < T(0);
+  if (scalar && ltz && !pw_flint(a1)) return std::numeric_limits<T>::quiet_NaN();
+  T z = pow_abs_t(a0, a1, scalar);
+  if (ltz && pw_odd(a1)) z = -z;
+  if (ltz && !pw_flint(a1)) z = std::numeric_limits<T>::quiet_NaN();
+  return z;
+}
+
 // atan2: simd/common/atan2.hpp:30-54, scalar/atan2.hpp:29-45
 template <class T, class K> inline T atan2_t(T a0, T a1, T pi, K kernel) {
   if (a0 != a0 || a1 != a1) return a0 + a1;
@@ -1507,18 +1700,20 @@ int oracle_map_f64(int op, const double* x, double* y, size_t n) {
 
 // out = atan2(x, y) over tables of n elements, NT2 transform semantics
 int oracle_map2_f32(int op, const float* x, const float* y, float* out, size_t n) {
-  if (op != 0) return -1;
+  if (op != 0 && op != 1) return -1;
   nt2_transform(n, 8, sizeof(float), [&](size_t i, bool pack) {
     const size_t w = pack ? 8 : 1;
-    for (size_t l = 0; l < w; ++l) out[i + l] = atan2_f(x[i + l], y[i + l], !pack);
+    for (size_t l = 0; l < w; ++l)
+      out[i + l] = op == 0 ? atan2_f(x[i + l], y[i + l], !pack) : pow_t<float>(x[i + l], y[i + l], !pack);
   });
   return 0;
 }
 int oracle_map2_f64(int op, const double* x, const double* y, double* out, size_t n) {
-  if (op != 0) return -1;
+  if (op != 0 && op != 1) return -1;
   nt2_transform(n, 4, sizeof(double), [&](size_t i, bool pack) {
     const size_t w = pack ? 4 : 1;
-    for (size_t l = 0; l < w; ++l) out[i + l] = atan2_d(x[i + l], y[i + l], !pack);
+    for (size_t l = 0; l < w; ++l)
+      out[i + l] = op == 0 ? atan2_d(x[i + l], y[i + l], !pack) : pow_t<double>(x[i + l], y[i + l], !pack);
   });
   return 0;
 }

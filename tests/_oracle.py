@@ -277,3 +277,13 @@ def atan2(y, x):
     f = lib().oracle_map2_f32 if y.dtype == np.float32 else lib().oracle_map2_f64
     assert f(0, _p(y), _p(x), _p(out), y.size) == 0
     return out
+
+
+def pow(a, b):
+    """NT2's pow(a, b) over flat tables (packs of 8 / 4, scalar tail)."""
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b, dtype=a.dtype)
+    out = np.empty_like(a)
+    f = lib().oracle_map2_f32 if a.dtype == np.float32 else lib().oracle_map2_f64
+    assert f(1, _p(a), _p(b), _p(out), a.size) == 0
+    return out

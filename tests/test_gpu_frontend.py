@@ -142,6 +142,9 @@ def test_exponential_rounding_statements(outputs):
     want = O.map_f32("asin", a / f32(3)) * O.map_f32("acos", b / f32(10))
     got = outputs("asin_f32", f32)
     assert same_bits(got, want), first_diff(got, want)
+    want = O.pow(np.abs(a) + f32(0.5), b / f32(4)) + O.pow(a, O.map_f32("round", b))
+    got = outputs("pow_f32", f32)
+    assert same_bits(got, want), first_diff(got, want)
     d = outputs("M", f32).astype(np.float64) / 100.0
     want = (O.map_f64("trunc", d) * O.map_f64("exp2", O.map_f64("ceil", d / 50.0)) +
             O.map_f64("sign", d) * O.map_f64("log10", np.abs(d) + 1.0))
@@ -157,6 +160,9 @@ def test_exponential_rounding_statements(outputs):
     assert q.size == 1001
     want = O.map_f64("asin", q) - O.map_f64("acos", q * 0.9)
     got = outputs("asin_f64", np.float64)
+    assert same_bits(got, want), first_diff(got, want)
+    want = O.pow(q + 2.0, q * 300.0)
+    got = outputs("pow_f64", np.float64)
     assert same_bits(got, want), first_diff(got, want)
 
 

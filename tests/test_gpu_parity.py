@@ -479,3 +479,65 @@ def test_atan2_exact_bitwise(dt, n):
     got = ops.map2("atan2", torch.from_numpy(y).cuda(), torch.from_numpy(x).cuda(), EXACT).cpu().numpy()
     want = O.atan2(y, x)
     assert same_bits(got, want), first_diff(got, want)
+
+
+def _pow_inputs(dt, n, rng):
+    """positive bases over a wide range, negative bases with integral and
+    non-integral exponents, exact powers of two down to the subnormals, results
+    past overflow and underflow, and every pair of special values; the last
+    n % 8 (n % 4) pairs fall in the scalar tail"""
+    lim = 120 if dt is np.float32 else 1000
+    q = n // 5
+    a = [np.exp2(rng.uniform(-30, 30, q)), -np.exp2(rng.uniform(-8, 8, q)), np.full(q, 2.0),
+         np.exp2(rng.uniform(-4, 4, q))]
+    b = [rng.uniform(-8, 8, q), np.round(rng.uniform(-20, 20, q)) + (rng.random(q) < 0.3) * 0.5,
+         np.round(rng.uniform(-lim - 30, lim, q)), rng.uniform(-lim, lim, q)]
+    v = np.array([0.0, -0.0, 0.5, 1.0, -1.0, 2.0, -2.0, 3.0, np.inf, -np.inf, np.nan])
+    ga, gb = (g.ravel() for g in np.meshgrid(v, v, indexing="ij"))
+    rest = n - 4 * q - ga.size
+    a += [ga, np.exp2(rng.uniform(-10, 10, rest))]
+    b += [gb, rng.uniform(-3, 3, rest)]
+    return np.concatenate(a).astype(dt), np.concatenate(b).astype(dt)
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("n", [1000, 20005, 200003])
+def test_pow_exact_bitwise(dt, n):
+    rng = np.random.default_rng(800 + n)
+    a, b = _pow_inputs(dt, n, rng)
+    # the special pairs (NaN, Inf) and (Inf, NaN), whose forms differ, also in the tail
+    a[-2:], b[-2:] = [np.nan, np.inf], [np.inf, np.nan]
+    say("pow", dt.__name__, "n=%d" % n)
+    got = ops.map2("pow", torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda(), EXACT).cpu().numpy()
+    want = O.pow(a, b)
+    assert same_bits(got, want), first_diff(got, want)
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_pow_fast_within_bound(dt):
+    """Fused policy: within 2 + |b|/10 ulp of the exact policy's value (one
+    rounding more or less in log2 of the significand is multiplied by |b|, as
+    is NT2's own error: test_pow_f32_error_grows_with_the_exponent) where the
+    result is normal and more than half an octave inside the
+    limits (there the fused steps can move the exponent, counted in
+    sixteenths, across NT2's Powlargelim / Powlowlim); the special values
+    unchanged."""
+    rng = np.random.default_rng(900)
+    a, b = _pow_inputs(dt, 40000, rng)
+    say("pow fast", dt.__name__)
+    got = ops.map2("pow", torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda(), FAST).cpu().numpy()
+    ex = O.pow(a, b)
+    normal = np.isfinite(ex) & (np.abs(ex) >= np.finfo(dt).tiny)
+    assert np.array_equal(np.isnan(got), np.isnan(ex))
+    with np.errstate(divide="ignore"):
+        l2 = np.log2(np.abs(ex.astype(np.float64)))
+    inside = normal & (l2 < np.log2(np.finfo(dt).max) - 0.5) & (l2 > np.log2(np.finfo(dt).tiny) + 0.5)
+    special = ~np.isfinite(a) | ~np.isfinite(b) | (a == 0) | (np.abs(a) == 1)
+    assert same_bits(got[special], ex[special])
+    inside &= ~special
+    err = classic_ulp(got[inside], ex[inside].astype(np.float64), dt) - np.abs(b[inside]) / 10.0
+    k = np.argsort(-err)[:4]
+    assert err.max() <= 2.0, list(zip(a[inside][k], b[inside][k], got[inside][k], ex[inside][k], err[k]))
+    edge = normal & ~inside  # near the limits: the exact value, or the neighbouring limit
+    assert np.all((got[edge] == ex[edge]) | np.isinf(got[edge]) | (np.abs(got[edge]) < 4 * np.finfo(dt).tiny) |
+                  (classic_ulp(got[edge], ex[edge].astype(np.float64), dt) <= 2.0 + np.abs(b[edge]) / 10.0))
