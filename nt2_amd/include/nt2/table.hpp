@@ -553,12 +553,19 @@ NT2H_FN1(if_zero_else_one, if_zero_else_one_)
 NT2H_FN2(max, max_) NT2H_FN2(min, min_) NT2H_FN2(adds, adds_) NT2H_FN2(subs, subs_)
 NT2H_FN2(selinc, selinc_) NT2H_FN2(seldec, seldec_) NT2H_FN2(atan2, atan2_)
 #undef NT2H_FN2
-// pow(a, b) with floating-point operands (NT2's Cephes pow).  NT2 dispatches an
-// integral exponent to repeated squaring (simd/common/pow.hpp:49-72), which this
-// front end does not provide: such a call does not compile.
+// pow(a, b): Cephes' pow for floating-point exponents; an integral exponent
+// (a scalar, kept as int, or an integer table) takes NT2's repeated squaring
+// instead (modules/core/exponential/include/nt2/exponential/functions/generic/pow.hpp:110-168).
 template <class A, class B, NT2H_ENABLE2(A, B)> auto pow(const A& a, const B& b) {
-  static_assert(!std::is_integral<B>::value, "nt2::pow with an integral exponent (repeated squaring) is not provided");
-  return make2<tagfn::pow_>(a, b);
+  if constexpr (std::is_integral<B>::value && expr::is_dsl<A>::value) {
+    typedef typename expr::wrap<A>::type EA;
+    expr::node2<tagfn::pow_, EA, expr::scalar<int>> n;
+    n.a = expr::wrap<A>::get(a);
+    n.b.v = (int)b;
+    return n;
+  } else {
+    return make2<tagfn::pow_>(a, b);
+  }
 }
 // fma(a,b,c) = a*b+c and fnms(a,b,c) = c-a*b, NT2's two-rounding forms
 template <class A, class B, class C,
@@ -656,7 +663,23 @@ __device__ __forceinline__ void ev(const n2v<Tag, VA, VB>& v, size_t i, R (&r)[G
   TB b[G];
   ev<G>(v.a, i, a);
   ev<G>(v.b, i, b);
-  if constexpr (std::is_same<Tag, tagfn::pow_>::value) {
+  if constexpr (std::is_same<Tag, tagfn::pow_>::value && std::is_integral<TB>::value) {
+    // generic/pow.hpp:110-132 and :154-168: |b| by repeated squaring, 1/p for b < 0
+    static_assert(std::is_floating_point<R>::value, "pow is defined for floating-point bases only");
+#pragma unroll
+    for (int l = 0; l < G; ++l) {
+      _Pragma("clang fp contract(off)")
+      R base = (R)a[l], p = R(1);
+      typedef typename std::make_unsigned<TB>::type UB;
+      UB e = b[l] < 0 ? (UB)(UB(0) - (UB)b[l]) : (UB)b[l];
+      while (e) {
+        if (e & 1u) p = p * base;
+        e >>= 1;
+        base = base * base;
+      }
+      r[l] = b[l] < 0 ? R(1) / p : p;
+    }
+  } else if constexpr (std::is_same<Tag, tagfn::pow_>::value) {
     static_assert(std::is_floating_point<R>::value, "pow is defined for floating-point tables only");
 #pragma unroll
     for (int l = 0; l < G; ++l) {

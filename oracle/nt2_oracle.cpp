@@ -1718,6 +1718,29 @@ int oracle_map2_f64(int op, const double* x, const double* y, double* out, size_
   return 0;
 }
 
+// pow(x, k) with an integral exponent:
+// modules/core/exponential/include/nt2/exponential/functions/generic/pow.hpp:110-132
+// (|k| by repeated squaring in the base's precision) and :154-168 (1/p for k < 0);
+// elementwise, so the pack schedule does not matter
+extern "C++" template <class T> T powi_t(T base, int64_t k) {
+  uint64_t e = k < 0 ? (uint64_t)0 - (uint64_t)k : (uint64_t)k;
+  T p = T(1);
+  while (e) {
+    if (e & 1u) p = p * base;
+    e >>= 1;
+    base = base * base;
+  }
+  return k < 0 ? T(1) / p : p;
+}
+int oracle_powi_f32(const float* x, const int64_t* k, float* out, size_t n) {
+  for (size_t i = 0; i < n; ++i) out[i] = powi_t(x[i], k[i]);
+  return 0;
+}
+int oracle_powi_f64(const double* x, const int64_t* k, double* out, size_t n) {
+  for (size_t i = 0; i < n; ++i) out[i] = powi_t(x[i], k[i]);
+  return 0;
+}
+
 // [s, c] = sincos(x): generic/sincos.hpp:50-64
 int oracle_sincos_f32(const float* x, float* s, float* c, size_t n) {
   nt2_transform(n, 8, sizeof(float), [&](size_t i, bool pack) {

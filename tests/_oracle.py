@@ -59,6 +59,8 @@ def lib():
         _lib.oracle_rgb2grey_u8.argtypes = [S, P, P, P, P]
         _lib.oracle_map2_f32.argtypes = [ctypes.c_int, P, P, P, S]
         _lib.oracle_map2_f64.argtypes = [ctypes.c_int, P, P, P, S]
+        _lib.oracle_powi_f32.argtypes = [P, P, P, S]
+        _lib.oracle_powi_f64.argtypes = [P, P, P, S]
         _lib.oracle_linspace_f32.argtypes = [S, ctypes.c_float, ctypes.c_float, P]
         _lib.oracle_linspace_f64.argtypes = [S, ctypes.c_double, ctypes.c_double, P]
         _lib.oracle_mandelbrot_f32.argtypes = [S, P, P, P, ctypes.c_int]
@@ -276,6 +278,16 @@ def atan2(y, x):
     out = np.empty_like(y)
     f = lib().oracle_map2_f32 if y.dtype == np.float32 else lib().oracle_map2_f64
     assert f(0, _p(y), _p(x), _p(out), y.size) == 0
+    return out
+
+
+def pow_int(a, k):
+    """NT2's pow with an integral exponent over flat tables (repeated squaring)."""
+    a = np.ascontiguousarray(a)
+    k = np.ascontiguousarray(np.broadcast_to(np.asarray(k, dtype=np.int64), a.shape))
+    out = np.empty_like(a)
+    f = lib().oracle_powi_f32 if a.dtype == np.float32 else lib().oracle_powi_f64
+    assert f(_p(a), _p(k), _p(out), a.size) == 0
     return out
 
 

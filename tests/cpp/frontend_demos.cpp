@@ -211,6 +211,8 @@ int main(int argc, char** argv) {
   dump("asin_f32", at2);
   at2 = nt2::pow(nt2::abs(at) + 0.5f, bt / 4.0f) + nt2::pow(at, nt2::round(bt));
   dump("pow_f32", at2);
+  at2 = nt2::pow(at, 3) + nt2::pow(bt, -2);  // integral exponents: repeated squaring
+  dump("powi_f32", at2);
   table<double> dm(of_size(mh * mw)), dr;
   for (std::size_t i = 1; i <= mh * mw; ++i) dm(i) = (double)M(i) / 100.0;
   dr = nt2::trunc(dm) * nt2::exp2(nt2::ceil(dm / 50.0)) + nt2::sign(dm) * nt2::log10(nt2::abs(dm) + 1.0);
@@ -226,6 +228,8 @@ int main(int argc, char** argv) {
   dump("asin_f64", dq2);
   dq2 = nt2::pow(dq + 2.0, dq * 300.0);
   dump("pow_f64", dq2);
+  dq2 = nt2::pow(dq * 10.0, 7);
+  dump("powi_f64", dq2);
 
   // ---- rgb2grey and rgb2yuv on a 96 x 131 image: the demos' ramp and random planes
   const std::size_t ih = 96, iw = 131;

@@ -145,6 +145,9 @@ def test_exponential_rounding_statements(outputs):
     want = O.pow(np.abs(a) + f32(0.5), b / f32(4)) + O.pow(a, O.map_f32("round", b))
     got = outputs("pow_f32", f32)
     assert same_bits(got, want), first_diff(got, want)
+    want = O.pow_int(a, 3) + O.pow_int(b, -2)
+    got = outputs("powi_f32", f32)
+    assert same_bits(got, want), first_diff(got, want)
     d = outputs("M", f32).astype(np.float64) / 100.0
     want = (O.map_f64("trunc", d) * O.map_f64("exp2", O.map_f64("ceil", d / 50.0)) +
             O.map_f64("sign", d) * O.map_f64("log10", np.abs(d) + 1.0))
@@ -163,6 +166,9 @@ def test_exponential_rounding_statements(outputs):
     assert same_bits(got, want), first_diff(got, want)
     want = O.pow(q + 2.0, q * 300.0)
     got = outputs("pow_f64", np.float64)
+    assert same_bits(got, want), first_diff(got, want)
+    want = O.pow_int(q * 10.0, 7)
+    got = outputs("powi_f64", np.float64)
     assert same_bits(got, want), first_diff(got, want)
 
 

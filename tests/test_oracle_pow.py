@@ -153,3 +153,26 @@ def test_pow_f32_error_grows_with_the_exponent():
     err = classic_ulp(got[ok], ref[ok], np.float32)
     assert np.all(err <= 2.0 + np.abs(b[ok]) / 10.0)
     assert err[np.abs(b[ok]) < 2].max() <= 1.0
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_pow_integral_exponent_nt2_unit_kats(dt):
+    """modules/core/exponential/unit/simd/pow.cpp:77-116 (tolerance 0), NT2's
+    repeated squaring for an integral exponent; and against the exact power
+    wherever that is representable (small integer bases, |k| <= 12)."""
+    inf, nan = np.inf, np.nan
+    kats = [(inf, 3, inf), (-inf, 3, -inf), (nan, 3, nan), (inf, 4, inf), (-inf, 4, inf), (nan, 4, nan),
+            (2.0, -3, 0.125), (-1.0, 3, -1.0), (1.0, 3, 1.0), (2.0, 3, 8.0), (0.0, 0, 1.0), (0.0, 3, 0.0),
+            (-1.0, 4, 1.0), (1.0, 4, 1.0), (0.0, 4, 0.0)]
+    a = np.array([x for x, _, _ in kats], dt)
+    k = np.array([e for _, e, _ in kats])
+    want = np.array([w for _, _, w in kats], dt)
+    assert _same(O.pow_int(a, k), want)
+    base = np.repeat(np.arange(-7, 8), 13).astype(dt)
+    kk = np.tile(np.arange(0, 13), 15)
+    exact = np.array([float(int(x) ** int(e)) for x, e in zip(base, kk)])
+    rep = np.abs(exact) < 2.0 ** (24 if dt is np.float32 else 53)
+    assert np.array_equal(O.pow_int(base, kk)[rep], exact[rep].astype(dt))
+    # a negative exponent is the reciprocal of the positive power
+    x = np.exp2(np.random.default_rng(2).uniform(-3, 3, 500)).astype(dt)
+    assert np.array_equal(O.pow_int(x, -5), dt(1) / O.pow_int(x, 5))
