@@ -187,7 +187,7 @@ struct izeo1 { template <class C> struct result { typedef uint8_t type; }; };
 NT2H_TAG(plus_, same2) NT2H_TAG(minus_, same2) NT2H_TAG(multiplies_, same2) NT2H_TAG(divides_, same2)
 NT2H_TAG(max_, same2) NT2H_TAG(min_, same2) NT2H_TAG(adds_, same2) NT2H_TAG(subs_, same2)
 NT2H_TAG(lt_, logical2) NT2H_TAG(gt_, logical2) NT2H_TAG(le_, logical2) NT2H_TAG(ge_, logical2)
-NT2H_TAG(eq_, logical2) NT2H_TAG(neq_, logical2)
+NT2H_TAG(eq_, logical2) NT2H_TAG(neq_, logical2) NT2H_TAG(land_, logical2) NT2H_TAG(lor_, logical2)
 NT2H_TAG(neg_, same1) NT2H_TAG(exp_, same1) NT2H_TAG(log_, same1) NT2H_TAG(sqrt_, same1)
 NT2H_TAG(erfc_, same1) NT2H_TAG(normcdf_, same1) NT2H_TAG(fastnormcdf_, same1) NT2H_TAG(cos_, same1)
 NT2H_TAG(sin_, same1) NT2H_TAG(abs_, same1) NT2H_TAG(sqr_, same1) NT2H_TAG(rec_, same1)
@@ -319,6 +319,9 @@ NT2H_BIN(le_, x <= y)
 NT2H_BIN(ge_, x >= y)
 NT2H_BIN(eq_, x == y)
 NT2H_BIN(neq_, x != y)
+// logical_and / logical_or: a nonzero (or NaN) operand is true, as NT2's is_nez
+NT2H_BIN(land_, (x != 0) && (y != 0))
+NT2H_BIN(lor_, (x != 0) || (y != 0))
 #undef NT2H_BIN
 template <> struct apply2<tagfn::adds_> {
   template <class R, class T, class U> __device__ __forceinline__ static R one(T x, U y) {
@@ -515,10 +518,13 @@ auto make2(const A& a, const B& b) {
 }
 template <class Tag, class A, class B, class C>
 auto make3(const A& a, const B& b, const C& c) {
+  // the branches' type; if_else(c, 1.0f, 0.0f) takes its scalars' type, not the condition's
   typedef typename std::conditional<
       expr::is_dsl<B>::value, typename expr::vt_or<typename expr::wrap<B>::type>::type,
-      typename std::conditional<expr::is_dsl<C>::value, typename expr::vt_or<typename expr::wrap<C>::type>::type,
-                                typename expr::vt_or<typename expr::wrap<A>::type>::type>::type>::type hint;
+      typename std::conditional<
+          expr::is_dsl<C>::value, typename expr::vt_or<typename expr::wrap<C>::type>::type,
+          typename std::conditional<std::is_same<Tag, tagfn::if_else_>::value && std::is_arithmetic<B>::value, B,
+                                    typename expr::vt_or<typename expr::wrap<A>::type>::type>::type>::type>::type hint;
   typedef typename expr::as_expr<A, hint>::type EA;
   typedef typename expr::as_expr<B, hint>::type EB;
   typedef typename expr::as_expr<C, hint>::type EC;
@@ -533,7 +539,12 @@ auto make3(const A& a, const B& b, const C& c) {
   template <class A, class B, NT2H_ENABLE2(A, B)> auto operator op(const A& a, const B& b) { return make2<tagfn::Tag>(a, b); }
 NT2H_OP2(+, plus_) NT2H_OP2(-, minus_) NT2H_OP2(*, multiplies_) NT2H_OP2(/, divides_)
 NT2H_OP2(<, lt_) NT2H_OP2(>, gt_) NT2H_OP2(<=, le_) NT2H_OP2(>=, ge_) NT2H_OP2(==, eq_) NT2H_OP2(!=, neq_)
+NT2H_OP2(&&, land_) NT2H_OP2(||, lor_)
 #undef NT2H_OP2
+#define NT2H_FN2(name, Tag) \
+  template <class A, class B, NT2H_ENABLE2(A, B)> auto name(const A& a, const B& b) { return make2<tagfn::Tag>(a, b); }
+NT2H_FN2(logical_and, land_) NT2H_FN2(logical_or, lor_)
+#undef NT2H_FN2
 template <class A, NT2H_ENABLE1(A)> auto operator-(const A& a) { return make1<tagfn::neg_>(a); }
 
 #define NT2H_FN1(name, Tag) \

@@ -213,6 +213,8 @@ int main(int argc, char** argv) {
   dump("pow_f32", at2);
   at2 = nt2::pow(at, 3) + nt2::pow(bt, -2);  // integral exponents: repeated squaring
   dump("powi_f32", at2);
+  at2 = if_else((at > 0.0f) && (bt < 1.0f), at, bt) + if_else(logical_or(at < -1.0f, bt > 2.0f), 1.0f, 0.0f);
+  dump("logical_f32", at2);
   table<double> dm(of_size(mh * mw)), dr;
   for (std::size_t i = 1; i <= mh * mw; ++i) dm(i) = (double)M(i) / 100.0;
   dr = nt2::trunc(dm) * nt2::exp2(nt2::ceil(dm / 50.0)) + nt2::sign(dm) * nt2::log10(nt2::abs(dm) + 1.0);

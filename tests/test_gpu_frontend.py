@@ -148,6 +148,9 @@ def test_exponential_rounding_statements(outputs):
     want = O.pow_int(a, 3) + O.pow_int(b, -2)
     got = outputs("powi_f32", f32)
     assert same_bits(got, want), first_diff(got, want)
+    want = np.where((a > 0) & (b < 1), a, b) + np.where((a < -1) | (b > 2), f32(1), f32(0))
+    got = outputs("logical_f32", f32)
+    assert same_bits(got, want.astype(f32)), first_diff(got, want)
     d = outputs("M", f32).astype(np.float64) / 100.0
     want = (O.map_f64("trunc", d) * O.map_f64("exp2", O.map_f64("ceil", d / 50.0)) +
             O.map_f64("sign", d) * O.map_f64("log10", np.abs(d) + 1.0))
