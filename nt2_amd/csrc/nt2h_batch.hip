@@ -4,13 +4,14 @@
 // s_i = sum(exp(-x.*x/2).*cos(y) + sqrt(abs(x))) for every table.  The
 // tables are cut into tiles of `tile` elements (a multiple of the 8-float
 // pack, so NT2's pack grouping restarts at each table start); a persistent
-// grid pulls tile indices from a device work queue, reduces each tile to a
-// compensated FP32 partial, and the block that completes a table's last tile
-// sums that table's partials in tile order in FP64 (deterministic).  Tiles
-// with arguments that need the out-of-line trig reduction are listed and
-// redone by a second kernel, so that the persistent one holds no calls.  The
-// reset, persistent and slow-tile kernels are captured once in a hipGraph and
-// replayed.  An autotuner picks block size, elements per thread and
+// grid pulls tile indices from a device work queue and reduces each tile to a
+// compensated FP32 partial stored as a double; a last kernel sums each
+// table's partials in a fixed order (deterministic; a table of length 0 gets
+// 0).  Tiles with arguments that need the out-of-line trig reduction are
+// listed and redone by a second kernel, so that the persistent one holds no
+// calls.  No tile publishes anything with a fence: the per-table sums are
+// read by the next kernel on the stream.  The reset, persistent, slow-tile
+// and sum kernels are captured once in a hipGraph and replayed.  An autotuner picks block size, elements per thread and
 // workgroups per CU by timing the candidates.
 //
 // No counterpart in the reference: NT2 evaluates one expression per
@@ -56,10 +57,8 @@ __device__ __forceinline__ void expr4_g(const float* x, const float* y, float* z
   }
 }
 
-__global__ void batch_reset_k(unsigned* state, unsigned* done, int ntables) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < 2) state[i] = 0u;
-  if (i < ntables) done[i] = 0u;
+__global__ void batch_reset_k(unsigned* state) {
+  if (threadIdx.x < 2) state[threadIdx.x] = 0u;
 }
 
 // one thread's compensated sum over its elements of a tile: 8-lane packs,
@@ -93,13 +92,10 @@ __device__ __forceinline__ void tile_sum(const TableDesc& d, uint64_t start, uin
   }
 }
 
-// the tile's FP64 partial; the block that brings a table's done count to its
-// tile count sums that table's partials in tile order
+// the tile's FP64 partial: error-free wave merge, FP64 across waves in order
 template <int BLOCK>
-__device__ __forceinline__ void finish_tile(float s, float c, unsigned t, uint32_t ti, uint32_t ntiles_of_table,
-                                            uint32_t first_tile, unsigned* done, double* __restrict__ tile_partial,
-                                            double* __restrict__ sums, double* wsum, bool* last) {
-  // block reduction (error-free wave merge, FP64 across waves in order)
+__device__ __forceinline__ void finish_tile(float s, float c, unsigned t, double* __restrict__ tile_partial,
+                                            double* wsum) {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1) {
     const float s2 = __shfl_xor(s, off, 64);
@@ -113,19 +109,24 @@ __device__ __forceinline__ void finish_tile(float s, float c, unsigned t, uint32
 #pragma unroll
     for (int w = 0; w < BLOCK / 64; ++w) v += wsum[w];
     tile_partial[t] = v;
-    __threadfence();
-    const unsigned prev = atomicAdd(&done[ti], 1u);
-    *last = (prev + 1 == ntiles_of_table);
   }
   __syncthreads();
-  if (*last && threadIdx.x == 0) {
-    __threadfence();
-    double v = 0.0;
-    for (uint32_t k = 0; k < ntiles_of_table; ++k)
-      v += __builtin_nontemporal_load(&tile_partial[first_tile + k]);
-    sums[ti] = v;
-  }
-  __syncthreads();
+}
+
+// sums[i] = the fixed-order sum of table i's tile partials (one wave per
+// table: lane l adds partials l, l + 64, ... in order, then a butterfly)
+__global__ __launch_bounds__(256) void batch_sums_k(const TableDesc* __restrict__ tables, int count,
+                                                    const double* __restrict__ tile_partial,
+                                                    double* __restrict__ sums) {
+  const int ti = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (ti >= count) return;  // uniform per wave; no barrier below
+  const uint32_t first = tables[ti].first_tile, nt = tables[ti].ntiles;
+  double v = 0.0;
+  for (uint32_t k = lane; k < nt; k += 64) v += tile_partial[first + k];
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+  if (lane == 0) sums[ti] = v;
 }
 
 // state[0]: the work queue head, state[1]: the number of tiles handed to the
@@ -135,11 +136,10 @@ __device__ __forceinline__ void finish_tile(float s, float c, unsigned t, uint32
 template <class P, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void batch_persistent_k(
     const TableDesc* __restrict__ tables, const uint32_t* __restrict__ tile_table,
-    uint32_t ntiles, uint32_t tile, unsigned* state, unsigned* done, uint32_t* __restrict__ slow,
-    double* __restrict__ tile_partial, double* __restrict__ sums) {
+    uint32_t ntiles, uint32_t tile, unsigned* state, uint32_t* __restrict__ slow,
+    double* __restrict__ tile_partial) {
   __shared__ unsigned cur;
   __shared__ double wsum[BLOCK / 64];
-  __shared__ bool last;
   for (;;) {
     if (threadIdx.x == 0) cur = atomicAdd(&state[0], 1u);
     __syncthreads();
@@ -163,7 +163,7 @@ __global__ __launch_bounds__(BLOCK) void batch_persistent_k(
       __syncthreads();
       continue;
     }
-    finish_tile<BLOCK>(s, c, t, ti, d.ntiles, d.first_tile, done, tile_partial, sums, wsum, &last);
+    finish_tile<BLOCK>(s, c, t, tile_partial, wsum);
   }
 }
 
@@ -171,10 +171,9 @@ __global__ __launch_bounds__(BLOCK) void batch_persistent_k(
 template <class P, int BLOCK>
 __global__ __launch_bounds__(BLOCK) void batch_slow_k(
     const TableDesc* __restrict__ tables, const uint32_t* __restrict__ tile_table,
-    uint32_t tile, const unsigned* state, unsigned* done, const uint32_t* __restrict__ slow,
-    double* __restrict__ tile_partial, double* __restrict__ sums) {
+    uint32_t tile, const unsigned* state, const uint32_t* __restrict__ slow,
+    double* __restrict__ tile_partial) {
   __shared__ double wsum[BLOCK / 64];
-  __shared__ bool last;
   const unsigned nslow = state[1];
   for (unsigned u = blockIdx.x; u < nslow; u += gridDim.x) {  // uniform across the block
     const unsigned t = slow[u];
@@ -185,7 +184,7 @@ __global__ __launch_bounds__(BLOCK) void batch_slow_k(
     const uint64_t full_end = start + ((end - start) / 8) * 8;
     float s = 0.0f, c = 0.0f;
     tile_sum<P, BLOCK, false>(d, start, full_end, end, s, c, nullptr);
-    finish_tile<BLOCK>(s, c, t, ti, d.ntiles, d.first_tile, done, tile_partial, sums, wsum, &last);
+    finish_tile<BLOCK>(s, c, t, tile_partial, wsum);
   }
 }
 
@@ -206,7 +205,6 @@ struct Batch {
   double* tile_partial = nullptr;
   uint32_t* slow = nullptr;     // tiles left to the slow kernel
   unsigned* state = nullptr;    // queue head, slow tile count
-  unsigned* done = nullptr;
   uint32_t ntiles = 0;
   size_t tile_cap = 0;
   int cus = 256;
@@ -232,7 +230,6 @@ static int layout(Batch* b) {
   }
   b->ntiles = (uint32_t)tt.size();
   if (!b->tables) NT2H_CHECK(hipMalloc(&b->tables, sizeof(TableDesc) * b->count), "batch alloc");
-  if (!b->done) NT2H_CHECK(hipMalloc(&b->done, sizeof(unsigned) * b->count), "batch alloc");
   if (!b->state) NT2H_CHECK(hipMalloc(&b->state, 2 * sizeof(unsigned)), "batch alloc");
   if (b->tile_cap < tt.size()) {
     if (b->tile_table) { NT2H_CHECK(hipFree(b->tile_table), "batch free"); b->tile_table = nullptr; }
@@ -255,15 +252,16 @@ static void launch_persistent(Batch* b, double* sums, hipStream_t s) {
   const unsigned grid = (unsigned)std::max(1, std::min<int>(b->cus * b->cfg.wg_per_cu, (int)b->ntiles));
   const uint32_t tile = (uint32_t)(BLOCK * b->cfg.ept);
   hipLaunchKernelGGL((batch_persistent_k<P, BLOCK>), dim3(grid), dim3(BLOCK), 0, s, b->tables,
-                     b->tile_table, b->ntiles, tile, b->state, b->done, b->slow, b->tile_partial, sums);
+                     b->tile_table, b->ntiles, tile, b->state, b->slow, b->tile_partial);
   const unsigned sgrid = (unsigned)std::max(1, std::min<int>(b->cus, (int)b->ntiles));
   hipLaunchKernelGGL((batch_slow_k<P, BLOCK>), dim3(sgrid), dim3(BLOCK), 0, s, b->tables, b->tile_table,
-                     tile, b->state, b->done, b->slow, b->tile_partial, sums);
+                     tile, b->state, b->slow, b->tile_partial);
+  hipLaunchKernelGGL(batch_sums_k, dim3((b->count + 3) / 4), dim3(256), 0, s, b->tables, b->count,
+                     b->tile_partial, sums);
 }
 
 static int enqueue(Batch* b, double* sums, hipStream_t s) {
-  const int rb = 256;
-  hipLaunchKernelGGL(batch_reset_k, dim3((b->count + rb - 1) / rb), dim3(rb), 0, s, b->state, b->done, b->count);
+  hipLaunchKernelGGL(batch_reset_k, dim3(1), dim3(64), 0, s, b->state);
   const bool fast = b->policy == NT2H_FAST;
   switch (b->cfg.block) {
     case 128: fast ? launch_persistent<fast_policy, 128>(b, sums, s) : launch_persistent<exact_policy, 128>(b, sums, s); break;
@@ -423,7 +421,6 @@ int nt2h_batch_destroy(nt2h_batch* hb) {
   (void)hipFree(b->tile_partial);
   (void)hipFree(b->slow);
   (void)hipFree(b->state);
-  (void)hipFree(b->done);
   delete b;
   return NT2H_OK;
 }

@@ -53,15 +53,11 @@ int get_scratch(size_t nblocks, Scratch* out, hipStream_t s) {
     if (sc.partial) {
       NT2H_CHECK(hipStreamSynchronize(s), "scratch sync");
       NT2H_CHECK(hipFree(sc.partial), "scratch free");
-      NT2H_CHECK(hipFree(sc.counter), "scratch free");
       sc.partial = nullptr;
-      sc.counter = nullptr;
+      sc.cap = 0;
     }
     const size_t cap = nblocks < kMaxFoldBlocks ? kMaxFoldBlocks : nblocks;
     NT2H_CHECK(hipMalloc(&sc.partial, cap * sizeof(double)), "scratch alloc");
-    NT2H_CHECK(hipMalloc(&sc.counter, sizeof(unsigned)), "scratch alloc");
-    NT2H_CHECK(hipMemsetAsync(sc.counter, 0, sizeof(unsigned), s), "scratch init");
-    NT2H_CHECK(hipStreamSynchronize(s), "scratch init");
     sc.cap = cap;
   }
   *out = sc;
@@ -389,39 +385,70 @@ __global__ __launch_bounds__(kBlock) void stream_copy_k(const v4f* __restrict__ 
 // ============================================================================
 // inner_fold_: sum(x) and sum(x.*y), compensated FP32 per lane
 // ============================================================================
-template <bool Dot>
-__global__ __launch_bounds__(kBlock) void fold_f32_k(const float* __restrict__ x,
-                                                     const float* __restrict__ y, size_t n,
-                                                     int vec, double* partial, unsigned* counter,
-                                                     double* out) {
+// Block b folds the contiguous range [b*per, min((b+1)*per, units)) of units
+// (float4s when every table is 16-byte aligned, else floats); each thread
+// strides through it by the block size with kFoldUnroll units of every table
+// loaded ahead of the dependent TwoSum chain, and the last block adds the
+// n % 4 trailing elements of the float4 form.  The block's compensated sum is
+// stored as partial[b]; fold_final_k adds the partials in a fixed order.
+constexpr int kFoldUnroll = 4;
+constexpr unsigned kFoldBlocks = 256;  // partials of one fold (at most)
+typedef float f32x4v __attribute__((ext_vector_type(4)));
+
+template <bool Dot, bool Vec>
+__global__ __launch_bounds__(kBlock) void fold_part_k(const float* __restrict__ x, const float* __restrict__ y,
+                                                      size_t n, size_t per, double* __restrict__ partial) {
   float s = 0.0f, c = 0.0f;
-  const size_t gtid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  size_t done = 0;
-  if (vec) {
-    const size_t n4 = n / 4;
-    const float4* x4 = reinterpret_cast<const float4*>(x);
-    const float4* y4 = reinterpret_cast<const float4*>(y);
-    for (size_t i = gtid; i < n4; i += stride) {
-      const float4 a = x4[i];
+  constexpr int U = Vec ? kFoldUnroll : 4 * kFoldUnroll;
+  const size_t units = Vec ? n / 4 : n;
+  const size_t lo = (size_t)blockIdx.x * per;
+  const size_t hi = lo + per < units ? lo + per : units;
+  size_t i = lo + threadIdx.x;
+  if constexpr (Vec) {
+    const f32x4v* x4 = reinterpret_cast<const f32x4v*>(x);
+    const f32x4v* y4 = reinterpret_cast<const f32x4v*>(y);
+    for (; i + (size_t)(U - 1) * kBlock < hi; i += (size_t)U * kBlock) {
+      f32x4v a[U], b[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        a[u] = __builtin_nontemporal_load(x4 + i + u * kBlock);
+        if (Dot) b[u] = __builtin_nontemporal_load(y4 + i + u * kBlock);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int l = 0; l < 4; ++l) comp_add(s, c, Dot ? a[u][l] * b[u][l] : a[u][l]);
+    }
+    for (; i < hi; i += kBlock) {
+      const f32x4v a = x4[i];
       if (Dot) {
-        const float4 b = y4[i];
-        comp_add(s, c, a.x * b.x);
-        comp_add(s, c, a.y * b.y);
-        comp_add(s, c, a.z * b.z);
-        comp_add(s, c, a.w * b.w);
+        const f32x4v b = y4[i];
+#pragma unroll
+        for (int l = 0; l < 4; ++l) comp_add(s, c, a[l] * b[l]);
       } else {
-        comp_add(s, c, a.x);
-        comp_add(s, c, a.y);
-        comp_add(s, c, a.z);
-        comp_add(s, c, a.w);
+#pragma unroll
+        for (int l = 0; l < 4; ++l) comp_add(s, c, a[l]);
       }
     }
-    done = n4 * 4;
+    if (blockIdx.x == gridDim.x - 1) {
+      const size_t t = units * 4 + threadIdx.x;
+      if (t < n) comp_add(s, c, Dot ? x[t] * y[t] : x[t]);
+    }
+  } else {
+    for (; i + (size_t)(U - 1) * kBlock < hi; i += (size_t)U * kBlock) {
+      float a[U], b[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        a[u] = __builtin_nontemporal_load(x + i + u * kBlock);
+        if (Dot) b[u] = __builtin_nontemporal_load(y + i + u * kBlock);
+      }
+#pragma unroll
+      for (int u = 0; u < U; ++u) comp_add(s, c, Dot ? a[u] * b[u] : a[u]);
+    }
+    for (; i < hi; i += kBlock) comp_add(s, c, Dot ? x[i] * y[i] : x[i]);
   }
-  for (size_t i = done + gtid; i < n; i += stride) comp_add(s, c, Dot ? x[i] * y[i] : x[i]);
   const double b = block_sum_comp(s, c);
-  grid_combine(b, partial, counter, out);
+  if (threadIdx.x == 0) partial[blockIdx.x] = b;
 }
 
 // ============================================================================
@@ -773,11 +800,11 @@ __global__ __launch_bounds__(kBlock) void expr4_k(const float* __restrict__ x,
   }
 }
 
+// per-block compensated partials of sum(z), added by fold_final_k
 template <class P>
 __global__ __launch_bounds__(kBlock) void expr4_sum_k(const float* __restrict__ x,
                                                       const float* __restrict__ y, size_t n,
-                                                      double* partial, unsigned* counter,
-                                                      double* out) {
+                                                      double* __restrict__ partial) {
   float s = 0.0f, c = 0.0f;
   const size_t npacks = n / 8;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -798,7 +825,7 @@ __global__ __launch_bounds__(kBlock) void expr4_sum_k(const float* __restrict__ 
     }
   }
   const double bsum = block_sum_comp(s, c);
-  grid_combine(bsum, partial, counter, out);
+  if (threadIdx.x == 0) partial[blockIdx.x] = bsum;
 }
 
 // ---- transform_ ---------------------------------------------------------
@@ -1075,16 +1102,48 @@ static unsigned fold_grid(size_t work_items) {
   return (unsigned)g;
 }
 
-static int launch_fold(size_t n, const float* x, const float* y, double* out, hipStream_t s) {
-  const int vec = aligned16(x) && (!y || aligned16(y));
-  const unsigned g = fold_grid(vec ? (n + 3) / 4 : n);
-  Scratch sc;
-  int rc = get_scratch(g, &sc, s);
-  if (rc) return rc;
-  if (y) hipLaunchKernelGGL(fold_f32_k<true>, dim3(g), dim3(kBlock), 0, s, x, y, n, vec, sc.partial, sc.counter, out);
-  else hipLaunchKernelGGL(fold_f32_k<false>, dim3(g), dim3(kBlock), 0, s, x, y, n, vec, sc.partial, sc.counter, out);
+// fold geometry: at most kFoldBlocks blocks, each a contiguous range of a
+// multiple of kBlock units, at least kBlock*kFoldUnroll units per block when
+// there are that many
+static void fold_geometry(size_t units, unsigned* g, size_t* per) {
+  const size_t chunk = (size_t)kBlock * kFoldUnroll;
+  size_t gg = (units + chunk - 1) / chunk;
+  if (gg < 1) gg = 1;
+  if (gg > kFoldBlocks) gg = kFoldBlocks;
+  size_t p = (units + gg - 1) / gg;
+  p = (p + kBlock - 1) / kBlock * kBlock;
+  if (p == 0) p = kBlock;
+  gg = (units + p - 1) / p;
+  if (gg < 1) gg = 1;
+  *g = (unsigned)gg;
+  *per = p;
+}
+
+// the two launches of a fold into the partials buffer (kFoldBlocks doubles)
+static int enqueue_fold(size_t n, const float* x, const float* y, double* partial, double* out, hipStream_t s) {
+  const bool vec = aligned16(x) && (!y || aligned16(y));
+  unsigned g;
+  size_t per;
+  fold_geometry(vec ? n / 4 : n, &g, &per);
+  const dim3 grid(g), block(kBlock);
+  if (y) {
+    if (vec) hipLaunchKernelGGL((fold_part_k<true, true>), grid, block, 0, s, x, y, n, per, partial);
+    else hipLaunchKernelGGL((fold_part_k<true, false>), grid, block, 0, s, x, y, n, per, partial);
+  } else {
+    if (vec) hipLaunchKernelGGL((fold_part_k<false, true>), grid, block, 0, s, x, y, n, per, partial);
+    else hipLaunchKernelGGL((fold_part_k<false, false>), grid, block, 0, s, x, y, n, per, partial);
+  }
   NT2H_CHECK_LAUNCH("fold_f32");
+  hipLaunchKernelGGL(fold_final_k, dim3(1), block, 0, s, partial, g, out);
+  NT2H_CHECK_LAUNCH("fold_f32 final");
   return NT2H_OK;
+}
+
+static int launch_fold(size_t n, const float* x, const float* y, double* out, hipStream_t s) {
+  Scratch sc;
+  int rc = get_scratch(kFoldBlocks, &sc, s);
+  if (rc) return rc;
+  return enqueue_fold(n, x, y, sc.partial, out, s);
 }
 
 int nt2h_sum_f32(size_t n, const float* x, double* out, void* stream) {
@@ -1254,10 +1313,12 @@ int nt2h_expr4_sum_f32(size_t n, const float* x, const float* y, double* out, in
   Scratch sc;
   int rc = get_scratch(g, &sc, s);
   if (rc) return rc;
-  if (policy == NT2H_EXACT) hipLaunchKernelGGL(expr4_sum_k<exact_policy>, dim3(g), dim3(kBlock), 0, s, x, y, n, sc.partial, sc.counter, out);
-  else if (policy == NT2H_FAST) hipLaunchKernelGGL(expr4_sum_k<fast_policy>, dim3(g), dim3(kBlock), 0, s, x, y, n, sc.partial, sc.counter, out);
+  if (policy == NT2H_EXACT) hipLaunchKernelGGL(expr4_sum_k<exact_policy>, dim3(g), dim3(kBlock), 0, s, x, y, n, sc.partial);
+  else if (policy == NT2H_FAST) hipLaunchKernelGGL(expr4_sum_k<fast_policy>, dim3(g), dim3(kBlock), 0, s, x, y, n, sc.partial);
   else return fail(NT2H_EINVAL, "nt2h_expr4_sum_f32: bad policy");
   NT2H_CHECK_LAUNCH("expr4_sum_f32");
+  hipLaunchKernelGGL(fold_final_k, dim3(1), dim3(kBlock), 0, s, sc.partial, g, out);
+  NT2H_CHECK_LAUNCH("expr4_sum_f32 final");
   return NT2H_OK;
 }
 

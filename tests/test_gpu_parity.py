@@ -189,8 +189,11 @@ def test_map_f32_fast_within_bound(op):
     rng = np.random.default_rng(300 + F32_OPS.index(op))
     x = _inputs_f32(op, rng, 40000)
     if op in ("cos", "sin", "tan"):
+        # the band [1e4, 2^18 pi) of NT2's f64 reduction for f32 inputs, uniform and log-uniform
+        band = np.concatenate([rng.uniform(1e4, 823549, 10000),
+                               np.exp2(rng.uniform(np.log2(1e4), np.log2(823549), 10000))])
+        x = np.concatenate([x, (band * rng.choice([-1, 1], band.size)).astype(np.float32)])
         x = x[np.abs(x) < 823549]  # beyond: fdlibm path, identical in both policies
-        x = x[np.abs(x) < 1e4]
     say("map_f32 fast", op, "n=%d" % x.size)
     got = ops.map(op, torch.from_numpy(x).cuda(), FAST).cpu().numpy()
     truth = _truth_f32(op, x)
@@ -245,7 +248,7 @@ def test_saxpy_bitwise(n):
     assert same_bits(y.cpu().numpy(), O.saxpy(2.5, x_h, y_h))
 
 
-@pytest.mark.parametrize("n", [1 << 20, (1 << 20) + 3, 1, 1000, 1 << 24])
+@pytest.mark.parametrize("n", [1 << 20, (1 << 20) + 3, 1, 5, 1000, 4095, 1 << 24, (1 << 26) + 1])
 def test_dot_and_sum(n):
     say("dot/sum n=%d" % n)
     x = ops.fill_uniform(n, -1.0, 1.0, 1)
@@ -263,6 +266,24 @@ def test_dot_and_sum(n):
     s = ops.sum(x).item()
     sx = O.sum_f32_exact(xh)
     assert abs(s - sx) <= 2 ** -40 * O.sum_f32_exact(np.abs(xh)) + 1e-30
+
+
+@pytest.mark.parametrize("n", [3, 1 << 12, (1 << 20) + 7])
+def test_dot_and_sum_unaligned(n):
+    """Tables that start off a 16-byte boundary take the float path of the
+    fold (no float4 loads); same bound and determinism as the aligned form."""
+    say("dot/sum unaligned n=%d" % n)
+    xa = ops.fill_uniform(n + 1, -1.0, 1.0, 3)
+    ya = ops.fill_uniform(n + 2, -1.0, 1.0, 13)
+    x, y = xa[1:], ya[2:]
+    xh, yh = x.cpu().numpy(), y.cpu().numpy()
+    p = (xh * yh).astype(np.float32)
+    d = ops.dot(x, y).item()
+    ex, mag = O.sum_f32_exact(p), O.sum_f32_exact(np.abs(p))
+    assert abs(d - ex) <= 2 ** -40 * mag + 1e-30, (d, ex, mag)
+    assert ops.dot(x, y).item() == d
+    s = ops.sum(y).item()
+    assert abs(s - O.sum_f32_exact(yh)) <= 2 ** -40 * O.sum_f32_exact(np.abs(yh)) + 1e-30
 
 
 def test_fold_kats():
@@ -386,10 +407,10 @@ def test_batch_persistent_matches_plain_and_oracle(huge):
     rng = np.random.default_rng(4)
     count = 64
     lens = np.floor(np.exp(rng.uniform(np.log(1e3), np.log(1e5), count))).astype(np.int64)
-    lens[0], lens[1] = 1, 7  # ragged: pure scalar tails
-    xs = [ops.fill_normal(int(n), 1000 + i) for i, n in enumerate(lens)]
+    lens[0], lens[1], lens[2] = 1, 7, 0  # ragged: pure scalar tails; an empty table sums to 0
+    xs = [ops.fill_normal(max(int(n), 1), 1000 + i) for i, n in enumerate(lens)]
     ymax = [1e7 if huge and i % 4 == 1 else 1e4 for i in range(count)]
-    ys = [ops.fill_uniform(int(n), -ymax[i], ymax[i], 2000 + i) for i, n in enumerate(lens)]
+    ys = [ops.fill_uniform(max(int(n), 1), -ymax[i], ymax[i], 2000 + i) for i, n in enumerate(lens)]
     say("batch count=%d total=%d huge=%s" % (count, int(lens.sum()), huge))
     XP = (ctypes.c_void_p * count)(*[t.data_ptr() for t in xs])
     YP = (ctypes.c_void_p * count)(*[t.data_ptr() for t in ys])
@@ -397,8 +418,8 @@ def test_batch_persistent_matches_plain_and_oracle(huge):
     b = ctypes.c_void_p()
     check(lib().nt2h_batch_create(ctypes.byref(b), count, XP, YP, LN, EXACT), "batch_create")
     try:
-        sums = torch.zeros(count, dtype=torch.float64, device="cuda")
-        plain = torch.zeros(count, dtype=torch.float64, device="cuda")
+        sums = torch.full((count,), 7.0, dtype=torch.float64, device="cuda")
+        plain = torch.full((count,), 7.0, dtype=torch.float64, device="cuda")
         check(lib().nt2h_batch_set_config(b, 256, 16, 4), "set_config")
         check(lib().nt2h_batch_run(b, ptr(sums), stream_handle()), "batch_run")
         first = sums.clone()
@@ -409,8 +430,10 @@ def test_batch_persistent_matches_plain_and_oracle(huge):
             assert torch.equal(sums, first)
         check(lib().nt2h_batch_run_plain(b, ptr(plain), stream_handle()), "batch_plain")
         torch.cuda.synchronize()
+        assert first[2].item() == 0.0 and plain[2].item() == 0.0  # the empty table
         for i in range(count):
-            z = O.expr4(xs[i].cpu().numpy(), ys[i].cpu().numpy())
+            n = int(lens[i])
+            z = O.expr4(xs[i][:n].cpu().numpy(), ys[i][:n].cpu().numpy())
             ex = O.sum_f32_exact(z)
             mag = O.sum_f32_exact(np.abs(z))
             assert abs(first[i].item() - ex) <= 2 ** -40 * mag + 1e-30, (i, first[i].item(), ex)
@@ -423,6 +446,40 @@ def test_batch_persistent_matches_plain_and_oracle(huge):
         assert float(rel.max()) <= 2 ** -38
     finally:
         lib().nt2h_batch_destroy(b)
+
+
+def test_batch_full_size():
+    """The batched workload at its configured size: 4096 tables with lengths
+    log-uniform in [1e3, 1e6] (about 5.9e8 elements, the generator of the
+    benchmark), x ~ N(0,1), y ~ U[-1e4, 1e4].  Exact policy, persistent graph
+    launch: 64 seeded tables are recomputed by the oracle and each sum is
+    within 2^-40 of the exact sum of the oracle's f32 terms; the fast policy's
+    sums stay within 1e-5 of the exact policy's; replays are bit-identical."""
+    from nt2_amd.batch import Batch, new_sums, uneven_tables
+    count = 4096
+    x, y, offs, lens = uneven_tables(count, seed=4)
+    say("batch full size count=%d total=%d" % (count, int(lens.sum())))
+    sums = new_sums(count)
+    with Batch(x, y, offs, lens, EXACT) as b:
+        b.run(sums)
+        torch.cuda.synchronize()
+        first = sums.cpu().numpy()
+        b.run(sums)
+        torch.cuda.synchronize()
+        assert np.array_equal(sums.cpu().numpy().view(np.uint64), first.view(np.uint64))
+    with Batch(x, y, offs, lens, FAST) as b:
+        b.run(sums)
+        torch.cuda.synchronize()
+        fast = sums.cpu().numpy()
+    assert float(np.max(np.abs(fast - first) / np.maximum(np.abs(first), 1e-30))) <= 1e-5
+    pick = np.random.default_rng(44).choice(count, 64, replace=False)
+    for i in pick:
+        o, n = int(offs[i]), int(lens[i])
+        z = O.expr4(x[o:o + n].cpu().numpy(), y[o:o + n].cpu().numpy())
+        ex, mag = O.sum_f32_exact(z), O.sum_f32_exact(np.abs(z))
+        assert abs(first[i] - ex) <= 2 ** -40 * mag + 1e-30, (int(i), first[i], ex)
+    del x, y
+    torch.cuda.empty_cache()
 
 
 @pytest.mark.parametrize("r,v,tmax", [(0.02, 0.30, 10.0), (0.09, 0.30, 10.0), (-0.012, 0.45, 30.0)])
