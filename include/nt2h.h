@@ -143,6 +143,14 @@ int nt2h_saxpy_f32(size_t n, float a, const float* x, float* y, void* stream);
  * results are deterministic. */
 int nt2h_sum_f32(size_t n, const float* x, double* out, void* stream);
 int nt2h_dot_f32(size_t n, const float* x, const float* y, double* out, void* stream);
+/* nt2h_dot_f32 for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops): no counterpart in the reference.
+ * Same kernels and bits as nt2h_dot_f32; the block partials live in a static
+ * device array of 256 doubles per device, looked up on the first call on a
+ * device, and no hipMalloc, hipFree or memset is made.  The fold never needs
+ * more than those 256 partials, whatever n (NT2H_EINVAL otherwise).  Calls
+ * that overlap on one device share the array and race: serialize them. */
+int nt2h_dot_f32_noalloc(size_t n, const float* x, const float* y, double* out, void* stream);
 
 /* ---- sum(a, dim) and cumsum(a, dim) over an N-D table -------------------- */
 /* x: device table of extent ext[0..3] (host array; column-major, trailing

@@ -886,6 +886,32 @@ static int launch_map_f64(int op, size_t n, const double* x, double* y, hipStrea
   return NT2H_OK;
 }
 
+// ---- static scratch of nt2h_dot_f32_noalloc ---------------------------------
+// 256 FP64 block partials (2 KiB) per device, held until the process ends.
+constexpr unsigned kScratchDoubles = 256;
+static_assert(kScratchDoubles >= kFoldBlocks, "the fold's partials must fit the static scratch");
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_sym_mu;
+static double* g_scratch_sym[kMaxDevices];
+
+// the current device's address of `scratch`, looked up on its first use
+static int scratch_ptr(double** out) {
+  int dev = 0;
+  NT2H_CHECK(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices) return fail(NT2H_EINVAL, "nt2h_dot_f32_noalloc: device index beyond 64");
+  std::lock_guard<std::mutex> lk(g_scratch_sym_mu);
+  if (!g_scratch_sym[dev]) {
+    void* p = nullptr;
+    NT2H_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!p) return fail(NT2H_EHIP, "hipGetSymbolAddress(scratch): null address");
+    g_scratch_sym[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_sym[dev];
+  return NT2H_OK;
+}
+
 }  // namespace nt2h
 
 using namespace nt2h;
@@ -1156,6 +1182,24 @@ int nt2h_dot_f32(size_t n, const float* x, const float* y, double* out, void* st
   if (!out || (n && (!x || !y))) return fail(NT2H_EINVAL, "nt2h_dot_f32: null pointer");
   if (n == 0) return hipMemsetAsync(out, 0, sizeof(double), as_stream(stream)) == hipSuccess ? NT2H_OK : NT2H_EHIP;
   return launch_fold(n, x, y, out, as_stream(stream));
+}
+
+// nt2h_dot_f32 without allocation: the fold's block partials live in the
+// file-scope array `scratch` (one copy per device), whose device address is
+// looked up once per device with hipGetSymbolAddress (the array's name is
+// only a host-side stand-in and must never be passed to a kernel).  The fold
+// writes every partial it reads, so nothing has to be cleared between calls;
+// calls that overlap on one device share the array and race.
+int nt2h_dot_f32_noalloc(size_t n, const float* x, const float* y, double* out, void* stream) {
+  if (!out || (n && (!x || !y))) return fail(NT2H_EINVAL, "nt2h_dot_f32_noalloc: null pointer");
+  unsigned g;
+  size_t per;
+  fold_geometry(aligned16(x) && aligned16(y) ? n / 4 : n, &g, &per);
+  if (g > kScratchDoubles) return fail(NT2H_EINVAL, "nt2h_dot_f32_noalloc: more block partials than the static scratch holds");
+  double* partial = nullptr;
+  if (int rc = scratch_ptr(&partial)) return rc;
+  // n == 0 runs the fold too (one block, no element read): it writes 0 without a memset node
+  return enqueue_fold(n, x, y, partial, out, as_stream(stream));
 }
 
 // ---- Black-Scholes ----------------------------------------------------------

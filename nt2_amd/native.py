@@ -61,6 +61,7 @@ def lib():
         "nt2h_stream_copy": ([P, P, S, I, P], I),
         "nt2h_sum_f32": ([S, P, P, P], I),
         "nt2h_dot_f32": ([S, P, P, P, P], I),
+        "nt2h_dot_f32_noalloc": ([S, P, P, P, P], I),
         "nt2h_blackscholes_f32": ([S, P, P, P, F, F, P, P, I, I, P], I),
         "nt2h_blackscholes_f64": ([S, P, P, P, D, D, P, P, I, P], I),
         "nt2h_bs_set_grid": ([I], I),
