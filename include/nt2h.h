@@ -40,8 +40,9 @@ extern "C" {
 /* evaluation policy of floating-point kernels */
 #define NT2H_EXACT 0 /* NT2's arithmetic: fma = a*b+c with two roundings; bit-identical to the CPU evaluation */
 #define NT2H_FAST 1  /* NT2's fma call sites fused in hardware; within each function's ULP bound */
-#define NT2H_ACCURATE 2 /* Black-Scholes f32 only: FP64 internal evaluation with this library's
-                           own approximations; within a few ulp of the exact price */
+#define NT2H_ACCURATE 2 /* Black-Scholes only: f32 tables with an FP64 internal evaluation, f64
+                           tables with double-double tails, both with this library's own
+                           approximations; within 4 ulp of the exact price */
 
 /* elementwise operations for nt2h_map_* (NT2 functors exp_, log_, sqrt_, erfc_,
  * normcdf_, fastnormcdf_, cos_, sin_, abs_, exp2_ ... atan_, tan_, asin_, acos_) */
@@ -211,6 +212,11 @@ int nt2h_bs_set_grid(int workgroups);
  * 4; results do not depend on it (no counterpart in the reference: a tuning
  * hook) */
 int nt2h_sigmadelta_set_config(int pixels_per_thread);
+/* f64 tables: policy NT2H_EXACT / NT2H_FAST run NT2's double-precision
+ * evaluation (demo/blackscholes/nt2/blackandscholes_nt2.cpp:53-60 with the
+ * erfc-based normcdf, modules/core/statistics/include/nt2/statistics/
+ * functions/generic/normcdf.hpp:28-35); NT2H_ACCURATE prices every option
+ * within 4 ulp of the exact price (S, X, T, v > 0). */
 int nt2h_blackscholes_f64(size_t n, const double* S, const double* X, const double* T,
                           double r, double v, double* call, double* put, int policy,
                           void* stream);
@@ -267,7 +273,7 @@ typedef struct {
   size_t budget_bytes; /* bytes of admitted requests' tables (5*n*sizeof(T) each) */
   int max_inflight;    /* concurrent batched launches (streams), 1..16 */
   int max_batch;       /* requests per launch; 1 with max_inflight 1 = plain schedule */
-  int policy;          /* NT2H_EXACT, NT2H_FAST, or NT2H_ACCURATE (f32 requests; f64 ones exact) */
+  int policy;          /* NT2H_EXACT, NT2H_FAST, or NT2H_ACCURATE (f32 and f64 requests) */
   int backend;         /* NT2H_BACKEND_HIP or NT2H_BACKEND_HOST */
   int fail_alloc_every; /* test hook: fail every k-th output allocation (0 = never) */
 } nt2h_server_config;

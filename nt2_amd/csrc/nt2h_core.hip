@@ -12,6 +12,7 @@
 #include "bs_device.hpp"
 #include "common.hpp"
 #include "nt2/hip/bs_accurate.hpp"
+#include "nt2/hip/bs_accurate64.hpp"
 #include "nt2/hip/math.hpp"
 #include "nt2h.h"
 
@@ -488,6 +489,33 @@ __global__ __launch_bounds__(kBlock) void bs_acc_f32_k(I n4, size_t n, const flo
   const size_t sstride = (size_t)gridDim.x * blockDim.x;
   for (size_t i = (size_t)n4 * 4 + (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += sstride)
     ba::bs_accurate(S[i], X[i], T[i], prm, tab, ex2, call[i], put[i]);
+}
+
+// accurate f64 policy: grid-stride over options, one per thread and
+// iteration (a wave reads 512 contiguous bytes of each table; two options per
+// thread made the compiler interleave them at 242 VGPRs, 2 waves per SIMD),
+// tables in LDS
+template <class I>
+__global__ __launch_bounds__(kBlock) void bs_acc_f64_k(I n, const double* __restrict__ S,
+                                                       const double* __restrict__ X,
+                                                       const double* __restrict__ T,
+                                                       nt2::hip::bsacc64::Params prm,
+                                                       double* __restrict__ call,
+                                                       double* __restrict__ put) {
+  namespace bb = nt2::hip::bsacc64;
+  __shared__ double tab[bb::kTabDoubles];
+  bb::load_tables(tab);
+  const double* gt = tab;
+  const double* ex2h = tab + bb::kGRows * bb::kGN;
+  const double* ex2l = ex2h + 32;
+  const I stride = (I)gridDim.x * blockDim.x;
+  for (I i = (I)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    double c, p;
+    bb::bs_price(__builtin_nontemporal_load(S + i), __builtin_nontemporal_load(X + i),
+                 __builtin_nontemporal_load(T + i), prm, gt, ex2h, ex2l, c, p);
+    __builtin_nontemporal_store(c, call + i);
+    __builtin_nontemporal_store(p, put + i);
+  }
 }
 
 template <class P, int MODE>
@@ -1255,7 +1283,14 @@ int nt2h_blackscholes_f64(size_t n, const double* S, const double* X, const doub
   const int vec = aligned16(S) && aligned16(X) && aligned16(T) && aligned16(call) && aligned16(put);
   const dim3 g(grid_for(n / 4 + n % 4)), b(kBlock);
   hipStream_t s = as_stream(stream);
-  if (policy == NT2H_EXACT) hipLaunchKernelGGL(bs_f64_k<exact_policy>, g, b, 0, s, n, S, X, T, r, v, call, put, vec);
+  if (policy == NT2H_ACCURATE) {
+    const unsigned ga = std::min<unsigned>(grid_for(n), g_acc_grid);
+    const auto prm = nt2::hip::bsacc64::make_params(r, v);
+    if (n < (size_t(1) << 31))
+      hipLaunchKernelGGL(bs_acc_f64_k<uint32_t>, dim3(ga), b, 0, s, (uint32_t)n, S, X, T, prm, call, put);
+    else
+      hipLaunchKernelGGL(bs_acc_f64_k<size_t>, dim3(ga), b, 0, s, n, S, X, T, prm, call, put);
+  } else if (policy == NT2H_EXACT) hipLaunchKernelGGL(bs_f64_k<exact_policy>, g, b, 0, s, n, S, X, T, r, v, call, put, vec);
   else if (policy == NT2H_FAST) hipLaunchKernelGGL(bs_f64_k<fast_policy>, g, b, 0, s, n, S, X, T, r, v, call, put, vec);
   else return fail(NT2H_EINVAL, "nt2h_blackscholes_f64: bad policy");
   NT2H_CHECK_LAUNCH("blackscholes_f64");

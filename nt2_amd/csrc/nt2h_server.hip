@@ -39,6 +39,7 @@
 
 #include "bs_device.hpp"
 #include "nt2/hip/bs_accurate.hpp"
+#include "nt2/hip/bs_accurate64.hpp"
 #include "common.hpp"
 #include "nt2h.h"
 
@@ -81,18 +82,20 @@ typedef float srv_f32x4 __attribute__((ext_vector_type(4)));
 // of kSrvTile (a request's tiles are consecutive, starting at tile0);
 // workgroup k walks the contiguous tile range [k tpb, (k+1) tpb), so the
 // owning request is found by one binary search and then advances linearly.
-// P: the NT2 policy of the f64 requests and (ACC false) of the f32 ones;
-// ACC: f32 requests use the accurate FP64-internal evaluation; F64: the
-// group's precision.
+// P: the NT2 policy of the requests when ACC is false; ACC: the accurate
+// evaluations (FP64-internal for f32 requests, double-double tails for f64
+// ones); F64: the group's precision.
 template <class P, bool ACC, bool F64>
 __global__ __launch_bounds__(kSrvBlock) void serve_bs_k(const ReqDesc* __restrict__ reqs, int nreq,
                                                       uint32_t ntiles, uint32_t tpb) {
   _Pragma("clang fp contract(off)")
   namespace ba = nt2::hip::bsacc;
-  constexpr bool kTables = ACC && !F64;
-  __shared__ double tab[kTables ? (ba::kTailDeg + 1) * ba::kTailN : 1];
-  __shared__ double ex2[kTables ? 32 : 1];
-  if (kTables) ba::load_tables(tab, ex2);
+  namespace bb = nt2::hip::bsacc64;
+  constexpr bool kTables32 = ACC && !F64, kTables64 = ACC && F64;
+  __shared__ double tab[kTables32 ? (ba::kTailDeg + 1) * ba::kTailN : (kTables64 ? bb::kTabDoubles : 1)];
+  __shared__ double ex2[kTables32 ? 32 : 1];
+  if (kTables32) ba::load_tables(tab, ex2);
+  if (kTables64) bb::load_tables(tab);
   const uint32_t b0 = blockIdx.x * tpb;
   const uint32_t b1 = min(b0 + tpb, ntiles);
   if (b0 >= b1) return;
@@ -149,16 +152,27 @@ __global__ __launch_bounds__(kSrvBlock) void serve_bs_k(const ReqDesc* __restric
       double* C = (double*)d.call;
       double* Pt = (double*)d.put;
       const double r = d.r, v = d.v;
-      const double drift = (v * v) * 0.5 + r, mr = -r;
-      // NT2 packs of 4 from the request's first option; its last n%4 options
-      // take NT2's scalar erfc
-      const uint64_t npk = d.n / 4 * 4;
-      for (uint64_t i = base + threadIdx.x; i < base + kSrvTile && i < d.n; i += kSrvBlock) {
-        double c, p;
-        if (i < npk) bs_one_f64<P, false>(S[i], X[i], T[i], mr, v, drift, c, p);
-        else bs_one_f64<P, true>(S[i], X[i], T[i], mr, v, drift, c, p);
-        C[i] = c;
-        Pt[i] = p;
+      if constexpr (ACC) {
+        const bb::Params prm = bb::make_params(r, v);
+        const double* ex2h = tab + bb::kGRows * bb::kGN;
+        for (uint64_t i = base + threadIdx.x; i < base + kSrvTile && i < d.n; i += kSrvBlock) {
+          double c, p;
+          bb::bs_price(S[i], X[i], T[i], prm, tab, ex2h, ex2h + 32, c, p);
+          C[i] = c;
+          Pt[i] = p;
+        }
+      } else {
+        const double drift = (v * v) * 0.5 + r, mr = -r;
+        // NT2 packs of 4 from the request's first option; its last n%4 options
+        // take NT2's scalar erfc
+        const uint64_t npk = d.n / 4 * 4;
+        for (uint64_t i = base + threadIdx.x; i < base + kSrvTile && i < d.n; i += kSrvBlock) {
+          double c, p;
+          if (i < npk) bs_one_f64<P, false>(S[i], X[i], T[i], mr, v, drift, c, p);
+          else bs_one_f64<P, true>(S[i], X[i], T[i], mr, v, drift, c, p);
+          C[i] = c;
+          Pt[i] = p;
+        }
       }
     }
   }

@@ -25,6 +25,7 @@
 #include <algorithm>
 #include <limits>
 #include <omp.h>
+#include <quadmath.h>
 #include <unistd.h>
 
 #include "pio2_tables.h"
@@ -1927,6 +1928,39 @@ int oracle_bs_fp64ref_f64in(size_t n, const double* S, const double* X, const do
     const double e = x * std::exp(-r * t);
     call[j] = s * (0.5 * std::erfc(-d1 / std::sqrt(2.0))) - e * (0.5 * std::erfc(-d2 / std::sqrt(2.0)));
     put[j] = e * (0.5 * std::erfc(d2 / std::sqrt(2.0))) - s * (0.5 * std::erfc(d1 / std::sqrt(2.0)));
+  }
+  return 0;
+}
+
+// Quad-precision (113-bit, libquadmath) Black-Scholes reference for f64
+// prices, returned as the double-double (hi, lo) of the quad value.  The
+// textbook formula (the same as oracle_bs_fp64ref_*) with the out-of-the-money
+// side evaluated from normal tails, N(-|d|) = erfc(|d|/sqrt2)/2, and the other
+// side by put-call parity, so the two tails cancel by at most a few hundred
+// times against 113 bits: the result is good to ~2^-100 relative.
+static inline __float128 qncdf(__float128 x) {
+  return (__float128)0.5 * erfcq(-x / sqrtq((__float128)2));
+}
+int oracle_bs_quad_f64in(size_t n, const double* S, const double* X, const double* T, double r,
+                         double v, double* call_hi, double* call_lo, double* put_hi, double* put_lo) {
+#pragma omp parallel for schedule(dynamic, 1024)
+  for (size_t j = 0; j < n; ++j) {
+    const __float128 s = S[j], x = X[j], t = T[j], rr = r, vv = v;
+    const __float128 tau = vv * sqrtq(t);
+    const __float128 d1 = (logq(s / x) + (rr + vv * vv / 2) * t) / tau, d2 = d1 - tau;
+    const __float128 F = x * expq(-rr * t);
+    __float128 c, p;
+    if (d1 < 0) {
+      c = s * qncdf(d1) - F * qncdf(d2);
+      p = c + (F - s);
+    } else {
+      p = F * qncdf(-d2) - s * qncdf(-d1);
+      c = p + (s - F);
+    }
+    call_hi[j] = (double)c;
+    call_lo[j] = (double)(c - (__float128)call_hi[j]);
+    put_hi[j] = (double)p;
+    put_lo[j] = (double)(p - (__float128)put_hi[j]);
   }
   return 0;
 }

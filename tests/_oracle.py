@@ -45,6 +45,7 @@ def lib():
         _lib.oracle_bs_f64.argtypes = [S, P, P, P, ctypes.c_double, ctypes.c_double, P, P]
         _lib.oracle_bs_fp64ref_f32in.argtypes = [S, P, P, P, ctypes.c_double, ctypes.c_double, P, P]
         _lib.oracle_bs_fp64ref_f64in.argtypes = [S, P, P, P, ctypes.c_double, ctypes.c_double, P, P]
+        _lib.oracle_bs_quad_f64in.argtypes = [S, P, P, P, ctypes.c_double, ctypes.c_double, P, P, P, P]
         _lib.oracle_u8_binop.argtypes = [ctypes.c_int, P, P, P, S]
         _lib.oracle_sigmadelta_u8.argtypes = [S, ctypes.c_int, P, S, P, P, P, ctypes.c_int, P]
         _lib.oracle_expr4_f32.argtypes = [P, P, P, S]
@@ -162,6 +163,15 @@ def bs_fp64ref(S, X, T, r, v):
     p = np.zeros(S.size)
     assert f(S.size, _p(S), _p(X), _p(T), r, v, _p(c), _p(p)) == 0
     return c, p
+
+
+def bs_quad(S, X, T, r, v):
+    """Quad-precision Black-Scholes of f64 options: (call_hi, call_lo,
+    put_hi, put_lo), each price the double-double hi + lo."""
+    S, X, T = (np.ascontiguousarray(a, dtype=np.float64) for a in (S, X, T))
+    out = [np.zeros(S.size) for _ in range(4)]
+    assert lib().oracle_bs_quad_f64in(S.size, _p(S), _p(X), _p(T), r, v, *(_p(o) for o in out)) == 0
+    return tuple(out)
 
 
 def u8_binop(op, a, b):

@@ -44,3 +44,20 @@ def nt2_ulpdist(a0, a1):
         e = abs(np.ldexp(a0, expo) - np.ldexp(a1, expo))
     eps = np.finfo(t).eps
     return float(e / eps)
+
+
+def dd_ulp(a, hi, lo):
+    """|a - (hi + lo)| in units of the float64 spacing at hi, for a reference
+    price given as the double-double hi + lo (a - hi is exact when a is
+    within a few ulp of hi)."""
+    a = np.asarray(a, dtype=np.float64)
+    hi = np.asarray(hi, dtype=np.float64)
+    lo = np.asarray(lo, dtype=np.float64)
+    info = np.finfo(np.float64)
+    mag = np.abs(hi)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        e = np.floor(np.log2(np.where(mag > 0, mag, info.tiny)))
+    e = np.maximum(e, np.log2(info.tiny))
+    spacing = np.exp2(e - info.nmant)
+    err = np.abs((a - hi) - lo) / spacing
+    return np.where(np.isnan(err), np.inf, err)

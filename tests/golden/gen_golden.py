@@ -8,6 +8,10 @@
   arguments for the Payne-Hanek path of cos/sin).
 * bs_golden.npz — Black-Scholes call/put in mpmath for 512 options drawn
   from the benchmark distribution (f32 inputs promoted exactly).
+* bs64_golden.npz — Black-Scholes call/put of 480 double-precision options
+  in mpmath, each price stored as the double-double hi + lo: the benchmark
+  distribution, two other rate/volatility pairs, short maturities (deep
+  out-of-the-money tails down to ~1e-250) and near-the-money options.
 * kats.json — the known-answer tests of NT2's own unit tests for this path,
   transcribed as data (input, expected, tolerance in NT2 ulpdist units, the
   file:line of the reference test).
@@ -176,6 +180,43 @@ def build_bs():
              v=np.float64(0.30), call=call, put=put)
 
 
+def _dd(v):
+    hi = float(v)
+    return hi, float(v - mpmath.mpf(hi))
+
+
+def build_bs64():
+    rng = np.random.default_rng(64)
+    groups = [  # (count, r, v, S range, X range, T range)
+        (160, 0.02, 0.30, (5, 30), (1, 100), (0.25, 10)),
+        (80, 0.09, 0.30, (5, 30), (1, 100), (0.25, 10)),
+        (80, -0.012, 0.45, (5, 30), (1, 100), (0.25, 30)),
+        (80, 0.02, 0.30, (5, 30), (1, 100), (0.002, 0.25)),
+        (80, 0.02, 0.30, (10, 20), (9.5, 21), (0.25, 10)),
+    ]
+    cols = {k: [] for k in ("S", "X", "T", "r", "v", "call_hi", "call_lo", "put_hi", "put_lo")}
+    for cnt, r, v, sr, xr, tr in groups:
+        S = rng.uniform(*sr, cnt)
+        X = rng.uniform(*xr, cnt)
+        T = rng.uniform(*tr, cnt)
+        rm, vm = mpmath.mpf(r), mpmath.mpf(v)
+        for i in range(cnt):
+            s, x, t = (mpmath.mpf(float(a)) for a in (S[i], X[i], T[i]))
+            vst = vm * mpmath.sqrt(t)
+            d1 = (mpmath.log(s / x) + (rm + vm * vm / 2) * t) / vst
+            d2 = d1 - vst
+            e = x * mpmath.exp(-rm * t)
+            c = s * mpmath.ncdf(d1) - e * mpmath.ncdf(d2)
+            pt = e * mpmath.ncdf(-d2) - s * mpmath.ncdf(-d1)
+            for k, val in (("S", S[i]), ("X", X[i]), ("T", T[i]), ("r", r), ("v", v)):
+                cols[k].append(float(val))
+            for k, val in (("call", c), ("put", pt)):
+                h, lo = _dd(val)
+                cols[k + "_hi"].append(h)
+                cols[k + "_lo"].append(lo)
+    np.savez(os.path.join(HERE, "bs64_golden.npz"), **{k: np.array(v) for k, v in cols.items()})
+
+
 # Known answers from NT2's unit tests (value, expected, tolerance in ulpdist)
 KATS = [
     # modules/core/exponential/unit/simd/exp.cpp:42-49
@@ -321,5 +362,6 @@ if __name__ == "__main__":
     build_math(np.float32, "f32")
     build_math(np.float64, "f64")
     build_bs()
+    build_bs64()
     build_kats()
     print("golden fixtures written")

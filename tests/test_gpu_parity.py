@@ -14,8 +14,8 @@ if not torch.cuda.is_available():
 
 import _oracle as O  # noqa: E402
 from _gpu import first_diff, same_bits, say, splitmix_uniform  # noqa: E402
-from _ulp import classic_ulp  # noqa: E402
-from nt2_amd import EXACT, FAST, ops  # noqa: E402
+from _ulp import classic_ulp, dd_ulp  # noqa: E402
+from nt2_amd import ACCURATE, EXACT, FAST, ops  # noqa: E402
 
 G = os.path.join(os.path.dirname(__file__), "golden")
 ROUNDING = ["floor", "ceil", "trunc", "round", "round2even", "sign"]
@@ -502,14 +502,55 @@ def test_blackscholes_accurate_within_4ulp(r, v, tmax):
         assert e.max() <= 4.0, (float(e.max()), int(np.argmax(e)))
 
 
-def test_blackscholes_accurate_neighbour_independent():
+@pytest.mark.parametrize("r,v,tlo,thi", [(0.02, 0.30, 0.25, 10.0), (0.09, 0.30, 0.25, 10.0),
+                                         (-0.012, 0.45, 0.25, 30.0), (0.02, 0.30, 0.002, 0.25)])
+def test_blackscholes_f64_accurate_within_4ulp(r, v, tlo, thi):
+    """NT2H_ACCURATE on f64 tables: every option of a 2^22 sample within 4 ulp
+    of the quad-precision price (the oracle's libquadmath evaluation, itself
+    pinned to mpmath); the benchmark distribution, two other rate/volatility
+    pairs, and short maturities whose out-of-the-money tails reach ~1e-250."""
+    n = (1 << 22) + 3
+    say("bs f64 accurate n=%d r=%g v=%g T=[%g,%g]" % (n, r, v, tlo, thi))
+    S, X, _ = _bs_inputs(n, torch.float64, seed=5)
+    T = ops.fill_uniform(n, tlo, thi, 5 * 16 + 4, dtype=torch.float64)
+    c, p = ops.blackscholes(S, X, T, r, v, policy=ACCURATE)
+    ch, cl, ph, pl = O.bs_quad(S.cpu().numpy(), X.cpu().numpy(), T.cpu().numpy(), r, v)
+    for name, got, hi, lo in (("call", c, ch, cl), ("put", p, ph, pl)):
+        e = dd_ulp(got.cpu().numpy(), hi, lo)
+        say("  %s: max %.3f ulp, mean %.4f" % (name, float(e.max()), float(e.mean())))
+        assert e.max() <= 4.0, (name, float(e.max()), int(np.argmax(e)))
+
+
+def test_blackscholes_f64_full_size_sampled():
+    """2^26 f64 options (the fp64 half of the Black-Scholes workload), NT2's
+    exact policy: a seeded sample of 2^18 indices matches the oracle bit for
+    bit; the accurate policy on the same tables is within 4 ulp of the quad
+    price on that sample."""
+    n = 1 << 26
+    say("bs f64 full n=%d" % n)
+    S, X, T = _bs_inputs(n, torch.float64)
+    c, p = ops.blackscholes(S, X, T, 0.02, 0.30, policy=EXACT)
+    idx = torch.from_numpy(np.sort(np.random.default_rng(10).choice(n, 1 << 18, replace=False))).cuda()
+    s_, x_, t_ = (a[idx].cpu().numpy() for a in (S, X, T))
+    wc, wp = O.bs_f64(s_, x_, t_, 0.02, 0.30)
+    assert same_bits(c[idx].cpu().numpy(), wc) and same_bits(p[idx].cpu().numpy(), wp)
+    c, p = ops.blackscholes(S, X, T, 0.02, 0.30, policy=ACCURATE, call=c, put=p)
+    ch, cl, ph, pl = O.bs_quad(s_, x_, t_, 0.02, 0.30)
+    assert dd_ulp(c[idx].cpu().numpy(), ch, cl).max() <= 4.0
+    assert dd_ulp(p[idx].cpu().numpy(), ph, pl).max() <= 4.0
+    del S, X, T, c, p
+    torch.cuda.empty_cache()
+
+
+@pytest.mark.parametrize("dt", [torch.float32, torch.float64])
+def test_blackscholes_accurate_neighbour_independent(dt):
     """An option's accurate price depends on its own inputs only: pricing a
     permuted copy of a table whose discount arguments straddle the
     polynomial's range gives the permuted prices, bit for bit."""
     n = (1 << 18) + 1
-    say("bs accurate permutation n=%d" % n)
-    S, X, _ = _bs_inputs(n)
-    T = ops.fill_uniform(n, 0.25, 10.0, 77)
+    say("bs accurate permutation n=%d %s" % (n, dt))
+    S, X, _ = _bs_inputs(n, dt)
+    T = ops.fill_uniform(n, 0.25, 10.0, 77, dtype=dt)
     c, p = ops.blackscholes(S, X, T, 0.05, 0.30, policy=2)
     perm = torch.from_numpy(np.random.default_rng(9).permutation(n)).cuda()
     c2, p2 = ops.blackscholes(S[perm].contiguous(), X[perm].contiguous(), T[perm].contiguous(), 0.05, 0.30,

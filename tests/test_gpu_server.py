@@ -41,9 +41,9 @@ def _outputs(srv, rid, n, f64):
 
 @pytest.mark.parametrize("policy", [EXACT, FAST, ACCURATE])
 def test_alone_equals_mixed(policy):
-    """Under NT2H_ACCURATE the f32 requests use the accurate evaluation and the
-    f64 ones the exact policy; either way each result is bit-identical alone,
-    in the mix and from the standalone kernel."""
+    """Under NT2H_ACCURATE the f32 and f64 requests use the accurate
+    evaluations; under every policy each result is bit-identical alone, in the
+    mix and from the standalone kernel."""
     rng = np.random.default_rng(5)
     sizes = [int(2 ** rng.uniform(10, 18)) for _ in range(40)]
     kinds = [bool(rng.random() < 0.2) for _ in sizes]
@@ -71,9 +71,9 @@ def test_alone_equals_mixed(policy):
             assert np.array_equal(c.view(np.uint8), mixed[k][0].view(np.uint8)), k
             assert np.array_equal(p.view(np.uint8), mixed[k][1].view(np.uint8)), k
             # and equal to the standalone kernel
-            kpol = EXACT if (policy == ACCURATE and f) else policy
-            wc, wp = ops.blackscholes(S, X, T, 0.02, 0.30, policy=kpol)
+            wc, wp = ops.blackscholes(S, X, T, 0.02, 0.30, policy=policy)
             assert np.array_equal(wc.cpu().numpy().view(np.uint8), c.view(np.uint8)), k
+            assert np.array_equal(wp.cpu().numpy().view(np.uint8), p.view(np.uint8)), k
             srv.release(rid)
 
 
