@@ -276,6 +276,7 @@ typedef struct {
   int policy;          /* NT2H_EXACT, NT2H_FAST, or NT2H_ACCURATE (f32 and f64 requests) */
   int backend;         /* NT2H_BACKEND_HIP or NT2H_BACKEND_HOST */
   int fail_alloc_every; /* test hook: fail every k-th output allocation (0 = never) */
+  int fail_launch_every; /* test hook: report every k-th launched batch as failed (0 = never) */
 } nt2h_server_config;
 typedef struct {
   int dtype;           /* NT2H_F32 or NT2H_F64 */

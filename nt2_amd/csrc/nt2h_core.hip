@@ -949,7 +949,7 @@ using namespace nt2h;
 // ============================================================================
 extern "C" {
 
-int nt2h_abi_version(void) { return 1; }
+int nt2h_abi_version(void) { return 2; }
 
 const char* nt2h_last_error(void) { return g_last_error.c_str(); }
 

@@ -219,6 +219,7 @@ struct Request {
   uint64_t id;
   ReqDesc d;
   size_t bytes;        // budget charge: inputs + outputs
+  bool charged;        // `bytes` is counted in the server's used budget
   bool own_out;        // outputs allocated by the server (in blk)
   OutBlock* blk = nullptr;
   int state;
@@ -235,6 +236,7 @@ struct Slot {
   int cap = 0;
   bool busy = false;
   bool launched = false;       // a batch is recorded on the stream, awaiting completion
+  bool fail_next = false;      // test hook: report this slot's launched batch as failed
   std::vector<uint64_t> reqs;
   std::thread host_worker;
   std::thread done_thr;        // this slot's completion thread
@@ -254,6 +256,7 @@ struct Server {
   bool sched_exited = false;
   std::thread sched;
   int alloc_count = 0;
+  int launch_count = 0;
   hipMemPool_t pool = nullptr;  // the server's own stream-ordered pool: it keeps up to the
                                 // budget cached, so steady-state admissions do not reach the driver
   // statistics
@@ -291,13 +294,28 @@ static void srv_free(Server* s, Request* r) {
   r->own_out = false;
 }
 
+// a request that will not complete: its outputs go back to the pool and its
+// bytes to the budget; called with mu held
+static void fail_request(Server* s, Request* r, int status, int64_t t) {
+  r->state = RS_FAILED;
+  r->status = status;
+  r->t_done = t;
+  srv_free(s, r);
+  if (r->charged) {
+    s->used -= r->bytes;
+    r->charged = false;
+  }
+  ++s->failed;
+}
+
 // one precision group of a batch: nr descriptors (device) covering `tiles` tiles
 template <class P, bool ACC>
-static void launch_group(bool f64, const ReqDesc* dd, int nr, uint32_t tiles, hipStream_t st) {
+static hipError_t launch_group(bool f64, const ReqDesc* dd, int nr, uint32_t tiles, hipStream_t st) {
   const uint32_t grid = std::min<uint32_t>(tiles, kSrvGridMax);
   const uint32_t tpb = (tiles + grid - 1) / grid;
   if (f64) hipLaunchKernelGGL((serve_bs_k<P, ACC, true>), dim3(grid), dim3(kSrvBlock), 0, st, dd, nr, tiles, tpb);
   else hipLaunchKernelGGL((serve_bs_k<P, ACC, false>), dim3(grid), dim3(kSrvBlock), 0, st, dd, nr, tiles, tpb);
+  return hipGetLastError();
 }
 
 // launch every admitted request (up to max_batch) on a free slot; called with mu held
@@ -333,11 +351,7 @@ static void launch_batch(Server* s, int si, std::unique_lock<std::mutex>& lk) {
   for (Request* r : batch) {
     if (!r->d.call) {
       if (arc) {
-        r->state = RS_FAILED;
-        r->status = arc;
-        r->t_done = now_ns();
-        s->used -= r->bytes;
-        ++s->failed;
+        fail_request(s, r, arc, now_ns());
         continue;
       }
       r->d.call = (char*)base + off;
@@ -352,14 +366,32 @@ static void launch_batch(Server* s, int si, std::unique_lock<std::mutex>& lk) {
   s->cv_slot.notify_all();
   if (ok.empty()) return;
   if ((int)ok.size() > sl.cap) {
+    int drc = NT2H_OK;
     if (s->cfg.backend == NT2H_BACKEND_HOST) {
       delete[] sl.h_desc;
       sl.h_desc = new ReqDesc[ok.size()];
     } else {
+      // the slot is idle (its last batch completed), so its old descriptors can go
       if (sl.h_desc) (void)hipHostFree(sl.h_desc);
       if (sl.d_desc) (void)hipFree(sl.d_desc);
-      (void)hipHostMalloc((void**)&sl.h_desc, sizeof(ReqDesc) * ok.size(), 0);
-      (void)hipMalloc((void**)&sl.d_desc, sizeof(ReqDesc) * ok.size());
+      sl.h_desc = nullptr;
+      sl.d_desc = nullptr;
+      if (hipHostMalloc((void**)&sl.h_desc, sizeof(ReqDesc) * ok.size(), 0) != hipSuccess ||
+          hipMalloc((void**)&sl.d_desc, sizeof(ReqDesc) * ok.size()) != hipSuccess) {
+        (void)hipGetLastError();
+        if (sl.h_desc) (void)hipHostFree(sl.h_desc);
+        sl.h_desc = nullptr;
+        sl.d_desc = nullptr;
+        drc = NT2H_ENOMEM;
+      }
+    }
+    if (drc) {
+      sl.cap = 0;
+      const int64_t t = now_ns();
+      for (Request* r : ok) fail_request(s, r, drc, t);
+      s->cv_done.notify_all();
+      s->cv_slot.notify_all();
+      return;
     }
     sl.cap = (int)ok.size();
   }
@@ -384,6 +416,8 @@ static void launch_batch(Server* s, int si, std::unique_lock<std::mutex>& lk) {
   sl.busy = true;
   ++s->launches;
   s->batched_reqs += ok.size();
+  ++s->launch_count;
+  sl.fail_next = s->cfg.fail_launch_every > 0 && s->launch_count % s->cfg.fail_launch_every == 0;
   const int nreq = (int)ok.size();
   if (s->cfg.backend == NT2H_BACKEND_HOST) {
     std::vector<ReqDesc> descs(sl.h_desc, sl.h_desc + nreq);
@@ -393,16 +427,29 @@ static void launch_batch(Server* s, int si, std::unique_lock<std::mutex>& lk) {
     });
   } else {
     lk.unlock();
-    (void)hipMemcpyAsync(sl.d_desc, sl.h_desc, sizeof(ReqDesc) * nreq, hipMemcpyHostToDevice, sl.stream);
-    for (int g = 0; g < 2; ++g) {
+    hipError_t e = hipMemcpyAsync(sl.d_desc, sl.h_desc, sizeof(ReqDesc) * nreq, hipMemcpyHostToDevice, sl.stream);
+    for (int g = 0; g < 2 && e == hipSuccess; ++g) {
       if (!ngroup[g]) continue;
       const ReqDesc* dd = sl.d_desc + (g ? ngroup[0] : 0);
-      if (s->cfg.policy == NT2H_EXACT) launch_group<exact_policy, false>(g == 1, dd, ngroup[g], tiles[g], sl.stream);
-      else if (s->cfg.policy == NT2H_FAST) launch_group<fast_policy, false>(g == 1, dd, ngroup[g], tiles[g], sl.stream);
-      else launch_group<exact_policy, true>(g == 1, dd, ngroup[g], tiles[g], sl.stream);
+      if (s->cfg.policy == NT2H_EXACT) e = launch_group<exact_policy, false>(g == 1, dd, ngroup[g], tiles[g], sl.stream);
+      else if (s->cfg.policy == NT2H_FAST) e = launch_group<fast_policy, false>(g == 1, dd, ngroup[g], tiles[g], sl.stream);
+      else e = launch_group<exact_policy, true>(g == 1, dd, ngroup[g], tiles[g], sl.stream);
     }
-    (void)hipEventRecord(sl.event, sl.stream);
+    if (e == hipSuccess) e = hipEventRecord(sl.event, sl.stream);
     lk.lock();
+    if (e != hipSuccess) {
+      // nothing will signal this batch: fail its requests now (output frees are
+      // stream-ordered after whatever did launch)
+      set_error(std::string("nt2h_server: batch launch failed: ") + hipGetErrorString(e));
+      const int64_t t = now_ns();
+      for (uint64_t id : sl.reqs) fail_request(s, s->reqs[id].get(), NT2H_EHIP, t);
+      sl.reqs.clear();
+      sl.busy = false;
+      sl.fail_next = false;
+      s->cv_done.notify_all();
+      s->cv_slot.notify_all();
+      return;
+    }
   }
   // publish to the slot's completion thread only once the batch's event is recorded
   sl.launched = true;
@@ -419,6 +466,7 @@ static void sched_loop(Server* s) {
       if (s->used + r->bytes > s->cfg.budget_bytes) break;
       s->pending.pop_front();
       s->used += r->bytes;
+      r->charged = true;
       s->peak_used = std::max(s->peak_used, s->used);
       r->state = RS_ADMITTED;
       r->t_admit = now_ns();
@@ -455,19 +503,21 @@ static void done_loop(Server* s, int si) {
     }
     const int64_t t = now_ns();
     lk.lock();
+    if (sl.fail_next) status = NT2H_EHIP;  // test hook
+    sl.fail_next = false;
     sl.launched = false;
     for (uint64_t id : sl.reqs) {
       Request* r = s->reqs[id].get();
-      r->state = status == NT2H_OK ? RS_DONE : RS_FAILED;
-      r->status = status;
-      r->t_done = t;
-      if (status == NT2H_OK) {
-        s->lat_us.push_back((r->t_done - r->t_arrival) * 1e-3);
-        s->options_done += r->d.n;
-        s->t_last_done = std::max(s->t_last_done, t);
-      } else {
-        ++s->failed;
+      if (status != NT2H_OK) {
+        fail_request(s, r, status, t);
+        continue;
       }
+      r->state = RS_DONE;
+      r->status = NT2H_OK;
+      r->t_done = t;
+      s->lat_us.push_back((r->t_done - r->t_arrival) * 1e-3);
+      s->options_done += r->d.n;
+      s->t_last_done = std::max(s->t_last_done, t);
     }
     sl.reqs.clear();
     sl.busy = false;
@@ -494,6 +544,7 @@ extern "C" {
 int nt2h_server_create(const nt2h_server_config* cfg, nt2h_server** out) {
   if (!cfg || !out || cfg->budget_bytes == 0 || cfg->max_inflight < 1 || cfg->max_inflight > 16 ||
       cfg->max_batch < 1 || (cfg->policy != NT2H_EXACT && cfg->policy != NT2H_FAST && cfg->policy != NT2H_ACCURATE) ||
+      cfg->fail_alloc_every < 0 || cfg->fail_launch_every < 0 ||
       (cfg->backend != NT2H_BACKEND_HIP && cfg->backend != NT2H_BACKEND_HOST))
     return fail(NT2H_EINVAL, "nt2h_server_create: bad configuration");
   Server* s = new Server();
@@ -538,6 +589,7 @@ int nt2h_server_submit(nt2h_server* hs, const nt2h_bs_request* q, uint64_t* id) 
   r->d = ReqDesc{q->S, q->X, q->T, q->call, q->put, q->n, q->r, q->v, 0u,
                  (uint32_t)(q->dtype == NT2H_F64)};
   r->bytes = bytes;
+  r->charged = false;
   r->own_out = false;
   r->state = RS_PENDING;
   r->status = NT2H_EAGAIN;
@@ -615,6 +667,7 @@ int nt2h_server_cancel(nt2h_server* hs, uint64_t id) {
   } else if (r->state == RS_ADMITTED) {
     r->state = RS_CANCELLED;
     s->used -= r->bytes;
+    r->charged = false;
     ++s->cancelled;
   } else {
     return fail(NT2H_ESTATE, "nt2h_server_cancel: request already running or finished");
@@ -634,9 +687,10 @@ int nt2h_server_release(nt2h_server* hs, uint64_t id) {
   Request* r = it->second.get();
   if (r->state == RS_PENDING || r->state == RS_ADMITTED || r->state == RS_RUNNING)
     return fail(NT2H_ESTATE, "nt2h_server_release: request not finished");
-  if (r->state == RS_DONE) {
-    srv_free(s, r);
+  srv_free(s, r);
+  if (r->charged) {
     s->used -= r->bytes;
+    r->charged = false;
   }
   s->reqs.erase(it);
   s->cv_sched.notify_all();

@@ -93,12 +93,13 @@ def blackscholes(S, X, T, r, v, mode=BS_TEXTBOOK, policy=EXACT, call=None, put=N
     return call, put
 
 
-def sigmadelta(frames, N, counts=True, bkg=None, var=None, label=None):
+def sigmadelta(frames, N, counts=True, bkg=None, var=None, label=None, cnt=None):
     nf, npix = frames.shape
     bkg = frames[0].clone() if bkg is None else bkg
     var = torch.ones(npix, dtype=torch.uint8, device=frames.device) if var is None else var
     label = torch.zeros(npix, dtype=torch.uint8, device=frames.device) if label is None else label
-    cnt = torch.zeros(nf, dtype=torch.int32, device=frames.device) if counts else None
+    if cnt is None and counts:
+        cnt = torch.zeros(nf, dtype=torch.int32, device=frames.device)
     check(lib().nt2h_sigmadelta_u8(npix, nf, ptr(frames), frames.stride(0), ptr(bkg), ptr(var),
                                    ptr(label), N, ptr(cnt), _s()), "sigmadelta")
     return bkg, var, label, cnt

@@ -11,7 +11,7 @@ EAGAIN = -5
 class Config(ctypes.Structure):
     _fields_ = [("budget_bytes", ctypes.c_size_t), ("max_inflight", ctypes.c_int),
                 ("max_batch", ctypes.c_int), ("policy", ctypes.c_int), ("backend", ctypes.c_int),
-                ("fail_alloc_every", ctypes.c_int)]
+                ("fail_alloc_every", ctypes.c_int), ("fail_launch_every", ctypes.c_int)]
 
 
 class Request(ctypes.Structure):
@@ -62,9 +62,10 @@ class Server:
     """Owns one nt2h_server."""
 
     def __init__(self, budget_bytes, max_inflight=4, max_batch=4096, policy=1,
-                 backend=BACKEND_HIP, fail_alloc_every=0):
+                 backend=BACKEND_HIP, fail_alloc_every=0, fail_launch_every=0):
         self.L = _bind()
-        cfg = Config(budget_bytes, max_inflight, max_batch, policy, backend, fail_alloc_every)
+        cfg = Config(budget_bytes, max_inflight, max_batch, policy, backend, fail_alloc_every,
+                     fail_launch_every)
         self.h = ctypes.c_void_p()
         check(self.L.nt2h_server_create(ctypes.byref(cfg), ctypes.byref(self.h)), "server_create")
 
@@ -102,10 +103,12 @@ class Server:
     def reset_stats(self):
         check(self.L.nt2h_server_reset_stats(self.h), "server_reset_stats")
 
-    def run_trace(self, items, arrivals, timeout=120.0):
-        n = len(items)
-        arr = (Request * n)(*items)
-        ts = (ctypes.c_double * n)(*arrivals)
+    def run_trace(self, items, arrivals, timeout=120.0, count=None):
+        """Replay items[:count] (a list of Request or a ctypes Request array)
+        at the given arrival times (seconds from the start)."""
+        n = len(items) if count is None else count
+        arr = items if isinstance(items, ctypes.Array) else (Request * n)(*items)
+        ts = (ctypes.c_double * n)(*[float(a) for a in arrivals[:n]])
         return self.L.nt2h_server_run_trace(self.h, n, arr, ts, timeout)
 
     def close(self):

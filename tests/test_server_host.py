@@ -139,6 +139,39 @@ def test_allocation_failure_is_isolated():
         assert st["failed"] == 3 and st["completed"] == 6 and st["used_bytes"] == 0
 
 
+def test_launch_failure_returns_budget_and_outputs():
+    """Every 3rd launched batch fails after launch (a test hook standing in for
+    a device error): its requests fail with EHIP and give their bytes back at
+    once, so that a budget of two requests keeps admitting the rest; nothing
+    stays charged after the releases."""
+    n = 64
+    tabs = [_tables(n, np.float32, k) for k in range(12)]
+    budget = 2 * 5 * n * 4
+    EHIP = -2
+    with Server(budget, max_inflight=1, max_batch=1, backend=BACKEND_HOST, fail_launch_every=3) as srv:
+        ids = [srv.submit(F32, n, *(_p(t) for t in tb), 0.02, 0.3)[1] for tb in tabs]
+        codes = []
+        for r in ids:  # completed results are released at once, failed ones only at the end
+            codes.append(srv.wait(r, 30))
+            if codes[-1] == 0:
+                srv.release(r)
+        assert codes.count(EHIP) == 4 and codes.count(0) == 8, codes
+        assert srv.result(ids[2])[0] == EHIP
+        assert srv.stats()["used_bytes"] == 0  # the failed requests hold no budget
+        for r, c in zip(ids, codes):
+            if c != 0:
+                srv.release(r)
+        st = srv.stats()
+        assert st["failed"] == 4 and st["completed"] == 8 and st["used_bytes"] == 0
+    # batched: one failing launch fails all of its requests
+    with Server(1 << 24, max_inflight=1, max_batch=64, backend=BACKEND_HOST, fail_launch_every=1) as srv:
+        ids = [srv.submit(F32, n, *(_p(t) for t in tb), 0.02, 0.3)[1] for tb in tabs]
+        assert all(srv.wait(r, 30) == EHIP for r in ids)
+        for r in ids:
+            srv.release(r)
+        assert srv.stats()["used_bytes"] == 0
+
+
 def test_plain_schedule_one_request_per_launch():
     tabs = [_tables(200, np.float32, k) for k in range(10)]
     with Server(1 << 24, max_inflight=1, max_batch=1, backend=BACKEND_HOST) as srv:
