@@ -29,6 +29,8 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -212,7 +214,7 @@ enum ReqState { RS_PENDING = 0, RS_ADMITTED = 1, RS_RUNNING = 2, RS_DONE = 3, RS
 struct OutBlock {
   void* base = nullptr;
   int refs = 0;
-  int slot = -1;
+  int cls = 0;  // size class: 2^cls bytes
 };
 
 struct Request {
@@ -237,6 +239,7 @@ struct Slot {
   bool busy = false;
   bool launched = false;       // a batch is recorded on the stream, awaiting completion
   bool fail_next = false;      // test hook: report this slot's launched batch as failed
+  int64_t t_freed = 0, t_launched = 0;  // timing breakdown (NT2H_SERVER_TIMING)
   std::vector<uint64_t> reqs;
   std::thread host_worker;
   std::thread done_thr;        // this slot's completion thread
@@ -257,36 +260,90 @@ struct Server {
   std::thread sched;
   int alloc_count = 0;
   int launch_count = 0;
-  hipMemPool_t pool = nullptr;  // the server's own stream-ordered pool: it keeps up to the
-                                // budget cached, so steady-state admissions do not reach the driver
+  // Output blocks are cached on the host side in power-of-two size classes
+  // (from 1 MiB): a block whose requests are all released (their batch has
+  // completed, so the device no longer touches it) goes back to its class and
+  // the next batch of that class takes it without any HIP call.  Only a class
+  // with no free block reaches hipMalloc, and blocks go back to the device
+  // (hipFree) only while the cache holds more than the budget.  (Stream-ordered
+  // pool allocations cost ~14 us per hipFreeAsync and slowed every batch.)
+  std::map<int, std::vector<void*>> free_blocks;
+  size_t cached_bytes = 0;
   // statistics
   std::vector<double> lat_us;
   uint64_t options_done = 0, launches = 0, batched_reqs = 0;
   int64_t t_first_arrival = -1, t_last_done = -1;
   uint64_t failed = 0, cancelled = 0;
+  // per-batch time breakdown, printed at destroy when NT2H_SERVER_TIMING is set:
+  // slot freed -> next launch, host launch work, launch -> completion seen,
+  // completion seen -> lock taken (ns totals over `timed` batches)
+  int64_t tm_idle = 0, tm_launch = 0, tm_gpu = 0, tm_lock = 0;
+  uint64_t timed = 0;
 };
 
 static size_t elem(const ReqDesc& d) { return d.f64 ? 8 : 4; }
 
-static int srv_alloc(Server* s, Slot& sl, size_t bytes, void** p) {
+static int class_of(size_t bytes) {
+  int c = 20;
+  while ((size_t(1) << c) < bytes) ++c;
+  return c;
+}
+
+static void block_release(Server* s, void* p) {
+  if (s->cfg.backend == NT2H_BACKEND_HOST) std::free(p);
+  else (void)hipFree(p);
+}
+
+// a block of at least `bytes` (class *cls); called with mu held
+static int srv_alloc(Server* s, size_t bytes, void** p, int* cls) {
   ++s->alloc_count;
   if (s->cfg.fail_alloc_every > 0 && s->alloc_count % s->cfg.fail_alloc_every == 0)
     return NT2H_ENOMEM;
+  const int c = class_of(bytes);
+  *cls = c;
+  auto it = s->free_blocks.find(c);
+  if (it != s->free_blocks.end() && !it->second.empty()) {
+    *p = it->second.back();
+    it->second.pop_back();
+    s->cached_bytes -= size_t(1) << c;
+    return NT2H_OK;
+  }
   if (s->cfg.backend == NT2H_BACKEND_HOST) {
-    *p = std::malloc(bytes);
+    *p = std::malloc(size_t(1) << c);
     return *p ? NT2H_OK : NT2H_ENOMEM;
   }
-  hipError_t e = s->pool ? hipMallocFromPoolAsync(p, bytes, s->pool, sl.stream)
-                         : hipMallocAsync(p, bytes, sl.stream);
-  return e == hipSuccess ? NT2H_OK : NT2H_ENOMEM;
+  if (hipMalloc(p, size_t(1) << c) != hipSuccess) {
+    (void)hipGetLastError();
+    // give the cached blocks back and try once more
+    for (auto& kv : s->free_blocks)
+      for (void* q : kv.second) block_release(s, q);
+    s->free_blocks.clear();
+    s->cached_bytes = 0;
+    if (hipMalloc(p, size_t(1) << c) != hipSuccess) {
+      (void)hipGetLastError();
+      return NT2H_ENOMEM;
+    }
+  }
+  return NT2H_OK;
 }
 
+// drop the request's reference to its output block; the last reference puts
+// the block back in the cache.  Called with mu held, once the device is done
+// with the block.
 static void srv_free(Server* s, Request* r) {
   if (!r->own_out) return;
   OutBlock* b = r->blk;
   if (--b->refs == 0) {
-    if (s->cfg.backend == NT2H_BACKEND_HOST) std::free(b->base);
-    else (void)hipFreeAsync(b->base, s->slots[b->slot].stream);
+    s->free_blocks[b->cls].push_back(b->base);
+    s->cached_bytes += size_t(1) << b->cls;
+    while (s->cached_bytes > s->cfg.budget_bytes) {  // trim the largest classes first
+      auto it = s->free_blocks.rbegin();
+      while (it != s->free_blocks.rend() && it->second.empty()) ++it;
+      if (it == s->free_blocks.rend()) break;
+      block_release(s, it->second.back());
+      it->second.pop_back();
+      s->cached_bytes -= size_t(1) << it->first;
+    }
     delete b;
   }
   r->blk = nullptr;
@@ -294,7 +351,7 @@ static void srv_free(Server* s, Request* r) {
   r->own_out = false;
 }
 
-// a request that will not complete: its outputs go back to the pool and its
+// a request that will not complete: its outputs go back to the cache and its
 // bytes to the budget; called with mu held
 static void fail_request(Server* s, Request* r, int status, int64_t t) {
   r->state = RS_FAILED;
@@ -321,6 +378,8 @@ static hipError_t launch_group(bool f64, const ReqDesc* dd, int nr, uint32_t til
 // launch every admitted request (up to max_batch) on a free slot; called with mu held
 static void launch_batch(Server* s, int si, std::unique_lock<std::mutex>& lk) {
   Slot& sl = s->slots[si];
+  const int64_t t_start = now_ns();
+  if (sl.t_freed) s->tm_idle += t_start - sl.t_freed;
   std::vector<Request*> batch;
   while (!s->admitted.empty() && (int)batch.size() < s->cfg.max_batch) {
     Request* r = s->reqs[s->admitted.front()].get();
@@ -338,13 +397,14 @@ static void launch_batch(Server* s, int si, std::unique_lock<std::mutex>& lk) {
     if (!r->d.call) { total += 2 * out_bytes(r); ++owners; }
   }
   void* base = nullptr;
-  const int arc = owners ? srv_alloc(s, sl, total, &base) : NT2H_OK;
+  int cls = 0;
+  const int arc = owners ? srv_alloc(s, total, &base, &cls) : NT2H_OK;
   OutBlock* blk = nullptr;
   if (owners && arc == NT2H_OK) {
     blk = new OutBlock();
     blk->base = base;
     blk->refs = owners;
-    blk->slot = si;
+    blk->cls = cls;
   }
   std::vector<Request*> ok;
   size_t off = 0;
@@ -438,8 +498,10 @@ static void launch_batch(Server* s, int si, std::unique_lock<std::mutex>& lk) {
     if (e == hipSuccess) e = hipEventRecord(sl.event, sl.stream);
     lk.lock();
     if (e != hipSuccess) {
-      // nothing will signal this batch: fail its requests now (output frees are
-      // stream-ordered after whatever did launch)
+      // nothing will signal this batch: wait for whatever did launch, then
+      // fail its requests (their output block goes back to the cache)
+      (void)hipStreamSynchronize(sl.stream);
+      (void)hipGetLastError();
       set_error(std::string("nt2h_server: batch launch failed: ") + hipGetErrorString(e));
       const int64_t t = now_ns();
       for (uint64_t id : sl.reqs) fail_request(s, s->reqs[id].get(), NT2H_EHIP, t);
@@ -452,6 +514,9 @@ static void launch_batch(Server* s, int si, std::unique_lock<std::mutex>& lk) {
     }
   }
   // publish to the slot's completion thread only once the batch's event is recorded
+  sl.t_launched = now_ns();
+  s->tm_launch += sl.t_launched - t_start;
+  ++s->timed;
   sl.launched = true;
   s->cv_done.notify_all();
 }
@@ -498,11 +563,17 @@ static void done_loop(Server* s, int si) {
     int status = NT2H_OK;
     if (s->cfg.backend == NT2H_BACKEND_HOST) {
       if (sl.host_worker.joinable()) sl.host_worker.join();
-    } else if (hipEventSynchronize(sl.event) != hipSuccess) {
-      status = NT2H_EHIP;
+    } else {
+      // poll the event (a blocking hipEventSynchronize sleeps in the driver and
+      // wakes up ~100 us late, which a serving loop cannot afford)
+      hipError_t e;
+      while ((e = hipEventQuery(sl.event)) == hipErrorNotReady) std::this_thread::yield();
+      if (e != hipSuccess) status = NT2H_EHIP;
     }
     const int64_t t = now_ns();
     lk.lock();
+    s->tm_gpu += t - sl.t_launched;
+    s->tm_lock += now_ns() - t;
     if (sl.fail_next) status = NT2H_EHIP;  // test hook
     sl.fail_next = false;
     sl.launched = false;
@@ -521,6 +592,7 @@ static void done_loop(Server* s, int si) {
     }
     sl.reqs.clear();
     sl.busy = false;
+    sl.t_freed = now_ns();
     s->cv_sched.notify_all();
     s->cv_slot.notify_all();
   }
@@ -557,19 +629,6 @@ int nt2h_server_create(const nt2h_server_config* cfg, nt2h_server** out) {
         delete s;
         return fail(NT2H_EHIP, "nt2h_server_create: stream/event creation failed");
       }
-    }
-    int dev = 0;
-    (void)hipGetDevice(&dev);
-    hipMemPoolProps props = {};
-    props.allocType = hipMemAllocationTypePinned;
-    props.location.type = hipMemLocationTypeDevice;
-    props.location.id = dev;
-    if (hipMemPoolCreate(&s->pool, &props) == hipSuccess) {
-      uint64_t keep = cfg->budget_bytes;
-      (void)hipMemPoolSetAttribute(s->pool, hipMemPoolAttrReleaseThreshold, &keep);
-    } else {
-      s->pool = nullptr;  // fall back to the device's default pool
-      (void)hipGetLastError();
     }
   }
   s->sched = std::thread(sched_loop, s);
@@ -771,11 +830,16 @@ int nt2h_server_run_trace(nt2h_server* hs, int count, const nt2h_bs_request* ite
   ids.reserve(count);
   const int64_t t0 = now_ns();
   size_t next_release = 0;
+  int64_t last_poll = 0;
   for (int k = 0; k < count; ++k) {
     const int64_t due = t0 + (int64_t)(arrival_s[k] * 1e9);
-    while (now_ns() < due) {
-      // release finished requests while waiting (their outputs leave the budget)
-      if (next_release < ids.size() && nt2h_server_poll(hs, ids[next_release]) != NT2H_EAGAIN) {
+    for (int64_t now = now_ns(); now < due; now = now_ns()) {
+      // release finished requests while waiting (their outputs leave the
+      // budget), at most every 20 us so that the pacing loop does not keep
+      // the server's lock from the scheduler and completion threads
+      if (now - last_poll < 20000 || next_release >= ids.size()) continue;
+      last_poll = now;
+      while (next_release < ids.size() && nt2h_server_poll(hs, ids[next_release]) != NT2H_EAGAIN) {
         nt2h_server_release(hs, ids[next_release]);
         ++next_release;
       }
@@ -827,7 +891,13 @@ int nt2h_server_destroy(nt2h_server* hs) {
       if (sl.stream) (void)hipStreamDestroy(sl.stream);
     }
   }
-  if (s->pool) (void)hipMemPoolDestroy(s->pool);  // every free above completed on its slot stream
+  for (auto& kv : s->free_blocks)  // every slot stream has completed
+    for (void* q : kv.second) block_release(s, q);
+  if (std::getenv("NT2H_SERVER_TIMING") && s->timed)
+    std::fprintf(stderr, "nt2h_server: %llu batches, per batch (us): slot free -> launch %.1f, launch %.1f, "
+                 "launch -> done %.1f, done -> lock %.1f\n", (unsigned long long)s->timed,
+                 s->tm_idle * 1e-3 / s->timed, s->tm_launch * 1e-3 / s->timed, s->tm_gpu * 1e-3 / s->timed,
+                 s->tm_lock * 1e-3 / s->timed);
   delete s;
   return NT2H_OK;
 }

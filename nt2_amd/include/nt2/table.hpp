@@ -40,9 +40,12 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <algorithm>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
+#include <utility>
 #include <vector>
 
 #include "nt2/hip/dimwise.hpp"
@@ -209,12 +212,27 @@ typedef NT2_HIP_POLICY P;
 
 template <int G, class T> struct pk { T v[G]; };
 
-template <class T, int G>
+// A: i is a pack start and the terminal 16-byte aligned, so the pack is read
+// with 16-byte loads
+template <class T, int G, bool A = false>
 __device__ __forceinline__ void load(const expr::term_v<T>& t, size_t i, T (&r)[G]) {
+  if constexpr (A && std::is_arithmetic<T>::value && !std::is_same<T, bool>::value &&
+                (G * sizeof(T)) % 16 == 0) {
+    constexpr int V = 16 / (int)sizeof(T);
+    typedef T vt __attribute__((ext_vector_type(V)));
+    const vt* q = reinterpret_cast<const vt*>(t.p + i);
 #pragma unroll
-  for (int l = 0; l < G; ++l) r[l] = t.p[i + l];
+    for (int k = 0; k < G / V; ++k) {
+      const vt w = q[k];
+#pragma unroll
+      for (int l = 0; l < V; ++l) r[k * V + l] = w[l];
+    }
+  } else {
+#pragma unroll
+    for (int l = 0; l < G; ++l) r[l] = t.p[i + l];
+  }
 }
-template <class T, int G>
+template <class T, int G, bool A = false>
 __device__ __forceinline__ void load(const expr::scal_v<T>& s, size_t, T (&r)[G]) {
 #pragma unroll
   for (int l = 0; l < G; ++l) r[l] = s.v;
@@ -597,10 +615,10 @@ inline double sqr(double a) { return a * a; }
 // ---- device evaluation -------------------------------------------------------------
 namespace hip { namespace fe {
 
-template <int G, class T>
-__device__ __forceinline__ void ev(const expr::term_v<T>& v, size_t i, T (&r)[G]) { load<T, G>(v, i, r); }
-template <int G, class T>
-__device__ __forceinline__ void ev(const expr::scal_v<T>& v, size_t i, T (&r)[G]) { load<T, G>(v, i, r); }
+template <int G, bool A = false, class T>
+__device__ __forceinline__ void ev(const expr::term_v<T>& v, size_t i, T (&r)[G]) { load<T, G, A>(v, i, r); }
+template <int G, bool A = false, class T>
+__device__ __forceinline__ void ev(const expr::scal_v<T>& v, size_t i, T (&r)[G]) { load<T, G, A>(v, i, r); }
 
 template <class E> struct vview;  // value type of a view
 template <class T> struct vview<expr::term_v<T>> { typedef T type; };
@@ -653,11 +671,11 @@ template <class Tag, class VA, class VB, class VC> struct vview<n3v<Tag, VA, VB,
                                         typename vview<VC>::type>::type type;
 };
 
-template <int G, class Tag, class VA, class R>
+template <int G, bool A = false, class Tag, class VA, class R>
 __device__ __forceinline__ void ev(const n1v<Tag, VA>& v, size_t i, R (&r)[G]) {
   typedef typename vview<VA>::type TA;
   TA a[G];
-  ev<G>(v.a, i, a);
+  ev<G, A>(v.a, i, a);
   if constexpr (std::is_same<Tag, tagfn::if_zero_else_one_>::value) {
 #pragma unroll
     for (int l = 0; l < G; ++l) r[l] = a[l] ? (R)0 : (R)1;
@@ -666,14 +684,14 @@ __device__ __forceinline__ void ev(const n1v<Tag, VA>& v, size_t i, R (&r)[G]) {
   }
 }
 
-template <int G, class Tag, class VA, class VB, class R>
+template <int G, bool A = false, class Tag, class VA, class VB, class R>
 __device__ __forceinline__ void ev(const n2v<Tag, VA, VB>& v, size_t i, R (&r)[G]) {
   typedef typename vview<VA>::type TA;
   typedef typename vview<VB>::type TB;
   TA a[G];
   TB b[G];
-  ev<G>(v.a, i, a);
-  ev<G>(v.b, i, b);
+  ev<G, A>(v.a, i, a);
+  ev<G, A>(v.b, i, b);
   if constexpr (std::is_same<Tag, tagfn::pow_>::value && std::is_integral<TB>::value) {
     // generic/pow.hpp:110-132 and :154-168: |b| by repeated squaring, 1/p for b < 0
     static_assert(std::is_floating_point<R>::value, "pow is defined for floating-point bases only");
@@ -714,7 +732,7 @@ __device__ __forceinline__ void ev(const n2v<Tag, VA, VB>& v, size_t i, R (&r)[G
   }
 }
 
-template <int G, class Tag, class VA, class VB, class VC, class R>
+template <int G, bool A = false, class Tag, class VA, class VB, class VC, class R>
 __device__ __forceinline__ void ev(const n3v<Tag, VA, VB, VC>& v, size_t i, R (&r)[G]) {
   _Pragma("clang fp contract(off)")
   typedef typename vview<VA>::type TA;
@@ -723,9 +741,9 @@ __device__ __forceinline__ void ev(const n3v<Tag, VA, VB, VC>& v, size_t i, R (&
   TA a[G];
   TB b[G];
   TC c[G];
-  ev<G>(v.a, i, a);
-  ev<G>(v.b, i, b);
-  ev<G>(v.c, i, c);
+  ev<G, A>(v.a, i, a);
+  ev<G, A>(v.b, i, b);
+  ev<G, A>(v.c, i, c);
 #pragma unroll
   for (int l = 0; l < G; ++l) {
     if constexpr (std::is_same<Tag, tagfn::if_else_>::value) r[l] = a[l] ? (R)b[l] : (R)c[l];
@@ -734,46 +752,73 @@ __device__ __forceinline__ void ev(const n3v<Tag, VA, VB, VC>& v, size_t i, R (&
   }
 }
 
-// one output: packs of W from index 0, then the scalar tail
-template <class T, class V>
+// the pack's W results stored with 16-byte nontemporal stores (A) or one by one
+template <bool A, class T, int W>
+__device__ __forceinline__ void store_pack(T* __restrict__ out, size_t i, const T (&r)[W]) {
+  if constexpr (A && std::is_arithmetic<T>::value && !std::is_same<T, bool>::value &&
+                (W * sizeof(T)) % 16 == 0) {
+    constexpr int V = 16 / (int)sizeof(T);
+    typedef T vt __attribute__((ext_vector_type(V)));
+#pragma unroll
+    for (int k = 0; k < W / V; ++k) {
+      vt w;
+#pragma unroll
+      for (int l = 0; l < V; ++l) w[l] = r[k * V + l];
+      __builtin_nontemporal_store(w, reinterpret_cast<vt*>(out + i) + k);
+    }
+  } else {
+#pragma unroll
+    for (int l = 0; l < W; ++l) out[i + l] = r[l];
+  }
+}
+
+// one output: packs of W from index 0, then the scalar tail; a grid-stride
+// loop over work items (pack t < npacks, then the tail elements); A: every
+// terminal and the output are 16-byte aligned
+template <class T, class V, bool A>
 __global__ __launch_bounds__(256) void transform_k(T* __restrict__ out, V v, size_t n) {
   constexpr int W = pack_width<T>::value;
   const size_t npacks = n / W;
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < npacks) {
-    T r[W];
-    ev<W>(v, t * W, r);
-#pragma unroll
-    for (int l = 0; l < W; ++l) out[t * W + l] = r[l];
-  } else if (t < npacks + (n - npacks * W)) {
-    const size_t i = npacks * W + (t - npacks);
-    T r[1];
-    ev<1>(v, i, r);
-    out[i] = r[0];
+  const size_t items = npacks + (n - npacks * W);
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < items; t += stride) {
+    if (t < npacks) {
+      T r[W];
+      ev<W, A>(v, t * W, r);
+      store_pack<A>(out, t * W, r);
+    } else {
+      const size_t i = npacks * W + (t - npacks);
+      T r[1];
+      ev<1>(v, i, r);
+      out[i] = r[0];
+    }
   }
 }
 
 // tie: every right-hand side of an index is evaluated before any output of
 // that index is written
-template <class T, class V1, class V2>
+template <class T, class V1, class V2, bool A>
 __global__ __launch_bounds__(256) void transform2_k(T* __restrict__ o1, T* __restrict__ o2, V1 v1,
                                                     V2 v2, size_t n) {
   constexpr int W = pack_width<T>::value;
   const size_t npacks = n / W;
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < npacks) {
-    T r1[W], r2[W];
-    ev<W>(v1, t * W, r1);
-    ev<W>(v2, t * W, r2);
-#pragma unroll
-    for (int l = 0; l < W; ++l) { o1[t * W + l] = r1[l]; o2[t * W + l] = r2[l]; }
-  } else if (t < npacks + (n - npacks * W)) {
-    const size_t i = npacks * W + (t - npacks);
-    T r1[1], r2[1];
-    ev<1>(v1, i, r1);
-    ev<1>(v2, i, r2);
-    o1[i] = r1[0];
-    o2[i] = r2[0];
+  const size_t items = npacks + (n - npacks * W);
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < items; t += stride) {
+    if (t < npacks) {
+      T r1[W], r2[W];
+      ev<W, A>(v1, t * W, r1);
+      ev<W, A>(v2, t * W, r2);
+      store_pack<A>(o1, t * W, r1);
+      store_pack<A>(o2, t * W, r2);
+    } else {
+      const size_t i = npacks * W + (t - npacks);
+      T r1[1], r2[1];
+      ev<1>(v1, i, r1);
+      ev<1>(v2, i, r2);
+      o1[i] = r1[0];
+      o2[i] = r2[0];
+    }
   }
 }
 
@@ -789,7 +834,17 @@ __device__ __forceinline__ void acc_add(T& s, T& c, T v) {
   c = c + e;
 }
 
-template <class T, class V>
+// fixed-order sum of one double per thread of a 256-thread block (valid in thread 0)
+__device__ __forceinline__ double block_sum256(double d) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) d += __shfl_xor(d, off, 64);
+  __shared__ double w[4];
+  if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = d;
+  __syncthreads();
+  return ((w[0] + w[1]) + w[2]) + w[3];
+}
+
+template <class T, class V, bool A>
 __global__ __launch_bounds__(256) void fold_k(V v, size_t n, double* __restrict__ partial) {
   constexpr int W = pack_width<T>::value;
   const size_t npacks = n / W;
@@ -798,7 +853,7 @@ __global__ __launch_bounds__(256) void fold_k(V v, size_t n, double* __restrict_
   for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < npacks + (n - npacks * W); t += stride) {
     if (t < npacks) {
       T r[W];
-      ev<W>(v, t * W, r);
+      ev<W, A>(v, t * W, r);
 #pragma unroll
       for (int l = 0; l < W; ++l) acc_add(s, c, r[l]);
     } else {
@@ -807,28 +862,55 @@ __global__ __launch_bounds__(256) void fold_k(V v, size_t n, double* __restrict_
       acc_add(s, c, r[0]);
     }
   }
-  double d = (double)s + (double)c;
-  __shared__ double red[256];
-  red[threadIdx.x] = d;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double b = 0.0;
-    for (int k = 0; k < 256; ++k) b += red[k];
-    partial[blockIdx.x] = b;
-  }
+  const double b = block_sum256((double)s + (double)c);
+  if (threadIdx.x == 0) partial[blockIdx.x] = b;
 }
 
-__global__ inline void fold_final_k(const double* __restrict__ partial, int nb, double* out) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    double s = 0.0;
-    for (int k = 0; k < nb; ++k) s += partial[k];
-    *out = s;
-  }
+__global__ __launch_bounds__(256) inline void fold_final_k(const double* __restrict__ partial, int nb,
+                                                          double* __restrict__ out) {
+  double d = 0.0;
+  for (int k = threadIdx.x; k < nb; k += 256) d += partial[k];
+  const double t = block_sum256(d);
+  if (threadIdx.x == 0) *out = t;
 }
 
+// workgroups of a grid-stride launch over `items` work items (at most 32 per
+// compute unit of a 256-CU MI355X)
 inline unsigned grid(size_t items) {
   size_t g = (items + 255) / 256;
+  if (g > 8192) g = 8192;
   return (unsigned)(g ? g : 1);
+}
+
+// every terminal of a view 16-byte aligned (the output is checked by the caller)
+template <class T> inline bool aligned16(const expr::term_v<T>& v) { return ((uintptr_t)v.p & 15u) == 0; }
+template <class T> inline bool aligned16(const expr::scal_v<T>&) { return true; }
+template <class Tag, class VA> inline bool aligned16(const n1v<Tag, VA>& v) { return aligned16(v.a); }
+template <class Tag, class VA, class VB> inline bool aligned16(const n2v<Tag, VA, VB>& v) {
+  return aligned16(v.a) && aligned16(v.b);
+}
+template <class Tag, class VA, class VB, class VC> inline bool aligned16(const n3v<Tag, VA, VB, VC>& v) {
+  return aligned16(v.a) && aligned16(v.b) && aligned16(v.c);
+}
+template <class V> inline bool aligned16(const V&) { return false; }  // other views: element loads
+
+// the fold's partials: one device buffer per device, grown on demand, kept
+// until the process ends
+inline double* fold_scratch(size_t nd) {
+  static std::mutex mu;
+  static std::vector<std::pair<double*, size_t>> per_dev;
+  int dev = 0;
+  check(hipGetDevice(&dev), "hipGetDevice");
+  std::lock_guard<std::mutex> lk(mu);
+  if ((int)per_dev.size() <= dev) per_dev.resize(dev + 1, {nullptr, 0});
+  auto& e = per_dev[dev];
+  if (e.second < nd) {
+    if (e.first) check(hipFree(e.first), "fold scratch free");
+    e.first = nullptr;
+    check(hipMalloc(&e.first, sizeof(double) * nd), "fold scratch alloc");
+    e.second = nd;
+  }
+  return e.first;
 }
 
 }}  // namespace hip::fe
@@ -844,26 +926,32 @@ void transform(T* out, const EX& x, size_t n, hipStream_t s = nullptr) {
   if (n == 0) return;
   constexpr int W = fe::pack_width<T>::value;
   auto v = expr::dview<EX>::get(x);
-  hipLaunchKernelGGL((fe::transform_k<T, decltype(v)>), dim3(fe::grid(n / W + n % W)), dim3(256), 0, s,
-                     out, v, n);
+  const dim3 g(fe::grid(n / W + n % W)), b(256);
+  if (fe::aligned16(v) && ((uintptr_t)out & 15u) == 0)
+    hipLaunchKernelGGL((fe::transform_k<T, decltype(v), true>), g, b, 0, s, out, v, n);
+  else
+    hipLaunchKernelGGL((fe::transform_k<T, decltype(v), false>), g, b, 0, s, out, v, n);
   fe::check(hipGetLastError(), "transform launch");
 }
 
-// FP64 value of the compensated fold of x's n elements (synchronises s)
+// FP64 value of the compensated fold of x's n elements (synchronises s: the
+// value is returned on the host, as NT2's globalsum returns it)
 template <class T, class EX>
 double fold(const EX& x, size_t n, hipStream_t s = nullptr) {
   constexpr int W = fe::pack_width<T>::value;
   const size_t items = n / W + n % W;
   const unsigned nb = (unsigned)std::min<size_t>(1024, (items + 255) / 256 ? (items + 255) / 256 : 1);
-  double* buf = nullptr;
-  fe::check(hipMalloc(&buf, sizeof(double) * (nb + 1)), "fold alloc");
+  double* buf = fe::fold_scratch(nb + 1);
   auto v = expr::dview<EX>::get(x);
-  hipLaunchKernelGGL((fe::fold_k<T, decltype(v)>), dim3(nb), dim3(256), 0, s, v, n, buf);
-  hipLaunchKernelGGL(fe::fold_final_k, dim3(1), dim3(64), 0, s, buf, (int)nb, buf + nb);
+  if (fe::aligned16(v))
+    hipLaunchKernelGGL((fe::fold_k<T, decltype(v), true>), dim3(nb), dim3(256), 0, s, v, n, buf);
+  else
+    hipLaunchKernelGGL((fe::fold_k<T, decltype(v), false>), dim3(nb), dim3(256), 0, s, v, n, buf);
+  fe::check(hipGetLastError(), "fold launch");
+  hipLaunchKernelGGL(fe::fold_final_k, dim3(1), dim3(256), 0, s, buf, (int)nb, buf + nb);
   double r = 0.0;
   fe::check(hipMemcpyAsync(&r, buf + nb, sizeof(double), hipMemcpyDeviceToHost, s), "fold result");
   fe::check(hipStreamSynchronize(s), "fold sync");
-  fe::check(hipFree(buf), "fold free");
   return r;
 }
 }  // namespace hip
@@ -1039,8 +1127,11 @@ template <class T> struct tie2_lhs {
     constexpr int W = hip::fe::pack_width<T>::value;
     auto v1 = expr::dview<E1>::get(r.e1);
     auto v2 = expr::dview<E2>::get(r.e2);
-    hipLaunchKernelGGL((hip::fe::transform2_k<T, decltype(v1), decltype(v2)>),
-                       dim3(hip::fe::grid(n / W + n % W)), dim3(256), 0, 0, o1, o2, v1, v2, n);
+    const dim3 g(hip::fe::grid(n / W + n % W)), blk(256);
+    if (hip::fe::aligned16(v1) && hip::fe::aligned16(v2) && (((uintptr_t)o1 | (uintptr_t)o2) & 15u) == 0)
+      hipLaunchKernelGGL((hip::fe::transform2_k<T, decltype(v1), decltype(v2), true>), g, blk, 0, 0, o1, o2, v1, v2, n);
+    else
+      hipLaunchKernelGGL((hip::fe::transform2_k<T, decltype(v1), decltype(v2), false>), g, blk, 0, 0, o1, o2, v1, v2, n);
     hip::fe::check(hipGetLastError(), "tie launch");
     return *this;
   }
@@ -1144,7 +1235,7 @@ template <class F, class VA, class VB> struct vview<afun2_v<F, VA, VB>> {
 
 // linspace: hi at the last index, p*step + lo elsewhere
 // (modules/core/generative/include/nt2/core/functions/details/linspace.hpp:33-45)
-template <int G, class T>
+template <int G, bool A = false, class T>
 __device__ __forceinline__ void ev(const lin_v<T>& v, size_t i, T (&r)[G]) {
 #pragma unroll
   for (int l = 0; l < G; ++l) {
@@ -1154,7 +1245,7 @@ __device__ __forceinline__ void ev(const lin_v<T>& v, size_t i, T (&r)[G]) {
 }
 // expand_to: each output index reads the source at its coordinates, 0 along
 // the source's singleton dimensions (one element at a time)
-template <int G, class VA, class R>
+template <int G, bool A = false, class VA, class R>
 __device__ __forceinline__ void ev(const bcast_v<VA>& v, size_t i, R (&r)[G]) {
 #pragma unroll
   for (int l = 0; l < G; ++l) {
@@ -1172,12 +1263,12 @@ __device__ __forceinline__ void ev(const bcast_v<VA>& v, size_t i, R (&r)[G]) {
   }
 }
 // arrayfun: the functor sees the whole pack
-template <int G, class F, class VA, class VB, class R>
+template <int G, bool A = false, class F, class VA, class VB, class R>
 __device__ __forceinline__ void ev(const afun2_v<F, VA, VB>& v, size_t i, R (&r)[G]) {
   nt2::pack<typename vview<VA>::type, G> a;
   nt2::pack<typename vview<VB>::type, G> b;
-  ev<G>(v.a, i, a.v);
-  ev<G>(v.b, i, b.v);
+  ev<G, A>(v.a, i, a.v);
+  ev<G, A>(v.b, i, b.v);
   const auto res = v.f(a, b);
 #pragma unroll
   for (int l = 0; l < G; ++l) r[l] = (R)res[l];
