@@ -317,6 +317,19 @@ def leg_bs_f64(torch, ops, O):
     return res
 
 
+def leg_frontend():
+    """NT2 user code on the HIP site: Y = a*X + Y, globalsum(X*Y) and the
+    Black-Scholes call/put tie statement at 2^26 (tests/cpp/frontend_bench.cpp,
+    compiled against nt2/table.hpp), each beside its C ABI twin on the same
+    device tables, with the results compared."""
+    import subprocess
+    exe = os.path.join(ROOT, "tests", "cpp", "bin", "frontend_bench")
+    p = subprocess.run([exe, "26", "20"], capture_output=True, text=True, timeout=300)
+    if p.returncode != 0:
+        raise RuntimeError("frontend_bench failed: %s" % p.stderr[-500:])
+    return json.loads(p.stdout.strip().splitlines()[-1])
+
+
 def leg_dimwise(torch, ops, O):
     """sum(a, dim) and cumsum(a, dim) of an 8192 x 8192 f32 table (256 MiB),
     dimensions 1 and 2, exact (NT2's order) and fast policies; algorithmic
@@ -653,6 +666,7 @@ def main():
     if not args.no_extra:
         for name, fn in (("saxpy_sdot_2p20", lambda: leg_saxpy_sdot(torch, ops, O)),
                          ("bs_f64_2p26", lambda: leg_bs_f64(torch, ops, O)),
+                         ("frontend_2p26", leg_frontend),
                          ("sigmadelta_64x1920x1080", lambda: leg_sigmadelta(torch, ops, O)),
                          ("batched_4096_tables", lambda: leg_batch(torch, O)),
                          ("dimwise_8192x8192", lambda: leg_dimwise(torch, ops, O)),

@@ -355,14 +355,13 @@ __global__ __launch_bounds__(kBlock) void saxpy_k(size_t n, float a, const float
                                                   float* __restrict__ y, int vec) {
   const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t n4 = vec ? n / 4 : 0;
-  if (t < n4) {
-    const float4 xv = reinterpret_cast<const float4*>(x)[t];
-    float4 yv = reinterpret_cast<float4*>(y)[t];
-    yv.x = a * xv.x + yv.x;
-    yv.y = a * xv.y + yv.y;
-    yv.z = a * xv.z + yv.z;
-    yv.w = a * xv.w + yv.w;
-    reinterpret_cast<float4*>(y)[t] = yv;
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  if (t < n4) {  // streamed once: nontemporal 16-byte loads and stores
+    const f4 xv = __builtin_nontemporal_load(reinterpret_cast<const f4*>(x) + t);
+    f4 yv = __builtin_nontemporal_load(reinterpret_cast<const f4*>(y) + t);
+#pragma unroll
+    for (int l = 0; l < 4; ++l) yv[l] = a * xv[l] + yv[l];
+    __builtin_nontemporal_store(yv, reinterpret_cast<f4*>(y) + t);
   } else {
     const size_t i = n4 * 4 + (t - n4);
     if (i < n) y[i] = a * x[i] + y[i];

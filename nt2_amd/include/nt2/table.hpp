@@ -212,9 +212,11 @@ typedef NT2_HIP_POLICY P;
 
 template <int G, class T> struct pk { T v[G]; };
 
-// A: i is a pack start and the terminal 16-byte aligned, so the pack is read
-// with 16-byte loads
-template <class T, int G, bool A = false>
+// A (1 or 2): i is a pack start and the terminal 16-byte aligned, so the pack
+// is read with 16-byte loads, nontemporal for A == 2 (a cheap statement whose
+// terminals are each read once; a costly one may read a terminal several
+// times, which nontemporal loads would not share)
+template <class T, int G, int A = 0>
 __device__ __forceinline__ void load(const expr::term_v<T>& t, size_t i, T (&r)[G]) {
   if constexpr (A && std::is_arithmetic<T>::value && !std::is_same<T, bool>::value &&
                 (G * sizeof(T)) % 16 == 0) {
@@ -223,7 +225,9 @@ __device__ __forceinline__ void load(const expr::term_v<T>& t, size_t i, T (&r)[
     const vt* q = reinterpret_cast<const vt*>(t.p + i);
 #pragma unroll
     for (int k = 0; k < G / V; ++k) {
-      const vt w = q[k];
+      vt w;
+      if constexpr (A == 2) w = __builtin_nontemporal_load(q + k);  // streamed once
+      else w = q[k];
 #pragma unroll
       for (int l = 0; l < V; ++l) r[k * V + l] = w[l];
     }
@@ -232,7 +236,7 @@ __device__ __forceinline__ void load(const expr::term_v<T>& t, size_t i, T (&r)[
     for (int l = 0; l < G; ++l) r[l] = t.p[i + l];
   }
 }
-template <class T, int G, bool A = false>
+template <class T, int G, int A = 0>
 __device__ __forceinline__ void load(const expr::scal_v<T>& s, size_t, T (&r)[G]) {
 #pragma unroll
   for (int l = 0; l < G; ++l) r[l] = s.v;
@@ -615,9 +619,9 @@ inline double sqr(double a) { return a * a; }
 // ---- device evaluation -------------------------------------------------------------
 namespace hip { namespace fe {
 
-template <int G, bool A = false, class T>
+template <int G, int A = 0, class T>
 __device__ __forceinline__ void ev(const expr::term_v<T>& v, size_t i, T (&r)[G]) { load<T, G, A>(v, i, r); }
-template <int G, bool A = false, class T>
+template <int G, int A = 0, class T>
 __device__ __forceinline__ void ev(const expr::scal_v<T>& v, size_t i, T (&r)[G]) { load<T, G, A>(v, i, r); }
 
 template <class E> struct vview;  // value type of a view
@@ -671,7 +675,7 @@ template <class Tag, class VA, class VB, class VC> struct vview<n3v<Tag, VA, VB,
                                         typename vview<VC>::type>::type type;
 };
 
-template <int G, bool A = false, class Tag, class VA, class R>
+template <int G, int A = 0, class Tag, class VA, class R>
 __device__ __forceinline__ void ev(const n1v<Tag, VA>& v, size_t i, R (&r)[G]) {
   typedef typename vview<VA>::type TA;
   TA a[G];
@@ -684,7 +688,7 @@ __device__ __forceinline__ void ev(const n1v<Tag, VA>& v, size_t i, R (&r)[G]) {
   }
 }
 
-template <int G, bool A = false, class Tag, class VA, class VB, class R>
+template <int G, int A = 0, class Tag, class VA, class VB, class R>
 __device__ __forceinline__ void ev(const n2v<Tag, VA, VB>& v, size_t i, R (&r)[G]) {
   typedef typename vview<VA>::type TA;
   typedef typename vview<VB>::type TB;
@@ -732,7 +736,7 @@ __device__ __forceinline__ void ev(const n2v<Tag, VA, VB>& v, size_t i, R (&r)[G
   }
 }
 
-template <int G, bool A = false, class Tag, class VA, class VB, class VC, class R>
+template <int G, int A = 0, class Tag, class VA, class VB, class VC, class R>
 __device__ __forceinline__ void ev(const n3v<Tag, VA, VB, VC>& v, size_t i, R (&r)[G]) {
   _Pragma("clang fp contract(off)")
   typedef typename vview<VA>::type TA;
@@ -753,7 +757,7 @@ __device__ __forceinline__ void ev(const n3v<Tag, VA, VB, VC>& v, size_t i, R (&
 }
 
 // the pack's W results stored with 16-byte nontemporal stores (A) or one by one
-template <bool A, class T, int W>
+template <int A, class T, int W>
 __device__ __forceinline__ void store_pack(T* __restrict__ out, size_t i, const T (&r)[W]) {
   if constexpr (A && std::is_arithmetic<T>::value && !std::is_same<T, bool>::value &&
                 (W * sizeof(T)) % 16 == 0) {
@@ -772,47 +776,66 @@ __device__ __forceinline__ void store_pack(T* __restrict__ out, size_t i, const 
   }
 }
 
+// Elements a thread evaluates at once for an expression in which no function
+// decides per pack (expr::pack_free): a quarter (f32) or half (f64) pack,
+// which keeps the registers of a long expression at half the occupancy cost
+// and gives the same bits, every lane's result being its own.  Otherwise the
+// whole pack of W.
+template <class T> struct sub_width { static constexpr int value = pack_width<T>::value; };
+template <> struct sub_width<float> { static constexpr int value = 4; };
+template <> struct sub_width<double> { static constexpr int value = 2; };
+
 // one output: packs of W from index 0, then the scalar tail; a grid-stride
-// loop over work items (pack t < npacks, then the tail elements); A: every
-// terminal and the output are 16-byte aligned
-template <class T, class V, bool A>
+// loop over work items (SG elements of a pack for t < nsub, then the tail
+// elements one by one); A: every terminal and the output are 16-byte aligned
+// U work items per thread and iteration are evaluated before any of them is
+// stored (an in-place statement reads only its own indices, so the loads of
+// later items may go ahead of the earlier items' stores)
+template <class T, class V, int A, int SG, int U>
 __global__ __launch_bounds__(256) void transform_k(T* __restrict__ out, V v, size_t n) {
   constexpr int W = pack_width<T>::value;
+  static_assert(W % SG == 0, "a sub-pack divides the pack");
   const size_t npacks = n / W;
-  const size_t items = npacks + (n - npacks * W);
+  const size_t nsub = npacks * (W / SG);
   const size_t stride = (size_t)gridDim.x * blockDim.x;
-  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < items; t += stride) {
-    if (t < npacks) {
-      T r[W];
-      ev<W, A>(v, t * W, r);
-      store_pack<A>(out, t * W, r);
-    } else {
-      const size_t i = npacks * W + (t - npacks);
-      T r[1];
-      ev<1>(v, i, r);
-      out[i] = r[0];
-    }
+  const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (size_t t0 = g; t0 < nsub; t0 += U * stride) {
+    T r[U][SG];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (t0 + u * stride < nsub) ev<SG, A>(v, (t0 + u * stride) * SG, r[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (t0 + u * stride < nsub) store_pack<A>(out, (t0 + u * stride) * SG, r[u]);
+  }
+  for (size_t t = g; t < n - npacks * W; t += stride) {  // the scalar tail
+    const size_t i = npacks * W + t;
+    T r[1];
+    ev<1>(v, i, r);
+    out[i] = r[0];
   }
 }
 
 // tie: every right-hand side of an index is evaluated before any output of
 // that index is written
-template <class T, class V1, class V2, bool A>
+template <class T, class V1, class V2, int A, int SG>
 __global__ __launch_bounds__(256) void transform2_k(T* __restrict__ o1, T* __restrict__ o2, V1 v1,
                                                     V2 v2, size_t n) {
   constexpr int W = pack_width<T>::value;
+  static_assert(W % SG == 0, "a sub-pack divides the pack");
   const size_t npacks = n / W;
-  const size_t items = npacks + (n - npacks * W);
+  const size_t nsub = npacks * (W / SG);
+  const size_t items = nsub + (n - npacks * W);
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < items; t += stride) {
-    if (t < npacks) {
-      T r1[W], r2[W];
-      ev<W, A>(v1, t * W, r1);
-      ev<W, A>(v2, t * W, r2);
-      store_pack<A>(o1, t * W, r1);
-      store_pack<A>(o2, t * W, r2);
+    if (t < nsub) {
+      T r1[SG], r2[SG];
+      ev<SG, A>(v1, t * SG, r1);
+      ev<SG, A>(v2, t * SG, r2);
+      store_pack<A>(o1, t * SG, r1);
+      store_pack<A>(o2, t * SG, r2);
     } else {
-      const size_t i = npacks * W + (t - npacks);
+      const size_t i = npacks * W + (t - nsub);
       T r1[1], r2[1];
       ev<1>(v1, i, r1);
       ev<1>(v2, i, r2);
@@ -844,23 +867,30 @@ __device__ __forceinline__ double block_sum256(double d) {
   return ((w[0] + w[1]) + w[2]) + w[3];
 }
 
-template <class T, class V, bool A>
+template <class T, class V, int A, int SG, int U>
 __global__ __launch_bounds__(256) void fold_k(V v, size_t n, double* __restrict__ partial) {
   constexpr int W = pack_width<T>::value;
+  static_assert(W % SG == 0, "a sub-pack divides the pack");
   const size_t npacks = n / W;
+  const size_t nsub = npacks * (W / SG);
   const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   T s = 0, c = 0;
-  for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < npacks + (n - npacks * W); t += stride) {
-    if (t < npacks) {
-      T r[W];
-      ev<W, A>(v, t * W, r);
+  for (size_t t0 = g; t0 < nsub; t0 += U * stride) {
+    T r[U][SG];
 #pragma unroll
-      for (int l = 0; l < W; ++l) acc_add(s, c, r[l]);
-    } else {
-      T r[1];
-      ev<1>(v, npacks * W + (t - npacks), r);
-      acc_add(s, c, r[0]);
-    }
+    for (int u = 0; u < U; ++u)
+      if (t0 + u * stride < nsub) ev<SG, A>(v, (t0 + u * stride) * SG, r[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (t0 + u * stride < nsub)
+#pragma unroll
+        for (int l = 0; l < SG; ++l) acc_add(s, c, r[u][l]);
+  }
+  for (size_t t = g; t < n - npacks * W; t += stride) {  // the scalar tail
+    T r[1];
+    ev<1>(v, npacks * W + t, r);
+    acc_add(s, c, r[0]);
   }
   const double b = block_sum256((double)s + (double)c);
   if (threadIdx.x == 0) partial[blockIdx.x] = b;
@@ -882,6 +912,12 @@ inline unsigned grid(size_t items) {
   return (unsigned)(g ? g : 1);
 }
 
+inline unsigned grid_full(size_t items) {
+  size_t g = (items + 255) / 256;
+  if (g > 0x7fffffffu) g = 0x7fffffffu;
+  return (unsigned)(g ? g : 1);
+}
+
 // every terminal of a view 16-byte aligned (the output is checked by the caller)
 template <class T> inline bool aligned16(const expr::term_v<T>& v) { return ((uintptr_t)v.p & 15u) == 0; }
 template <class T> inline bool aligned16(const expr::scal_v<T>&) { return true; }
@@ -893,6 +929,14 @@ template <class Tag, class VA, class VB, class VC> inline bool aligned16(const n
   return aligned16(v.a) && aligned16(v.b) && aligned16(v.c);
 }
 template <class V> inline bool aligned16(const V&) { return false; }  // other views: element loads
+
+// a pinned host double per thread for fold results (a copy to pageable memory
+// goes through a staging buffer), kept until the process ends
+inline double* fold_result_host() {
+  thread_local double* p = nullptr;
+  if (!p) check(hipHostMalloc((void**)&p, sizeof(double), 0), "fold result buffer");
+  return p;
+}
 
 // the fold's partials: one device buffer per device, grown on demand, kept
 // until the process ends
@@ -915,6 +959,78 @@ inline double* fold_scratch(size_t nd) {
 
 }}  // namespace hip::fe
 
+namespace expr {
+// false when a function of the expression decides per pack (NT2's
+// trigonometric range reduction) or has a separate scalar form for a pack's
+// tail (atan; the f64 erfc, normcdf, asin and acos)
+template <class X> struct pack_free { static constexpr bool value = true; };
+template <class Tag, class A> struct pack_free<node1<Tag, A>> {
+  static constexpr bool value = pack_free<A>::value && !std::is_same<Tag, tagfn::cos_>::value &&
+                                !std::is_same<Tag, tagfn::sin_>::value && !std::is_same<Tag, tagfn::tan_>::value &&
+                                !std::is_same<Tag, tagfn::atan_>::value &&
+                                !((std::is_same<Tag, tagfn::erfc_>::value || std::is_same<Tag, tagfn::normcdf_>::value ||
+                                   std::is_same<Tag, tagfn::asin_>::value || std::is_same<Tag, tagfn::acos_>::value) &&
+                                  std::is_same<typename A::value_type, double>::value);
+};
+template <class Tag, class A, class B> struct pack_free<node2<Tag, A, B>> {
+  static constexpr bool value = pack_free<A>::value && pack_free<B>::value && !std::is_same<Tag, tagfn::atan2_>::value &&
+                                !std::is_same<Tag, tagfn::pow_>::value;
+};
+template <class Tag, class A, class B, class C> struct pack_free<node3<Tag, A, B, C>> {
+  static constexpr bool value = pack_free<A>::value && pack_free<B>::value && pack_free<C>::value;
+};
+
+// A rough VALU cost of one element (arithmetic 1, a division or square root
+// 4, a transcendental function 20), which picks the lowering's work shape:
+// cheap expressions are memory bound and get several packs in flight per
+// thread; costly ones get fewer elements per thread.  Views other than nodes,
+// terminals and scalars count as costly.
+template <class X> struct cost { static constexpr int value = 20; };
+template <class T> struct cost<terminal<T>> { static constexpr int value = 0; };
+template <class T> struct cost<scalar<T>> { static constexpr int value = 0; };
+template <class Tag> struct tag_cost {
+  static constexpr int value =
+      std::is_same<Tag, tagfn::sqrt_>::value || std::is_same<Tag, tagfn::rec_>::value ||
+              std::is_same<Tag, tagfn::divides_>::value
+          ? 4
+          : (std::is_same<Tag, tagfn::neg_>::value || std::is_same<Tag, tagfn::abs_>::value ||
+                     std::is_same<Tag, tagfn::sqr_>::value || std::is_same<Tag, tagfn::sign_>::value ||
+                     std::is_same<Tag, tagfn::floor_>::value || std::is_same<Tag, tagfn::ceil_>::value ||
+                     std::is_same<Tag, tagfn::trunc_>::value || std::is_same<Tag, tagfn::round_>::value ||
+                     std::is_same<Tag, tagfn::round2even_>::value ||
+                     std::is_same<Tag, tagfn::if_zero_else_one_>::value || std::is_same<Tag, tagfn::plus_>::value ||
+                     std::is_same<Tag, tagfn::minus_>::value || std::is_same<Tag, tagfn::multiplies_>::value ||
+                     std::is_same<Tag, tagfn::max_>::value || std::is_same<Tag, tagfn::min_>::value ||
+                     std::is_same<Tag, tagfn::adds_>::value || std::is_same<Tag, tagfn::subs_>::value ||
+                     std::is_base_of<tagfn::logical2, Tag>::value || std::is_same<Tag, tagfn::fma_>::value ||
+                     std::is_same<Tag, tagfn::fnms_>::value || std::is_same<Tag, tagfn::if_else_>::value ||
+                     std::is_same<Tag, tagfn::selinc_>::value || std::is_same<Tag, tagfn::seldec_>::value
+                 ? 1
+                 : 20);
+};
+template <class Tag, class A> struct cost<node1<Tag, A>> { static constexpr int value = cost<A>::value + tag_cost<Tag>::value; };
+template <class Tag, class A, class B> struct cost<node2<Tag, A, B>> {
+  static constexpr int value = cost<A>::value + cost<B>::value + tag_cost<Tag>::value;
+};
+template <class Tag, class A, class B, class C> struct cost<node3<Tag, A, B, C>> {
+  static constexpr int value = cost<A>::value + cost<B>::value + cost<C>::value + tag_cost<Tag>::value;
+};
+}  // namespace expr
+
+namespace hip { namespace fe {
+// elements per work item (SG) and work items in flight per thread (U) for an
+// expression type
+template <class T, class EX> struct shape {
+  static constexpr bool cheap = expr::cost<EX>::value < 8;
+  static constexpr int SG = expr::pack_free<EX>::value ? sub_width<T>::value : pack_width<T>::value;
+  static constexpr int U = 1;
+  // a cheap statement streams: one work item per thread over a full grid (the
+  // shape of the library's own saxpy); a costly one loops over a grid of at
+  // most 32 workgroups per compute unit
+  static constexpr bool full_grid = cheap;
+};
+}}  // namespace hip::fe
+
 // ---- lowering entry points -------------------------------------------------------
 // What an NT2 site override calls: transform_(out, in, range) over the whole
 // table (range = {0, numel}) and inner_fold_ of a whole expression.  Each
@@ -924,13 +1040,15 @@ namespace hip {
 template <class T, class EX>
 void transform(T* out, const EX& x, size_t n, hipStream_t s = nullptr) {
   if (n == 0) return;
-  constexpr int W = fe::pack_width<T>::value;
+  typedef fe::shape<T, EX> sh;
   auto v = expr::dview<EX>::get(x);
-  const dim3 g(fe::grid(n / W + n % W)), b(256);
+  const size_t items = (n / sh::SG + sh::U - 1) / sh::U;
+  const dim3 g(sh::full_grid ? fe::grid_full(items) : fe::grid(items)), b(256);
+  constexpr int AL = sh::cheap ? 2 : 1;
   if (fe::aligned16(v) && ((uintptr_t)out & 15u) == 0)
-    hipLaunchKernelGGL((fe::transform_k<T, decltype(v), true>), g, b, 0, s, out, v, n);
+    hipLaunchKernelGGL((fe::transform_k<T, decltype(v), AL, sh::SG, sh::U>), g, b, 0, s, out, v, n);
   else
-    hipLaunchKernelGGL((fe::transform_k<T, decltype(v), false>), g, b, 0, s, out, v, n);
+    hipLaunchKernelGGL((fe::transform_k<T, decltype(v), 0, sh::SG, sh::U>), g, b, 0, s, out, v, n);
   fe::check(hipGetLastError(), "transform launch");
 }
 
@@ -938,21 +1056,22 @@ void transform(T* out, const EX& x, size_t n, hipStream_t s = nullptr) {
 // value is returned on the host, as NT2's globalsum returns it)
 template <class T, class EX>
 double fold(const EX& x, size_t n, hipStream_t s = nullptr) {
-  constexpr int W = fe::pack_width<T>::value;
-  const size_t items = n / W + n % W;
+  typedef fe::shape<T, EX> sh;
+  const size_t items = (n / sh::SG + sh::U - 1) / sh::U;
   const unsigned nb = (unsigned)std::min<size_t>(1024, (items + 255) / 256 ? (items + 255) / 256 : 1);
   double* buf = fe::fold_scratch(nb + 1);
   auto v = expr::dview<EX>::get(x);
+  constexpr int AL = sh::cheap ? 2 : 1;
   if (fe::aligned16(v))
-    hipLaunchKernelGGL((fe::fold_k<T, decltype(v), true>), dim3(nb), dim3(256), 0, s, v, n, buf);
+    hipLaunchKernelGGL((fe::fold_k<T, decltype(v), AL, sh::SG, sh::U>), dim3(nb), dim3(256), 0, s, v, n, buf);
   else
-    hipLaunchKernelGGL((fe::fold_k<T, decltype(v), false>), dim3(nb), dim3(256), 0, s, v, n, buf);
+    hipLaunchKernelGGL((fe::fold_k<T, decltype(v), 0, sh::SG, sh::U>), dim3(nb), dim3(256), 0, s, v, n, buf);
   fe::check(hipGetLastError(), "fold launch");
   hipLaunchKernelGGL(fe::fold_final_k, dim3(1), dim3(256), 0, s, buf, (int)nb, buf + nb);
-  double r = 0.0;
-  fe::check(hipMemcpyAsync(&r, buf + nb, sizeof(double), hipMemcpyDeviceToHost, s), "fold result");
+  double* hr = fe::fold_result_host();
+  fe::check(hipMemcpyAsync(hr, buf + nb, sizeof(double), hipMemcpyDeviceToHost, s), "fold result");
   fe::check(hipStreamSynchronize(s), "fold sync");
-  return r;
+  return *hr;
 }
 }  // namespace hip
 
@@ -990,27 +1109,6 @@ template <class T, class V, bool PF> struct expr_src {
   template <int G> __device__ __forceinline__ void load(size_t i, T (&r)[G]) const { ev<G>(v, i, r); }
 };
 }}  // namespace hip::fe
-namespace expr {
-// false when a function of the expression decides per pack (NT2's
-// trigonometric range reduction) or has a separate scalar form for a pack's
-// tail (atan; the f64 erfc, normcdf, asin and acos)
-template <class X> struct pack_free { static constexpr bool value = true; };
-template <class Tag, class A> struct pack_free<node1<Tag, A>> {
-  static constexpr bool value = pack_free<A>::value && !std::is_same<Tag, tagfn::cos_>::value &&
-                                !std::is_same<Tag, tagfn::sin_>::value && !std::is_same<Tag, tagfn::tan_>::value &&
-                                !std::is_same<Tag, tagfn::atan_>::value &&
-                                !((std::is_same<Tag, tagfn::erfc_>::value || std::is_same<Tag, tagfn::normcdf_>::value ||
-                                   std::is_same<Tag, tagfn::asin_>::value || std::is_same<Tag, tagfn::acos_>::value) &&
-                                  std::is_same<typename A::value_type, double>::value);
-};
-template <class Tag, class A, class B> struct pack_free<node2<Tag, A, B>> {
-  static constexpr bool value = pack_free<A>::value && pack_free<B>::value && !std::is_same<Tag, tagfn::atan2_>::value &&
-                                !std::is_same<Tag, tagfn::pow_>::value;
-};
-template <class Tag, class A, class B, class C> struct pack_free<node3<Tag, A, B, C>> {
-  static constexpr bool value = pack_free<A>::value && pack_free<B>::value && pack_free<C>::value;
-};
-}  // namespace expr
 namespace hip {
 // sum (scan false) or cumsum (scan true) of x along dim into a new table
 template <class T, class EX>
@@ -1127,11 +1225,12 @@ template <class T> struct tie2_lhs {
     constexpr int W = hip::fe::pack_width<T>::value;
     auto v1 = expr::dview<E1>::get(r.e1);
     auto v2 = expr::dview<E2>::get(r.e2);
-    const dim3 g(hip::fe::grid(n / W + n % W)), blk(256);
+    constexpr int SG = expr::pack_free<E1>::value && expr::pack_free<E2>::value ? hip::fe::sub_width<T>::value : W;
+    const dim3 g(hip::fe::grid(n / SG + n % W)), blk(256);
     if (hip::fe::aligned16(v1) && hip::fe::aligned16(v2) && (((uintptr_t)o1 | (uintptr_t)o2) & 15u) == 0)
-      hipLaunchKernelGGL((hip::fe::transform2_k<T, decltype(v1), decltype(v2), true>), g, blk, 0, 0, o1, o2, v1, v2, n);
+      hipLaunchKernelGGL((hip::fe::transform2_k<T, decltype(v1), decltype(v2), 1, SG>), g, blk, 0, 0, o1, o2, v1, v2, n);
     else
-      hipLaunchKernelGGL((hip::fe::transform2_k<T, decltype(v1), decltype(v2), false>), g, blk, 0, 0, o1, o2, v1, v2, n);
+      hipLaunchKernelGGL((hip::fe::transform2_k<T, decltype(v1), decltype(v2), 0, SG>), g, blk, 0, 0, o1, o2, v1, v2, n);
     hip::fe::check(hipGetLastError(), "tie launch");
     return *this;
   }
@@ -1235,7 +1334,7 @@ template <class F, class VA, class VB> struct vview<afun2_v<F, VA, VB>> {
 
 // linspace: hi at the last index, p*step + lo elsewhere
 // (modules/core/generative/include/nt2/core/functions/details/linspace.hpp:33-45)
-template <int G, bool A = false, class T>
+template <int G, int A = 0, class T>
 __device__ __forceinline__ void ev(const lin_v<T>& v, size_t i, T (&r)[G]) {
 #pragma unroll
   for (int l = 0; l < G; ++l) {
@@ -1245,7 +1344,7 @@ __device__ __forceinline__ void ev(const lin_v<T>& v, size_t i, T (&r)[G]) {
 }
 // expand_to: each output index reads the source at its coordinates, 0 along
 // the source's singleton dimensions (one element at a time)
-template <int G, bool A = false, class VA, class R>
+template <int G, int A = 0, class VA, class R>
 __device__ __forceinline__ void ev(const bcast_v<VA>& v, size_t i, R (&r)[G]) {
 #pragma unroll
   for (int l = 0; l < G; ++l) {
@@ -1263,7 +1362,7 @@ __device__ __forceinline__ void ev(const bcast_v<VA>& v, size_t i, R (&r)[G]) {
   }
 }
 // arrayfun: the functor sees the whole pack
-template <int G, bool A = false, class F, class VA, class VB, class R>
+template <int G, int A = 0, class F, class VA, class VB, class R>
 __device__ __forceinline__ void ev(const afun2_v<F, VA, VB>& v, size_t i, R (&r)[G]) {
   nt2::pack<typename vview<VA>::type, G> a;
   nt2::pack<typename vview<VB>::type, G> b;
