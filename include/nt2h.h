@@ -45,7 +45,8 @@ extern "C" {
                            approximations; within 4 ulp of the exact price */
 
 /* elementwise operations for nt2h_map_* (NT2 functors exp_, log_, sqrt_, erfc_,
- * normcdf_, fastnormcdf_, cos_, sin_, abs_, exp2_ ... atan_, tan_, asin_, acos_) */
+ * normcdf_, fastnormcdf_, cos_, sin_, abs_, exp2_ ... atan_, tan_, asin_, acos_, erf_,
+ * cbrt_, asinh_, acosh_, atanh_) */
 #define NT2H_OP_EXP 0
 #define NT2H_OP_LOG 1
 #define NT2H_OP_SQRT 2
@@ -74,6 +75,11 @@ extern "C" {
 #define NT2H_OP_TAN 25
 #define NT2H_OP_ASIN 26
 #define NT2H_OP_ACOS 27
+#define NT2H_OP_ERF 28   /* f64: a pack's tail takes NT2's scalar form (+1 beyond |x| = 6) */
+#define NT2H_OP_CBRT 29
+#define NT2H_OP_ASINH 30
+#define NT2H_OP_ACOSH 31
+#define NT2H_OP_ATANH 32
 
 /* Black-Scholes formula variants */
 #define NT2H_BS_TEXTBOOK 0 /* d1 = (log(S/X)+(r+v^2/2)T)/(v sqrt T), call and put, erfc-based normcdf */

@@ -168,6 +168,11 @@ __device__ __forceinline__ void apply_f32(const float* x, float* y) {
       else if constexpr (OP == NT2H_OP_ATAN) r = dm::atan_f<P, scalar>(a);
       else if constexpr (OP == NT2H_OP_ASIN) r = dm::asin_f<P>(a);
       else if constexpr (OP == NT2H_OP_ACOS) r = dm::acos_f<P>(a);
+      else if constexpr (OP == NT2H_OP_ERF) r = dm::erf_f<P>(a);
+      else if constexpr (OP == NT2H_OP_CBRT) r = dm::cbrt_f<P>(a);
+      else if constexpr (OP == NT2H_OP_ASINH) r = dm::asinh_f<P, scalar>(a);
+      else if constexpr (OP == NT2H_OP_ACOSH) r = dm::acosh_f<P, scalar>(a);
+      else if constexpr (OP == NT2H_OP_ATANH) r = dm::atanh_f<P>(a);
       else r = __builtin_fabsf(a);
       y[l] = r;
     }
@@ -208,6 +213,11 @@ __device__ __forceinline__ void apply_f64(const double* x, double* y) {
       else if constexpr (OP == NT2H_OP_ATAN) r = dm::atan_d<P, scalar>(a);
       else if constexpr (OP == NT2H_OP_ASIN) r = dm::asin_d<P, scalar>(a);
       else if constexpr (OP == NT2H_OP_ACOS) r = dm::acos_d<P, scalar>(a);
+      else if constexpr (OP == NT2H_OP_ERF) r = dm::erf_d<P, scalar>(a);
+      else if constexpr (OP == NT2H_OP_CBRT) r = dm::cbrt_d<P>(a);
+      else if constexpr (OP == NT2H_OP_ASINH) r = dm::asinh_d<P, scalar>(a);
+      else if constexpr (OP == NT2H_OP_ACOSH) r = dm::acosh_d<P, scalar>(a);
+      else if constexpr (OP == NT2H_OP_ATANH) r = dm::atanh_d<P>(a);
       else r = __builtin_fabs(a);
       y[l] = r;
     }
@@ -878,6 +888,8 @@ static int launch_map_f32(int op, size_t n, const float* x, float* y, hipStream_
     NT2H_MAP32(NT2H_OP_EXPM1) NT2H_MAP32(NT2H_OP_LOG1P) NT2H_MAP32(NT2H_OP_TANH)
     NT2H_MAP32(NT2H_OP_SINH) NT2H_MAP32(NT2H_OP_COSH) NT2H_MAP32(NT2H_OP_ATAN) NT2H_MAP32(NT2H_OP_TAN)
     NT2H_MAP32(NT2H_OP_ASIN) NT2H_MAP32(NT2H_OP_ACOS)
+    NT2H_MAP32(NT2H_OP_ERF) NT2H_MAP32(NT2H_OP_CBRT) NT2H_MAP32(NT2H_OP_ASINH) NT2H_MAP32(NT2H_OP_ACOSH)
+    NT2H_MAP32(NT2H_OP_ATANH)
 #undef NT2H_MAP32
     default: return fail(NT2H_EINVAL, "nt2h_map_f32: unknown op");
   }
@@ -906,6 +918,8 @@ static int launch_map_f64(int op, size_t n, const double* x, double* y, hipStrea
     NT2H_MAP64(NT2H_OP_EXPM1) NT2H_MAP64(NT2H_OP_LOG1P) NT2H_MAP64(NT2H_OP_TANH)
     NT2H_MAP64(NT2H_OP_SINH) NT2H_MAP64(NT2H_OP_COSH) NT2H_MAP64(NT2H_OP_ATAN) NT2H_MAP64(NT2H_OP_TAN)
     NT2H_MAP64(NT2H_OP_ASIN) NT2H_MAP64(NT2H_OP_ACOS)
+    NT2H_MAP64(NT2H_OP_ERF) NT2H_MAP64(NT2H_OP_CBRT) NT2H_MAP64(NT2H_OP_ASINH) NT2H_MAP64(NT2H_OP_ACOSH)
+    NT2H_MAP64(NT2H_OP_ATANH)
 #undef NT2H_MAP64
     default: return fail(NT2H_EINVAL, "nt2h_map_f64: unknown op (fastnormcdf is single precision only)");
   }

@@ -978,13 +978,19 @@ NT2H_FN double acos_d(double a0) {
 // erfc_abs_f returns erfc(|a|) before the sign fold, so erfc(a) and erfc(-a)
 // can share one evaluation.
 // ---------------------------------------------------------------------------
+// erfc(z/(1-z)) exp((z/(1-z))^2) for z = x/(1+x) - 0.4: erf_kernel.hpp:48-70
 template <class P>
-NT2H_FN float erfc_abs_f(float x) {
+NT2H_FN float erfc2_kernel_f(float z) {
   NT2H_STRICT
   constexpr float c2[11] = {NT2H_F(0xc0283611), NT2H_F(0x40d67e3b), NT2H_F(0xc0cd1a85),
                             NT2H_F(0x4050b063), NT2H_F(0xbfaea865), NT2H_F(0x3e2006f0),
                             NT2H_F(0x3e1175c7), NT2H_F(0x3ec4ca6e), NT2H_F(0x3e243828),
                             NT2H_F(0xbf918a62), NT2H_F(0x3f0a0e8b)};
+  return horner<P>(c2, z);
+}
+template <class P>
+NT2H_FN float erfc_abs_f(float x) {
+  NT2H_STRICT
   constexpr float c3[9] = {NT2H_F(0x41aa8e55), NT2H_F(0x401b5680), NT2H_F(0xc010d956),
                            NT2H_F(0x3f2cff3b), NT2H_F(0xc016c985), NT2H_F(0xbffc9284),
                            NT2H_F(0xbfa11698), NT2H_F(0xbe036d7e), NT2H_F(0x3f7ffffe)};
@@ -993,7 +999,7 @@ NT2H_FN float erfc_abs_f(float x) {
   if (x < NT2H_F(0x3f2aaaab)) {
     r = (1.0f - z) * horner<P>(c3, z);
   } else {
-    r = exp_f<P>(-(x * x)) * horner<P>(c2, z - 0.4f);
+    r = exp_f<P>(-(x * x)) * erfc2_kernel_f<P>(z - 0.4f);
   }
   return (x == __builtin_inff()) ? 0.0f : r;
 }
@@ -1006,9 +1012,10 @@ NT2H_FN float erfc_f(float a0) {
   return erfc_fold_f(a0, erfc_abs_f<P>(__builtin_fabsf(a0)));
 }
 
-// f64 SIMD: simd/common/erfc.hpp:46-89; scalar (pack tail): scalar/erfc.hpp:38-70.
-template <class P, bool Scalar>
-NT2H_FN double erfc_d(double a0) {
+// f64 rational kernels, erf_kernel.hpp:101-166: erf(x)/x in x^2 on [0, 0.65),
+// erfc(x) exp(x^2) on [0.65, 2.2) and on [2.2, 6)
+template <class P>
+NT2H_FN double erf1_kernel_d(double xx) {
   NT2H_STRICT
   constexpr double e1p[5] = {NT2H_D(0x3f110512d5b20332), NT2H_D(0x3f53b7664358865a),
                              NT2H_D(0x3fa4a59a4f02579c), NT2H_D(0x3fc16500f106c0a5),
@@ -1016,6 +1023,11 @@ NT2H_FN double erfc_d(double a0) {
   constexpr double e1q[5] = {NT2H_D(0x3f37ea4332348252), NT2H_D(0x3f8166f75999dbd1),
                              NT2H_D(0x3fb64536ca92ea2f), NT2H_D(0x3fdd0a84eb1ca867),
                              NT2H_D(0x3ff0000000000000)};
+  return horner<P>(e1p, xx) / horner<P>(e1q, xx);
+}
+template <class P>
+NT2H_FN double erfc2_kernel_d(double x) {
+  NT2H_STRICT
   constexpr double e2p[7] = {0.0, NT2H_D(0x3f7cf4cfe0aacbb4), NT2H_D(0x3fb2488a6b5cb5e5),
                              NT2H_D(0x3fd53dd7a67c7e9f), NT2H_D(0x3fec1986509e687b),
                              NT2H_D(0x3ff54dfe9b258a60), NT2H_D(0x3feffffffbbb552b)};
@@ -1023,6 +1035,11 @@ NT2H_FN double erfc_d(double a0) {
                              NT2H_D(0x3fe307622fcff772), NT2H_D(0x3ff9e677c2777c3c),
                              NT2H_D(0x40053b1052dca8bd), NT2H_D(0x4003adeae79b9708),
                              NT2H_D(0x3ff0000000000000)};
+  return horner<P>(e2p, x) / horner<P>(e2q, x);
+}
+template <class P>
+NT2H_FN double erfc3_kernel_d(double x) {
+  NT2H_STRICT
   constexpr double e3p[7] = {0.0, NT2H_D(0x3f971d0907ea7a92), NT2H_D(0x3fc42210f88b9d43),
                              NT2H_D(0x3fe29be1cff90d94), NT2H_D(0x3ff44744306832ae),
                              NT2H_D(0x3ff9fa202deb88e5), NT2H_D(0x3fefff5a9e697ae2)};
@@ -1030,22 +1047,237 @@ NT2H_FN double erfc_d(double a0) {
                              NT2H_D(0x3ff0cfd4cb6cde9f), NT2H_D(0x400315ffdfd5ce91),
                              NT2H_D(0x400afd487397568f), NT2H_D(0x400602f24bf3fdb6),
                              NT2H_D(0x3ff0000000000000)};
+  return horner<P>(e3p, x) / horner<P>(e3q, x);
+}
+
+// f64 SIMD: simd/common/erfc.hpp:46-89; scalar (pack tail): scalar/erfc.hpp:38-70.
+template <class P, bool Scalar>
+NT2H_FN double erfc_d(double a0) {
+  NT2H_STRICT
   const double x = __builtin_fabs(a0), xx = x * x;
   double r;
   if (Scalar) {
     if (a0 != a0) return a0;
     r = 0.0;
     if (x <= 0.0000000001) r = 1.0 - (x * 2.0) / __builtin_sqrt(NT2H_D(0x400921fb54442d18));
-    else if (x < 0.65) r = 1.0 - x * (horner<P>(e1p, xx) / horner<P>(e1q, xx));
-    else if (x < 2.2) r = exp_d<P>(-xx) * (horner<P>(e2p, x) / horner<P>(e2q, x));
-    else if (x < 6.0) r = exp_d<P>(-xx) * (horner<P>(e3p, x) / horner<P>(e3q, x));
+    else if (x < 0.65) r = 1.0 - x * erf1_kernel_d<P>(xx);
+    else if (x < 2.2) r = exp_d<P>(-xx) * erfc2_kernel_d<P>(x);
+    else if (x < 6.0) r = exp_d<P>(-xx) * erfc3_kernel_d<P>(x);
     return (a0 < 0.0) ? 2.0 - r : r;
   }
-  if (x < 0.65) r = 1.0 - x * (horner<P>(e1p, xx) / horner<P>(e1q, xx));
-  else if (x < 2.2) r = exp_d<P>(-xx) * (horner<P>(e2p, x) / horner<P>(e2q, x));
-  else r = exp_d<P>(-xx) * (horner<P>(e3p, x) / horner<P>(e3q, x));
+  if (x < 0.65) r = 1.0 - x * erf1_kernel_d<P>(xx);
+  else if (x < 2.2) r = exp_d<P>(-xx) * erfc2_kernel_d<P>(x);
+  else r = exp_d<P>(-xx) * erfc3_kernel_d<P>(x);
   r = (x == __builtin_inf()) ? 0.0 : r;
   return (a0 < 0.0) ? 2.0 - r : r;
+}
+
+// ---------------------------------------------------------------------------
+// erf.  f32: simd/common/erf.hpp:94-124, a0 erf1(x^2) below 2/3, else
+// 1 - exp(-x^2) erfc2(x/(1+x) - 0.4) with the sign of a0, +-1 at +-Inf; the
+// scalar form (scalar/erf.hpp:76-104) gives the same values.
+// ---------------------------------------------------------------------------
+template <class P>
+NT2H_FN float erf_f(float a0) {
+  NT2H_STRICT
+  constexpr float c1[6] = {NT2H_F(0xba1fc83b), NT2H_F(0x3ba4468d), NT2H_F(0xbcdb61f4),
+                           NT2H_F(0x3de70f22), NT2H_F(0xbec0937e), NT2H_F(0x3f906eba)};
+  const float x = __builtin_fabsf(a0);
+  float r;
+  if (x < NT2H_F(0x3f2aaaab)) {
+    r = a0 * horner<P>(c1, x * x);
+  } else {
+    const float z = x / (1.0f + x) - 0.4f;
+    r = 1.0f - exp_f<P>(-(x * x)) * erfc2_kernel_f<P>(z);
+    r = (a0 < 0.0f) ? -r : r;
+  }
+  return (x == __builtin_inff()) ? (a0 > 0.0f ? 1.0f : -1.0f) : r;
+}
+
+// f64 packs: simd/common/erf.hpp:47-92, regions split at 0.65 and 2.2 as in
+// erfc, +-1 at +-Inf.  The scalar form NT2 uses for a pack's tail
+// (scalar/erf.hpp:42-74) returns a0 for NaN, 2 a0 / sqrt(pi) up to 1e-10,
+// closes its regions at 0.65, 2.2 and 6 (<=), and returns +1 beyond 6 for
+// either sign: erf(-7) is +1 in a tail.  Scalar selects it.
+template <class P, bool Scalar>
+NT2H_FN double erf_d(double a0) {
+  NT2H_STRICT
+  const double x = __builtin_fabs(a0), xx = x * x;
+  if constexpr (Scalar) {
+    if (a0 != a0) return a0;
+    if (x <= 0.0000000001) return (a0 * 2.0) / __builtin_sqrt(NT2H_D(0x400921fb54442d18));
+    if (x <= 0.65) return a0 * erf1_kernel_d<P>(xx);
+    if (x <= 6.0) {
+      const double k = (x <= 2.2) ? erfc2_kernel_d<P>(x) : erfc3_kernel_d<P>(x);
+      const double z = 1.0 - exp_d<P>(-xx) * k;
+      return (a0 < 0.0) ? -z : z;
+    }
+    return 1.0;
+  } else {
+    if (x < 0.65) return a0 * erf1_kernel_d<P>(xx);
+    const double k = (x < 2.2) ? erfc2_kernel_d<P>(x) : erfc3_kernel_d<P>(x);
+    double z = 1.0 - exp_d<P>(-xx) * k;
+    z = (a0 < 0.0) ? -z : z;
+    return (x == __builtin_inf()) ? (a0 > 0.0 ? 1.0 : -1.0) : z;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// cbrt: exponential/functions/simd/common/cbrt.hpp:52-166 (the scalar forms,
+// scalar/cbrt.hpp:42-156, give the same values).  Denormals scaled by
+// 2^nmb, frexp mantissa m in [1/2, 1) through a degree-4 polynomial, times
+// 2^(e/3) as cbrt(2)^(e mod 3) and a power of two, then Newton steps
+// x -= (x - z/x^2)/3 (one in f32, two in f64); zero and Inf unchanged.
+// ---------------------------------------------------------------------------
+template <class P>
+NT2H_FN float cbrt_f(float a0) {
+  NT2H_STRICT
+  constexpr float c[5] = {NT2H_F(0xbe09e49a), NT2H_F(0x3f0bf0fe), NT2H_F(0xbf745265),
+                          NT2H_F(0x3f91eb77), NT2H_F(0x3ece0609)};
+  float z = __builtin_fabsf(a0);
+  const bool den = z < NT2H_F(0x00800000);
+  z = den ? z * NT2H_F(0x4B000000) : z;
+  const float f = den ? NT2H_F(0x3ba14518) : 1.0f;
+  const uint32_t b = bits(z);
+  const int e = (int)((b & 0x7f800000u) >> 23) - 126;
+  float x = horner<P>(c, as_f((b & ~0x7f800000u) | 0x3f000000u));
+  const bool pos = e >= 0;
+  const int ae = pos ? e : -e, q = ae / 3, rem = ae - 3 * q;
+  const float c2 = pos ? NT2H_F(0x3fa14518) : NT2H_F(0x3f4b2ff5);
+  const float c4 = pos ? NT2H_F(0x3fcb2ff5) : NT2H_F(0x3f214518);
+  const float fact = (rem == 2) ? c4 : ((rem == 1) ? c2 : 1.0f);
+  x = (x * fact) * as_f((uint32_t)((pos ? q : -q) + 127) << 23);
+  x = x - (x - z / (x * x)) * NT2H_F(0x3eaaaaab);
+  x = as_f(bits(x) | (bits(a0) & 0x80000000u)) * f;
+  return (a0 == 0.0f || __builtin_isinf(a0)) ? a0 : x;
+}
+template <class P>
+NT2H_FN double cbrt_d(double a0) {
+  NT2H_STRICT
+  constexpr double c[5] = {NT2H_D(0xbfc13c93386fdff6), NT2H_D(0x3fe17e1fc7e59d58), NT2H_D(0xbfee8a4ca3ba37b8),
+                           NT2H_D(0x3ff23d6ee505873a), NT2H_D(0x3fd9c0c12122a4fe)};
+  double z = __builtin_fabs(a0);
+  const bool den = z < NT2H_D(0x0010000000000000);
+  z = den ? z * NT2H_D(0x4330000000000000) : z;
+  const double f = den ? NT2H_D(0x3ed428a2f98d7286) : 1.0;
+  const uint64_t b = bits(z);
+  const int e = (int)((b & 0x7ff0000000000000ull) >> 52) - 1022;
+  double x = horner<P>(c, as_d((b & ~0x7ff0000000000000ull) | 0x3fe0000000000000ull));
+  const bool pos = e >= 0;
+  const int ae = pos ? e : -e, q = ae / 3, rem = ae - 3 * q;
+  const double c2 = pos ? NT2H_D(0x3ff428a2f98d728b) : NT2H_D(0x3fe965fea53d6e3d);
+  const double c4 = pos ? NT2H_D(0x3ff965fea53d6e3d) : NT2H_D(0x3fe428a2f98d728b);
+  const double fact = (rem == 2) ? c4 : ((rem == 1) ? c2 : 1.0);
+  x = (x * fact) * as_d((uint64_t)((pos ? q : -q) + 1023) << 52);
+  x = x - (x - z / (x * x)) * NT2H_D(0x3fd5555555555555);
+  x = x - (x - z / (x * x)) * NT2H_D(0x3fd5555555555555);
+  x = as_d(bits(x) | (bits(a0) & 0x8000000000000000ull)) * f;
+  return (a0 == 0.0 || __builtin_isinf(a0)) ? a0 : x;
+}
+
+// ---------------------------------------------------------------------------
+// Inverse hyperbolic functions (hyperbolic/functions/{simd/common,scalar}/).
+// hypot(1, x) is evaluated as sqrt(1 + x*x): NT2's hypot
+// (boost/simd/arithmetic/functions/simd/common/hypot.hpp:24-45) scales both
+// operands by 2^-e, e = exponent(max(1, x)), which is exact and commutes with
+// every rounding while no result leaves the normal range, and that holds for
+// each x whose hypot is used here (x <= 2^26 in f64, 2^12 in f32).
+// ---------------------------------------------------------------------------
+// asinh f32 packs: simd/common/asinh.hpp:66-106, x + x^3 p(x^2) below 1/2,
+// else log(average(x, hypot(1, x))) + log 2 (log(x) + log 2 above
+// 1/sqrt(eps)).  The scalar form (scalar/asinh.hpp:68-111) takes
+// log(x + hypot(1, x)) in the middle range: Scalar selects it.
+template <class P, bool Scalar>
+NT2H_FN float asinh_f(float a0) {
+  NT2H_STRICT
+  constexpr float c[5] = {NT2H_F(0x3ca4d6e6), NT2H_F(0xbd2ee581), NT2H_F(0x3d9949b1),
+                          NT2H_F(0xbe2aa9ad), NT2H_F(0x3f800000)};
+  const float x = __builtin_fabsf(a0), ln2 = NT2H_F(0x3f317218);
+  float z;
+  if (x < 0.5f) {
+    z = horner<P>(c, x * x) * x;
+  } else if (x > NT2H_F(0x453504f3)) {
+    z = log_f<P>(x) + ln2;
+  } else {
+    const float h = __builtin_sqrtf(1.0f + x * x);
+    z = Scalar ? log_f<P>(x + h) : log_f<P>(fma<P>(x, 0.5f, h * 0.5f)) + ln2;
+  }
+  return as_f(bits(z) ^ (bits(a0) & 0x80000000u));
+}
+// asinh f64 packs: simd/common/asinh.hpp:46-64, log1p(x + x^2/(1 + hypot(1, x)))
+// up to 1/sqrt(eps) = 2^26, log1p(x - 1) + log 2 above.  The scalar form
+// (scalar/asinh.hpp:32-66): a0 below sqrt(eps), log1p(x + x/(1/x +
+// sqrt(1/x^2 + 1))) below 1/2, log(x) + log 2 above 2^26, else
+// log(x + hypot(1, x)).
+template <class P, bool Scalar>
+NT2H_FN double asinh_d(double a0) {
+  NT2H_STRICT
+  const double x = __builtin_fabs(a0), ln2 = NT2H_D(0x3fe62e42fefa39ef);
+  double z;
+  if constexpr (Scalar) {
+    if (x < NT2H_D(0x3e50000000000000)) return a0;
+    if (x < 0.5) {
+      const double ix = 1.0 / x;
+      z = log1p_d<P>(x + x / (ix + __builtin_sqrt(fma<P>(ix, ix, 1.0))));
+    } else if (x > NT2H_D(0x4190000000000000)) {
+      z = log_d<P>(x) + ln2;
+    } else {
+      z = log_d<P>(x + __builtin_sqrt(1.0 + x * x));
+    }
+  } else {
+    const bool big = x > NT2H_D(0x4190000000000000);
+    double u = big ? x - 1.0 : x + (x * x) / (1.0 + __builtin_sqrt(1.0 + x * x));
+    u = (x == __builtin_inf()) ? x : u;
+    z = log1p_d<P>(u);
+    z = big ? z + ln2 : z;
+  }
+  return as_d(bits(z) ^ (bits(a0) & 0x8000000000000000ull));
+}
+
+// acosh: simd/common/acosh.hpp:26-42 for packs, t = a0 - 1,
+// log1p(t + sqrt(2t + t^2)) up to 1/(2 eps), log1p(a0) + log 2 above; the
+// scalar form (scalar/acosh.hpp:22-38) takes log(t) + log 2 above.
+template <class P, bool Scalar>
+NT2H_FN float acosh_f(float a0) {
+  NT2H_STRICT
+  const float t = a0 - 1.0f, ln2 = NT2H_F(0x3f317218);
+  if (Scalar && !(t <= NT2H_F(0x4A800000))) return log_f<P>(t) + ln2;
+  const bool big = t > NT2H_F(0x4A800000);
+  const float z = big ? a0 : t + __builtin_sqrtf((t + t) + t * t);
+  const float r = log1p_f<P>(z);
+  return big ? r + ln2 : r;
+}
+template <class P, bool Scalar>
+NT2H_FN double acosh_d(double a0) {
+  NT2H_STRICT
+  const double t = a0 - 1.0, ln2 = NT2H_D(0x3fe62e42fefa39ef);
+  if (Scalar && !(t <= NT2H_D(0x4320000000000000))) return log_d<P>(t) + ln2;
+  const bool big = t > NT2H_D(0x4320000000000000);
+  const double z = big ? a0 : t + __builtin_sqrt((t + t) + t * t);
+  const double r = log1p_d<P>(z);
+  return big ? r + ln2 : r;
+}
+
+// atanh: simd/common/atanh.hpp:28-46 (scalar/atanh.hpp:22-44 gives the same
+// values), 1/2 log1p(2x + 2x x/(1-x)) below 1/2, 1/2 log1p(2x/(1-x)) above,
+// with the sign of a0.
+template <class P>
+NT2H_FN float atanh_f(float a0) {
+  NT2H_STRICT
+  const float x = __builtin_fabsf(a0), t = x + x, z1 = 1.0f - x;
+  const bool small = x < 0.5f;
+  const float q = (small ? x : t) / z1;
+  const float r = 0.5f * log1p_f<P>(small ? fma<P>(t, q, t) : q);
+  return as_f(bits(r) ^ (bits(a0) & 0x80000000u));
+}
+template <class P>
+NT2H_FN double atanh_d(double a0) {
+  NT2H_STRICT
+  const double x = __builtin_fabs(a0), t = x + x, z1 = 1.0 - x;
+  const bool small = x < 0.5;
+  const double q = (small ? x : t) / z1;
+  const double r = 0.5 * log1p_d<P>(small ? fma<P>(t, q, t) : q);
+  return as_d(bits(r) ^ (bits(a0) & 0x8000000000000000ull));
 }
 
 // normcdf(x) = Half*erfc(-Sqrt_2o_2*x):

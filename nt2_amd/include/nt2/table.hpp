@@ -199,6 +199,8 @@ NT2H_TAG(floor_, same1) NT2H_TAG(ceil_, same1) NT2H_TAG(trunc_, same1) NT2H_TAG(
 NT2H_TAG(round2even_, same1) NT2H_TAG(sign_, same1) NT2H_TAG(expm1_, same1) NT2H_TAG(log1p_, same1)
 NT2H_TAG(tanh_, same1) NT2H_TAG(sinh_, same1) NT2H_TAG(cosh_, same1) NT2H_TAG(atan_, same1)
 NT2H_TAG(atan2_, same2) NT2H_TAG(tan_, same1) NT2H_TAG(asin_, same1) NT2H_TAG(acos_, same1)
+NT2H_TAG(erf_, same1) NT2H_TAG(cbrt_, same1) NT2H_TAG(asinh_, same1) NT2H_TAG(acosh_, same1)
+NT2H_TAG(atanh_, same1)
 NT2H_TAG(pow_, same2)
 NT2H_TAG(fma_, same3) NT2H_TAG(fnms_, same3) NT2H_TAG(if_else_, sel3)
 NT2H_TAG(selinc_, selinc2) NT2H_TAG(seldec_, selinc2) NT2H_TAG(if_zero_else_one_, izeo1)
@@ -299,6 +301,15 @@ NT2H_UN(atan_, (G == 1 ? dm::atan_f<P, true>(x) : dm::atan_f<P, false>(x)),
 // asin, acos: one f32 form; f64 packs and tails differ (G == 1 is the tail)
 NT2H_UN(asin_, dm::asin_f<P>(x), (G == 1 ? dm::asin_d<P, true>(x) : dm::asin_d<P, false>(x)), no_integer_form<T>())
 NT2H_UN(acos_, dm::acos_f<P>(x), (G == 1 ? dm::acos_d<P, true>(x) : dm::acos_d<P, false>(x)), no_integer_form<T>())
+// erf: one f32 form, f64 tails differ; cbrt, atanh: one form; asinh, acosh:
+// tails differ in both precisions
+NT2H_UN(erf_, dm::erf_f<P>(x), (G == 1 ? dm::erf_d<P, true>(x) : dm::erf_d<P, false>(x)), no_integer_form<T>())
+NT2H_UN(cbrt_, dm::cbrt_f<P>(x), dm::cbrt_d<P>(x), no_integer_form<T>())
+NT2H_UN(asinh_, (G == 1 ? dm::asinh_f<P, true>(x) : dm::asinh_f<P, false>(x)),
+        (G == 1 ? dm::asinh_d<P, true>(x) : dm::asinh_d<P, false>(x)), no_integer_form<T>())
+NT2H_UN(acosh_, (G == 1 ? dm::acosh_f<P, true>(x) : dm::acosh_f<P, false>(x)),
+        (G == 1 ? dm::acosh_d<P, true>(x) : dm::acosh_d<P, false>(x)), no_integer_form<T>())
+NT2H_UN(atanh_, dm::atanh_f<P>(x), dm::atanh_d<P>(x), no_integer_form<T>())
 #undef NT2H_UN
 
 template <> struct apply1<tagfn::cos_> {
@@ -579,6 +590,7 @@ NT2H_FN1(floor, floor_) NT2H_FN1(ceil, ceil_) NT2H_FN1(trunc, trunc_) NT2H_FN1(r
 NT2H_FN1(round2even, round2even_) NT2H_FN1(sign, sign_)
 NT2H_FN1(expm1, expm1_) NT2H_FN1(log1p, log1p_) NT2H_FN1(tanh, tanh_) NT2H_FN1(sinh, sinh_)
 NT2H_FN1(cosh, cosh_) NT2H_FN1(atan, atan_) NT2H_FN1(tan, tan_) NT2H_FN1(asin, asin_) NT2H_FN1(acos, acos_)
+NT2H_FN1(erf, erf_) NT2H_FN1(cbrt, cbrt_) NT2H_FN1(asinh, asinh_) NT2H_FN1(acosh, acosh_) NT2H_FN1(atanh, atanh_)
 NT2H_FN1(if_zero_else_one, if_zero_else_one_)
 #undef NT2H_FN1
 #define NT2H_FN2(name, Tag) \
@@ -962,14 +974,16 @@ inline double* fold_scratch(size_t nd) {
 namespace expr {
 // false when a function of the expression decides per pack (NT2's
 // trigonometric range reduction) or has a separate scalar form for a pack's
-// tail (atan; the f64 erfc, normcdf, asin and acos)
+// tail (atan, asinh, acosh; the f64 erfc, normcdf, asin, acos and erf)
 template <class X> struct pack_free { static constexpr bool value = true; };
 template <class Tag, class A> struct pack_free<node1<Tag, A>> {
   static constexpr bool value = pack_free<A>::value && !std::is_same<Tag, tagfn::cos_>::value &&
                                 !std::is_same<Tag, tagfn::sin_>::value && !std::is_same<Tag, tagfn::tan_>::value &&
-                                !std::is_same<Tag, tagfn::atan_>::value &&
+                                !std::is_same<Tag, tagfn::atan_>::value && !std::is_same<Tag, tagfn::asinh_>::value &&
+                                !std::is_same<Tag, tagfn::acosh_>::value &&
                                 !((std::is_same<Tag, tagfn::erfc_>::value || std::is_same<Tag, tagfn::normcdf_>::value ||
-                                   std::is_same<Tag, tagfn::asin_>::value || std::is_same<Tag, tagfn::acos_>::value) &&
+                                   std::is_same<Tag, tagfn::asin_>::value || std::is_same<Tag, tagfn::acos_>::value ||
+                                   std::is_same<Tag, tagfn::erf_>::value) &&
                                   std::is_same<typename A::value_type, double>::value);
 };
 template <class Tag, class A, class B> struct pack_free<node2<Tag, A, B>> {

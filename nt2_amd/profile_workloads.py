@@ -1,9 +1,10 @@
 """Launch sequences for rocprofv3 runs (kernel traces and HBM byte counters).
 
     python3 -m nt2_amd.profile_workloads bs      # Black-Scholes f32 2^26, every policy, x3
+    python3 -m nt2_amd.profile_workloads bs64    # Black-Scholes f64 2^26, accurate and exact, x3
     python3 -m nt2_amd.profile_workloads fold    # saxpy + dot 2^20 and 2^26
     python3 -m nt2_amd.profile_workloads sd      # sigma-delta 64 x 1920x1080
-    python3 -m nt2_amd.profile_workloads batch   # 4096 uneven tables, persistent (512x32, 2/CU) vs plain
+    python3 -m nt2_amd.profile_workloads batch   # 4096 uneven tables, persistent (256x32, 8/CU) vs plain
     python3 -m nt2_amd.profile_workloads dimwise # sum / cumsum of 8192 x 8192 f32, dims 1 and 2, both policies
     python3 -m nt2_amd.profile_workloads images  # rgb2grey / rgb2yuv 8192 x 8192, mandelbrot 1600 x 1600
 
@@ -41,6 +42,19 @@ def bs(reps=3):
         _copy()
 
 
+def bs64(reps=3):
+    n = 1 << 26
+    S = ops.fill_uniform(n, 5.0, 30.0, 33, dtype=torch.float64)
+    X = ops.fill_uniform(n, 1.0, 100.0, 34, dtype=torch.float64)
+    T = ops.fill_uniform(n, 0.25, 10.0, 35, dtype=torch.float64)
+    c = torch.empty_like(S)
+    p = torch.empty_like(S)
+    for _ in range(reps):
+        for pol in (ACCURATE, EXACT):
+            ops.blackscholes(S, X, T, 0.02, 0.30, policy=pol, call=c, put=p)
+        _copy()
+
+
 def fold(reps=3):
     for n in (1 << 20, 1 << 26):
         x = ops.fill_uniform(n, -1.0, 1.0, 1)
@@ -63,7 +77,7 @@ def batch(reps=2, count=4096, seed=4):
     x, y, offs, lens = uneven_tables(count, seed=seed)
     sums = new_sums(count)
     with Batch(x, y, offs, lens, FAST) as b:
-        b.set_config(512, 32, 2)  # the autotuner's choice for this workload on MI355X
+        b.set_config(256, 32, 8)  # the autotuner's choice for this workload on MI355X
         for _ in range(reps):
             b.run(sums)
             b.run_plain(sums)
@@ -106,6 +120,6 @@ def images(reps=3):
 
 if __name__ == "__main__":
     what = sys.argv[1] if len(sys.argv) > 1 else "bs"
-    {"bs": bs, "fold": fold, "sd": sd, "batch": batch, "dimwise": dimwise, "images": images}[what]()
+    {"bs": bs, "bs64": bs64, "fold": fold, "sd": sd, "batch": batch, "dimwise": dimwise, "images": images}[what]()
     torch.cuda.synchronize()
     print("profile_workloads %s done" % what, flush=True)

@@ -960,6 +960,200 @@ inline float fastnormcdf_f(float a0) {
 }
 
 // ===========================================================================
+// erf, cbrt, asinh, acosh, atanh
+// ===========================================================================
+// erf f32 (one form): modules/core/euler/include/nt2/euler/functions/simd/
+// common/erf.hpp:94-124; scalar/erf.hpp:76-104 gives the same values.
+// erf1: erf_kernel.hpp:34-44.
+const float fErf1C[6] = {F32(0xba1fc83b), F32(0x3ba4468d), F32(0xbcdb61f4),
+                         F32(0x3de70f22), F32(0xbec0937e), F32(0x3f906eba)};
+inline float erf_f(float a0) {
+  float x = std::fabs(a0);
+  if (std::isinf(a0)) return a0 > 0.0f ? 1.0f : -1.0f;
+  if (x < fTwothird) return a0 * horner(fErf1C, x * x);
+  float z = x / (1.0f + x) - 0.4f;
+  float r = 1.0f - exp_f(-(x * x)) * horner(fErfc2C, z);
+  return (a0 < 0.0f) ? -r : r;
+}
+
+// erf f64.  SIMD: simd/common/erf.hpp:47-92.  Scalar (a pack's tail):
+// scalar/erf.hpp:42-74, whose last branch returns One for |x| > 6 whatever
+// the sign of a0.
+inline double erf_d(double a0, bool scalar) {
+  double x = std::fabs(a0), xx = x * x;
+  if (scalar) {
+    if (a0 != a0) return a0;
+    if (x <= 0.0000000001) return a0 * 2.0 / std::sqrt(dPi);
+    if (x <= 0.65) return a0 * (horner(dErf1P, xx) / horner(dErf1Q, xx));
+    if (x <= 2.2) {
+      double z = 1.0 - exp_d(-xx) * (horner(dErfc2P, x) / horner(dErfc2Q, x));
+      return (a0 < 0.0) ? -z : z;
+    }
+    if (x <= 6.0) {
+      double z = 1.0 - exp_d(-xx) * (horner(dErfc3P, x) / horner(dErfc3Q, x));
+      return (a0 < 0.0) ? -z : z;
+    }
+    return 1.0;
+  }
+  if (x < 0.65) return a0 * (horner(dErf1P, xx) / horner(dErf1Q, xx));
+  double ex = exp_d(-xx), z;
+  if (x < 2.2) z = 1.0 - ex * (horner(dErfc2P, x) / horner(dErfc2Q, x));
+  else z = 1.0 - ex * (horner(dErfc3P, x) / horner(dErfc3Q, x));
+  z = (a0 < 0.0) ? -z : z;
+  if (std::isinf(a0)) z = (a0 > 0.0) ? 1.0 : -1.0;
+  return z;
+}
+
+// cbrt: modules/core/exponential/include/nt2/exponential/functions/simd/
+// common/cbrt.hpp:52-166 (scalar/cbrt.hpp:42-156: the same steps).
+// fast_frexp: mantissa in [1/2, 1) and exponent from the bits; fast_ldexp:
+// times 2^e built from bits (boost/simd/ieee/functions/scalar/fast_ldexp.hpp).
+template <class T> struct cbrt_k;
+template <> struct cbrt_k<float> {
+  using U = uint32_t;
+  static constexpr int nmb = 23, bias = 127, newton = 1;
+  static constexpr U expmask = 0x7f800000u, half = 0x3f000000u, sign = 0x80000000u;
+  static float smallest() { return F32(0x00800000); }
+  static float twotonmb() { return F32(0x4B000000); }
+  static float twotomnmbo3() { return F32(0x3ba14518); }
+  static float third() { return F32(0x3EAAAAAB); }
+  static float cb2(bool p) { return p ? F32(0x3fa14518) : F32(0x3f4b2ff5); }
+  static float cb4(bool p) { return p ? F32(0x3fcb2ff5) : F32(0x3f214518); }
+  static float poly(float x) {
+    static const float c[5] = {F32(0xbe09e49a), F32(0x3f0bf0fe), F32(0xbf745265),
+                               F32(0x3f91eb77), F32(0x3ece0609)};
+    return horner(c, x);
+  }
+};
+template <> struct cbrt_k<double> {
+  using U = uint64_t;
+  static constexpr int nmb = 52, bias = 1023, newton = 2;
+  static constexpr U expmask = 0x7ff0000000000000ull, half = 0x3fe0000000000000ull,
+                     sign = 0x8000000000000000ull;
+  static double smallest() { return F64(0x0010000000000000); }
+  static double twotonmb() { return F64(0x4330000000000000); }
+  static double twotomnmbo3() { return F64(0x3ed428a2f98d7286); }
+  static double third() { return F64(0x3FD5555555555555); }
+  static double cb2(bool p) { return p ? F64(0x3ff428a2f98d728b) : F64(0x3fe965fea53d6e3d); }
+  static double cb4(bool p) { return p ? F64(0x3ff965fea53d6e3d) : F64(0x3fe428a2f98d728b); }
+  static double poly(double x) {
+    static const double c[5] = {F64(0xbfc13c93386fdff6), F64(0x3fe17e1fc7e59d58), F64(0xbfee8a4ca3ba37b8),
+                                F64(0x3ff23d6ee505873a), F64(0x3fd9c0c12122a4fe)};
+    return horner(c, x);
+  }
+};
+template <class T> inline T cbrt_t(T a0) {
+  using K = cbrt_k<T>;
+  using U = typename K::U;
+  T z = std::fabs(a0);
+  T f = T(1);
+  if (z < K::smallest()) {
+    z *= K::twotonmb();
+    f = K::twotomnmbo3();
+  }
+  U b = __builtin_bit_cast(U, z);
+  long e = (long)((b & K::expmask) >> K::nmb) - (K::bias - 1);
+  T x = K::poly(__builtin_bit_cast(T, (U)((b & ~K::expmask) | K::half)));
+  bool flag = e >= 0;
+  long e1 = flag ? e : -e;
+  long rem = e1;
+  e1 /= 3;
+  rem -= e1 * 3;
+  e = (e > 0) ? e1 : ((e < 0) ? -e1 : 0);  // negate(e1, e) = e1 * sign(e)
+  T fact = (rem == 1) ? K::cb2(flag) : T(1);
+  fact = (rem == 2) ? K::cb4(flag) : fact;
+  x = (x * fact) * __builtin_bit_cast(T, (U)((U)(e + K::bias) << K::nmb));
+  for (int k = 0; k < K::newton; ++k) x -= (x - z / (x * x)) * K::third();
+  x = __builtin_bit_cast(T, (U)(__builtin_bit_cast(U, x) | (__builtin_bit_cast(U, a0) & K::sign))) * f;
+  if (a0 == T(0) || std::isinf(a0)) return a0;
+  return x;
+}
+
+// hypot as NT2 writes it (boost/simd/arithmetic/functions/scalar/hypot.hpp:
+// 24-40): both operands scaled by 2^-e, e = exponent(max) clamped to
+// [Minexponent, Maxexponentm1], the root scaled back.
+template <class T> inline T hypot_nt2(T a, T b) {
+  if ((std::isnan(a) && std::isinf(b)) || (std::isinf(a) && std::isnan(b))) return INFINITY;
+  T r = std::fabs(a), i = std::fabs(b);
+  T m = std::max(i, r);
+  int e = (m == T(0) || !std::isfinite(m)) ? 0 : std::ilogb(m);
+  const int lo = sizeof(T) == 4 ? -126 : -1022, hi = sizeof(T) == 4 ? 126 : 1022;
+  e = std::min(std::max(e, lo), hi);
+  T sr = std::ldexp(r, -e), si = std::ldexp(i, -e);
+  return std::ldexp(std::sqrt(sr * sr + si * si), e);
+}
+
+// asinh f32.  SIMD: modules/core/hyperbolic/include/nt2/hyperbolic/functions/
+// simd/common/asinh.hpp:66-106; scalar (tail): scalar/asinh.hpp:68-111.
+// average(a, b) = fma(a, 1/2, b/2) (boost/simd/arithmetic/functions/generic/average.hpp:40).
+const float fAsinhC[5] = {F32(0x3ca4d6e6), F32(0xbd2ee581), F32(0x3d9949b1), F32(0xbe2aa9ad),
+                          F32(0x3f800000)};
+const float fLog_2 = F32(0x3f317218), fOneosqrteps = F32(0x453504f3), fOneotwoeps = F32(0x4A800000);
+const double dLog_2 = F64(0x3fe62e42fefa39ef), dOneosqrteps = F64(0x4190000000000000),
+             dOneotwoeps = F64(0x4320000000000000), dSqrteps = F64(0x3e50000000000000);
+inline float asinh_f(float a0, bool scalar) {
+  float x = std::fabs(a0), z;
+  if (x < 0.5f) {
+    z = horner(fAsinhC, x * x) * x;
+  } else if (x > fOneosqrteps) {
+    z = log_f(x) + fLog_2;
+  } else if (scalar) {
+    z = log_f(x + hypot_nt2(1.0f, x));
+  } else {
+    z = log_f(x * 0.5f + hypot_nt2(1.0f, x) * 0.5f) + fLog_2;
+  }
+  return as_f(bits(z) ^ (bits(a0) & 0x80000000u));
+}
+// asinh f64.  SIMD: simd/common/asinh.hpp:46-64; scalar: scalar/asinh.hpp:32-66.
+inline double asinh_d(double a0, bool scalar) {
+  double x = std::fabs(a0), z;
+  if (scalar) {
+    if (x < dSqrteps) return a0;
+    if (x < 0.5) {
+      double invx = 1.0 / x;
+      z = log1p_d(x + x / (invx + std::sqrt(invx * invx + 1.0)));
+    } else if (x > dOneosqrteps) {
+      z = log_d(x) + dLog_2;
+    } else {
+      z = log_d(x + hypot_nt2(1.0, x));
+    }
+  } else {
+    bool test = x > dOneosqrteps;
+    z = test ? x - 1.0 : x + (x * x) / (1.0 + hypot_nt2(1.0, x));
+    if (x == (double)INFINITY) z = x;
+    z = test ? log1p_d(z) + dLog_2 : log1p_d(z);
+  }
+  return as_d(bits(z) ^ (bits(a0) & 0x8000000000000000ull));
+}
+
+// acosh: simd/common/acosh.hpp:26-42; scalar (tail): scalar/acosh.hpp:22-38.
+template <class T, class L1, class L>
+inline T acosh_t(T a0, bool scalar, T big, T ln2, L1 lg1p, L lg) {
+  T t = a0 - T(1);
+  if (scalar) {
+    if (t <= big) return lg1p(t + std::sqrt(t + t + t * t));
+    return lg(t) + ln2;
+  }
+  bool test = t > big;
+  T z = test ? a0 : t + std::sqrt(t + t + t * t);
+  return test ? lg1p(z) + ln2 : lg1p(z);
+}
+inline float acosh_f(float a0, bool scalar) { return acosh_t(a0, scalar, fOneotwoeps, fLog_2, log1p_f, log_f); }
+inline double acosh_d(double a0, bool scalar) { return acosh_t(a0, scalar, dOneotwoeps, dLog_2, log1p_d, log_d); }
+
+// atanh (one form): simd/common/atanh.hpp:28-46 = scalar/atanh.hpp:22-44.
+template <class T, class L1> inline T atanh_t(T a0, L1 lg1p) {
+  T ax = std::fabs(a0), t = ax + ax, z1 = T(1) - ax;
+  bool test = ax < T(0.5);
+  T tmp = (test ? ax : t) / z1;
+  T r = T(0.5) * lg1p(test ? t * tmp + t : tmp);
+  if constexpr (sizeof(T) == 4) return as_f(bits(r) ^ (bits(a0) & 0x80000000u));
+  else return as_d(bits(r) ^ (bits(a0) & 0x8000000000000000ull));
+}
+inline float atanh_f(float a0) { return atanh_t(a0, log1p_f); }
+inline double atanh_d(double a0) { return atanh_t(a0, log1p_d); }
+
+// ===========================================================================
 // trigonometric range reduction
 // ===========================================================================
 int g_ph_fix = 0;
@@ -1601,8 +1795,9 @@ enum {
   OR_FASTNORMCDF = 5, OR_COS = 6, OR_SIN = 7, OR_ABS = 8, OR_EXP2 = 9, OR_EXP10 = 10,
   OR_LOG2 = 11, OR_LOG10 = 12, OR_FLOOR = 13, OR_CEIL = 14, OR_TRUNC = 15, OR_ROUND = 16,
   OR_ROUND2EVEN = 17, OR_SIGN = 18, OR_EXPM1 = 19, OR_LOG1P = 20, OR_TANH = 21, OR_SINH = 22,
-  OR_COSH = 23, OR_ATAN = 24, OR_TAN = 25, OR_ASIN = 26, OR_ACOS = 27,
-  OR_LAST = OR_ACOS
+  OR_COSH = 23, OR_ATAN = 24, OR_TAN = 25, OR_ASIN = 26, OR_ACOS = 27, OR_ERF = 28,
+  OR_CBRT = 29, OR_ASINH = 30, OR_ACOSH = 31, OR_ATANH = 32,
+  OR_LAST = OR_ATANH
 };
 
 int oracle_abi_version(void) { return 1; }
@@ -1648,6 +1843,11 @@ int oracle_map_f32(int op, const float* x, float* y, size_t n) {
       case OR_ATAN: for (size_t l = 0; l < w; ++l) y[i + l] = atan_f(x[i + l], !pack); break;
       case OR_ASIN: for (size_t l = 0; l < w; ++l) y[i + l] = asin_f(x[i + l]); break;
       case OR_ACOS: for (size_t l = 0; l < w; ++l) y[i + l] = acos_f(x[i + l]); break;
+      case OR_ERF: for (size_t l = 0; l < w; ++l) y[i + l] = erf_f(x[i + l]); break;
+      case OR_CBRT: for (size_t l = 0; l < w; ++l) y[i + l] = cbrt_t(x[i + l]); break;
+      case OR_ASINH: for (size_t l = 0; l < w; ++l) y[i + l] = asinh_f(x[i + l], !pack); break;
+      case OR_ACOSH: for (size_t l = 0; l < w; ++l) y[i + l] = acosh_f(x[i + l], !pack); break;
+      case OR_ATANH: for (size_t l = 0; l < w; ++l) y[i + l] = atanh_f(x[i + l]); break;
       case OR_COS:
       case OR_SIN:
       case OR_TAN:
@@ -1688,6 +1888,11 @@ int oracle_map_f64(int op, const double* x, double* y, size_t n) {
       case OR_ATAN: for (size_t l = 0; l < w; ++l) y[i + l] = atan_d(x[i + l], !pack); break;
       case OR_ASIN: for (size_t l = 0; l < w; ++l) y[i + l] = asin_d(x[i + l], !pack); break;
       case OR_ACOS: for (size_t l = 0; l < w; ++l) y[i + l] = acos_d(x[i + l], !pack); break;
+      case OR_ERF: for (size_t l = 0; l < w; ++l) y[i + l] = erf_d(x[i + l], !pack); break;
+      case OR_CBRT: for (size_t l = 0; l < w; ++l) y[i + l] = cbrt_t(x[i + l]); break;
+      case OR_ASINH: for (size_t l = 0; l < w; ++l) y[i + l] = asinh_d(x[i + l], !pack); break;
+      case OR_ACOSH: for (size_t l = 0; l < w; ++l) y[i + l] = acosh_d(x[i + l], !pack); break;
+      case OR_ATANH: for (size_t l = 0; l < w; ++l) y[i + l] = atanh_d(x[i + l]); break;
       case OR_COS:
       case OR_SIN:
       case OR_TAN:

@@ -213,6 +213,10 @@ int main(int argc, char** argv) {
   dump("pow_f32", at2);
   at2 = nt2::pow(at, 3) + nt2::pow(bt, -2);  // integral exponents: repeated squaring
   dump("powi_f32", at2);
+  at2 = nt2::erf(at) + nt2::cbrt(bt * 50.0f) - nt2::asinh(bt * at * 100.0f);
+  dump("erf_f32", at2);
+  at2 = nt2::acosh(nt2::abs(bt) * 1.0e6f + 1.0f) * nt2::atanh(at / 3.0f);
+  dump("acosh_f32", at2);
   at2 = if_else((at > 0.0f) && (bt < 1.0f), at, bt) + if_else(logical_or(at < -1.0f, bt > 2.0f), 1.0f, 0.0f);
   dump("logical_f32", at2);
   table<double> dm(of_size(mh * mw)), dr;
@@ -232,6 +236,10 @@ int main(int argc, char** argv) {
   dump("pow_f64", dq2);
   dq2 = nt2::pow(dq * 10.0, 7);
   dump("powi_f64", dq2);
+  dq2 = nt2::erf(dq * 8.0) + nt2::cbrt(dq * 1e5) - nt2::asinh(dq * 1e9);
+  dump("erf_f64", dq2);
+  dq2 = nt2::acosh(nt2::abs(dq) * 1e17 + 1.0) * nt2::atanh(dq * 0.999);
+  dump("acosh_f64", dq2);
 
   // ---- rgb2grey and rgb2yuv on a 96 x 131 image: the demos' ramp and random planes
   const std::size_t ih = 96, iw = 131;

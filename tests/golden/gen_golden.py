@@ -2,7 +2,8 @@
 """Writes the golden fixtures under tests/golden/ (run once; outputs committed).
 
 * math_f32.npz / math_f64.npz — inputs and correctly rounded outputs of exp,
-  log, erfc, normcdf, cos, sin, exp2, exp10, log2 and log10, computed with
+  log, erfc, normcdf, cos, sin, exp2, exp10, log2, log10, expm1, log1p, tanh,
+  sinh, cosh, tan, asin, acos, erf, cbrt, asinh, acosh and atanh, computed with
   mpmath at 60 significant digits.  Inputs: edge values, the NT2 reduction-range boundaries, and seeded
   random points spread over each function's domain (including huge
   arguments for the Payne-Hanek path of cos/sin).
@@ -77,6 +78,12 @@ FUNCS = {
     "tan": lambda x: mpmath.tan(x),
     "asin": lambda x: mpmath.asin(x) if abs(x) <= 1 else mpmath.nan,
     "acos": lambda x: mpmath.acos(x) if abs(x) <= 1 else mpmath.nan,
+    "erf": lambda x: mpmath.erf(x),
+    "cbrt": lambda x: mpmath.cbrt(x) if x >= 0 else -mpmath.cbrt(-x),
+    "asinh": lambda x: mpmath.asinh(x),
+    "acosh": lambda x: mpmath.acosh(x) if x >= 1 else mpmath.nan,
+    "atanh": lambda x: (mpmath.atanh(x) if abs(x) < 1 else
+                        (mpmath.inf * mpmath.sign(x) if abs(x) == 1 else mpmath.nan)),
 }
 
 
@@ -134,6 +141,32 @@ def inputs(name, dtype, rng):
         pts += list((1 - np.exp2(rng.uniform(-50, -1, 200))) * rng.choice([-1.0, 1.0], 200))
         pts += list(np.exp2(rng.uniform(-60, -2, 200)) * rng.choice([-1.0, 1.0], 200))
         pts += [0.0, 0.5, -0.5, 0.625, -0.625, 1.0, -1.0, 0.9999, 1e-4, 1.5e-8, 0.25, 0.75]
+    elif name == "erf":
+        pts += list(rng.uniform(-6.5, 6.5, 800)) + list(rng.uniform(-1, 1, 300))
+        pts += list(np.exp2(rng.uniform(-60, -1, 200)) * rng.choice([-1.0, 1.0], 200))
+        pts += [0.0, 0.5, 1.0, 2.0, 2.0 / 3.0, 0.65, 2.2, 6.0, -0.5, -1.0, -2.0, 1e-11, 3.0, 5.9]
+    elif name == "cbrt":
+        lo = -126 if dtype is np.float32 else -1022
+        hi = 127 if dtype is np.float32 else 1023
+        pts += list(np.exp2(rng.uniform(lo, hi, 900)) * rng.choice([-1.0, 1.0], 900))
+        pts += list(rng.uniform(-10, 10, 300))
+        pts += [1.0, -1.0, 8.0, -8.0, 27.0, 0.125, 2.0, 4.0, 3.0, 1e-30, 1e30]
+    elif name == "asinh":
+        lim = 127 if dtype is np.float32 else 1000
+        pts += list(rng.uniform(-3, 3, 500)) + list(rng.uniform(-0.5, 0.5, 200))
+        pts += list(np.exp2(rng.uniform(-60, lim, 500)) * rng.choice([-1.0, 1.0], 500))
+        pts += list(rng.uniform(2000, 4000, 100))
+        pts += [0.0, 0.5, -0.5, 0.4999, 1.0, -1.0, 2896.0, 2897.0, 67108864.0, 67108866.0, 1e-9, 1e-20]
+    elif name == "acosh":
+        lim = 127 if dtype is np.float32 else 1000
+        pts += list(1.0 + rng.uniform(0, 3, 500)) + list(1.0 + np.exp2(rng.uniform(-50, -2, 200)))
+        pts += list(np.exp2(rng.uniform(1, lim, 500)))
+        pts += [1.0, 1.5, 2.0, 10.0, 4194305.0, 4194306.0, 2.0 ** 51 + 2.0 ** 40, 2.0 ** 53]
+    elif name == "atanh":
+        pts += list(rng.uniform(-1, 1, 800)) + list(rng.uniform(-0.5, 0.5, 300))
+        pts += list((1 - np.exp2(rng.uniform(-50, -1, 200))) * rng.choice([-1.0, 1.0], 200))
+        pts += list(np.exp2(rng.uniform(-60, -2, 200)) * rng.choice([-1.0, 1.0], 200))
+        pts += [0.0, 0.5, -0.5, 0.4999, 0.9, -0.9, 0.25, 1e-9]
     else:  # cos, sin
         pts += list(rng.uniform(-np.pi / 4, np.pi / 4, 150))
         pts += list(rng.uniform(-np.pi / 2, np.pi / 2, 150))
@@ -349,6 +382,43 @@ KATS = [
     ("acos", "-1", "pi", 0.0, "modules/core/trigonometric/unit/simd/acos.cpp:55"),
     ("acos", "1", "0", 0.0, "modules/core/trigonometric/unit/simd/acos.cpp:56"),
     ("acos", "0", "pi/2", 0.0, "modules/core/trigonometric/unit/simd/acos.cpp:57"),
+    # modules/core/euler/unit/simd/erf.cpp:45-53
+    ("erf", "inf", "1", 10, "modules/core/euler/unit/simd/erf.cpp:45"),
+    ("erf", "nan", "nan", 10, "modules/core/euler/unit/simd/erf.cpp:46"),
+    ("erf", "-0", "0", 10, "modules/core/euler/unit/simd/erf.cpp:48"),
+    ("erf", "1", "0.842700792949715", 10, "modules/core/euler/unit/simd/erf.cpp:49"),
+    ("erf", "0.5", "0.520499877813047", 10, "modules/core/euler/unit/simd/erf.cpp:50"),
+    ("erf", "2", "0.995322265018953", 10, "modules/core/euler/unit/simd/erf.cpp:51"),
+    ("erf", "10", "1", 10, "modules/core/euler/unit/simd/erf.cpp:52"),
+    ("erf", "0", "0", 10, "modules/core/euler/unit/simd/erf.cpp:53"),
+    # modules/core/exponential/unit/simd/cbrt.cpp:43-50
+    ("cbrt", "inf", "inf", 0.5, "modules/core/exponential/unit/simd/cbrt.cpp:43"),
+    ("cbrt", "-inf", "-inf", 0.5, "modules/core/exponential/unit/simd/cbrt.cpp:44"),
+    ("cbrt", "nan", "nan", 0.5, "modules/core/exponential/unit/simd/cbrt.cpp:45"),
+    ("cbrt", "8", "2", 0.5, "modules/core/exponential/unit/simd/cbrt.cpp:47"),
+    ("cbrt", "-1", "-1", 0.5, "modules/core/exponential/unit/simd/cbrt.cpp:48"),
+    ("cbrt", "1", "1", 0.5, "modules/core/exponential/unit/simd/cbrt.cpp:49"),
+    ("cbrt", "0", "0", 0.5, "modules/core/exponential/unit/simd/cbrt.cpp:50"),
+    # modules/core/hyperbolic/unit/simd/asinh.cpp:50-57 (the Valmax and 2/sqrt(eps)
+    # rows compare with boost::math; their mpmath values are in the golden vectors)
+    ("asinh", "inf", "inf", 0.5, "modules/core/hyperbolic/unit/simd/asinh.cpp:50"),
+    ("asinh", "-inf", "-inf", 0.5, "modules/core/hyperbolic/unit/simd/asinh.cpp:51"),
+    ("asinh", "nan", "nan", 0.5, "modules/core/hyperbolic/unit/simd/asinh.cpp:52"),
+    ("asinh", "0", "0", 0.5, "modules/core/hyperbolic/unit/simd/asinh.cpp:54"),
+    ("asinh", "eps", "eps", 0.5, "modules/core/hyperbolic/unit/simd/asinh.cpp:57"),
+    # modules/core/hyperbolic/unit/simd/acosh.cpp:41-46
+    ("acosh", "inf", "inf", 0.5, "modules/core/hyperbolic/unit/simd/acosh.cpp:41"),
+    ("acosh", "-inf", "nan", 0.5, "modules/core/hyperbolic/unit/simd/acosh.cpp:42"),
+    ("acosh", "nan", "nan", 0.5, "modules/core/hyperbolic/unit/simd/acosh.cpp:43"),
+    ("acosh", "0", "nan", 0.5, "modules/core/hyperbolic/unit/simd/acosh.cpp:44"),
+    ("acosh", "1", "0", 0.5, "modules/core/hyperbolic/unit/simd/acosh.cpp:46"),
+    # modules/core/hyperbolic/unit/simd/atanh.cpp:43-49
+    ("atanh", "inf", "nan", 0.5, "modules/core/hyperbolic/unit/simd/atanh.cpp:43"),
+    ("atanh", "-inf", "nan", 0.5, "modules/core/hyperbolic/unit/simd/atanh.cpp:44"),
+    ("atanh", "nan", "nan", 0.5, "modules/core/hyperbolic/unit/simd/atanh.cpp:45"),
+    ("atanh", "-1", "-inf", 0.5, "modules/core/hyperbolic/unit/simd/atanh.cpp:46"),
+    ("atanh", "1", "inf", 0.5, "modules/core/hyperbolic/unit/simd/atanh.cpp:47"),
+    ("atanh", "0", "0", 0.5, "modules/core/hyperbolic/unit/simd/atanh.cpp:49"),
 ]
 
 

@@ -148,6 +148,12 @@ def test_exponential_rounding_statements(outputs):
     want = O.pow_int(a, 3) + O.pow_int(b, -2)
     got = outputs("powi_f32", f32)
     assert same_bits(got, want), first_diff(got, want)
+    want = (O.map_f32("erf", a) + O.map_f32("cbrt", b * f32(50))) - O.map_f32("asinh", b * a * f32(100))
+    got = outputs("erf_f32", f32)
+    assert same_bits(got, want), first_diff(got, want)
+    want = O.map_f32("acosh", np.abs(b) * f32(1.0e6) + f32(1)) * O.map_f32("atanh", a / f32(3))
+    got = outputs("acosh_f32", f32)
+    assert same_bits(got, want), first_diff(got, want)
     want = np.where((a > 0) & (b < 1), a, b) + np.where((a < -1) | (b > 2), f32(1), f32(0))
     got = outputs("logical_f32", f32)
     assert same_bits(got, want.astype(f32)), first_diff(got, want)
@@ -172,6 +178,12 @@ def test_exponential_rounding_statements(outputs):
     assert same_bits(got, want), first_diff(got, want)
     want = O.pow_int(q * 10.0, 7)
     got = outputs("powi_f64", np.float64)
+    assert same_bits(got, want), first_diff(got, want)
+    want = (O.map_f64("erf", q * 8.0) + O.map_f64("cbrt", q * 1e5)) - O.map_f64("asinh", q * 1e9)
+    got = outputs("erf_f64", np.float64)
+    assert same_bits(got, want), first_diff(got, want)
+    want = O.map_f64("acosh", np.abs(q) * 1e17 + 1.0) * O.map_f64("atanh", q * 0.999)
+    got = outputs("acosh_f64", np.float64)
     assert same_bits(got, want), first_diff(got, want)
 
 

@@ -20,9 +20,11 @@ from nt2_amd import ACCURATE, EXACT, FAST, ops  # noqa: E402
 G = os.path.join(os.path.dirname(__file__), "golden")
 ROUNDING = ["floor", "ceil", "trunc", "round", "round2even", "sign"]
 F32_OPS = ["exp", "log", "sqrt", "erfc", "normcdf", "fastnormcdf", "cos", "sin", "abs",
-           "exp2", "exp10", "log2", "log10"] + ROUNDING + ["expm1", "log1p", "tanh", "sinh", "cosh", "atan", "tan", "asin", "acos"]
+           "exp2", "exp10", "log2", "log10"] + ROUNDING + ["expm1", "log1p", "tanh", "sinh", "cosh", "atan", "tan", "asin", "acos"] + \
+    ["erf", "cbrt", "asinh", "acosh", "atanh"]
 F64_OPS = ["exp", "log", "sqrt", "erfc", "normcdf", "cos", "sin", "abs",
-           "exp2", "exp10", "log2", "log10"] + ROUNDING + ["expm1", "log1p", "tanh", "sinh", "cosh", "atan", "tan", "asin", "acos"]
+           "exp2", "exp10", "log2", "log10"] + ROUNDING + ["expm1", "log1p", "tanh", "sinh", "cosh", "atan", "tan", "asin", "acos"] + \
+    ["erf", "cbrt", "asinh", "acosh", "atanh"]
 
 
 def _mixed(rng, n, lo, hi, dt):
@@ -65,8 +67,18 @@ def _inputs_f32(op, rng, n):
                                np.float32([0.41421357, 0.41421354, 2.4142137, 2.4142134, 0.66])])
     if op in ("erfc", "normcdf", "fastnormcdf"):
         return rng.uniform(-12, 12, n).astype(np.float32)
-    if op in ("asin", "acos"):
+    if op in ("asin", "acos", "atanh"):
         return _mixed(rng, n, -1.05, 1.05, np.float32)
+    if op == "erf":
+        return _mixed(rng, n, -8.0, 8.0, np.float32)
+    if op == "cbrt":
+        return (np.exp2(rng.uniform(-149, 128, n)) * rng.choice([-1, 1], n)).astype(np.float32)
+    if op == "asinh":
+        return np.concatenate([_mixed(rng, n // 2, -5.0, 5.0, np.float32),
+                               (np.exp2(rng.uniform(-30, 127, n - n // 2)) * rng.choice([-1, 1], n - n // 2)).astype(np.float32)])
+    if op == "acosh":
+        return np.concatenate([rng.uniform(0.5, 4.0, n // 2),
+                               1.0 + np.exp2(rng.uniform(-30, 127, n - n // 2))]).astype(np.float32)
     if op in ("cos", "sin", "tan"):
         parts = [rng.uniform(-1, 1, n // 4), rng.uniform(-60, 60, n // 4),
                  rng.uniform(-1e4, 1e4, n // 4),
@@ -100,8 +112,17 @@ def _inputs_f64(op, rng, n):
                                [0.41421356237309503, 2.414213562373095, 0.66, 0.6600000000000001]])
     if op in ("erfc", "normcdf"):
         return rng.uniform(-30, 30, n)
-    if op in ("asin", "acos"):
+    if op in ("asin", "acos", "atanh"):
         return _mixed(rng, n, -1.05, 1.05, np.float64)
+    if op == "erf":
+        return _mixed(rng, n, -8.0, 8.0, np.float64)
+    if op == "cbrt":
+        return np.exp2(rng.uniform(-1074, 1024, n)) * rng.choice([-1, 1], n)
+    if op == "asinh":
+        return np.concatenate([_mixed(rng, n // 2, -5.0, 5.0, np.float64),
+                               np.exp2(rng.uniform(-30, 1020, n - n // 2)) * rng.choice([-1, 1], n - n // 2)])
+    if op == "acosh":
+        return np.concatenate([rng.uniform(0.5, 4.0, n // 2), 1.0 + np.exp2(rng.uniform(-30, 1020, n - n // 2))])
     if op in ("cos", "sin", "tan"):
         parts = [rng.uniform(-1, 1, n // 4), rng.uniform(-60, 60, n // 4),
                  rng.uniform(-1e6, 1e6, n // 4),
@@ -154,7 +175,8 @@ def _truth_f32(op, x):
          "floor": np.floor, "ceil": np.ceil, "trunc": np.trunc, "round2even": np.rint, "sign": np.sign,
          "round": lambda v: np.sign(v) * np.floor(np.abs(v) + 0.5),
          "expm1": np.expm1, "log1p": lambda v: np.log1p(np.where(v >= -1, v, np.nan)), "tanh": np.tanh, "sinh": np.sinh, "cosh": np.cosh, "atan": np.arctan,
-         "tan": np.tan, "asin": np.arcsin, "acos": np.arccos}[op]
+         "tan": np.tan, "asin": np.arcsin, "acos": np.arccos, "erf": sp.erf, "cbrt": np.cbrt,
+         "asinh": np.arcsinh, "acosh": np.arccosh, "atanh": np.arctanh}[op]
     with np.errstate(all="ignore"):
         return f(xd)
 
@@ -167,7 +189,8 @@ def _fast_bound(op, x, y):
         return np.full(x.shape, 1.5)
     if op == "log10":
         return np.full(x.shape, 2.0)
-    if op in ("expm1", "log1p", "tanh", "sinh", "cosh", "atan", "asin", "acos"):
+    if op in ("expm1", "log1p", "tanh", "sinh", "cosh", "atan", "asin", "acos", "erf", "cbrt", "asinh", "acosh",
+              "atanh"):
         return np.full(x.shape, 2.5)
     if op in ROUNDING:
         return np.zeros(x.shape)

@@ -54,8 +54,16 @@ def _bound(name, x, y, dt):
     ax = np.abs(x.astype(np.float64))
     if name in ("exp", "log", "exp2", "exp10", "log2"):
         return np.full(x.shape, 1.0)
-    if name in ("expm1", "log1p", "tanh", "sinh", "cosh", "asin", "acos"):
+    if name in ("expm1", "log1p", "tanh", "sinh", "cosh", "asin", "acos", "cbrt", "asinh", "acosh", "atanh"):
         return np.full(x.shape, BOUND3[name][dt])
+    if name == "erf":
+        if dt is np.float32:
+            return np.full(x.shape, 2.0)
+        # f64: the constant term of erf1 (erf_kernel.hpp:108, 1.12837916709551)
+        # is 12 ulp below 2/sqrt(pi), so erf is ~20 ulp off near 0; above 0.65
+        # erf is 1 - erfc and carries erfc's absolute error (see erfc above),
+        # largest at the left end of erfc3's interval
+        return np.where(ax < 0.65, 24.0, np.where(ax < 2.2, 8.0, np.where(ax < 6.0, 80.0, 0.5)))
     if name == "tan":
         # 2 ulp in f32 / 1 in f64 away from the poles; the -1/y branch of an
         # argument within a few ulp of (k + 1/2) pi amplifies the reduction's error
@@ -88,7 +96,8 @@ def _bound(name, x, y, dt):
 
 @pytest.mark.parametrize("dt,tag", [(np.float32, "f32"), (np.float64, "f64")])
 @pytest.mark.parametrize("name", ["exp", "log", "erfc", "normcdf", "cos", "sin", "exp2", "exp10", "log2", "log10",
-                                  "expm1", "log1p", "tanh", "sinh", "cosh", "tan", "asin", "acos"])
+                                  "expm1", "log1p", "tanh", "sinh", "cosh", "tan", "asin", "acos",
+                                  "erf", "cbrt", "asinh", "acosh", "atanh"])
 def test_golden_mpmath(dt, tag, name):
     g = np.load(os.path.join(G, "math_%s.npz" % tag))
     x, y = g[name + "_x"], g[name + "_y"]
@@ -134,7 +143,11 @@ BOUND3 = {"expm1": {np.float32: 2.0, np.float64: 1.0}, "log1p": {np.float32: 1.0
           "cosh": {np.float32: 1.0, np.float64: 1.0},
           # asin f32: pi/2 - 2 asin(sqrt((1-x)/2)) above 1/2 rounds pi/2 and the
           # doubling; NT2's unit test allows 0.5 ulpdist at the special values only
-          "asin": {np.float32: 2.0, np.float64: 1.0}, "acos": {np.float32: 1.0, np.float64: 1.0}}
+          "asin": {np.float32: 2.0, np.float64: 1.0}, "acos": {np.float32: 1.0, np.float64: 1.0},
+          # cbrt: one Newton step (f32) / two (f64) after a degree-4 start;
+          # asinh, acosh, atanh: log / log1p of a rounded argument
+          "cbrt": {np.float32: 1.0, np.float64: 1.0}, "asinh": {np.float32: 2.0, np.float64: 2.0},
+          "acosh": {np.float32: 2.0, np.float64: 1.0}, "atanh": {np.float32: 1.0, np.float64: 1.0}}
 
 MINLOG = {"exp2": {np.float32: np.float32(-127.0), np.float64: -1022.0},
           "exp10": {np.float32: np.uint32(0xc2179999).view(np.float32),
@@ -502,3 +515,124 @@ def test_asin_acos_structure(dt):
         # the tail's acos: (pi/4 - asin(x) + pi/4lo) + pi/4, within an ulp of the pack's
         tc = np.array([mapf("acos", np.array([v]))[0] for v in lo[:3000]])
         assert np.all(classic_ulp(tc, _iso(mapf, "acos", lo[:3000], W), dt) <= 1.0)
+
+
+def _tail(mapf, name, x, dt):
+    """each input alone: NT2's scalar form (a pack's tail)"""
+    return np.array([mapf(name, np.array([v], dt))[0] for v in x], dtype=dt)
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_erf_structure(dt):
+    """erf shares erfc's kernels (erf_kernel.hpp): where erfc is a product p,
+    erf is the rounded 1 - p, and where erf is a product, erfc is 1 - erf,
+    so erf is pinned bit for bit to the erfc pinned above.  erf is odd in a
+    pack and +-1 at +-Inf.  The f64 scalar form (a pack's tail,
+    scalar/erf.hpp:42-74) returns +1 beyond |x| = 6 for either sign and
+    2 a0 / sqrt(pi) up to 1e-10."""
+    mapf = O.map_f32 if dt is np.float32 else O.map_f64
+    W = 8 if dt is np.float32 else 4
+    rng = np.random.default_rng(43)
+    x = np.concatenate([rng.uniform(0, 7, 20000), np.exp2(rng.uniform(-40, -1, 2000))]).astype(dt)
+    e, em = _iso(mapf, "erf", x, W), _iso(mapf, "erf", -x, W)
+    c = _iso(mapf, "erfc", x, W)
+    assert np.array_equal(e, -em)
+    split = dt(2.0 / 3.0) if dt is np.float32 else 0.65
+    hi = x >= split
+    assert np.array_equal(e[hi], (dt(1) - c[hi]).astype(dt))
+    if dt is np.float64:
+        lo = x < split
+        assert np.array_equal(c[lo], 1.0 - e[lo])
+    inf = np.array([np.inf, -np.inf], dt)
+    assert np.array_equal(_iso(mapf, "erf", inf, W), np.array([1, -1], dt))
+    t = _tail(mapf, "erf", np.array([-6.5, -7.0, -30.0, -np.inf, 6.5], dt), dt)
+    if dt is np.float64:
+        assert np.all(t == 1.0)  # NT2's scalar f64 erf: One<A0>() for |x| > 6, sign dropped
+        small = np.array([1e-11, -3e-12, 5e-324], dt)
+        assert np.array_equal(_tail(mapf, "erf", small, dt), small * 2.0 / np.sqrt(np.float64(np.pi)))
+        mid = rng.uniform(-6, 6, 3000)
+        mid = mid[np.abs(np.abs(mid) - 0.65) > 1e-9]
+        assert np.array_equal(_tail(mapf, "erf", mid, dt), _iso(mapf, "erf", mid, W))
+    else:
+        assert np.array_equal(t, np.array([-1, -1, -1, -1, 1], dt))
+        v = rng.uniform(-7, 7, 3000).astype(dt)
+        assert np.array_equal(_tail(mapf, "erf", v, dt), _iso(mapf, "erf", v, W))
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_cbrt_structure(dt):
+    """cbrt is odd bit for bit; scaling the argument by 8^k scales a normal
+    result by exactly 2^k (the exponent's residue mod 3 is unchanged); the
+    pack and scalar forms agree.  Subnormal inputs are scaled by 2^nmb and the
+    root by Twotomnmbo_3 (cbrt.hpp:60-62, 94): in f32 that constant is
+    2^(-23/3) and the result is accurate; in f64 NT2's constant is 2^(-53/3)
+    (exponential/constants/twotomnmbo_3.hpp:30), not 2^(-52/3), so every
+    subnormal f64 cube root is 2^(-1/3) times the true one."""
+    mapf = O.map_f32 if dt is np.float32 else O.map_f64
+    W = 8 if dt is np.float32 else 4
+    U = np.uint32 if dt is np.float32 else np.uint64
+    rng = np.random.default_rng(44)
+    lo, hi = (-100, 100) if dt is np.float32 else (-1000, 1000)
+    x = (np.exp2(rng.uniform(lo, hi, 20000)) * rng.choice([-1, 1], 20000)).astype(dt)
+    r, rm = _iso(mapf, "cbrt", x, W), _iso(mapf, "cbrt", -x, W)
+    assert np.array_equal(r, -rm)
+    for k in (1, 2, 5, -3):
+        r8 = _iso(mapf, "cbrt", (x * dt(8.0) ** k).astype(dt), W)
+        assert np.array_equal(r8, (r * dt(2.0) ** k).astype(dt))
+    assert np.array_equal(_tail(mapf, "cbrt", x[:3000], dt), r[:3000])
+    k = np.arange(1, 200, dtype=dt)
+    assert np.all(classic_ulp(_iso(mapf, "cbrt", k ** 3, W), k.astype(np.float64), dt) <= 1.0)
+    nmb = 23 if dt is np.float32 else 52
+    c = np.array([0x3ba14518], np.uint32).view(np.float32)[0] if dt is np.float32 else \
+        np.array([0x3ed428a2f98d7286], np.uint64).view(np.float64)[0]
+    tiny = np.finfo(dt).tiny
+    sub = (rng.uniform(1e-3, 1, 4000) * tiny).astype(dt)
+    sub = np.concatenate([sub, np.array([1, 2, 3], U).view(dt)])
+    rs = _iso(mapf, "cbrt", sub, W)
+    scaled = _iso(mapf, "cbrt", (sub * dt(2.0) ** nmb).astype(dt), W)
+    assert np.array_equal(rs, (scaled * c).astype(dt))
+    truth = np.cbrt(sub.astype(np.float64)) if dt is np.float32 else np.array(
+        [float(mpmath.cbrt(mpmath.mpf(float(v)))) for v in sub])
+    if dt is np.float32:
+        # one more rounding: the product with the rounded constant 2^(-23/3)
+        assert np.all(classic_ulp(rs, truth, dt) <= 2.0)
+    else:
+        # the constant also sits 5 ulp below 2^(-53/3)
+        ratio = rs / truth
+        assert np.all(np.abs(ratio * 2.0 ** (1.0 / 3.0) - 1.0) <= 8 * np.finfo(dt).eps)
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_inverse_hyperbolic_structure(dt):
+    """asinh and atanh are odd bit for bit; atanh(+-1) = +-Inf and |x| > 1 is
+    NaN; acosh(1) = 0, acosh below 1 is NaN.  Where NT2's scalar and pack forms
+    share their formula (asinh below 1/2 and above 1/sqrt(eps); acosh up to
+    1/(2 eps); every atanh) the tail equals the pack bit for bit; elsewhere
+    the two forms stay within 2 ulp of each other."""
+    mapf = O.map_f32 if dt is np.float32 else O.map_f64
+    W = 8 if dt is np.float32 else 4
+    rng = np.random.default_rng(45)
+    big = 127 if dt is np.float32 else 1000
+    x = np.concatenate([rng.uniform(0, 4, 10000), np.exp2(rng.uniform(-40, big, 10000))]).astype(dt)
+    a, am = _iso(mapf, "asinh", x, W), _iso(mapf, "asinh", -x, W)
+    assert np.array_equal(a, -am)
+    u = rng.uniform(0, 1, 20000).astype(dt)
+    t, tm = _iso(mapf, "atanh", u, W), _iso(mapf, "atanh", -u, W)
+    assert np.array_equal(t, -tm)
+    assert np.array_equal(_iso(mapf, "atanh", np.array([1, -1], dt), W), np.array([np.inf, -np.inf], dt))
+    assert np.all(np.isnan(_iso(mapf, "atanh", np.array([1.5, -2, np.inf], dt), W)))
+    assert _iso(mapf, "acosh", np.array([1.0], dt), W)[0] == 0
+    assert np.all(np.isnan(_iso(mapf, "acosh", np.array([0.999, 0, -1, -np.inf], dt), W)))
+    s = x[:3000]
+    ta, pa = _tail(mapf, "asinh", s, dt), a[:3000]
+    thr = 2896.3093 if dt is np.float32 else 2.0 ** 26
+    same = (s < 0.5) | (s > thr) if dt is np.float32 else (s > thr)
+    assert np.array_equal(ta[same], pa[same])
+    assert np.all(classic_ulp(ta, pa.astype(np.float64), dt) <= 2.0)
+    c = (1.0 + x[:3000]).astype(dt)
+    tc, pc = _tail(mapf, "acosh", c, dt), _iso(mapf, "acosh", c, W)
+    lim = 2.0 ** 22 if dt is np.float32 else 2.0 ** 51
+    low = (c - 1) <= lim
+    assert np.array_equal(tc[low], pc[low])
+    assert np.all(classic_ulp(tc, pc.astype(np.float64), dt) <= 2.0)
+    assert np.array_equal(_tail(mapf, "atanh", u[:3000], dt), t[:3000])
