@@ -178,6 +178,20 @@ int nt2h_sum_dim_f32(const float* x, const size_t* ext, int dim, float* out, int
 int nt2h_sum_dim_f64(const double* x, const size_t* ext, int dim, double* out, int policy, void* stream);
 int nt2h_cumsum_dim_f32(const float* x, const size_t* ext, int dim, float* out, int policy, void* stream);
 int nt2h_cumsum_dim_f64(const double* x, const size_t* ext, int dim, double* out, int policy, void* stream);
+/* cumprod(a, dim), cummin(a, dim), cummax(a, dim): arguments as for cumsum;
+ * each line runs NT2's sequential step bop(summary, x) from the neutral (1,
+ * +Inf, -Inf) under both policies, bit-identical to NT2's CPU result.  min
+ * and max are NT2's (s < x ? s : x), (s > x ? s : x): a NaN element yields NaN
+ * at its position only.  They replace inner_scan_ / outer_scan_ for the
+ * cumulative_ functors cumprod_, cummin_, cummax_
+ * (modules/boost/simd/base/include/boost/simd/swar/functions/cumprod.hpp:27,
+ * cummin.hpp:26, cummax.hpp:27). */
+int nt2h_cumprod_dim_f32(const float* x, const size_t* ext, int dim, float* out, int policy, void* stream);
+int nt2h_cumprod_dim_f64(const double* x, const size_t* ext, int dim, double* out, int policy, void* stream);
+int nt2h_cummin_dim_f32(const float* x, const size_t* ext, int dim, float* out, int policy, void* stream);
+int nt2h_cummin_dim_f64(const double* x, const size_t* ext, int dim, double* out, int policy, void* stream);
+int nt2h_cummax_dim_f32(const float* x, const size_t* ext, int dim, float* out, int policy, void* stream);
+int nt2h_cummax_dim_f64(const double* x, const size_t* ext, int dim, double* out, int policy, void* stream);
 
 /* ---- image demos ------------------------------------------------------ */
 /* grey = 0.3 r + 0.59 g + 0.11 b over n pixels (planar float tables):

@@ -1,5 +1,6 @@
-// sum(a, dim) and cumsum(a, dim) on device tables: the C ABI over
-// nt2/hip/dimwise.hpp (NT2's order, or the parallel fast order).
+// sum(a, dim), cumsum(a, dim), cumprod, cummin and cummax on device tables:
+// the C ABI over nt2/hip/dimwise.hpp (NT2's order, or for sums the parallel
+// fast order).
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
@@ -43,6 +44,26 @@ static int run_dim(const char* name, bool scan, const T* x, const size_t* ext, i
   return NT2H_OK;
 }
 
+// cumprod, cummin, cummax: NT2's sequential order under both policies (the
+// chunked fast form is a sum's: products and NT2's min/max have no exact
+// regrouping)
+template <class T, class Op>
+static int run_cum(const char* name, const T* x, const size_t* ext, int dim, T* out, int policy, void* stream) {
+  if (!ext || dim < 1) return fail(NT2H_EINVAL, name);
+  if (policy != NT2H_EXACT && policy != NT2H_FAST) return fail(NT2H_EINVAL, name);
+  size_t e[4];
+  size_t numel = 1;
+  for (int k = 0; k < 4; ++k) {
+    e[k] = ext[k];
+    numel *= e[k];
+  }
+  if (numel == 0) return NT2H_OK;
+  if (!out || !x) return fail(NT2H_EINVAL, name);
+  dw::launch_scan_ordered<T, Op>(dw::ptr_src<T>{x}, dw::lines_of(e, dim), out, as_stream(stream));
+  NT2H_CHECK_LAUNCH(name);
+  return NT2H_OK;
+}
+
 }  // namespace nt2h
 
 using namespace nt2h;
@@ -60,6 +81,24 @@ int nt2h_cumsum_dim_f32(const float* x, const size_t* ext, int dim, float* out, 
 }
 int nt2h_cumsum_dim_f64(const double* x, const size_t* ext, int dim, double* out, int policy, void* stream) {
   return run_dim<double>("nt2h_cumsum_dim_f64: bad argument", true, x, ext, dim, out, policy, stream);
+}
+int nt2h_cumprod_dim_f32(const float* x, const size_t* ext, int dim, float* out, int policy, void* stream) {
+  return run_cum<float, dw::op_prod>("nt2h_cumprod_dim_f32: bad argument", x, ext, dim, out, policy, stream);
+}
+int nt2h_cumprod_dim_f64(const double* x, const size_t* ext, int dim, double* out, int policy, void* stream) {
+  return run_cum<double, dw::op_prod>("nt2h_cumprod_dim_f64: bad argument", x, ext, dim, out, policy, stream);
+}
+int nt2h_cummin_dim_f32(const float* x, const size_t* ext, int dim, float* out, int policy, void* stream) {
+  return run_cum<float, dw::op_min>("nt2h_cummin_dim_f32: bad argument", x, ext, dim, out, policy, stream);
+}
+int nt2h_cummin_dim_f64(const double* x, const size_t* ext, int dim, double* out, int policy, void* stream) {
+  return run_cum<double, dw::op_min>("nt2h_cummin_dim_f64: bad argument", x, ext, dim, out, policy, stream);
+}
+int nt2h_cummax_dim_f32(const float* x, const size_t* ext, int dim, float* out, int policy, void* stream) {
+  return run_cum<float, dw::op_max>("nt2h_cummax_dim_f32: bad argument", x, ext, dim, out, policy, stream);
+}
+int nt2h_cummax_dim_f64(const double* x, const size_t* ext, int dim, double* out, int policy, void* stream) {
+  return run_cum<double, dw::op_max>("nt2h_cummax_dim_f64: bad argument", x, ext, dim, out, policy, stream);
 }
 
 }  // extern "C"

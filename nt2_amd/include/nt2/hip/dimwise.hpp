@@ -1,5 +1,6 @@
 // Reductions and scans along one dimension of an N-D table (sum(a, dim),
-// cumsum(a, dim)) on gfx950.
+// cumsum(a, dim), and cumprod, cummin, cummax in NT2's order under every
+// policy) on gfx950.
 //
 // A table of extent (d0, d1, d2, d3), column-major, is seen from dimension
 // `dim` as lines: inner = d0 ... d(dim-2) (the stride of the dimension), len =
@@ -84,10 +85,34 @@ __device__ __forceinline__ void two_sum_add(T& s, T& c, T v) {
 
 constexpr int kUnroll = 16;  // loads issued ahead of a chain's additions
 
-// running sum over n elements from idx with stride `step`, added to acc in
-// order (A: accumulator type); f(m, acc) after each addition.  Loads go
-// kUnroll ahead of the additions.
-template <class A, class T, class Src, class F>
+// The scans' operations: NT2's cumulative_<tag, bop, neutral> functors
+// (modules/boost/simd/base/include/boost/simd/swar/functions/cumsum.hpp:27,
+// cumprod.hpp:27, cummin.hpp:26, cummax.hpp:27), each step bop(summary, x)
+// (details/scan_step.hpp:39).  min and max are NT2's scalar forms
+// (boost/simd/arithmetic/functions/scalar/min.hpp:27, max.hpp:27): a NaN
+// element gives NaN at its own position and the next element restarts the
+// chain; of two equal values (+0 and -0) the element wins.
+struct op_sum {
+  template <class A> __device__ __forceinline__ static A neutral() { return A(0); }
+  template <class A> __device__ __forceinline__ static A step(A s, A x) { return s + x; }
+};
+struct op_prod {
+  template <class A> __device__ __forceinline__ static A neutral() { return A(1); }
+  template <class A> __device__ __forceinline__ static A step(A s, A x) { return s * x; }
+};
+struct op_min {
+  template <class A> __device__ __forceinline__ static A neutral() { return A(__builtin_inf()); }
+  template <class A> __device__ __forceinline__ static A step(A s, A x) { return (s < x) ? s : x; }
+};
+struct op_max {
+  template <class A> __device__ __forceinline__ static A neutral() { return A(-__builtin_inf()); }
+  template <class A> __device__ __forceinline__ static A step(A s, A x) { return (s > x) ? s : x; }
+};
+
+// running Op over n elements from idx with stride `step`, applied to acc in
+// order (A: accumulator type); f(m, acc) after each step.  Loads go kUnroll
+// ahead of the dependent steps.
+template <class A, class T, class Op = op_sum, class Src, class F>
 __device__ __forceinline__ void chain(const Src& src, size_t idx, size_t step, size_t n, A& acc, F f) {
   _Pragma("clang fp contract(off)")
   size_t m = 0;
@@ -101,14 +126,14 @@ __device__ __forceinline__ void chain(const Src& src, size_t idx, size_t step, s
     }
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
-      acc = acc + (A)v[u];
+      acc = Op::step(acc, (A)v[u]);
       f(m + u, acc);
     }
   }
   for (; m < n; ++m) {
     T x[1];
     src.template load<1>(idx + m * step, x);
-    acc = acc + (A)x[0];
+    acc = Op::step(acc, (A)x[0]);
     f(m, acc);
   }
 }
@@ -278,7 +303,7 @@ __global__ __launch_bounds__(kStageThreads) void inner_fold_staged_k(Src src, si
 
 // scan of contiguous lines: thread 0 runs the line's chain on each tile in LDS,
 // the workgroup loads and stores the tiles (A: accumulator type)
-template <class T, class A, class Src>
+template <class T, class A, class Src, class Op = op_sum>
 __global__ __launch_bounds__(kStageThreads) void scan_staged_k(Src src, size_t len, T* __restrict__ out) {
   _Pragma("clang fp contract(off)")
   constexpr int TE = stage_elems<T>::value;
@@ -286,7 +311,7 @@ __global__ __launch_bounds__(kStageThreads) void scan_staged_k(Src src, size_t l
   __shared__ T buf[2][TE];
   const size_t base = (size_t)blockIdx.x * len;
   const int t = threadIdx.x;
-  A acc = A(0);
+  A acc = Op::template neutral<A>();
   T v[PER];
   auto load_tile = [&](size_t tb) {
 #pragma unroll
@@ -314,12 +339,12 @@ __global__ __launch_bounds__(kStageThreads) void scan_staged_k(Src src, size_t l
         for (int u = 0; u < kUnroll; ++u) r[u] = buf[b][i + u];
 #pragma unroll
         for (int u = 0; u < kUnroll; ++u) {
-          acc = acc + (A)r[u];
+          acc = Op::step(acc, (A)r[u]);
           buf[b][i + u] = (T)acc;
         }
       }
       for (; i < n; ++i) {
-        acc = acc + (A)buf[b][i];
+        acc = Op::step(acc, (A)buf[b][i]);
         buf[b][i] = (T)acc;
       }
     }
@@ -394,7 +419,7 @@ __global__ __launch_bounds__(256) void outer_fold_exact_k(Src src, size_t inner,
 // ---- scans (A: the accumulator type: T in the exact order, double in the fast) ----
 
 // strided lines: one thread per line, in order over m (coalesced across threads)
-template <class T, class A, class Src>
+template <class T, class A, class Src, class Op = op_sum>
 __global__ __launch_bounds__(256) void scan_strided_k(Src src, size_t inner, size_t len, size_t outer,
                                                       T* __restrict__ out) {
   const size_t line = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -402,8 +427,8 @@ __global__ __launch_bounds__(256) void scan_strided_k(Src src, size_t inner, siz
   const size_t o = line / inner;
   const size_t i = line - o * inner;
   const size_t idx = i + o * inner * len;
-  A acc = A(0);
-  chain<A, T>(src, idx, inner, len, acc, [&](size_t m, A a) { out[idx + m * inner] = (T)a; });
+  A acc = Op::template neutral<A>();
+  chain<A, T, Op>(src, idx, inner, len, acc, [&](size_t m, A a) { out[idx + m * inner] = (T)a; });
 }
 
 // contiguous lines: one wave scans 64 lines, tile by tile of K elements staged
@@ -414,14 +439,14 @@ constexpr size_t kStagedLines = 4096;  // inner folds of fewer, long columns: a 
 constexpr size_t kStagedLen = 65536;
 template <class T> struct row_tile { static constexpr int value = 256 / (int)sizeof(T); };  // 64 f32, 32 f64
 
-template <class T, class A, class Src>
+template <class T, class A, class Src, class Op = op_sum>
 __global__ __launch_bounds__(kRowLines) void scan_rows_k(Src src, size_t len, size_t nlines, T* __restrict__ out) {
   _Pragma("clang fp contract(off)")
   constexpr int K = row_tile<T>::value;
   __shared__ T tile[kRowLines][K + 1];
   const size_t line0 = (size_t)blockIdx.x * kRowLines;
   const int t = threadIdx.x;
-  A acc = A(0);
+  A acc = Op::template neutral<A>();
   for (size_t c0 = 0; c0 < len; c0 += K) {
     // lane t loads column t % K of rows t / K + e * (64 / K): 256-byte runs of one line
     T xs[K];
@@ -442,7 +467,7 @@ __global__ __launch_bounds__(kRowLines) void scan_rows_k(Src src, size_t len, si
     __syncthreads();
     const int kmax = (int)(len - c0 < (size_t)K ? len - c0 : (size_t)K);
     for (int c = 0; c < kmax; ++c) {
-      acc = acc + (A)tile[t][c];
+      acc = Op::step(acc, (A)tile[t][c]);
       tile[t][c] = (T)acc;
     }
     __syncthreads();
@@ -747,20 +772,29 @@ void launch_sum(const Src& src, const lines& L, bool fast, double* scratch, T* o
   }
 }
 
+// NT2's order for any scan operation: one chain per line, in T
+template <class T, class Op, class Src>
+void launch_scan_ordered(const Src& src, const lines& L, T* out, hipStream_t s) {
+  const size_t nl = L.inner * L.outer;
+  if (nl == 0 || L.len == 0) return;
+  if (L.inner == 1 && nl >= kRowsMin)
+    hipLaunchKernelGGL((scan_rows_k<T, T, Src, Op>), dim3(blocks_for(nl, kRowLines)), dim3(kRowLines), 0, s, src,
+                       L.len, nl, out);
+  else if (L.inner == 1 && nl <= 65535)
+    hipLaunchKernelGGL((scan_staged_k<T, T, Src, Op>), dim3((unsigned)nl), dim3(kStageThreads), 0, s, src, L.len,
+                       out);
+  else
+    hipLaunchKernelGGL((scan_strided_k<T, T, Src, Op>), dim3(blocks_for(nl)), dim3(256), 0, s, src, L.inner, L.len,
+                       L.outer, out);
+}
+
 // out has the source's extent (it may be the source's own table)
 template <class T, class Src>
 void launch_scan(const Src& src, const lines& L, bool fast, double* scratch, T* out, hipStream_t s) {
   const size_t nl = L.inner * L.outer;
   if (nl == 0 || L.len == 0) return;
   if (!fast) {
-    if (L.inner == 1 && nl >= kRowsMin)
-      hipLaunchKernelGGL((scan_rows_k<T, T, Src>), dim3(blocks_for(nl, kRowLines)), dim3(kRowLines), 0, s, src,
-                         L.len, nl, out);
-    else if (L.inner == 1 && nl <= 65535)
-      hipLaunchKernelGGL((scan_staged_k<T, T, Src>), dim3((unsigned)nl), dim3(kStageThreads), 0, s, src, L.len, out);
-    else
-      hipLaunchKernelGGL((scan_strided_k<T, T, Src>), dim3(blocks_for(nl)), dim3(256), 0, s, src, L.inner, L.len,
-                         L.outer, out);
+    launch_scan_ordered<T, op_sum>(src, L, out, s);
   } else if (chunked(L)) {
     const size_t nch = chunks_of(L.len);
     hipLaunchKernelGGL((chunk_sum_k<T, Src>), dim3((unsigned)nch, (unsigned)L.outer), dim3(kChunkThreads), 0, s,

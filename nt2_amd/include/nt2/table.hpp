@@ -24,7 +24,8 @@
 //  * sum(a[, dim]) and cumsum(a[, dim]) reduce or scan along a dimension
 //    (default: the first non-singleton one) in NT2's order under the exact
 //    policy (modules/core/reduction/include/nt2/dsl/functions/reduction/run.hpp:30-93),
-//    in the parallel fast order under the fast policy (nt2/hip/dimwise.hpp).
+//    in the parallel fast order under the fast policy (nt2/hip/dimwise.hpp);
+//    cumprod, cummin and cummax scan in NT2's order under both policies.
 //
 // Storage is mirrored: a table keeps a host copy and a device copy with
 // validity flags, so host element access (setup loops) and device evaluation
@@ -1124,11 +1125,13 @@ template <class T, class V, bool PF> struct expr_src {
 };
 }}  // namespace hip::fe
 namespace hip {
-// sum (scan false) or cumsum (scan true) of x along dim into a new table
-template <class T, class EX>
+// sum (scan false) or cumsum (scan true) of x along dim into a new table;
+// Op other than dw::op_sum: that scan (cumprod, cummin, cummax) in NT2's
+// order under every policy
+template <class T, class Op = dw::op_sum, class EX>
 table<T> dimwise(const EX& x, int dim, bool scan, hipStream_t s = nullptr) {
   static_assert(std::is_same<T, float>::value || std::is_same<T, double>::value,
-                "sum/cumsum along a dimension: float or double tables");
+                "sum/cumsum/cumprod/cummin/cummax along a dimension: float or double tables");
   if (dim < 1) throw std::invalid_argument("nt2: dimension must be >= 1");
   const ext_t ex = x.ext();
   if (!ex.has) throw std::invalid_argument("nt2: reduction of a scalar expression");
@@ -1139,13 +1142,15 @@ table<T> dimwise(const EX& x, int dim, bool scan, hipStream_t s = nullptr) {
   if (r.numel() == 0) return r;
   const size_t e4[4] = {ex.s.d[0], ex.s.d[1], ex.s.d[2], ex.s.d[3]};
   const dw::lines L = dw::lines_of(e4, dim);
-  const bool fast = NT2_HIP_POLICY::fused;
+  constexpr bool summing = std::is_same<Op, dw::op_sum>::value;
+  const bool fast = NT2_HIP_POLICY::fused && summing;
   double* scratch = nullptr;
   if (const size_t need = dw::scratch_doubles(L, fast, scan)) fe::check(hipMalloc(&scratch, need * sizeof(double)), "scratch");
   auto v = expr::dview<EX>::get(x);
   const fe::expr_src<T, decltype(v), expr::pack_free<EX>::value> src{v};
   T* out = r.device_data_for_write();
-  if (scan) dw::launch_scan<T>(src, L, fast, scratch, out, s);
+  if (!summing) dw::launch_scan_ordered<T, Op>(src, L, out, s);
+  else if (scan) dw::launch_scan<T>(src, L, fast, scratch, out, s);
   else dw::launch_sum<T>(src, L, fast, scratch, out, s);
   fe::check(hipGetLastError(), "dimwise launch");
   if (scratch) {
@@ -1187,6 +1192,27 @@ auto cumsum(const E& e) {
   const ext_t ex = x.ext();
   return cumsum(e, ex.has ? firstnonsingleton(ex.s) : 1);
 }
+// cumprod, cummin, cummax (modules/core/swar/include/nt2/swar/functions/
+// container/cumprod.hpp, cummin.hpp, cummax.hpp): running product / minimum /
+// maximum along a dimension, the first non-singleton one by default
+#define NT2H_CUMOP(name, Op)                                                      \
+  template <class E, NT2H_ENABLE1(E)>                                            \
+  auto name(const E& e, int dim) {                                               \
+    typedef typename expr::wrap<E>::type EX;                                     \
+    typedef typename expr::vt<EX>::type T;                                       \
+    return hip::dimwise<T, hip::dw::Op>(expr::wrap<E>::get(e), dim, true);      \
+  }                                                                              \
+  template <class E, NT2H_ENABLE1(E)>                                            \
+  auto name(const E& e) {                                                        \
+    typedef typename expr::wrap<E>::type EX;                                     \
+    const EX& x = expr::wrap<E>::get(e);                                         \
+    const ext_t ex = x.ext();                                                    \
+    return name(e, ex.has ? firstnonsingleton(ex.s) : 1);                        \
+  }
+NT2H_CUMOP(cumprod, op_prod)
+NT2H_CUMOP(cummin, op_min)
+NT2H_CUMOP(cummax, op_max)
+#undef NT2H_CUMOP
 
 // ---- reshape and colon ---------------------------------------------------------------
 // reshape(a, sizes): the elements of a (evaluated if an expression) with a new

@@ -74,6 +74,12 @@ def lib():
         "nt2h_sum_dim_f64": ([P, P, I, P, I, P], I),
         "nt2h_cumsum_dim_f32": ([P, P, I, P, I, P], I),
         "nt2h_cumsum_dim_f64": ([P, P, I, P, I, P], I),
+        "nt2h_cumprod_dim_f32": ([P, P, I, P, I, P], I),
+        "nt2h_cumprod_dim_f64": ([P, P, I, P, I, P], I),
+        "nt2h_cummin_dim_f32": ([P, P, I, P, I, P], I),
+        "nt2h_cummin_dim_f64": ([P, P, I, P, I, P], I),
+        "nt2h_cummax_dim_f32": ([P, P, I, P, I, P], I),
+        "nt2h_cummax_dim_f64": ([P, P, I, P, I, P], I),
         "nt2h_rgb2grey_f32": ([S, P, P, P, P, I, P], I),
         "nt2h_rgb2yuv_f32": ([S, P, P, P, P, P, P, I, P], I),
         "nt2h_rgb2grey_u8": ([S, P, P, P, P, P], I),

@@ -147,12 +147,15 @@ def sum_dim(x, ext, dim, policy=EXACT, out=None):
     return out
 
 
-def cumsum_dim(x, ext, dim, policy=EXACT, out=None):
-    """NT2's cumsum(a, dim) of the device table x (flat, column-major)."""
+def cumsum_dim(x, ext, dim, policy=EXACT, out=None, op="cumsum"):
+    """NT2's cumsum(a, dim) of the device table x (flat, column-major); op
+    "cumprod", "cummin" or "cummax" runs that scan instead."""
+    if op not in ("cumsum", "cumprod", "cummin", "cummax"):
+        raise ValueError("unknown scan %r" % op)
     e4, _ = _ext4(ext)
     out = torch.empty_like(x) if out is None else out
-    f = lib().nt2h_cumsum_dim_f32 if x.dtype == torch.float32 else lib().nt2h_cumsum_dim_f64
-    check(f(ptr(x), ctypes.cast(e4, ctypes.c_void_p), dim, ptr(out), policy, _s()), "cumsum_dim")
+    f = getattr(lib(), "nt2h_%s_dim_%s" % (op, "f32" if x.dtype == torch.float32 else "f64"))
+    check(f(ptr(x), ctypes.cast(e4, ctypes.c_void_p), dim, ptr(out), policy, _s()), op + "_dim")
     return out
 
 

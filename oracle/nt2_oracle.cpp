@@ -1782,6 +1782,30 @@ void cumsum_dim(const T* x, const size_t* ext, int dim, T* out) {
       }
   }
 }
+// cumprod, cummin, cummax: the cumulative_ functors
+// (modules/boost/simd/base/include/boost/simd/swar/functions/cumprod.hpp:27
+// multiplies_ from One, cummin.hpp:26 min_ from Inf, cummax.hpp:27 max_ from
+// Minf), each step summary = bop(summary, x) in order (details/scan_step.hpp:39);
+// scalar min / max: boost/simd/arithmetic/functions/scalar/min.hpp:27, max.hpp:27.
+template <class T>
+void cumop_dim(const T* x, const size_t* ext, int dim, int op, T* out) {
+  const Lines L = lines_of(ext, dim);
+  const T neutral = op == 1 ? T(1) : (op == 2 ? T(INFINITY) : T(-INFINITY));
+  for (size_t o = 0; o < L.outer; ++o) {
+    const size_t b = L.inner * L.len * o;
+    for (size_t i = 0; i < L.inner; ++i) {
+      T s = neutral;
+      for (size_t m = 0; m < L.len; ++m) {
+        const size_t idx = b + i + L.inner * m;
+        const T v = x[idx];
+        if (op == 1) s = s * v;
+        else if (op == 2) s = (s < v) ? s : v;
+        else s = (s > v) ? s : v;
+        out[idx] = s;
+      }
+    }
+  }
+}
 
 }  // namespace
 
@@ -2287,6 +2311,17 @@ int oracle_cumsum_dim_f32(const float* x, const size_t* ext, int dim, float* out
 int oracle_cumsum_dim_f64(const double* x, const size_t* ext, int dim, double* out) {
   if (dim < 1) return -1;
   cumsum_dim<double>(x, ext, dim, out);
+  return 0;
+}
+// op: 1 cumprod, 2 cummin, 3 cummax
+int oracle_cumop_dim_f32(const float* x, const size_t* ext, int dim, int op, float* out) {
+  if (dim < 1 || op < 1 || op > 3) return -1;
+  cumop_dim<float>(x, ext, dim, op, out);
+  return 0;
+}
+int oracle_cumop_dim_f64(const double* x, const size_t* ext, int dim, int op, double* out) {
+  if (dim < 1 || op < 1 || op > 3) return -1;
+  cumop_dim<double>(x, ext, dim, op, out);
   return 0;
 }
 

@@ -55,6 +55,8 @@ def lib():
         _lib.oracle_expr4_sum_fp64.restype = ctypes.c_double
         for f in ("oracle_sum_dim_f32", "oracle_sum_dim_f64", "oracle_cumsum_dim_f32", "oracle_cumsum_dim_f64"):
             getattr(_lib, f).argtypes = [P, P, ctypes.c_int, P]
+        for f in ("oracle_cumop_dim_f32", "oracle_cumop_dim_f64"):
+            getattr(_lib, f).argtypes = [P, P, ctypes.c_int, ctypes.c_int, P]
         _lib.oracle_rgb2grey_f32.argtypes = [S, P, P, P, P]
         _lib.oracle_rgb2yuv_f32.argtypes = [S, P, P, P, P, P, P]
         _lib.oracle_rgb2grey_u8.argtypes = [S, P, P, P, P]
@@ -240,6 +242,21 @@ def cumsum_dim(x, ext, dim):
     out = np.empty_like(x)
     f = lib().oracle_cumsum_dim_f32 if x.dtype == np.float32 else lib().oracle_cumsum_dim_f64
     assert f(_p(x), _p(e), dim, _p(out)) == 0
+    return out
+
+
+CUMOPS = {"cumprod": 1, "cummin": 2, "cummax": 3}
+
+
+def cumop_dim(x, ext, dim, op):
+    """NT2's cumprod / cummin / cummax(a, dim) of the flat column-major table x."""
+    if op == "cumsum":
+        return cumsum_dim(x, ext, dim)
+    x = np.ascontiguousarray(x)
+    e = _ext4(ext)
+    out = np.empty_like(x)
+    f = lib().oracle_cumop_dim_f32 if x.dtype == np.float32 else lib().oracle_cumop_dim_f64
+    assert f(_p(x), _p(e), dim, CUMOPS[op], _p(out)) == 0
     return out
 
 

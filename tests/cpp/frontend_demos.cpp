@@ -184,6 +184,12 @@ int main(int argc, char** argv) {
   dump("M_sumcos2", s2);
   s1 = sum(M * 2.0f + 1.0f);  // first non-singleton dimension: 1
   dump("M_sumexpr", s1);
+  c1 = cumprod(M / 1.0e4f + 1.0f, 1);
+  dump("M_cumprod1", c1);
+  c2 = cummin(M, 2);
+  dump("M_cummin2", c2);
+  c1 = cummax(nt2::cos(M));  // scans evaluate each element in NT2's scalar form
+  dump("M_cummaxcos1", c1);
   table<float> v = M(1) * ones(of_size(1000), meta::as_<float>());
   const float vs = sum(v);  // a one-element result used as a scalar
   dump_scalar("v_sum", vs);

@@ -105,3 +105,50 @@ def test_argument_errors():
         ops.sum_dim(xd, (8,), 0, EXACT)
     with pytest.raises(Nt2hError):
         ops.sum_dim(xd, (8,), 1, 7)
+
+
+def _cum_inputs(op, n, dtype, rng):
+    if op == "cumprod":  # magnitudes near 1: long chains stay finite
+        x = rng.uniform(0.97, 1.03, n) * rng.choice([-1, 1], n)
+    else:
+        x = rng.uniform(-1, 1, n) * np.where(rng.random(n) < 0.1, 1e6, 1)
+    x = x.astype(dtype)
+    k = max(1, n // 50)
+    idx = rng.integers(0, max(n, 1), k) if n else np.zeros(0, int)
+    x[idx[: k // 3]] = np.nan
+    x[idx[k // 3: 2 * k // 3]] = 0.0
+    x[idx[2 * k // 3:]] = -0.0
+    return x
+
+
+@pytest.mark.parametrize("op", ["cumprod", "cummin", "cummax"])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("shape", SHAPES + [(3, 20000), (20000, 3)])
+def test_cumulative_matches_oracle(shape, dtype, op):
+    """cumprod / cummin / cummax: NT2's order under both policies, bit for bit
+    (NaN, +0 and -0 sprinkled in; (3, 20000) along 1 takes the many-lines
+    kernel, (20000, 3) along 1 the workgroup-per-line one, dim 2 the strided)."""
+    rng = np.random.default_rng(sum(shape) + 11 * len(shape))
+    n = int(np.prod(shape))
+    x = _cum_inputs(op, n, dtype, rng)
+    xd = _dev(x)
+    for dim in range(1, 4):
+        want = O.cumop_dim(x, shape, dim, op)
+        for pol in (EXACT, FAST):
+            say("cumulative", op, shape, dtype.__name__, "dim=%d pol=%d" % (dim, pol))
+            got = ops.cumsum_dim(xd, shape, dim, pol, op=op).cpu().numpy()
+            assert same_bits(got, want), (op, shape, dim, pol)
+
+
+def test_cumulative_in_place_and_errors():
+    from nt2_amd.native import Nt2hError
+    x = np.random.default_rng(4).uniform(0.5, 1.5, 3 * 20000).astype(np.float32)
+    for op in ("cumprod", "cummin", "cummax"):
+        want = ops.cumsum_dim(_dev(x), (3, 20000), 1, EXACT, op=op)
+        xd = _dev(x.copy())
+        ops.cumsum_dim(xd, (3, 20000), 1, EXACT, out=xd, op=op)
+        assert torch.equal(xd, want)
+        with pytest.raises(Nt2hError):
+            ops.cumsum_dim(_dev(x), (3, 20000), 0, EXACT, op=op)
+        with pytest.raises(Nt2hError):
+            ops.cumsum_dim(_dev(x), (3, 20000), 1, 7, op=op)

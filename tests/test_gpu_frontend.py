@@ -113,6 +113,11 @@ def test_dimwise_statements_bitwise(outputs):
     assert same_bits(outputs("M_sumexpr", np.float32), O.sum_dim(e.astype(np.float32), ext, 1))
     v = np.full(1000, M[0], dtype=np.float32)
     assert outputs("v_sum", np.float32)[0] == O.sum_dim(v, (1000,), 1)[0]
+    p = (M / np.float32(1.0e4)).astype(np.float32) + np.float32(1.0)
+    assert same_bits(outputs("M_cumprod1", np.float32), O.cumop_dim(p, ext, 1, "cumprod"))
+    assert same_bits(outputs("M_cummin2", np.float32), O.cumop_dim(M, ext, 2, "cummin"))
+    cs = np.array([O.map_f32("cos", np.array([v], np.float32))[0] for v in M], np.float32)
+    assert same_bits(outputs("M_cummaxcos1", np.float32), O.cumop_dim(cs, ext, 1, "cummax"))
 
 
 def test_exponential_rounding_statements(outputs):

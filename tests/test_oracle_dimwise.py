@@ -1,4 +1,5 @@
-"""Pinning the oracle's sum(a, dim) and cumsum(a, dim) (CPU only).
+"""Pinning the oracle's sum(a, dim), cumsum(a, dim), cumprod, cummin and
+cummax (CPU only).
 
 * NT2's own unit-test known answers: modules/core/reduction/unit/table/sum.cpp
   (sum_1D, sum_2D) and modules/core/swar/unit/table/cumsum.cpp (cumsum_inner,
@@ -103,3 +104,54 @@ def test_sum_and_cumsum_bounds(shape, dtype):
             # a dimension beyond the extent has size 1: both are copies (0 + x)
             assert np.array_equal(s, x + dtype(0))
             assert np.array_equal(c, x + dtype(0))
+
+
+def test_nt2_cumprod_kats():
+    # modules/core/swar/unit/table/cumprod.cpp:20-44: ones(3, 5, 2) is its own
+    # cumprod along 1 and 2; cumprod of the row 1..5 along 2 is 1 2 6 24 120,
+    # along 1 the row itself
+    a = np.ones(30)
+    assert list(O.cumop_dim(a, (3, 5, 2), 1, "cumprod")) == list(a)
+    assert list(O.cumop_dim(a, (3, 5, 2), 2, "cumprod")) == list(a)
+    aa = np.arange(1, 6, dtype=np.float64)
+    assert list(O.cumop_dim(aa, (1, 5), 2, "cumprod")) == [1, 2, 6, 24, 120]
+    assert list(O.cumop_dim(aa, (1, 5), 1, "cumprod")) == list(aa)
+    assert list(O.cumop_dim(aa.astype(np.float32), (1, 5), 2, "cumprod")) == [1, 2, 6, 24, 120]
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("shape,dim", [((1000,), 1), ((7, 300), 1), ((7, 300), 2), ((4, 5, 6), 2), ((4, 5, 6), 3)])
+def test_cumulative_against_numpy_accumulate(shape, dim, dtype):
+    """NaN-free tables without zeros: NT2's sequential product equals numpy's
+    multiply.accumulate in the same precision (one rounding per step, in
+    order), and its min / max chains equal minimum / maximum.accumulate."""
+    rng = np.random.default_rng(len(shape) * 10 + dim)
+    n = int(np.prod(shape))
+    x = (rng.uniform(0.5, 1.5, n) * rng.choice([-1, 1], n)).astype(dtype)
+    a = x.reshape(shape, order="F")
+    ax = dim - 1
+    for op, ref in (("cumprod", np.multiply.accumulate), ("cummin", np.minimum.accumulate),
+                    ("cummax", np.maximum.accumulate)):
+        want = ref(a, axis=ax, dtype=dtype).ravel(order="F")
+        got = O.cumop_dim(x, shape, dim, op)
+        assert np.array_equal(got.view(np.uint8), want.view(np.uint8)), (op, shape, dim)
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_cummin_cummax_nan_and_signed_zero(dtype):
+    """NT2's scalar min / max (s < x ? s : x, s > x ? s : x): a NaN element is
+    kept at its own position and the chain restarts from the next element;
+    between +0 and -0 the later element wins.  fmin/fmax or a NaN-propagating
+    min would give other values at every position marked here."""
+    nan = np.nan
+    x = np.array([3.0, nan, 5.0, 1.0, 2.0, nan, nan, -4.0], dtype)
+    mn = O.cumop_dim(x, (8,), 1, "cummin")
+    mx = O.cumop_dim(x, (8,), 1, "cummax")
+    assert np.array_equal(mn, np.array([3, nan, 5, 1, 1, nan, nan, -4], dtype), equal_nan=True)
+    assert np.array_equal(mx, np.array([3, nan, 5, 5, 5, nan, nan, -4], dtype), equal_nan=True)
+    z = np.array([0.0, -0.0, 0.0, -0.0], dtype)
+    assert np.signbit(O.cumop_dim(z, (4,), 1, "cummin")).tolist() == [False, True, False, True]
+    assert np.signbit(O.cumop_dim(z, (4,), 1, "cummax")).tolist() == [False, True, False, True]
+    inf = np.array([np.inf, -np.inf, 1.0], dtype)
+    assert O.cumop_dim(inf, (3,), 1, "cummin").tolist() == [np.inf, -np.inf, -np.inf]
+    assert O.cumop_dim(inf, (3,), 1, "cummax").tolist() == [np.inf, np.inf, np.inf]
