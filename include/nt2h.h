@@ -46,7 +46,7 @@ extern "C" {
 
 /* elementwise operations for nt2h_map_* (NT2 functors exp_, log_, sqrt_, erfc_,
  * normcdf_, fastnormcdf_, cos_, sin_, abs_, exp2_ ... atan_, tan_, asin_, acos_, erf_,
- * cbrt_, asinh_, acosh_, atanh_) */
+ * cbrt_, asinh_, acosh_, atanh_, cot_, sec_, csc_, sinpi_ ... cotpi_, sind_ ... cotd_) */
 #define NT2H_OP_EXP 0
 #define NT2H_OP_LOG 1
 #define NT2H_OP_SQRT 2
@@ -80,6 +80,17 @@ extern "C" {
 #define NT2H_OP_ASINH 30
 #define NT2H_OP_ACOSH 31
 #define NT2H_OP_ATANH 32
+#define NT2H_OP_COT 33   /* radian, per-pack range reduction as cos/sin/tan */
+#define NT2H_OP_SEC 34   /* 1/cos */
+#define NT2H_OP_CSC 35   /* 1/sin */
+#define NT2H_OP_SINPI 36 /* sin(pi x); pi and degree forms reduce lane by lane */
+#define NT2H_OP_COSPI 37
+#define NT2H_OP_TANPI 38
+#define NT2H_OP_COTPI 39
+#define NT2H_OP_SIND 40  /* sin of x degrees */
+#define NT2H_OP_COSD 41
+#define NT2H_OP_TAND 42
+#define NT2H_OP_COTD 43
 
 /* Black-Scholes formula variants */
 #define NT2H_BS_TEXTBOOK 0 /* d1 = (log(S/X)+(r+v^2/2)T)/(v sqrt T), call and put, erfc-based normcdf */

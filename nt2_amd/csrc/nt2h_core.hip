@@ -133,12 +133,25 @@ __global__ void fill_frames_k(uint8_t* frames, int h, int w, int nframes, Rects 
 // transform_: elementwise maps with NT2 pack semantics.  Work item t < npacks
 // is the pack [W*t, W*t+W); items beyond are the scalar tail elements.
 // ============================================================================
+// the radian trigonometric ops, whose range reduction is chosen per pack
+template <int OP> constexpr bool radian_trig() {
+  return OP == NT2H_OP_COS || OP == NT2H_OP_SIN || OP == NT2H_OP_TAN || OP == NT2H_OP_COT || OP == NT2H_OP_SEC ||
+         OP == NT2H_OP_CSC;
+}
+template <int OP> constexpr int radian_kind() {
+  return OP == NT2H_OP_COS ? dm::k_cos
+         : OP == NT2H_OP_SIN ? dm::k_sin
+         : OP == NT2H_OP_TAN ? dm::k_tan
+         : OP == NT2H_OP_COT ? dm::k_cot
+         : OP == NT2H_OP_SEC ? dm::k_sec
+                             : dm::k_csc;
+}
+
 template <class P, int OP, int G>
 __device__ __forceinline__ void apply_f32(const float* x, float* y) {
   constexpr bool scalar = (G == 1);
-  if constexpr (OP == NT2H_OP_COS || OP == NT2H_OP_SIN || OP == NT2H_OP_TAN) {
-    dm::trig_group_f<P, G>(OP == NT2H_OP_COS ? dm::k_cos : (OP == NT2H_OP_SIN ? dm::k_sin : dm::k_tan), x, y,
-                           nullptr);
+  if constexpr (radian_trig<OP>()) {
+    dm::trig_group_f<P, G>(radian_kind<OP>(), x, y, nullptr);
   } else {
 #pragma unroll
     for (int l = 0; l < G; ++l) {
@@ -173,6 +186,14 @@ __device__ __forceinline__ void apply_f32(const float* x, float* y) {
       else if constexpr (OP == NT2H_OP_ASINH) r = dm::asinh_f<P, scalar>(a);
       else if constexpr (OP == NT2H_OP_ACOSH) r = dm::acosh_f<P, scalar>(a);
       else if constexpr (OP == NT2H_OP_ATANH) r = dm::atanh_f<P>(a);
+      else if constexpr (OP == NT2H_OP_SINPI) r = dm::trigu_f<P, dm::u_pi>(dm::k_sin, a);
+      else if constexpr (OP == NT2H_OP_COSPI) r = dm::trigu_f<P, dm::u_pi>(dm::k_cos, a);
+      else if constexpr (OP == NT2H_OP_TANPI) r = dm::trigu_f<P, dm::u_pi>(dm::k_tan, a);
+      else if constexpr (OP == NT2H_OP_COTPI) r = dm::trigu_f<P, dm::u_pi>(dm::k_cot, a);
+      else if constexpr (OP == NT2H_OP_SIND) r = dm::trigu_f<P, dm::u_deg>(dm::k_sin, a);
+      else if constexpr (OP == NT2H_OP_COSD) r = dm::trigu_f<P, dm::u_deg>(dm::k_cos, a);
+      else if constexpr (OP == NT2H_OP_TAND) r = dm::trigu_f<P, dm::u_deg>(dm::k_tan, a);
+      else if constexpr (OP == NT2H_OP_COTD) r = dm::trigu_f<P, dm::u_deg>(dm::k_cot, a);
       else r = __builtin_fabsf(a);
       y[l] = r;
     }
@@ -182,9 +203,8 @@ __device__ __forceinline__ void apply_f32(const float* x, float* y) {
 template <class P, int OP, int G>
 __device__ __forceinline__ void apply_f64(const double* x, double* y) {
   constexpr bool scalar = (G == 1);
-  if constexpr (OP == NT2H_OP_COS || OP == NT2H_OP_SIN || OP == NT2H_OP_TAN) {
-    dm::trig_group_d<P, G>(OP == NT2H_OP_COS ? dm::k_cos : (OP == NT2H_OP_SIN ? dm::k_sin : dm::k_tan), x, y,
-                           nullptr);
+  if constexpr (radian_trig<OP>()) {
+    dm::trig_group_d<P, G>(radian_kind<OP>(), x, y, nullptr);
   } else {
 #pragma unroll
     for (int l = 0; l < G; ++l) {
@@ -218,6 +238,14 @@ __device__ __forceinline__ void apply_f64(const double* x, double* y) {
       else if constexpr (OP == NT2H_OP_ASINH) r = dm::asinh_d<P, scalar>(a);
       else if constexpr (OP == NT2H_OP_ACOSH) r = dm::acosh_d<P, scalar>(a);
       else if constexpr (OP == NT2H_OP_ATANH) r = dm::atanh_d<P>(a);
+      else if constexpr (OP == NT2H_OP_SINPI) r = dm::trigu_d<P, dm::u_pi>(dm::k_sin, a);
+      else if constexpr (OP == NT2H_OP_COSPI) r = dm::trigu_d<P, dm::u_pi>(dm::k_cos, a);
+      else if constexpr (OP == NT2H_OP_TANPI) r = dm::trigu_d<P, dm::u_pi>(dm::k_tan, a);
+      else if constexpr (OP == NT2H_OP_COTPI) r = dm::trigu_d<P, dm::u_pi>(dm::k_cot, a);
+      else if constexpr (OP == NT2H_OP_SIND) r = dm::trigu_d<P, dm::u_deg>(dm::k_sin, a);
+      else if constexpr (OP == NT2H_OP_COSD) r = dm::trigu_d<P, dm::u_deg>(dm::k_cos, a);
+      else if constexpr (OP == NT2H_OP_TAND) r = dm::trigu_d<P, dm::u_deg>(dm::k_tan, a);
+      else if constexpr (OP == NT2H_OP_COTD) r = dm::trigu_d<P, dm::u_deg>(dm::k_cot, a);
       else r = __builtin_fabs(a);
       y[l] = r;
     }
@@ -889,7 +917,9 @@ static int launch_map_f32(int op, size_t n, const float* x, float* y, hipStream_
     NT2H_MAP32(NT2H_OP_SINH) NT2H_MAP32(NT2H_OP_COSH) NT2H_MAP32(NT2H_OP_ATAN) NT2H_MAP32(NT2H_OP_TAN)
     NT2H_MAP32(NT2H_OP_ASIN) NT2H_MAP32(NT2H_OP_ACOS)
     NT2H_MAP32(NT2H_OP_ERF) NT2H_MAP32(NT2H_OP_CBRT) NT2H_MAP32(NT2H_OP_ASINH) NT2H_MAP32(NT2H_OP_ACOSH)
-    NT2H_MAP32(NT2H_OP_ATANH)
+    NT2H_MAP32(NT2H_OP_ATANH) NT2H_MAP32(NT2H_OP_COT) NT2H_MAP32(NT2H_OP_SEC) NT2H_MAP32(NT2H_OP_CSC)
+    NT2H_MAP32(NT2H_OP_SINPI) NT2H_MAP32(NT2H_OP_COSPI) NT2H_MAP32(NT2H_OP_TANPI) NT2H_MAP32(NT2H_OP_COTPI)
+    NT2H_MAP32(NT2H_OP_SIND) NT2H_MAP32(NT2H_OP_COSD) NT2H_MAP32(NT2H_OP_TAND) NT2H_MAP32(NT2H_OP_COTD)
 #undef NT2H_MAP32
     default: return fail(NT2H_EINVAL, "nt2h_map_f32: unknown op");
   }
@@ -919,7 +949,9 @@ static int launch_map_f64(int op, size_t n, const double* x, double* y, hipStrea
     NT2H_MAP64(NT2H_OP_SINH) NT2H_MAP64(NT2H_OP_COSH) NT2H_MAP64(NT2H_OP_ATAN) NT2H_MAP64(NT2H_OP_TAN)
     NT2H_MAP64(NT2H_OP_ASIN) NT2H_MAP64(NT2H_OP_ACOS)
     NT2H_MAP64(NT2H_OP_ERF) NT2H_MAP64(NT2H_OP_CBRT) NT2H_MAP64(NT2H_OP_ASINH) NT2H_MAP64(NT2H_OP_ACOSH)
-    NT2H_MAP64(NT2H_OP_ATANH)
+    NT2H_MAP64(NT2H_OP_ATANH) NT2H_MAP64(NT2H_OP_COT) NT2H_MAP64(NT2H_OP_SEC) NT2H_MAP64(NT2H_OP_CSC)
+    NT2H_MAP64(NT2H_OP_SINPI) NT2H_MAP64(NT2H_OP_COSPI) NT2H_MAP64(NT2H_OP_TANPI) NT2H_MAP64(NT2H_OP_COTPI)
+    NT2H_MAP64(NT2H_OP_SIND) NT2H_MAP64(NT2H_OP_COSD) NT2H_MAP64(NT2H_OP_TAND) NT2H_MAP64(NT2H_OP_COTD)
 #undef NT2H_MAP64
     default: return fail(NT2H_EINVAL, "nt2h_map_f64: unknown op (fastnormcdf is single precision only)");
   }

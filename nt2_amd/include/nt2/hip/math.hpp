@@ -1773,7 +1773,7 @@ NT2H_FN double sin_eval_d(double z, double x) {
   return fma<P>(y1 * z, x, x);
 }
 
-enum trig_kind { k_cos = 0, k_sin = 1, k_sincos = 2, k_tan = 3 };
+enum trig_kind { k_cos = 0, k_sin = 1, k_sincos = 2, k_tan = 3, k_cot = 4, k_sec = 5, k_csc = 6 };
 
 // tan on the reduced argument: f_trig_evaluation.hpp:47-64 (f32 polynomial in
 // z^2, z + z^3 P) and d_trig_evaluation.hpp:58-78 (f64 rational); n odd gives
@@ -1798,6 +1798,18 @@ NT2H_FN double tan_eval_d(double z, int n) {
   const double zz = z * z;
   const double y = z + z * (zz * (horner<P>(num, zz) / horner<P>(den, zz)));
   return (n & 1) ? -(1.0 / y) : y;
+}
+// cot on the reduced argument: the same base y, 1/y for even n, -y for odd
+// (f_trig_evaluation.hpp / d_trig_evaluation.hpp cot_eval)
+template <class P>
+NT2H_FN float cot_eval_f(float z, int n) {
+  const float y = tan_eval_f<P>(z, 0);
+  return (n & 1) ? -y : 1.0f / y;
+}
+template <class P>
+NT2H_FN double cot_eval_d(double z, int n) {
+  const double y = tan_eval_d<P>(z, 0);
+  return (n & 1) ? -y : 1.0 / y;
 }
 
 // cos / sin / sincos over a group of G lanes: SIMD trig_base
@@ -1831,9 +1843,19 @@ NT2H_FN void trig_group_f(int kind, const float* a0, float* out0, float* out1, b
       out0[i] = t;
       continue;
     }
+    if (kind == k_cot) {
+      // packs: trig_base.hpp:126-138, rec(a0) at +-0; scalar: :178-188
+      float t = as_f(bits(cot_eval_f<P>(xr[i], n[i])) ^ (bits(a0[i]) & 0x80000000u));
+      if (G == 1 && !__builtin_isfinite(a0[i])) t = __builtin_nanf("");
+      else if (a0[i] == 0.0f) t = 1.0f / a0[i];
+      out0[i] = t;
+      continue;
+    }
     if (G == 1 && !__builtin_isfinite(a0[i])) { c = __builtin_nanf(""); s = c; }
     if (kind == k_cos) out0[i] = c;
     else if (kind == k_sin) out0[i] = s;
+    else if (kind == k_sec) out0[i] = 1.0f / c;  // generic/sec.hpp: rec(cos(a0))
+    else if (kind == k_csc) out0[i] = 1.0f / s;  // generic/csc.hpp: rec(sin(a0))
     else { out0[i] = s; out1[i] = c; }
   }
 }
@@ -1863,11 +1885,94 @@ NT2H_FN void trig_group_d(int kind, const double* a0, double* out0, double* out1
       out0[i] = t;
       continue;
     }
+    if (kind == k_cot) {
+      double t = as_d(bits(cot_eval_d<P>(xr[i], n[i])) ^ (bits(a0[i]) & 0x8000000000000000ull));
+      if (G == 1 && !__builtin_isfinite(a0[i])) t = __builtin_nan("");
+      else if (a0[i] == 0.0) t = 1.0 / a0[i];
+      out0[i] = t;
+      continue;
+    }
     if (G == 1 && !__builtin_isfinite(a0[i])) { c = __builtin_nan(""); s = c; }
     if (kind == k_cos) out0[i] = c;
     else if (kind == k_sin) out0[i] = s;
+    else if (kind == k_sec) out0[i] = 1.0 / c;
+    else if (kind == k_csc) out0[i] = 1.0 / s;
     else { out0[i] = s; out1[i] = c; }
   }
+}
+
+// ---------------------------------------------------------------------------
+// Trigonometric functions of pi x and of x degrees: sinpi, cospi, tanpi,
+// cotpi, sind, cosd, tand, cotd (generic/sinpi.hpp ... cotd.hpp: trig_base
+// with pi_tag / degree_tag).  Their reductions work lane by lane
+// (scalar/impl/trigo/trig_reduction.hpp:262-318), so packs and the scalar
+// tail give the same values:
+//   pi:     xi = round2even(2|x|), xr = (|x| - xi/2) * Pi
+//   degree: xi = round2even(|x| * (1/90)), xr = (|x| - 90 xi) * Deginrad
+// n = xi converted as x86 converts (cvtt: beyond the int range, or NaN, the
+// most negative integer).  Then sin/cos/tan/cot of xr with the quadrant's
+// swap and sign as in the radian functions.  tan is NaN where x - 1/2 (pi) or
+// (x - 90)/180 (degree, times 1/180 rounded) is an integer; cot is NaN at
+// nonzero integers (multiples of 180) and 1/x at +-0.
+// ---------------------------------------------------------------------------
+enum trig_unit { u_pi = 0, u_deg = 1 };
+NT2H_FN bool is_flint(float v) { return v - __builtin_truncf(v) == 0.0f; }
+NT2H_FN bool is_flint(double v) { return v - __builtin_trunc(v) == 0.0; }
+template <class P, int U>
+NT2H_FN float trigu_f(int kind, float a0) {
+  NT2H_STRICT
+  const float x = __builtin_fabsf(a0);
+  float xi, xr;
+  if (U == u_pi) {
+    xi = __builtin_rintf(x * 2.0f);
+    xr = (x - xi * 0.5f) * NT2H_F(0x40490fdb);
+  } else {
+    xi = __builtin_rintf(x * NT2H_F(0x3c360b61));
+    xr = (x - xi * 90.0f) * NT2H_F(0x3c8efa35);
+  }
+  const int n = (xi < 2147483648.0f) ? (int)xi : (int)0x80000000u;
+  const uint32_t sa = bits(a0) & 0x80000000u;
+  if (kind == k_tan || kind == k_cot) {
+    float t = as_f(bits(kind == k_tan ? tan_eval_f<P>(xr, n) : cot_eval_f<P>(xr, n)) ^ sa);
+    bool bad;
+    if (kind == k_tan) bad = is_flint(U == u_pi ? a0 - 0.5f : (a0 - 90.0f) * NT2H_F(0x3bb60b61));
+    else bad = a0 != 0.0f && is_flint(U == u_pi ? a0 : a0 * NT2H_F(0x3bb60b61));
+    t = bad ? __builtin_nanf("") : t;
+    return (kind == k_cot && a0 == 0.0f) ? 1.0f / a0 : t;
+  }
+  const uint32_t swap = (uint32_t)n & 1u;
+  const float z = xr * xr;
+  const float se = sin_eval_f<P>(z, xr), ce = cos_eval_f<P>(z);
+  if (kind == k_cos) return as_f(bits(swap ? se : ce) ^ ((swap ^ (((uint32_t)n & 2u) >> 1)) << 31));
+  return as_f(bits(swap ? ce : se) ^ ((((uint32_t)n & 2u) << 30) ^ sa));
+}
+template <class P, int U>
+NT2H_FN double trigu_d(int kind, double a0) {
+  NT2H_STRICT
+  const double x = __builtin_fabs(a0);
+  double xi, xr;
+  if (U == u_pi) {
+    xi = __builtin_rint(x * 2.0);
+    xr = (x - xi * 0.5) * NT2H_D(0x400921fb54442d18);
+  } else {
+    xi = __builtin_rint(x * NT2H_D(0x3f86c16c16c16c17));
+    xr = (x - xi * 90.0) * NT2H_D(0x3f91df46a2529d39);
+  }
+  const int64_t n = (xi < 9223372036854775808.0) ? (int64_t)xi : (int64_t)0x8000000000000000ull;
+  const uint64_t sa = bits(a0) & 0x8000000000000000ull;
+  if (kind == k_tan || kind == k_cot) {
+    double t = as_d(bits(kind == k_tan ? tan_eval_d<P>(xr, (int)(n & 1)) : cot_eval_d<P>(xr, (int)(n & 1))) ^ sa);
+    bool bad;
+    if (kind == k_tan) bad = is_flint(U == u_pi ? a0 - 0.5 : (a0 - 90.0) * NT2H_D(0x3f76c16c16c16c17));
+    else bad = a0 != 0.0 && is_flint(U == u_pi ? a0 : a0 * NT2H_D(0x3f76c16c16c16c17));
+    t = bad ? __builtin_nan("") : t;
+    return (kind == k_cot && a0 == 0.0) ? 1.0 / a0 : t;
+  }
+  const uint64_t swap = (uint64_t)n & 1u;
+  const double z = xr * xr;
+  const double se = sin_eval_d<P>(z, xr), ce = cos_eval_d<P>(z);
+  if (kind == k_cos) return as_d(bits(swap ? se : ce) ^ ((swap ^ (((uint64_t)n & 2u) >> 1)) << 63));
+  return as_d(bits(swap ? ce : se) ^ ((((uint64_t)n & 2u) << 62) ^ sa));
 }
 
 // ---------------------------------------------------------------------------

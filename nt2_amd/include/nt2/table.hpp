@@ -201,7 +201,9 @@ NT2H_TAG(round2even_, same1) NT2H_TAG(sign_, same1) NT2H_TAG(expm1_, same1) NT2H
 NT2H_TAG(tanh_, same1) NT2H_TAG(sinh_, same1) NT2H_TAG(cosh_, same1) NT2H_TAG(atan_, same1)
 NT2H_TAG(atan2_, same2) NT2H_TAG(tan_, same1) NT2H_TAG(asin_, same1) NT2H_TAG(acos_, same1)
 NT2H_TAG(erf_, same1) NT2H_TAG(cbrt_, same1) NT2H_TAG(asinh_, same1) NT2H_TAG(acosh_, same1)
-NT2H_TAG(atanh_, same1)
+NT2H_TAG(atanh_, same1) NT2H_TAG(cot_, same1) NT2H_TAG(sec_, same1) NT2H_TAG(csc_, same1)
+NT2H_TAG(sinpi_, same1) NT2H_TAG(cospi_, same1) NT2H_TAG(tanpi_, same1) NT2H_TAG(cotpi_, same1)
+NT2H_TAG(sind_, same1) NT2H_TAG(cosd_, same1) NT2H_TAG(tand_, same1) NT2H_TAG(cotd_, same1)
 NT2H_TAG(pow_, same2)
 NT2H_TAG(fma_, same3) NT2H_TAG(fnms_, same3) NT2H_TAG(if_else_, sel3)
 NT2H_TAG(selinc_, selinc2) NT2H_TAG(seldec_, selinc2) NT2H_TAG(if_zero_else_one_, izeo1)
@@ -311,6 +313,13 @@ NT2H_UN(asinh_, (G == 1 ? dm::asinh_f<P, true>(x) : dm::asinh_f<P, false>(x)),
 NT2H_UN(acosh_, (G == 1 ? dm::acosh_f<P, true>(x) : dm::acosh_f<P, false>(x)),
         (G == 1 ? dm::acosh_d<P, true>(x) : dm::acosh_d<P, false>(x)), no_integer_form<T>())
 NT2H_UN(atanh_, dm::atanh_f<P>(x), dm::atanh_d<P>(x), no_integer_form<T>())
+// trigonometric functions of pi x and of x degrees: lane-wise reductions, one form
+#define NT2H_UNIT(TAG, U, K) \
+  NT2H_UN(TAG, (dm::trigu_f<P, dm::U>(dm::K, x)), (dm::trigu_d<P, dm::U>(dm::K, x)), no_integer_form<T>())
+NT2H_UNIT(sinpi_, u_pi, k_sin) NT2H_UNIT(cospi_, u_pi, k_cos) NT2H_UNIT(tanpi_, u_pi, k_tan)
+NT2H_UNIT(cotpi_, u_pi, k_cot) NT2H_UNIT(sind_, u_deg, k_sin) NT2H_UNIT(cosd_, u_deg, k_cos)
+NT2H_UNIT(tand_, u_deg, k_tan) NT2H_UNIT(cotd_, u_deg, k_cot)
+#undef NT2H_UNIT
 #undef NT2H_UN
 
 template <> struct apply1<tagfn::cos_> {
@@ -331,6 +340,16 @@ template <> struct apply1<tagfn::sin_> {
     else dm::trig_group_d<P, G>(dm::k_sin, a, r, nullptr);
   }
 };
+// cot (radian, per-pack reduction), sec = 1/cos, csc = 1/sin
+#define NT2H_RADIAN(TAG, K)                                                                          \
+  template <> struct apply1<tagfn::TAG> {                                                            \
+    template <int G, class T> __device__ __forceinline__ static void run(T (&a)[G], T (&r)[G]) {    \
+      if constexpr (std::is_same<T, float>::value) dm::trig_group_f<P, G>(dm::K, a, r, nullptr);     \
+      else dm::trig_group_d<P, G>(dm::K, a, r, nullptr);                                             \
+    }                                                                                                \
+  };
+NT2H_RADIAN(cot_, k_cot) NT2H_RADIAN(sec_, k_sec) NT2H_RADIAN(csc_, k_csc)
+#undef NT2H_RADIAN
 
 template <class Tag> struct apply2;
 #define NT2H_BIN(TAG, EXPR)                                                           \
@@ -592,6 +611,9 @@ NT2H_FN1(round2even, round2even_) NT2H_FN1(sign, sign_)
 NT2H_FN1(expm1, expm1_) NT2H_FN1(log1p, log1p_) NT2H_FN1(tanh, tanh_) NT2H_FN1(sinh, sinh_)
 NT2H_FN1(cosh, cosh_) NT2H_FN1(atan, atan_) NT2H_FN1(tan, tan_) NT2H_FN1(asin, asin_) NT2H_FN1(acos, acos_)
 NT2H_FN1(erf, erf_) NT2H_FN1(cbrt, cbrt_) NT2H_FN1(asinh, asinh_) NT2H_FN1(acosh, acosh_) NT2H_FN1(atanh, atanh_)
+NT2H_FN1(cot, cot_) NT2H_FN1(sec, sec_) NT2H_FN1(csc, csc_) NT2H_FN1(sinpi, sinpi_) NT2H_FN1(cospi, cospi_)
+NT2H_FN1(tanpi, tanpi_) NT2H_FN1(cotpi, cotpi_) NT2H_FN1(sind, sind_) NT2H_FN1(cosd, cosd_) NT2H_FN1(tand, tand_)
+NT2H_FN1(cotd, cotd_)
 NT2H_FN1(if_zero_else_one, if_zero_else_one_)
 #undef NT2H_FN1
 #define NT2H_FN2(name, Tag) \
@@ -980,6 +1002,8 @@ template <class X> struct pack_free { static constexpr bool value = true; };
 template <class Tag, class A> struct pack_free<node1<Tag, A>> {
   static constexpr bool value = pack_free<A>::value && !std::is_same<Tag, tagfn::cos_>::value &&
                                 !std::is_same<Tag, tagfn::sin_>::value && !std::is_same<Tag, tagfn::tan_>::value &&
+                                !std::is_same<Tag, tagfn::cot_>::value && !std::is_same<Tag, tagfn::sec_>::value &&
+                                !std::is_same<Tag, tagfn::csc_>::value &&
                                 !std::is_same<Tag, tagfn::atan_>::value && !std::is_same<Tag, tagfn::asinh_>::value &&
                                 !std::is_same<Tag, tagfn::acosh_>::value &&
                                 !((std::is_same<Tag, tagfn::erfc_>::value || std::is_same<Tag, tagfn::normcdf_>::value ||

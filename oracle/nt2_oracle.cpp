@@ -1607,9 +1607,21 @@ void trig_group_f(int kind, const float* a0, float* out0, float* out1) {
       out0[i] = t;
       continue;
     }
+    if (kind == 4) {  // cot: cot_eval (1/y even, -y odd), trig_base.hpp:126-138 (scalar :178-188)
+      const float zz = xr[i] * xr[i];
+      float y = ((horner(fTanC, zz) * zz) * xr[i]) + xr[i];
+      y = (n[i] & 1) ? -y : 1.0f / y;
+      float t = as_f(bits(y) ^ (bits(a0[i]) & 0x80000000u));
+      if (G == 1 && !std::isfinite(a0[i])) t = NAN;
+      else if (a0[i] == 0.0f) t = 1.0f / a0[i];
+      out0[i] = t;
+      continue;
+    }
     if (G == 1 && !std::isfinite(a0[i])) { c = NAN; s = NAN; }
     if (kind == 0) out0[i] = c;
     else if (kind == 1) out0[i] = s;
+    else if (kind == 5) out0[i] = 1.0f / c;  // sec: generic/sec.hpp, rec(cos(a0))
+    else if (kind == 6) out0[i] = 1.0f / s;  // csc: generic/csc.hpp, rec(sin(a0))
     else { out0[i] = s; out1[i] = c; }
   }
 }
@@ -1639,11 +1651,101 @@ void trig_group_d(int kind, const double* a0, double* out0, double* out1) {
       out0[i] = t;
       continue;
     }
+    if (kind == 4) {  // cot
+      const double zz = xr[i] * xr[i];
+      double y = xr[i] + xr[i] * (zz * (horner(dTanN, zz) / horner(dTanD, zz)));
+      y = (n[i] & 1) ? -y : 1.0 / y;
+      double t = as_d(bits(y) ^ (bits(a0[i]) & 0x8000000000000000ull));
+      if (G == 1 && !std::isfinite(a0[i])) t = NAN;
+      else if (a0[i] == 0.0) t = 1.0 / a0[i];
+      out0[i] = t;
+      continue;
+    }
     if (G == 1 && !std::isfinite(a0[i])) { c = NAN; s = NAN; }
     if (kind == 0) out0[i] = c;
     else if (kind == 1) out0[i] = s;
+    else if (kind == 5) out0[i] = 1.0 / c;
+    else if (kind == 6) out0[i] = 1.0 / s;
     else { out0[i] = s; out1[i] = c; }
   }
+}
+
+// sinpi ... cotd: trig_base<A0, pi_tag | degree_tag, ..., big_>
+// (simd/common/impl/trigo/trig_base.hpp:53-138, scalar/impl/trigo/trig_base.hpp:
+// 126-188) with the lane-wise reductions of scalar/impl/trigo/trig_reduction.hpp:
+// 262-318: pi: xi = round2even(x*2), xr = (x - xi*Half)*Pi; degree:
+// xi = round2even(x*Oneo_90), xr = inrad(x - xi*_90) = (...)*Deginrad
+// (generic/inrad.hpp); n = fast_toint(xi) with x86's truncating conversion
+// (the integer indefinite value out of range).  tan_invalid / cot_invalid:
+// trig_reduction.hpp:268-269 (degree, times Oneo_180), :308-309 (pi).
+// kind: 0 cos, 1 sin, 3 tan, 4 cot.
+inline bool is_flint_v(float v) { return v - std::trunc(v) == 0.0f; }
+inline bool is_flint_v(double v) { return v - std::trunc(v) == 0.0; }
+template <class T> struct unit_c;
+template <> struct unit_c<float> {
+  using I = int32_t;
+  using U = uint32_t;
+  static constexpr U sign = 0x80000000u;
+  static float pi() { return F32(0x40490fdb); }
+  static float oneo90() { return F32(0x3c360b61); }
+  static float oneo180() { return F32(0x3bb60b61); }
+  static float deginrad() { return F32(0x3c8efa35); }
+  static I toint(float v) { return (v < 2147483648.0f) ? (I)v : (I)INT32_MIN; }
+  static float tan_base(float z) {
+    const float zz = z * z;
+    return ((horner(fTanC, zz) * zz) * z) + z;
+  }
+  static float sin_eval(float z, float x) { return sin_eval_f(z, x); }
+  static float cos_eval(float z) { return cos_eval_f(z); }
+};
+template <> struct unit_c<double> {
+  using I = int64_t;
+  using U = uint64_t;
+  static constexpr U sign = 0x8000000000000000ull;
+  static double pi() { return F64(0x400921fb54442d18); }
+  static double oneo90() { return F64(0x3f86c16c16c16c17); }
+  static double oneo180() { return F64(0x3f76c16c16c16c17); }
+  static double deginrad() { return F64(0x3f91df46a2529d39); }
+  static I toint(double v) { return (v < 9223372036854775808.0) ? (I)v : (I)INT64_MIN; }
+  static double tan_base(double z) {
+    const double zz = z * z;
+    return z + z * (zz * (horner(dTanN, zz) / horner(dTanD, zz)));
+  }
+  static double sin_eval(double z, double x) { return sin_eval_d(z, x); }
+  static double cos_eval(double z) { return cos_eval_d(z); }
+};
+template <class T, bool PI> T trig_unit(int kind, T a0) {
+  using C = unit_c<T>;
+  using U = typename C::U;
+  const T x = std::fabs(a0);
+  T xi, xr;
+  if (PI) {
+    xi = std::nearbyint(x * T(2));
+    xr = (x - xi * T(0.5)) * C::pi();
+  } else {
+    xi = std::nearbyint(x * C::oneo90());
+    xr = (x - xi * T(90)) * C::deginrad();
+  }
+  const typename C::I n = C::toint(xi);
+  const U sa = __builtin_bit_cast(U, a0) & C::sign;
+  auto flip = [](T v, U m) { return __builtin_bit_cast(T, (U)(__builtin_bit_cast(U, v) ^ m)); };
+  if (kind == 3 || kind == 4) {
+    const T y = C::tan_base(xr);
+    T r = (kind == 3) ? ((n & 1) ? -(T(1) / y) : y) : ((n & 1) ? -y : T(1) / y);
+    r = flip(r, sa);
+    bool bad;
+    if (kind == 3) bad = is_flint_v(PI ? a0 - T(0.5) : (a0 - T(90)) * C::oneo180());
+    else bad = a0 != T(0) && is_flint_v(PI ? a0 : a0 * C::oneo180());
+    if (bad) r = NAN;
+    if (kind == 4 && a0 == T(0)) r = T(1) / a0;
+    return r;
+  }
+  const U swap = (U)n & 1u;
+  const T z = xr * xr;
+  const T se = C::sin_eval(z, xr), ce = C::cos_eval(z);
+  const int sh = sizeof(T) * 8 - 1;
+  if (kind == 0) return flip(swap ? se : ce, (swap ^ (((U)n & 2u) >> 1)) << sh);
+  return flip(swap ? ce : se, (((U)n & 2u) << (sh - 1)) ^ sa);
 }
 
 // ===========================================================================
@@ -1820,8 +1922,10 @@ enum {
   OR_LOG2 = 11, OR_LOG10 = 12, OR_FLOOR = 13, OR_CEIL = 14, OR_TRUNC = 15, OR_ROUND = 16,
   OR_ROUND2EVEN = 17, OR_SIGN = 18, OR_EXPM1 = 19, OR_LOG1P = 20, OR_TANH = 21, OR_SINH = 22,
   OR_COSH = 23, OR_ATAN = 24, OR_TAN = 25, OR_ASIN = 26, OR_ACOS = 27, OR_ERF = 28,
-  OR_CBRT = 29, OR_ASINH = 30, OR_ACOSH = 31, OR_ATANH = 32,
-  OR_LAST = OR_ATANH
+  OR_CBRT = 29, OR_ASINH = 30, OR_ACOSH = 31, OR_ATANH = 32, OR_COT = 33, OR_SEC = 34, OR_CSC = 35,
+  OR_SINPI = 36, OR_COSPI = 37, OR_TANPI = 38, OR_COTPI = 39, OR_SIND = 40, OR_COSD = 41, OR_TAND = 42,
+  OR_COTD = 43,
+  OR_LAST = OR_COTD
 };
 
 int oracle_abi_version(void) { return 1; }
@@ -1875,9 +1979,22 @@ int oracle_map_f32(int op, const float* x, float* y, size_t n) {
       case OR_COS:
       case OR_SIN:
       case OR_TAN:
-        if (pack) trig_group_f<8>(op == OR_COS ? 0 : (op == OR_SIN ? 1 : 3), x + i, y + i, nullptr);
-        else trig_group_f<1>(op == OR_COS ? 0 : (op == OR_SIN ? 1 : 3), x + i, y + i, nullptr);
+      case OR_COT:
+      case OR_SEC:
+      case OR_CSC: {
+        const int kind = op == OR_COS ? 0 : op == OR_SIN ? 1 : op == OR_TAN ? 3 : op == OR_COT ? 4 : op == OR_SEC ? 5 : 6;
+        if (pack) trig_group_f<8>(kind, x + i, y + i, nullptr);
+        else trig_group_f<1>(kind, x + i, y + i, nullptr);
         break;
+      }
+      case OR_SINPI: for (size_t l = 0; l < w; ++l) y[i + l] = trig_unit<float, true>(1, x[i + l]); break;
+      case OR_COSPI: for (size_t l = 0; l < w; ++l) y[i + l] = trig_unit<float, true>(0, x[i + l]); break;
+      case OR_TANPI: for (size_t l = 0; l < w; ++l) y[i + l] = trig_unit<float, true>(3, x[i + l]); break;
+      case OR_COTPI: for (size_t l = 0; l < w; ++l) y[i + l] = trig_unit<float, true>(4, x[i + l]); break;
+      case OR_SIND: for (size_t l = 0; l < w; ++l) y[i + l] = trig_unit<float, false>(1, x[i + l]); break;
+      case OR_COSD: for (size_t l = 0; l < w; ++l) y[i + l] = trig_unit<float, false>(0, x[i + l]); break;
+      case OR_TAND: for (size_t l = 0; l < w; ++l) y[i + l] = trig_unit<float, false>(3, x[i + l]); break;
+      case OR_COTD: for (size_t l = 0; l < w; ++l) y[i + l] = trig_unit<float, false>(4, x[i + l]); break;
     }
   });
   return 0;
@@ -1920,9 +2037,22 @@ int oracle_map_f64(int op, const double* x, double* y, size_t n) {
       case OR_COS:
       case OR_SIN:
       case OR_TAN:
-        if (pack) trig_group_d<4>(op == OR_COS ? 0 : (op == OR_SIN ? 1 : 3), x + i, y + i, nullptr);
-        else trig_group_d<1>(op == OR_COS ? 0 : (op == OR_SIN ? 1 : 3), x + i, y + i, nullptr);
+      case OR_COT:
+      case OR_SEC:
+      case OR_CSC: {
+        const int kind = op == OR_COS ? 0 : op == OR_SIN ? 1 : op == OR_TAN ? 3 : op == OR_COT ? 4 : op == OR_SEC ? 5 : 6;
+        if (pack) trig_group_d<4>(kind, x + i, y + i, nullptr);
+        else trig_group_d<1>(kind, x + i, y + i, nullptr);
         break;
+      }
+      case OR_SINPI: for (size_t l = 0; l < w; ++l) y[i + l] = trig_unit<double, true>(1, x[i + l]); break;
+      case OR_COSPI: for (size_t l = 0; l < w; ++l) y[i + l] = trig_unit<double, true>(0, x[i + l]); break;
+      case OR_TANPI: for (size_t l = 0; l < w; ++l) y[i + l] = trig_unit<double, true>(3, x[i + l]); break;
+      case OR_COTPI: for (size_t l = 0; l < w; ++l) y[i + l] = trig_unit<double, true>(4, x[i + l]); break;
+      case OR_SIND: for (size_t l = 0; l < w; ++l) y[i + l] = trig_unit<double, false>(1, x[i + l]); break;
+      case OR_COSD: for (size_t l = 0; l < w; ++l) y[i + l] = trig_unit<double, false>(0, x[i + l]); break;
+      case OR_TAND: for (size_t l = 0; l < w; ++l) y[i + l] = trig_unit<double, false>(3, x[i + l]); break;
+      case OR_COTD: for (size_t l = 0; l < w; ++l) y[i + l] = trig_unit<double, false>(4, x[i + l]); break;
     }
   });
   return 0;

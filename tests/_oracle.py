@@ -13,7 +13,9 @@ OPS = {"exp": 0, "log": 1, "sqrt": 2, "erfc": 3, "normcdf": 4,
        "fastnormcdf": 5, "cos": 6, "sin": 7, "abs": 8, "exp2": 9, "exp10": 10, "log2": 11,
        "log10": 12, "floor": 13, "ceil": 14, "trunc": 15, "round": 16, "round2even": 17, "sign": 18,
        "expm1": 19, "log1p": 20, "tanh": 21, "sinh": 22, "cosh": 23, "atan": 24, "tan": 25,
-       "asin": 26, "acos": 27, "erf": 28, "cbrt": 29, "asinh": 30, "acosh": 31, "atanh": 32}
+       "asin": 26, "acos": 27, "erf": 28, "cbrt": 29, "asinh": 30, "acosh": 31, "atanh": 32,
+       "cot": 33, "sec": 34, "csc": 35, "sinpi": 36, "cospi": 37, "tanpi": 38, "cotpi": 39, "sind": 40,
+       "cosd": 41, "tand": 42, "cotd": 43}
 
 _lib = None
 

@@ -223,6 +223,12 @@ int main(int argc, char** argv) {
   dump("erf_f32", at2);
   at2 = nt2::acosh(nt2::abs(bt) * 1.0e6f + 1.0f) * nt2::atanh(at / 3.0f);
   dump("acosh_f32", at2);
+  at2 = nt2::cot(at) + nt2::sec(bt) * nt2::csc(at * 7.0f);
+  dump("cot_f32", at2);
+  at2 = nt2::sinpi(at) - nt2::cospi(bt) + nt2::tanpi(at / 7.0f) * nt2::cotpi(bt / 3.0f);
+  dump("sinpi_f32", at2);
+  at2 = nt2::sind(at * 100.0f) - nt2::cosd(bt * 50.0f) + nt2::tand(at * 30.0f) * nt2::cotd(bt * 40.0f);
+  dump("sind_f32", at2);
   at2 = if_else((at > 0.0f) && (bt < 1.0f), at, bt) + if_else(logical_or(at < -1.0f, bt > 2.0f), 1.0f, 0.0f);
   dump("logical_f32", at2);
   table<double> dm(of_size(mh * mw)), dr;
@@ -246,6 +252,12 @@ int main(int argc, char** argv) {
   dump("erf_f64", dq2);
   dq2 = nt2::acosh(nt2::abs(dq) * 1e17 + 1.0) * nt2::atanh(dq * 0.999);
   dump("acosh_f64", dq2);
+  dq2 = nt2::cot(dq * 30.0) + nt2::sec(dq * 9.0) * nt2::csc(dq * 7.0);
+  dump("cot_f64", dq2);
+  dq2 = nt2::sinpi(dq * 5.0) - nt2::cospi(dq * 3.0) + nt2::tanpi(dq / 0.7) * nt2::cotpi(dq * 1.3);
+  dump("sinpi_f64", dq2);
+  dq2 = nt2::sind(dq * 500.0) - nt2::cosd(dq * 300.0) + nt2::tand(dq * 200.0) * nt2::cotd(dq * 170.0);
+  dump("sind_f64", dq2);
 
   // ---- rgb2grey and rgb2yuv on a 96 x 131 image: the demos' ramp and random planes
   const std::size_t ih = 96, iw = 131;

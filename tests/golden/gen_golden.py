@@ -3,7 +3,8 @@
 
 * math_f32.npz / math_f64.npz — inputs and correctly rounded outputs of exp,
   log, erfc, normcdf, cos, sin, exp2, exp10, log2, log10, expm1, log1p, tanh,
-  sinh, cosh, tan, asin, acos, erf, cbrt, asinh, acosh and atanh, computed with
+  sinh, cosh, tan, asin, acos, erf, cbrt, asinh, acosh, atanh, cot, sec, csc,
+  sinpi, cospi, tanpi, cotpi, sind, cosd, tand and cotd, computed with
   mpmath at 60 significant digits.  Inputs: edge values, the NT2 reduction-range boundaries, and seeded
   random points spread over each function's domain (including huge
   arguments for the Payne-Hanek path of cos/sin).
@@ -84,7 +85,26 @@ FUNCS = {
     "acosh": lambda x: mpmath.acosh(x) if x >= 1 else mpmath.nan,
     "atanh": lambda x: (mpmath.atanh(x) if abs(x) < 1 else
                         (mpmath.inf * mpmath.sign(x) if abs(x) == 1 else mpmath.nan)),
+    "cot": lambda x: _ratio(mpmath.cos(x), mpmath.sin(x), x),
+    "sec": lambda x: 1 / mpmath.cos(x),
+    "csc": lambda x: _ratio(mpmath.mpf(1), mpmath.sin(x), x),
+    "sinpi": lambda x: mpmath.sinpi(x),
+    "cospi": lambda x: mpmath.cospi(x),
+    "tanpi": lambda x: _ratio(mpmath.sinpi(x), mpmath.cospi(x), x, zero_is_nan=True),
+    "cotpi": lambda x: _ratio(mpmath.cospi(x), mpmath.sinpi(x), x),
+    "sind": lambda x: mpmath.sinpi(x / 180),
+    "cosd": lambda x: mpmath.cospi(x / 180),
+    "tand": lambda x: _ratio(mpmath.sinpi(x / 180), mpmath.cospi(x / 180), x, zero_is_nan=True),
+    "cotd": lambda x: _ratio(mpmath.cospi(x / 180), mpmath.sinpi(x / 180), x),
 }
+
+
+def _ratio(num, den, x, zero_is_nan=False):
+    """num / den; at an exact zero of den: +-Inf at x = +-0, NaN elsewhere
+    (a pole: NT2 returns NaN there, and the sign of the limit is undefined)."""
+    if den == 0:
+        return mpmath.nan if (zero_is_nan or x != 0) else mpmath.inf * (1 if mpmath.sign(x) >= 0 else -1)
+    return num / den
 
 
 def inputs(name, dtype, rng):
@@ -167,6 +187,21 @@ def inputs(name, dtype, rng):
         pts += list((1 - np.exp2(rng.uniform(-50, -1, 200))) * rng.choice([-1.0, 1.0], 200))
         pts += list(np.exp2(rng.uniform(-60, -2, 200)) * rng.choice([-1.0, 1.0], 200))
         pts += [0.0, 0.5, -0.5, 0.4999, 0.9, -0.9, 0.25, 1e-9]
+    elif name in ("cot", "sec", "csc"):
+        pts += list(rng.uniform(-np.pi / 4, np.pi / 4, 300)) + list(rng.uniform(-np.pi / 2, np.pi / 2, 300))
+        pts += list(rng.uniform(-20 * np.pi, 20 * np.pi, 300)) + list(rng.uniform(-1e4, 1e4, 300))
+        pts += list(np.exp2(rng.uniform(20, 100, 200)) * rng.choice([-1.0, 1.0], 200))
+        pts += [1.0, -1.0, 0.5, np.pi / 4, 1.5, -1.5, 100.0, 1e5, 3.0]
+    elif name in ("sinpi", "cospi", "tanpi", "cotpi"):
+        pts += list(rng.uniform(-4, 4, 800)) + list(rng.uniform(-0.25, 0.25, 300))
+        pts += list(np.exp2(rng.uniform(-30, 30, 300)) * rng.choice([-1.0, 1.0], 300))
+        pts += list(rng.integers(-50, 50, 60) * 0.25)
+        pts += [0.25, -0.25, 0.5, 1.0, 1.5, 2.0, 0.125, 1e-20, 12345.25, 1e7 + 0.5]
+    elif name in ("sind", "cosd", "tand", "cotd"):
+        pts += list(rng.uniform(-720, 720, 800)) + list(rng.uniform(-45, 45, 300))
+        pts += list(np.exp2(rng.uniform(-20, 40, 300)) * rng.choice([-1.0, 1.0], 300))
+        pts += list(rng.integers(-64, 64, 60) * 22.5)
+        pts += [45.0, -45.0, 90.0, 180.0, 270.0, 30.0, 60.0, 1e-20, 123456.0, 1e7 + 45]
     else:  # cos, sin
         pts += list(rng.uniform(-np.pi / 4, np.pi / 4, 150))
         pts += list(rng.uniform(-np.pi / 2, np.pi / 2, 150))
@@ -419,6 +454,117 @@ KATS = [
     ("atanh", "-1", "-inf", 0.5, "modules/core/hyperbolic/unit/simd/atanh.cpp:46"),
     ("atanh", "1", "inf", 0.5, "modules/core/hyperbolic/unit/simd/atanh.cpp:47"),
     ("atanh", "0", "0", 0.5, "modules/core/hyperbolic/unit/simd/atanh.cpp:49"),
+    # modules/core/trigonometric/unit/simd/cot.cpp:45-54
+    ("cot", "-0", "-inf", 0.5, "modules/core/trigonometric/unit/simd/cot.cpp:45"),
+    ("cot", "inf", "nan", 0.5, "modules/core/trigonometric/unit/simd/cot.cpp:46"),
+    ("cot", "-inf", "nan", 0.5, "modules/core/trigonometric/unit/simd/cot.cpp:47"),
+    ("cot", "nan", "nan", 0.5, "modules/core/trigonometric/unit/simd/cot.cpp:48"),
+    ("cot", "0", "inf", 0.5, "modules/core/trigonometric/unit/simd/cot.cpp:49"),
+    ("cot", "-pi/2", "0", 0.5, "modules/core/trigonometric/unit/simd/cot.cpp:51"),
+    ("cot", "-pi/4", "-1", 0.5, "modules/core/trigonometric/unit/simd/cot.cpp:52"),
+    ("cot", "pi/2", "0", 0.5, "modules/core/trigonometric/unit/simd/cot.cpp:53"),
+    ("cot", "pi/4", "1", 0.5, "modules/core/trigonometric/unit/simd/cot.cpp:54"),
+    # modules/core/trigonometric/unit/simd/sec.cpp:46-54 and csc.cpp:46-55
+    ("sec", "inf", "nan", 0.5, "modules/core/trigonometric/unit/simd/sec.cpp:46"),
+    ("sec", "-inf", "nan", 0.5, "modules/core/trigonometric/unit/simd/sec.cpp:47"),
+    ("sec", "nan", "nan", 0.5, "modules/core/trigonometric/unit/simd/sec.cpp:48"),
+    ("sec", "-pi", "-1", 0.5, "modules/core/trigonometric/unit/simd/sec.cpp:50"),
+    ("sec", "-pi/4", "sqrt(2)", 0.5, "modules/core/trigonometric/unit/simd/sec.cpp:51"),
+    ("sec", "pi", "-1", 0.5, "modules/core/trigonometric/unit/simd/sec.cpp:52"),
+    ("sec", "pi/4", "sqrt(2)", 0.5, "modules/core/trigonometric/unit/simd/sec.cpp:53"),
+    ("sec", "0", "1", 0.5, "modules/core/trigonometric/unit/simd/sec.cpp:54"),
+    ("csc", "-0", "-inf", 0.5, "modules/core/trigonometric/unit/simd/csc.cpp:46"),
+    ("csc", "inf", "nan", 0.5, "modules/core/trigonometric/unit/simd/csc.cpp:47"),
+    ("csc", "-inf", "nan", 0.5, "modules/core/trigonometric/unit/simd/csc.cpp:48"),
+    ("csc", "nan", "nan", 0.5, "modules/core/trigonometric/unit/simd/csc.cpp:49"),
+    ("csc", "0", "inf", 0.5, "modules/core/trigonometric/unit/simd/csc.cpp:50"),
+    ("csc", "-pi/2", "-1", 0.5, "modules/core/trigonometric/unit/simd/csc.cpp:52"),
+    ("csc", "-pi/4", "-sqrt(2)", 0.5, "modules/core/trigonometric/unit/simd/csc.cpp:53"),
+    ("csc", "pi/2", "1", 0.5, "modules/core/trigonometric/unit/simd/csc.cpp:54"),
+    ("csc", "pi/4", "sqrt(2)", 0.5, "modules/core/trigonometric/unit/simd/csc.cpp:55"),
+    # modules/core/trigonometric/unit/simd/sinpi.cpp:48-57, cospi.cpp:49-58
+    ("sinpi", "inf", "nan", 0.5, "modules/core/trigonometric/unit/simd/sinpi.cpp:48"),
+    ("sinpi", "-inf", "nan", 0.5, "modules/core/trigonometric/unit/simd/sinpi.cpp:49"),
+    ("sinpi", "nan", "nan", 0.5, "modules/core/trigonometric/unit/simd/sinpi.cpp:50"),
+    ("sinpi", "-0.25", "-sqrt(2)/2", 0.5, "modules/core/trigonometric/unit/simd/sinpi.cpp:52"),
+    ("sinpi", "0.5", "1", 0.5, "modules/core/trigonometric/unit/simd/sinpi.cpp:53"),
+    ("sinpi", "-0.5", "-1", 0.5, "modules/core/trigonometric/unit/simd/sinpi.cpp:54"),
+    ("sinpi", "1", "0", 0.5, "modules/core/trigonometric/unit/simd/sinpi.cpp:55"),
+    ("sinpi", "0.25", "sqrt(2)/2", 0.5, "modules/core/trigonometric/unit/simd/sinpi.cpp:56"),
+    ("sinpi", "0", "0", 0.5, "modules/core/trigonometric/unit/simd/sinpi.cpp:57"),
+    ("cospi", "inf", "nan", 0.5, "modules/core/trigonometric/unit/simd/cospi.cpp:49"),
+    ("cospi", "-inf", "nan", 0.5, "modules/core/trigonometric/unit/simd/cospi.cpp:50"),
+    ("cospi", "nan", "nan", 0.5, "modules/core/trigonometric/unit/simd/cospi.cpp:51"),
+    ("cospi", "-0.25", "sqrt(2)/2", 0.5, "modules/core/trigonometric/unit/simd/cospi.cpp:53"),
+    ("cospi", "0.5", "0", 0.5, "modules/core/trigonometric/unit/simd/cospi.cpp:54"),
+    ("cospi", "-0.5", "0", 0.5, "modules/core/trigonometric/unit/simd/cospi.cpp:55"),
+    ("cospi", "1", "-1", 0.5, "modules/core/trigonometric/unit/simd/cospi.cpp:56"),
+    ("cospi", "0.25", "sqrt(2)/2", 0.5, "modules/core/trigonometric/unit/simd/cospi.cpp:57"),
+    ("cospi", "0", "1", 0.5, "modules/core/trigonometric/unit/simd/cospi.cpp:58"),
+    # modules/core/trigonometric/unit/simd/tanpi.cpp:47-57, cotpi.cpp:47-57
+    ("tanpi", "0.5", "nan", 0.5, "modules/core/trigonometric/unit/simd/tanpi.cpp:47"),
+    ("tanpi", "inf", "nan", 0.5, "modules/core/trigonometric/unit/simd/tanpi.cpp:48"),
+    ("tanpi", "-0.5", "nan", 0.5, "modules/core/trigonometric/unit/simd/tanpi.cpp:49"),
+    ("tanpi", "-inf", "nan", 0.5, "modules/core/trigonometric/unit/simd/tanpi.cpp:50"),
+    ("tanpi", "nan", "nan", 0.5, "modules/core/trigonometric/unit/simd/tanpi.cpp:51"),
+    ("tanpi", "-0.25", "-1", 1.0, "modules/core/trigonometric/unit/simd/tanpi.cpp:53"),
+    ("tanpi", "-1", "0", 0.5, "modules/core/trigonometric/unit/simd/tanpi.cpp:54"),
+    ("tanpi", "1", "0", 0.5, "modules/core/trigonometric/unit/simd/tanpi.cpp:55"),
+    ("tanpi", "0.25", "1", 1.0, "modules/core/trigonometric/unit/simd/tanpi.cpp:56"),
+    ("tanpi", "0", "0", 0.5, "modules/core/trigonometric/unit/simd/tanpi.cpp:57"),
+    ("cotpi", "-0", "-inf", 0.5, "modules/core/trigonometric/unit/simd/cotpi.cpp:47"),
+    ("cotpi", "inf", "nan", 0.5, "modules/core/trigonometric/unit/simd/cotpi.cpp:48"),
+    ("cotpi", "-inf", "nan", 0.5, "modules/core/trigonometric/unit/simd/cotpi.cpp:49"),
+    ("cotpi", "nan", "nan", 0.5, "modules/core/trigonometric/unit/simd/cotpi.cpp:50"),
+    ("cotpi", "1", "nan", 0.5, "modules/core/trigonometric/unit/simd/cotpi.cpp:51"),
+    ("cotpi", "0", "inf", 0.5, "modules/core/trigonometric/unit/simd/cotpi.cpp:52"),
+    ("cotpi", "-0.25", "-1", 0.5, "modules/core/trigonometric/unit/simd/cotpi.cpp:54"),
+    ("cotpi", "0.5", "0", 0.5, "modules/core/trigonometric/unit/simd/cotpi.cpp:55"),
+    ("cotpi", "-0.5", "0", 0.5, "modules/core/trigonometric/unit/simd/cotpi.cpp:56"),
+    ("cotpi", "0.25", "1", 0.5, "modules/core/trigonometric/unit/simd/cotpi.cpp:57"),
+    # modules/core/trigonometric/unit/simd/sind.cpp:48-58, cosd.cpp:46-56
+    ("sind", "inf", "nan", 0.5, "modules/core/trigonometric/unit/simd/sind.cpp:48"),
+    ("sind", "-inf", "nan", 0.5, "modules/core/trigonometric/unit/simd/sind.cpp:49"),
+    ("sind", "nan", "nan", 0.5, "modules/core/trigonometric/unit/simd/sind.cpp:50"),
+    ("sind", "-180", "0", 0.5, "modules/core/trigonometric/unit/simd/sind.cpp:52"),
+    ("sind", "-45", "-sqrt(2)/2", 0.5, "modules/core/trigonometric/unit/simd/sind.cpp:53"),
+    ("sind", "-90", "-1", 0.5, "modules/core/trigonometric/unit/simd/sind.cpp:54"),
+    ("sind", "0", "0", 0.5, "modules/core/trigonometric/unit/simd/sind.cpp:55"),
+    ("sind", "180", "0", 0.5, "modules/core/trigonometric/unit/simd/sind.cpp:56"),
+    ("sind", "45", "sqrt(2)/2", 0.5, "modules/core/trigonometric/unit/simd/sind.cpp:57"),
+    ("sind", "90", "1", 0.5, "modules/core/trigonometric/unit/simd/sind.cpp:58"),
+    ("cosd", "inf", "nan", 0.5, "modules/core/trigonometric/unit/simd/cosd.cpp:46"),
+    ("cosd", "-inf", "nan", 0.5, "modules/core/trigonometric/unit/simd/cosd.cpp:47"),
+    ("cosd", "nan", "nan", 0.5, "modules/core/trigonometric/unit/simd/cosd.cpp:48"),
+    ("cosd", "-180", "-1", 0.5, "modules/core/trigonometric/unit/simd/cosd.cpp:50"),
+    ("cosd", "-45", "sqrt(2)/2", 0.5, "modules/core/trigonometric/unit/simd/cosd.cpp:51"),
+    ("cosd", "-90", "0", 0.5, "modules/core/trigonometric/unit/simd/cosd.cpp:52"),
+    ("cosd", "180", "-1", 0.5, "modules/core/trigonometric/unit/simd/cosd.cpp:53"),
+    ("cosd", "45", "sqrt(2)/2", 0.5, "modules/core/trigonometric/unit/simd/cosd.cpp:54"),
+    ("cosd", "90", "0", 0.5, "modules/core/trigonometric/unit/simd/cosd.cpp:55"),
+    ("cosd", "0", "1", 0.5, "modules/core/trigonometric/unit/simd/cosd.cpp:56"),
+    # modules/core/trigonometric/unit/simd/tand.cpp:47-57, cotd.cpp:47-58
+    ("tand", "-90", "nan", 0.5, "modules/core/trigonometric/unit/simd/tand.cpp:47"),
+    ("tand", "inf", "nan", 0.5, "modules/core/trigonometric/unit/simd/tand.cpp:48"),
+    ("tand", "-inf", "nan", 0.5, "modules/core/trigonometric/unit/simd/tand.cpp:49"),
+    ("tand", "nan", "nan", 0.5, "modules/core/trigonometric/unit/simd/tand.cpp:50"),
+    ("tand", "90", "nan", 0.5, "modules/core/trigonometric/unit/simd/tand.cpp:51"),
+    ("tand", "-180", "0", 0.5, "modules/core/trigonometric/unit/simd/tand.cpp:53"),
+    ("tand", "-45", "-1", 0.5, "modules/core/trigonometric/unit/simd/tand.cpp:54"),
+    ("tand", "0", "0", 0.5, "modules/core/trigonometric/unit/simd/tand.cpp:55"),
+    ("tand", "180", "0", 0.5, "modules/core/trigonometric/unit/simd/tand.cpp:56"),
+    ("tand", "45", "1", 0.5, "modules/core/trigonometric/unit/simd/tand.cpp:57"),
+    ("cotd", "-0", "-inf", 0.5, "modules/core/trigonometric/unit/simd/cotd.cpp:47"),
+    ("cotd", "-180", "nan", 0.5, "modules/core/trigonometric/unit/simd/cotd.cpp:48"),
+    ("cotd", "inf", "nan", 0.5, "modules/core/trigonometric/unit/simd/cotd.cpp:49"),
+    ("cotd", "-inf", "nan", 0.5, "modules/core/trigonometric/unit/simd/cotd.cpp:50"),
+    ("cotd", "nan", "nan", 0.5, "modules/core/trigonometric/unit/simd/cotd.cpp:51"),
+    ("cotd", "0", "inf", 0.5, "modules/core/trigonometric/unit/simd/cotd.cpp:52"),
+    ("cotd", "180", "nan", 0.5, "modules/core/trigonometric/unit/simd/cotd.cpp:53"),
+    ("cotd", "-45", "-1", 4.0, "modules/core/trigonometric/unit/simd/cotd.cpp:55"),
+    ("cotd", "-90", "0", 0.5, "modules/core/trigonometric/unit/simd/cotd.cpp:56"),
+    ("cotd", "45", "1", 4.0, "modules/core/trigonometric/unit/simd/cotd.cpp:57"),
+    ("cotd", "90", "0", 0.5, "modules/core/trigonometric/unit/simd/cotd.cpp:58"),
 ]
 
 

@@ -159,6 +159,17 @@ def test_exponential_rounding_statements(outputs):
     want = O.map_f32("acosh", np.abs(b) * f32(1.0e6) + f32(1)) * O.map_f32("atanh", a / f32(3))
     got = outputs("acosh_f32", f32)
     assert same_bits(got, want), first_diff(got, want)
+    want = O.map_f32("cot", a) + O.map_f32("sec", b) * O.map_f32("csc", a * f32(7))
+    got = outputs("cot_f32", f32)
+    assert same_bits(got, want), first_diff(got, want)
+    want = (O.map_f32("sinpi", a) - O.map_f32("cospi", b)) + \
+        O.map_f32("tanpi", a / f32(7)) * O.map_f32("cotpi", b / f32(3))
+    got = outputs("sinpi_f32", f32)
+    assert same_bits(got, want), first_diff(got, want)
+    want = (O.map_f32("sind", a * f32(100)) - O.map_f32("cosd", b * f32(50))) + \
+        O.map_f32("tand", a * f32(30)) * O.map_f32("cotd", b * f32(40))
+    got = outputs("sind_f32", f32)
+    assert same_bits(got, want), first_diff(got, want)
     want = np.where((a > 0) & (b < 1), a, b) + np.where((a < -1) | (b > 2), f32(1), f32(0))
     got = outputs("logical_f32", f32)
     assert same_bits(got, want.astype(f32)), first_diff(got, want)
@@ -189,6 +200,17 @@ def test_exponential_rounding_statements(outputs):
     assert same_bits(got, want), first_diff(got, want)
     want = O.map_f64("acosh", np.abs(q) * 1e17 + 1.0) * O.map_f64("atanh", q * 0.999)
     got = outputs("acosh_f64", np.float64)
+    assert same_bits(got, want), first_diff(got, want)
+    want = O.map_f64("cot", q * 30.0) + O.map_f64("sec", q * 9.0) * O.map_f64("csc", q * 7.0)
+    got = outputs("cot_f64", np.float64)
+    assert same_bits(got, want), first_diff(got, want)
+    want = (O.map_f64("sinpi", q * 5.0) - O.map_f64("cospi", q * 3.0)) + \
+        O.map_f64("tanpi", q / 0.7) * O.map_f64("cotpi", q * 1.3)
+    got = outputs("sinpi_f64", np.float64)
+    assert same_bits(got, want), first_diff(got, want)
+    want = (O.map_f64("sind", q * 500.0) - O.map_f64("cosd", q * 300.0)) + \
+        O.map_f64("tand", q * 200.0) * O.map_f64("cotd", q * 170.0)
+    got = outputs("sind_f64", np.float64)
     assert same_bits(got, want), first_diff(got, want)
 
 

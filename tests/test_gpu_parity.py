@@ -19,12 +19,15 @@ from nt2_amd import ACCURATE, EXACT, FAST, ops  # noqa: E402
 
 G = os.path.join(os.path.dirname(__file__), "golden")
 ROUNDING = ["floor", "ceil", "trunc", "round", "round2even", "sign"]
+TRIG_MORE = ["cot", "sec", "csc", "sinpi", "cospi", "tanpi", "cotpi", "sind", "cosd", "tand", "cotd"]
+UNIT_DOMAIN32 = {"sinpi": 2.0 ** 22, "cospi": 2.0 ** 22, "tanpi": 2.0 ** 22, "cotpi": 2.0 ** 22,
+                 "sind": 2.0 ** 24, "cosd": 2.0 ** 24, "tand": 2.0 ** 24, "cotd": 2.0 ** 24}
 F32_OPS = ["exp", "log", "sqrt", "erfc", "normcdf", "fastnormcdf", "cos", "sin", "abs",
            "exp2", "exp10", "log2", "log10"] + ROUNDING + ["expm1", "log1p", "tanh", "sinh", "cosh", "atan", "tan", "asin", "acos"] + \
-    ["erf", "cbrt", "asinh", "acosh", "atanh"]
+    ["erf", "cbrt", "asinh", "acosh", "atanh"] + TRIG_MORE
 F64_OPS = ["exp", "log", "sqrt", "erfc", "normcdf", "cos", "sin", "abs",
            "exp2", "exp10", "log2", "log10"] + ROUNDING + ["expm1", "log1p", "tanh", "sinh", "cosh", "atan", "tan", "asin", "acos"] + \
-    ["erf", "cbrt", "asinh", "acosh", "atanh"]
+    ["erf", "cbrt", "asinh", "acosh", "atanh"] + TRIG_MORE
 
 
 def _mixed(rng, n, lo, hi, dt):
@@ -79,7 +82,13 @@ def _inputs_f32(op, rng, n):
     if op == "acosh":
         return np.concatenate([rng.uniform(0.5, 4.0, n // 2),
                                1.0 + np.exp2(rng.uniform(-30, 127, n - n // 2))]).astype(np.float32)
-    if op in ("cos", "sin", "tan"):
+    if op in ("sinpi", "cospi", "tanpi", "cotpi", "sind", "cosd", "tand", "cotd"):
+        s = 1.0 if op.endswith("pi") else 90.0
+        parts = [rng.uniform(-4, 4, n // 4) * s, rng.integers(-400, 400, n // 4) * (s / 4),
+                 rng.uniform(-1e4, 1e4, n // 4) * s,
+                 np.exp2(rng.uniform(-20, 40, n - 3 * (n // 4))) * rng.choice([-1, 1], n - 3 * (n // 4))]
+        return np.concatenate(parts).astype(np.float32)
+    if op in ("cos", "sin", "tan", "cot", "sec", "csc"):
         parts = [rng.uniform(-1, 1, n // 4), rng.uniform(-60, 60, n // 4),
                  rng.uniform(-1e4, 1e4, n // 4),
                  np.exp2(rng.uniform(0, 127, n - 3 * (n // 4))) * rng.choice([-1, 1], n - 3 * (n // 4))]
@@ -123,7 +132,13 @@ def _inputs_f64(op, rng, n):
                                np.exp2(rng.uniform(-30, 1020, n - n // 2)) * rng.choice([-1, 1], n - n // 2)])
     if op == "acosh":
         return np.concatenate([rng.uniform(0.5, 4.0, n // 2), 1.0 + np.exp2(rng.uniform(-30, 1020, n - n // 2))])
-    if op in ("cos", "sin", "tan"):
+    if op in ("sinpi", "cospi", "tanpi", "cotpi", "sind", "cosd", "tand", "cotd"):
+        s = 1.0 if op.endswith("pi") else 90.0
+        parts = [rng.uniform(-4, 4, n // 4) * s, rng.integers(-400, 400, n // 4) * (s / 4),
+                 rng.uniform(-1e6, 1e6, n // 4) * s,
+                 np.exp2(rng.uniform(-20, 80, n - 3 * (n // 4))) * rng.choice([-1, 1], n - 3 * (n // 4))]
+        return np.concatenate(parts)
+    if op in ("cos", "sin", "tan", "cot", "sec", "csc"):
         parts = [rng.uniform(-1, 1, n // 4), rng.uniform(-60, 60, n // 4),
                  rng.uniform(-1e6, 1e6, n // 4),
                  np.exp2(rng.uniform(0, 1000, n - 3 * (n // 4))) * rng.choice([-1, 1], n - 3 * (n // 4))]
@@ -176,7 +191,17 @@ def _truth_f32(op, x):
          "round": lambda v: np.sign(v) * np.floor(np.abs(v) + 0.5),
          "expm1": np.expm1, "log1p": lambda v: np.log1p(np.where(v >= -1, v, np.nan)), "tanh": np.tanh, "sinh": np.sinh, "cosh": np.cosh, "atan": np.arctan,
          "tan": np.tan, "asin": np.arcsin, "acos": np.arccos, "erf": sp.erf, "cbrt": np.cbrt,
-         "asinh": np.arcsinh, "acosh": np.arccosh, "atanh": np.arctanh}[op]
+         "asinh": np.arcsinh, "acosh": np.arccosh, "atanh": np.arctanh,
+         "cot": lambda v: 1.0 / np.tan(v), "sec": lambda v: 1.0 / np.cos(v), "csc": lambda v: 1.0 / np.sin(v),
+         # pi and degree arguments reduced exactly in float64 before the radian call
+         "sinpi": lambda v: np.sin(np.pi * np.remainder(v, 2.0)),
+         "cospi": lambda v: np.cos(np.pi * np.remainder(v, 2.0)),
+         "tanpi": lambda v: np.tan(np.pi * np.remainder(v, 1.0)),
+         "cotpi": lambda v: 1.0 / np.tan(np.pi * np.remainder(v, 1.0)),
+         "sind": lambda v: np.sin(np.deg2rad(np.remainder(v, 360.0))),
+         "cosd": lambda v: np.cos(np.deg2rad(np.remainder(v, 360.0))),
+         "tand": lambda v: np.tan(np.deg2rad(np.remainder(v, 180.0))),
+         "cotd": lambda v: 1.0 / np.tan(np.deg2rad(np.remainder(v, 180.0)))}[op]
     with np.errstate(all="ignore"):
         return f(xd)
 
@@ -211,12 +236,14 @@ def test_map_f32_fast_within_bound(op):
     same bounds the oracle is pinned to), and a few ulp of the exact policy."""
     rng = np.random.default_rng(300 + F32_OPS.index(op))
     x = _inputs_f32(op, rng, 40000)
-    if op in ("cos", "sin", "tan"):
+    if op in ("cos", "sin", "tan", "cot", "sec", "csc"):
         # the band [1e4, 2^18 pi) of NT2's f64 reduction for f32 inputs, uniform and log-uniform
         band = np.concatenate([rng.uniform(1e4, 823549, 10000),
                                np.exp2(rng.uniform(np.log2(1e4), np.log2(823549), 10000))])
         x = np.concatenate([x, (band * rng.choice([-1, 1], band.size)).astype(np.float32)])
         x = x[np.abs(x) < 823549]  # beyond: fdlibm path, identical in both policies
+    if op in UNIT_DOMAIN32:  # NT2's working range of the f32 pi / degree reductions
+        x = x[np.abs(x) < UNIT_DOMAIN32[op]]
     say("map_f32 fast", op, "n=%d" % x.size)
     got = ops.map(op, torch.from_numpy(x).cuda(), FAST).cpu().numpy()
     truth = _truth_f32(op, x)
@@ -231,15 +258,17 @@ def test_map_f32_fast_within_bound(op):
     ok = (err <= b) | ~normal | (e_ex <= 4)
     if op == "fastnormcdf":
         ok = np.abs(got.astype(np.float64) - truth) <= 3 * np.finfo(np.float32).eps
-    if op in ("cos", "sin"):
+    ok |= np.isnan(got) & np.isnan(exact)  # poles and invalid arguments, as NT2 marks them
+    if op in ("cos", "sin", "sinpi", "cospi", "sind", "cosd"):
         # packs reduced by NT2's pi/2 step evaluate the polynomials near |xr| = pi/2,
         # where terms of magnitude 1 cancel: compare those absolutely
         ok |= np.abs(got.astype(np.float64) - truth) <= 2 * np.finfo(np.float32).eps
         ok |= np.abs(got.astype(np.float64) - exact) <= 4 * np.finfo(np.float32).eps
-    if op == "tan":
-        # the same reduction differences, carried through d tan / dx = 1 + tan^2
+    if op in ("tan", "cot", "sec", "csc", "tanpi", "cotpi", "tand", "cotd"):
+        # the same reduction differences, carried through derivatives of size 1 + f^2
         e64 = exact.astype(np.float64)
-        ok |= np.abs(got.astype(np.float64) - e64) <= 4 * np.finfo(np.float32).eps * (1 + e64 * e64)
+        with np.errstate(over="ignore", invalid="ignore"):
+            ok |= np.abs(got.astype(np.float64) - e64) <= 4 * np.finfo(np.float32).eps * (1 + e64 * e64)
     assert ok.all(), (x[~ok][:4], got[~ok][:4], truth[~ok][:4], err[~ok][:4])
 
 

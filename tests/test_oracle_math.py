@@ -20,6 +20,7 @@ def _val(s, dt):
              "pi": np.pi, "-pi": -np.pi, "pi/2": np.pi / 2, "-pi/2": -np.pi / 2,
              "pi/4": np.pi / 4, "-pi/4": -np.pi / 4, "pi/6": np.pi / 6, "-pi/6": -np.pi / 6,
              "pi/3": np.pi / 3, "2pi/3": 2 * np.pi / 3, "sqrt(2)/2": np.sqrt(2) / 2, "-0": -0.0,
+             "-sqrt(2)/2": -np.sqrt(2) / 2, "sqrt(2)": np.sqrt(2), "-sqrt(2)": -np.sqrt(2),
              "ln2": np.log(2.0), "eps": np.finfo(dt).eps, "tiny": np.finfo(dt).tiny}
     if s == "e-1":  # NT2 forms these in the target precision: Exp_1<T>() - One<T>()
         return dt(np.e) - dt(1)
@@ -56,6 +57,9 @@ def _bound(name, x, y, dt):
         return np.full(x.shape, 1.0)
     if name in ("expm1", "log1p", "tanh", "sinh", "cosh", "asin", "acos", "cbrt", "asinh", "acosh", "atanh"):
         return np.full(x.shape, BOUND3[name][dt])
+    if name in TRIG_VARIANTS:
+        # 2 ulp away from poles (the reduced argument and the polynomial round)
+        return np.where(np.abs(y.astype(np.float64)) < 1e4, 2.0, np.inf)
     if name == "erf":
         if dt is np.float32:
             return np.full(x.shape, 2.0)
@@ -97,7 +101,8 @@ def _bound(name, x, y, dt):
 @pytest.mark.parametrize("dt,tag", [(np.float32, "f32"), (np.float64, "f64")])
 @pytest.mark.parametrize("name", ["exp", "log", "erfc", "normcdf", "cos", "sin", "exp2", "exp10", "log2", "log10",
                                   "expm1", "log1p", "tanh", "sinh", "cosh", "tan", "asin", "acos",
-                                  "erf", "cbrt", "asinh", "acosh", "atanh"])
+                                  "erf", "cbrt", "asinh", "acosh", "atanh", "cot", "sec", "csc", "sinpi",
+                                  "cospi", "tanpi", "cotpi", "sind", "cosd", "tand", "cotd"])
 def test_golden_mpmath(dt, tag, name):
     g = np.load(os.path.join(G, "math_%s.npz" % tag))
     x, y = g[name + "_x"], g[name + "_y"]
@@ -112,9 +117,14 @@ def test_golden_mpmath(dt, tag, name):
         # separately in test_expa_limits
         keep = x > MINLOG[name][dt]
         x, y = x[keep], y[keep]
+    if name in UNIT_DOMAIN32 and dt is np.float32:
+        # where NT2's f32 reductions of pi / degree arguments stop working
+        # (checked separately in test_trig_units_nt2_limits)
+        keep = np.abs(x) < UNIT_DOMAIN32[name]
+        x, y = x[keep], y[keep]
     mapf = O.map_f32 if dt is np.float32 else O.map_f64
     W = 8 if dt is np.float32 else 4
-    if name in ("cos", "sin", "tan") and dt is np.float32:
+    if name in ("cos", "sin", "tan", "cot", "sec", "csc") and dt is np.float32:
         O.set_ph_fix(True)  # NT2's Payne-Hanek shift is tested separately
     try:
         got = _iso(mapf, name, x, W)
@@ -125,6 +135,8 @@ def test_golden_mpmath(dt, tag, name):
     b = _bound(name, x, y, dt)
     bad = normal & (err > b)
     assert not bad.any(), list(zip(x[bad][:5], got[bad][:5], y[bad][:5], err[bad][:5]))
+    if name in TRIG_VARIANTS:  # NaN exactly where the function is undefined
+        assert np.array_equal(np.isnan(got), np.isnan(y)), x[np.isnan(got) != np.isnan(y)][:5]
     if name in ("erfc", "normcdf") and dt is np.float64:
         far = ~np.isfinite(b) & np.isfinite(y)
         ae = np.abs(got.astype(np.float64) - y.astype(np.float64))
@@ -148,6 +160,9 @@ BOUND3 = {"expm1": {np.float32: 2.0, np.float64: 1.0}, "log1p": {np.float32: 1.0
           # asinh, acosh, atanh: log / log1p of a rounded argument
           "cbrt": {np.float32: 1.0, np.float64: 1.0}, "asinh": {np.float32: 2.0, np.float64: 2.0},
           "acosh": {np.float32: 2.0, np.float64: 1.0}, "atanh": {np.float32: 1.0, np.float64: 1.0}}
+
+TRIG_VARIANTS = ("cot", "sec", "csc", "sinpi", "cospi", "tanpi", "cotpi", "sind", "cosd", "tand", "cotd")
+UNIT_DOMAIN32 = {"tanpi": 2.0 ** 22, "sind": 2.0 ** 24, "cosd": 2.0 ** 24, "tand": 2.0 ** 24, "cotd": 2.0 ** 24}
 
 MINLOG = {"exp2": {np.float32: np.float32(-127.0), np.float64: -1022.0},
           "exp10": {np.float32: np.uint32(0xc2179999).view(np.float32),
@@ -636,3 +651,57 @@ def test_inverse_hyperbolic_structure(dt):
     assert np.array_equal(tc[low], pc[low])
     assert np.all(classic_ulp(tc, pc.astype(np.float64), dt) <= 2.0)
     assert np.array_equal(_tail(mapf, "atanh", u[:3000], dt), t[:3000])
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_trig_units_structure(dt):
+    """pi and degree functions (lane-wise reductions): odd / even bit for bit;
+    exact argument shifts move the quadrant only: sinpi(x + 2m) = sinpi(x),
+    sinpi(x + 1) = -sinpi(x), cospi(x + 1/2) = -sinpi(x), sinpi(1/2 - x) =
+    cospi(x), sind(x + 360 m) = sind(x), sind(90 - x) = cosd(x), all bit for
+    bit on arguments the shifts keep exact; tail and pack forms agree; sec and
+    csc are 1/cos and 1/sin of the pinned cos and sin."""
+    mapf = O.map_f32 if dt is np.float32 else O.map_f64
+    W = 8 if dt is np.float32 else 4
+    rng = np.random.default_rng(46)
+    v = rng.uniform(-50, 50, 4000).astype(dt)
+    for f in ("sinpi", "tanpi", "cotpi", "sind", "tand", "cotd"):
+        assert np.array_equal(_iso(mapf, f, v, W), -_iso(mapf, f, -v, W), equal_nan=True), f
+    for f in ("cospi", "cosd"):
+        assert np.array_equal(_iso(mapf, f, v, W), _iso(mapf, f, -v, W)), f
+    x = (rng.integers(1, 256, 2000) / 1024.0).astype(dt)  # (0, 1/4): every shift below is exact
+    sp = _iso(mapf, "sinpi", x, W)
+    for m in (1, 3, 10):
+        assert np.array_equal(_iso(mapf, "sinpi", x + dt(2 * m), W), sp)
+    assert np.array_equal(_iso(mapf, "sinpi", x + dt(1), W), -sp)
+    assert np.array_equal(_iso(mapf, "cospi", x + dt(0.5), W), -sp)
+    assert np.array_equal(_iso(mapf, "sinpi", dt(0.5) - x, W), _iso(mapf, "cospi", x, W))
+    d = (rng.integers(1, 180, 2000) / 4.0).astype(dt)  # (0, 45) degrees
+    sd = _iso(mapf, "sind", d, W)
+    for m in (1, 2, 7):
+        assert np.array_equal(_iso(mapf, "sind", d + dt(360 * m), W), sd)
+    assert np.array_equal(_iso(mapf, "sind", dt(90) - d, W), _iso(mapf, "cosd", d, W))
+    w = rng.uniform(-1e3, 1e3, 1500).astype(dt)
+    for f in ("sinpi", "cospi", "tanpi", "cotpi", "sind", "cosd", "tand", "cotd"):
+        assert np.array_equal(_tail(mapf, f, w, dt), _iso(mapf, f, w, W), equal_nan=True), f
+    r = rng.uniform(-100, 100, 4000).astype(dt)
+    with np.errstate(divide="ignore"):
+        assert np.array_equal(_iso(mapf, "sec", r, W), (dt(1) / _iso(mapf, "cos", r, W)).astype(dt))
+        assert np.array_equal(_iso(mapf, "csc", r, W), (dt(1) / _iso(mapf, "sin", r, W)).astype(dt))
+        assert np.array_equal(_tail(mapf, "sec", r[:500], dt), (dt(1) / _tail(mapf, "cos", r[:500], dt)).astype(dt))
+
+
+def test_trig_units_nt2_limits():
+    """Where NT2's single-precision pi / degree functions stop working, as
+    written (trig_reduction.hpp:262-318): tanpi's pole test is_flint(x - 1/2)
+    holds for every |x| >= 2^23 (x - 1/2 rounds to an integer), so tanpi is
+    NaN there although tan(pi x) = 0; the degree reduction x - 90 round(x/90)
+    loses the quadrant once x/90 rounds off by more than 1/2 (|x| beyond about
+    2^31), and sind / cosd then evaluate their polynomials far outside
+    [-pi/4, pi/4], giving values far above 1.  f64 keeps both within range."""
+    big = np.array([2.0 ** 23 + 2, 2.0 ** 24, 3.0e9], np.float32)
+    assert np.all(np.isnan(_iso(O.map_f32, "tanpi", big, 8)))
+    assert np.all(_iso(O.map_f64, "tanpi", big.astype(np.float64), 4) == 0)
+    huge = np.array([5e11, 8.2e11, 1e12], np.float32)
+    assert np.all(np.abs(_iso(O.map_f32, "sind", huge, 8)) > 1e3)
+    assert np.all(np.abs(_iso(O.map_f64, "sind", huge.astype(np.float64), 4)) <= 1)
