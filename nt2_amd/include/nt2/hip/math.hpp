@@ -1050,6 +1050,18 @@ NT2H_FN double erfc3_kernel_d(double x) {
   return horner<P>(e3p, x) / horner<P>(e3q, x);
 }
 
+// erfc(x) of the f64 pack form for x = |a| >= 0, before the sign fold
+template <class P>
+NT2H_FN double erfc_abs_d(double x) {
+  NT2H_STRICT
+  const double xx = x * x;
+  double r;
+  if (x < 0.65) r = 1.0 - x * erf1_kernel_d<P>(xx);
+  else if (x < 2.2) r = exp_d<P>(-xx) * erfc2_kernel_d<P>(x);
+  else r = exp_d<P>(-xx) * erfc3_kernel_d<P>(x);
+  return (x == __builtin_inf()) ? 0.0 : r;
+}
+
 // f64 SIMD: simd/common/erfc.hpp:46-89; scalar (pack tail): scalar/erfc.hpp:38-70.
 template <class P, bool Scalar>
 NT2H_FN double erfc_d(double a0) {
@@ -1065,10 +1077,7 @@ NT2H_FN double erfc_d(double a0) {
     else if (x < 6.0) r = exp_d<P>(-xx) * erfc3_kernel_d<P>(x);
     return (a0 < 0.0) ? 2.0 - r : r;
   }
-  if (x < 0.65) r = 1.0 - x * erf1_kernel_d<P>(xx);
-  else if (x < 2.2) r = exp_d<P>(-xx) * erfc2_kernel_d<P>(x);
-  else r = exp_d<P>(-xx) * erfc3_kernel_d<P>(x);
-  r = (x == __builtin_inf()) ? 0.0 : r;
+  r = erfc_abs_d<P>(x);
   return (a0 < 0.0) ? 2.0 - r : r;
 }
 
@@ -1282,15 +1291,25 @@ NT2H_FN double atanh_d(double a0) {
 
 // normcdf(x) = Half*erfc(-Sqrt_2o_2*x):
 // modules/core/statistics/include/nt2/statistics/functions/generic/normcdf.hpp:33
+// erfc's magnitude is taken at c|x|, which is |(-c) x| exactly (rounding is
+// symmetric in sign): normcdf(x) and normcdf(-x) then share one evaluation
+// wherever both appear with the same x.
 template <class P>
 NT2H_FN float normcdf_f(float a0) {
   NT2H_STRICT
-  return 0.5f * erfc_f<P>((-NT2H_F(0x3f3504f3)) * a0);
+  const float c = NT2H_F(0x3f3504f3);
+  return 0.5f * erfc_fold_f((-c) * a0, erfc_abs_f<P>(c * __builtin_fabsf(a0)));
 }
 template <class P, bool Scalar>
 NT2H_FN double normcdf_d(double a0) {
   NT2H_STRICT
-  return 0.5 * erfc_d<P, Scalar>((-NT2H_D(0x3FE6A09E667F3BCD)) * a0);
+  const double c = NT2H_D(0x3FE6A09E667F3BCD);
+  if constexpr (Scalar) {
+    return 0.5 * erfc_d<P, true>((-c) * a0);
+  } else {
+    const double t = (-c) * a0, r = erfc_abs_d<P>(c * __builtin_fabs(a0));
+    return 0.5 * ((t < 0.0) ? 2.0 - r : r);
+  }
 }
 
 // fastnormcdf (single only): generic/fastnormcdf.hpp:39-54

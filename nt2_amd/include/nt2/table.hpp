@@ -46,6 +46,7 @@
 #include <stdexcept>
 #include <string>
 #include <type_traits>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -710,11 +711,123 @@ template <class Tag, class VA, class VB, class VC> struct vview<n3v<Tag, VA, VB,
                                         typename vview<VC>::type>::type type;
 };
 
-template <int G, int A = 0, class Tag, class VA, class R>
-__device__ __forceinline__ void ev(const n1v<Tag, VA>& v, size_t i, R (&r)[G]) {
+// ---- common subexpressions ----------------------------------------------------------
+// A statement evaluates each node view once per element group even when the
+// user wrote it several times (the textbook Black-Scholes tie statement holds
+// d1, sqrt(T), log(S/X) and exp(-rT) two to six times each; NT2 evaluates
+// every occurrence).  Views are numbered in pre-order; an occurrence of a node
+// type whose view equals (same tables, same scalars, bit for bit) that of an
+// earlier occurrence of the same type reuses its value: the test is on kernel
+// arguments, uniform over the wave, and the values are those a second
+// evaluation would give, bit for bit.  The earliest member of each class of
+// equal views is always evaluated (a later member can only be skipped inside a
+// subtree that an earlier equal subtree contains), so the reused value exists.
+template <class A, class B> struct vpair { A a; B b; };  // the two right-hand sides of a tie
+
+template <class V> struct vsize { static constexpr int value = 1; };
+template <class Tag, class VA> struct vsize<n1v<Tag, VA>> { static constexpr int value = 1 + vsize<VA>::value; };
+template <class Tag, class VA, class VB> struct vsize<n2v<Tag, VA, VB>> {
+  static constexpr int value = 1 + vsize<VA>::value + vsize<VB>::value;
+};
+template <class Tag, class VA, class VB, class VC> struct vsize<n3v<Tag, VA, VB, VC>> {
+  static constexpr int value = 1 + vsize<VA>::value + vsize<VB>::value + vsize<VC>::value;
+};
+template <class VA, class VB> struct vsize<vpair<VA, VB>> {
+  static constexpr int value = 1 + vsize<VA>::value + vsize<VB>::value;
+};
+
+template <class V> struct is_node { static constexpr bool value = false; };
+template <class Tag, class VA> struct is_node<n1v<Tag, VA>> { static constexpr bool value = true; };
+template <class Tag, class VA, class VB> struct is_node<n2v<Tag, VA, VB>> { static constexpr bool value = true; };
+template <class Tag, class VA, class VB, class VC> struct is_node<n3v<Tag, VA, VB, VC>> {
+  static constexpr bool value = true;
+};
+
+template <class V> struct nchild { static constexpr int value = 0; };
+template <class Tag, class VA> struct nchild<n1v<Tag, VA>> { static constexpr int value = 1; };
+template <class Tag, class VA, class VB> struct nchild<n2v<Tag, VA, VB>> { static constexpr int value = 2; };
+template <class Tag, class VA, class VB, class VC> struct nchild<n3v<Tag, VA, VB, VC>> {
+  static constexpr int value = 3;
+};
+template <class VA, class VB> struct nchild<vpair<VA, VB>> { static constexpr int value = 2; };
+
+// the view at pre-order index K
+template <int K, class V>
+__host__ __device__ __forceinline__ const auto& vat(const V& v) {
+  if constexpr (K == 0) {
+    return v;
+  } else {
+    constexpr int sa = vsize<decltype(v.a)>::value;
+    if constexpr (K - 1 < sa) {
+      return vat<K - 1>(v.a);
+    } else if constexpr (nchild<V>::value == 2) {
+      return vat<K - 1 - sa>(v.b);
+    } else {
+      constexpr int sb = vsize<decltype(v.b)>::value;
+      if constexpr (K - 1 - sa < sb) return vat<K - 1 - sa>(v.b);
+      else return vat<K - 1 - sa - sb>(v.c);
+    }
+  }
+}
+template <class V, int K> struct vat_t { typedef std::decay_t<decltype(vat<K>(std::declval<const V&>()))> type; };
+
+// occurrences of type X in root R at pre-order indices in [K0, K1)
+template <class R, class X, int K0, int K1>
+constexpr int occurrences() {
+  if constexpr (K0 >= K1) return 0;
+  else return (std::is_same<typename vat_t<R, K0>::type, X>::value ? 1 : 0) + occurrences<R, X, K0 + 1, K1>();
+}
+
+// equality of two views of one type: the same tables and scalars
+template <class T> __device__ __forceinline__ bool veq(const expr::term_v<T>& x, const expr::term_v<T>& y) {
+  return x.p == y.p;
+}
+template <class T> __device__ __forceinline__ bool veq(const expr::scal_v<T>& x, const expr::scal_v<T>& y) {
+  if constexpr (std::is_floating_point<T>::value) {
+    typedef typename std::conditional<sizeof(T) == 4, uint32_t, uint64_t>::type U;
+    return __builtin_bit_cast(U, x.v) == __builtin_bit_cast(U, y.v);
+  } else {
+    return x.v == y.v;
+  }
+}
+template <class Tag, class VA> __device__ __forceinline__ bool veq(const n1v<Tag, VA>& x, const n1v<Tag, VA>& y) {
+  return veq(x.a, y.a);
+}
+template <class Tag, class VA, class VB>
+__device__ __forceinline__ bool veq(const n2v<Tag, VA, VB>& x, const n2v<Tag, VA, VB>& y) {
+  return veq(x.a, y.a) && veq(x.b, y.b);
+}
+template <class Tag, class VA, class VB, class VC>
+__device__ __forceinline__ bool veq(const n3v<Tag, VA, VB, VC>& x, const n3v<Tag, VA, VB, VC>& y) {
+  return veq(x.a, y.a) && veq(x.b, y.b) && veq(x.c, y.c);
+}
+template <class X> __device__ __forceinline__ bool veq(const X&, const X&) { return false; }  // other views: never shared
+
+// per-index value slots of one element group (only node indices hold values)
+template <class R, int G, int K, bool N = is_node<typename vat_t<R, K>::type>::value> struct slot_of {
+  typedef char type;
+};
+template <class R, int G, int K> struct slot_of<R, G, K, true> {
+  typedef pk<G, typename vview<typename vat_t<R, K>::type>::type> type;
+};
+template <class R, int G, class Seq> struct slots;
+template <class R, int G, int... Ps> struct slots<R, G, std::integer_sequence<int, Ps...>> {
+  typedef std::tuple<typename slot_of<R, G, Ps>::type...> type;
+};
+template <class R, int G> struct cse_ctx {
+  typedef R root_t;
+  const R& root;
+  typename slots<R, G, std::make_integer_sequence<int, vsize<R>::value>>::type s;
+};
+
+template <int G, int A, int K, class C, class V, class R>
+__device__ __forceinline__ void evm(C& c, const V& v, size_t i, R (&r)[G]);
+
+template <int G, int A, int K, class C, class Tag, class VA, class R>
+__device__ __forceinline__ void evn(C& c, const n1v<Tag, VA>& v, size_t i, R (&r)[G]) {
   typedef typename vview<VA>::type TA;
   TA a[G];
-  ev<G, A>(v.a, i, a);
+  evm<G, A, K + 1>(c, v.a, i, a);
   if constexpr (std::is_same<Tag, tagfn::if_zero_else_one_>::value) {
 #pragma unroll
     for (int l = 0; l < G; ++l) r[l] = a[l] ? (R)0 : (R)1;
@@ -723,14 +836,14 @@ __device__ __forceinline__ void ev(const n1v<Tag, VA>& v, size_t i, R (&r)[G]) {
   }
 }
 
-template <int G, int A = 0, class Tag, class VA, class VB, class R>
-__device__ __forceinline__ void ev(const n2v<Tag, VA, VB>& v, size_t i, R (&r)[G]) {
+template <int G, int A, int K, class C, class Tag, class VA, class VB, class R>
+__device__ __forceinline__ void evn(C& c, const n2v<Tag, VA, VB>& v, size_t i, R (&r)[G]) {
   typedef typename vview<VA>::type TA;
   typedef typename vview<VB>::type TB;
   TA a[G];
   TB b[G];
-  ev<G, A>(v.a, i, a);
-  ev<G, A>(v.b, i, b);
+  evm<G, A, K + 1>(c, v.a, i, a);
+  evm<G, A, K + 1 + vsize<VA>::value>(c, v.b, i, b);
   if constexpr (std::is_same<Tag, tagfn::pow_>::value && std::is_integral<TB>::value) {
     // generic/pow.hpp:110-132 and :154-168: |b| by repeated squaring, 1/p for b < 0
     static_assert(std::is_floating_point<R>::value, "pow is defined for floating-point bases only");
@@ -771,24 +884,87 @@ __device__ __forceinline__ void ev(const n2v<Tag, VA, VB>& v, size_t i, R (&r)[G
   }
 }
 
-template <int G, int A = 0, class Tag, class VA, class VB, class VC, class R>
-__device__ __forceinline__ void ev(const n3v<Tag, VA, VB, VC>& v, size_t i, R (&r)[G]) {
+template <int G, int A, int K, class C, class Tag, class VA, class VB, class VC, class R>
+__device__ __forceinline__ void evn(C& c, const n3v<Tag, VA, VB, VC>& v, size_t i, R (&r)[G]) {
   _Pragma("clang fp contract(off)")
   typedef typename vview<VA>::type TA;
   typedef typename vview<VB>::type TB;
   typedef typename vview<VC>::type TC;
   TA a[G];
   TB b[G];
-  TC c[G];
-  ev<G, A>(v.a, i, a);
-  ev<G, A>(v.b, i, b);
-  ev<G, A>(v.c, i, c);
+  TC cc[G];
+  evm<G, A, K + 1>(c, v.a, i, a);
+  evm<G, A, K + 1 + vsize<VA>::value>(c, v.b, i, b);
+  evm<G, A, K + 1 + vsize<VA>::value + vsize<VB>::value>(c, v.c, i, cc);
 #pragma unroll
   for (int l = 0; l < G; ++l) {
-    if constexpr (std::is_same<Tag, tagfn::if_else_>::value) r[l] = a[l] ? (R)b[l] : (R)c[l];
-    else if constexpr (std::is_same<Tag, tagfn::fma_>::value) r[l] = dm::fma<P>((R)a[l], (R)b[l], (R)c[l]);
-    else r[l] = dm::fnms<P>((R)a[l], (R)b[l], (R)c[l]);
+    if constexpr (std::is_same<Tag, tagfn::if_else_>::value) r[l] = a[l] ? (R)b[l] : (R)cc[l];
+    else if constexpr (std::is_same<Tag, tagfn::fma_>::value) r[l] = dm::fma<P>((R)a[l], (R)b[l], (R)cc[l]);
+    else r[l] = dm::fnms<P>((R)a[l], (R)b[l], (R)cc[l]);
   }
+}
+
+// occurrence K of node type V: the value of the first earlier occurrence Q
+// with an equal view, else evaluated
+template <int G, int A, int K, int Q, class C, class V, class R>
+__device__ __forceinline__ void reuse_or_eval(C& c, const V& v, size_t i, R (&r)[G]) {
+  typedef typename C::root_t Root;
+  if constexpr (Q >= K) {
+    evn<G, A, K>(c, v, i, r);
+  } else if constexpr (std::is_same<typename vat_t<Root, Q>::type, V>::value) {
+    if (veq(v, vat<Q>(c.root))) {
+      const auto& q = std::get<Q>(c.s);
+#pragma unroll
+      for (int l = 0; l < G; ++l) r[l] = q.v[l];
+    } else {
+      reuse_or_eval<G, A, K, Q + 1>(c, v, i, r);
+    }
+  } else {
+    reuse_or_eval<G, A, K, Q + 1>(c, v, i, r);
+  }
+}
+
+template <int G, int A, int K, class C, class V, class R>
+__device__ __forceinline__ void evm(C& c, const V& v, size_t i, R (&r)[G]) {
+  if constexpr (!is_node<V>::value) {
+    ev<G, A>(v, i, r);
+  } else {
+    typedef typename C::root_t Root;
+    constexpr int before = occurrences<Root, V, 0, K>();
+    constexpr int after = occurrences<Root, V, K + 1, vsize<Root>::value>();
+    if constexpr (before > 0) reuse_or_eval<G, A, K, 0>(c, v, i, r);
+    else evn<G, A, K>(c, v, i, r);
+    if constexpr (after > 0) {
+      auto& q = std::get<K>(c.s);
+#pragma unroll
+      for (int l = 0; l < G; ++l) q.v[l] = r[l];
+    }
+  }
+}
+
+// a node view evaluated with its common subexpressions shared
+template <int G, int A = 0, class Tag, class VA, class R>
+__device__ __forceinline__ void ev(const n1v<Tag, VA>& v, size_t i, R (&r)[G]) {
+  cse_ctx<n1v<Tag, VA>, G> c{v, {}};
+  evm<G, A, 0>(c, v, i, r);
+}
+template <int G, int A = 0, class Tag, class VA, class VB, class R>
+__device__ __forceinline__ void ev(const n2v<Tag, VA, VB>& v, size_t i, R (&r)[G]) {
+  cse_ctx<n2v<Tag, VA, VB>, G> c{v, {}};
+  evm<G, A, 0>(c, v, i, r);
+}
+template <int G, int A = 0, class Tag, class VA, class VB, class VC, class R>
+__device__ __forceinline__ void ev(const n3v<Tag, VA, VB, VC>& v, size_t i, R (&r)[G]) {
+  cse_ctx<n3v<Tag, VA, VB, VC>, G> c{v, {}};
+  evm<G, A, 0>(c, v, i, r);
+}
+// the two right-hand sides of a tie, sharing their common subexpressions
+template <int G, int A = 0, class V1, class V2, class T>
+__device__ __forceinline__ void ev2(const V1& v1, const V2& v2, size_t i, T (&r1)[G], T (&r2)[G]) {
+  const vpair<V1, V2> root{v1, v2};
+  cse_ctx<vpair<V1, V2>, G> c{root, {}};
+  evm<G, A, 1>(c, root.a, i, r1);
+  evm<G, A, 1 + vsize<V1>::value>(c, root.b, i, r2);
 }
 
 // the pack's W results stored with 16-byte nontemporal stores (A) or one by one
@@ -865,15 +1041,13 @@ __global__ __launch_bounds__(256) void transform2_k(T* __restrict__ o1, T* __res
   for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < items; t += stride) {
     if (t < nsub) {
       T r1[SG], r2[SG];
-      ev<SG, A>(v1, t * SG, r1);
-      ev<SG, A>(v2, t * SG, r2);
+      ev2<SG, A>(v1, v2, t * SG, r1, r2);
       store_pack<A>(o1, t * SG, r1);
       store_pack<A>(o2, t * SG, r2);
     } else {
       const size_t i = npacks * W + (t - nsub);
       T r1[1], r2[1];
-      ev<1>(v1, i, r1);
-      ev<1>(v2, i, r2);
+      ev2<1>(v1, v2, i, r1, r2);
       o1[i] = r1[0];
       o2[i] = r2[0];
     }
