@@ -21,6 +21,7 @@
 //  * globalsum/dot fold the whole table (modules/core/reduction/include/nt2/
 //    core/functions/globalsum.hpp:24-88, reduction/functions/expr/dot.hpp:22-29),
 //    here with compensated FP32 partials and an FP64 cross-block combine;
+//    integer tables sum in the element type, wrapping, as NT2's sum does;
 //  * sum(a[, dim]) and cumsum(a[, dim]) reduce or scan along a dimension
 //    (default: the first non-singleton one) in NT2's order under the exact
 //    policy (modules/core/reduction/include/nt2/dsl/functions/reduction/run.hpp:30-93),
@@ -42,6 +43,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <cstring>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -187,6 +189,12 @@ struct sel3 { template <class C, class A, class B> struct result { typedef typen
 struct same3 { template <class A, class B, class C> struct result { typedef typename common<typename common<A, B>::type, C>::type type; }; };
 struct selinc2 { template <class C, class A> struct result { typedef A type; }; };
 struct izeo1 { template <class C> struct result { typedef uint8_t type; }; };
+// pow: the base's type for an integral exponent (generic/pow.hpp:110-168)
+struct pow2 {
+  template <class A, class B> struct result {
+    typedef typename std::conditional<std::is_integral<B>::value, A, typename common<A, B>::type>::type type;
+  };
+};
 
 #define NT2H_TAG(name, base) struct name : base {};
 NT2H_TAG(plus_, same2) NT2H_TAG(minus_, same2) NT2H_TAG(multiplies_, same2) NT2H_TAG(divides_, same2)
@@ -205,7 +213,7 @@ NT2H_TAG(erf_, same1) NT2H_TAG(cbrt_, same1) NT2H_TAG(asinh_, same1) NT2H_TAG(ac
 NT2H_TAG(atanh_, same1) NT2H_TAG(cot_, same1) NT2H_TAG(sec_, same1) NT2H_TAG(csc_, same1)
 NT2H_TAG(sinpi_, same1) NT2H_TAG(cospi_, same1) NT2H_TAG(tanpi_, same1) NT2H_TAG(cotpi_, same1)
 NT2H_TAG(sind_, same1) NT2H_TAG(cosd_, same1) NT2H_TAG(tand_, same1) NT2H_TAG(cotd_, same1)
-NT2H_TAG(pow_, same2)
+NT2H_TAG(pow_, pow2)
 NT2H_TAG(fma_, same3) NT2H_TAG(fnms_, same3) NT2H_TAG(if_else_, sel3)
 NT2H_TAG(selinc_, selinc2) NT2H_TAG(seldec_, selinc2) NT2H_TAG(if_zero_else_one_, izeo1)
 #undef NT2H_TAG
@@ -623,14 +631,16 @@ NT2H_FN2(max, max_) NT2H_FN2(min, min_) NT2H_FN2(adds, adds_) NT2H_FN2(subs, sub
 NT2H_FN2(selinc, selinc_) NT2H_FN2(seldec, seldec_) NT2H_FN2(atan2, atan2_)
 #undef NT2H_FN2
 // pow(a, b): Cephes' pow for floating-point exponents; an integral exponent
-// (a scalar, kept as int, or an integer table) takes NT2's repeated squaring
-// instead (modules/core/exponential/include/nt2/exponential/functions/generic/pow.hpp:110-168).
+// (a scalar, kept in its own type, or an integer table) takes NT2's repeated
+// squaring instead, 1/p only for a negative signed exponent
+// (modules/core/exponential/include/nt2/exponential/functions/generic/pow.hpp:110-168).
 template <class A, class B, NT2H_ENABLE2(A, B)> auto pow(const A& a, const B& b) {
+  static_assert(!std::is_same<B, bool>::value, "pow: a logical exponent");
   if constexpr (std::is_integral<B>::value && expr::is_dsl<A>::value) {
     typedef typename expr::wrap<A>::type EA;
-    expr::node2<tagfn::pow_, EA, expr::scalar<int>> n;
+    expr::node2<tagfn::pow_, EA, expr::scalar<B>> n;
     n.a = expr::wrap<A>::get(a);
-    n.b.v = (int)b;
+    n.b.v = b;
     return n;
   } else {
     return make2<tagfn::pow_>(a, b);
@@ -852,13 +862,15 @@ __device__ __forceinline__ void evn(C& c, const n2v<Tag, VA, VB>& v, size_t i, R
       _Pragma("clang fp contract(off)")
       R base = (R)a[l], p = R(1);
       typedef typename std::make_unsigned<TB>::type UB;
-      UB e = b[l] < 0 ? (UB)(UB(0) - (UB)b[l]) : (UB)b[l];
+      bool neg = false;
+      if constexpr (std::is_signed<TB>::value) neg = b[l] < 0;
+      UB e = neg ? (UB)(UB(0) - (UB)b[l]) : (UB)b[l];
       while (e) {
         if (e & 1u) p = p * base;
         e >>= 1;
         base = base * base;
       }
-      r[l] = b[l] < 0 ? R(1) / p : p;
+      r[l] = neg ? R(1) / p : p;
     }
   } else if constexpr (std::is_same<Tag, tagfn::pow_>::value) {
     static_assert(std::is_floating_point<R>::value, "pow is defined for floating-point tables only");
@@ -1113,6 +1125,46 @@ __global__ __launch_bounds__(256) inline void fold_final_k(const double* __restr
   if (threadIdx.x == 0) *out = t;
 }
 
+// integer tables: NT2 sums in the element type, wrapping; the sum modulo 2^64
+// (in any order) reduced to the type's width is that sum
+__device__ __forceinline__ uint64_t block_sum256_u64(uint64_t d) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) d += __shfl_xor(d, off, 64);
+  __shared__ uint64_t w[4];
+  if ((threadIdx.x & 63) == 0) w[threadIdx.x >> 6] = d;
+  __syncthreads();
+  return ((w[0] + w[1]) + w[2]) + w[3];
+}
+template <class T, class V, int A, int SG>
+__global__ __launch_bounds__(256) void fold_int_k(V v, size_t n, uint64_t* __restrict__ partial) {
+  constexpr int W = pack_width<T>::value;
+  const size_t npacks = n / W;
+  const size_t nsub = npacks * (W / SG);
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const size_t g = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint64_t s = 0;
+  for (size_t t = g; t < nsub; t += stride) {
+    T r[SG];
+    ev<SG, A>(v, t * SG, r);
+#pragma unroll
+    for (int l = 0; l < SG; ++l) s += (uint64_t)(long long)r[l];
+  }
+  for (size_t t = g; t < n - npacks * W; t += stride) {
+    T r[1];
+    ev<1>(v, npacks * W + t, r);
+    s += (uint64_t)(long long)r[0];
+  }
+  const uint64_t b = block_sum256_u64(s);
+  if (threadIdx.x == 0) partial[blockIdx.x] = b;
+}
+__global__ __launch_bounds__(256) inline void fold_final_int_k(const uint64_t* __restrict__ partial, int nb,
+                                                              uint64_t* __restrict__ out) {
+  uint64_t d = 0;
+  for (int k = threadIdx.x; k < nb; k += 256) d += partial[k];
+  const uint64_t t = block_sum256_u64(d);
+  if (threadIdx.x == 0) *out = t;
+}
+
 // workgroups of a grid-stride launch over `items` work items (at most 32 per
 // compute unit of a 256-CU MI355X)
 inline unsigned grid(size_t items) {
@@ -1265,6 +1317,26 @@ void transform(T* out, const EX& x, size_t n, hipStream_t s = nullptr) {
   fe::check(hipGetLastError(), "transform launch");
 }
 
+// sum of an integer expression's n elements in T (wrapping as NT2's sum in T
+// does), returned on the host
+template <class T, class EX>
+T fold_int(const EX& x, size_t n, hipStream_t s = nullptr) {
+  typedef fe::shape<T, EX> sh;
+  const size_t items = n / sh::SG;
+  const unsigned nb = (unsigned)std::min<size_t>(1024, (items + 255) / 256 ? (items + 255) / 256 : 1);
+  uint64_t* buf = reinterpret_cast<uint64_t*>(fe::fold_scratch(nb + 1));  // 8-byte slots either way
+  auto v = expr::dview<EX>::get(x);
+  hipLaunchKernelGGL((fe::fold_int_k<T, decltype(v), 0, sh::SG>), dim3(nb), dim3(256), 0, s, v, n, buf);
+  fe::check(hipGetLastError(), "fold launch");
+  hipLaunchKernelGGL(fe::fold_final_int_k, dim3(1), dim3(256), 0, s, buf, (int)nb, buf + nb);
+  double* hr = fe::fold_result_host();
+  fe::check(hipMemcpyAsync(hr, buf + nb, sizeof(uint64_t), hipMemcpyDeviceToHost, s), "fold result");
+  fe::check(hipStreamSynchronize(s), "fold sync");
+  uint64_t u;
+  std::memcpy(&u, hr, sizeof u);
+  return (T)(typename std::make_unsigned<T>::type)u;
+}
+
 // FP64 value of the compensated fold of x's n elements (synchronises s: the
 // value is returned on the host, as NT2's globalsum returns it)
 template <class T, class EX>
@@ -1310,8 +1382,10 @@ typename expr::vt<typename expr::wrap<E>::type>::type globalsum(const E& e) {
   const EX& x = expr::wrap<E>::get(e);
   const ext_t ex = x.ext();
   const size_t n = ex.has ? ex.s.numel() : 1;
+  static_assert(!std::is_same<T, bool>::value, "globalsum of a logical expression: convert it to a number first");
   x.prepare();
-  return (T)hip::fold<T>(x, n);
+  if constexpr (std::is_integral<T>::value) return hip::fold_int<T>(x, n);
+  else return (T)hip::fold<T>(x, n);
 }
 // ---- reductions and scans along a dimension ------------------------------------------
 namespace hip { namespace fe {

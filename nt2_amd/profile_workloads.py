@@ -5,6 +5,7 @@
     python3 -m nt2_amd.profile_workloads fold    # saxpy + dot 2^20 and 2^26
     python3 -m nt2_amd.profile_workloads sd      # sigma-delta 64 x 1920x1080
     python3 -m nt2_amd.profile_workloads batch   # 4096 uneven tables, persistent (256x32, 8/CU) vs plain
+    python3 -m nt2_amd.profile_workloads batchp  # the persistent schedule alone
     python3 -m nt2_amd.profile_workloads dimwise # sum / cumsum of 8192 x 8192 f32, dims 1 and 2, both policies
     python3 -m nt2_amd.profile_workloads images  # rgb2grey / rgb2yuv 8192 x 8192, mandelbrot 1600 x 1600
 
@@ -72,7 +73,7 @@ def sd(reps=3):
     _copy()
 
 
-def batch(reps=2, count=4096, seed=4):
+def batch(reps=2, count=4096, seed=4, plain=True):
     from .batch import Batch, new_sums, uneven_tables
     x, y, offs, lens = uneven_tables(count, seed=seed)
     sums = new_sums(count)
@@ -80,9 +81,16 @@ def batch(reps=2, count=4096, seed=4):
         b.set_config(256, 32, 8)  # the autotuner's choice for this workload on MI355X
         for _ in range(reps):
             b.run(sums)
-            b.run_plain(sums)
+            if plain:
+                b.run_plain(sums)
         torch.cuda.synchronize()
     _copy()
+
+
+def batchp():
+    """the persistent schedule alone (counter passes: 4096 short plain
+    launches per run add nothing to its figures)"""
+    batch(plain=False)
 
 
 def dimwise(reps=2):
@@ -120,6 +128,7 @@ def images(reps=3):
 
 if __name__ == "__main__":
     what = sys.argv[1] if len(sys.argv) > 1 else "bs"
-    {"bs": bs, "bs64": bs64, "fold": fold, "sd": sd, "batch": batch, "dimwise": dimwise, "images": images}[what]()
+    {"bs": bs, "bs64": bs64, "fold": fold, "sd": sd, "batch": batch, "batchp": batchp, "dimwise": dimwise,
+     "images": images}[what]()
     torch.cuda.synchronize()
     print("profile_workloads %s done" % what, flush=True)

@@ -153,6 +153,7 @@ int main(int argc, char** argv) {
   dump("sd_bkg", background_img);
   dump("sd_var", image_variance);
   dump("sd_label", binary_label);
+  dump_scalar("frame1_sum", nt2::globalsum(frames[1]));  // uint8: NT2 sums in the element type, wrapping
 
   // ---- sum / cumsum along dimensions (NT2's unit tests
   // modules/core/reduction/unit/table/sum.cpp and modules/core/swar/unit/table/cumsum.cpp)
@@ -219,6 +220,9 @@ int main(int argc, char** argv) {
   dump("pow_f32", at2);
   at2 = nt2::pow(at, 3) + nt2::pow(bt, -2);  // integral exponents: repeated squaring
   dump("powi_f32", at2);
+  // unsigned and 64-bit exponents keep their type: 3e9 squares, no 1/p
+  at2 = nt2::pow(at, 2u) + nt2::pow(bt * 0.001f + 1.0f, 3000000000u) - nt2::pow(at, (long long)-3);
+  dump("powu_f32", at2);
   at2 = nt2::erf(at) + nt2::cbrt(bt * 50.0f) - nt2::asinh(bt * at * 100.0f);
   dump("erf_f32", at2);
   at2 = nt2::acosh(nt2::abs(bt) * 1.0e6f + 1.0f) * nt2::atanh(at / 3.0f);
@@ -293,6 +297,7 @@ int main(int argc, char** argv) {
   dump("mandel_a", MA);
   dump("mandel_b", MB);
   dump("mandel", MC);
+  dump_scalar("mandel_sum", nt2::globalsum(MC * 1000003));  // int32, wrapping past 2^31
   table<float> lin = linspace(0.0f, 1.0f);
   dump("linspace100", lin);
 

@@ -78,6 +78,8 @@ def test_sigmadelta_statements(outputs):
     assert np.array_equal(outputs("sd_bkg", np.uint8), b)
     assert np.array_equal(outputs("sd_var", np.uint8), v)
     assert np.array_equal(outputs("sd_label", np.uint8), l)
+    # globalsum of an integer table: the sum in the element type, wrapping
+    assert outputs("frame1_sum", np.uint8)[0] == int(frames[1].astype(np.uint64).sum()) % 256
 
 
 def test_dimwise_kats(outputs):
@@ -152,6 +154,10 @@ def test_exponential_rounding_statements(outputs):
     assert same_bits(got, want), first_diff(got, want)
     want = O.pow_int(a, 3) + O.pow_int(b, -2)
     got = outputs("powi_f32", f32)
+    assert same_bits(got, want), first_diff(got, want)
+    near1 = (b * f32(0.001)).astype(f32) + f32(1)
+    want = (O.pow_int(a, 2) + O.pow_int(near1, 3000000000)) - O.pow_int(a, -3)
+    got = outputs("powu_f32", f32)
     assert same_bits(got, want), first_diff(got, want)
     want = (O.map_f32("erf", a) + O.map_f32("cbrt", b * f32(50))) - O.map_f32("asinh", b * a * f32(100))
     got = outputs("erf_f32", f32)
@@ -240,6 +246,9 @@ def test_mandelbrot_demo_statements(outputs):
     C = outputs("mandel", np.int32)
     want = O.mandelbrot(A, B, 256)
     assert np.array_equal(C, want), first_diff(C, want)
+    wrapped = np.array([int((C.astype(np.int64) * 1000003).sum()) % 2 ** 32], np.uint32).view(np.int32)[0]
+    assert (C.astype(np.int64) * 1000003).sum() > 2 ** 31  # the int32 sum wraps
+    assert outputs("mandel_sum", np.int32)[0] == wrapped
     assert same_bits(outputs("linspace100", np.float32), O.linspace(0.0, 1.0, 100))
 
 
