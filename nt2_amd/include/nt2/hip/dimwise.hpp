@@ -541,7 +541,7 @@ __global__ __launch_bounds__(256) void chunk_final_k(const double* __restrict__ 
 
 // exclusive prefix of the chunk totals of each line (in place), FP64: one
 // wave per line, 64 chunks at a time (inclusive shuffle scan, then the base)
-__global__ __launch_bounds__(256) inline void chunk_offsets_k(double* __restrict__ partial, size_t nchunk,
+static __global__ __launch_bounds__(256) void chunk_offsets_k(double* __restrict__ partial, size_t nchunk,
                                                               size_t nline) {
   const size_t line = ((size_t)blockIdx.x * blockDim.x + threadIdx.x) / 64;
   const int lane = threadIdx.x & 63;

@@ -115,7 +115,8 @@ inline int firstnonsingleton(const of_size_& s) {
   return 1;
 }
 
-template <class T> class table;
+// table<T> is table<T, void>; table<T, S> takes NT2's settings S (below)
+template <class T, class S = void> class table;
 
 // ---- expression node types ----------------------------------------------------
 namespace expr {
@@ -412,7 +413,7 @@ template <> struct apply2<tagfn::seldec_> {
 
 // ---- user-facing table ------------------------------------------------------
 template <class T>
-class table {
+class table<T, void> {
  public:
   typedef T value_type;
   table() : ext_{{0, 1, 1, 1}} {}
@@ -539,7 +540,7 @@ template <class T> void terminal<T>::prepare() const { t->sync_dev(); }
 template <class T> term_v<T> terminal<T>::view() const { return term_v<T>{t->device_data()}; }
 
 template <class X> struct wrap { typedef X type; static const X& get(const X& x) { return x; } };
-template <class T> struct wrap<table<T>> {
+template <class T, class S> struct wrap<table<T, S>> {
   typedef terminal<T> type;
   static terminal<T> get(const table<T>& t) { terminal<T> r; r.t = &t; return r; }
 };
@@ -551,11 +552,141 @@ template <class X, class Hint> struct as_expr<X, Hint, true> {
 template <class X> struct is_dsl {
   static constexpr bool value = std::is_base_of<is_expr_base, X>::value;
 };
-template <class T> struct is_dsl<table<T>> { static constexpr bool value = true; };
+template <class T, class S> struct is_dsl<table<T, S>> { static constexpr bool value = true; };
 template <class X> struct vt { typedef typename X::value_type type; };
 template <class X, bool = std::is_arithmetic<X>::value> struct vt_or { typedef typename X::value_type type; };
 template <class X> struct vt_or<X, true> { typedef X type; };
 }  // namespace expr
+
+// ---- table settings -----------------------------------------------------------------
+// NT2's table<T, S> takes one option or settings(options...)
+// (modules/core/sdk/include/nt2/core/settings/forward/storage_order.hpp:17-33,
+// index.hpp:34-57, sharing.hpp:17-18, shape.hpp:32-49, alignment.hpp:16-23,
+// storage_duration.hpp:16-17; _1D.._4D in core/utility/of_size/predef.hpp:25-29
+// are of_size_ types with every size set at run time).  Here the element
+// storage is always column-major, owned, aligned and dense; options that ask
+// for exactly that are accepted, the index base and the number of dimensions
+// are honoured, and the others stop the compilation with a message.
+struct settings {};
+struct column_major_ {};
+typedef column_major_ matlab_order_;
+typedef column_major_ fortran_order_;
+struct row_major_ {};
+typedef row_major_ C_order_;
+template <int B> struct index_ {};
+typedef index_<0> C_index_;
+typedef index_<1> matlab_index_;
+template <int N> struct dims_ {};  // NT2's _1D ... _4D: at most N dimensions, sizes set at run time
+typedef dims_<1> _1D;
+typedef dims_<2> _2D;
+typedef dims_<3> _3D;
+typedef dims_<4> _4D;
+struct automatic_ {};
+struct dynamic_ {};
+struct owned_ {};
+struct shared_ {};
+struct aligned_ {};
+struct unaligned_ {};
+struct rectangular_ {};
+struct conventional_ {};
+struct packed_ {};
+struct upper_triangular_ {};
+struct lower_triangular_ {};
+struct diagonal_ {};
+struct symmetric_ {};
+struct positive_definite_ {};
+
+namespace detail {
+// what one option sets: the index base, the number of dimensions, or nothing
+template <class O> struct option {
+  static constexpr bool known = false;
+  static constexpr int base = 0, nbase = 0, dims = 0, ndims = 0;
+};
+template <int B> struct option<index_<B>> {
+  static constexpr bool known = true;
+  static constexpr int base = B, nbase = 1, dims = 0, ndims = 0;
+};
+template <int N> struct option<dims_<N>> {
+  static constexpr bool known = true;
+  static constexpr int base = 0, nbase = 0, dims = N, ndims = 1;
+};
+#define NT2H_NEUTRAL(O)                                           \
+  template <> struct option<O> {                                  \
+    static constexpr bool known = true;                           \
+    static constexpr int base = 0, nbase = 0, dims = 0, ndims = 0; \
+  };
+NT2H_NEUTRAL(column_major_) NT2H_NEUTRAL(automatic_) NT2H_NEUTRAL(dynamic_) NT2H_NEUTRAL(owned_)
+NT2H_NEUTRAL(aligned_) NT2H_NEUTRAL(unaligned_) NT2H_NEUTRAL(rectangular_) NT2H_NEUTRAL(conventional_)
+#undef NT2H_NEUTRAL
+template <class... O> struct options_of {
+  static constexpr bool known = (option<O>::known && ... && true);
+  static constexpr int nbase = (option<O>::nbase + ... + 0), ndims = (option<O>::ndims + ... + 0);
+  static constexpr long base = nbase ? (option<O>::base + ... + 0) : 1;  // NT2's default: matlab_index_
+  static constexpr int dims = ndims ? (option<O>::dims + ... + 0) : 4;   // NT2_MAX_DIMENSIONS
+};
+template <class S> struct parse : options_of<S> {};
+template <class... O> struct parse<settings(O...)> : options_of<O...> {};
+}  // namespace detail
+
+template <class T, class S>
+class table : public table<T, void> {
+  typedef table<T, void> base_t;
+  typedef detail::parse<S> cfg;
+  static_assert(cfg::known, "nt2 table settings: only column-major, owned, dense storage (with an index "
+                             "base and a number of dimensions) is provided by this implementation");
+  static_assert(cfg::nbase <= 1 && cfg::ndims <= 1, "nt2 table settings: one index base and one number of dimensions");
+  static_assert(cfg::dims >= 1 && cfg::dims <= 4, "nt2 table settings: 1 to 4 dimensions");
+
+ public:
+  static constexpr long index_base = cfg::base;
+  static constexpr int max_dims = cfg::dims;
+  typedef T value_type;
+
+  table() = default;
+  explicit table(of_size_ s) : base_t() { resize(s); }
+  table(const table& o) : base_t(o) {}
+  table(const base_t& o) : base_t() { *this = o; }
+  template <class E, class = typename std::enable_if<std::is_base_of<expr::is_expr_base, E>::value>::type>
+  table(const E& e) : base_t() { *this = e; }
+
+  table& operator=(const table& o) {
+    base_t::operator=(o);
+    return *this;
+  }
+  table& operator=(const base_t& o) {
+    check_dims(o.extent());
+    base_t::operator=(o);
+    return *this;
+  }
+  template <class E>
+  typename std::enable_if<std::is_base_of<expr::is_expr_base, E>::value, table&>::type operator=(const E& e) {
+    const ext_t x = expr::wrap<E>::get(e).ext();
+    if (x.has) check_dims(x.s);
+    base_t::operator=(e);
+    return *this;
+  }
+  void resize(of_size_ s) {
+    check_dims(s);
+    base_t::resize(s);
+  }
+  // index_base-based, column-major host access
+  T& operator()(size_t i) { return base_t::operator()(i + 1 - index_base); }
+  T operator()(size_t i) const { return base_t::operator()(i + 1 - index_base); }
+  T& operator()(size_t i, size_t j) { return base_t::operator()(i + 1 - index_base, j + 1 - index_base); }
+  T operator()(size_t i, size_t j) const { return base_t::operator()(i + 1 - index_base, j + 1 - index_base); }
+  T& operator()(size_t i, size_t j, size_t k) {
+    return base_t::operator()(i + 1 - index_base, j + 1 - index_base, k + 1 - index_base);
+  }
+  T operator()(size_t i, size_t j, size_t k) const {
+    return base_t::operator()(i + 1 - index_base, j + 1 - index_base, k + 1 - index_base);
+  }
+
+ private:
+  static void check_dims(const of_size_& s) {
+    for (int d = max_dims; d < 4; ++d)
+      if (s.d[d] != 1) throw std::invalid_argument("nt2: extent has more dimensions than the table's settings allow");
+  }
+};
 
 // ---- builders ---------------------------------------------------------------------
 #define NT2H_ENABLE1(A) typename std::enable_if<expr::is_dsl<A>::value, int>::type = 0
@@ -1117,7 +1248,7 @@ __global__ __launch_bounds__(256) void fold_k(V v, size_t n, double* __restrict_
   if (threadIdx.x == 0) partial[blockIdx.x] = b;
 }
 
-__global__ __launch_bounds__(256) inline void fold_final_k(const double* __restrict__ partial, int nb,
+static __global__ __launch_bounds__(256) void fold_final_k(const double* __restrict__ partial, int nb,
                                                           double* __restrict__ out) {
   double d = 0.0;
   for (int k = threadIdx.x; k < nb; k += 256) d += partial[k];
@@ -1157,7 +1288,7 @@ __global__ __launch_bounds__(256) void fold_int_k(V v, size_t n, uint64_t* __res
   const uint64_t b = block_sum256_u64(s);
   if (threadIdx.x == 0) partial[blockIdx.x] = b;
 }
-__global__ __launch_bounds__(256) inline void fold_final_int_k(const uint64_t* __restrict__ partial, int nb,
+static __global__ __launch_bounds__(256) void fold_final_int_k(const uint64_t* __restrict__ partial, int nb,
                                                               uint64_t* __restrict__ out) {
   uint64_t d = 0;
   for (int k = threadIdx.x; k < nb; k += 256) d += partial[k];

@@ -327,6 +327,30 @@ int main(int argc, char** argv) {
     dump("coherence", probe);
     dump("coherence_cp", cp);
   }
+  // ---- table settings: index base and number of dimensions
+  {
+    table<float, settings(C_index_, _2D)> c0(of_size(3, 4));
+    for (std::size_t j = 0; j < 4; ++j)
+      for (std::size_t i = 0; i < 3; ++i) c0(i, j) = float(10 * i + j);  // 0-based
+    table<float, _2D> m1 = c0 * 2.0f;                                     // 1-based
+    table<double, matlab_index_> a6 = reshape(_(1.0, 6.0), 2, 3);
+    table<double, settings(matlab_order_, matlab_index_, _2D)> d1 = cumsum(a6);
+    table<float> probe(of_size(7));
+    probe(1) = m1(1, 1);
+    probe(2) = m1(3, 4);
+    probe(3) = c0(2, 3);
+    probe(4) = float(d1(2, 3));
+    bool threw = false;
+    try { table<float, _2D> bad(of_size(2, 2, 2)); } catch (const std::invalid_argument&) { threw = true; }
+    probe(5) = threw ? 1.0f : 0.0f;
+    threw = false;
+    try { table<float, _1D> v = c0 + 1.0f; } catch (const std::invalid_argument&) { threw = true; }
+    probe(6) = threw ? 1.0f : 0.0f;
+    table<float, _1D> v1 = reshape(_(1.0f, 5.0f), of_size(5));  // a column: _(a, b) is a row
+    probe(7) = globalsum(v1 * v1);
+    dump("settings", probe);
+    dump("settings_m1", m1);
+  }
   std::printf("frontend demos done: %zu elements\n", size);
   return 0;
 }

@@ -265,3 +265,15 @@ def test_device_storage_coherence(outputs):
     want[6] = np.float32(0.0)  # -1 + 1
     want[0] = np.float32(43.0)
     assert np.array_equal(cp, want)
+
+
+def test_table_settings(outputs):
+    """table<T, S> with NT2's settings: C_index_ tables index from 0,
+    matlab_index_ (the default) from 1, _1D/_2D reject extents with more
+    dimensions (construction and assignment from an expression)."""
+    p = outputs("settings", np.float32)
+    # c0(i, j) = 10 i + j (0-based); m1 = 2 c0 read 1-based; d1 = cumsum of reshape(1:6, 2, 3)
+    assert list(p) == [0.0, 46.0, 23.0, 11.0, 1.0, 1.0, 55.0]
+    m1 = outputs("settings_m1", np.float32).reshape(4, 3).T  # column-major 3x4
+    i, j = np.meshgrid(np.arange(3), np.arange(4), indexing="ij")
+    assert np.array_equal(m1, (2 * (10 * i + j)).astype(np.float32))
