@@ -1294,21 +1294,37 @@ NT2H_FN double atanh_d(double a0) {
 // erfc's magnitude is taken at c|x|, which is |(-c) x| exactly (rounding is
 // symmetric in sign): normcdf(x) and normcdf(-x) then share one evaluation
 // wherever both appear with the same x.
+// normcdf_mag(|x|) is that shared part, normcdf_join(x, m) the rest.
+template <class P>
+NT2H_FN float normcdf_mag_f(float ax) {
+  NT2H_STRICT
+  return erfc_abs_f<P>(NT2H_F(0x3f3504f3) * ax);
+}
+NT2H_FN float normcdf_join_f(float a0, float m) {
+  NT2H_STRICT
+  return 0.5f * erfc_fold_f((-NT2H_F(0x3f3504f3)) * a0, m);
+}
 template <class P>
 NT2H_FN float normcdf_f(float a0) {
+  return normcdf_join_f(a0, normcdf_mag_f<P>(__builtin_fabsf(a0)));
+}
+template <class P>
+NT2H_FN double normcdf_mag_d(double ax) {
   NT2H_STRICT
-  const float c = NT2H_F(0x3f3504f3);
-  return 0.5f * erfc_fold_f((-c) * a0, erfc_abs_f<P>(c * __builtin_fabsf(a0)));
+  return erfc_abs_d<P>(NT2H_D(0x3FE6A09E667F3BCD) * ax);
+}
+NT2H_FN double normcdf_join_d(double a0, double m) {
+  NT2H_STRICT
+  const double t = (-NT2H_D(0x3FE6A09E667F3BCD)) * a0;
+  return 0.5 * ((t < 0.0) ? 2.0 - m : m);
 }
 template <class P, bool Scalar>
 NT2H_FN double normcdf_d(double a0) {
   NT2H_STRICT
-  const double c = NT2H_D(0x3FE6A09E667F3BCD);
   if constexpr (Scalar) {
-    return 0.5 * erfc_d<P, true>((-c) * a0);
+    return 0.5 * erfc_d<P, true>((-NT2H_D(0x3FE6A09E667F3BCD)) * a0);
   } else {
-    const double t = (-c) * a0, r = erfc_abs_d<P>(c * __builtin_fabs(a0));
-    return 0.5 * ((t < 0.0) ? 2.0 - r : r);
+    return normcdf_join_d(a0, normcdf_mag_d<P>(__builtin_fabs(a0)));
   }
 }
 

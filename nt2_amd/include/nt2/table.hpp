@@ -203,7 +203,7 @@ NT2H_TAG(max_, same2) NT2H_TAG(min_, same2) NT2H_TAG(adds_, same2) NT2H_TAG(subs
 NT2H_TAG(lt_, logical2) NT2H_TAG(gt_, logical2) NT2H_TAG(le_, logical2) NT2H_TAG(ge_, logical2)
 NT2H_TAG(eq_, logical2) NT2H_TAG(neq_, logical2) NT2H_TAG(land_, logical2) NT2H_TAG(lor_, logical2)
 NT2H_TAG(neg_, same1) NT2H_TAG(exp_, same1) NT2H_TAG(log_, same1) NT2H_TAG(sqrt_, same1)
-NT2H_TAG(erfc_, same1) NT2H_TAG(normcdf_, same1) NT2H_TAG(fastnormcdf_, same1) NT2H_TAG(cos_, same1)
+NT2H_TAG(erfc_, same1) NT2H_TAG(normcdf_mag_, same1) NT2H_TAG(normcdf_join_, same2) NT2H_TAG(fastnormcdf_, same1) NT2H_TAG(cos_, same1)
 NT2H_TAG(sin_, same1) NT2H_TAG(abs_, same1) NT2H_TAG(sqr_, same1) NT2H_TAG(rec_, same1)
 NT2H_TAG(exp2_, same1) NT2H_TAG(exp10_, same1) NT2H_TAG(log2_, same1) NT2H_TAG(log10_, same1)
 NT2H_TAG(floor_, same1) NT2H_TAG(ceil_, same1) NT2H_TAG(trunc_, same1) NT2H_TAG(round_, same1)
@@ -219,6 +219,32 @@ NT2H_TAG(fma_, same3) NT2H_TAG(fnms_, same3) NT2H_TAG(if_else_, sel3)
 NT2H_TAG(selinc_, selinc2) NT2H_TAG(seldec_, selinc2) NT2H_TAG(if_zero_else_one_, izeo1)
 #undef NT2H_TAG
 }  // namespace tagfn
+
+namespace expr {
+// A rough VALU cost of a node's own operation (arithmetic 1, a division or
+// square root 4, a transcendental function 20)
+template <class Tag> struct tag_cost {
+  static constexpr int value =
+      std::is_same<Tag, tagfn::sqrt_>::value || std::is_same<Tag, tagfn::rec_>::value ||
+              std::is_same<Tag, tagfn::divides_>::value
+          ? 4
+          : (std::is_same<Tag, tagfn::neg_>::value || std::is_same<Tag, tagfn::abs_>::value ||
+                     std::is_same<Tag, tagfn::sqr_>::value || std::is_same<Tag, tagfn::sign_>::value ||
+                     std::is_same<Tag, tagfn::floor_>::value || std::is_same<Tag, tagfn::ceil_>::value ||
+                     std::is_same<Tag, tagfn::trunc_>::value || std::is_same<Tag, tagfn::round_>::value ||
+                     std::is_same<Tag, tagfn::round2even_>::value ||
+                     std::is_same<Tag, tagfn::if_zero_else_one_>::value || std::is_same<Tag, tagfn::plus_>::value ||
+                     std::is_same<Tag, tagfn::minus_>::value || std::is_same<Tag, tagfn::multiplies_>::value ||
+                     std::is_same<Tag, tagfn::max_>::value || std::is_same<Tag, tagfn::min_>::value ||
+                     std::is_same<Tag, tagfn::adds_>::value || std::is_same<Tag, tagfn::subs_>::value ||
+                     std::is_base_of<tagfn::logical2, Tag>::value || std::is_same<Tag, tagfn::fma_>::value ||
+                     std::is_same<Tag, tagfn::fnms_>::value || std::is_same<Tag, tagfn::if_else_>::value ||
+                     std::is_same<Tag, tagfn::selinc_>::value || std::is_same<Tag, tagfn::seldec_>::value ||
+                     std::is_same<Tag, tagfn::normcdf_join_>::value
+                 ? 1
+                 : 20);
+};
+}  // namespace expr
 
 // ---- device evaluation of packs -------------------------------------------------
 namespace hip { namespace fe {
@@ -284,8 +310,8 @@ NT2H_UN(exp_, dm::exp_f<P>(x), dm::exp_d<P>(x), no_integer_form<T>())
 NT2H_UN(log_, dm::log_f<P>(x), dm::log_d<P>(x), no_integer_form<T>())
 NT2H_UN(sqrt_, __builtin_sqrtf(x), __builtin_sqrt(x), no_integer_form<T>())
 NT2H_UN(erfc_, dm::erfc_f<P>(x), (G == 1 ? dm::erfc_d<P, true>(x) : dm::erfc_d<P, false>(x)), no_integer_form<T>())
-NT2H_UN(normcdf_, dm::normcdf_f<P>(x), (G == 1 ? dm::normcdf_d<P, true>(x) : dm::normcdf_d<P, false>(x)),
-        no_integer_form<T>())
+// normcdf(x) is built as normcdf_join(x, normcdf_mag(|x|)) (the builder below)
+NT2H_UN(normcdf_mag_, dm::normcdf_mag_f<P>(x), dm::normcdf_mag_d<P>(x), no_integer_form<T>())
 NT2H_UN(fastnormcdf_, dm::fastnormcdf_f<P>(x), (double)dm::fastnormcdf_f<P>((float)x), no_integer_form<T>())
 NT2H_UN(abs_, __builtin_fabsf(x), __builtin_fabs(x), x)
 NT2H_UN(sqr_, x * x, x * x, x * x)
@@ -743,7 +769,7 @@ template <class A, NT2H_ENABLE1(A)> auto operator-(const A& a) { return make1<ta
 #define NT2H_FN1(name, Tag) \
   template <class A, NT2H_ENABLE1(A)> auto name(const A& a) { return make1<tagfn::Tag>(a); }
 NT2H_FN1(exp, exp_) NT2H_FN1(log, log_) NT2H_FN1(sqrt, sqrt_) NT2H_FN1(erfc, erfc_)
-NT2H_FN1(normcdf, normcdf_) NT2H_FN1(fastnormcdf, fastnormcdf_) NT2H_FN1(cos, cos_)
+NT2H_FN1(fastnormcdf, fastnormcdf_) NT2H_FN1(cos, cos_)
 NT2H_FN1(sin, sin_) NT2H_FN1(abs, abs_) NT2H_FN1(sqr, sqr_) NT2H_FN1(rec, rec_)
 NT2H_FN1(exp2, exp2_) NT2H_FN1(exp10, exp10_) NT2H_FN1(log2, log2_) NT2H_FN1(log10, log10_)
 NT2H_FN1(floor, floor_) NT2H_FN1(ceil, ceil_) NT2H_FN1(trunc, trunc_) NT2H_FN1(round, round_)
@@ -776,6 +802,26 @@ template <class A, class B, NT2H_ENABLE2(A, B)> auto pow(const A& a, const B& b)
   } else {
     return make2<tagfn::pow_>(a, b);
   }
+}
+// normcdf(a) = Half*erfc(-Sqrt_2o_2*a)
+// (modules/core/statistics/include/nt2/statistics/functions/generic/normcdf.hpp:33),
+// built as normcdf_join(a, normcdf_mag(abs(b))) with b = a stripped of a
+// negation: |-b| = |b| bit for bit, so normcdf(b) and normcdf(-b) in one
+// statement hold the same normcdf_mag node, which is evaluated once.
+namespace expr {
+template <class X> struct strip_neg {
+  typedef X type;
+  static const X& get(const X& x) { return x; }
+};
+template <class A> struct strip_neg<node1<tagfn::neg_, A>> {
+  typedef A type;
+  static const A& get(const node1<tagfn::neg_, A>& x) { return x.a; }
+};
+}  // namespace expr
+template <class A, NT2H_ENABLE1(A)> auto normcdf(const A& a) {
+  typedef typename expr::wrap<A>::type EA;
+  const EA x = expr::wrap<A>::get(a);
+  return make2<tagfn::normcdf_join_>(x, make1<tagfn::normcdf_mag_>(make1<tagfn::abs_>(expr::strip_neg<EA>::get(x))));
 }
 // fma(a,b,c) = a*b+c and fnms(a,b,c) = c-a*b, NT2's two-rounding forms
 template <class A, class B, class C,
@@ -853,16 +899,20 @@ template <class Tag, class VA, class VB, class VC> struct vview<n3v<Tag, VA, VB,
 };
 
 // ---- common subexpressions ----------------------------------------------------------
-// A statement evaluates each node view once per element group even when the
-// user wrote it several times (the textbook Black-Scholes tie statement holds
-// d1, sqrt(T), log(S/X) and exp(-rT) two to six times each; NT2 evaluates
-// every occurrence).  Views are numbered in pre-order; an occurrence of a node
-// type whose view equals (same tables, same scalars, bit for bit) that of an
-// earlier occurrence of the same type reuses its value: the test is on kernel
-// arguments, uniform over the wave, and the values are those a second
-// evaluation would give, bit for bit.  The earliest member of each class of
-// equal views is always evaluated (a later member can only be skipped inside a
-// subtree that an earlier equal subtree contains), so the reused value exists.
+// A statement evaluates each costly node view (a division, square root or
+// function) once per element group even when the user wrote it several times
+// (the textbook Black-Scholes tie statement holds d1, sqrt(T), log(S/X) and
+// exp(-rT) two to six times each; NT2 evaluates every occurrence).  Views are
+// numbered in pre-order.  Before a launch the host tests whether every
+// occurrence of each costly node type has the view of that type's first
+// occurrence (same tables, same scalars, bit for bit).  If so, the kernel
+// instantiated for that case takes every later occurrence's value from the
+// first one, a choice made at compile time: the compiler keeps one live value
+// per repeated node and no test runs in the kernel.  Otherwise every
+// occurrence is evaluated, as NT2 does (tests in the kernel would keep every
+// candidate value and every occurrence's operands live, and their registers
+// cost more than the evaluations they save).  Either way the values are those
+// NT2's evaluation gives, bit for bit.
 template <class A, class B> struct vpair { A a; B b; };  // the two right-hand sides of a tie
 
 template <class V> struct vsize { static constexpr int value = 1; };
@@ -920,10 +970,10 @@ constexpr int occurrences() {
 }
 
 // equality of two views of one type: the same tables and scalars
-template <class T> __device__ __forceinline__ bool veq(const expr::term_v<T>& x, const expr::term_v<T>& y) {
+template <class T> inline bool veq(const expr::term_v<T>& x, const expr::term_v<T>& y) {
   return x.p == y.p;
 }
-template <class T> __device__ __forceinline__ bool veq(const expr::scal_v<T>& x, const expr::scal_v<T>& y) {
+template <class T> inline bool veq(const expr::scal_v<T>& x, const expr::scal_v<T>& y) {
   if constexpr (std::is_floating_point<T>::value) {
     typedef typename std::conditional<sizeof(T) == 4, uint32_t, uint64_t>::type U;
     return __builtin_bit_cast(U, x.v) == __builtin_bit_cast(U, y.v);
@@ -931,18 +981,62 @@ template <class T> __device__ __forceinline__ bool veq(const expr::scal_v<T>& x,
     return x.v == y.v;
   }
 }
-template <class Tag, class VA> __device__ __forceinline__ bool veq(const n1v<Tag, VA>& x, const n1v<Tag, VA>& y) {
+template <class Tag, class VA> inline bool veq(const n1v<Tag, VA>& x, const n1v<Tag, VA>& y) {
   return veq(x.a, y.a);
 }
 template <class Tag, class VA, class VB>
-__device__ __forceinline__ bool veq(const n2v<Tag, VA, VB>& x, const n2v<Tag, VA, VB>& y) {
+inline bool veq(const n2v<Tag, VA, VB>& x, const n2v<Tag, VA, VB>& y) {
   return veq(x.a, y.a) && veq(x.b, y.b);
 }
 template <class Tag, class VA, class VB, class VC>
-__device__ __forceinline__ bool veq(const n3v<Tag, VA, VB, VC>& x, const n3v<Tag, VA, VB, VC>& y) {
+inline bool veq(const n3v<Tag, VA, VB, VC>& x, const n3v<Tag, VA, VB, VC>& y) {
   return veq(x.a, y.a) && veq(x.b, y.b) && veq(x.c, y.c);
 }
-template <class X> __device__ __forceinline__ bool veq(const X&, const X&) { return false; }  // other views: never shared
+template <class X> inline bool veq(const X&, const X&) { return false; }  // other views: never shared
+
+// pre-order index of the first occurrence of type X in root R (from K)
+template <class R, class X, int K = 0>
+constexpr int first_of() {
+  if constexpr (K >= vsize<R>::value) return -1;
+  else if constexpr (std::is_same<typename vat_t<R, K>::type, X>::value) return K;
+  else return first_of<R, X, K + 1>();
+}
+// node types worth sharing: a division, square root or function at the top
+// (an arithmetic node is evaluated at every occurrence, from shared operands)
+template <class V> struct shareable { static constexpr bool value = false; };
+template <class Tag, class VA> struct shareable<n1v<Tag, VA>> {
+  static constexpr bool value = expr::tag_cost<Tag>::value > 1;
+};
+template <class Tag, class VA, class VB> struct shareable<n2v<Tag, VA, VB>> {
+  static constexpr bool value = expr::tag_cost<Tag>::value > 1;
+};
+template <class Tag, class VA, class VB, class VC> struct shareable<n3v<Tag, VA, VB, VC>> {
+  static constexpr bool value = expr::tag_cost<Tag>::value > 1;
+};
+// some shareable node type occurs more than once in R
+template <class R, int K = 0>
+constexpr bool has_repeats() {
+  if constexpr (K >= vsize<R>::value) return false;
+  else {
+    typedef typename vat_t<R, K>::type X;
+    if constexpr (shareable<X>::value && first_of<R, X>() < K) return true;
+    else return has_repeats<R, K + 1>();
+  }
+}
+// the host test before a launch: every occurrence of each shareable node type
+// has the view of that type's first occurrence
+template <class R, int K = 0>
+bool shared_by_type(const R& root) {
+  if constexpr (K >= vsize<R>::value) {
+    return true;
+  } else {
+    typedef typename vat_t<R, K>::type X;
+    if constexpr (shareable<X>::value && first_of<R, X>() < K) {
+      if (!veq(vat<K>(root), vat<first_of<R, X>()>(root))) return false;
+    }
+    return shared_by_type<R, K + 1>(root);
+  }
+}
 
 // per-index value slots of one element group (only node indices hold values)
 template <class R, int G, int K, bool N = is_node<typename vat_t<R, K>::type>::value> struct slot_of {
@@ -955,9 +1049,9 @@ template <class R, int G, class Seq> struct slots;
 template <class R, int G, int... Ps> struct slots<R, G, std::integer_sequence<int, Ps...>> {
   typedef std::tuple<typename slot_of<R, G, Ps>::type...> type;
 };
-template <class R, int G> struct cse_ctx {
+template <class R, int G, bool TS> struct cse_ctx {
   typedef R root_t;
-  const R& root;
+  static constexpr bool by_type = TS;  // shared_by_type holds for the launch
   typename slots<R, G, std::make_integer_sequence<int, vsize<R>::value>>::type s;
 };
 
@@ -1012,6 +1106,15 @@ __device__ __forceinline__ void evn(C& c, const n2v<Tag, VA, VB>& v, size_t i, R
       else
         r[l] = G == 1 ? dm::pow_d<P, true>((double)a[l], (double)b[l]) : dm::pow_d<P, false>((double)a[l], (double)b[l]);
     }
+  } else if constexpr (std::is_same<Tag, tagfn::normcdf_join_>::value) {
+    // the f64 scalar form (a pack's tail) is erfc's own, which ignores b
+#pragma unroll
+    for (int l = 0; l < G; ++l) {
+      if constexpr (std::is_same<R, float>::value) r[l] = dm::normcdf_join_f((float)a[l], (float)b[l]);
+      else if constexpr (std::is_same<R, double>::value)
+        r[l] = G == 1 ? dm::normcdf_d<P, true>((double)a[l]) : dm::normcdf_join_d((double)a[l], (double)b[l]);
+      else r[l] = no_integer_form<R>();
+    }
   } else if constexpr (std::is_same<Tag, tagfn::atan2_>::value) {
     static_assert(std::is_floating_point<R>::value, "atan2 is defined for floating-point tables only");
 #pragma unroll
@@ -1047,37 +1150,23 @@ __device__ __forceinline__ void evn(C& c, const n3v<Tag, VA, VB, VC>& v, size_t 
   }
 }
 
-// occurrence K of node type V: the value of the first earlier occurrence Q
-// with an equal view, else evaluated
-template <int G, int A, int K, int Q, class C, class V, class R>
-__device__ __forceinline__ void reuse_or_eval(C& c, const V& v, size_t i, R (&r)[G]) {
-  typedef typename C::root_t Root;
-  if constexpr (Q >= K) {
-    evn<G, A, K>(c, v, i, r);
-  } else if constexpr (std::is_same<typename vat_t<Root, Q>::type, V>::value) {
-    if (veq(v, vat<Q>(c.root))) {
-      const auto& q = std::get<Q>(c.s);
-#pragma unroll
-      for (int l = 0; l < G; ++l) r[l] = q.v[l];
-    } else {
-      reuse_or_eval<G, A, K, Q + 1>(c, v, i, r);
-    }
-  } else {
-    reuse_or_eval<G, A, K, Q + 1>(c, v, i, r);
-  }
-}
-
 template <int G, int A, int K, class C, class V, class R>
 __device__ __forceinline__ void evm(C& c, const V& v, size_t i, R (&r)[G]) {
   if constexpr (!is_node<V>::value) {
     ev<G, A>(v, i, r);
   } else {
     typedef typename C::root_t Root;
-    constexpr int before = occurrences<Root, V, 0, K>();
-    constexpr int after = occurrences<Root, V, K + 1, vsize<Root>::value>();
-    if constexpr (before > 0) reuse_or_eval<G, A, K, 0>(c, v, i, r);
-    else evn<G, A, K>(c, v, i, r);
-    if constexpr (after > 0) {
+    constexpr bool sh = C::by_type && shareable<V>::value;
+    constexpr int before = sh ? occurrences<Root, V, 0, K>() : 0;
+    constexpr int after = sh ? occurrences<Root, V, K + 1, vsize<Root>::value>() : 0;
+    if constexpr (before > 0) {
+      const auto& q = std::get<first_of<Root, V>()>(c.s);
+#pragma unroll
+      for (int l = 0; l < G; ++l) r[l] = q.v[l];
+    } else {
+      evn<G, A, K>(c, v, i, r);
+    }
+    if constexpr (before == 0 && after > 0) {
       auto& q = std::get<K>(c.s);
 #pragma unroll
       for (int l = 0; l < G; ++l) q.v[l] = r[l];
@@ -1086,26 +1175,36 @@ __device__ __forceinline__ void evm(C& c, const V& v, size_t i, R (&r)[G]) {
 }
 
 // a node view evaluated with its common subexpressions shared
+// a node view evaluated on its own (nested evaluations: expand_to's operand)
 template <int G, int A = 0, class Tag, class VA, class R>
 __device__ __forceinline__ void ev(const n1v<Tag, VA>& v, size_t i, R (&r)[G]) {
-  cse_ctx<n1v<Tag, VA>, G> c{v, {}};
+  cse_ctx<n1v<Tag, VA>, G, false> c{};
   evm<G, A, 0>(c, v, i, r);
 }
 template <int G, int A = 0, class Tag, class VA, class VB, class R>
 __device__ __forceinline__ void ev(const n2v<Tag, VA, VB>& v, size_t i, R (&r)[G]) {
-  cse_ctx<n2v<Tag, VA, VB>, G> c{v, {}};
+  cse_ctx<n2v<Tag, VA, VB>, G, false> c{};
   evm<G, A, 0>(c, v, i, r);
 }
 template <int G, int A = 0, class Tag, class VA, class VB, class VC, class R>
 __device__ __forceinline__ void ev(const n3v<Tag, VA, VB, VC>& v, size_t i, R (&r)[G]) {
-  cse_ctx<n3v<Tag, VA, VB, VC>, G> c{v, {}};
+  cse_ctx<n3v<Tag, VA, VB, VC>, G, false> c{};
   evm<G, A, 0>(c, v, i, r);
 }
+// a statement's root, its repeats shared when TS (shared_by_type held)
+template <int G, int A, bool TS, class V, class R>
+__device__ __forceinline__ void ev_root(const V& v, size_t i, R (&r)[G]) {
+  if constexpr (is_node<V>::value) {
+    cse_ctx<V, G, TS> c{};
+    evm<G, A, 0>(c, v, i, r);
+  } else {
+    ev<G, A>(v, i, r);
+  }
+}
 // the two right-hand sides of a tie, sharing their common subexpressions
-template <int G, int A = 0, class V1, class V2, class T>
-__device__ __forceinline__ void ev2(const V1& v1, const V2& v2, size_t i, T (&r1)[G], T (&r2)[G]) {
-  const vpair<V1, V2> root{v1, v2};
-  cse_ctx<vpair<V1, V2>, G> c{root, {}};
+template <int G, int A, bool TS, class V1, class V2, class T>
+__device__ __forceinline__ void ev2(const vpair<V1, V2>& root, size_t i, T (&r1)[G], T (&r2)[G]) {
+  cse_ctx<vpair<V1, V2>, G, TS> c{};
   evm<G, A, 1>(c, root.a, i, r1);
   evm<G, A, 1 + vsize<V1>::value>(c, root.b, i, r2);
 }
@@ -1145,7 +1244,7 @@ template <> struct sub_width<double> { static constexpr int value = 2; };
 // U work items per thread and iteration are evaluated before any of them is
 // stored (an in-place statement reads only its own indices, so the loads of
 // later items may go ahead of the earlier items' stores)
-template <class T, class V, int A, int SG, int U>
+template <class T, class V, int A, int SG, int U, bool TS>
 __global__ __launch_bounds__(256) void transform_k(T* __restrict__ out, V v, size_t n) {
   constexpr int W = pack_width<T>::value;
   static_assert(W % SG == 0, "a sub-pack divides the pack");
@@ -1157,7 +1256,7 @@ __global__ __launch_bounds__(256) void transform_k(T* __restrict__ out, V v, siz
     T r[U][SG];
 #pragma unroll
     for (int u = 0; u < U; ++u)
-      if (t0 + u * stride < nsub) ev<SG, A>(v, (t0 + u * stride) * SG, r[u]);
+      if (t0 + u * stride < nsub) ev_root<SG, A, TS>(v, (t0 + u * stride) * SG, r[u]);
 #pragma unroll
     for (int u = 0; u < U; ++u)
       if (t0 + u * stride < nsub) store_pack<A>(out, (t0 + u * stride) * SG, r[u]);
@@ -1165,16 +1264,16 @@ __global__ __launch_bounds__(256) void transform_k(T* __restrict__ out, V v, siz
   for (size_t t = g; t < n - npacks * W; t += stride) {  // the scalar tail
     const size_t i = npacks * W + t;
     T r[1];
-    ev<1>(v, i, r);
+    ev_root<1, 0, TS>(v, i, r);
     out[i] = r[0];
   }
 }
 
 // tie: every right-hand side of an index is evaluated before any output of
 // that index is written
-template <class T, class V1, class V2, int A, int SG>
-__global__ __launch_bounds__(256) void transform2_k(T* __restrict__ o1, T* __restrict__ o2, V1 v1,
-                                                    V2 v2, size_t n) {
+template <class T, class V1, class V2, int A, int SG, bool TS>
+__global__ __launch_bounds__(256) void transform2_k(T* __restrict__ o1, T* __restrict__ o2, vpair<V1, V2> v,
+                                                    size_t n) {
   constexpr int W = pack_width<T>::value;
   static_assert(W % SG == 0, "a sub-pack divides the pack");
   const size_t npacks = n / W;
@@ -1184,13 +1283,13 @@ __global__ __launch_bounds__(256) void transform2_k(T* __restrict__ o1, T* __res
   for (size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x; t < items; t += stride) {
     if (t < nsub) {
       T r1[SG], r2[SG];
-      ev2<SG, A>(v1, v2, t * SG, r1, r2);
+      ev2<SG, A, TS>(v, t * SG, r1, r2);
       store_pack<A>(o1, t * SG, r1);
       store_pack<A>(o2, t * SG, r2);
     } else {
       const size_t i = npacks * W + (t - nsub);
       T r1[1], r2[1];
-      ev2<1>(v1, v2, i, r1, r2);
+      ev2<1, 0, TS>(v, i, r1, r2);
       o1[i] = r1[0];
       o2[i] = r2[0];
     }
@@ -1219,7 +1318,7 @@ __device__ __forceinline__ double block_sum256(double d) {
   return ((w[0] + w[1]) + w[2]) + w[3];
 }
 
-template <class T, class V, int A, int SG, int U>
+template <class T, class V, int A, int SG, int U, bool TS>
 __global__ __launch_bounds__(256) void fold_k(V v, size_t n, double* __restrict__ partial) {
   constexpr int W = pack_width<T>::value;
   static_assert(W % SG == 0, "a sub-pack divides the pack");
@@ -1232,7 +1331,7 @@ __global__ __launch_bounds__(256) void fold_k(V v, size_t n, double* __restrict_
     T r[U][SG];
 #pragma unroll
     for (int u = 0; u < U; ++u)
-      if (t0 + u * stride < nsub) ev<SG, A>(v, (t0 + u * stride) * SG, r[u]);
+      if (t0 + u * stride < nsub) ev_root<SG, A, TS>(v, (t0 + u * stride) * SG, r[u]);
 #pragma unroll
     for (int u = 0; u < U; ++u)
       if (t0 + u * stride < nsub)
@@ -1241,7 +1340,7 @@ __global__ __launch_bounds__(256) void fold_k(V v, size_t n, double* __restrict_
   }
   for (size_t t = g; t < n - npacks * W; t += stride) {  // the scalar tail
     T r[1];
-    ev<1>(v, npacks * W + t, r);
+    ev_root<1, 0, TS>(v, npacks * W + t, r);
     acc_add(s, c, r[0]);
   }
   const double b = block_sum256((double)s + (double)c);
@@ -1266,7 +1365,7 @@ __device__ __forceinline__ uint64_t block_sum256_u64(uint64_t d) {
   __syncthreads();
   return ((w[0] + w[1]) + w[2]) + w[3];
 }
-template <class T, class V, int A, int SG>
+template <class T, class V, int A, int SG, bool TS>
 __global__ __launch_bounds__(256) void fold_int_k(V v, size_t n, uint64_t* __restrict__ partial) {
   constexpr int W = pack_width<T>::value;
   const size_t npacks = n / W;
@@ -1276,13 +1375,13 @@ __global__ __launch_bounds__(256) void fold_int_k(V v, size_t n, uint64_t* __res
   uint64_t s = 0;
   for (size_t t = g; t < nsub; t += stride) {
     T r[SG];
-    ev<SG, A>(v, t * SG, r);
+    ev_root<SG, A, TS>(v, t * SG, r);
 #pragma unroll
     for (int l = 0; l < SG; ++l) s += (uint64_t)(long long)r[l];
   }
   for (size_t t = g; t < n - npacks * W; t += stride) {
     T r[1];
-    ev<1>(v, npacks * W + t, r);
+    ev_root<1, 0, TS>(v, npacks * W + t, r);
     s += (uint64_t)(long long)r[0];
   }
   const uint64_t b = block_sum256_u64(s);
@@ -1302,6 +1401,25 @@ inline unsigned grid(size_t items) {
   size_t g = (items + 255) / 256;
   if (g > 8192) g = 8192;
   return (unsigned)(g ? g : 1);
+}
+
+// f(integral_constant<int, A>, bool_constant<TS>) with A = AL when al holds,
+// else 0, and TS = shared_by_type(root); the TS kernel is instantiated only
+// for statements that repeat a shareable node type
+template <int AL, class R, class F>
+void dispatch(bool al, const R& root, F&& f) {
+  typedef std::integral_constant<int, AL> a1;
+  typedef std::integral_constant<int, 0> a0;
+  if constexpr (has_repeats<R>()) {
+    const bool ts = shared_by_type(root);
+    if (al && ts) f(a1{}, std::true_type{});
+    else if (al) f(a1{}, std::false_type{});
+    else if (ts) f(a0{}, std::true_type{});
+    else f(a0{}, std::false_type{});
+  } else {
+    if (al) f(a1{}, std::false_type{});
+    else f(a0{}, std::false_type{});
+  }
 }
 
 inline unsigned grid_full(size_t items) {
@@ -1363,47 +1481,29 @@ template <class Tag, class A> struct pack_free<node1<Tag, A>> {
                                 !std::is_same<Tag, tagfn::csc_>::value &&
                                 !std::is_same<Tag, tagfn::atan_>::value && !std::is_same<Tag, tagfn::asinh_>::value &&
                                 !std::is_same<Tag, tagfn::acosh_>::value &&
-                                !((std::is_same<Tag, tagfn::erfc_>::value || std::is_same<Tag, tagfn::normcdf_>::value ||
+                                !((std::is_same<Tag, tagfn::erfc_>::value ||
                                    std::is_same<Tag, tagfn::asin_>::value || std::is_same<Tag, tagfn::acos_>::value ||
                                    std::is_same<Tag, tagfn::erf_>::value) &&
                                   std::is_same<typename A::value_type, double>::value);
 };
 template <class Tag, class A, class B> struct pack_free<node2<Tag, A, B>> {
   static constexpr bool value = pack_free<A>::value && pack_free<B>::value && !std::is_same<Tag, tagfn::atan2_>::value &&
-                                !std::is_same<Tag, tagfn::pow_>::value;
+                                !std::is_same<Tag, tagfn::pow_>::value &&
+                                !(std::is_same<Tag, tagfn::normcdf_join_>::value &&
+                                  std::is_same<typename A::value_type, double>::value);
 };
 template <class Tag, class A, class B, class C> struct pack_free<node3<Tag, A, B, C>> {
   static constexpr bool value = pack_free<A>::value && pack_free<B>::value && pack_free<C>::value;
 };
 
-// A rough VALU cost of one element (arithmetic 1, a division or square root
-// 4, a transcendental function 20), which picks the lowering's work shape:
+// A rough VALU cost of one element (the sum of its nodes' tag_cost), which
+// picks the lowering's work shape:
 // cheap expressions are memory bound and get several packs in flight per
 // thread; costly ones get fewer elements per thread.  Views other than nodes,
 // terminals and scalars count as costly.
 template <class X> struct cost { static constexpr int value = 20; };
 template <class T> struct cost<terminal<T>> { static constexpr int value = 0; };
 template <class T> struct cost<scalar<T>> { static constexpr int value = 0; };
-template <class Tag> struct tag_cost {
-  static constexpr int value =
-      std::is_same<Tag, tagfn::sqrt_>::value || std::is_same<Tag, tagfn::rec_>::value ||
-              std::is_same<Tag, tagfn::divides_>::value
-          ? 4
-          : (std::is_same<Tag, tagfn::neg_>::value || std::is_same<Tag, tagfn::abs_>::value ||
-                     std::is_same<Tag, tagfn::sqr_>::value || std::is_same<Tag, tagfn::sign_>::value ||
-                     std::is_same<Tag, tagfn::floor_>::value || std::is_same<Tag, tagfn::ceil_>::value ||
-                     std::is_same<Tag, tagfn::trunc_>::value || std::is_same<Tag, tagfn::round_>::value ||
-                     std::is_same<Tag, tagfn::round2even_>::value ||
-                     std::is_same<Tag, tagfn::if_zero_else_one_>::value || std::is_same<Tag, tagfn::plus_>::value ||
-                     std::is_same<Tag, tagfn::minus_>::value || std::is_same<Tag, tagfn::multiplies_>::value ||
-                     std::is_same<Tag, tagfn::max_>::value || std::is_same<Tag, tagfn::min_>::value ||
-                     std::is_same<Tag, tagfn::adds_>::value || std::is_same<Tag, tagfn::subs_>::value ||
-                     std::is_base_of<tagfn::logical2, Tag>::value || std::is_same<Tag, tagfn::fma_>::value ||
-                     std::is_same<Tag, tagfn::fnms_>::value || std::is_same<Tag, tagfn::if_else_>::value ||
-                     std::is_same<Tag, tagfn::selinc_>::value || std::is_same<Tag, tagfn::seldec_>::value
-                 ? 1
-                 : 20);
-};
 template <class Tag, class A> struct cost<node1<Tag, A>> { static constexpr int value = cost<A>::value + tag_cost<Tag>::value; };
 template <class Tag, class A, class B> struct cost<node2<Tag, A, B>> {
   static constexpr int value = cost<A>::value + cost<B>::value + tag_cost<Tag>::value;
@@ -1441,10 +1541,11 @@ void transform(T* out, const EX& x, size_t n, hipStream_t s = nullptr) {
   const size_t items = (n / sh::SG + sh::U - 1) / sh::U;
   const dim3 g(sh::full_grid ? fe::grid_full(items) : fe::grid(items)), b(256);
   constexpr int AL = sh::cheap ? 2 : 1;
-  if (fe::aligned16(v) && ((uintptr_t)out & 15u) == 0)
-    hipLaunchKernelGGL((fe::transform_k<T, decltype(v), AL, sh::SG, sh::U>), g, b, 0, s, out, v, n);
-  else
-    hipLaunchKernelGGL((fe::transform_k<T, decltype(v), 0, sh::SG, sh::U>), g, b, 0, s, out, v, n);
+  typedef decltype(v) V;
+  fe::dispatch<AL>(fe::aligned16(v) && ((uintptr_t)out & 15u) == 0, v, [&](auto a, auto ts) {
+    hipLaunchKernelGGL((fe::transform_k<T, V, decltype(a)::value, sh::SG, sh::U, decltype(ts)::value>), g, b, 0, s,
+                       out, v, n);
+  });
   fe::check(hipGetLastError(), "transform launch");
 }
 
@@ -1457,7 +1558,11 @@ T fold_int(const EX& x, size_t n, hipStream_t s = nullptr) {
   const unsigned nb = (unsigned)std::min<size_t>(1024, (items + 255) / 256 ? (items + 255) / 256 : 1);
   uint64_t* buf = reinterpret_cast<uint64_t*>(fe::fold_scratch(nb + 1));  // 8-byte slots either way
   auto v = expr::dview<EX>::get(x);
-  hipLaunchKernelGGL((fe::fold_int_k<T, decltype(v), 0, sh::SG>), dim3(nb), dim3(256), 0, s, v, n, buf);
+  typedef decltype(v) V;
+  fe::dispatch<0>(false, v, [&](auto a, auto ts) {
+    hipLaunchKernelGGL((fe::fold_int_k<T, V, decltype(a)::value, sh::SG, decltype(ts)::value>), dim3(nb), dim3(256),
+                       0, s, v, n, buf);
+  });
   fe::check(hipGetLastError(), "fold launch");
   hipLaunchKernelGGL(fe::fold_final_int_k, dim3(1), dim3(256), 0, s, buf, (int)nb, buf + nb);
   double* hr = fe::fold_result_host();
@@ -1478,10 +1583,11 @@ double fold(const EX& x, size_t n, hipStream_t s = nullptr) {
   double* buf = fe::fold_scratch(nb + 1);
   auto v = expr::dview<EX>::get(x);
   constexpr int AL = sh::cheap ? 2 : 1;
-  if (fe::aligned16(v))
-    hipLaunchKernelGGL((fe::fold_k<T, decltype(v), AL, sh::SG, sh::U>), dim3(nb), dim3(256), 0, s, v, n, buf);
-  else
-    hipLaunchKernelGGL((fe::fold_k<T, decltype(v), 0, sh::SG, sh::U>), dim3(nb), dim3(256), 0, s, v, n, buf);
+  typedef decltype(v) V;
+  fe::dispatch<AL>(fe::aligned16(v), v, [&](auto a, auto ts) {
+    hipLaunchKernelGGL((fe::fold_k<T, V, decltype(a)::value, sh::SG, sh::U, decltype(ts)::value>), dim3(nb),
+                       dim3(256), 0, s, v, n, buf);
+  });
   fe::check(hipGetLastError(), "fold launch");
   hipLaunchKernelGGL(fe::fold_final_k, dim3(1), dim3(256), 0, s, buf, (int)nb, buf + nb);
   double* hr = fe::fold_result_host();
@@ -1670,10 +1776,15 @@ template <class T> struct tie2_lhs {
     auto v2 = expr::dview<E2>::get(r.e2);
     constexpr int SG = expr::pack_free<E1>::value && expr::pack_free<E2>::value ? hip::fe::sub_width<T>::value : W;
     const dim3 g(hip::fe::grid(n / SG + n % W)), blk(256);
-    if (hip::fe::aligned16(v1) && hip::fe::aligned16(v2) && (((uintptr_t)o1 | (uintptr_t)o2) & 15u) == 0)
-      hipLaunchKernelGGL((hip::fe::transform2_k<T, decltype(v1), decltype(v2), 1, SG>), g, blk, 0, 0, o1, o2, v1, v2, n);
-    else
-      hipLaunchKernelGGL((hip::fe::transform2_k<T, decltype(v1), decltype(v2), 0, SG>), g, blk, 0, 0, o1, o2, v1, v2, n);
+    typedef decltype(v1) V1;
+    typedef decltype(v2) V2;
+    const hip::fe::vpair<V1, V2> root{v1, v2};
+    hip::fe::dispatch<1>(hip::fe::aligned16(v1) && hip::fe::aligned16(v2) && (((uintptr_t)o1 | (uintptr_t)o2) & 15u) == 0,
+                         root, [&](auto al, auto ts) {
+                           hipLaunchKernelGGL((hip::fe::transform2_k<T, V1, V2, decltype(al)::value, SG,
+                                                                     decltype(ts)::value>),
+                                              g, blk, 0, 0, o1, o2, root, n);
+                         });
     hip::fe::check(hipGetLastError(), "tie launch");
     return *this;
   }

@@ -231,6 +231,23 @@ int main(int argc, char** argv) {
   dump("cot_f32", at2);
   at2 = nt2::sinpi(at) - nt2::cospi(bt) + nt2::tanpi(at / 7.0f) * nt2::cotpi(bt / 3.0f);
   dump("sinpi_f32", at2);
+  // repeated subexpressions: shared when each repeated node type has one view,
+  // otherwise every occurrence is evaluated; the values are the same either way
+  at2 = nt2::exp(at) * nt2::exp(at) + nt2::sqrt(nt2::exp(at)) - nt2::cos(bt * 2.0f) / nt2::cos(bt * 2.0f);
+  dump("rep_same", at2);
+  at2 = nt2::exp(at) * nt2::exp(bt) + nt2::cos(bt * 2.0f) - nt2::cos(bt * 3.0f) + nt2::exp(at);
+  dump("rep_mixed", at2);
+  {
+    table<float> p1, p2;
+    nt2::tie(p1, p2) = nt2::tie(nt2::normcdf(at) + nt2::normcdf(bt), nt2::normcdf(-at) - nt2::normcdf(-bt));
+    dump("rep_tie1", p1);
+    dump("rep_tie2", p2);
+    nt2::tie(p1, p2) = nt2::tie(nt2::normcdf(at) * 2.0f, nt2::normcdf(-at));
+    dump("rep_tie3", p1);
+    dump("rep_tie4", p2);
+  }
+  dump_scalar("rep_sum_mixed", nt2::globalsum(nt2::exp(at) * nt2::exp(at) + nt2::exp(bt)));
+  dump_scalar("rep_sum_same", nt2::globalsum(nt2::exp(at) * nt2::exp(at)));
   at2 = nt2::sind(at * 100.0f) - nt2::cosd(bt * 50.0f) + nt2::tand(at * 30.0f) * nt2::cotd(bt * 40.0f);
   dump("sind_f32", at2);
   at2 = if_else((at > 0.0f) && (bt < 1.0f), at, bt) + if_else(logical_or(at < -1.0f, bt > 2.0f), 1.0f, 0.0f);

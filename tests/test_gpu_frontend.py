@@ -234,6 +234,34 @@ def test_image_demo_statements(outputs):
     assert same_bits(outputs("yuv_v", f32), v)
 
 
+def test_repeated_subexpressions(outputs):
+    """Statements that repeat subexpressions over one view (shared, decided at
+    compile time after a host test) and over different views of one node type
+    (every occurrence evaluated) give the oracle's bits either way; normcdf(x)
+    and normcdf(-x) share erfc's magnitude."""
+    f32 = np.float32
+    a, b = outputs("at_a", f32), outputs("at_b", f32)
+    ea, eb = O.map_f32("exp", a), O.map_f32("exp", b)
+    c2 = O.map_f32("cos", b * f32(2))
+    want = ea * ea + np.sqrt(ea) - c2 / c2
+    got = outputs("rep_same", f32)
+    assert same_bits(got, want), first_diff(got, want)
+    want = ea * eb + c2 - O.map_f32("cos", b * f32(3)) + ea
+    got = outputs("rep_mixed", f32)
+    assert same_bits(got, want), first_diff(got, want)
+    na, nb = O.map_f32("normcdf", a), O.map_f32("normcdf", b)
+    nma, nmb = O.map_f32("normcdf", -a), O.map_f32("normcdf", -b)
+    for name, want in (("rep_tie1", na + nb), ("rep_tie2", nma - nmb), ("rep_tie3", na * f32(2)),
+                       ("rep_tie4", nma)):
+        got = outputs(name, f32)
+        assert same_bits(got, want), (name, first_diff(got, want))
+    for name, z in (("rep_sum_mixed", ea * ea + eb), ("rep_sum_same", ea * ea)):
+        ex = O.sum_f32_exact(z)
+        mag = O.sum_f32_exact(np.abs(z))
+        r = float(outputs(name, f32)[0])
+        assert abs(r - ex) <= abs(float(np.float32(ex)) - ex) + 2 ** -40 * mag, (name, r, ex)
+
+
 def test_mandelbrot_demo_statements(outputs):
     """A = expand_to(linspace(...)), B = expand_to(colvect(linspace(...))) and
     C = arrayfun(step, A, B) with a functor deciding per pack of 8
