@@ -494,7 +494,8 @@ def leg_served(torch, ops, O):
         return st
 
     def poisson(rate, seconds):
-        count = max(100, int(rate * seconds))
+        # at most the nreq prepared requests (a shorter window at the highest rates)
+        count = min(nreq, max(100, int(rate * seconds)))
         return np.cumsum(np.random.default_rng(int(rate) % 1000 + 17).exponential(1.0 / rate, count))
 
     keep = ("requests", "p50_us", "p99_us", "mean_us", "max_us", "options_per_s", "launches", "mean_batch",

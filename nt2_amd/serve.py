@@ -107,6 +107,9 @@ class Server:
         """Replay items[:count] (a list of Request or a ctypes Request array)
         at the given arrival times (seconds from the start)."""
         n = len(items) if count is None else count
+        if n < 0 or n > len(items) or n > len(arrivals):
+            raise ValueError("run_trace: count %d exceeds the %d items or %d arrival times"
+                             % (n, len(items), len(arrivals)))
         arr = items if isinstance(items, ctypes.Array) else (Request * n)(*items)
         ts = (ctypes.c_double * n)(*[float(a) for a in arrivals[:n]])
         return self.L.nt2h_server_run_trace(self.h, n, arr, ts, timeout)

@@ -192,3 +192,17 @@ def test_trace_replay_host():
         assert srv.run_trace(items, arrivals, 60) == 0
         st = srv.stats()
         assert st["completed"] == 50 and st["used_bytes"] == 0 and st["options"] == 50 * 256
+
+
+def test_trace_replay_count_bounded():
+    """A replay never reads past the prepared requests or arrival times."""
+    tabs = [_tables(64, np.float32, k) for k in range(4)]
+    from nt2_amd.serve import Request
+    items = (Request * 4)(*[Request(F32, 64, _p(S), _p(X), _p(T), 0.02, 0.3, None, None, 0) for (S, X, T) in tabs])
+    with Server(1 << 24, max_inflight=2, backend=BACKEND_HOST) as srv:
+        with pytest.raises(ValueError):
+            srv.run_trace(items, np.zeros(8), 10, count=5)
+        with pytest.raises(ValueError):
+            srv.run_trace(items, np.zeros(3), 10, count=4)
+        assert srv.run_trace(items, np.zeros(4), 10, count=4) == 0
+        assert srv.stats()["completed"] == 4
