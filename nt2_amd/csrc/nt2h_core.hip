@@ -655,39 +655,67 @@ __device__ __forceinline__ uint32_t wave_count(uint32_t c) {
 }
 
 // 16 pixels per thread (one 128-bit load per frame), two pixels per 32-bit
-// register as 16-bit halves so that every step is one packed integer op per
-// pixel pair.  Per pair (values 0..255 in signed 16-bit lanes):
-//   b = min(b+1, max(b-1, f))            selinc(b<f, seldec(b>f, b))
-//   d = max(b,f) - min(b,f)
-//   m = min(N d, 255)                    adds applied N-1 times
-//   v = min(v+nz, max(v-nz, m)), nz = min(d,1)   (no change when d == 0)
-//   l = (d + 256 - v) >> 8               if_zero_else_one(d < v)
-// The frame loop loads 4 frames ahead.  Per-frame label counts: a wave sum
-// per frame into LDS, the workgroup's sum per frame to partials[k][block],
-// and sd_counts_k adds the partials of each frame in block order.
-typedef short sd_i16x2 __attribute__((ext_vector_type(2)));
-typedef unsigned short sd_u16x2 __attribute__((ext_vector_type(2)));
-constexpr int kSdChunk = 256;  // frames per launch (LDS holds their wave counts)
+// register as the halves of a packed f16 pair, so that every step is one
+// packed op per pixel pair.  Values 0..255 and every intermediate below are
+// integers of magnitude < 2048 (exact in f16), except N d - 1, which only
+// feeds a maximum whose result is then capped at 254 (above 2048 it rounds to
+// a value >= 2048); f16 gives the absolute value (a sign mask) and the
+// [0, 1] clamp as free modifiers, and gfx950 has three-input packed
+// minimum/maximum.  Per pair, with B = b + 1024 (the f16 whose low byte is b:
+// unpacking and packing are byte permutes) and W = v - 1:
+//   B = B + clamp(F - B) - clamp(B - F)  selinc(b<f, seldec(b>f, b))
+//   d = |F - B|                          max(b,f) - min(b,f)
+//   W = min(W + nz, 254, max(W - nz, N d - 1)), nz = min(d, 1)
+//                                        v toward m = adds(...adds(d,d)...),
+//                                        no change when d == 0
+//   l = clamp(d - W) = (d >= v)          if_zero_else_one(d < v)
+// With W in [-1, 254] the cap at 254 on W + nz gives what capping m at 255
+// gives.  The frame loop keeps 4 frames in flight (the load of frame k + 4 is
+// issued as frame k is consumed).  Per-frame label counts: each lane's count
+// (at most 16) goes to one byte of a word per 4 frames in LDS; at the end the
+// workgroup sums them per frame into partials[k][block], and sd_counts_k
+// adds the partials of each frame in block order.
+typedef _Float16 sd_h2 __attribute__((ext_vector_type(2)));
+constexpr int kSdChunk = 64;   // frames per launch (LDS holds their per-lane counts)
+constexpr int kSdAhead = 4;    // frames in flight per thread
 
-__device__ __forceinline__ sd_i16x2 sd_lo(uint32_t w) {  // bytes 0,1 -> 16-bit lanes
-  return __builtin_bit_cast(sd_i16x2, __builtin_amdgcn_perm(0u, w, 0x0c010c00u));
+__device__ __forceinline__ sd_h2 sd_lo(uint32_t w) {  // bytes 0,1 -> 1024 + byte
+  return __builtin_bit_cast(sd_h2, __builtin_amdgcn_perm(0x64646464u, w, 0x04010400u));
 }
-__device__ __forceinline__ sd_i16x2 sd_hi(uint32_t w) {  // bytes 2,3 -> 16-bit lanes
-  return __builtin_bit_cast(sd_i16x2, __builtin_amdgcn_perm(0u, w, 0x0c030c02u));
+__device__ __forceinline__ sd_h2 sd_hi(uint32_t w) {  // bytes 2,3 -> 1024 + byte
+  return __builtin_bit_cast(sd_h2, __builtin_amdgcn_perm(0x64646464u, w, 0x04030402u));
 }
-__device__ __forceinline__ uint32_t sd_pack(sd_i16x2 lo, sd_i16x2 hi) {  // low bytes of 4 lanes
+__device__ __forceinline__ uint32_t sd_pack(sd_h2 lo, sd_h2 hi) {  // low bytes of 4 halves (each 1024 + byte)
   return __builtin_amdgcn_perm(__builtin_bit_cast(uint32_t, hi), __builtin_bit_cast(uint32_t, lo), 0x06040200u);
 }
-__device__ __forceinline__ sd_i16x2 sd_pair(sd_i16x2 f, sd_i16x2& b, sd_i16x2& v, sd_u16x2 n) {
-  const sd_i16x2 one = {1, 1};
-  b = __builtin_elementwise_min(b + one, __builtin_elementwise_max(b - one, f));
-  const sd_i16x2 d = __builtin_elementwise_max(b, f) - __builtin_elementwise_min(b, f);
-  const sd_u16x2 mu = __builtin_elementwise_min(__builtin_bit_cast(sd_u16x2, d) * n, (sd_u16x2){255, 255});
-  const sd_i16x2 m = __builtin_bit_cast(sd_i16x2, mu);
-  const sd_i16x2 nz = __builtin_elementwise_min(d, one);
-  v = __builtin_elementwise_min(v + nz, __builtin_elementwise_max(v - nz, m));
-  // d >= v  <=>  (d + 256 - v) >> 8 == 1, for d, v in [0, 255]
-  return __builtin_bit_cast(sd_i16x2, __builtin_bit_cast(sd_u16x2, d + (sd_i16x2){256, 256} - v) >> (sd_u16x2){8, 8});
+__device__ __forceinline__ sd_h2 sd_clamp01(sd_h2 x) {
+  return __builtin_elementwise_min(__builtin_elementwise_max(x, (sd_h2){0, 0}), (sd_h2){1, 1});
+}
+// one frame step of P pixel pairs, stage by stage over the pairs (each
+// stage of one pair is independent of the other pairs, so the packed ops of
+// consecutive pairs fill the wait states a dependent chain would need)
+template <int P>
+__device__ __forceinline__ void sd_pairs(const sd_h2 (&F)[P], sd_h2 (&B)[P], sd_h2 (&W)[P], sd_h2 (&L)[P], sd_h2 n) {
+  const sd_h2 one = {1, 1};
+  sd_h2 c1[P], c2[P], d[P], nz[P], m1[P];
+#pragma unroll
+  for (int q = 0; q < P; ++q) c1[q] = sd_clamp01(F[q] - B[q]);
+#pragma unroll
+  for (int q = 0; q < P; ++q) c2[q] = sd_clamp01(B[q] - F[q]);
+#pragma unroll
+  for (int q = 0; q < P; ++q) B[q] = (B[q] + c1[q]) - c2[q];
+#pragma unroll
+  for (int q = 0; q < P; ++q) { const sd_h2 e = F[q] - B[q]; d[q] = __builtin_elementwise_max(e, -e); }
+#pragma unroll
+  for (int q = 0; q < P; ++q) nz[q] = __builtin_elementwise_min(d[q], one);
+#pragma unroll
+  for (int q = 0; q < P; ++q) m1[q] = __builtin_elementwise_fma(d[q], n, -one);
+#pragma unroll
+  for (int q = 0; q < P; ++q)
+    W[q] = __builtin_elementwise_minimum(__builtin_elementwise_minimum(W[q] + nz[q], (sd_h2){254, 254}),
+                                         __builtin_elementwise_maximum(W[q] - nz[q], m1[q]));
+#pragma unroll
+  for (int q = 0; q < P; ++q) L[q] = sd_clamp01(d[q] - W[q]);
 }
 
 // NW 32-bit words (4 pixels each) per thread: 16 pixels (NW = 4) keeps the
@@ -705,7 +733,8 @@ __global__ __launch_bounds__(kBlock) void sigmadelta_k(const uint8_t* __restrict
                                                        uint8_t* __restrict__ label, int N,
                                                        uint32_t* __restrict__ partials) {
   typedef typename sd_vec<NW>::type V;
-  __shared__ uint32_t wc[kSdChunk][kBlock / 64];
+  static_assert(kSdChunk % kSdAhead == 0 && kSdAhead == 4, "one count word per group of frames in flight");
+  __shared__ uint32_t wc[kSdChunk / 4][kBlock];
   // XCD-aware tiles: workgroup b runs on XCD b % 8, and each XCD takes a
   // contiguous eighth of the image, so that its L2 sees contiguous spans of
   // every frame (the grid is a multiple of 8)
@@ -713,10 +742,16 @@ __global__ __launch_bounds__(kBlock) void sigmadelta_k(const uint8_t* __restrict
   const size_t tile = (size_t)(blockIdx.x % 8) * per + blockIdx.x / 8;
   const size_t t = tile * blockDim.x + threadIdx.x;
   const bool active = t < n16;
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  const unsigned short nn = (unsigned short)(N < 255 ? N : 255);
-  const sd_u16x2 n = {nn, nn};
-  sd_i16x2 b[2 * NW], v[2 * NW], l[2 * NW];
+  const size_t ta = active ? t : 0;
+  const _Float16 nn = (_Float16)(N < 255 ? N : 255);
+  const sd_h2 n = {nn, nn};
+  // frames k0 .. k0+3 in flight (clamped indices: every load in bounds and
+  // issued unconditionally)
+  V ff[kSdAhead];
+#pragma unroll
+  for (int j = 0; j < kSdAhead; ++j)
+    ff[j] = reinterpret_cast<const V*>(frames + (size_t)(k0 + j < k1 ? k0 + j : k1 - 1) * stride)[ta];
+  sd_h2 B[2 * NW], W[2 * NW], L[2 * NW];
   {
     uint32_t bw[NW], vw[NW];
 #pragma unroll
@@ -725,44 +760,49 @@ __global__ __launch_bounds__(kBlock) void sigmadelta_k(const uint8_t* __restrict
       sd_vec<NW>::words(reinterpret_cast<const V*>(bkg)[t], bw);
       sd_vec<NW>::words(reinterpret_cast<const V*>(var)[t], vw);
     }
+    const sd_h2 off = {1025, 1025};
 #pragma unroll
     for (int q = 0; q < NW; ++q) {
-      b[2 * q] = sd_lo(bw[q]);
-      b[2 * q + 1] = sd_hi(bw[q]);
-      v[2 * q] = sd_lo(vw[q]);
-      v[2 * q + 1] = sd_hi(vw[q]);
-      l[2 * q] = l[2 * q + 1] = (sd_i16x2){0, 0};
+      B[2 * q] = sd_lo(bw[q]);
+      B[2 * q + 1] = sd_hi(bw[q]);
+      W[2 * q] = sd_lo(vw[q]) - off;
+      W[2 * q + 1] = sd_hi(vw[q]) - off;
+      L[2 * q] = L[2 * q + 1] = (sd_h2){0, 0};
     }
   }
-  for (int k = k0; k < k1; k += 4) {
-    V ff[4];
+  uint32_t packed = 0u;
+  for (int k = k0; k < k1; k += kSdAhead) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j)  // clamped indices: every load in bounds and issued unconditionally
-      ff[j] = reinterpret_cast<const V*>(frames + (size_t)(k + j < k1 ? k + j : k1 - 1) * stride)[active ? t : 0];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < kSdAhead; ++j) {
       if (k + j >= k1) break;
       uint32_t fw[NW];
       sd_vec<NW>::words(ff[j], fw);
-      sd_i16x2 acc = {0, 0};
+      const int kn = k + j + kSdAhead;
+      ff[j] = reinterpret_cast<const V*>(frames + (size_t)(kn < k1 ? kn : k1 - 1) * stride)[ta];
+      sd_h2 F[2 * NW];
 #pragma unroll
-      for (int q = 0; q < NW; ++q) {
-        l[2 * q] = sd_pair(sd_lo(fw[q]), b[2 * q], v[2 * q], n);
-        l[2 * q + 1] = sd_pair(sd_hi(fw[q]), b[2 * q + 1], v[2 * q + 1], n);
-        acc += l[2 * q] + l[2 * q + 1];
-      }
-      uint32_t cnt = active ? (uint32_t)(acc.x + acc.y) : 0u;
-      cnt = wave_count(cnt);
-      if (lane == 0) wc[k + j - k0][wid] = cnt;
+      for (int q = 0; q < NW; ++q) { F[2 * q] = sd_lo(fw[q]); F[2 * q + 1] = sd_hi(fw[q]); }
+      sd_pairs<2 * NW>(F, B, W, L, n);
+      sd_h2 acc = {1024, 0};  // 1024 + count: its low byte is the count
+#pragma unroll
+      for (int q = 0; q < 2 * NW; ++q) acc += L[q];
+      const uint32_t c = __builtin_bit_cast(unsigned short, (_Float16)(acc.x + acc.y));
+      // byte j of the group's word <- the count's low byte
+      packed = __builtin_amdgcn_perm(c, packed, j == 0 ? 0x03020104u : j == 1 ? 0x03020400u
+                                                : j == 2 ? 0x03040100u : 0x04020100u);
+      if (j == kSdAhead - 1) wc[(k + j - k0) >> 2][threadIdx.x] = active ? packed : 0u;
     }
   }
+  const int nf = k1 - k0;
+  if (nf & 3) wc[(nf - 1) >> 2][threadIdx.x] = active ? packed : 0u;
   if (active) {
     uint32_t bw[NW], vw[NW], lw[NW];
+    const sd_h2 off = {1025, 1025}, loff = {1024, 1024};
 #pragma unroll
     for (int q = 0; q < NW; ++q) {
-      bw[q] = sd_pack(b[2 * q], b[2 * q + 1]);
-      vw[q] = sd_pack(v[2 * q], v[2 * q + 1]);
-      lw[q] = sd_pack(l[2 * q], l[2 * q + 1]);
+      bw[q] = sd_pack(B[2 * q], B[2 * q + 1]);
+      vw[q] = sd_pack(W[2 * q] + off, W[2 * q + 1] + off);
+      lw[q] = sd_pack(L[2 * q] + loff, L[2 * q + 1] + loff);
     }
     reinterpret_cast<V*>(bkg)[t] = sd_vec<NW>::make(bw);
     reinterpret_cast<V*>(var)[t] = sd_vec<NW>::make(vw);
@@ -770,11 +810,29 @@ __global__ __launch_bounds__(kBlock) void sigmadelta_k(const uint8_t* __restrict
   }
   if (partials) {
     __syncthreads();
-    for (int i = threadIdx.x; i < k1 - k0; i += blockDim.x) {
-      uint32_t c = 0;
+    // 16 threads per group of 4 frames: each adds 16 words (the lanes
+    // s, s + 16, ...) with the bytes split over 16-bit fields (at most
+    // 16 x 16 per field), then the 16 threads combine (at most 4096)
+    const int g = threadIdx.x >> 4, s = threadIdx.x & 15;
+    uint32_t e = 0u, o = 0u;
+    if (g < (nf + 3) >> 2) {
 #pragma unroll
-      for (int w = 0; w < kBlock / 64; ++w) c += wc[i][w];
-      partials[(size_t)i * gridDim.x + blockIdx.x] = c;
+      for (int u = 0; u < kBlock / 16; ++u) {
+        const uint32_t w = wc[g][u * 16 + s];
+        e += w & 0x00ff00ffu;
+        o += (w >> 8) & 0x00ff00ffu;
+      }
+    }
+#pragma unroll
+    for (int off = 8; off >= 1; off >>= 1) {
+      e += __shfl_xor(e, off, 64);
+      o += __shfl_xor(o, off, 64);
+    }
+    if (s == 0) {
+      const uint32_t c4[4] = {e & 0xffffu, o & 0xffffu, e >> 16, o >> 16};
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (4 * g + r < nf) partials[(size_t)(4 * g + r) * gridDim.x + blockIdx.x] = c4[r];
     }
   }
 }

@@ -186,13 +186,15 @@ def u8_binop(op, a, b):
     return y
 
 
-def sigmadelta(frames, N):
-    """frames: uint8 array [nframes, npix].  Returns bkg, var, label, counts."""
+def sigmadelta(frames, N, bkg=None, var=None, label=None):
+    """frames: uint8 array [nframes, npix]; the state starts from bkg, var and
+    label (by default the demo's: frame 0, ones, zeros).  Returns bkg, var,
+    label, counts."""
     frames = np.ascontiguousarray(frames, dtype=np.uint8)
     nf, npix = frames.shape
-    bkg = frames[0].copy()
-    var = np.ones(npix, dtype=np.uint8)
-    label = np.zeros(npix, dtype=np.uint8)
+    bkg = frames[0].copy() if bkg is None else np.array(bkg, dtype=np.uint8)
+    var = np.ones(npix, dtype=np.uint8) if var is None else np.array(var, dtype=np.uint8)
+    label = np.zeros(npix, dtype=np.uint8) if label is None else np.array(label, dtype=np.uint8)
     counts = np.zeros(nf, dtype=np.uint32)
     assert lib().oracle_sigmadelta_u8(npix, nf, _p(frames), npix, _p(bkg), _p(var),
                                       _p(label), N, _p(counts)) == 0

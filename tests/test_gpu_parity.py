@@ -422,6 +422,38 @@ def test_sigmadelta_bitwise(h, w, nf, N):
     assert np.array_equal(cnt.cpu().numpy().astype(np.uint32), wc)
 
 
+@pytest.mark.parametrize("px", [16, 8, 4])
+@pytest.mark.parametrize("N", [1, 2, 3, 200, 255, 1000])
+def test_sigmadelta_any_state(px, N):
+    """Random frames (every difference 0..255 occurs) from a random state
+    (background and variance 0..255, 0 and 255 included), 150 frames (three
+    launches of at most 64, the last not a multiple of 4), 4099 pixels (a
+    tail for every pixel-per-thread shape), factors up to beyond 255."""
+    from nt2_amd.native import check, lib
+    say("sigmadelta any state px=%d N=%d" % (px, N))
+    rng = np.random.default_rng(1000 * px + N)
+    nf, npix = 150, 4099
+    fr = rng.integers(0, 256, size=(nf, npix), dtype=np.uint8)
+    fr[:, :8] = [0, 255, 0, 255, 128, 1, 254, 7]  # extreme steps
+    b0 = rng.integers(0, 256, size=npix, dtype=np.uint8)
+    v0 = rng.integers(0, 256, size=npix, dtype=np.uint8)
+    b0[:4], v0[:4] = [0, 255, 0, 255], [0, 255, 255, 0]
+    frames = torch.from_numpy(fr).cuda()
+    bkg, var = torch.from_numpy(b0).cuda(), torch.from_numpy(v0).cuda()
+    label = torch.zeros(npix, dtype=torch.uint8, device="cuda")
+    check(lib().nt2h_sigmadelta_set_config(px), "sigmadelta_set_config")
+    try:
+        bkg, var, label, cnt = ops.sigmadelta(frames, N, bkg=bkg, var=var, label=label)
+        torch.cuda.synchronize()
+    finally:
+        check(lib().nt2h_sigmadelta_set_config(16), "sigmadelta_set_config")
+    wb, wv, wl, wc = O.sigmadelta(fr, N, bkg=b0, var=v0)
+    assert np.array_equal(bkg.cpu().numpy(), wb)
+    assert np.array_equal(var.cpu().numpy(), wv)
+    assert np.array_equal(label.cpu().numpy(), wl)
+    assert np.array_equal(cnt.cpu().numpy().astype(np.uint32), wc)
+
+
 def test_sigmadelta_step_equals_fused():
     say("sigmadelta step vs fused")
     frames = ops.fill_frames(240, 320, 12, seed=5)
