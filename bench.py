@@ -139,6 +139,46 @@ def _cpu_time(fn, reps=3):
     return float(np.median(t))
 
 
+def graph_step(torch, step, x, y, out, replays=200):
+    """The saxpy + sdot step captured once in a HIP graph (torch's capture of
+    the library's launches on the current stream) and replayed: the launch
+    cost of the three kernels without the per-call host path.  Three replays
+    are checked bit for bit against the eager step from the same y (an eager
+    copy between them), then `replays` are timed back to back."""
+    y0 = y.clone()
+    cap = torch.cuda.Stream()  # the library keeps the fold's partials per stream:
+    cap.wait_stream(torch.cuda.current_stream())  # make them on the capture stream first
+    with torch.cuda.stream(cap):
+        step()
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=cap):
+        step()
+    same = True
+    for _ in range(3):
+        y.copy_(y0)
+        step()
+        ye, oe = y.clone(), out.clone()
+        y.copy_(y0)
+        g.replay()
+        torch.cuda.synchronize()
+        same = same and torch.equal(y, ye) and torch.equal(out, oe)
+    stream = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for _ in range(10):
+        g.replay()
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(replays):
+        g.replay()
+    e1.record(stream)
+    torch.cuda.synchronize()
+    us = 1e3 * e0.elapsed_time(e1) / replays
+    y.copy_(y0)
+    del g
+    return {"us_per_step": us, "replays_equal_eager": bool(same)}
+
+
 def leg_saxpy_sdot(torch, ops, O):
     """saxpy then sdot on 2^20 floats (the plain configuration: launch-latency
     bound, 21 MB sits in the Infinity Cache), and each kernel alone at 2^26
@@ -154,6 +194,7 @@ def leg_saxpy_sdot(torch, ops, O):
         ops.dot(x, y, out)
     ms = time_steps(torch, step, 200, 20)
     dot_us = 1e3 * time_steps(torch, lambda: ops.dot(x, y, out), 200, 20)
+    graph = graph_step(torch, step, x, y, out)
     xs, ys = x.cpu().numpy(), y.cpu().numpy()
     threads = O.threads()
 
@@ -174,6 +215,7 @@ def leg_saxpy_sdot(torch, ops, O):
     torch.cuda.empty_cache()
     # bytes: saxpy reads x, y and writes y (12 B); dot reads x, y (8 B)
     return {"n": n, "us_per_step": ms * 1e3, "gbs": 20.0 * n / (ms * 1e-3) / 1e9, "dot_us_2p20": dot_us,
+            "graph": graph,
             "at_2p26": {"dot_us": dot_big * 1e3, "dot_gbs": 8.0 * big / (dot_big * 1e-3) / 1e9,
                         "sum_us": sum_big * 1e3, "sum_gbs": 4.0 * big / (sum_big * 1e-3) / 1e9,
                         "saxpy_us": saxpy_big * 1e3, "saxpy_gbs": 12.0 * big / (saxpy_big * 1e-3) / 1e9},

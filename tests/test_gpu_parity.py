@@ -466,6 +466,42 @@ def test_sigmadelta_step_equals_fused():
     assert torch.equal(b1, b2) and torch.equal(v1, v2) and torch.equal(l1, l2)
 
 
+@pytest.mark.parametrize("n", [1 << 20, 100003])
+def test_graph_replay_saxpy_dot(n):
+    """saxpy then dot captured in a HIP graph on torch's capture stream: five
+    replays, each from the same y restored by an eager copy, equal the eager
+    step bit for bit (the fold's partials and result are written by kernels,
+    with no memset node in the graph)."""
+    say("graph replay saxpy+dot n=%d" % n)
+    x = ops.fill_uniform(n, -1.0, 1.0, 61)
+    y = ops.fill_uniform(n, -1.0, 1.0, 62)
+    out = torch.empty(1, dtype=torch.float64, device="cuda")
+
+    def step():
+        ops.saxpy(2.5, x, y)
+        ops.dot(x, y, out)
+    y0 = y.clone()
+    cap = torch.cuda.Stream()  # the fold's partials are kept per stream: make them
+    cap.wait_stream(torch.cuda.current_stream())  # on the capture stream before capturing
+    with torch.cuda.stream(cap):
+        step()
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=cap):
+        step()
+    for _ in range(5):
+        y.copy_(y0)
+        step()
+        ye, oe = y.clone(), out.clone()
+        y.copy_(y0)
+        out.fill_(-1.0)
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(y, ye)
+        assert torch.equal(out, oe)
+    del g
+
+
 @pytest.mark.parametrize("n", [100003, 8, 3])
 def test_expr4_exact_bitwise(n):
     say("expr4 n=%d" % n)
