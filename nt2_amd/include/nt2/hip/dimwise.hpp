@@ -608,7 +608,8 @@ __global__ __launch_bounds__(kChunkThreads) void chunk_scan_k(Src src, size_t le
     tot[t] += a;
     __syncthreads();
   }
-  const double before = offsets[line * nchunk + c] + (t ? tot[t - 1] : 0.0);
+  const double off = offsets ? offsets[line * nchunk + c] : 0.0;  // null: one chunk per line
+  const double before = off + (t ? tot[t - 1] : 0.0);
 #pragma unroll
   for (int k = 0; k < E; ++k) tile[t * E + k + t] = (T)(before + v[k]);
   __syncthreads();
@@ -797,6 +798,11 @@ void launch_scan(const Src& src, const lines& L, bool fast, double* scratch, T* 
     launch_scan_ordered<T, op_sum>(src, L, out, s);
   } else if (chunked(L)) {
     const size_t nch = chunks_of(L.len);
+    if (nch == 1) {  // every line is one chunk: its offset is 0, a single pass
+      hipLaunchKernelGGL((chunk_scan_k<T, Src>), dim3(1u, (unsigned)L.outer), dim3(kChunkThreads), 0, s, src,
+                         L.len, nch, (const double*)nullptr, out);
+      return;
+    }
     hipLaunchKernelGGL((chunk_sum_k<T, Src>), dim3((unsigned)nch, (unsigned)L.outer), dim3(kChunkThreads), 0, s,
                        src, L.len, nch, scratch);
     hipLaunchKernelGGL(chunk_offsets_k, dim3(blocks_for(L.outer * 64)), dim3(256), 0, s, scratch, nch, L.outer);
