@@ -84,6 +84,7 @@ __device__ __forceinline__ void two_sum_add(T& s, T& c, T v) {
 }
 
 constexpr int kUnroll = 16;  // loads issued ahead of a chain's additions
+constexpr int kDeep = 64;    // the same for the few long chains of strided folds and scans
 
 // The scans' operations: NT2's cumulative_<tag, bop, neutral> functors
 // (modules/boost/simd/base/include/boost/simd/swar/functions/cumsum.hpp:27,
@@ -111,8 +112,10 @@ struct op_max {
 
 // running Op over n elements from idx with stride `step`, applied to acc in
 // order (A: accumulator type); f(m, acc) after each step.  Loads go kUnroll
-// ahead of the dependent steps.
-template <class A, class T, class Op = op_sum, class Src, class F>
+// ahead of the dependent steps: a group of kUnroll loads, then their steps,
+// so the memory latency is paid once per group (kDeep for the strided folds
+// and scans whose chains, one per output, are few and each stream a line).
+template <class A, class T, class Op = op_sum, int kUnroll = 16, class Src, class F>
 __device__ __forceinline__ void chain(const Src& src, size_t idx, size_t step, size_t n, A& acc, F f) {
   _Pragma("clang fp contract(off)")
   size_t m = 0;
@@ -368,7 +371,7 @@ __global__ __launch_bounds__(256) void outer_fold_scalar_k(Src src, size_t inner
   const size_t o = line / inner;
   const size_t i = line - o * inner;
   T acc = T(0);
-  chain<T, T>(src, i + o * inner * len, inner, len, acc, [](size_t, T) {});
+  chain<T, T, op_sum, kDeep>(src, i + o * inner * len, inner, len, acc, [](size_t, T) {});
   out[line] = acc;
 }
 
@@ -411,7 +414,7 @@ __global__ __launch_bounds__(256) void outer_fold_exact_k(Src src, size_t inner,
   } else {
     const size_t i = ipack + (q - ipack / W);
     T acc = T(0);
-    chain<T, T>(src, base + i, inner, len, acc, [](size_t, T) {});
+    chain<T, T, op_sum, kDeep>(src, base + i, inner, len, acc, [](size_t, T) {});
     out[o * inner + i] = acc;
   }
 }
@@ -428,7 +431,7 @@ __global__ __launch_bounds__(256) void scan_strided_k(Src src, size_t inner, siz
   const size_t i = line - o * inner;
   const size_t idx = i + o * inner * len;
   A acc = Op::template neutral<A>();
-  chain<A, T, Op>(src, idx, inner, len, acc, [&](size_t m, A a) { out[idx + m * inner] = (T)a; });
+  chain<A, T, Op, kDeep>(src, idx, inner, len, acc, [&](size_t m, A a) { out[idx + m * inner] = (T)a; });
 }
 
 // contiguous lines: one wave scans 64 lines, tile by tile of K elements staged

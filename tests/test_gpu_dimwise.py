@@ -140,6 +140,33 @@ def test_cumulative_matches_oracle(shape, dtype, op):
             assert same_bits(got, want), (op, shape, dim, pol)
 
 
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("op", ["cumsum", "cumprod", "cummin", "cummax"])
+@pytest.mark.parametrize("shape", [(200, 20000), (2500, 1000)])
+def test_scan_many_long_lines(shape, op, dtype):
+    """Along dimension 1: 20000 contiguous lines of 200 (the 64-lines-per-wave
+    kernel: 3 full tiles of 64 floats or 6 of 32 doubles, then a tail; 20000
+    lines leave a partial wave) and 1000 lines of 2500 (a workgroup per line);
+    along dimension 2 the strided chains (64 loads in flight per chain).
+    NT2's order bit for bit: cumsum under the exact policy, the other scans
+    under both."""
+    rng = np.random.default_rng(sum(shape) + len(op))
+    n = int(np.prod(shape))
+    x = _cum_inputs("cumsum" if op == "cumsum" else op, n, dtype, rng)
+    if op == "cumsum":
+        x = np.nan_to_num(x)
+    xd = _dev(x)
+    for dim in (1, 2):
+        if op == "cumsum":
+            want, pols = O.cumsum_dim(x, shape, dim), (EXACT,)
+        else:
+            want, pols = O.cumop_dim(x, shape, dim, op), (EXACT, FAST)
+        for pol in pols:
+            say("scan many long lines", op, shape, dtype.__name__, "dim=%d pol=%d" % (dim, pol))
+            got = ops.cumsum_dim(xd, shape, dim, pol, op=op).cpu().numpy()
+            assert same_bits(got, want), (op, shape, dim, pol)
+
+
 def test_cumulative_in_place_and_errors():
     from nt2_amd.native import Nt2hError
     x = np.random.default_rng(4).uniform(0.5, 1.5, 3 * 20000).astype(np.float32)
