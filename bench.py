@@ -179,6 +179,21 @@ def graph_step(torch, step, x, y, out, replays=200):
     return {"us_per_step": us, "replays_equal_eager": bool(same)}
 
 
+def cxx_step(lg, reps):
+    """The same sizes driven from C++ (tests/cpp/frontend_bench: the caller an
+    NT2 program is): the C ABI's saxpy, dot and saxpy-then-dot step, and the
+    front end's two statements (globalsum returning to the host)."""
+    import subprocess
+    exe = os.path.join(ROOT, "tests", "cpp", "bin", "frontend_bench")
+    p = subprocess.run([exe, str(lg), str(reps)], capture_output=True, text=True, timeout=300)
+    if p.returncode != 0:
+        raise RuntimeError("frontend_bench failed: %s" % p.stderr[-500:])
+    r = json.loads(p.stdout.strip().splitlines()[-1])
+    return {"step_us": 1e3 * r["saxpy_sdot_step"]["cabi_ms"], "saxpy_us": 1e3 * r["saxpy"]["cabi_ms"],
+            "dot_us": 1e3 * r["dot"]["cabi_ms"], "frontend_step_us": 1e3 * r["saxpy_sdot_step"]["frontend_ms"],
+            "frontend_saxpy_bitwise": r["saxpy"]["bitwise_equal"]}
+
+
 def leg_saxpy_sdot(torch, ops, O):
     """saxpy then sdot on 2^20 floats (the plain configuration: launch-latency
     bound, 21 MB sits in the Infinity Cache), and each kernel alone at 2^26
@@ -195,6 +210,7 @@ def leg_saxpy_sdot(torch, ops, O):
     ms = time_steps(torch, step, 200, 20)
     dot_us = 1e3 * time_steps(torch, lambda: ops.dot(x, y, out), 200, 20)
     graph = graph_step(torch, step, x, y, out)
+    cxx = cxx_step(20, 200)
     xs, ys = x.cpu().numpy(), y.cpu().numpy()
     threads = O.threads()
 
@@ -215,7 +231,7 @@ def leg_saxpy_sdot(torch, ops, O):
     torch.cuda.empty_cache()
     # bytes: saxpy reads x, y and writes y (12 B); dot reads x, y (8 B)
     return {"n": n, "us_per_step": ms * 1e3, "gbs": 20.0 * n / (ms * 1e-3) / 1e9, "dot_us_2p20": dot_us,
-            "graph": graph,
+            "graph": graph, "cxx_driven": cxx,
             "at_2p26": {"dot_us": dot_big * 1e3, "dot_gbs": 8.0 * big / (dot_big * 1e-3) / 1e9,
                         "sum_us": sum_big * 1e3, "sum_gbs": 4.0 * big / (sum_big * 1e-3) / 1e9,
                         "saxpy_us": saxpy_big * 1e3, "saxpy_gbs": 12.0 * big / (saxpy_big * 1e-3) / 1e9},

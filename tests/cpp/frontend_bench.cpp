@@ -4,6 +4,7 @@
 //   Y = alpha * X + Y            vs nt2h_saxpy_f32      (demo/saxpy/nt2/saxpy_nt2.cpp:39-42)
 //   s = globalsum(X * Y)         vs nt2h_dot_f32        (demo/sdot/nt2/sdot_nt2.cpp:35-38)
 //   tie(call, put) = tie(...)    vs nt2h_blackscholes_f32, exact policy (one fused statement)
+//   the saxpy then sdot step     vs nt2h_saxpy_f32 then nt2h_dot_f32
 // Prints one JSON object.  Usage: frontend_bench [log2 n] [reps]
 #include <nt2/table.hpp>
 
@@ -81,6 +82,18 @@ int main(int argc, char** argv) {
     ck(nt2h_memcpy(&h, d_s, sizeof(double), NT2H_D2H, nullptr), "copy");
   }, reps);
 
+  // the saxpy then sdot step (the two demo statements back to back): the C
+  // ABI's three launches, and the front end's two statements (globalsum returns
+  // its value to the host, as the sdot demo does)
+  const double abi_step = time_ms([&] {
+    ck(nt2h_saxpy_f32(n, alpha, X.device_data(), Y2.device_data_for_write(), nullptr), "saxpy");
+    ck(nt2h_dot_f32(n, X.device_data(), Y2.device_data(), d_s, nullptr), "dot");
+  }, reps);
+  const double fe_step = time_ms([&] {
+    Y = alpha * X + Y;
+    fe_s = globalsum(X * Y);
+  }, reps);
+
   // Black-Scholes: call and put in one tie statement vs the fixed exact kernel
   table<float> S(of_size(n)), K(of_size(n)), T(of_size(n)), call, put, c2(of_size(n)), p2(of_size(n));
   ck(nt2h_fill_uniform_f32(S.device_data_for_write(), n, 5.0, 30.0, 33, nullptr), "fill");
@@ -110,11 +123,12 @@ int main(int argc, char** argv) {
               "\"frontend_over_cabi\": %.4f, \"bitwise_equal\": %s}, "
               "\"dot\": {\"frontend_ms\": %.5f, \"cabi_ms\": %.5f, \"cabi_with_result_copy_ms\": %.5f, "
               "\"frontend_gbs\": %.1f, \"cabi_gbs\": %.1f, \"frontend_over_cabi_with_copy\": %.4f, \"close\": %s}, "
+              "\"saxpy_sdot_step\": {\"frontend_ms\": %.5f, \"cabi_ms\": %.5f}, "
               "\"blackscholes_tie\": {\"frontend_ms\": %.5f, \"cabi_exact_ms\": %.5f, \"frontend_gbs\": %.1f, "
               "\"cabi_gbs\": %.1f, \"frontend_over_cabi\": %.4f, \"bitwise_equal\": %s}}\n",
               n, reps, fe_saxpy, abi_saxpy, gbs(12, fe_saxpy), gbs(12, abi_saxpy), fe_saxpy / abi_saxpy,
               saxpy_equal ? "true" : "false", fe_dot, abi_dot, abi_dot_sync, gbs(8, fe_dot), gbs(8, abi_dot),
-              fe_dot / abi_dot_sync, dot_close ? "true" : "false", fe_bs, abi_bs, gbs(20, fe_bs), gbs(20, abi_bs),
+              fe_dot / abi_dot_sync, dot_close ? "true" : "false", fe_step, abi_step, fe_bs, abi_bs, gbs(20, fe_bs), gbs(20, abi_bs),
               fe_bs / abi_bs, bs_equal ? "true" : "false");
   return 0;
 }
