@@ -206,3 +206,20 @@ def test_trace_replay_count_bounded():
             srv.run_trace(items, np.zeros(3), 10, count=4)
         assert srv.run_trace(items, np.zeros(4), 10, count=4) == 0
         assert srv.stats()["completed"] == 4
+
+
+def test_concurrent_clients_stress():
+    """tests/cpp/server_stress: four client threads submitting, polling,
+    waiting, cancelling and releasing concurrently against the host back end
+    (forced allocation and launch failures); every request ends and the
+    budget returns to zero."""
+    import json
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "bin", "server_stress")
+    if not os.path.exists(exe):
+        pytest.skip("tests/cpp/bin/server_stress not built")
+    p = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stdout + p.stderr
+    r = json.loads(p.stdout.strip().splitlines()[-1])
+    assert r["used_bytes"] == 0 and r["done"] > 0 and r["failed"] > 0 and r["cancelled"] > 0
