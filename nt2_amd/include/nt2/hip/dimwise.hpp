@@ -305,13 +305,19 @@ __global__ __launch_bounds__(kStageThreads) void inner_fold_staged_k(Src src, si
 }
 
 // scan of contiguous lines: thread 0 runs the line's chain on each tile in LDS,
-// the workgroup loads and stores the tiles (A: accumulator type)
+// the workgroup (one wave) loads and stores the tiles (A: accumulator type).
+// Small workgroups and 4 KB tiles keep up to 20 chains per CU (LDS-bound).
+constexpr int kScanThreads = 64;
+constexpr int kScanAhead = 32;  // LDS reads issued ahead of the chain's steps
+template <class T> struct scan_elems { static constexpr int value = 4096 / (int)sizeof(T); };  // 4 KB per tile
 template <class T, class A, class Src, class Op = op_sum>
-__global__ __launch_bounds__(kStageThreads) void scan_staged_k(Src src, size_t len, T* __restrict__ out) {
+__global__ __launch_bounds__(kScanThreads) void scan_staged_k(Src src, size_t len, T* __restrict__ out) {
   _Pragma("clang fp contract(off)")
-  constexpr int TE = stage_elems<T>::value;
+  constexpr int kStageThreads = kScanThreads;
+  constexpr int kUnroll = kScanAhead;
+  constexpr int TE = scan_elems<T>::value;
   constexpr int PER = TE / kStageThreads;
-  __shared__ T buf[2][TE];
+  __shared__ __attribute__((aligned(16))) T buf[2][TE];
   const size_t base = (size_t)blockIdx.x * len;
   const int t = threadIdx.x;
   A acc = Op::template neutral<A>();
@@ -335,16 +341,25 @@ __global__ __launch_bounds__(kStageThreads) void scan_staged_k(Src src, size_t l
     if (more) load_tile(tb + TE);  // in flight during the chain below
     const int n = (int)(len - tb < (size_t)TE ? len - tb : (size_t)TE);
     if (t == 0) {
+      // 16-byte LDS reads and writes (4 floats or 2 doubles per access), all
+      // reads of a group issued ahead of its steps
+      constexpr int VE = 16 / (int)sizeof(T);
+      typedef T vec_t __attribute__((ext_vector_type(VE)));
+      vec_t* bv = reinterpret_cast<vec_t*>(&buf[b][0]);
       int i = 0;
-      for (; i + kUnroll <= n; i += kUnroll) {  // LDS reads issued ahead of the additions
-        T r[kUnroll];
+      for (; i + kUnroll <= n; i += kUnroll) {
+        vec_t r[kUnroll / VE];
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) r[u] = buf[b][i + u];
+        for (int u = 0; u < kUnroll / VE; ++u) r[u] = bv[i / VE + u];
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) {
-          acc = Op::step(acc, (A)r[u]);
-          buf[b][i + u] = (T)acc;
-        }
+        for (int u = 0; u < kUnroll / VE; ++u)
+#pragma unroll
+          for (int l = 0; l < VE; ++l) {
+            acc = Op::step(acc, (A)r[u][l]);
+            r[u][l] = (T)acc;
+          }
+#pragma unroll
+        for (int u = 0; u < kUnroll / VE; ++u) bv[i / VE + u] = r[u];
       }
       for (; i < n; ++i) {
         acc = Op::step(acc, (A)buf[b][i]);
@@ -480,6 +495,87 @@ __global__ __launch_bounds__(kRowLines) void scan_rows_k(Src src, size_t len, si
       const int r = f / K, c = f % K;
       const size_t line = line0 + r;
       if (line < nlines && c0 + c < len) out[line * len + c0 + c] = tile[r][c];
+    }
+    __syncthreads();
+  }
+}
+
+// scan of contiguous lines, kLinesPerWave lines per wave (one wave per
+// workgroup): the wave loads a tile of K elements of each of its lines
+// (64 consecutive elements of one line per load), the next tile is in flight
+// during the chains; lanes 0 .. R-1 each run one line's chain over the tile in
+// LDS with 16-byte reads and writes (rows padded by 16 bytes, so the R lanes'
+// accesses fall in distinct banks); then the wave stores the tile.  One
+// instruction advances R chains, against one for the workgroup-per-line form.
+constexpr int kLinesPerWave = 8;
+constexpr size_t kLinesMin = 2048;  // contiguous lines from which the R-lines kernel is used (below kRowsMin)
+template <class T> struct lines_tile { static constexpr int value = 2048 / (int)sizeof(T); };  // 2 KB of a line
+template <class T, class A, class Src, class Op = op_sum>
+__global__ __launch_bounds__(64) void scan_lines_k(Src src, size_t len, size_t nlines, T* __restrict__ out) {
+  _Pragma("clang fp contract(off)")
+  constexpr int R = kLinesPerWave;
+  constexpr int K = lines_tile<T>::value;
+  constexpr int VE = 16 / (int)sizeof(T);
+  constexpr int KP = K + VE;          // padded row
+  constexpr int PER = R * K / 64;     // elements per thread and tile
+  constexpr int G = 32;               // elements per group of LDS reads
+  typedef T vec_t __attribute__((ext_vector_type(VE)));
+  __shared__ __attribute__((aligned(16))) T buf[R * KP];
+  const size_t line0 = (size_t)blockIdx.x * R;
+  const int t = threadIdx.x;
+  A acc = Op::template neutral<A>();
+  T xs[PER];
+  auto load_tile = [&](size_t c0) {
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      const int f = e * 64 + t;
+      const int r = f / K, c = f % K;
+      const size_t line = line0 + r;
+      T x[1] = {T(0)};
+      if (line < nlines && c0 + c < len) src.template load<1>(line * len + c0 + c, x);
+      xs[e] = x[0];
+    }
+  };
+  load_tile(0);
+  for (size_t c0 = 0; c0 < len; c0 += K) {
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      const int f = e * 64 + t;
+      buf[(f / K) * KP + f % K] = xs[e];
+    }
+    __syncthreads();
+    if (c0 + K < len) load_tile(c0 + K);  // in flight during the chains and the stores
+    const int kmax = (int)(len - c0 < (size_t)K ? len - c0 : (size_t)K);
+    if (t < R) {
+      vec_t* row = reinterpret_cast<vec_t*>(buf + t * KP);
+      int c = 0;
+      for (; c + G <= kmax; c += G) {
+        vec_t rr[G / VE];
+#pragma unroll
+        for (int u = 0; u < G / VE; ++u) rr[u] = row[c / VE + u];
+#pragma unroll
+        for (int u = 0; u < G / VE; ++u)
+#pragma unroll
+          for (int l = 0; l < VE; ++l) {
+            acc = Op::step(acc, (A)rr[u][l]);
+            rr[u][l] = (T)acc;
+          }
+#pragma unroll
+        for (int u = 0; u < G / VE; ++u) row[c / VE + u] = rr[u];
+      }
+      T* rs = buf + t * KP;
+      for (; c < kmax; ++c) {
+        acc = Op::step(acc, (A)rs[c]);
+        rs[c] = (T)acc;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < PER; ++e) {
+      const int f = e * 64 + t;
+      const int r = f / K, c = f % K;
+      const size_t line = line0 + r;
+      if (line < nlines && c0 + c < len) out[line * len + c0 + c] = buf[r * KP + c];
     }
     __syncthreads();
   }
@@ -784,8 +880,11 @@ void launch_scan_ordered(const Src& src, const lines& L, T* out, hipStream_t s) 
   if (L.inner == 1 && nl >= kRowsMin)
     hipLaunchKernelGGL((scan_rows_k<T, T, Src, Op>), dim3(blocks_for(nl, kRowLines)), dim3(kRowLines), 0, s, src,
                        L.len, nl, out);
+  else if (L.inner == 1 && nl >= kLinesMin)
+    hipLaunchKernelGGL((scan_lines_k<T, T, Src, Op>), dim3(blocks_for(nl, kLinesPerWave)), dim3(64), 0, s, src,
+                       L.len, nl, out);
   else if (L.inner == 1 && nl <= 65535)
-    hipLaunchKernelGGL((scan_staged_k<T, T, Src, Op>), dim3((unsigned)nl), dim3(kStageThreads), 0, s, src, L.len,
+    hipLaunchKernelGGL((scan_staged_k<T, T, Src, Op>), dim3((unsigned)nl), dim3(kScanThreads), 0, s, src, L.len,
                        out);
   else
     hipLaunchKernelGGL((scan_strided_k<T, T, Src, Op>), dim3(blocks_for(nl)), dim3(256), 0, s, src, L.inner, L.len,
@@ -814,8 +913,11 @@ void launch_scan(const Src& src, const lines& L, bool fast, double* scratch, T* 
   } else if (L.inner == 1 && nl >= kRowsMin) {
     hipLaunchKernelGGL((scan_rows_k<T, double, Src>), dim3(blocks_for(nl, kRowLines)), dim3(kRowLines), 0, s, src,
                        L.len, nl, out);
+  } else if (L.inner == 1 && nl >= kLinesMin) {
+    hipLaunchKernelGGL((scan_lines_k<T, double, Src>), dim3(blocks_for(nl, kLinesPerWave)), dim3(64), 0, s, src,
+                       L.len, nl, out);
   } else if (L.inner == 1) {
-    hipLaunchKernelGGL((scan_staged_k<T, double, Src>), dim3((unsigned)nl), dim3(kStageThreads), 0, s, src, L.len,
+    hipLaunchKernelGGL((scan_staged_k<T, double, Src>), dim3((unsigned)nl), dim3(kScanThreads), 0, s, src, L.len,
                        out);
   } else {
     const size_t nch = strided_chunks(L);

@@ -142,10 +142,12 @@ def test_cumulative_matches_oracle(shape, dtype, op):
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("op", ["cumsum", "cumprod", "cummin", "cummax"])
-@pytest.mark.parametrize("shape", [(200, 20000), (2500, 1000)])
+@pytest.mark.parametrize("shape", [(200, 20000), (1000, 2500), (2500, 1000)])
 def test_scan_many_long_lines(shape, op, dtype):
     """Along dimension 1: 20000 contiguous lines of 200 (the 64-lines-per-wave
     kernel: 3 full tiles of 64 floats or 6 of 32 doubles, then a tail; 20000
+    lines leave a partial wave), 2500 lines of 1000 (the 8-lines-per-wave
+    kernel: one full tile of 512 floats or 3 of 256 doubles, then a tail; 2500
     lines leave a partial wave) and 1000 lines of 2500 (a workgroup per line);
     along dimension 2 the strided chains (64 loads in flight per chain).
     NT2's order bit for bit: cumsum under the exact policy, the other scans
