@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <vector>
 #include <mutex>
 #include <string>
 
@@ -40,10 +41,15 @@ int hip_fail(hipError_t e, const char* what) {
 }
 
 // ============================================================================
-// reduction scratch, one per (device, stream)
+// reduction scratch, one per (device, stream).  A buffer that has to grow is
+// replaced by one at least twice its size and the old one is kept until the
+// process ends (not freed): work already queued on the stream, or a graph
+// captured from it, may still hold its address.  Growth is rare and the
+// retained total stays below the largest buffer; no stream synchronisation.
 // ============================================================================
 static std::mutex g_scratch_mu;
 static std::map<std::pair<int, void*>, Scratch> g_scratch;
+static std::vector<double*> g_scratch_retired;
 
 int get_scratch(size_t nblocks, Scratch* out, hipStream_t s) {
   int dev = 0;
@@ -51,14 +57,12 @@ int get_scratch(size_t nblocks, Scratch* out, hipStream_t s) {
   std::lock_guard<std::mutex> lk(g_scratch_mu);
   Scratch& sc = g_scratch[{dev, (void*)s}];
   if (sc.cap < nblocks) {
-    if (sc.partial) {
-      NT2H_CHECK(hipStreamSynchronize(s), "scratch sync");
-      NT2H_CHECK(hipFree(sc.partial), "scratch free");
-      sc.partial = nullptr;
-      sc.cap = 0;
-    }
-    const size_t cap = nblocks < kMaxFoldBlocks ? kMaxFoldBlocks : nblocks;
-    NT2H_CHECK(hipMalloc(&sc.partial, cap * sizeof(double)), "scratch alloc");
+    size_t cap = nblocks < kMaxFoldBlocks ? kMaxFoldBlocks : nblocks;
+    if (sc.partial && cap < 2 * sc.cap) cap = 2 * sc.cap;
+    double* p = nullptr;
+    NT2H_CHECK(hipMalloc(&p, cap * sizeof(double)), "scratch alloc");
+    if (sc.partial) g_scratch_retired.push_back(sc.partial);
+    sc.partial = p;
     sc.cap = cap;
   }
   *out = sc;
