@@ -23,13 +23,13 @@
 //           ratio) a piecewise degree-9 polynomial in y (32 pieces, relative
 //           error 2.4e-20) whose constant and linear coefficients are
 //           double-double; x is clamped at 60, where Q < 2^-2590.
-// Outputs.  Only the out-of-the-money side is evaluated from the tails (the
-// call when d1 < 0, else the put): D = A - B with A = S Q(|d1|) and
-// B = F Q(|d2|) for the call, A = F W and B = S Q(d1) for the put, where
-// W = N(-d2) is Q(d2) or 1 - Q(-d2).  The terms cancel by at most a factor of
-// a few hundred over practical inputs, which the ~2^-62 relative accuracy of
-// each term absorbs.  The other price is D + (F - S) (put-call parity, a term
-// of the same sign for the call side), or D - (F - S) for the put side.
+// Outputs.  Only the smaller price is evaluated from the tails (the call
+// when S < F, else the put): D = A - B with A = S N(d1) and B = F N(d2) for
+// the call, A = F N(-d2) and B = S N(-d1) for the put, each N(.) being
+// Q(|.|) or 1 - Q(|.|).  The terms cancel by at most a factor of a few
+// hundred over practical inputs, which the ~2^-62 relative accuracy of each
+// term absorbs.  The other price is D + (F - S) (put-call parity) on the call
+// side, or D - (F - S) on the put side: a non-negative term either way.
 // Domain: S, X, T, v > 0; NaN inputs give NaN prices.
 #pragma once
 
@@ -174,6 +174,13 @@ __device__ __forceinline__ dd ldexp_dd(dd x, int k) {
   return {__builtin_amdgcn_ldexp(x.h, k), __builtin_amdgcn_ldexp(x.l, k)};
 }
 
+// 1 - q 2^k for a tail q 2^k <= 1/2, as a double-double
+__device__ __forceinline__ dd one_minus(dd q, int k) {
+  const dd x = ldexp_dd(q, k);
+  const dd w = fast_two_sum(1.0, -x.h);
+  return {w.h, w.l - x.l};
+}
+
 // call and put of one option
 __device__ __forceinline__ void bs_price(double S, double X, double T, const Params& P,
                                          const double* __restrict__ gt, const double* __restrict__ ex2h,
@@ -217,23 +224,30 @@ __device__ __forceinline__ void bs_price(double S, double X, double T, const Par
       k2 = kq;
     }
   }
-  // call side (d1 < 0): D = S Q1 - F Q2 = call.
-  // put side (d1 >= 0): D = F W - S Q1 = put, W = Q2 or (d2 < 0) 1 - Q2.
-  const bool call_side = d1 < 0.0;
-  dd W = Q2;
-  int kw = k2;
-  if (!call_side && d2neg) {
-    const dd q2 = ldexp_dd(Q2, k2);
-    const dd w0 = fast_two_sum(1.0, -q2.h);
-    W = dd{w0.h, w0.l - q2.l};
-    kw = 0;
+  // The smaller price is evaluated from the tails (the call when S < F,
+  // else the put) and the other follows by put-call parity, which then adds
+  // a non-negative term.  call = S N(d1) - F N(d2), put = F N(-d2) - S N(-d1),
+  // each N(.) being Q or 1 - Q by the sign of its argument.
+  const bool call_side = F.h > S || (F.h == S && F.l > 0.0);
+  const bool u1 = call_side ? !(d1 < 0.0) : (d1 < 0.0);  // N of a positive argument: 1 - Q
+  const bool u2 = call_side ? !d2neg : d2neg;
+  dd P1 = Q1, P2 = Q2;
+  int k1p = k1, k2p = k2;
+  if (u1) {
+    P1 = one_minus(Q1, k1);
+    k1p = 0;
   }
-  const dd SQ1 = {S * Q1.h, __builtin_fma(S, Q1.h, -(S * Q1.h)) + S * Q1.l};
-  const dd FW = mul_dd(F, W);
-  const dd A = call_side ? SQ1 : FW;
-  const dd B = call_side ? FW : SQ1;
-  const int ka = call_side ? k1 : kw;
-  const int kb = call_side ? kw : k1;
+  if (u2) {
+    P2 = one_minus(Q2, k2);
+    k2p = 0;
+  }
+  const dd SP1 = {S * P1.h, __builtin_fma(S, P1.h, -(S * P1.h)) + S * P1.l};
+  const dd FP2 = mul_dd(F, P2);
+  // D = A - B: the call (call side) or the put (put side)
+  const dd A = call_side ? SP1 : FP2;
+  const dd B = call_side ? FP2 : SP1;
+  const int ka = call_side ? k1p : k2p;
+  const int kb = call_side ? k2p : k1p;
   const dd Bs = ldexp_dd(B, kb - ka);
   const dd Dh = two_sum(A.h, -Bs.h);
   const dd D = ldexp_dd(dd{Dh.h, Dh.l + (A.l - Bs.l)}, ka);

@@ -662,6 +662,49 @@ def test_blackscholes_f64_full_size_sampled():
     torch.cuda.empty_cache()
 
 
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("case", ["tau_low", "tau_0.1_0.3", "tau_up_to_10"])
+def test_blackscholes_accurate_domain(case, dt):
+    """The accurate policy's stated domain, away from the benchmark
+    distribution: every option within 4 ulp of the quad-precision price for
+    v sqrt(T) >= 0.01 (f32) or 0.03 (f64) with |d1| up to ~30 (prices down to
+    the bottom of the type's range), S and X over six decades, and for
+    v sqrt(T) up to 10 (v = 1.5, T to 47 years), where the call can be tiny
+    beside the put while d1 >= 0 (the smaller price is evaluated, the other by
+    parity).  Below that floor the tails' cancellation grows as 1/(v sqrt(T))
+    and so does the error."""
+    rng = np.random.default_rng({"tau_low": 1, "tau_0.1_0.3": 2, "tau_up_to_10": 3}[case])
+    n = 1 << 16
+    S = np.exp(rng.uniform(np.log(1e-2), np.log(1e4), n))
+    if case == "tau_up_to_10":
+        v, r = 1.5, -0.03
+        T = np.exp(rng.uniform(np.log(1e-2), np.log(47.0), n))
+        X = np.exp(rng.uniform(np.log(1e-2), np.log(1e4), n))
+    else:
+        v, r = 0.3, 0.02
+        if case == "tau_low":
+            lo, hi = (0.01, 0.03) if dt == np.float32 else (0.03, 0.09)
+        else:
+            lo, hi = 0.1, 0.3
+        tau = np.exp(rng.uniform(np.log(lo), np.log(hi), n))
+        T = (tau / v) ** 2
+        X = S * np.exp(rng.uniform(-30.0, 30.0, n) * tau)
+    s, x, t = (a.astype(dt) for a in (S, X, T))
+    say("bs accurate domain", case, dt.__name__)
+    c, p = ops.blackscholes(torch.from_numpy(s).cuda(), torch.from_numpy(x).cuda(), torch.from_numpy(t).cuda(),
+                            r, v, policy=ACCURATE)
+    rr, vv = (float(np.float32(r)), float(np.float32(v))) if dt == np.float32 else (r, v)
+    ch, cl, ph, pl = O.bs_quad(s.astype(np.float64), x.astype(np.float64), t.astype(np.float64), rr, vv)
+    for name, got, hi_, lo_ in (("call", c, ch, cl), ("put", p, ph, pl)):
+        g = got.cpu().numpy()
+        if dt == np.float32:
+            e = classic_ulp(g, (hi_ + lo_).astype(np.float32).astype(np.float64), np.float32)
+        else:
+            e = dd_ulp(g, hi_, lo_)
+        assert np.all(np.isfinite(g)), name
+        assert e.max() <= 4.0, (case, name, float(e.max()), int(np.argmax(e)))
+
+
 @pytest.mark.parametrize("dt", [torch.float32, torch.float64])
 def test_blackscholes_accurate_neighbour_independent(dt):
     """An option's accurate price depends on its own inputs only: pricing a
