@@ -26,8 +26,11 @@
 // Outputs.  Only the out-of-the-money side is evaluated from the tails (the
 // call when d1 < 0, else the put); the other price follows from put-call
 // parity, C - P = S - F, which adds a term of the same sign (d1 < 0 implies
-// F > S) or, for d1 >= 0 > d2, cancels by at most a factor of ~2.  On MI355X an FP64 FMA issues at the rate of an FP32 one,
-// which is what makes the FP64 evaluation affordable.
+// F > S) or, for d1 >= 0 > d2, cancels: by at most a factor of ~2 near the
+// money over the benchmark inputs, and by up to ~1e6 at v sqrt(T) ~ 10 where
+// the call is tiny beside the put, which the FP64 intermediates still absorb
+// for a float result.  On MI355X an FP64 FMA issues at the rate of an FP32
+// one, which is what makes the FP64 evaluation affordable.
 // Error budget of the f32 result: final rounding 0.5 ulp plus ~170 x 1.6e-10
 // from the approximations at the worst cancellation (0.4 ulp).
 #pragma once
