@@ -1669,7 +1669,7 @@ NT2H_FN void reduce_d(const double* x, double* xr, int* n, bool* bail = nullptr)
       double r = x[i] - xi * NT2H_D(0x3FF921FB54400000);
       r -= xi * NT2H_D(0x3DD0B4611A600000);
       r -= xi * NT2H_D(0x3BA3198A2E000000);
-      xr[i] = r; n[i] = ((int)(int64_t)xi) & 3;
+      xr[i] = r; n[i] = ((int)xi) & 3;  // |xi| <= 13: the int32 conversion gives NT2's int64 quadrant bits
     }
   } else if (allm) {
 #pragma unroll
@@ -1687,7 +1687,7 @@ NT2H_FN void reduce_d(const double* x, double* xr, int* n, bool* bail = nullptr)
       r = t2 - w;
       w = fn * NT2H_D(0x397B839A252049C1) - ((t2 - r) - w);
       xr[i] = r - w;
-      n[i] = ((int)(int64_t)fn) & 3;
+      n[i] = ((int)fn) & 3;  // |fn| <= 2^19: as NT2's int64 conversion, in one instruction
     }
   } else if constexpr (BAIL) {
     *bail = true;
