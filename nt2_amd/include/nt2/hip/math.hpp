@@ -114,7 +114,7 @@ NT2H_FN double exp_d(double a0) {
   double c = fnms<P>(t, horner<P>(cExp, t), x);
   c = 1.0 - ((lo - (x * c) / (2.0 - c)) - hi);
   const double kc = __builtin_fmin(__builtin_fmax(k, -2000.0), 2000.0);
-  c = c * as_d((uint64_t)((int64_t)kc + 1023) << 52);
+  c = c * as_d((uint64_t)((int64_t)(int32_t)kc + 1023) << 52);  // |kc| <= 2000: an int32 conversion
   c = (a0 <= NT2H_D(0xc086232bdd7abcd2)) ? 0.0 : c;
   c = (a0 >= NT2H_D(0x40862e42fefa39ef)) ? __builtin_inf() : c;
   return (a0 != a0) ? a0 : c;
@@ -197,7 +197,7 @@ NT2H_FN float expa_finish_f(float a0, float r, float k, float minlog, float maxl
 NT2H_FN double expa_finish_d(double a0, double c, double k, double minlog, double maxlog) {
   NT2H_STRICT
   const double kc = __builtin_fmin(__builtin_fmax(k, -2000.0), 2000.0);
-  c = c * as_d((uint64_t)((int64_t)kc + 1023) << 52);
+  c = c * as_d((uint64_t)((int64_t)(int32_t)kc + 1023) << 52);  // |kc| <= 2000: an int32 conversion
   c = (a0 <= minlog) ? 0.0 : c;
   c = (a0 >= maxlog) ? __builtin_inf() : c;
   return (a0 != a0) ? a0 : c;
@@ -428,7 +428,7 @@ NT2H_FN double expm1_d(double a0) {
   double e = hxs * ((r1 - t) / (6.0 - x * t));
   const double cc = (hi - x) - lo;
   e = (x * (e - cc) - cc) - hxs;
-  const int64_t ik = (k == k) ? (int64_t)__builtin_fmin(__builtin_fmax(k, -2000.0), 2000.0) : 0;
+  const int64_t ik = (k == k) ? (int64_t)(int32_t)__builtin_fmin(__builtin_fmax(k, -2000.0), 2000.0) : 0;
   const double two2mk = as_d((uint64_t)(1023 - ik) << 52);
   const double ct1 = (1.0 - two2mk) - (e - x);
   const double ct2 = 1.0 + (x - (e + two2mk));
