@@ -91,6 +91,34 @@ extern "C" {
 #define NT2H_OP_COSD 41
 #define NT2H_OP_TAND 42
 #define NT2H_OP_COTD 43
+/* Compositions, NT2's definitions over the kernels above (NT2's scalar form
+ * for the tail where the inner function or the definition has one): */
+#define NT2H_OP_COTH 44   /* hyperbolic/functions/simd/common/coth.hpp:35-63 */
+#define NT2H_OP_SECH 45   /* 1/cosh */
+#define NT2H_OP_CSCH 46   /* 1/sinh */
+#define NT2H_OP_ACOTH 47  /* atanh(1/x) */
+#define NT2H_OP_ASECH 48  /* acosh(1/x) */
+#define NT2H_OP_ACSCH 49  /* asinh(1/x) */
+#define NT2H_OP_ACOT 50   /* atan(1/x) */
+#define NT2H_OP_ASEC 51   /* pi/2 - acsc(x); scalar forms: trigonometric/functions/scalar/asec.hpp:26-67 */
+#define NT2H_OP_ACSC 52   /* asin(1/x) */
+#define NT2H_OP_ACOTD 53  /* degrees: 90 - atand(|x|), sign of x */
+#define NT2H_OP_ASECD 54
+#define NT2H_OP_ACSCD 55
+#define NT2H_OP_ACOSD 56
+#define NT2H_OP_ACOSPI 57 /* units of pi: acos(x)/pi */
+#define NT2H_OP_ASIND 58
+#define NT2H_OP_ASINPI 59
+#define NT2H_OP_ATAND 60
+#define NT2H_OP_ATANPI 61
+#define NT2H_OP_INDEG 62  /* radians to degrees */
+#define NT2H_OP_INRAD 63  /* degrees to radians */
+#define NT2H_OP_SECD 64   /* 1/cosd, NaN at its zeros */
+#define NT2H_OP_CSCD 65   /* cscpi(x/180) */
+#define NT2H_OP_SECPI 66  /* 1/cospi, NaN at half-integers */
+#define NT2H_OP_CSCPI 67  /* 1/sinpi, NaN at nonzero integers */
+#define NT2H_OP_SINC 68   /* sin(x)/x, radian: per-pack range reduction */
+#define NT2H_OP_SINCPI 69 /* sinpi(x)/(pi x) */
 
 /* Black-Scholes formula variants */
 #define NT2H_BS_TEXTBOOK 0 /* d1 = (log(S/X)+(r+v^2/2)T)/(v sqrt T), call and put, erfc-based normcdf */
@@ -141,6 +169,7 @@ int nt2h_map_f64(int op, size_t n, const double* x, double* y, int policy, void*
  * pow_abs.hpp:51-111 for the pack tail). */
 #define NT2H_OP2_ATAN2 0
 #define NT2H_OP2_POW 1
+#define NT2H_OP2_ATAN2D 2 /* atan2 in degrees: trigonometric/functions/generic/atan2d.hpp:16-27 */
 int nt2h_map2_f32(int op, size_t n, const float* x, const float* y, float* out, int policy, void* stream);
 int nt2h_map2_f64(int op, size_t n, const double* x, const double* y, double* out, int policy, void* stream);
 /* [s, c] = sincos(x): modules/core/trigonometric/include/nt2/trigonometric/functions/generic/sincos.hpp:50-64 */

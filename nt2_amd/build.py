@@ -21,7 +21,8 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contra
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-I" + os.path.join(HERE, "include"), "-I" + os.path.join(ROOT, "include")]
 
-SOURCES = ["nt2h_core.hip", "nt2h_batch.hip", "nt2h_server.hip", "nt2h_dimwise.hip", "nt2h_demos.hip"]
+SOURCES = ["nt2h_core.hip", "nt2h_batch.hip", "nt2h_server.hip", "nt2h_dimwise.hip", "nt2h_demos.hip",
+           "nt2h_compose.hip"]
 
 
 def _run(cmd):

@@ -12,6 +12,8 @@
 
 #include "bs_device.hpp"
 #include "common.hpp"
+#include "map_packs.hpp"
+#include "nt2/hip/math_compose.hpp"
 #include "nt2/hip/bs_accurate.hpp"
 #include "nt2/hip/bs_accurate64.hpp"
 #include "nt2/hip/math.hpp"
@@ -256,69 +258,23 @@ __device__ __forceinline__ void apply_f64(const double* x, double* y) {
   }
 }
 
+template <class P, int OP> struct apply32 {
+  template <int G> __device__ __forceinline__ static void run(const float* a, float* r) { apply_f32<P, OP, G>(a, r); }
+};
+template <class P, int OP> struct apply64 {
+  template <int G> __device__ __forceinline__ static void run(const double* a, double* r) { apply_f64<P, OP, G>(a, r); }
+};
+
 template <class P, int OP>
 __global__ __launch_bounds__(kBlock) void map_f32_k(const float* __restrict__ x,
                                                     float* __restrict__ y, size_t n, int vec) {
-  const size_t npacks = n / 8;
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < npacks) {
-    float a[8], r[8];
-    const size_t i = t * 8;
-    if (vec) {
-      const float4 v0 = *reinterpret_cast<const float4*>(x + i);
-      const float4 v1 = *reinterpret_cast<const float4*>(x + i + 4);
-      a[0] = v0.x; a[1] = v0.y; a[2] = v0.z; a[3] = v0.w;
-      a[4] = v1.x; a[5] = v1.y; a[6] = v1.z; a[7] = v1.w;
-    } else {
-#pragma unroll
-      for (int l = 0; l < 8; ++l) a[l] = x[i + l];
-    }
-    apply_f32<P, OP, 8>(a, r);
-    if (vec) {
-      *reinterpret_cast<float4*>(y + i) = make_float4(r[0], r[1], r[2], r[3]);
-      *reinterpret_cast<float4*>(y + i + 4) = make_float4(r[4], r[5], r[6], r[7]);
-    } else {
-#pragma unroll
-      for (int l = 0; l < 8; ++l) y[i + l] = r[l];
-    }
-  } else if (t < npacks + (n - npacks * 8)) {
-    const size_t i = npacks * 8 + (t - npacks);
-    float a = x[i], r;
-    apply_f32<P, OP, 1>(&a, &r);
-    y[i] = r;
-  }
+  map_packs<apply32<P, OP>, float, 8>(x, y, n, vec);
 }
 
 template <class P, int OP>
 __global__ __launch_bounds__(kBlock) void map_f64_k(const double* __restrict__ x,
                                                     double* __restrict__ y, size_t n, int vec) {
-  const size_t npacks = n / 4;
-  const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < npacks) {
-    double a[4], r[4];
-    const size_t i = t * 4;
-    if (vec) {
-      const double2 v0 = *reinterpret_cast<const double2*>(x + i);
-      const double2 v1 = *reinterpret_cast<const double2*>(x + i + 2);
-      a[0] = v0.x; a[1] = v0.y; a[2] = v1.x; a[3] = v1.y;
-    } else {
-#pragma unroll
-      for (int l = 0; l < 4; ++l) a[l] = x[i + l];
-    }
-    apply_f64<P, OP, 4>(a, r);
-    if (vec) {
-      *reinterpret_cast<double2*>(y + i) = make_double2(r[0], r[1]);
-      *reinterpret_cast<double2*>(y + i + 2) = make_double2(r[2], r[3]);
-    } else {
-#pragma unroll
-      for (int l = 0; l < 4; ++l) y[i + l] = r[l];
-    }
-  } else if (t < npacks + (n - npacks * 4)) {
-    const size_t i = npacks * 4 + (t - npacks);
-    double a = x[i], r;
-    apply_f64<P, OP, 1>(&a, &r);
-    y[i] = r;
-  }
+  map_packs<apply64<P, OP>, double, 4>(x, y, n, vec);
 }
 
 // binary maps: out = f(x, y), packs of W from index 0 (f(pack) form), then
@@ -327,6 +283,7 @@ template <class P, int OP, class T, int G>
 __device__ __forceinline__ T apply2_one(T a, T b) {
   constexpr bool scalar = (G == 1);
   if constexpr (OP == NT2H_OP2_POW) return dm::pow_t<P, scalar, T>(a, b);
+  else if constexpr (OP == NT2H_OP2_ATAN2D) return dm::atan2d_t<P, scalar>(a, b);
   else if constexpr (sizeof(T) == 4) return dm::atan2_f<P, scalar>(a, b);
   else return dm::atan2_d<P, scalar>(a, b);
 }
@@ -1171,6 +1128,7 @@ int nt2h_fill_frames_u8(uint8_t* frames, int h, int w, int nframes, int nrect, i
 int nt2h_map_f32(int op, size_t n, const float* x, float* y, int policy, void* stream) {
   if (n == 0) return NT2H_OK;
   if (!x || !y) return fail(NT2H_EINVAL, "nt2h_map_f32: null pointer");
+  if (op >= NT2H_OP_COTH) return launch_compose(op, n, x, y, policy, as_stream(stream));
   if (policy == NT2H_EXACT) return launch_map_f32<exact_policy>(op, n, x, y, as_stream(stream));
   if (policy == NT2H_FAST) return launch_map_f32<fast_policy>(op, n, x, y, as_stream(stream));
   return fail(NT2H_EINVAL, "nt2h_map_f32: bad policy");
@@ -1179,6 +1137,7 @@ int nt2h_map_f32(int op, size_t n, const float* x, float* y, int policy, void* s
 int nt2h_map_f64(int op, size_t n, const double* x, double* y, int policy, void* stream) {
   if (n == 0) return NT2H_OK;
   if (!x || !y) return fail(NT2H_EINVAL, "nt2h_map_f64: null pointer");
+  if (op >= NT2H_OP_COTH) return launch_compose(op, n, x, y, policy, as_stream(stream));
   if (policy == NT2H_EXACT) return launch_map_f64<exact_policy>(op, n, x, y, as_stream(stream));
   if (policy == NT2H_FAST) return launch_map_f64<fast_policy>(op, n, x, y, as_stream(stream));
   return fail(NT2H_EINVAL, "nt2h_map_f64: bad policy");
@@ -1189,13 +1148,16 @@ int launch_map2(const char* name, int op, size_t n, const T* x, const T* y, T* o
                        void* stream) {
   if (n == 0) return NT2H_OK;
   if (!x || !y || !out) return fail(NT2H_EINVAL, name);
-  if (op != NT2H_OP2_ATAN2 && op != NT2H_OP2_POW) return fail(NT2H_EINVAL, name);
+  if (op != NT2H_OP2_ATAN2 && op != NT2H_OP2_POW && op != NT2H_OP2_ATAN2D) return fail(NT2H_EINVAL, name);
   if (policy != NT2H_EXACT && policy != NT2H_FAST) return fail(NT2H_EINVAL, name);
   const dim3 g(grid_for(n / W + n % W)), b(kBlock);
   hipStream_t s = as_stream(stream);
   if (op == NT2H_OP2_ATAN2) {
     if (policy == NT2H_EXACT) hipLaunchKernelGGL((map2_k<exact_policy, NT2H_OP2_ATAN2, T, W>), g, b, 0, s, x, y, out, n);
     else hipLaunchKernelGGL((map2_k<fast_policy, NT2H_OP2_ATAN2, T, W>), g, b, 0, s, x, y, out, n);
+  } else if (op == NT2H_OP2_ATAN2D) {
+    if (policy == NT2H_EXACT) hipLaunchKernelGGL((map2_k<exact_policy, NT2H_OP2_ATAN2D, T, W>), g, b, 0, s, x, y, out, n);
+    else hipLaunchKernelGGL((map2_k<fast_policy, NT2H_OP2_ATAN2D, T, W>), g, b, 0, s, x, y, out, n);
   } else {
     if (policy == NT2H_EXACT) hipLaunchKernelGGL((map2_k<exact_policy, NT2H_OP2_POW, T, W>), g, b, 0, s, x, y, out, n);
     else hipLaunchKernelGGL((map2_k<fast_policy, NT2H_OP2_POW, T, W>), g, b, 0, s, x, y, out, n);

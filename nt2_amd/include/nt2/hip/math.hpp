@@ -463,16 +463,32 @@ NT2H_FN double log1p_d(double a0) {
 // polynomial, :44-71 f64 rational) on |a0|, else 1 - 2/(1 + exp(2|a0|)); the
 // sign of a0 is xored in.  The choice is per lane, so a lane's bits do not
 // depend on its pack.
+// the kernels on |x| < 5/8 (x2 = x*x): tanh_kernel<float>::tanh, and the f64
+// rational's numerator x2 p(x2) and denominator q(x2), shared with coth
 template <class P>
-NT2H_FN float tanh_f(float a0) {
+NT2H_FN float tanh_small_f(float x, float x2) {
   NT2H_STRICT
   constexpr float c[5] = {NT2H_F(0xbbbaf0ea), NT2H_F(0x3ca9134e), NT2H_F(0xbd5c1e2d),
                           NT2H_F(0x3e088393), NT2H_F(0xbeaaaa99)};
+  return fma<P>(horner<P>(c, x2) * x2, x, x);
+}
+template <class P>
+NT2H_FN void tanh_pq_d(double x2, double& p_, double& q_) {
+  NT2H_STRICT
+  constexpr double p[3] = {NT2H_D(0xbfeedc5baafd6f4b), NT2H_D(0xc058d26a0e26682d),
+                           NT2H_D(0xc0993ac030580563)};
+  constexpr double q[4] = {NT2H_D(0x3FF0000000000000), NT2H_D(0x405c33f28a581b86),
+                           NT2H_D(0x40a176fa0e5535fa), NT2H_D(0x40b2ec102442040c)};
+  p_ = horner<P>(p, x2);
+  q_ = horner<P>(q, x2);
+}
+template <class P>
+NT2H_FN float tanh_f(float a0) {
+  NT2H_STRICT
   const float x = __builtin_fabsf(a0);
   float r;
   if (x < 0.625f) {
-    const float x2 = x * x;
-    r = fma<P>(horner<P>(c, x2) * x2, x, x);
+    r = tanh_small_f<P>(x, x * x);
   } else {
     r = fma<P>(-2.0f, 1.0f / (1.0f + exp_f<P>(x + x)), 1.0f);
   }
@@ -481,15 +497,13 @@ NT2H_FN float tanh_f(float a0) {
 template <class P>
 NT2H_FN double tanh_d(double a0) {
   NT2H_STRICT
-  constexpr double p[3] = {NT2H_D(0xbfeedc5baafd6f4b), NT2H_D(0xc058d26a0e26682d),
-                           NT2H_D(0xc0993ac030580563)};
-  constexpr double q[4] = {NT2H_D(0x3FF0000000000000), NT2H_D(0x405c33f28a581b86),
-                           NT2H_D(0x40a176fa0e5535fa), NT2H_D(0x40b2ec102442040c)};
   const double x = __builtin_fabs(a0);
   double r;
   if (x < 0.625) {
     const double x2 = x * x;
-    r = fma<P>((x2 * horner<P>(p, x2)) / horner<P>(q, x2), x, x);
+    double p, q;
+    tanh_pq_d<P>(x2, p, q);
+    r = fma<P>((x2 * p) / q, x, x);
   } else {
     r = fma<P>(-2.0, 1.0 / (1.0 + exp_d<P>(x + x)), 1.0);
   }

@@ -1,0 +1,219 @@
+// NT2's composed elementwise functions on the device: the reciprocal
+// hyperbolic functions and their inverses, the inverse trigonometric
+// functions of reciprocals, results in degrees or in units of pi, the
+// reciprocal trigonometric functions of degree and pi arguments, and sinc.
+// Each follows NT2's own definition (file:line below), whose operation order
+// fixes the bits.  P is the precision policy of the inner kernels; the outer
+// divisions, products and differences are single IEEE operations under both
+// policies.  S selects NT2's scalar form (the tail of a table) where the
+// inner kernel or the definition has one.
+#pragma once
+
+#include "math.hpp"
+
+namespace nt2 { namespace hip { namespace dm {
+
+// ---- type dispatch onto the float / double kernels ---------------------------
+template <class P, bool S> NT2H_FN float atan_t(float x) { return atan_f<P, S>(x); }
+template <class P, bool S> NT2H_FN double atan_t(double x) { return atan_d<P, S>(x); }
+template <class P, bool S> NT2H_FN float asin_t(float x) { return asin_f<P>(x); }
+template <class P, bool S> NT2H_FN double asin_t(double x) { return asin_d<P, S>(x); }
+template <class P, bool S> NT2H_FN float acos_t(float x) { return acos_f<P>(x); }
+template <class P, bool S> NT2H_FN double acos_t(double x) { return acos_d<P, S>(x); }
+template <class P, bool S> NT2H_FN float asinh_t(float x) { return asinh_f<P, S>(x); }
+template <class P, bool S> NT2H_FN double asinh_t(double x) { return asinh_d<P, S>(x); }
+template <class P, bool S> NT2H_FN float acosh_t(float x) { return acosh_f<P, S>(x); }
+template <class P, bool S> NT2H_FN double acosh_t(double x) { return acosh_d<P, S>(x); }
+template <class P> NT2H_FN float atanh_t(float x) { return atanh_f<P>(x); }
+template <class P> NT2H_FN double atanh_t(double x) { return atanh_d<P>(x); }
+template <class P> NT2H_FN float sinh_t(float x) { return sinh_f<P>(x); }
+template <class P> NT2H_FN double sinh_t(double x) { return sinh_d<P>(x); }
+template <class P> NT2H_FN float cosh_t(float x) { return cosh_f<P>(x); }
+template <class P> NT2H_FN double cosh_t(double x) { return cosh_d<P>(x); }
+template <class P> NT2H_FN float exp_t(float x) { return exp_f<P>(x); }
+template <class P> NT2H_FN double exp_t(double x) { return exp_d<P>(x); }
+template <class P, int U> NT2H_FN float trigu_t(int kind, float x) { return trigu_f<P, U>(kind, x); }
+template <class P, int U> NT2H_FN double trigu_t(int kind, double x) { return trigu_d<P, U>(kind, x); }
+template <class P, bool S> NT2H_FN float atan2_tt(float a, float b) { return atan2_f<P, S>(a, b); }
+template <class P, bool S> NT2H_FN double atan2_tt(double a, double b) { return atan2_d<P, S>(a, b); }
+
+// NT2's constants, by their float and double bit patterns
+template <class T> NT2H_FN T cst(uint32_t f, uint64_t d) {
+  if constexpr (sizeof(T) == 4) return as_f(f);
+  else return as_d(d);
+}
+template <class T> NT2H_FN T c_pio2() { return cst<T>(0x3fc90fdbu, 0x3ff921fb54442d18ull); }
+template <class T> NT2H_FN T c_pi() { return cst<T>(0x40490fdbu, 0x400921fb54442d18ull); }
+template <class T> NT2H_FN T c_invpi() { return cst<T>(0x3ea2f983u, 0x3fd45f306dc9c883ull); }
+template <class T> NT2H_FN T c_oneo180() { return cst<T>(0x3bb60b61u, 0x3f76c16c16c16c17ull); }
+template <class T> NT2H_FN T c_eps() { return cst<T>(0x34000000u, 0x3cb0000000000000ull); }
+template <class T> NT2H_FN T c_nan() { return cst<T>(0x7fc00000u, 0x7ff8000000000000ull); }
+
+template <class T> NT2H_FN T absv(T x) {
+  if constexpr (sizeof(T) == 4) return __builtin_fabsf(x);
+  else return __builtin_fabs(x);
+}
+
+// r with a0's sign bit xored in (b_xor(r, bitofsign(a0))) or or-ed in (b_or)
+template <class T> NT2H_FN T xor_sign(T r, T a0) {
+  if constexpr (sizeof(T) == 4) return as_f(bits(r) ^ (bits(a0) & 0x80000000u));
+  else return as_d(bits(r) ^ (bits(a0) & 0x8000000000000000ull));
+}
+template <class T> NT2H_FN T or_sign(T r, T a0) {
+  if constexpr (sizeof(T) == 4) return as_f(bits(r) | (bits(a0) & 0x80000000u));
+  else return as_d(bits(r) | (bits(a0) & 0x8000000000000000ull));
+}
+
+// indeg: trigonometric/functions/generic/indeg.hpp:20-30, a0*Radindeg -
+// a0*Radindegr (constants/radindeg.hpp:29-30, radindegr.hpp:28-29);
+// inrad: generic/inrad.hpp:17-29, a0*Deginrad (constants/deginrad.hpp:29-30)
+template <class T> NT2H_FN T indeg(T a0) {
+  NT2H_STRICT
+  return a0 * cst<T>(0x42652ee1u, 0x404ca5dc1a63c1f8ull) - a0 * cst<T>(0xb53387c1u, 0xbce1e7ab456405f8ull);
+}
+template <class T> NT2H_FN T inrad(T a0) {
+  NT2H_STRICT
+  return a0 * cst<T>(0x3c8efa35u, 0x3f91df46a2529d39ull);
+}
+
+// ---- hyperbolic ----------------------------------------------------------------
+// coth: hyperbolic/functions/simd/common/coth.hpp:35-63 (scalar/coth.hpp:29-55
+// gives the same value per lane): |a0| < 5/8 by tanh_kernel's coth
+// (details/tanh_kernel.hpp:40-43 f32: rec of the tanh polynomial; :55-59 f64:
+// q/(x fma(p, x2, q))), else fma(2, rec(exp(2|a0|) - 1), 1); a0's sign xored in.
+template <class P, class T> NT2H_FN T coth_t(T a0) {
+  NT2H_STRICT
+  const T x = absv(a0);
+  T r;
+  if (x < T(0.625)) {
+    const T x2 = x * x;
+    if constexpr (sizeof(T) == 4) {
+      r = 1.0f / tanh_small_f<P>(x, x2);
+    } else {
+      double p, q;
+      tanh_pq_d<P>(x2, p, q);
+      r = q / (x * fma<P>(p, x2, q));
+    }
+  } else {
+    r = fma<P>(T(2), T(1) / (exp_t<P>(x + x) - T(1)), T(1));
+  }
+  return xor_sign(r, a0);
+}
+// sech: generic/sech.hpp:19-29, rec(cosh); csch: generic/csch.hpp:18-30, rec(sinh)
+template <class P, class T> NT2H_FN T sech_t(T a0) { NT2H_STRICT return T(1) / cosh_t<P>(a0); }
+template <class P, class T> NT2H_FN T csch_t(T a0) { NT2H_STRICT return T(1) / sinh_t<P>(a0); }
+// acoth: generic/acoth.hpp:18-28, atanh(rec); asech: generic/asech.hpp:18-28,
+// acosh(rec); acsch: generic/acsch.hpp:18-28, asinh(rec)
+template <class P, class T> NT2H_FN T acoth_t(T a0) { NT2H_STRICT return atanh_t<P>(T(1) / a0); }
+template <class P, bool S, class T> NT2H_FN T asech_t(T a0) { NT2H_STRICT return acosh_t<P, S>(T(1) / a0); }
+template <class P, bool S, class T> NT2H_FN T acsch_t(T a0) { NT2H_STRICT return asinh_t<P, S>(T(1) / a0); }
+
+// ---- inverse trigonometric -------------------------------------------------------
+// acot: trigonometric/functions/generic/acot.hpp:18-28, atan(rec);
+// acsc: generic/acsc.hpp:17-27, asin(rec)
+template <class P, bool S, class T> NT2H_FN T acot_t(T a0) { NT2H_STRICT return atan_t<P, S>(T(1) / a0); }
+template <class P, bool S, class T> NT2H_FN T acsc_t(T a0) { NT2H_STRICT return asin_t<P, S>(T(1) / a0); }
+// asec: the pack form simd/common/asec.hpp:18-30 is Pio_2 - acsc(a0).  The
+// scalar forms: double, scalar/asec.hpp:26-38, 0 at 1, else
+// (Pio_2 - acsc(a0)) + 6.123e-17 (Pio_2's residue); float, :40-67, NaN below
+// |a0| = 1, the series sqrt(2e)(1 - e(5/12 + e(43/160 - e(177/896 +
+// e 2867/18432)))) in e = |a0| - 1 < 0.001 (pi minus it for a0 < 0), 0 at 1,
+// else Pio_2 - acsc(a0).
+template <class P, bool S, class T> NT2H_FN T asec_t(T a0) {
+  NT2H_STRICT
+  if constexpr (!S) {
+    return c_pio2<T>() - acsc_t<P, S>(a0);
+  } else if constexpr (sizeof(T) == 8) {
+    if (a0 == 1.0) return 0.0;
+    return (c_pio2<T>() - acsc_t<P, S>(a0)) + NT2H_D(0x3c91a62633145c07);
+  } else {
+    const float ax = __builtin_fabsf(a0);
+    if (ax < 1.0f) return c_nan<float>();
+    const float ax1 = ax - 1.0f;
+    if (ax1 < 0.001f) {
+      const float tmp = __builtin_sqrtf(2 * ax1) *
+                        (1 - ax1 * ((5 / 12.0f) + ax1 * ((43 / 160.0f) - ax1 * ((177 / 896.0f) + ax1 * (2867 / 18432.0f)))));
+      return a0 < 0.0f ? c_pi<float>() - tmp : tmp;
+    }
+    if (a0 == 1.0f) return 0.0f;
+    return c_pio2<float>() - acsc_t<P, S>(a0);
+  }
+}
+// atand: generic/atand.hpp:17-27, indeg(atan); asind, acosd: generic/asind.hpp:18-28,
+// acosd.hpp:18-28, indeg(asin), indeg(acos); asecd, acscd: generic/asecd.hpp:18-28,
+// acscd.hpp:17-27, indeg(asec), indeg(acsc)
+template <class P, bool S, class T> NT2H_FN T atand_t(T a0) { return indeg(atan_t<P, S>(a0)); }
+template <class P, bool S, class T> NT2H_FN T asind_t(T a0) { return indeg(asin_t<P, S>(a0)); }
+template <class P, bool S, class T> NT2H_FN T acosd_t(T a0) { return indeg(acos_t<P, S>(a0)); }
+template <class P, bool S, class T> NT2H_FN T asecd_t(T a0) { return indeg(asec_t<P, S>(a0)); }
+template <class P, bool S, class T> NT2H_FN T acscd_t(T a0) { return indeg(acsc_t<P, S>(a0)); }
+// acotd: generic/acotd.hpp:30-44, 90 - atand(|a0|) (0 subtracted at a0 = 0),
+// 0 at +-Inf, a0's sign bit or-ed in
+template <class P, bool S, class T> NT2H_FN T acotd_t(T a0) {
+  NT2H_STRICT
+  const T d = atand_t<P, S>(absv(a0));
+  T z = T(90) - (a0 != T(0) ? d : T(0));
+  z = __builtin_isinf(a0) ? T(0) : z;
+  return or_sign(z, a0);
+}
+// atanpi, asinpi, acospi: generic/atanpi.hpp:18-28, asinpi.hpp:18-30,
+// acospi.hpp:19-29, Invpi * f(a0)
+template <class P, bool S, class T> NT2H_FN T atanpi_t(T a0) { NT2H_STRICT return c_invpi<T>() * atan_t<P, S>(a0); }
+template <class P, bool S, class T> NT2H_FN T asinpi_t(T a0) { NT2H_STRICT return c_invpi<T>() * asin_t<P, S>(a0); }
+template <class P, bool S, class T> NT2H_FN T acospi_t(T a0) { NT2H_STRICT return c_invpi<T>() * acos_t<P, S>(a0); }
+// atan2d: generic/atan2d.hpp:16-27, indeg(atan2)
+template <class P, bool S, class T> NT2H_FN T atan2d_t(T a0, T a1) { return indeg(atan2_tt<P, S>(a0, a1)); }
+
+// ---- reciprocal trigonometric functions of degree and pi arguments ---------------
+// secd: generic/secd.hpp:20-31, rec of cosd with its zeros made NaN
+template <class P, class T> NT2H_FN T secd_t(T a0) {
+  NT2H_STRICT
+  const T r = trigu_t<P, u_deg>(k_cos, a0);
+  return T(1) / (r == T(0) ? c_nan<T>() : r);
+}
+// cscpi: generic/cscpi.hpp:22-34, NaN at nonzero integers, else rec(sinpi)
+template <class P, class T> NT2H_FN T cscpi_t(T a0) {
+  NT2H_STRICT
+  const T r = T(1) / trigu_t<P, u_pi>(k_sin, a0);
+  return (is_flint(a0) && a0 != T(0)) ? c_nan<T>() : r;
+}
+// cscd: generic/cscd.hpp:18-28, cscpi(Oneo_180 * a0)
+template <class P, class T> NT2H_FN T cscd_t(T a0) { NT2H_STRICT return cscpi_t<P>(c_oneo180<T>() * a0); }
+// secpi: generic/secpi.hpp:22-32, NaN where 2 a0 is odd (is_odd of a float:
+// boost/simd/predicates/functions/simd/common/is_odd.hpp:47, is_even(x - 1)
+// = is_flint((x - 1)/2), is_even.hpp:43), else rec(cospi)
+template <class P, class T> NT2H_FN T secpi_t(T a0) {
+  NT2H_STRICT
+  const T r = T(1) / trigu_t<P, u_pi>(k_cos, a0);
+  return is_flint((T(2) * a0 - T(1)) * T(0.5)) ? c_nan<T>() : r;
+}
+
+// ---- sinc ----------------------------------------------------------------------
+// sinc: simd/common/sinc.hpp:33-58, sin(a0)/a0 over the pack (sin's per-pack
+// reduction), 1 where |a0| < Eps, 0 at +-Inf; the scalar form
+// (scalar/sinc.hpp:28-49) gives the same per lane with the scalar sin.
+template <class P, int G, class T> NT2H_FN void sinc_group(const T* a0, T* r) {
+  NT2H_STRICT
+  T s[G];
+  if constexpr (sizeof(T) == 4) trig_group_f<P, G>(k_sin, a0, s, nullptr);
+  else trig_group_d<P, G>(k_sin, a0, s, nullptr);
+#pragma unroll
+  for (int l = 0; l < G; ++l) {
+    const T x = a0[l];
+    T v = s[l] / x;
+    v = absv(x) < c_eps<T>() ? T(1) : v;
+    r[l] = __builtin_isinf(x) ? T(0) : v;
+  }
+}
+// sincpi: the pack form simd/common/sincpi.hpp:35-60 is Invpi*(sinpi(a0)/a0),
+// the scalar form scalar/sincpi.hpp:29-49 (Invpi*sinpi(a0))/a0; both 1 where
+// |a0| < Eps and 0 at +-Inf
+template <class P, bool S, class T> NT2H_FN T sincpi_t(T a0) {
+  NT2H_STRICT
+  const T sp = trigu_t<P, u_pi>(k_sin, a0);
+  T v = S ? (c_invpi<T>() * sp) / a0 : c_invpi<T>() * (sp / a0);
+  v = absv(a0) < c_eps<T>() ? T(1) : v;
+  return __builtin_isinf(a0) ? T(0) : v;
+}
+
+}}}  // namespace nt2::hip::dm

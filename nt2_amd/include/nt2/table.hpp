@@ -54,6 +54,7 @@
 
 #include "nt2/hip/dimwise.hpp"
 #include "nt2/hip/math.hpp"
+#include "nt2/hip/math_compose.hpp"
 
 #ifndef NT2_HIP_POLICY
 #define NT2_HIP_POLICY nt2::hip::exact_policy
@@ -184,6 +185,8 @@ template <class X> struct common<X, bool> { typedef X type; };
 template <class Y> struct common<bool, Y> { typedef Y type; };
 
 struct same1 { template <class A> struct result { typedef A type; }; };
+// a unary function whose pack form and scalar (tail) form may differ
+struct same1p : same1 {};
 struct same2 { template <class A, class B> struct result { typedef typename common<A, B>::type type; }; };
 struct logical2 { template <class A, class B> struct result { typedef bool type; }; };
 struct sel3 { template <class C, class A, class B> struct result { typedef typename common<A, B>::type type; }; };
@@ -214,6 +217,13 @@ NT2H_TAG(erf_, same1) NT2H_TAG(cbrt_, same1) NT2H_TAG(asinh_, same1) NT2H_TAG(ac
 NT2H_TAG(atanh_, same1) NT2H_TAG(cot_, same1) NT2H_TAG(sec_, same1) NT2H_TAG(csc_, same1)
 NT2H_TAG(sinpi_, same1) NT2H_TAG(cospi_, same1) NT2H_TAG(tanpi_, same1) NT2H_TAG(cotpi_, same1)
 NT2H_TAG(sind_, same1) NT2H_TAG(cosd_, same1) NT2H_TAG(tand_, same1) NT2H_TAG(cotd_, same1)
+NT2H_TAG(coth_, same1) NT2H_TAG(sech_, same1) NT2H_TAG(csch_, same1) NT2H_TAG(acoth_, same1)
+NT2H_TAG(asech_, same1p) NT2H_TAG(acsch_, same1p) NT2H_TAG(acot_, same1p) NT2H_TAG(asec_, same1p)
+NT2H_TAG(acsc_, same1p) NT2H_TAG(acotd_, same1p) NT2H_TAG(asecd_, same1p) NT2H_TAG(acscd_, same1p)
+NT2H_TAG(acosd_, same1p) NT2H_TAG(acospi_, same1p) NT2H_TAG(asind_, same1p) NT2H_TAG(asinpi_, same1p)
+NT2H_TAG(atand_, same1p) NT2H_TAG(atanpi_, same1p) NT2H_TAG(indeg_, same1) NT2H_TAG(inrad_, same1)
+NT2H_TAG(secd_, same1) NT2H_TAG(cscd_, same1) NT2H_TAG(secpi_, same1) NT2H_TAG(cscpi_, same1)
+NT2H_TAG(sinc_, same1p) NT2H_TAG(sincpi_, same1p) NT2H_TAG(atan2d_, same2)
 NT2H_TAG(pow_, pow2)
 NT2H_TAG(fma_, same3) NT2H_TAG(fnms_, same3) NT2H_TAG(if_else_, sel3)
 NT2H_TAG(selinc_, selinc2) NT2H_TAG(seldec_, selinc2) NT2H_TAG(if_zero_else_one_, izeo1)
@@ -356,7 +366,32 @@ NT2H_UNIT(sinpi_, u_pi, k_sin) NT2H_UNIT(cospi_, u_pi, k_cos) NT2H_UNIT(tanpi_, 
 NT2H_UNIT(cotpi_, u_pi, k_cot) NT2H_UNIT(sind_, u_deg, k_sin) NT2H_UNIT(cosd_, u_deg, k_cos)
 NT2H_UNIT(tand_, u_deg, k_tan) NT2H_UNIT(cotd_, u_deg, k_cot)
 #undef NT2H_UNIT
+// the compositions (nt2/hip/math_compose.hpp); G == 1 is NT2's scalar form
+#define NT2H_COMP(TAG, CALL)                                                                   \
+  NT2H_UN(TAG, (G == 1 ? dm::CALL<P, true>(x) : dm::CALL<P, false>(x)),                        \
+          (G == 1 ? dm::CALL<P, true>(x) : dm::CALL<P, false>(x)), no_integer_form<T>())
+NT2H_COMP(asech_, asech_t) NT2H_COMP(acsch_, acsch_t) NT2H_COMP(acot_, acot_t) NT2H_COMP(asec_, asec_t)
+NT2H_COMP(acsc_, acsc_t) NT2H_COMP(acotd_, acotd_t) NT2H_COMP(asecd_, asecd_t) NT2H_COMP(acscd_, acscd_t)
+NT2H_COMP(acosd_, acosd_t) NT2H_COMP(acospi_, acospi_t) NT2H_COMP(asind_, asind_t) NT2H_COMP(asinpi_, asinpi_t)
+NT2H_COMP(atand_, atand_t) NT2H_COMP(atanpi_, atanpi_t) NT2H_COMP(sincpi_, sincpi_t)
+#undef NT2H_COMP
+NT2H_UN(coth_, dm::coth_t<P>(x), dm::coth_t<P>(x), no_integer_form<T>())
+NT2H_UN(sech_, dm::sech_t<P>(x), dm::sech_t<P>(x), no_integer_form<T>())
+NT2H_UN(csch_, dm::csch_t<P>(x), dm::csch_t<P>(x), no_integer_form<T>())
+NT2H_UN(acoth_, dm::acoth_t<P>(x), dm::acoth_t<P>(x), no_integer_form<T>())
+NT2H_UN(indeg_, dm::indeg(x), dm::indeg(x), no_integer_form<T>())
+NT2H_UN(inrad_, dm::inrad(x), dm::inrad(x), no_integer_form<T>())
+NT2H_UN(secd_, dm::secd_t<P>(x), dm::secd_t<P>(x), no_integer_form<T>())
+NT2H_UN(cscd_, dm::cscd_t<P>(x), dm::cscd_t<P>(x), no_integer_form<T>())
+NT2H_UN(secpi_, dm::secpi_t<P>(x), dm::secpi_t<P>(x), no_integer_form<T>())
+NT2H_UN(cscpi_, dm::cscpi_t<P>(x), dm::cscpi_t<P>(x), no_integer_form<T>())
 #undef NT2H_UN
+template <> struct apply1<tagfn::sinc_> {
+  template <int G, class T> __device__ __forceinline__ static void run(T (&a)[G], T (&r)[G]) {
+    static_assert(std::is_floating_point<T>::value, "sinc is defined for floating-point tables only");
+    dm::sinc_group<P, G>(a, r);
+  }
+};
 
 template <> struct apply1<tagfn::cos_> {
   template <int G, class T> __device__ __forceinline__ static void run(T (&a)[G], T (&r)[G]) {
@@ -780,12 +815,20 @@ NT2H_FN1(erf, erf_) NT2H_FN1(cbrt, cbrt_) NT2H_FN1(asinh, asinh_) NT2H_FN1(acosh
 NT2H_FN1(cot, cot_) NT2H_FN1(sec, sec_) NT2H_FN1(csc, csc_) NT2H_FN1(sinpi, sinpi_) NT2H_FN1(cospi, cospi_)
 NT2H_FN1(tanpi, tanpi_) NT2H_FN1(cotpi, cotpi_) NT2H_FN1(sind, sind_) NT2H_FN1(cosd, cosd_) NT2H_FN1(tand, tand_)
 NT2H_FN1(cotd, cotd_)
+NT2H_FN1(coth, coth_) NT2H_FN1(sech, sech_) NT2H_FN1(csch, csch_) NT2H_FN1(acoth, acoth_)
+NT2H_FN1(asech, asech_) NT2H_FN1(acsch, acsch_) NT2H_FN1(acot, acot_) NT2H_FN1(asec, asec_)
+NT2H_FN1(acsc, acsc_) NT2H_FN1(acotd, acotd_) NT2H_FN1(asecd, asecd_) NT2H_FN1(acscd, acscd_)
+NT2H_FN1(acosd, acosd_) NT2H_FN1(acospi, acospi_) NT2H_FN1(asind, asind_) NT2H_FN1(asinpi, asinpi_)
+NT2H_FN1(atand, atand_) NT2H_FN1(atanpi, atanpi_) NT2H_FN1(indeg, indeg_) NT2H_FN1(inrad, inrad_)
+NT2H_FN1(secd, secd_) NT2H_FN1(cscd, cscd_) NT2H_FN1(secpi, secpi_) NT2H_FN1(cscpi, cscpi_)
+NT2H_FN1(sinc, sinc_) NT2H_FN1(sincpi, sincpi_)
 NT2H_FN1(if_zero_else_one, if_zero_else_one_)
 #undef NT2H_FN1
 #define NT2H_FN2(name, Tag) \
   template <class A, class B, NT2H_ENABLE2(A, B)> auto name(const A& a, const B& b) { return make2<tagfn::Tag>(a, b); }
 NT2H_FN2(max, max_) NT2H_FN2(min, min_) NT2H_FN2(adds, adds_) NT2H_FN2(subs, subs_)
 NT2H_FN2(selinc, selinc_) NT2H_FN2(seldec, seldec_) NT2H_FN2(atan2, atan2_)
+NT2H_FN2(atan2d, atan2d_)
 #undef NT2H_FN2
 // pow(a, b): Cephes' pow for floating-point exponents; an integral exponent
 // (a scalar, kept in its own type, or an integer table) takes NT2's repeated
@@ -1115,6 +1158,11 @@ __device__ __forceinline__ void evn(C& c, const n2v<Tag, VA, VB>& v, size_t i, R
         r[l] = G == 1 ? dm::normcdf_d<P, true>((double)a[l]) : dm::normcdf_join_d((double)a[l], (double)b[l]);
       else r[l] = no_integer_form<R>();
     }
+  } else if constexpr (std::is_same<Tag, tagfn::atan2d_>::value) {
+    static_assert(std::is_floating_point<R>::value, "atan2d is defined for floating-point tables only");
+#pragma unroll
+    for (int l = 0; l < G; ++l)
+      r[l] = G == 1 ? dm::atan2d_t<P, true>((R)a[l], (R)b[l]) : dm::atan2d_t<P, false>((R)a[l], (R)b[l]);
   } else if constexpr (std::is_same<Tag, tagfn::atan2_>::value) {
     static_assert(std::is_floating_point<R>::value, "atan2 is defined for floating-point tables only");
 #pragma unroll
@@ -1478,7 +1526,7 @@ template <class Tag, class A> struct pack_free<node1<Tag, A>> {
   static constexpr bool value = pack_free<A>::value && !std::is_same<Tag, tagfn::cos_>::value &&
                                 !std::is_same<Tag, tagfn::sin_>::value && !std::is_same<Tag, tagfn::tan_>::value &&
                                 !std::is_same<Tag, tagfn::cot_>::value && !std::is_same<Tag, tagfn::sec_>::value &&
-                                !std::is_same<Tag, tagfn::csc_>::value &&
+                                !std::is_same<Tag, tagfn::csc_>::value && !std::is_base_of<tagfn::same1p, Tag>::value &&
                                 !std::is_same<Tag, tagfn::atan_>::value && !std::is_same<Tag, tagfn::asinh_>::value &&
                                 !std::is_same<Tag, tagfn::acosh_>::value &&
                                 !((std::is_same<Tag, tagfn::erfc_>::value ||
@@ -1488,6 +1536,7 @@ template <class Tag, class A> struct pack_free<node1<Tag, A>> {
 };
 template <class Tag, class A, class B> struct pack_free<node2<Tag, A, B>> {
   static constexpr bool value = pack_free<A>::value && pack_free<B>::value && !std::is_same<Tag, tagfn::atan2_>::value &&
+                                !std::is_same<Tag, tagfn::atan2d_>::value &&
                                 !std::is_same<Tag, tagfn::pow_>::value &&
                                 !(std::is_same<Tag, tagfn::normcdf_join_>::value &&
                                   std::is_same<typename A::value_type, double>::value);

@@ -8,14 +8,17 @@ LIB_NAME = "libnt2h.so"
 
 EXACT, FAST, ACCURATE = 0, 1, 2
 BS_TEXTBOOK, BS_DEMO = 0, 1
-OPS2 = {"atan2": 0, "pow": 1}
+OPS2 = {"atan2": 0, "pow": 1, "atan2d": 2}
 OPS = {"exp": 0, "log": 1, "sqrt": 2, "erfc": 3, "normcdf": 4,
        "fastnormcdf": 5, "cos": 6, "sin": 7, "abs": 8, "exp2": 9, "exp10": 10, "log2": 11,
        "log10": 12, "floor": 13, "ceil": 14, "trunc": 15, "round": 16, "round2even": 17, "sign": 18,
        "expm1": 19, "log1p": 20, "tanh": 21, "sinh": 22, "cosh": 23, "atan": 24, "tan": 25,
        "asin": 26, "acos": 27, "erf": 28, "cbrt": 29, "asinh": 30, "acosh": 31, "atanh": 32,
        "cot": 33, "sec": 34, "csc": 35, "sinpi": 36, "cospi": 37, "tanpi": 38, "cotpi": 39, "sind": 40,
-       "cosd": 41, "tand": 42, "cotd": 43}
+       "cosd": 41, "tand": 42, "cotd": 43, "coth": 44, "sech": 45, "csch": 46, "acoth": 47, "asech": 48, "acsch": 49,
+       "acot": 50, "asec": 51, "acsc": 52, "acotd": 53, "asecd": 54, "acscd": 55, "acosd": 56, "acospi": 57,
+       "asind": 58, "asinpi": 59, "atand": 60, "atanpi": 61, "indeg": 62, "inrad": 63, "secd": 64,
+       "cscd": 65, "secpi": 66, "cscpi": 67, "sinc": 68, "sincpi": 69}
 H2D, D2H, D2D = 1, 2, 3
 
 _lib = None

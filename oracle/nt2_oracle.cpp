@@ -1749,6 +1749,177 @@ template <class T, bool PI> T trig_unit(int kind, T a0) {
 }
 
 // ===========================================================================
+// Compositions over the kernels above, each NT2's own definition (its
+// operation order fixes the bits); `scalar` selects NT2's scalar form (the
+// tail of a table) where the inner function or the definition has one.
+// coth: modules/core/hyperbolic/include/nt2/hyperbolic/functions/simd/common/
+// coth.hpp:35-63 (scalar/coth.hpp:29-55), tanh_kernel's coth
+// (details/tanh_kernel.hpp:40-43, :55-59); sech, csch, acoth, asech, acsch:
+// hyperbolic/functions/generic/{sech.hpp:19-29,csch.hpp:18-30,acoth.hpp:18-28,
+// asech.hpp:18-28,acsch.hpp:18-28}; acot, acsc, atand, asind, acosd, asecd,
+// acscd, acotd, atanpi, asinpi, acospi, atan2d, indeg, inrad, secd, cscd,
+// secpi, cscpi: modules/core/trigonometric/include/nt2/trigonometric/functions/
+// generic/*.hpp; asec: simd/common/asec.hpp:18-30 and scalar/asec.hpp:26-67;
+// sinc, sincpi: simd/common/{sinc.hpp:33-58,sincpi.hpp:35-60} and
+// scalar/{sinc.hpp:28-49,sincpi.hpp:29-49}.
+// ===========================================================================
+namespace ov {
+inline float atan(float x, bool s) { return atan_f(x, s); }
+inline double atan(double x, bool s) { return atan_d(x, s); }
+inline float asin(float x, bool) { return asin_f(x); }
+inline double asin(double x, bool s) { return asin_d(x, s); }
+inline float acos(float x, bool) { return acos_f(x); }
+inline double acos(double x, bool s) { return acos_d(x, s); }
+inline float asinh(float x, bool s) { return asinh_f(x, s); }
+inline double asinh(double x, bool s) { return asinh_d(x, s); }
+inline float acosh(float x, bool s) { return acosh_f(x, s); }
+inline double acosh(double x, bool s) { return acosh_d(x, s); }
+inline float atanh(float x) { return atanh_f(x); }
+inline double atanh(double x) { return atanh_d(x); }
+inline float sinh(float x) { return sinh_f(x); }
+inline double sinh(double x) { return sinh_d(x); }
+inline float cosh(float x) { return cosh_f(x); }
+inline double cosh(double x) { return cosh_d(x); }
+inline float exp(float x) { return exp_f(x); }
+inline double exp(double x) { return exp_d(x); }
+inline float atan2(float a, float b, bool s) { return atan2_f(a, b, s); }
+inline double atan2(double a, double b, bool s) { return atan2_d(a, b, s); }
+template <class T> T cst(uint32_t f, uint64_t d) {
+  if constexpr (sizeof(T) == 4) return __builtin_bit_cast(float, f);
+  else return __builtin_bit_cast(double, d);
+}
+template <class T> T pio2() { return cst<T>(0x3fc90fdbu, 0x3ff921fb54442d18ull); }
+template <class T> T invpi() { return cst<T>(0x3ea2f983u, 0x3fd45f306dc9c883ull); }
+template <class T> T eps() { return std::numeric_limits<T>::epsilon(); }
+template <class T> T signbit_of(T a) {
+  using U = typename unit_c<T>::U;
+  return __builtin_bit_cast(T, (U)(__builtin_bit_cast(U, a) & unit_c<T>::sign));
+}
+template <class T> T bxor(T a, T b) {
+  using U = typename unit_c<T>::U;
+  return __builtin_bit_cast(T, (U)(__builtin_bit_cast(U, a) ^ __builtin_bit_cast(U, b)));
+}
+template <class T> T bor(T a, T b) {
+  using U = typename unit_c<T>::U;
+  return __builtin_bit_cast(T, (U)(__builtin_bit_cast(U, a) | __builtin_bit_cast(U, b)));
+}
+}  // namespace ov
+
+// indeg: a0*Radindeg - a0*Radindegr (constants/radindeg.hpp:29-30,
+// radindegr.hpp:28-29); inrad: a0*Deginrad (constants/deginrad.hpp:29-30)
+template <class T> T indeg_o(T a0) {
+  return a0 * ov::cst<T>(0x42652ee1u, 0x404ca5dc1a63c1f8ull) - a0 * ov::cst<T>(0xb53387c1u, 0xbce1e7ab456405f8ull);
+}
+template <class T> T inrad_o(T a0) { return a0 * ov::cst<T>(0x3c8efa35u, 0x3f91df46a2529d39ull); }
+inline float coth_o(float a0) {
+  const float cc[5] = {F32(0xbbbaf0ea), F32(0x3ca9134e), F32(0xbd5c1e2d), F32(0x3e088393), F32(0xbeaaaa99)};
+  const float x = std::fabs(a0);
+  float r;
+  if (x < 0.625f) {
+    const float x2 = x * x;
+    r = 1.0f / ((horner(cc, x2) * x2) * x + x);
+  } else {
+    r = 2.0f * (1.0f / (exp_f(x + x) - 1.0f)) + 1.0f;
+  }
+  return ov::bxor(r, ov::signbit_of(a0));
+}
+inline double coth_o(double a0) {
+  const double p[3] = {F64(0xbfeedc5baafd6f4b), F64(0xc058d26a0e26682d), F64(0xc0993ac030580563)};
+  const double q[4] = {F64(0x3FF0000000000000), F64(0x405c33f28a581b86), F64(0x40a176fa0e5535fa),
+                       F64(0x40b2ec102442040c)};
+  const double x = std::fabs(a0);
+  double r;
+  if (x < 0.625) {
+    const double x2 = x * x;
+    const double qv = horner(q, x2);
+    r = qv / (x * (horner(p, x2) * x2 + qv));
+  } else {
+    r = 2.0 * (1.0 / (exp_d(x + x) - 1.0)) + 1.0;
+  }
+  return ov::bxor(r, ov::signbit_of(a0));
+}
+template <class T> T asec_o(T a0, bool scalar) {
+  if (!scalar) return ov::pio2<T>() - ov::asin(T(1) / a0, false);
+  if constexpr (sizeof(T) == 8) {
+    if (a0 == 1.0) return 0.0;
+    return (ov::pio2<T>() - ov::asin(T(1) / a0, true)) + F64(0x3c91a62633145c07);
+  } else {
+    const float ax = std::fabs(a0);
+    if (ax < 1.0f) return NAN;
+    const float ax1 = ax - 1.0f;
+    if (ax1 < 0.001f) {
+      const float tmp = std::sqrt(2 * ax1) *
+                        (1 - ax1 * ((5 / 12.0f) + ax1 * ((43 / 160.0f) - ax1 * ((177 / 896.0f) + ax1 * (2867 / 18432.0f)))));
+      return a0 < 0.0f ? F32(0x40490fdb) - tmp : tmp;
+    }
+    if (a0 == 1.0f) return 0.0f;
+    return ov::pio2<T>() - ov::asin(T(1) / a0, true);
+  }
+}
+template <class T> T cscpi_o(T a0) {
+  const T r = T(1) / trig_unit<T, true>(1, a0);
+  return (is_flint_v(a0) && a0 != T(0)) ? T(NAN) : r;
+}
+// one lane of a composed op; kinds as in the C ABI (44..69, 68 = sinc is by pack)
+template <class T> T compose_o(int op, T x, bool sc) {
+  switch (op) {
+    case 44: return coth_o(x);
+    case 45: return T(1) / ov::cosh(x);
+    case 46: return T(1) / ov::sinh(x);
+    case 47: return ov::atanh(T(1) / x);
+    case 48: return ov::acosh(T(1) / x, sc);
+    case 49: return ov::asinh(T(1) / x, sc);
+    case 50: return ov::atan(T(1) / x, sc);
+    case 51: return asec_o(x, sc);
+    case 52: return ov::asin(T(1) / x, sc);
+    case 53: {
+      const T d = indeg_o(ov::atan(std::fabs(x), sc));
+      T z = T(90) - (x != T(0) ? d : T(0));
+      if (std::isinf(x)) z = T(0);
+      return ov::bor(z, ov::signbit_of(x));
+    }
+    case 54: return indeg_o(asec_o(x, sc));
+    case 55: return indeg_o(ov::asin(T(1) / x, sc));
+    case 56: return indeg_o(ov::acos(x, sc));
+    case 57: return ov::invpi<T>() * ov::acos(x, sc);
+    case 58: return indeg_o(ov::asin(x, sc));
+    case 59: return ov::invpi<T>() * ov::asin(x, sc);
+    case 60: return indeg_o(ov::atan(x, sc));
+    case 61: return ov::invpi<T>() * ov::atan(x, sc);
+    case 62: return indeg_o(x);
+    case 63: return inrad_o(x);
+    case 64: {
+      const T r = trig_unit<T, false>(0, x);
+      return T(1) / (r == T(0) ? T(NAN) : r);
+    }
+    case 65: return cscpi_o(ov::cst<T>(0x3bb60b61u, 0x3f76c16c16c16c17ull) * x);
+    case 66: {
+      const T r = T(1) / trig_unit<T, true>(0, x);
+      return is_flint_v((T(2) * x - T(1)) * T(0.5)) ? T(NAN) : r;
+    }
+    case 67: return cscpi_o(x);
+    case 69: {
+      const T sp = trig_unit<T, true>(1, x);
+      if (std::isinf(x)) return T(0);
+      if (std::fabs(x) < ov::eps<T>()) return T(1);
+      return sc ? (ov::invpi<T>() * sp) / x : ov::invpi<T>() * (sp / x);
+    }
+  }
+  return T(NAN);
+}
+// sinc over a pack of G lanes (sin with the pack's reduction)
+template <class T, int G> void sinc_o(const T* x, T* y) {
+  T s[G];
+  if constexpr (sizeof(T) == 4) trig_group_f<G>(1, x, s, nullptr);
+  else trig_group_d<G>(1, x, s, nullptr);
+  for (int l = 0; l < G; ++l) {
+    if (std::isinf(x[l])) y[l] = T(0);
+    else if (std::fabs(x[l]) < ov::eps<T>()) y[l] = T(1);
+    else y[l] = s[l] / x[l];
+  }
+}
+
+// ===========================================================================
 // NT2 transform schedule: modules/core/base/include/nt2/core/functions/simd/
 // transform.hpp:40-57 (packs of N from the range start, scalar tail), run in
 // parallel blocks of grain = L2/sizeof(T) when size > 8*grain
@@ -1925,7 +2096,9 @@ enum {
   OR_CBRT = 29, OR_ASINH = 30, OR_ACOSH = 31, OR_ATANH = 32, OR_COT = 33, OR_SEC = 34, OR_CSC = 35,
   OR_SINPI = 36, OR_COSPI = 37, OR_TANPI = 38, OR_COTPI = 39, OR_SIND = 40, OR_COSD = 41, OR_TAND = 42,
   OR_COTD = 43,
-  OR_LAST = OR_COTD
+  // 44..69: the compositions (compose_o; 68 = sinc by pack)
+  OR_SINC = 68, OR_SINCPI = 69,
+  OR_LAST = OR_SINCPI
 };
 
 int oracle_abi_version(void) { return 1; }
@@ -1995,6 +2168,11 @@ int oracle_map_f32(int op, const float* x, float* y, size_t n) {
       case OR_COSD: for (size_t l = 0; l < w; ++l) y[i + l] = trig_unit<float, false>(0, x[i + l]); break;
       case OR_TAND: for (size_t l = 0; l < w; ++l) y[i + l] = trig_unit<float, false>(3, x[i + l]); break;
       case OR_COTD: for (size_t l = 0; l < w; ++l) y[i + l] = trig_unit<float, false>(4, x[i + l]); break;
+      case OR_SINC:
+        if (pack) sinc_o<float, 8>(x + i, y + i);
+        else sinc_o<float, 1>(x + i, y + i);
+        break;
+      default: for (size_t l = 0; l < w; ++l) y[i + l] = compose_o<float>(op, x[i + l], !pack); break;
     }
   });
   return 0;
@@ -2053,6 +2231,11 @@ int oracle_map_f64(int op, const double* x, double* y, size_t n) {
       case OR_COSD: for (size_t l = 0; l < w; ++l) y[i + l] = trig_unit<double, false>(0, x[i + l]); break;
       case OR_TAND: for (size_t l = 0; l < w; ++l) y[i + l] = trig_unit<double, false>(3, x[i + l]); break;
       case OR_COTD: for (size_t l = 0; l < w; ++l) y[i + l] = trig_unit<double, false>(4, x[i + l]); break;
+      case OR_SINC:
+        if (pack) sinc_o<double, 4>(x + i, y + i);
+        else sinc_o<double, 1>(x + i, y + i);
+        break;
+      default: for (size_t l = 0; l < w; ++l) y[i + l] = compose_o<double>(op, x[i + l], !pack); break;
     }
   });
   return 0;
@@ -2060,20 +2243,24 @@ int oracle_map_f64(int op, const double* x, double* y, size_t n) {
 
 // out = atan2(x, y) over tables of n elements, NT2 transform semantics
 int oracle_map2_f32(int op, const float* x, const float* y, float* out, size_t n) {
-  if (op != 0 && op != 1) return -1;
+  if (op < 0 || op > 2) return -1;
   nt2_transform(n, 8, sizeof(float), [&](size_t i, bool pack) {
     const size_t w = pack ? 8 : 1;
     for (size_t l = 0; l < w; ++l)
-      out[i + l] = op == 0 ? atan2_f(x[i + l], y[i + l], !pack) : pow_t<float>(x[i + l], y[i + l], !pack);
+      out[i + l] = op == 0   ? atan2_f(x[i + l], y[i + l], !pack)
+                   : op == 1 ? pow_t<float>(x[i + l], y[i + l], !pack)
+                             : indeg_o(atan2_f(x[i + l], y[i + l], !pack));
   });
   return 0;
 }
 int oracle_map2_f64(int op, const double* x, const double* y, double* out, size_t n) {
-  if (op != 0 && op != 1) return -1;
+  if (op < 0 || op > 2) return -1;
   nt2_transform(n, 4, sizeof(double), [&](size_t i, bool pack) {
     const size_t w = pack ? 4 : 1;
     for (size_t l = 0; l < w; ++l)
-      out[i + l] = op == 0 ? atan2_d(x[i + l], y[i + l], !pack) : pow_t<double>(x[i + l], y[i + l], !pack);
+      out[i + l] = op == 0   ? atan2_d(x[i + l], y[i + l], !pack)
+                   : op == 1 ? pow_t<double>(x[i + l], y[i + l], !pack)
+                             : indeg_o(atan2_d(x[i + l], y[i + l], !pack));
   });
   return 0;
 }

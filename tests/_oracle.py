@@ -15,7 +15,10 @@ OPS = {"exp": 0, "log": 1, "sqrt": 2, "erfc": 3, "normcdf": 4,
        "expm1": 19, "log1p": 20, "tanh": 21, "sinh": 22, "cosh": 23, "atan": 24, "tan": 25,
        "asin": 26, "acos": 27, "erf": 28, "cbrt": 29, "asinh": 30, "acosh": 31, "atanh": 32,
        "cot": 33, "sec": 34, "csc": 35, "sinpi": 36, "cospi": 37, "tanpi": 38, "cotpi": 39, "sind": 40,
-       "cosd": 41, "tand": 42, "cotd": 43}
+       "cosd": 41, "tand": 42, "cotd": 43, "coth": 44, "sech": 45, "csch": 46, "acoth": 47, "asech": 48, "acsch": 49,
+       "acot": 50, "asec": 51, "acsc": 52, "acotd": 53, "asecd": 54, "acscd": 55, "acosd": 56, "acospi": 57,
+       "asind": 58, "asinpi": 59, "atand": 60, "atanpi": 61, "indeg": 62, "inrad": 63, "secd": 64,
+       "cscd": 65, "secpi": 66, "cscpi": 67, "sinc": 68, "sincpi": 69}
 
 _lib = None
 
@@ -329,4 +332,14 @@ def pow(a, b):
     out = np.empty_like(a)
     f = lib().oracle_map2_f32 if a.dtype == np.float32 else lib().oracle_map2_f64
     assert f(1, _p(a), _p(b), _p(out), a.size) == 0
+    return out
+
+
+def atan2d(y, x):
+    """NT2's atan2d(y, x) = indeg(atan2(y, x)) over flat tables."""
+    y = np.ascontiguousarray(y)
+    x = np.ascontiguousarray(x, dtype=y.dtype)
+    out = np.empty_like(y)
+    f = lib().oracle_map2_f32 if y.dtype == np.float32 else lib().oracle_map2_f64
+    assert f(2, _p(y), _p(x), _p(out), y.size) == 0
     return out

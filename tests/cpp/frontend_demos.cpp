@@ -250,6 +250,23 @@ int main(int argc, char** argv) {
   dump_scalar("rep_sum_same", nt2::globalsum(nt2::exp(at) * nt2::exp(at)));
   at2 = nt2::sind(at * 100.0f) - nt2::cosd(bt * 50.0f) + nt2::tand(at * 30.0f) * nt2::cotd(bt * 40.0f);
   dump("sind_f32", at2);
+  // the compositions: reciprocal hyperbolic, inverse trigonometric of 1/x and in
+  // degree / pi units, reciprocal trigonometric of degree / pi arguments, sinc
+  at2 = nt2::coth(at) + nt2::sech(bt) * nt2::csch(at * 3.0f) - nt2::acoth(bt * 4.0f);
+  dump("coth_f32", at2);
+  at2 = nt2::asech(nt2::abs(at) / 4.0f) + nt2::acsch(bt) - nt2::acot(at) * nt2::asec(bt * 3.0f);
+  dump("asech_f32", at2);
+  at2 = nt2::acsc(at * 2.0f) + nt2::acotd(bt) - nt2::asecd(at * 5.0f) + nt2::acscd(bt * 4.0f);
+  dump("acsc_f32", at2);
+  at2 = nt2::acosd(at / 4.0f) - nt2::acospi(bt / 4.0f) + nt2::asind(at / 3.0f) * nt2::asinpi(bt / 5.0f);
+  dump("acosd_f32", at2);
+  at2 = nt2::atand(at) + nt2::atanpi(bt) + nt2::indeg(at) - nt2::inrad(bt * 100.0f);
+  dump("atand_f32", at2);
+  at2 = nt2::secd(at * 90.0f) + nt2::cscd(bt * 45.0f) - nt2::secpi(at) * nt2::cscpi(bt / 3.0f) + nt2::sinc(at * 4.0f) +
+        nt2::sincpi(bt);
+  dump("secd_f32", at2);
+  at2 = nt2::atan2d(at, bt);
+  dump("atan2d_f32", at2);
   at2 = if_else((at > 0.0f) && (bt < 1.0f), at, bt) + if_else(logical_or(at < -1.0f, bt > 2.0f), 1.0f, 0.0f);
   dump("logical_f32", at2);
   table<double> dm(of_size(mh * mw)), dr;
@@ -279,6 +296,13 @@ int main(int argc, char** argv) {
   dump("sinpi_f64", dq2);
   dq2 = nt2::sind(dq * 500.0) - nt2::cosd(dq * 300.0) + nt2::tand(dq * 200.0) * nt2::cotd(dq * 170.0);
   dump("sind_f64", dq2);
+  dq2 = nt2::coth(dq * 3.0) - nt2::asech(nt2::abs(dq)) + nt2::acot(dq) * nt2::asec(dq * 30.0);
+  dump("coth_f64", dq2);
+  dq2 = nt2::acotd(dq * 7.0) + nt2::acscd(dq * 20.0) - nt2::asind(dq * 0.9) + nt2::atanpi(dq * 9.0);
+  dump("acotd_f64", dq2);
+  dq2 = nt2::secd(dq * 900.0) - nt2::cscpi(dq * 6.0) + nt2::sinc(dq * 40.0) * nt2::sincpi(dq * 5.0) +
+        nt2::atan2d(dq, dq * 0.5 - 0.1);
+  dump("secd_f64", dq2);
 
   // ---- rgb2grey and rgb2yuv on a 96 x 131 image: the demos' ramp and random planes
   const std::size_t ih = 96, iw = 131;

@@ -96,6 +96,36 @@ FUNCS = {
     "cosd": lambda x: mpmath.cospi(x / 180),
     "tand": lambda x: _ratio(mpmath.sinpi(x / 180), mpmath.cospi(x / 180), x, zero_is_nan=True),
     "cotd": lambda x: _ratio(mpmath.cospi(x / 180), mpmath.sinpi(x / 180), x),
+    # the compositions, on domains where NT2's composition is well conditioned
+    # (the rounding of its inner argument, 1/x or x/180, moves the result by
+    # at most a few ulp there); the rest is checked against the composition
+    # with the rounded argument in tests/test_oracle_math.py
+    "coth": lambda x: mpmath.coth(x),
+    "sech": lambda x: mpmath.sech(x),
+    "csch": lambda x: mpmath.csch(x),
+    "acoth": lambda x: mpmath.acoth(x),
+    "asech": lambda x: mpmath.asech(x),
+    "acsch": lambda x: mpmath.acsch(x),
+    "acot": lambda x: mpmath.acot(x) if x != 0 else mpmath.pi / 2,
+    "asec": lambda x: mpmath.asec(x),
+    "acsc": lambda x: mpmath.acsc(x),
+    "acotd": lambda x: mpmath.degrees(mpmath.acot(x)) if x != 0 else mpmath.mpf(90),
+    "asecd": lambda x: mpmath.degrees(mpmath.asec(x)),
+    "acscd": lambda x: mpmath.degrees(mpmath.acsc(x)),
+    "acosd": lambda x: mpmath.degrees(mpmath.acos(x)),
+    "acospi": lambda x: mpmath.acos(x) / mpmath.pi,
+    "asind": lambda x: mpmath.degrees(mpmath.asin(x)),
+    "asinpi": lambda x: mpmath.asin(x) / mpmath.pi,
+    "atand": lambda x: mpmath.degrees(mpmath.atan(x)),
+    "atanpi": lambda x: mpmath.atan(x) / mpmath.pi,
+    "indeg": lambda x: mpmath.degrees(x),
+    "inrad": lambda x: mpmath.radians(x),
+    "secd": lambda x: _ratio(mpmath.mpf(1), mpmath.cospi(x / 180), x, zero_is_nan=True),
+    "cscd": lambda x: _ratio(mpmath.mpf(1), mpmath.sinpi(x / 180), x),
+    "secpi": lambda x: _ratio(mpmath.mpf(1), mpmath.cospi(x), x, zero_is_nan=True),
+    "cscpi": lambda x: _ratio(mpmath.mpf(1), mpmath.sinpi(x), x),
+    "sinc": lambda x: mpmath.sin(x) / x if x != 0 else mpmath.mpf(1),
+    "sincpi": lambda x: mpmath.sinpi(x) / (mpmath.pi * x) if x != 0 else mpmath.mpf(1),
 }
 
 
@@ -202,6 +232,14 @@ def inputs(name, dtype, rng):
         pts += list(np.exp2(rng.uniform(-20, 40, 300)) * rng.choice([-1.0, 1.0], 300))
         pts += list(rng.integers(-64, 64, 60) * 22.5)
         pts += [45.0, -45.0, 90.0, 180.0, 270.0, 30.0, 60.0, 1e-20, 123456.0, 1e7 + 45]
+    elif name in COMPOSED_DOMAINS:
+        lo, hi, extra = COMPOSED_DOMAINS[name]
+        sgn = (lambda k: np.ones(k)) if lo >= 0 else (lambda k: rng.choice([-1.0, 1.0], k))
+        a = abs(lo) if lo < 0 else lo
+        pts += list(rng.uniform(a, hi, 700) * sgn(700))
+        if hi / max(a, 1e-300) > 100:  # log-uniform over wide ranges
+            pts += list(np.exp2(rng.uniform(np.log2(max(a, 1e-30)), np.log2(hi), 500)) * sgn(500))
+        pts += extra
     else:  # cos, sin
         pts += list(rng.uniform(-np.pi / 4, np.pi / 4, 150))
         pts += list(rng.uniform(-np.pi / 2, np.pi / 2, 150))
@@ -214,6 +252,37 @@ def inputs(name, dtype, rng):
         pts += [0.0, np.pi, -np.pi, np.pi / 2, np.pi / 4, 1.0, 100.0, 1e5, 1e10, big]
     x = np.array(pts, dtype=np.float64).astype(dtype)
     return x
+
+
+# (low |x| or signed low, high |x|, extra points); a negative low means both signs
+COMPOSED_DOMAINS = {
+    "coth": (-1e-6, 25.0, [0.625, 0.6249, -0.625, 1.0, 5.0, 20.0]),
+    "sech": (-1e-6, 80.0, [0.5, 1.0, 0.625, 10.0]),
+    "csch": (-1e-6, 80.0, [0.5, 1.0, 0.625, 10.0]),
+    "acoth": (-1.25, 1e6, [1.5, 2.0, -2.0, 10.0, 1.6]),
+    "asech": (0.001, 0.8, [0.5, 0.25, 0.001]),
+    "acsch": (-1e-20, 1e20, [0.5, 1.0, 2.0, -1.0]),
+    "acot": (-1e-20, 1e20, [0.5, 1.0, -1.0, 2.0, 0.41421357, 2.4142137]),
+    "asec": (-1.25, 1e20, [2.0, -2.0, 1.5, 10.0]),
+    "acsc": (-1.25, 1e20, [2.0, -2.0, 1.5, 10.0]),
+    "acotd": (-1e-20, 1.5, [0.5, 1.0, -1.0, 1e-3]),
+    "asecd": (-1.25, 1e20, [2.0, -2.0, 1.5, 10.0]),
+    "acscd": (-1.25, 1e20, [2.0, -2.0, 1.5, 10.0]),
+    "acosd": (-1e-20, 1.0, [0.5, -0.5, 1.0, -1.0, 0.0]),
+    "acospi": (-1e-20, 1.0, [0.5, -0.5, 1.0, -1.0, 0.0]),
+    "asind": (-1e-20, 1.0, [0.5, -0.5, 1.0, -1.0]),
+    "asinpi": (-1e-20, 1.0, [0.5, -0.5, 1.0, -1.0]),
+    "atand": (-1e-20, 1e20, [0.5, 1.0, -1.0, 2.0]),
+    "atanpi": (-1e-20, 1e20, [0.5, 1.0, -1.0, 2.0]),
+    "indeg": (-1e-30, 1e30, [np.pi, np.pi / 2, np.pi / 4, 1.0]),
+    "inrad": (-1e-30, 1e30, [180.0, 90.0, 45.0, 1.0]),
+    "secd": (-0.001, 720.0, [0.0, 45.0, 60.0, 90.0, 180.0, 270.0, -45.0]),
+    "cscd": (-0.001, 120.0, [45.0, 90.0, 30.0, -45.0, 1e-20]),
+    "secpi": (-1e-6, 4.0, [0.0, 0.25, 1.0, 0.5, 1.5, 1.0 / 3.0]),
+    "cscpi": (-1e-6, 4.0, [0.25, 0.5, 1.0, -0.25, 1.0 / 6.0]),
+    "sinc": (-1e-20, 60.0, [np.pi / 2, np.pi / 4, 1.0, 1e-4, 3.0]),
+    "sincpi": (-1e-20, 20.0, [0.5, 0.25, 1.0, 1e-4, 1.5]),
+}
 
 
 def build_math(dtype, tag):
@@ -565,6 +634,206 @@ KATS = [
     ("cotd", "-90", "0", 0.5, "modules/core/trigonometric/unit/simd/cotd.cpp:56"),
     ("cotd", "45", "1", 4.0, "modules/core/trigonometric/unit/simd/cotd.cpp:57"),
     ("cotd", "90", "0", 0.5, "modules/core/trigonometric/unit/simd/cotd.cpp:58"),
+    # compositions: the scalar unit tests of NT2 (evaluated in packs and as tails)
+    ("coth", "inf", "1", 0.5, "modules/core/hyperbolic/unit/scalar/coth.cpp:39"),
+    ("coth", "-inf", "-1", 0.5, "modules/core/hyperbolic/unit/scalar/coth.cpp:40"),
+    ("coth", "nan", "nan", 0.5, "modules/core/hyperbolic/unit/scalar/coth.cpp:41"),
+    ("coth", "-0", "-inf", 0.5, "modules/core/hyperbolic/unit/scalar/coth.cpp:42"),
+    ("coth", "0", "inf", 0.5, "modules/core/hyperbolic/unit/scalar/coth.cpp:43"),
+    ("sech", "inf", "0", 0.5, "modules/core/hyperbolic/unit/scalar/sech.cpp:37"),
+    ("sech", "-inf", "0", 0.5, "modules/core/hyperbolic/unit/scalar/sech.cpp:38"),
+    ("sech", "nan", "nan", 0.5, "modules/core/hyperbolic/unit/scalar/sech.cpp:39"),
+    ("sech", "0", "1", 0.5, "modules/core/hyperbolic/unit/scalar/sech.cpp:41"),
+    ("csch", "-0", "-inf", 0.5, "modules/core/hyperbolic/unit/scalar/csch.cpp:36"),
+    ("csch", "inf", "0", 0.5, "modules/core/hyperbolic/unit/scalar/csch.cpp:37"),
+    ("csch", "-inf", "0", 0.5, "modules/core/hyperbolic/unit/scalar/csch.cpp:38"),
+    ("csch", "nan", "nan", 0.5, "modules/core/hyperbolic/unit/scalar/csch.cpp:39"),
+    ("csch", "0", "inf", 0.5, "modules/core/hyperbolic/unit/scalar/csch.cpp:40"),
+    ("acoth", "inf", "0", 0.5, "modules/core/hyperbolic/unit/scalar/acoth.cpp:38"),
+    ("acoth", "-inf", "0", 0.5, "modules/core/hyperbolic/unit/scalar/acoth.cpp:39"),
+    ("acoth", "-1", "-inf", 0.5, "modules/core/hyperbolic/unit/scalar/acoth.cpp:40"),
+    ("acoth", "nan", "nan", 0.5, "modules/core/hyperbolic/unit/scalar/acoth.cpp:41"),
+    ("acoth", "1", "inf", 0.5, "modules/core/hyperbolic/unit/scalar/acoth.cpp:42"),
+    ("acoth", "0", "nan", 0.5, "modules/core/hyperbolic/unit/scalar/acoth.cpp:43"),
+    ("asech", "nan", "nan", 0.5, "modules/core/hyperbolic/unit/scalar/asech.cpp:36"),
+    ("asech", "0", "inf", 0.5, "modules/core/hyperbolic/unit/scalar/asech.cpp:37"),
+    ("asech", "1", "0", 0.5, "modules/core/hyperbolic/unit/scalar/asech.cpp:39"),
+    ("acsch", "inf", "0", 0.5, "modules/core/hyperbolic/unit/scalar/acsch.cpp:36"),
+    ("acsch", "-inf", "0", 0.5, "modules/core/hyperbolic/unit/scalar/acsch.cpp:37"),
+    ("acsch", "nan", "nan", 0.5, "modules/core/hyperbolic/unit/scalar/acsch.cpp:38"),
+    ("acsch", "-0", "-inf", 0.5, "modules/core/hyperbolic/unit/scalar/acsch.cpp:39"),
+    ("acsch", "0", "inf", 0.5, "modules/core/hyperbolic/unit/scalar/acsch.cpp:40"),
+    ("acot", "inf", "0", 1.5, "modules/core/trigonometric/unit/scalar/acot.cpp:43"),
+    ("acot", "-inf", "0", 1.5, "modules/core/trigonometric/unit/scalar/acot.cpp:44"),
+    ("acot", "nan", "nan", 1.5, "modules/core/trigonometric/unit/scalar/acot.cpp:45"),
+    ("acot", "0.5", "1.107148717794090", 1.5, "modules/core/trigonometric/unit/scalar/acot.cpp:47"),
+    ("acot", "-0.5", "-1.107148717794090", 1.5, "modules/core/trigonometric/unit/scalar/acot.cpp:48"),
+    ("acot", "-1", "-7.853981633974483e-01", 1.5, "modules/core/trigonometric/unit/scalar/acot.cpp:49"),
+    ("acot", "1", "7.853981633974483e-01", 1.5, "modules/core/trigonometric/unit/scalar/acot.cpp:50"),
+    ("acot", "0", "1.570796326794897", 1.5, "modules/core/trigonometric/unit/scalar/acot.cpp:51"),
+    ("acot", "1/eps", "eps", 1.5, "modules/core/trigonometric/unit/scalar/acot.cpp:52"),
+    ("asec", "inf", "pi/2", 0.5, "modules/core/trigonometric/unit/scalar/asec.cpp:45"),
+    ("asec", "-inf", "pi/2", 0.5, "modules/core/trigonometric/unit/scalar/asec.cpp:46"),
+    ("asec", "nan", "nan", 0.5, "modules/core/trigonometric/unit/scalar/asec.cpp:47"),
+    ("asec", "0", "nan", 0.5, "modules/core/trigonometric/unit/scalar/asec.cpp:48"),
+    ("asec", "-2", "2pi/3", 0.5, "modules/core/trigonometric/unit/scalar/asec.cpp:50"),
+    ("asec", "-1", "pi", 0.5, "modules/core/trigonometric/unit/scalar/asec.cpp:51"),
+    ("asec", "1", "0", 0.5, "modules/core/trigonometric/unit/scalar/asec.cpp:52"),
+    ("asec", "2", "pi/3", 0.5, "modules/core/trigonometric/unit/scalar/asec.cpp:53"),
+    ("acsc", "inf", "0", 0.5, "modules/core/trigonometric/unit/scalar/acsc.cpp:44"),
+    ("acsc", "-inf", "0", 0.5, "modules/core/trigonometric/unit/scalar/acsc.cpp:45"),
+    ("acsc", "nan", "nan", 0.5, "modules/core/trigonometric/unit/scalar/acsc.cpp:46"),
+    ("acsc", "0", "nan", 0.5, "modules/core/trigonometric/unit/scalar/acsc.cpp:47"),
+    ("acsc", "-2", "-pi/6", 0.5, "modules/core/trigonometric/unit/scalar/acsc.cpp:49"),
+    ("acsc", "-1", "-pi/2", 0.5, "modules/core/trigonometric/unit/scalar/acsc.cpp:50"),
+    ("acsc", "1", "pi/2", 0.5, "modules/core/trigonometric/unit/scalar/acsc.cpp:51"),
+    ("acsc", "2", "pi/6", 0.5, "modules/core/trigonometric/unit/scalar/acsc.cpp:52"),
+    ("acotd", "inf", "0", 0.5, "modules/core/trigonometric/unit/scalar/acotd.cpp:40"),
+    ("acotd", "-inf", "0", 0.5, "modules/core/trigonometric/unit/scalar/acotd.cpp:41"),
+    ("acotd", "nan", "nan", 0.5, "modules/core/trigonometric/unit/scalar/acotd.cpp:42"),
+    ("acotd", "-0", "-90", 0.5, "modules/core/trigonometric/unit/scalar/acotd.cpp:44"),
+    ("acotd", "-1", "-45", 0.5, "modules/core/trigonometric/unit/scalar/acotd.cpp:45"),
+    ("acotd", "1", "45", 0.5, "modules/core/trigonometric/unit/scalar/acotd.cpp:46"),
+    ("acotd", "0", "90", 0.5, "modules/core/trigonometric/unit/scalar/acotd.cpp:47"),
+    ("asecd", "inf", "90", 0.5, "modules/core/trigonometric/unit/scalar/asecd.cpp:41"),
+    ("asecd", "-inf", "90", 0.5, "modules/core/trigonometric/unit/scalar/asecd.cpp:42"),
+    ("asecd", "nan", "nan", 0.5, "modules/core/trigonometric/unit/scalar/asecd.cpp:43"),
+    ("asecd", "0", "nan", 0.5, "modules/core/trigonometric/unit/scalar/asecd.cpp:44"),
+    ("asecd", "-2", "120", 0.5, "modules/core/trigonometric/unit/scalar/asecd.cpp:46"),
+    ("asecd", "-1", "180", 0.5, "modules/core/trigonometric/unit/scalar/asecd.cpp:47"),
+    ("asecd", "1", "0", 0.5, "modules/core/trigonometric/unit/scalar/asecd.cpp:48"),
+    ("asecd", "2", "60", 0.5, "modules/core/trigonometric/unit/scalar/asecd.cpp:49"),
+    ("acscd", "inf", "0", 0.5, "modules/core/trigonometric/unit/scalar/acscd.cpp:41"),
+    ("acscd", "-inf", "0", 0.5, "modules/core/trigonometric/unit/scalar/acscd.cpp:42"),
+    ("acscd", "nan", "nan", 0.5, "modules/core/trigonometric/unit/scalar/acscd.cpp:43"),
+    ("acscd", "0", "nan", 0.5, "modules/core/trigonometric/unit/scalar/acscd.cpp:44"),
+    ("acscd", "-2", "-30", 0.5, "modules/core/trigonometric/unit/scalar/acscd.cpp:46"),
+    ("acscd", "-1", "-90", 0.5, "modules/core/trigonometric/unit/scalar/acscd.cpp:47"),
+    ("acscd", "1", "90", 0.5, "modules/core/trigonometric/unit/scalar/acscd.cpp:48"),
+    ("acscd", "2", "30", 0.5, "modules/core/trigonometric/unit/scalar/acscd.cpp:49"),
+    ("acosd", "inf", "nan", 0.5, "modules/core/trigonometric/unit/scalar/acosd.cpp:42"),
+    ("acosd", "-inf", "nan", 0.5, "modules/core/trigonometric/unit/scalar/acosd.cpp:43"),
+    ("acosd", "nan", "nan", 0.5, "modules/core/trigonometric/unit/scalar/acosd.cpp:44"),
+    ("acosd", "0.5", "60", 0.5, "modules/core/trigonometric/unit/scalar/acosd.cpp:46"),
+    ("acosd", "-0.5", "120", 0.5, "modules/core/trigonometric/unit/scalar/acosd.cpp:47"),
+    ("acosd", "-1", "180", 0.5, "modules/core/trigonometric/unit/scalar/acosd.cpp:48"),
+    ("acosd", "1", "0", 0.5, "modules/core/trigonometric/unit/scalar/acosd.cpp:49"),
+    ("acosd", "0", "90", 0.5, "modules/core/trigonometric/unit/scalar/acosd.cpp:50"),
+    ("acospi", "nan", "nan", 0.5, "modules/core/trigonometric/unit/scalar/acospi.cpp:40"),
+    ("acospi", "0.5", "1/3", 0.5, "modules/core/trigonometric/unit/scalar/acospi.cpp:42"),
+    ("acospi", "-0.5", "2/3", 0.5, "modules/core/trigonometric/unit/scalar/acospi.cpp:43"),
+    ("acospi", "-1", "1", 0.5, "modules/core/trigonometric/unit/scalar/acospi.cpp:44"),
+    ("acospi", "1", "0", 0.5, "modules/core/trigonometric/unit/scalar/acospi.cpp:45"),
+    ("acospi", "0", "0.5", 0.5, "modules/core/trigonometric/unit/scalar/acospi.cpp:46"),
+    ("asind", "inf", "nan", 0.5, "modules/core/trigonometric/unit/scalar/asind.cpp:42"),
+    ("asind", "-inf", "nan", 0.5, "modules/core/trigonometric/unit/scalar/asind.cpp:43"),
+    ("asind", "nan", "nan", 0.5, "modules/core/trigonometric/unit/scalar/asind.cpp:44"),
+    ("asind", "0.5", "30", 0.5, "modules/core/trigonometric/unit/scalar/asind.cpp:46"),
+    ("asind", "-0.5", "-30", 0.5, "modules/core/trigonometric/unit/scalar/asind.cpp:47"),
+    ("asind", "-1", "-90", 0.5, "modules/core/trigonometric/unit/scalar/asind.cpp:48"),
+    ("asind", "1", "90", 0.5, "modules/core/trigonometric/unit/scalar/asind.cpp:49"),
+    ("asind", "0", "0", 0.5, "modules/core/trigonometric/unit/scalar/asind.cpp:50"),
+    ("asinpi", "nan", "nan", 0.5, "modules/core/trigonometric/unit/scalar/asinpi.cpp:40"),
+    ("asinpi", "0.5", "1/6", 0.5, "modules/core/trigonometric/unit/scalar/asinpi.cpp:42"),
+    ("asinpi", "-0.5", "-1/6", 0.5, "modules/core/trigonometric/unit/scalar/asinpi.cpp:43"),
+    ("asinpi", "-1", "-0.5", 0.5, "modules/core/trigonometric/unit/scalar/asinpi.cpp:44"),
+    ("asinpi", "1", "0.5", 0.5, "modules/core/trigonometric/unit/scalar/asinpi.cpp:45"),
+    ("asinpi", "0", "0", 0.5, "modules/core/trigonometric/unit/scalar/asinpi.cpp:46"),
+    ("atand", "inf", "90", 0.5, "modules/core/trigonometric/unit/scalar/atand.cpp:42"),
+    ("atand", "-inf", "-90", 0.5, "modules/core/trigonometric/unit/scalar/atand.cpp:43"),
+    ("atand", "nan", "nan", 0.5, "modules/core/trigonometric/unit/scalar/atand.cpp:44"),
+    ("atand", "0.5", "2.656505117707799e+01", 0.5, "modules/core/trigonometric/unit/scalar/atand.cpp:46"),
+    ("atand", "-0.5", "-2.656505117707799e+01", 0.5, "modules/core/trigonometric/unit/scalar/atand.cpp:47"),
+    ("atand", "-1", "-45", 0.5, "modules/core/trigonometric/unit/scalar/atand.cpp:48"),
+    ("atand", "1", "45", 0.5, "modules/core/trigonometric/unit/scalar/atand.cpp:49"),
+    ("atand", "0", "0", 0.5, "modules/core/trigonometric/unit/scalar/atand.cpp:50"),
+    ("atanpi", "inf", "0.5", 0.5, "modules/core/trigonometric/unit/scalar/atanpi.cpp:40"),
+    ("atanpi", "-inf", "-0.5", 0.5, "modules/core/trigonometric/unit/scalar/atanpi.cpp:41"),
+    ("atanpi", "nan", "nan", 0.5, "modules/core/trigonometric/unit/scalar/atanpi.cpp:42"),
+    ("atanpi", "-1", "-0.25", 0.5, "modules/core/trigonometric/unit/scalar/atanpi.cpp:44"),
+    ("atanpi", "1", "0.25", 0.5, "modules/core/trigonometric/unit/scalar/atanpi.cpp:45"),
+    ("atanpi", "0", "0", 0.5, "modules/core/trigonometric/unit/scalar/atanpi.cpp:46"),
+    ("indeg", "inf", "inf", 0.5, "modules/core/trigonometric/unit/scalar/indeg.cpp:44"),
+    ("indeg", "-inf", "-inf", 0.5, "modules/core/trigonometric/unit/scalar/indeg.cpp:45"),
+    ("indeg", "nan", "nan", 0.5, "modules/core/trigonometric/unit/scalar/indeg.cpp:46"),
+    ("indeg", "-pi", "-180", 0.5, "modules/core/trigonometric/unit/scalar/indeg.cpp:48"),
+    ("indeg", "-pi/2", "-90", 0.5, "modules/core/trigonometric/unit/scalar/indeg.cpp:49"),
+    ("indeg", "-pi/4", "-45", 0.5, "modules/core/trigonometric/unit/scalar/indeg.cpp:50"),
+    ("indeg", "pi", "180", 0.5, "modules/core/trigonometric/unit/scalar/indeg.cpp:51"),
+    ("indeg", "pi/2", "90", 0.5, "modules/core/trigonometric/unit/scalar/indeg.cpp:52"),
+    ("indeg", "pi/4", "45", 0.5, "modules/core/trigonometric/unit/scalar/indeg.cpp:53"),
+    ("indeg", "0", "0", 0.5, "modules/core/trigonometric/unit/scalar/indeg.cpp:54"),
+    ("inrad", "inf", "inf", 0.5, "modules/core/trigonometric/unit/scalar/inrad.cpp:44"),
+    ("inrad", "-inf", "-inf", 0.5, "modules/core/trigonometric/unit/scalar/inrad.cpp:45"),
+    ("inrad", "nan", "nan", 0.5, "modules/core/trigonometric/unit/scalar/inrad.cpp:46"),
+    ("inrad", "-180", "-pi", 0.5, "modules/core/trigonometric/unit/scalar/inrad.cpp:48"),
+    ("inrad", "-45", "-pi/4", 0.5, "modules/core/trigonometric/unit/scalar/inrad.cpp:49"),
+    ("inrad", "-90", "-pi/2", 0.5, "modules/core/trigonometric/unit/scalar/inrad.cpp:50"),
+    ("inrad", "0", "0", 0.5, "modules/core/trigonometric/unit/scalar/inrad.cpp:51"),
+    ("inrad", "180", "pi", 0.5, "modules/core/trigonometric/unit/scalar/inrad.cpp:52"),
+    ("inrad", "45", "pi/4", 0.5, "modules/core/trigonometric/unit/scalar/inrad.cpp:53"),
+    ("inrad", "90", "pi/2", 0.5, "modules/core/trigonometric/unit/scalar/inrad.cpp:54"),
+    ("secd", "-90", "nan", 0.5, "modules/core/trigonometric/unit/scalar/secd.cpp:41"),
+    ("secd", "inf", "nan", 0.5, "modules/core/trigonometric/unit/scalar/secd.cpp:42"),
+    ("secd", "-inf", "nan", 0.5, "modules/core/trigonometric/unit/scalar/secd.cpp:43"),
+    ("secd", "nan", "nan", 0.5, "modules/core/trigonometric/unit/scalar/secd.cpp:44"),
+    ("secd", "90", "nan", 0.5, "modules/core/trigonometric/unit/scalar/secd.cpp:45"),
+    ("secd", "-180", "-1", 0.5, "modules/core/trigonometric/unit/scalar/secd.cpp:47"),
+    ("secd", "-45", "sqrt(2)", 0.5, "modules/core/trigonometric/unit/scalar/secd.cpp:48"),
+    ("secd", "0", "1", 0.5, "modules/core/trigonometric/unit/scalar/secd.cpp:49"),
+    ("secd", "180", "-1", 0.5, "modules/core/trigonometric/unit/scalar/secd.cpp:50"),
+    ("secd", "45", "sqrt(2)", 0.5, "modules/core/trigonometric/unit/scalar/secd.cpp:51"),
+    ("cscd", "-0", "-inf", 0.5, "modules/core/trigonometric/unit/scalar/cscd.cpp:41"),
+    ("cscd", "-180", "nan", 0.5, "modules/core/trigonometric/unit/scalar/cscd.cpp:42"),
+    ("cscd", "inf", "nan", 0.5, "modules/core/trigonometric/unit/scalar/cscd.cpp:43"),
+    ("cscd", "-inf", "nan", 0.5, "modules/core/trigonometric/unit/scalar/cscd.cpp:44"),
+    ("cscd", "nan", "nan", 0.5, "modules/core/trigonometric/unit/scalar/cscd.cpp:45"),
+    ("cscd", "0", "inf", 0.5, "modules/core/trigonometric/unit/scalar/cscd.cpp:46"),
+    ("cscd", "180", "nan", 0.5, "modules/core/trigonometric/unit/scalar/cscd.cpp:47"),
+    ("cscd", "-45", "-sqrt(2)", 0.5, "modules/core/trigonometric/unit/scalar/cscd.cpp:49"),
+    ("cscd", "-90", "-1", 0.5, "modules/core/trigonometric/unit/scalar/cscd.cpp:50"),
+    ("cscd", "45", "sqrt(2)", 0.5, "modules/core/trigonometric/unit/scalar/cscd.cpp:51"),
+    ("cscd", "90", "1", 0.5, "modules/core/trigonometric/unit/scalar/cscd.cpp:52"),
+    ("secpi", "0.5", "nan", 0.5, "modules/core/trigonometric/unit/scalar/secpi.cpp:44"),
+    ("secpi", "inf", "nan", 0.5, "modules/core/trigonometric/unit/scalar/secpi.cpp:45"),
+    ("secpi", "-0.5", "nan", 0.5, "modules/core/trigonometric/unit/scalar/secpi.cpp:46"),
+    ("secpi", "-inf", "nan", 0.5, "modules/core/trigonometric/unit/scalar/secpi.cpp:47"),
+    ("secpi", "nan", "nan", 0.5, "modules/core/trigonometric/unit/scalar/secpi.cpp:48"),
+    ("secpi", "-0.25", "sqrt(2)", 0.5, "modules/core/trigonometric/unit/scalar/secpi.cpp:50"),
+    ("secpi", "1", "-1", 0.5, "modules/core/trigonometric/unit/scalar/secpi.cpp:51"),
+    ("secpi", "0.25", "sqrt(2)", 0.5, "modules/core/trigonometric/unit/scalar/secpi.cpp:52"),
+    ("secpi", "0", "1", 0.5, "modules/core/trigonometric/unit/scalar/secpi.cpp:53"),
+    ("cscpi", "-0", "-inf", 0.5, "modules/core/trigonometric/unit/scalar/cscpi.cpp:44"),
+    ("cscpi", "inf", "nan", 0.5, "modules/core/trigonometric/unit/scalar/cscpi.cpp:45"),
+    ("cscpi", "-inf", "nan", 0.5, "modules/core/trigonometric/unit/scalar/cscpi.cpp:46"),
+    ("cscpi", "-1", "nan", 0.5, "modules/core/trigonometric/unit/scalar/cscpi.cpp:47"),
+    ("cscpi", "nan", "nan", 0.5, "modules/core/trigonometric/unit/scalar/cscpi.cpp:48"),
+    ("cscpi", "1", "nan", 0.5, "modules/core/trigonometric/unit/scalar/cscpi.cpp:49"),
+    ("cscpi", "0", "inf", 0.5, "modules/core/trigonometric/unit/scalar/cscpi.cpp:50"),
+    ("cscpi", "-0.25", "-sqrt(2)", 0.5, "modules/core/trigonometric/unit/scalar/cscpi.cpp:52"),
+    ("cscpi", "0.5", "1", 0.5, "modules/core/trigonometric/unit/scalar/cscpi.cpp:53"),
+    ("cscpi", "-0.5", "-1", 0.5, "modules/core/trigonometric/unit/scalar/cscpi.cpp:54"),
+    ("cscpi", "0.25", "sqrt(2)", 0.5, "modules/core/trigonometric/unit/scalar/cscpi.cpp:55"),
+    ("sinc", "inf", "0", 0.5, "modules/core/trigonometric/unit/scalar/sinc.cpp:43"),
+    ("sinc", "-inf", "0", 0.5, "modules/core/trigonometric/unit/scalar/sinc.cpp:44"),
+    ("sinc", "nan", "nan", 0.5, "modules/core/trigonometric/unit/scalar/sinc.cpp:45"),
+    ("sinc", "-pi/2", "0.6366197723675814", 0.5, "modules/core/trigonometric/unit/scalar/sinc.cpp:47"),
+    ("sinc", "-pi/4", "0.9003163161571061", 0.5, "modules/core/trigonometric/unit/scalar/sinc.cpp:48"),
+    ("sinc", "pi/2", "0.6366197723675814", 0.5, "modules/core/trigonometric/unit/scalar/sinc.cpp:49"),
+    ("sinc", "pi/4", "0.9003163161571061", 0.5, "modules/core/trigonometric/unit/scalar/sinc.cpp:50"),
+    ("sinc", "eps", "1", 0.5, "modules/core/trigonometric/unit/scalar/sinc.cpp:51"),
+    ("sinc", "mindenormal", "1", 0.5, "modules/core/trigonometric/unit/scalar/sinc.cpp:52"),
+    ("sinc", "0", "1", 0.5, "modules/core/trigonometric/unit/scalar/sinc.cpp:53"),
+    ("sincpi", "inf", "0", 1.0, "modules/core/trigonometric/unit/scalar/sincpi.cpp:41"),
+    ("sincpi", "-inf", "0", 1.0, "modules/core/trigonometric/unit/scalar/sincpi.cpp:42"),
+    ("sincpi", "nan", "nan", 1.0, "modules/core/trigonometric/unit/scalar/sincpi.cpp:43"),
+    ("sincpi", "-0.5", "0.6366197723675814", 1.0, "modules/core/trigonometric/unit/scalar/sincpi.cpp:45"),
+    ("sincpi", "-0.25", "0.9003163161571061", 1.0, "modules/core/trigonometric/unit/scalar/sincpi.cpp:46"),
+    ("sincpi", "0.5", "0.6366197723675814", 1.0, "modules/core/trigonometric/unit/scalar/sincpi.cpp:47"),
+    ("sincpi", "0.25", "0.9003163161571061", 1.0, "modules/core/trigonometric/unit/scalar/sincpi.cpp:48"),
+    ("sincpi", "eps", "1", 1.0, "modules/core/trigonometric/unit/scalar/sincpi.cpp:49"),
+    ("sincpi", "mindenormal", "1", 1.0, "modules/core/trigonometric/unit/scalar/sincpi.cpp:50"),
+    ("sincpi", "0", "1", 1.0, "modules/core/trigonometric/unit/scalar/sincpi.cpp:51"),
 ]
 
 

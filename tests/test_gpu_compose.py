@@ -1,0 +1,114 @@
+"""GPU parity of the composed elementwise functions (C ABI op codes 44-69 of
+nt2h_map_f32/f64 and atan2d of nt2h_map2): the reciprocal hyperbolic
+functions and their inverses, inverse trigonometric functions of reciprocals
+and in degree / pi units, reciprocal trigonometric functions of degree and pi
+arguments, and sinc.  Exact policy: bit-identical to the oracle on inputs that
+cover every branch (both signs, |x| near 1 and the poles, tiny and huge
+magnitudes, multiples of 1/4 and of 22.5, NaN, +-Inf, +-0) and on the golden
+inputs; packs and scalar tails both.  Fast policy: within 4 ulp of the exact
+policy, or, where NT2's definition subtracts two nearly equal terms (asec near
+|x| = 1, acotd at large |x|), within 4 ulp of the larger term."""
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+if not torch.cuda.is_available():
+    pytest.skip("no GPU", allow_module_level=True)
+
+import _oracle as O  # noqa: E402
+from _gpu import first_diff, same_bits, say  # noqa: E402
+from _ulp import classic_ulp  # noqa: E402
+from nt2_amd import EXACT, FAST, ops  # noqa: E402
+
+G = os.path.join(os.path.dirname(__file__), "golden")
+COMPOSED = ["coth", "sech", "csch", "acoth", "asech", "acsch", "acot", "asec", "acsc", "acotd", "asecd",
+            "acscd", "acosd", "acospi", "asind", "asinpi", "atand", "atanpi", "indeg", "inrad", "secd",
+            "cscd", "secpi", "cscpi", "sinc", "sincpi"]
+# the larger term of a cancelling difference in NT2's definition
+CANCEL_SCALE = {"asec": np.pi / 2, "asecd": 90.0, "acotd": 90.0}
+
+
+def _inputs(rng, n, dt):
+    k = n // 8
+    big = 3e38 if dt is np.float32 else 1e300
+    parts = [rng.normal(0, 10, k), rng.uniform(-2, 2, k),
+             (1 + np.exp2(rng.uniform(-40, -1, k))) * rng.choice([-1, 1], k),
+             (1 - np.exp2(rng.uniform(-40, -1, k))) * rng.choice([-1, 1], k),
+             np.exp2(rng.uniform(-60, 60, k)) * rng.choice([-1, 1], k),
+             rng.integers(-80, 80, k) * 0.25, rng.integers(-64, 64, k) * 22.5,
+             rng.uniform(-800, 800, n - 7 * k)]
+    spec = [0.0, -0.0, np.inf, -np.inf, np.nan, 1.0, -1.0, 0.5, -0.5, 0.625, 90.0, 180.0, -180.0, 45.0,
+            1e-30, -1e-30, big, -big, 1.0009, 1.0011, -1.0009, 2.0 ** -23, 2.0 ** -52, 5e-45]
+    return np.concatenate(parts + [spec]).astype(dt)
+
+
+def _golden(op, tag):
+    g = np.load(os.path.join(G, "math_%s.npz" % tag))
+    return g[op + "_x"] if op + "_x" in g else np.zeros(0)
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("op", COMPOSED)
+def test_composed_exact_bitwise(op, dt):
+    rng = np.random.default_rng(500 + COMPOSED.index(op) + (0 if dt is np.float32 else 100))
+    tag = "f32" if dt is np.float32 else "f64"
+    x = np.concatenate([_inputs(rng, 40001, dt), _golden(op, tag).astype(dt)])
+    mapf = O.map_f32 if dt is np.float32 else O.map_f64
+    say("composed exact", op, dt.__name__, "n=%d" % x.size)
+    got = ops.map(op, torch.from_numpy(x).cuda(), EXACT).cpu().numpy()
+    want = mapf(op, x)
+    assert same_bits(got, want), first_diff(got, want, x)
+    # unaligned start (element-wise loads) and a short table (tail only)
+    xu = torch.from_numpy(x[: 4099]).cuda()[1:]
+    got = ops.map(op, xu, EXACT).cpu().numpy()
+    assert same_bits(got, mapf(op, x[1:4099])), op
+    got = ops.map(op, torch.from_numpy(x[:3]).cuda(), EXACT).cpu().numpy()
+    assert same_bits(got, mapf(op, x[:3])), op
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+@pytest.mark.parametrize("op", COMPOSED)
+def test_composed_fast_close_to_exact(op, dt):
+    rng = np.random.default_rng(700 + COMPOSED.index(op))
+    x = _inputs(rng, 20001, dt)
+    say("composed fast", op, dt.__name__, "n=%d" % x.size)
+    xd = torch.from_numpy(x).cuda()
+    got = ops.map(op, xd, FAST).cpu().numpy()
+    ex = ops.map(op, xd, EXACT).cpu().numpy()
+    err = classic_ulp(got, ex.astype(np.float64), dt)
+    ok = (err <= 4) | (np.isnan(got) & np.isnan(ex)) | (got == ex)
+    eps = np.finfo(dt).eps
+    if op in CANCEL_SCALE:
+        ok |= np.abs(got.astype(np.float64) - ex) <= 4 * eps * CANCEL_SCALE[op]
+    if op in ("secd", "cscd", "secpi", "cscpi", "sinc", "sincpi", "coth", "csch"):
+        # near poles and zeros the inner kernels' policies differ by a few ulp
+        # of their own results, which the reciprocal carries relatively
+        with np.errstate(over="ignore", invalid="ignore", divide="ignore"):
+            ok |= np.abs(1.0 / got.astype(np.float64) - 1.0 / ex) <= 4 * eps
+        ok |= np.abs(got.astype(np.float64) - ex) <= 4 * eps
+    if op == "sinc":
+        # sin's policies differ by up to 4 eps absolutely where NT2's pack
+        # reduction leaves |xr| near pi/2 (the sin/cos tests allow the same);
+        # sinc divides that difference by x
+        ok |= np.abs(got.astype(np.float64) - ex) * np.abs(x.astype(np.float64)) <= 4 * eps
+    assert ok.all(), (x[~ok][:4], got[~ok][:4], ex[~ok][:4])
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_atan2d_exact_bitwise(dt):
+    rng = np.random.default_rng(31)
+    n = 20003
+    y = np.concatenate([rng.normal(0, 5, n), [0.0, -0.0, np.inf, -np.inf, np.nan, 1.0, 0.0, np.inf, 1.0]]).astype(dt)
+    x = np.concatenate([rng.normal(0, 5, n), [1.0, -1.0, np.inf, -np.inf, 0.0, 0.0, -0.0, 1.0, -np.inf]]).astype(dt)
+    say("atan2d", dt.__name__, "n=%d" % y.size)
+    yd, xd = torch.from_numpy(y).cuda(), torch.from_numpy(x).cuda()
+    got = ops.map2("atan2d", yd, xd, EXACT).cpu().numpy()
+    want = O.atan2d(y, x)
+    assert same_bits(got, want), first_diff(got, want, y)
+    fast = ops.map2("atan2d", yd, xd, FAST).cpu().numpy()
+    err = classic_ulp(fast, want.astype(np.float64), dt)
+    assert ((err <= 4) | (np.isnan(fast) & np.isnan(want))).all()

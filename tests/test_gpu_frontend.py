@@ -176,6 +176,28 @@ def test_exponential_rounding_statements(outputs):
         O.map_f32("tand", a * f32(30)) * O.map_f32("cotd", b * f32(40))
     got = outputs("sind_f32", f32)
     assert same_bits(got, want), first_diff(got, want)
+    m = O.map_f32
+    want = (m("coth", a) + m("sech", b) * m("csch", a * f32(3))) - m("acoth", b * f32(4))
+    got = outputs("coth_f32", f32)
+    assert same_bits(got, want), first_diff(got, want)
+    want = (m("asech", np.abs(a) / f32(4)) + m("acsch", b)) - m("acot", a) * m("asec", b * f32(3))
+    got = outputs("asech_f32", f32)
+    assert same_bits(got, want), first_diff(got, want)
+    want = ((m("acsc", a * f32(2)) + m("acotd", b)) - m("asecd", a * f32(5))) + m("acscd", b * f32(4))
+    got = outputs("acsc_f32", f32)
+    assert same_bits(got, want), first_diff(got, want)
+    want = (m("acosd", a / f32(4)) - m("acospi", b / f32(4))) + m("asind", a / f32(3)) * m("asinpi", b / f32(5))
+    got = outputs("acosd_f32", f32)
+    assert same_bits(got, want), first_diff(got, want)
+    want = ((m("atand", a) + m("atanpi", b)) + m("indeg", a)) - m("inrad", b * f32(100))
+    got = outputs("atand_f32", f32)
+    assert same_bits(got, want), first_diff(got, want)
+    want = (((m("secd", a * f32(90)) + m("cscd", b * f32(45))) - m("secpi", a) * m("cscpi", b / f32(3))) +
+            m("sinc", a * f32(4))) + m("sincpi", b)
+    got = outputs("secd_f32", f32)
+    assert same_bits(got, want), first_diff(got, want)
+    got = outputs("atan2d_f32", f32)
+    assert same_bits(got, O.atan2d(a, b)), first_diff(got, O.atan2d(a, b))
     want = np.where((a > 0) & (b < 1), a, b) + np.where((a < -1) | (b > 2), f32(1), f32(0))
     got = outputs("logical_f32", f32)
     assert same_bits(got, want.astype(f32)), first_diff(got, want)
@@ -217,6 +239,17 @@ def test_exponential_rounding_statements(outputs):
     want = (O.map_f64("sind", q * 500.0) - O.map_f64("cosd", q * 300.0)) + \
         O.map_f64("tand", q * 200.0) * O.map_f64("cotd", q * 170.0)
     got = outputs("sind_f64", np.float64)
+    assert same_bits(got, want), first_diff(got, want)
+    m = O.map_f64
+    want = (m("coth", q * 3.0) - m("asech", np.abs(q))) + m("acot", q) * m("asec", q * 30.0)
+    got = outputs("coth_f64", np.float64)
+    assert same_bits(got, want), first_diff(got, want)
+    want = ((m("acotd", q * 7.0) + m("acscd", q * 20.0)) - m("asind", q * 0.9)) + m("atanpi", q * 9.0)
+    got = outputs("acotd_f64", np.float64)
+    assert same_bits(got, want), first_diff(got, want)
+    want = ((m("secd", q * 900.0) - m("cscpi", q * 6.0)) + m("sinc", q * 40.0) * m("sincpi", q * 5.0)) + \
+        O.atan2d(q, q * 0.5 - 0.1)
+    got = outputs("secd_f64", np.float64)
     assert same_bits(got, want), first_diff(got, want)
 
 

@@ -22,6 +22,15 @@ def _val(s, dt):
              "pi/3": np.pi / 3, "2pi/3": 2 * np.pi / 3, "sqrt(2)/2": np.sqrt(2) / 2, "-0": -0.0,
              "-sqrt(2)/2": -np.sqrt(2) / 2, "sqrt(2)": np.sqrt(2), "-sqrt(2)": -np.sqrt(2),
              "ln2": np.log(2.0), "eps": np.finfo(dt).eps, "tiny": np.finfo(dt).tiny}
+    if s == "1/eps":
+        return dt(1) / np.finfo(dt).eps
+    if s == "mindenormal":
+        return np.finfo(dt).smallest_subnormal
+    if s in ("1/3", "2/3", "1/6", "-1/6"):  # T(1)/3 etc. in the target precision
+        neg = s.startswith("-")
+        num, den = s.lstrip("-").split("/")
+        v = dt(int(num)) / dt(int(den))
+        return -v if neg else v
     if s == "e-1":  # NT2 forms these in the target precision: Exp_1<T>() - One<T>()
         return dt(np.e) - dt(1)
     if s == "1/e-1":
@@ -47,6 +56,10 @@ def test_nt2_unit_kats(dt):
         got = _iso(mapf, k["func"], np.array([x], dtype=dt), W)[0]
         d = nt2_ulpdist(dt(got), dt(exp))
         assert d <= k["tol_ulpdist"], (k, got, d)
+        if "/unit/scalar/" in k["source"]:  # NT2's scalar test: the scalar form (a tail) too
+            got = mapf(k["func"], np.array([x], dtype=dt))[0]
+            d = nt2_ulpdist(dt(got), dt(exp))
+            assert d <= k["tol_ulpdist"], (k, "tail", got, d)
 
 
 def _bound(name, x, y, dt):
@@ -57,6 +70,8 @@ def _bound(name, x, y, dt):
         return np.full(x.shape, 1.0)
     if name in ("expm1", "log1p", "tanh", "sinh", "cosh", "asin", "acos", "cbrt", "asinh", "acosh", "atanh"):
         return np.full(x.shape, BOUND3[name][dt])
+    if name in COMPOSED_BOUND:
+        return np.full(x.shape, COMPOSED_BOUND[name][0 if dt is np.float32 else 1])
     if name in TRIG_VARIANTS:
         # 2 ulp away from poles (the reduced argument and the polynomial round)
         return np.where(np.abs(y.astype(np.float64)) < 1e4, 2.0, np.inf)
@@ -102,7 +117,10 @@ def _bound(name, x, y, dt):
 @pytest.mark.parametrize("name", ["exp", "log", "erfc", "normcdf", "cos", "sin", "exp2", "exp10", "log2", "log10",
                                   "expm1", "log1p", "tanh", "sinh", "cosh", "tan", "asin", "acos",
                                   "erf", "cbrt", "asinh", "acosh", "atanh", "cot", "sec", "csc", "sinpi",
-                                  "cospi", "tanpi", "cotpi", "sind", "cosd", "tand", "cotd"])
+                                  "cospi", "tanpi", "cotpi", "sind", "cosd", "tand", "cotd", "coth", "sech", "csch",
+                                  "acoth", "asech", "acsch", "acot", "asec", "acsc", "acotd", "asecd", "acscd",
+                                  "acosd", "acospi", "asind", "asinpi", "atand", "atanpi", "indeg", "inrad", "secd",
+                                  "cscd", "secpi", "cscpi", "sinc", "sincpi"])
 def test_golden_mpmath(dt, tag, name):
     g = np.load(os.path.join(G, "math_%s.npz" % tag))
     x, y = g[name + "_x"], g[name + "_y"]
@@ -160,6 +178,18 @@ BOUND3 = {"expm1": {np.float32: 2.0, np.float64: 1.0}, "log1p": {np.float32: 1.0
           # asinh, acosh, atanh: log / log1p of a rounded argument
           "cbrt": {np.float32: 1.0, np.float64: 1.0}, "asinh": {np.float32: 2.0, np.float64: 2.0},
           "acosh": {np.float32: 2.0, np.float64: 1.0}, "atanh": {np.float32: 1.0, np.float64: 1.0}}
+
+# compositions (classic ulp, f32 / f64) on their golden domains, where the
+# inner argument's rounding is well conditioned: the inner kernel's bound,
+# plus the reciprocal's or the outer product's rounding (measured maxima
+# 0.7-5 ulp); acotd subtracts atand(|x|) from 90, asind/acscd scale
+# asin's 2-ulp f32 bound by indeg, cscd carries the rounding of x/180
+COMPOSED_BOUND = {"coth": (2, 2), "sech": (3, 2), "csch": (3, 2), "acoth": (3, 2), "asech": (3, 2),
+                  "acsch": (3, 2), "acot": (2, 2), "asec": (2, 2), "acsc": (3, 2), "acotd": (4, 6),
+                  "asecd": (3, 2), "acscd": (4, 3), "acosd": (3, 2), "acospi": (3, 2), "asind": (5, 2),
+                  "asinpi": (3, 2), "atand": (3, 2), "atanpi": (3, 2), "indeg": (1, 1), "inrad": (1, 1),
+                  "secd": (3, 2), "cscd": (4, 3), "secpi": (3, 2), "cscpi": (3, 2), "sinc": (2, 2),
+                  "sincpi": (3, 3)}
 
 TRIG_VARIANTS = ("cot", "sec", "csc", "sinpi", "cospi", "tanpi", "cotpi", "sind", "cosd", "tand", "cotd")
 UNIT_DOMAIN32 = {"tanpi": 2.0 ** 22, "sind": 2.0 ** 24, "cosd": 2.0 ** 24, "tand": 2.0 ** 24, "cotd": 2.0 ** 24}
@@ -705,3 +735,107 @@ def test_trig_units_nt2_limits():
     huge = np.array([5e11, 8.2e11, 1e12], np.float32)
     assert np.all(np.abs(_iso(O.map_f32, "sind", huge, 8)) > 1e3)
     assert np.all(np.abs(_iso(O.map_f64, "sind", huge.astype(np.float64), 4)) <= 1)
+
+
+def _one_lane(mapf, name, x, W, tail):
+    """Each input alone: in a pack of identical lanes, or as a one-element
+    table (NT2's scalar form)."""
+    if tail:
+        return np.array([mapf(name, x[i:i + 1])[0] for i in range(x.size)], dtype=x.dtype)
+    return _iso(mapf, name, x, W)
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_composed_rounded_argument(dt):
+    """Where NT2's composition is ill conditioned (|x| near 1 for the
+    functions of 1/x, x near multiples of 180 for cscd, large |x| for
+    acotd's 90 - atand(|x|)), the oracle follows the composition with the
+    inner argument rounded as NT2 rounds it: mpmath at that argument, within
+    the inner kernel's bound (a dropped reciprocal, a wrong constant or a
+    missing rounding fails by orders of magnitude here)."""
+    mpmath.mp.dps = 40
+    rng = np.random.default_rng(77)
+    mapf = O.map_f32 if dt is np.float32 else O.map_f64
+    W = 8 if dt is np.float32 else 4
+    eps = float(np.finfo(dt).eps)
+    near1 = ((1 + np.exp2(rng.uniform(-20 if dt is np.float32 else -45, -3, 300))) *
+             rng.choice([-1.0, 1.0], 300)).astype(dt)
+    rec = (dt(1) / near1).astype(dt)
+    m = lambda f, v: np.array([float(f(mpmath.mpf(float(a)))) for a in v])  # noqa: E731
+    for tail in (False, True):
+        cases = [("acoth", m(mpmath.atanh, rec), 2), ("acsc", m(mpmath.asin, rec), 2),
+                 ("acscd", m(lambda u: mpmath.degrees(mpmath.asin(u)), rec), 3)]
+        for name, ref, b in cases:
+            got = _one_lane(mapf, name, near1, W, tail)
+            err = classic_ulp(got, ref, dt)
+            assert (err <= b).all(), (name, tail, near1[err > b][:3], err.max())
+        pos = np.abs(near1[:150])
+        got = _one_lane(mapf, "asech", (dt(1) / pos).astype(dt), W, tail)
+        ref = m(mpmath.acosh, pos)  # 1/(1/x) rounds back to x here only approximately:
+        rr = (dt(1) / (dt(1) / pos).astype(dt)).astype(dt)
+        ref = m(mpmath.acosh, rr)
+        assert (classic_ulp(got, ref, dt) <= 2).all(), "asech"
+    # asec's pack form: Pio_2 - acsc(x), the difference of two terms near pi/2
+    got = _iso(mapf, "asec", near1, W)
+    ref = float(mpmath.pi / 2) - m(mpmath.asin, rec)
+    assert (np.abs(got - ref) <= 3 * eps * np.pi / 2).all()
+    # acotd at large |x|: 90 - atand(|x|)
+    big = (np.exp2(rng.uniform(1, 60, 300)) * rng.choice([-1.0, 1.0], 300)).astype(dt)
+    got = _iso(mapf, "acotd", big, W)
+    ref = np.sign(big) * (90.0 - m(lambda u: mpmath.degrees(mpmath.atan(abs(u))), big))
+    assert (np.abs(got - ref) <= 4 * eps * 90).all()
+    # cscd near its poles: cscpi of x * Oneo_180 rounded in the target precision
+    x = (rng.integers(-8, 8, 300) * 180.0 + rng.uniform(-3, 3, 300)).astype(dt)
+    x = x[x != 0]
+    oneo180 = dt(np.frombuffer(np.uint32(0x3bb60b61).tobytes(), np.float32)[0]) if dt is np.float32 else \
+        np.frombuffer(np.uint64(0x3f76c16c16c16c17).tobytes(), np.float64)[0]
+    y = (x * oneo180).astype(dt)
+    ref = m(lambda u: 1 / mpmath.sinpi(u), y)
+    assert (classic_ulp(_iso(mapf, "cscd", x, W), ref, dt) <= 3).all()
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_composed_structure(dt):
+    """Parity symmetries bit for bit (NT2's definitions are odd or even by
+    construction), the f32 scalar asec series near |x| = 1 (accurate where the
+    pack form cancels), atan2d = indeg(atan2) against mpmath, and sinc at the
+    Eps threshold."""
+    mpmath.mp.dps = 40
+    rng = np.random.default_rng(78)
+    mapf = O.map_f32 if dt is np.float32 else O.map_f64
+    W = 8 if dt is np.float32 else 4
+    x = np.concatenate([rng.normal(0, 5, 600), np.exp2(rng.uniform(-40, 40, 200)),
+                        1 + np.exp2(rng.uniform(-30, -1, 100))]).astype(dt)
+    x = np.abs(x)
+    odd = ["coth", "csch", "acoth", "acsch", "acot", "acsc", "acotd", "acscd", "asind", "asinpi", "atand",
+           "atanpi", "indeg", "inrad", "cscd", "cscpi"]
+    even = ["sech", "secd", "secpi", "sinc", "sincpi"]
+    for name in odd + even:
+        for tail in (False, True):
+            p = _one_lane(mapf, name, x, W, tail)
+            n = _one_lane(mapf, name, -x, W, tail)
+            want = -p if name in odd else p
+            same = (n.view(np.uint32 if dt is np.float32 else np.uint64) ==
+                    want.view(np.uint32 if dt is np.float32 else np.uint64)) | (np.isnan(n) & np.isnan(want))
+            assert same.all(), (name, tail, x[~same][:3])
+    # asec(1 + e), e < 0.001: NT2's f32 scalar form uses a series (accurate to
+    # its truncation), the pack form Pio_2 - acsc cancels; f64 has no series
+    e = np.exp2(rng.uniform(-22 if dt is np.float32 else -50, -10, 300))
+    xs = np.concatenate([1 + e, -(1 + e)]).astype(dt)
+    ref = np.array([float(mpmath.asec(mpmath.mpf(float(v)))) for v in xs])
+    tail = _one_lane(mapf, "asec", xs, W, True)
+    if dt is np.float32:  # the truncated series: up to 6.5 ulp at e = 0.001
+        assert (classic_ulp(tail, ref, dt) <= 8).all()
+        assert classic_ulp(_iso(mapf, "asec", xs, W), ref, dt).max() > 50
+    # atan2d
+    a = rng.normal(0, 3, 3000).astype(dt)
+    b = rng.normal(0, 3, 3000).astype(dt)
+    got = O.atan2d(a, b)
+    ref = np.array([float(mpmath.degrees(mpmath.atan2(float(u), float(v)))) for u, v in zip(a, b)])
+    assert (classic_ulp(got, ref, dt) <= 3).all()
+    assert np.array_equal(O.atan2d(a, b), mapf("indeg", O.atan2(a, b)))
+    # sinc and sincpi are exactly 1 below Eps, and sin(x)/x just above it
+    eps = np.finfo(dt).eps
+    t = np.array([eps * 0.75, eps, eps * 1.5, -eps * 0.75], dtype=dt)
+    assert list(_iso(mapf, "sinc", t, W)[[0, 3]]) == [1.0, 1.0]
+    assert list(_iso(mapf, "sincpi", t, W)[[0, 3]]) == [1.0, 1.0]
