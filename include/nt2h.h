@@ -119,6 +119,12 @@ extern "C" {
 #define NT2H_OP_CSCPI 67  /* 1/sinpi, NaN at nonzero integers */
 #define NT2H_OP_SINC 68   /* sin(x)/x, radian: per-pack range reduction */
 #define NT2H_OP_SINCPI 69 /* sinpi(x)/(pi x) */
+#define NT2H_OP_SQRT1PM1 70  /* sqrt(1 + x) - 1 */
+#define NT2H_OP_EXPRECNEG 71 /* exp(-1/x) */
+#define NT2H_OP_EXPRECNEGC 72 /* -expm1(-1/x) */
+#define NT2H_OP_REALLOG 73   /* log */
+#define NT2H_OP_REALSQRT 74  /* sqrt */
+#define NT2H_OP_EXPX2 75     /* exp(x^2): exponential/functions/simd/common/expx2.hpp:37-61 */
 
 /* Black-Scholes formula variants */
 #define NT2H_BS_TEXTBOOK 0 /* d1 = (log(S/X)+(r+v^2/2)T)/(v sqrt T), call and put, erfc-based normcdf */
@@ -170,6 +176,8 @@ int nt2h_map_f64(int op, size_t n, const double* x, double* y, int policy, void*
 #define NT2H_OP2_ATAN2 0
 #define NT2H_OP2_POW 1
 #define NT2H_OP2_ATAN2D 2 /* atan2 in degrees: trigonometric/functions/generic/atan2d.hpp:16-27 */
+#define NT2H_OP2_LOGSPACE_ADD 3 /* log(exp(x) + exp(y)): exponential/functions/scalar/logspace_add.hpp:25-42 */
+#define NT2H_OP2_LOGSPACE_SUB 4 /* log(exp(x) - exp(y)): exponential/functions/scalar/logspace_sub.hpp:20-32 */
 int nt2h_map2_f32(int op, size_t n, const float* x, const float* y, float* out, int policy, void* stream);
 int nt2h_map2_f64(int op, size_t n, const double* x, const double* y, double* out, int policy, void* stream);
 /* [s, c] = sincos(x): modules/core/trigonometric/include/nt2/trigonometric/functions/generic/sincos.hpp:50-64 */

@@ -1,8 +1,9 @@
 // Elementwise maps of NT2's composed functions (C ABI op codes NT2H_OP_COTH
-// to NT2H_OP_SINCPI): the reciprocal hyperbolic functions and their inverses,
+// to NT2H_OP_EXPX2): the reciprocal hyperbolic functions and their inverses,
 // the inverse trigonometric functions of reciprocals and in degree or pi
 // units, the reciprocal trigonometric functions of degree and pi arguments,
-// and sinc.  The schedule is the other maps' (packs of 8 floats / 4 doubles
+// sinc, and the exponential family's sqrt1pm1, exprecneg(c), reallog,
+// realsqrt and expx2.  The schedule is the other maps' (packs of 8 floats / 4 doubles
 // from index 0, NT2's scalar form for the tail); the functions are in
 // nt2/hip/math_compose.hpp.  A translation unit of its own so that the
 // library's sources compile in parallel.
@@ -57,7 +58,13 @@ template <class P, int OP> struct compose_op {
         else if constexpr (OP == NT2H_OP_CSCD) v = dm::cscd_t<P>(x);
         else if constexpr (OP == NT2H_OP_SECPI) v = dm::secpi_t<P>(x);
         else if constexpr (OP == NT2H_OP_CSCPI) v = dm::cscpi_t<P>(x);
-        else v = dm::sincpi_t<P, S>(x);
+        else if constexpr (OP == NT2H_OP_SINCPI) v = dm::sincpi_t<P, S>(x);
+        else if constexpr (OP == NT2H_OP_SQRT1PM1) v = dm::sqrt1pm1(x);
+        else if constexpr (OP == NT2H_OP_EXPRECNEG) v = dm::exprecneg_t<P>(x);
+        else if constexpr (OP == NT2H_OP_EXPRECNEGC) v = dm::exprecnegc_t<P>(x);
+        else if constexpr (OP == NT2H_OP_REALLOG) v = dm::log_t<P>(x);
+        else if constexpr (OP == NT2H_OP_REALSQRT) v = dm::sqrtv(x);
+        else v = dm::expx2_t<P, S>(x);
         r[l] = v;
       }
     }
@@ -84,6 +91,8 @@ static int launch_t(const char* name, int op, size_t n, const T* x, T* y, hipStr
     NT2H_COMPOSE(NT2H_OP_INDEG) NT2H_COMPOSE(NT2H_OP_INRAD) NT2H_COMPOSE(NT2H_OP_SECD)
     NT2H_COMPOSE(NT2H_OP_CSCD) NT2H_COMPOSE(NT2H_OP_SECPI) NT2H_COMPOSE(NT2H_OP_CSCPI)
     NT2H_COMPOSE(NT2H_OP_SINC) NT2H_COMPOSE(NT2H_OP_SINCPI)
+    NT2H_COMPOSE(NT2H_OP_SQRT1PM1) NT2H_COMPOSE(NT2H_OP_EXPRECNEG) NT2H_COMPOSE(NT2H_OP_EXPRECNEGC)
+    NT2H_COMPOSE(NT2H_OP_REALLOG) NT2H_COMPOSE(NT2H_OP_REALSQRT) NT2H_COMPOSE(NT2H_OP_EXPX2)
 #undef NT2H_COMPOSE
     default: return fail(NT2H_EINVAL, name);
   }

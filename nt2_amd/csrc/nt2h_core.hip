@@ -284,6 +284,8 @@ __device__ __forceinline__ T apply2_one(T a, T b) {
   constexpr bool scalar = (G == 1);
   if constexpr (OP == NT2H_OP2_POW) return dm::pow_t<P, scalar, T>(a, b);
   else if constexpr (OP == NT2H_OP2_ATAN2D) return dm::atan2d_t<P, scalar>(a, b);
+  else if constexpr (OP == NT2H_OP2_LOGSPACE_ADD) return dm::logspace_add_t<P>(a, b);
+  else if constexpr (OP == NT2H_OP2_LOGSPACE_SUB) return dm::logspace_sub_t<P>(a, b);
   else if constexpr (sizeof(T) == 4) return dm::atan2_f<P, scalar>(a, b);
   else return dm::atan2_d<P, scalar>(a, b);
 }
@@ -1148,13 +1150,19 @@ int launch_map2(const char* name, int op, size_t n, const T* x, const T* y, T* o
                        void* stream) {
   if (n == 0) return NT2H_OK;
   if (!x || !y || !out) return fail(NT2H_EINVAL, name);
-  if (op != NT2H_OP2_ATAN2 && op != NT2H_OP2_POW && op != NT2H_OP2_ATAN2D) return fail(NT2H_EINVAL, name);
+  if (op < NT2H_OP2_ATAN2 || op > NT2H_OP2_LOGSPACE_SUB) return fail(NT2H_EINVAL, name);
   if (policy != NT2H_EXACT && policy != NT2H_FAST) return fail(NT2H_EINVAL, name);
   const dim3 g(grid_for(n / W + n % W)), b(kBlock);
   hipStream_t s = as_stream(stream);
   if (op == NT2H_OP2_ATAN2) {
     if (policy == NT2H_EXACT) hipLaunchKernelGGL((map2_k<exact_policy, NT2H_OP2_ATAN2, T, W>), g, b, 0, s, x, y, out, n);
     else hipLaunchKernelGGL((map2_k<fast_policy, NT2H_OP2_ATAN2, T, W>), g, b, 0, s, x, y, out, n);
+  } else if (op == NT2H_OP2_LOGSPACE_ADD) {
+    if (policy == NT2H_EXACT) hipLaunchKernelGGL((map2_k<exact_policy, NT2H_OP2_LOGSPACE_ADD, T, W>), g, b, 0, s, x, y, out, n);
+    else hipLaunchKernelGGL((map2_k<fast_policy, NT2H_OP2_LOGSPACE_ADD, T, W>), g, b, 0, s, x, y, out, n);
+  } else if (op == NT2H_OP2_LOGSPACE_SUB) {
+    if (policy == NT2H_EXACT) hipLaunchKernelGGL((map2_k<exact_policy, NT2H_OP2_LOGSPACE_SUB, T, W>), g, b, 0, s, x, y, out, n);
+    else hipLaunchKernelGGL((map2_k<fast_policy, NT2H_OP2_LOGSPACE_SUB, T, W>), g, b, 0, s, x, y, out, n);
   } else if (op == NT2H_OP2_ATAN2D) {
     if (policy == NT2H_EXACT) hipLaunchKernelGGL((map2_k<exact_policy, NT2H_OP2_ATAN2D, T, W>), g, b, 0, s, x, y, out, n);
     else hipLaunchKernelGGL((map2_k<fast_policy, NT2H_OP2_ATAN2D, T, W>), g, b, 0, s, x, y, out, n);

@@ -31,6 +31,12 @@ template <class P> NT2H_FN double sinh_t(double x) { return sinh_d<P>(x); }
 template <class P> NT2H_FN float cosh_t(float x) { return cosh_f<P>(x); }
 template <class P> NT2H_FN double cosh_t(double x) { return cosh_d<P>(x); }
 template <class P> NT2H_FN float exp_t(float x) { return exp_f<P>(x); }
+template <class P> NT2H_FN float expm1_t(float x) { return expm1_f<P>(x); }
+template <class P> NT2H_FN double expm1_t(double x) { return expm1_d<P>(x); }
+template <class P> NT2H_FN float log_t(float x) { return log_f<P>(x); }
+template <class P> NT2H_FN double log_t(double x) { return log_d<P>(x); }
+template <class P> NT2H_FN float log1p_t(float x) { return log1p_f<P>(x); }
+template <class P> NT2H_FN double log1p_t(double x) { return log1p_d<P>(x); }
 template <class P> NT2H_FN double exp_t(double x) { return exp_d<P>(x); }
 template <class P, int U> NT2H_FN float trigu_t(int kind, float x) { return trigu_f<P, U>(kind, x); }
 template <class P, int U> NT2H_FN double trigu_t(int kind, double x) { return trigu_d<P, U>(kind, x); }
@@ -52,6 +58,15 @@ template <class T> NT2H_FN T c_nan() { return cst<T>(0x7fc00000u, 0x7ff800000000
 template <class T> NT2H_FN T absv(T x) {
   if constexpr (sizeof(T) == 4) return __builtin_fabsf(x);
   else return __builtin_fabs(x);
+}
+
+template <class T> NT2H_FN T sqrtv(T x) {
+  if constexpr (sizeof(T) == 4) return __builtin_sqrtf(x);
+  else return __builtin_sqrt(x);
+}
+template <class T> NT2H_FN T floorv(T x) {
+  if constexpr (sizeof(T) == 4) return __builtin_floorf(x);
+  else return __builtin_floor(x);
 }
 
 // r with a0's sign bit xored in (b_xor(r, bitofsign(a0))) or or-ed in (b_or)
@@ -214,6 +229,63 @@ template <class P, bool S, class T> NT2H_FN T sincpi_t(T a0) {
   T v = S ? (c_invpi<T>() * sp) / a0 : c_invpi<T>() * (sp / a0);
   v = absv(a0) < c_eps<T>() ? T(1) : v;
   return __builtin_isinf(a0) ? T(0) : v;
+}
+
+// ---- exponential family ------------------------------------------------------------
+// (modules/core/exponential/include/nt2/exponential/functions/)
+// sqrt1pm1: generic/sqrt1pm1.hpp:23-36, t = sqrt(1 + a0); a0/(1 + t) for
+// |a0| < 1/2, else t - 1
+template <class T> NT2H_FN T sqrt1pm1(T a0) {
+  NT2H_STRICT
+  const T t = sqrtv(T(1) + a0);
+  return absv(a0) < T(0.5) ? a0 / (T(1) + t) : t - T(1);
+}
+// exprecneg: generic/exprecneg.hpp:18-28, exp(-rec(a0));
+// exprecnegc: generic/exprecnegc.hpp:18-28, -expm1(-rec(a0))
+template <class P, class T> NT2H_FN T exprecneg_t(T a0) { NT2H_STRICT return exp_t<P>(-(T(1) / a0)); }
+template <class P, class T> NT2H_FN T exprecnegc_t(T a0) { NT2H_STRICT return -expm1_t<P>(-(T(1) / a0)); }
+// expx2: exp(a0^2) with a0 = m + f split (constants expx2c1.hpp:29-30 and
+// expx2c2.hpp:29-30: m a multiple of 32 in float, 128 in double), Inf above
+// Maxlog (maxlog.hpp:29-30) and at +-Inf.  The pack form (simd/common/
+// expx2.hpp:37-61) forms m with fma(c2, x, 1/2) and u1 = fma(2m, f, f^2); the
+// scalar form (scalar/expx2.hpp:32-54) with plain products, the same bits
+// when fma is two roundings.
+template <class P, bool S, class T> NT2H_FN T expx2_t(T a0) {
+  NT2H_STRICT
+  const T c1 = cst<T>(0x42000000u, 0x4060000000000000ull), c2 = cst<T>(0x3d000000u, 0x3f80000000000000ull);
+  const T maxlog = cst<T>(0x42b0c0a5u, 0x40862e42fefa39efull);
+  const T x = absv(a0);
+  T m, f, u1;
+  if constexpr (S) {
+    m = c1 * floorv(c2 * x + T(0.5));
+    f = x - m;
+    u1 = T(2) * m * f + f * f;
+  } else {
+    m = c1 * floorv(fma<P>(c2, x, T(0.5)));
+    f = x - m;
+    u1 = fma<P>(m + m, f, f * f);
+  }
+  const T u = m * m;
+  const T r = (u + u1 > maxlog) ? cst<T>(0x7f800000u, 0x7ff0000000000000ull) : exp_t<P>(u) * exp_t<P>(u1);
+  return __builtin_isinf(x) ? x : r;
+}
+// logspace_add: scalar/logspace_add.hpp:25-42 (simd/common/logspace_add.hpp:30-46
+// the same per lane), max(a0, a1) + log1p(exp(-|a0 - a1|)), a0 + a1 where the
+// difference is NaN
+template <class P, class T> NT2H_FN T logspace_add_t(T a0, T a1) {
+  NT2H_STRICT
+  const T tmp = -absv(a0 - a1);
+  const T r = (a0 > a1 ? a0 : a1) + log1p_t<P>(exp_t<P>(tmp));
+  return tmp != tmp ? a0 + a1 : r;
+}
+// logspace_sub: scalar/logspace_sub.hpp:20-32 (simd/common/logspace_sub.hpp:26-41
+// the same per lane), x = a1 - a0; a0 + (x > -Log_2 ? log(-expm1(x)) :
+// log1p(-exp(x))) (Log_2: constants/log_2.hpp:29-30)
+template <class P, class T> NT2H_FN T logspace_sub_t(T a0, T a1) {
+  NT2H_STRICT
+  const T x = a1 - a0;
+  const T tmp = x > -cst<T>(0x3f317218u, 0x3fe62e42fefa39efull) ? log_t<P>(-expm1_t<P>(x)) : log1p_t<P>(-exp_t<P>(x));
+  return a0 + tmp;
 }
 
 }}}  // namespace nt2::hip::dm

@@ -224,6 +224,8 @@ NT2H_TAG(acosd_, same1p) NT2H_TAG(acospi_, same1p) NT2H_TAG(asind_, same1p) NT2H
 NT2H_TAG(atand_, same1p) NT2H_TAG(atanpi_, same1p) NT2H_TAG(indeg_, same1) NT2H_TAG(inrad_, same1)
 NT2H_TAG(secd_, same1) NT2H_TAG(cscd_, same1) NT2H_TAG(secpi_, same1) NT2H_TAG(cscpi_, same1)
 NT2H_TAG(sinc_, same1p) NT2H_TAG(sincpi_, same1p) NT2H_TAG(atan2d_, same2)
+NT2H_TAG(sqrt1pm1_, same1) NT2H_TAG(exprecneg_, same1) NT2H_TAG(exprecnegc_, same1) NT2H_TAG(reallog_, same1)
+NT2H_TAG(realsqrt_, same1) NT2H_TAG(expx2_, same1p) NT2H_TAG(logspace_add_, same2) NT2H_TAG(logspace_sub_, same2)
 NT2H_TAG(pow_, pow2)
 NT2H_TAG(fma_, same3) NT2H_TAG(fnms_, same3) NT2H_TAG(if_else_, sel3)
 NT2H_TAG(selinc_, selinc2) NT2H_TAG(seldec_, selinc2) NT2H_TAG(if_zero_else_one_, izeo1)
@@ -385,6 +387,13 @@ NT2H_UN(secd_, dm::secd_t<P>(x), dm::secd_t<P>(x), no_integer_form<T>())
 NT2H_UN(cscd_, dm::cscd_t<P>(x), dm::cscd_t<P>(x), no_integer_form<T>())
 NT2H_UN(secpi_, dm::secpi_t<P>(x), dm::secpi_t<P>(x), no_integer_form<T>())
 NT2H_UN(cscpi_, dm::cscpi_t<P>(x), dm::cscpi_t<P>(x), no_integer_form<T>())
+NT2H_UN(sqrt1pm1_, dm::sqrt1pm1(x), dm::sqrt1pm1(x), no_integer_form<T>())
+NT2H_UN(exprecneg_, dm::exprecneg_t<P>(x), dm::exprecneg_t<P>(x), no_integer_form<T>())
+NT2H_UN(exprecnegc_, dm::exprecnegc_t<P>(x), dm::exprecnegc_t<P>(x), no_integer_form<T>())
+NT2H_UN(reallog_, dm::log_t<P>(x), dm::log_t<P>(x), no_integer_form<T>())
+NT2H_UN(realsqrt_, dm::sqrtv(x), dm::sqrtv(x), no_integer_form<T>())
+NT2H_UN(expx2_, (G == 1 ? dm::expx2_t<P, true>(x) : dm::expx2_t<P, false>(x)),
+        (G == 1 ? dm::expx2_t<P, true>(x) : dm::expx2_t<P, false>(x)), no_integer_form<T>())
 #undef NT2H_UN
 template <> struct apply1<tagfn::sinc_> {
   template <int G, class T> __device__ __forceinline__ static void run(T (&a)[G], T (&r)[G]) {
@@ -822,13 +831,15 @@ NT2H_FN1(acosd, acosd_) NT2H_FN1(acospi, acospi_) NT2H_FN1(asind, asind_) NT2H_F
 NT2H_FN1(atand, atand_) NT2H_FN1(atanpi, atanpi_) NT2H_FN1(indeg, indeg_) NT2H_FN1(inrad, inrad_)
 NT2H_FN1(secd, secd_) NT2H_FN1(cscd, cscd_) NT2H_FN1(secpi, secpi_) NT2H_FN1(cscpi, cscpi_)
 NT2H_FN1(sinc, sinc_) NT2H_FN1(sincpi, sincpi_)
+NT2H_FN1(sqrt1pm1, sqrt1pm1_) NT2H_FN1(exprecneg, exprecneg_) NT2H_FN1(exprecnegc, exprecnegc_)
+NT2H_FN1(reallog, reallog_) NT2H_FN1(realsqrt, realsqrt_) NT2H_FN1(expx2, expx2_)
 NT2H_FN1(if_zero_else_one, if_zero_else_one_)
 #undef NT2H_FN1
 #define NT2H_FN2(name, Tag) \
   template <class A, class B, NT2H_ENABLE2(A, B)> auto name(const A& a, const B& b) { return make2<tagfn::Tag>(a, b); }
 NT2H_FN2(max, max_) NT2H_FN2(min, min_) NT2H_FN2(adds, adds_) NT2H_FN2(subs, subs_)
 NT2H_FN2(selinc, selinc_) NT2H_FN2(seldec, seldec_) NT2H_FN2(atan2, atan2_)
-NT2H_FN2(atan2d, atan2d_)
+NT2H_FN2(atan2d, atan2d_) NT2H_FN2(logspace_add, logspace_add_) NT2H_FN2(logspace_sub, logspace_sub_)
 #undef NT2H_FN2
 // pow(a, b): Cephes' pow for floating-point exponents; an integral exponent
 // (a scalar, kept in its own type, or an integer table) takes NT2's repeated
@@ -1158,6 +1169,12 @@ __device__ __forceinline__ void evn(C& c, const n2v<Tag, VA, VB>& v, size_t i, R
         r[l] = G == 1 ? dm::normcdf_d<P, true>((double)a[l]) : dm::normcdf_join_d((double)a[l], (double)b[l]);
       else r[l] = no_integer_form<R>();
     }
+  } else if constexpr (std::is_same<Tag, tagfn::logspace_add_>::value || std::is_same<Tag, tagfn::logspace_sub_>::value) {
+    static_assert(std::is_floating_point<R>::value, "logspace_add / logspace_sub are defined for floating-point tables only");
+#pragma unroll
+    for (int l = 0; l < G; ++l)
+      r[l] = std::is_same<Tag, tagfn::logspace_add_>::value ? dm::logspace_add_t<P>((R)a[l], (R)b[l])
+                                                            : dm::logspace_sub_t<P>((R)a[l], (R)b[l]);
   } else if constexpr (std::is_same<Tag, tagfn::atan2d_>::value) {
     static_assert(std::is_floating_point<R>::value, "atan2d is defined for floating-point tables only");
 #pragma unroll

@@ -8,7 +8,7 @@ LIB_NAME = "libnt2h.so"
 
 EXACT, FAST, ACCURATE = 0, 1, 2
 BS_TEXTBOOK, BS_DEMO = 0, 1
-OPS2 = {"atan2": 0, "pow": 1, "atan2d": 2}
+OPS2 = {"atan2": 0, "pow": 1, "atan2d": 2, "logspace_add": 3, "logspace_sub": 4}
 OPS = {"exp": 0, "log": 1, "sqrt": 2, "erfc": 3, "normcdf": 4,
        "fastnormcdf": 5, "cos": 6, "sin": 7, "abs": 8, "exp2": 9, "exp10": 10, "log2": 11,
        "log10": 12, "floor": 13, "ceil": 14, "trunc": 15, "round": 16, "round2even": 17, "sign": 18,
@@ -18,7 +18,8 @@ OPS = {"exp": 0, "log": 1, "sqrt": 2, "erfc": 3, "normcdf": 4,
        "cosd": 41, "tand": 42, "cotd": 43, "coth": 44, "sech": 45, "csch": 46, "acoth": 47, "asech": 48, "acsch": 49,
        "acot": 50, "asec": 51, "acsc": 52, "acotd": 53, "asecd": 54, "acscd": 55, "acosd": 56, "acospi": 57,
        "asind": 58, "asinpi": 59, "atand": 60, "atanpi": 61, "indeg": 62, "inrad": 63, "secd": 64,
-       "cscd": 65, "secpi": 66, "cscpi": 67, "sinc": 68, "sincpi": 69}
+       "cscd": 65, "secpi": 66, "cscpi": 67, "sinc": 68, "sincpi": 69,
+       "sqrt1pm1": 70, "exprecneg": 71, "exprecnegc": 72, "reallog": 73, "realsqrt": 74, "expx2": 75}
 H2D, D2H, D2D = 1, 2, 3
 
 _lib = None

@@ -1782,6 +1782,12 @@ inline float cosh(float x) { return cosh_f(x); }
 inline double cosh(double x) { return cosh_d(x); }
 inline float exp(float x) { return exp_f(x); }
 inline double exp(double x) { return exp_d(x); }
+inline float expm1(float x) { return expm1_f(x); }
+inline double expm1(double x) { return expm1_d(x); }
+inline float log(float x) { return log_f(x); }
+inline double log(double x) { return log_d(x); }
+inline float log1p(float x) { return log1p_f(x); }
+inline double log1p(double x) { return log1p_d(x); }
 inline float atan2(float a, float b, bool s) { return atan2_f(a, b, s); }
 inline double atan2(double a, double b, bool s) { return atan2_d(a, b, s); }
 template <class T> T cst(uint32_t f, uint64_t d) {
@@ -1898,6 +1904,28 @@ template <class T> T compose_o(int op, T x, bool sc) {
       return is_flint_v((T(2) * x - T(1)) * T(0.5)) ? T(NAN) : r;
     }
     case 67: return cscpi_o(x);
+    // the exponential family: modules/core/exponential/include/nt2/exponential/
+    // functions/{generic/sqrt1pm1.hpp:23-36,generic/exprecneg.hpp:18-28,
+    // generic/exprecnegc.hpp:18-28,generic/reallog.hpp:20-32,
+    // generic/realsqrt.hpp:20-32,simd/common/expx2.hpp:37-61,scalar/expx2.hpp:32-54}
+    case 70: {
+      const T t = std::sqrt(T(1) + x);
+      return std::fabs(x) < T(0.5) ? x / (T(1) + t) : t - T(1);
+    }
+    case 71: return ov::exp(-(T(1) / x));
+    case 72: return -ov::expm1(-(T(1) / x));
+    case 73: return ov::log(x);
+    case 74: return std::sqrt(x);
+    case 75: {  // fma(c2, x, 1/2) and fma(2m, f, f^2) with two roundings: the scalar form's bits
+      const T c1 = ov::cst<T>(0x42000000u, 0x4060000000000000ull), c2 = ov::cst<T>(0x3d000000u, 0x3f80000000000000ull);
+      const T maxlog = ov::cst<T>(0x42b0c0a5u, 0x40862e42fefa39efull);
+      const T a = std::fabs(x);
+      const T m = c1 * std::floor(c2 * a + T(0.5));
+      const T f = a - m;
+      const T u = m * m, u1 = (m + m) * f + f * f;
+      const T r = (u + u1 > maxlog) ? T(INFINITY) : ov::exp(u) * ov::exp(u1);
+      return std::isinf(a) ? a : r;
+    }
     case 69: {
       const T sp = trig_unit<T, true>(1, x);
       if (std::isinf(x)) return T(0);
@@ -1906,6 +1934,18 @@ template <class T> T compose_o(int op, T x, bool sc) {
     }
   }
   return T(NAN);
+}
+// logspace_add: exponential/functions/scalar/logspace_add.hpp:25-42;
+// logspace_sub: scalar/logspace_sub.hpp:20-32 (Log_2: constants/log_2.hpp:29-30)
+template <class T> T logspace_add_o(T a0, T a1) {
+  const T tmp = -std::fabs(a0 - a1);
+  const T r = (a0 > a1 ? a0 : a1) + ov::log1p(ov::exp(tmp));
+  return std::isnan(tmp) ? a0 + a1 : r;
+}
+template <class T> T logspace_sub_o(T a0, T a1) {
+  const T x = a1 - a0;
+  const T tmp = x > -ov::cst<T>(0x3f317218u, 0x3fe62e42fefa39efull) ? ov::log(-ov::expm1(x)) : ov::log1p(-ov::exp(x));
+  return a0 + tmp;
 }
 // sinc over a pack of G lanes (sin with the pack's reduction)
 template <class T, int G> void sinc_o(const T* x, T* y) {
@@ -2097,8 +2137,8 @@ enum {
   OR_SINPI = 36, OR_COSPI = 37, OR_TANPI = 38, OR_COTPI = 39, OR_SIND = 40, OR_COSD = 41, OR_TAND = 42,
   OR_COTD = 43,
   // 44..69: the compositions (compose_o; 68 = sinc by pack)
-  OR_SINC = 68, OR_SINCPI = 69,
-  OR_LAST = OR_SINCPI
+  OR_SINC = 68, OR_SINCPI = 69, OR_EXPX2 = 75,
+  OR_LAST = OR_EXPX2
 };
 
 int oracle_abi_version(void) { return 1; }
@@ -2243,24 +2283,28 @@ int oracle_map_f64(int op, const double* x, double* y, size_t n) {
 
 // out = atan2(x, y) over tables of n elements, NT2 transform semantics
 int oracle_map2_f32(int op, const float* x, const float* y, float* out, size_t n) {
-  if (op < 0 || op > 2) return -1;
+  if (op < 0 || op > 4) return -1;
   nt2_transform(n, 8, sizeof(float), [&](size_t i, bool pack) {
     const size_t w = pack ? 8 : 1;
     for (size_t l = 0; l < w; ++l)
       out[i + l] = op == 0   ? atan2_f(x[i + l], y[i + l], !pack)
                    : op == 1 ? pow_t<float>(x[i + l], y[i + l], !pack)
-                             : indeg_o(atan2_f(x[i + l], y[i + l], !pack));
+                   : op == 2 ? indeg_o(atan2_f(x[i + l], y[i + l], !pack))
+                   : op == 3 ? logspace_add_o(x[i + l], y[i + l])
+                             : logspace_sub_o(x[i + l], y[i + l]);
   });
   return 0;
 }
 int oracle_map2_f64(int op, const double* x, const double* y, double* out, size_t n) {
-  if (op < 0 || op > 2) return -1;
+  if (op < 0 || op > 4) return -1;
   nt2_transform(n, 4, sizeof(double), [&](size_t i, bool pack) {
     const size_t w = pack ? 4 : 1;
     for (size_t l = 0; l < w; ++l)
       out[i + l] = op == 0   ? atan2_d(x[i + l], y[i + l], !pack)
                    : op == 1 ? pow_t<double>(x[i + l], y[i + l], !pack)
-                             : indeg_o(atan2_d(x[i + l], y[i + l], !pack));
+                   : op == 2 ? indeg_o(atan2_d(x[i + l], y[i + l], !pack))
+                   : op == 3 ? logspace_add_o(x[i + l], y[i + l])
+                             : logspace_sub_o(x[i + l], y[i + l]);
   });
   return 0;
 }

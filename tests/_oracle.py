@@ -18,7 +18,8 @@ OPS = {"exp": 0, "log": 1, "sqrt": 2, "erfc": 3, "normcdf": 4,
        "cosd": 41, "tand": 42, "cotd": 43, "coth": 44, "sech": 45, "csch": 46, "acoth": 47, "asech": 48, "acsch": 49,
        "acot": 50, "asec": 51, "acsc": 52, "acotd": 53, "asecd": 54, "acscd": 55, "acosd": 56, "acospi": 57,
        "asind": 58, "asinpi": 59, "atand": 60, "atanpi": 61, "indeg": 62, "inrad": 63, "secd": 64,
-       "cscd": 65, "secpi": 66, "cscpi": 67, "sinc": 68, "sincpi": 69}
+       "cscd": 65, "secpi": 66, "cscpi": 67, "sinc": 68, "sincpi": 69,
+       "sqrt1pm1": 70, "exprecneg": 71, "exprecnegc": 72, "reallog": 73, "realsqrt": 74, "expx2": 75}
 
 _lib = None
 
@@ -342,4 +343,14 @@ def atan2d(y, x):
     out = np.empty_like(y)
     f = lib().oracle_map2_f32 if y.dtype == np.float32 else lib().oracle_map2_f64
     assert f(2, _p(y), _p(x), _p(out), y.size) == 0
+    return out
+
+
+def logspace(op, a, b):
+    """NT2's logspace_add / logspace_sub(a, b) over flat tables."""
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b, dtype=a.dtype)
+    out = np.empty_like(a)
+    f = lib().oracle_map2_f32 if a.dtype == np.float32 else lib().oracle_map2_f64
+    assert f({"logspace_add": 3, "logspace_sub": 4}[op], _p(a), _p(b), _p(out), a.size) == 0
     return out

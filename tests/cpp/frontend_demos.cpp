@@ -267,6 +267,11 @@ int main(int argc, char** argv) {
   dump("secd_f32", at2);
   at2 = nt2::atan2d(at, bt);
   dump("atan2d_f32", at2);
+  at2 = nt2::sqrt1pm1(at) + nt2::exprecneg(bt) - nt2::exprecnegc(at * 2.0f) + nt2::reallog(nt2::abs(bt)) *
+        nt2::realsqrt(nt2::abs(at)) + nt2::expx2(bt / 2.0f);
+  dump("sqrt1pm1_f32", at2);
+  at2 = nt2::logspace_add(at, bt) - nt2::logspace_sub(at * 2.0f, at * 2.0f - nt2::abs(bt) - 0.5f);
+  dump("logspace_f32", at2);
   at2 = if_else((at > 0.0f) && (bt < 1.0f), at, bt) + if_else(logical_or(at < -1.0f, bt > 2.0f), 1.0f, 0.0f);
   dump("logical_f32", at2);
   table<double> dm(of_size(mh * mw)), dr;

@@ -126,6 +126,12 @@ FUNCS = {
     "cscpi": lambda x: _ratio(mpmath.mpf(1), mpmath.sinpi(x), x),
     "sinc": lambda x: mpmath.sin(x) / x if x != 0 else mpmath.mpf(1),
     "sincpi": lambda x: mpmath.sinpi(x) / (mpmath.pi * x) if x != 0 else mpmath.mpf(1),
+    "sqrt1pm1": lambda x: mpmath.sqrt(1 + x) - 1 if x >= -1 else mpmath.nan,
+    "exprecneg": lambda x: mpmath.exp(-1 / x),
+    "exprecnegc": lambda x: -mpmath.expm1(-1 / x),
+    "reallog": lambda x: mpmath.log(x),
+    "realsqrt": lambda x: mpmath.sqrt(x),
+    "expx2": lambda x: mpmath.exp(x * x),
 }
 
 
@@ -282,6 +288,12 @@ COMPOSED_DOMAINS = {
     "cscpi": (-1e-6, 4.0, [0.25, 0.5, 1.0, -0.25, 1.0 / 6.0]),
     "sinc": (-1e-20, 60.0, [np.pi / 2, np.pi / 4, 1.0, 1e-4, 3.0]),
     "sincpi": (-1e-20, 20.0, [0.5, 0.25, 1.0, 1e-4, 1.5]),
+    "sqrt1pm1": (-1e-20, 1.0, [0.5, -0.5, 0.49999, 2.0, 10.0, 1e10, 1e30, -1.0]),
+    "exprecneg": (-0.5, 1e20, [1.0, -1.0, 2.0, 0.5]),
+    "exprecnegc": (-0.5, 1e20, [1.0, -1.0, 2.0, 0.5]),
+    "reallog": (1e-30, 1e30, [1.0, 2.0, 0.5, 10.0]),
+    "realsqrt": (1e-30, 1e30, [1.0, 2.0, 4.0, 0.25]),
+    "expx2": (-1e-6, 9.0, [1.0, 0.5, 3.0, 5.0, 8.9, 0.01]),
 }
 
 
@@ -634,6 +646,40 @@ KATS = [
     ("cotd", "-90", "0", 0.5, "modules/core/trigonometric/unit/simd/cotd.cpp:56"),
     ("cotd", "45", "1", 4.0, "modules/core/trigonometric/unit/simd/cotd.cpp:57"),
     ("cotd", "90", "0", 0.5, "modules/core/trigonometric/unit/simd/cotd.cpp:58"),
+    ("sqrt1pm1", "inf", "inf", 0, "modules/core/exponential/unit/scalar/sqrt1pm1.cpp:41"),
+    ("sqrt1pm1", "-inf", "nan", 0, "modules/core/exponential/unit/scalar/sqrt1pm1.cpp:42"),
+    ("sqrt1pm1", "nan", "nan", 0, "modules/core/exponential/unit/scalar/sqrt1pm1.cpp:43"),
+    ("sqrt1pm1", "-1", "-1", 0, "modules/core/exponential/unit/scalar/sqrt1pm1.cpp:45"),
+    ("sqrt1pm1", "1", "0.41421356237309503", 2, "modules/core/exponential/unit/scalar/sqrt1pm1.cpp:46"),
+    ("sqrt1pm1", "0", "0", 0, "modules/core/exponential/unit/scalar/sqrt1pm1.cpp:47"),
+    ("exprecneg", "inf", "1", 0.75, "modules/core/exponential/unit/scalar/exprecneg.cpp:41"),
+    ("exprecneg", "-inf", "1", 0.75, "modules/core/exponential/unit/scalar/exprecneg.cpp:42"),
+    ("exprecneg", "nan", "nan", 0.75, "modules/core/exponential/unit/scalar/exprecneg.cpp:43"),
+    ("exprecneg", "-1", "e", 0.75, "modules/core/exponential/unit/scalar/exprecneg.cpp:45"),
+    ("exprecneg", "1", "1/e", 0.75, "modules/core/exponential/unit/scalar/exprecneg.cpp:46"),
+    ("exprecneg", "0", "0", 0.75, "modules/core/exponential/unit/scalar/exprecneg.cpp:47"),
+    ("exprecneg", "-0", "inf", 0.75, "modules/core/exponential/unit/scalar/exprecneg.cpp:48"),
+    ("exprecnegc", "inf", "0", 0.75, "modules/core/exponential/unit/scalar/exprecnegc.cpp:42"),
+    ("exprecnegc", "-inf", "0", 0.75, "modules/core/exponential/unit/scalar/exprecnegc.cpp:43"),
+    ("exprecnegc", "nan", "nan", 0.75, "modules/core/exponential/unit/scalar/exprecnegc.cpp:44"),
+    ("exprecnegc", "-1", "-1.718281828459045", 0.75, "modules/core/exponential/unit/scalar/exprecnegc.cpp:46"),
+    ("exprecnegc", "1", "0.6321205588285577", 0.75, "modules/core/exponential/unit/scalar/exprecnegc.cpp:48"),
+    ("exprecnegc", "0", "1", 0.75, "modules/core/exponential/unit/scalar/exprecnegc.cpp:50"),
+    ("exprecnegc", "-0", "-inf", 0.75, "modules/core/exponential/unit/scalar/exprecnegc.cpp:51"),
+    ("reallog", "inf", "inf", 0, "modules/core/exponential/unit/scalar/reallog.cpp:40"),
+    ("reallog", "nan", "nan", 0, "modules/core/exponential/unit/scalar/reallog.cpp:41"),
+    ("reallog", "0", "-inf", 0, "modules/core/exponential/unit/scalar/reallog.cpp:42"),
+    ("reallog", "1", "0", 0, "modules/core/exponential/unit/scalar/reallog.cpp:44"),
+    ("realsqrt", "inf", "inf", 0, "modules/core/exponential/unit/scalar/realsqrt.cpp:39"),
+    ("realsqrt", "nan", "nan", 0, "modules/core/exponential/unit/scalar/realsqrt.cpp:40"),
+    ("realsqrt", "1", "1", 0, "modules/core/exponential/unit/scalar/realsqrt.cpp:42"),
+    ("realsqrt", "0", "0", 0, "modules/core/exponential/unit/scalar/realsqrt.cpp:43"),
+    ("expx2", "inf", "inf", 1.0, "modules/core/exponential/unit/scalar/expx2.cpp:40"),
+    ("expx2", "-inf", "inf", 1.0, "modules/core/exponential/unit/scalar/expx2.cpp:41"),
+    ("expx2", "nan", "nan", 1.0, "modules/core/exponential/unit/scalar/expx2.cpp:42"),
+    ("expx2", "-1", "e", 1.0, "modules/core/exponential/unit/scalar/expx2.cpp:44"),
+    ("expx2", "1", "e", 1.0, "modules/core/exponential/unit/scalar/expx2.cpp:45"),
+    ("expx2", "0", "1", 1.0, "modules/core/exponential/unit/scalar/expx2.cpp:46"),
     # compositions: the scalar unit tests of NT2 (evaluated in packs and as tails)
     ("coth", "inf", "1", 0.5, "modules/core/hyperbolic/unit/scalar/coth.cpp:39"),
     ("coth", "-inf", "-1", 0.5, "modules/core/hyperbolic/unit/scalar/coth.cpp:40"),

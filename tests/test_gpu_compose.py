@@ -27,7 +27,8 @@ from nt2_amd import EXACT, FAST, ops  # noqa: E402
 G = os.path.join(os.path.dirname(__file__), "golden")
 COMPOSED = ["coth", "sech", "csch", "acoth", "asech", "acsch", "acot", "asec", "acsc", "acotd", "asecd",
             "acscd", "acosd", "acospi", "asind", "asinpi", "atand", "atanpi", "indeg", "inrad", "secd",
-            "cscd", "secpi", "cscpi", "sinc", "sincpi"]
+            "cscd", "secpi", "cscpi", "sinc", "sincpi", "sqrt1pm1", "exprecneg", "exprecnegc", "reallog",
+            "realsqrt", "expx2"]
 # the larger term of a cancelling difference in NT2's definition
 CANCEL_SCALE = {"asec": np.pi / 2, "asecd": 90.0, "acotd": 90.0}
 
@@ -112,3 +113,20 @@ def test_atan2d_exact_bitwise(dt):
     fast = ops.map2("atan2d", yd, xd, FAST).cpu().numpy()
     err = classic_ulp(fast, want.astype(np.float64), dt)
     assert ((err <= 4) | (np.isnan(fast) & np.isnan(want))).all()
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_logspace_exact_bitwise(dt):
+    rng = np.random.default_rng(37)
+    n = 20003
+    a = np.concatenate([rng.uniform(-50, 50, n), [np.inf, np.inf, -np.inf, np.nan, 0.0, 1.0, -1.0]]).astype(dt)
+    b = np.concatenate([a[:n] - np.exp2(rng.uniform(-30, 8, n)), [np.inf, 0.0, 0.0, 1.0, 0.0, 1.0, -1.0]]).astype(dt)
+    ad, bd = torch.from_numpy(a).cuda(), torch.from_numpy(b).cuda()
+    for op in ("logspace_add", "logspace_sub"):
+        say(op, dt.__name__, "n=%d" % a.size)
+        got = ops.map2(op, ad, bd, EXACT).cpu().numpy()
+        want = O.logspace(op, a, b)
+        assert same_bits(got, want), first_diff(got, want, a)
+        fast = ops.map2(op, ad, bd, FAST).cpu().numpy()
+        close = np.abs(fast.astype(np.float64) - want) <= 4 * np.finfo(dt).eps * np.maximum(np.abs(a), np.abs(want))
+        assert (close | (np.isnan(fast) & np.isnan(want)) | (fast == want)).all()

@@ -198,6 +198,13 @@ def test_exponential_rounding_statements(outputs):
     assert same_bits(got, want), first_diff(got, want)
     got = outputs("atan2d_f32", f32)
     assert same_bits(got, O.atan2d(a, b)), first_diff(got, O.atan2d(a, b))
+    want = (((m("sqrt1pm1", a) + m("exprecneg", b)) - m("exprecnegc", a * f32(2))) +
+            m("reallog", np.abs(b)) * m("realsqrt", np.abs(a))) + m("expx2", b / f32(2))
+    got = outputs("sqrt1pm1_f32", f32)
+    assert same_bits(got, want), first_diff(got, want)
+    want = O.logspace("logspace_add", a, b) - O.logspace("logspace_sub", a * f32(2), (a * f32(2) - np.abs(b)) - f32(0.5))
+    got = outputs("logspace_f32", f32)
+    assert same_bits(got, want), first_diff(got, want)
     want = np.where((a > 0) & (b < 1), a, b) + np.where((a < -1) | (b > 2), f32(1), f32(0))
     got = outputs("logical_f32", f32)
     assert same_bits(got, want.astype(f32)), first_diff(got, want)

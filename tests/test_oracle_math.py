@@ -70,6 +70,11 @@ def _bound(name, x, y, dt):
         return np.full(x.shape, 1.0)
     if name in ("expm1", "log1p", "tanh", "sinh", "cosh", "asin", "acos", "cbrt", "asinh", "acosh", "atanh"):
         return np.full(x.shape, BOUND3[name][dt])
+    if name == "expx2":
+        # NT2's split constants (expx2c1.hpp:29-30, expx2c2.hpp:29-30) make m a
+        # multiple of 32 (float) / 128 (double), so m = 0 wherever the result is
+        # finite and the value is exp of the rounded x*x: x^2 / 2 ulp from it
+        return 1.5 + x.astype(np.float64) ** 2
     if name in COMPOSED_BOUND:
         return np.full(x.shape, COMPOSED_BOUND[name][0 if dt is np.float32 else 1])
     if name in TRIG_VARIANTS:
@@ -120,7 +125,8 @@ def _bound(name, x, y, dt):
                                   "cospi", "tanpi", "cotpi", "sind", "cosd", "tand", "cotd", "coth", "sech", "csch",
                                   "acoth", "asech", "acsch", "acot", "asec", "acsc", "acotd", "asecd", "acscd",
                                   "acosd", "acospi", "asind", "asinpi", "atand", "atanpi", "indeg", "inrad", "secd",
-                                  "cscd", "secpi", "cscpi", "sinc", "sincpi"])
+                                  "cscd", "secpi", "cscpi", "sinc", "sincpi", "sqrt1pm1", "exprecneg", "exprecnegc",
+                                  "reallog", "realsqrt", "expx2"])
 def test_golden_mpmath(dt, tag, name):
     g = np.load(os.path.join(G, "math_%s.npz" % tag))
     x, y = g[name + "_x"], g[name + "_y"]
@@ -183,13 +189,15 @@ BOUND3 = {"expm1": {np.float32: 2.0, np.float64: 1.0}, "log1p": {np.float32: 1.0
 # inner argument's rounding is well conditioned: the inner kernel's bound,
 # plus the reciprocal's or the outer product's rounding (measured maxima
 # 0.7-5 ulp); acotd subtracts atand(|x|) from 90, asind/acscd scale
-# asin's 2-ulp f32 bound by indeg, cscd carries the rounding of x/180
+# asin's 2-ulp f32 bound by indeg, cscd carries the rounding of x/180,
+# sqrt1pm1's t - 1 above |x| = 1/2 multiplies the rounding of 1 + x by t/(t - 1) <= 5.5
 COMPOSED_BOUND = {"coth": (2, 2), "sech": (3, 2), "csch": (3, 2), "acoth": (3, 2), "asech": (3, 2),
                   "acsch": (3, 2), "acot": (2, 2), "asec": (2, 2), "acsc": (3, 2), "acotd": (4, 6),
                   "asecd": (3, 2), "acscd": (4, 3), "acosd": (3, 2), "acospi": (3, 2), "asind": (5, 2),
                   "asinpi": (3, 2), "atand": (3, 2), "atanpi": (3, 2), "indeg": (1, 1), "inrad": (1, 1),
                   "secd": (3, 2), "cscd": (4, 3), "secpi": (3, 2), "cscpi": (3, 2), "sinc": (2, 2),
-                  "sincpi": (3, 3)}
+                  "sincpi": (3, 3), "sqrt1pm1": (6, 6), "exprecneg": (3, 3), "exprecnegc": (3, 3),
+                  "reallog": (1, 1), "realsqrt": (0.5, 0.5)}
 
 TRIG_VARIANTS = ("cot", "sec", "csc", "sinpi", "cospi", "tanpi", "cotpi", "sind", "cosd", "tand", "cotd")
 UNIT_DOMAIN32 = {"tanpi": 2.0 ** 22, "sind": 2.0 ** 24, "cosd": 2.0 ** 24, "tand": 2.0 ** 24, "cotd": 2.0 ** 24}
@@ -839,3 +847,60 @@ def test_composed_structure(dt):
     t = np.array([eps * 0.75, eps, eps * 1.5, -eps * 0.75], dtype=dt)
     assert list(_iso(mapf, "sinc", t, W)[[0, 3]]) == [1.0, 1.0]
     assert list(_iso(mapf, "sincpi", t, W)[[0, 3]]) == [1.0, 1.0]
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.float64])
+def test_exponential_compositions(dt):
+    """exprecneg / exprecnegc at small |x| follow exp(-fl(1/x)) (the rounded
+    reciprocal, amplified by 1/|x|); expx2 is exp of the rounded square wherever
+    it is finite; logspace_add / logspace_sub against mpmath and NT2's unit
+    tests (modules/core/exponential/unit/scalar/logspace_add.cpp:41-51,
+    logspace_sub.cpp:39-45)."""
+    mpmath.mp.dps = 40
+    rng = np.random.default_rng(79)
+    mapf = O.map_f32 if dt is np.float32 else O.map_f64
+    W = 8 if dt is np.float32 else 4
+    x = (np.exp2(rng.uniform(-4, -1, 300)) * rng.choice([-1.0, 1.0], 300)).astype(dt)
+    x = x[np.abs(x) > (0.02 if dt is np.float32 else 0.0016)]  # finite results
+    r = (dt(1) / x).astype(dt)
+    ref = np.array([float(mpmath.exp(-mpmath.mpf(float(v)))) for v in r])
+    assert (classic_ulp(_iso(mapf, "exprecneg", x, W), ref, dt) <= 1.5).all()
+    ref = np.array([float(-mpmath.expm1(-mpmath.mpf(float(v)))) for v in r])
+    assert (classic_ulp(_iso(mapf, "exprecnegc", x, W), ref, dt) <= 2).all()
+    lim = 9.3 if dt is np.float32 else 26.6
+    x = rng.uniform(-lim, lim, 500).astype(dt)
+    sq = (x * x).astype(dt)
+    ref = np.array([float(mpmath.exp(mpmath.mpf(float(v)))) for v in sq])
+    for tail in (False, True):
+        assert (classic_ulp(_one_lane(mapf, "expx2", x, W, tail), ref, dt) <= 1.5).all()
+    big = np.array([lim + 0.2, 100.0, -50.0, 1e30], dtype=dt)
+    assert np.all(_iso(mapf, "expx2", big, W) == np.inf)
+    # logspace
+    a = rng.uniform(-30, 30, 2000).astype(dt)
+    b = (a - np.exp2(rng.uniform(-20, 6, 2000))).astype(dt)
+    b = np.where(b < a, b, np.nextafter(a, dt(-np.inf))).astype(dt)
+    add = O.logspace("logspace_add", a, b)
+    sub = O.logspace("logspace_sub", a, b)
+    ra = np.array([float(mpmath.log(mpmath.exp(mpmath.mpf(float(u))) + mpmath.exp(mpmath.mpf(float(v)))))
+                   for u, v in zip(a, b)])
+    rs = np.array([float(mpmath.log(mpmath.exp(mpmath.mpf(float(u))) - mpmath.exp(mpmath.mpf(float(v)))))
+                   for u, v in zip(a, b)])
+    eps = float(np.finfo(dt).eps)
+    # a0 + log1p(...) rounds twice: 2 ulp of the larger of |a0| and the log term
+    assert (np.abs(add - ra) <= 2 * eps * np.maximum(np.abs(a), 1.0) + 2 * eps * np.abs(ra)).all()
+    # the log term carries the rounding of a1 - a0 (relative eps / |1 - e^x|)
+    d = np.abs((b.astype(np.float64) - a) / np.expm1(b.astype(np.float64) - a))
+    tol = 2 * eps * np.maximum(np.abs(a), 1.0) + 3 * eps * (1 + d) * np.maximum(np.abs(rs - a), 1.0)
+    assert (np.abs(sub - rs) <= tol).all()
+    inf, nan = np.inf, np.nan
+    l2 = dt(np.log(2.0))
+    kat_add = [(inf, inf, inf), (inf, 0, inf), (-inf, 0, 0), (inf, nan, nan), (-inf, -inf, -inf), (nan, nan, nan),
+               (-1, -1, dt(-1) + l2), (1, 1, dt(1) + l2), (2, 2, dt(2) + l2), (0, 0, l2)]
+    kat_sub = [(inf, inf, nan), (-inf, -inf, nan), (nan, nan, nan), (-1, -1, -inf), (1, 1, -inf), (2, 2, -inf),
+               (0, 0, -inf)]
+    for op, kats in (("logspace_add", kat_add), ("logspace_sub", kat_sub)):
+        u = np.array([k[0] for k in kats], dt)
+        v = np.array([k[1] for k in kats], dt)
+        e = np.array([k[2] for k in kats], dt)
+        got = O.logspace(op, u, v)
+        assert all(nt2_ulpdist(g, w) == 0 for g, w in zip(got, e)), (op, got, e)
