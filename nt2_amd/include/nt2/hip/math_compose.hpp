@@ -288,4 +288,97 @@ template <class P, class T> NT2H_FN T logspace_sub_t(T a0, T a1) {
   return a0 + tmp;
 }
 
+// ---- inverse error functions -----------------------------------------------------------
+// (modules/core/euler/include/nt2/euler/functions/): the polynomials of
+// details/erfinv_kernel.hpp (float: erfinv1 :31-41, erfinv2 :44-54, the
+// erfcinv2 rational :57-71, Giles' single-precision approximation; double:
+// erfinv1 :80-104, erfinv2 :108-128, erfinv3 :132-150), coefficients by bit
+// pattern, highest degree first.
+template <class P> NT2H_FN float erfinv1_f(float w) {
+  constexpr float c[9] = {NT2H_F(0x32f16588), NT2H_F(0x34b84b36), NT2H_F(0xb66c7357), NT2H_F(0xb6935ac1),
+      NT2H_F(0x396532db), NT2H_F(0xbaa45408), NT2H_F(0xbb88e4ef), NT2H_F(0x3e7c8f63), NT2H_F(0x3fc02e2f)};
+  return horner<P>(c, w);
+}
+template <class P> NT2H_FN float erfinv2_f(float w) {
+  constexpr float c[9] = {NT2H_F(0xb951f09b), NT2H_F(0x38d3b56b), NT2H_F(0x3ab0dc72), NT2H_F(0xbb70bde7),
+      NT2H_F(0x3bbc127b), NT2H_F(0xbbf9c5d7), NT2H_F(0x3c1aa57e), NT2H_F(0x3f8036db), NT2H_F(0x40354f7e)};
+  return horner<P>(c, w);
+}
+template <class P> NT2H_FN float erfcinv2_f(float w) {
+  NT2H_STRICT
+  constexpr float p[5] = {NT2H_F(0x3b3a8aee), NT2H_F(0x3ba62c57), NT2H_F(0xbc8bcf63), NT2H_F(0x3c98e17c),
+      NT2H_F(0xbbaf8bfe)};
+  constexpr float q[5] = {NT2H_F(0x321631b2), NT2H_F(0x3b3a73c1), NT2H_F(0x3ba9025f), NT2H_F(0xbc5d80aa),
+      NT2H_F(0x3c2f4022)};
+  return horner<P>(p, w) / horner<P>(q, w);
+}
+template <class P> NT2H_FN double erfinv1_d(double w) {
+  constexpr double c[23] = {NT2H_D(0xbbb135d2e746e627), NT2H_D(0xbc08ddf93324d327),
+      NT2H_D(0x3c37b83eef0b7c9f), NT2H_D(0x3c69ba72cd589b91), NT2H_D(0xbca33689090a6b96),
+      NT2H_D(0x3c782e11898132e0), NT2H_D(0x3cfde4acfd9e26ba), NT2H_D(0xbd26d33eed66c487),
+      NT2H_D(0xbd36f2167040d8e2), NT2H_D(0x3d872a22c2d77e20), NT2H_D(0xbdac8859c4e5c0af),
+      NT2H_D(0xbdcdc583d118a561), NT2H_D(0x3e120f47ccf46b3c), NT2H_D(0xbe31a9e38dc84d60),
+      NT2H_D(0xbe5f36cd6d3d46a9), NT2H_D(0x3e9c6b4f5d03b787), NT2H_D(0xbeb6e8a5434ae8a2),
+      NT2H_D(0xbeed1d1f7b8736f6), NT2H_D(0x3f2879c2a212f024), NT2H_D(0xbf4845769484fca8),
+      NT2H_D(0xbf78b6c33114f909), NT2H_D(0x3fcebd80d9b13e28), NT2H_D(0x3ffa755e7c99ae86)};
+  return horner<P>(c, w);
+}
+template <class P> NT2H_FN double erfinv2_d(double w) {
+  constexpr double c[19] = {NT2H_D(0x3e23040f87dbd932), NT2H_D(0x3e785cbe52878635),
+      NT2H_D(0xbe92777453dd3955), NT2H_D(0x3e5395abcd554c6c), NT2H_D(0x3eb936388a3790ad),
+      NT2H_D(0xbed0d5db812b5083), NT2H_D(0x3ec8860cd5d652f6), NT2H_D(0x3eea29a0cacdfb23),
+      NT2H_D(0xbf08cef1f80281f2), NT2H_D(0x3f11e684d0b9188a), NT2H_D(0x3ef932cd54c8a222),
+      NT2H_D(0xbf37448a89ef8aa3), NT2H_D(0x3f4f3cc55ad40c25), NT2H_D(0xbf5ba924132f38b1),
+      NT2H_D(0x3f6468eeca533cf8), NT2H_D(0xbf6ebadabb891bbd), NT2H_D(0x3f75ffcfe5b76afc),
+      NT2H_D(0x3ff0158a6d641d39), NT2H_D(0x4008abcc380d5a48)};
+  return horner<P>(c, w);
+}
+template <class P> NT2H_FN double erfinv3_d(double w) {
+  constexpr double c[17] = {NT2H_D(0xbdbdcec3a7785389), NT2H_D(0xbdf18feec0e38727),
+      NT2H_D(0x3e19e6bf2dda45e3), NT2H_D(0xbe30468fb24e2f5f), NT2H_D(0x3e405ac6a8fba182),
+      NT2H_D(0xbe50102e495fb9c0), NT2H_D(0x3e5f4c20e1334af8), NT2H_D(0xbe722d220fdf9c3e),
+      NT2H_D(0x3e8ebc8bb824cb54), NT2H_D(0xbeb0a8d40ea372cc), NT2H_D(0x3ed2fbd29d093d2b),
+      NT2H_D(0xbef4a3497e1e0fac), NT2H_D(0x3f13ebf4eb00938f), NT2H_D(0xbf2c2f36a8fc5d53),
+      NT2H_D(0xbf222ea5df04047c), NT2H_D(0x3ff02a30d1fba0dc), NT2H_D(0x4013664ddd1ad7fb)};
+  return horner<P>(c, w);
+}
+template <class T> NT2H_FN T signnz(T a) { return __builtin_signbit(a) ? T(-1) : T(1); }
+// erfinv: simd/common/erfinv.hpp:32-57 (float), :64-108 (double); the scalar
+// forms (scalar/erfinv.hpp:22-52, :59-80) give the same value per lane.
+// w = -log((1 - a0)(1 + a0)); float: a0 erfinv1(w - 2.5) below 5, else
+// a0 erfinv2(sqrt(w) - 3); double: a0 erfinv1(w - 3.125) below 6.25,
+// a0 erfinv2(sqrt(w) - 3.25) below 16, else a0 erfinv3(sqrt(w) - 5);
+// signnz(a0) w where w is infinite (a0 = +-1)
+template <class P, class T> NT2H_FN T erfinv_t(T a0) {
+  NT2H_STRICT
+  const T w = -log_t<P>((T(1) - a0) * (T(1) + a0));
+  if (__builtin_isinf(w) && w > T(0)) return signnz(a0) * w;
+  if constexpr (sizeof(T) == 4) {
+    if (w < 5.0f) return a0 * erfinv1_f<P>(w - 2.5f);
+    return a0 * erfinv2_f<P>(__builtin_sqrtf(w) - 3.0f);
+  } else {
+    if (w < 6.25) return a0 * erfinv1_d<P>(w - 3.125);
+    if (w < 16.0) return a0 * erfinv2_d<P>(__builtin_sqrt(w) - 3.25);
+    return a0 * erfinv3_d<P>(__builtin_sqrt(w) - 5.0);
+  }
+}
+// erfcinv: simd/common/erfcinv.hpp:32-57 (float), :64-107 (double); the scalar
+// forms (scalar/erfcinv.hpp:22-51, :58-80) give the same value per lane.
+// a00 = 1 - a0, w = -log(a0 (2 - a0)); as erfinv with a00 for a0, except the
+// float tail copysign(erfcinv2(sqrt(w)), a00)
+template <class P, class T> NT2H_FN T erfcinv_t(T a0) {
+  NT2H_STRICT
+  const T a00 = T(1) - a0;
+  const T w = -log_t<P>(a0 * (T(2) - a0));
+  if (__builtin_isinf(w) && w > T(0)) return signnz(a00) * w;
+  if constexpr (sizeof(T) == 4) {
+    if (w < 5.0f) return a00 * erfinv1_f<P>(w - 2.5f);
+    return __builtin_copysignf(erfcinv2_f<P>(__builtin_sqrtf(w)), a00);
+  } else {
+    if (w < 6.25) return a00 * erfinv1_d<P>(w - 3.125);
+    if (w < 16.0) return a00 * erfinv2_d<P>(__builtin_sqrt(w) - 3.25);
+    return a00 * erfinv3_d<P>(__builtin_sqrt(w) - 5.0);
+  }
+}
+
 }}}  // namespace nt2::hip::dm

@@ -1866,6 +1866,8 @@ template <class T> T cscpi_o(T a0) {
   const T r = T(1) / trig_unit<T, true>(1, a0);
   return (is_flint_v(a0) && a0 != T(0)) ? T(NAN) : r;
 }
+template <class T> T erfinv_o(T a0);
+template <class T> T erfcinv_o(T a0);
 // one lane of a composed op; kinds as in the C ABI (44..69, 68 = sinc is by pack)
 template <class T> T compose_o(int op, T x, bool sc) {
   switch (op) {
@@ -1926,6 +1928,8 @@ template <class T> T compose_o(int op, T x, bool sc) {
       const T r = (u + u1 > maxlog) ? T(INFINITY) : ov::exp(u) * ov::exp(u1);
       return std::isinf(a) ? a : r;
     }
+    case 76: return erfinv_o(x);
+    case 77: return erfcinv_o(x);
     case 69: {
       const T sp = trig_unit<T, true>(1, x);
       if (std::isinf(x)) return T(0);
@@ -1935,6 +1939,96 @@ template <class T> T compose_o(int op, T x, bool sc) {
   }
   return T(NAN);
 }
+// ---- inverse error functions -----------------------------------------------------------
+// (modules/core/euler/include/nt2/euler/functions/): the polynomials of
+// details/erfinv_kernel.hpp (float: erfinv1 :31-41, erfinv2 :44-54, the
+// erfcinv2 rational :57-71, Giles' single-precision approximation; double:
+// erfinv1 :80-104, erfinv2 :108-128, erfinv3 :132-150), coefficients by bit
+// pattern, highest degree first.
+inline float erfinv1_f(float w) {
+  const float c[9] = {F32(0x32f16588), F32(0x34b84b36), F32(0xb66c7357), F32(0xb6935ac1),
+      F32(0x396532db), F32(0xbaa45408), F32(0xbb88e4ef), F32(0x3e7c8f63), F32(0x3fc02e2f)};
+  return horner(c, w);
+}
+inline float erfinv2_f(float w) {
+  const float c[9] = {F32(0xb951f09b), F32(0x38d3b56b), F32(0x3ab0dc72), F32(0xbb70bde7),
+      F32(0x3bbc127b), F32(0xbbf9c5d7), F32(0x3c1aa57e), F32(0x3f8036db), F32(0x40354f7e)};
+  return horner(c, w);
+}
+inline float erfcinv2_f(float w) {
+  const float p[5] = {F32(0x3b3a8aee), F32(0x3ba62c57), F32(0xbc8bcf63), F32(0x3c98e17c),
+      F32(0xbbaf8bfe)};
+  const float q[5] = {F32(0x321631b2), F32(0x3b3a73c1), F32(0x3ba9025f), F32(0xbc5d80aa),
+      F32(0x3c2f4022)};
+  return horner(p, w) / horner(q, w);
+}
+inline double erfinv1_d(double w) {
+  const double c[23] = {F64(0xbbb135d2e746e627), F64(0xbc08ddf93324d327),
+      F64(0x3c37b83eef0b7c9f), F64(0x3c69ba72cd589b91), F64(0xbca33689090a6b96),
+      F64(0x3c782e11898132e0), F64(0x3cfde4acfd9e26ba), F64(0xbd26d33eed66c487),
+      F64(0xbd36f2167040d8e2), F64(0x3d872a22c2d77e20), F64(0xbdac8859c4e5c0af),
+      F64(0xbdcdc583d118a561), F64(0x3e120f47ccf46b3c), F64(0xbe31a9e38dc84d60),
+      F64(0xbe5f36cd6d3d46a9), F64(0x3e9c6b4f5d03b787), F64(0xbeb6e8a5434ae8a2),
+      F64(0xbeed1d1f7b8736f6), F64(0x3f2879c2a212f024), F64(0xbf4845769484fca8),
+      F64(0xbf78b6c33114f909), F64(0x3fcebd80d9b13e28), F64(0x3ffa755e7c99ae86)};
+  return horner(c, w);
+}
+inline double erfinv2_d(double w) {
+  const double c[19] = {F64(0x3e23040f87dbd932), F64(0x3e785cbe52878635),
+      F64(0xbe92777453dd3955), F64(0x3e5395abcd554c6c), F64(0x3eb936388a3790ad),
+      F64(0xbed0d5db812b5083), F64(0x3ec8860cd5d652f6), F64(0x3eea29a0cacdfb23),
+      F64(0xbf08cef1f80281f2), F64(0x3f11e684d0b9188a), F64(0x3ef932cd54c8a222),
+      F64(0xbf37448a89ef8aa3), F64(0x3f4f3cc55ad40c25), F64(0xbf5ba924132f38b1),
+      F64(0x3f6468eeca533cf8), F64(0xbf6ebadabb891bbd), F64(0x3f75ffcfe5b76afc),
+      F64(0x3ff0158a6d641d39), F64(0x4008abcc380d5a48)};
+  return horner(c, w);
+}
+inline double erfinv3_d(double w) {
+  const double c[17] = {F64(0xbdbdcec3a7785389), F64(0xbdf18feec0e38727),
+      F64(0x3e19e6bf2dda45e3), F64(0xbe30468fb24e2f5f), F64(0x3e405ac6a8fba182),
+      F64(0xbe50102e495fb9c0), F64(0x3e5f4c20e1334af8), F64(0xbe722d220fdf9c3e),
+      F64(0x3e8ebc8bb824cb54), F64(0xbeb0a8d40ea372cc), F64(0x3ed2fbd29d093d2b),
+      F64(0xbef4a3497e1e0fac), F64(0x3f13ebf4eb00938f), F64(0xbf2c2f36a8fc5d53),
+      F64(0xbf222ea5df04047c), F64(0x3ff02a30d1fba0dc), F64(0x4013664ddd1ad7fb)};
+  return horner(c, w);
+}
+template <class T> T signnz_o(T a) { return std::signbit(a) ? T(-1) : T(1); }
+// erfinv: simd/common/erfinv.hpp:32-57 (float), :64-108 (double); the scalar
+// forms (scalar/erfinv.hpp:22-52, :59-80) give the same value per lane.
+// w = -log((1 - a0)(1 + a0)); float: a0 erfinv1(w - 2.5) below 5, else
+// a0 erfinv2(sqrt(w) - 3); double: a0 erfinv1(w - 3.125) below 6.25,
+// a0 erfinv2(sqrt(w) - 3.25) below 16, else a0 erfinv3(sqrt(w) - 5);
+// signnz_o(a0) w where w is infinite (a0 = +-1)
+template <class T> T erfinv_o(T a0) {
+  const T w = -ov::log((T(1) - a0) * (T(1) + a0));
+  if (std::isinf(w) && w > T(0)) return signnz_o(a0) * w;
+  if constexpr (sizeof(T) == 4) {
+    if (w < 5.0f) return a0 * erfinv1_f(w - 2.5f);
+    return a0 * erfinv2_f(std::sqrt(w) - 3.0f);
+  } else {
+    if (w < 6.25) return a0 * erfinv1_d(w - 3.125);
+    if (w < 16.0) return a0 * erfinv2_d(std::sqrt(w) - 3.25);
+    return a0 * erfinv3_d(std::sqrt(w) - 5.0);
+  }
+}
+// erfcinv: simd/common/erfcinv.hpp:32-57 (float), :64-107 (double); the scalar
+// forms (scalar/erfcinv.hpp:22-51, :58-80) give the same value per lane.
+// a00 = 1 - a0, w = -log(a0 (2 - a0)); as erfinv with a00 for a0, except the
+// float tail copysign(erfcinv2(sqrt(w)), a00)
+template <class T> T erfcinv_o(T a0) {
+  const T a00 = T(1) - a0;
+  const T w = -ov::log(a0 * (T(2) - a0));
+  if (std::isinf(w) && w > T(0)) return signnz_o(a00) * w;
+  if constexpr (sizeof(T) == 4) {
+    if (w < 5.0f) return a00 * erfinv1_f(w - 2.5f);
+    return std::copysign(erfcinv2_f(std::sqrt(w)), a00);
+  } else {
+    if (w < 6.25) return a00 * erfinv1_d(w - 3.125);
+    if (w < 16.0) return a00 * erfinv2_d(std::sqrt(w) - 3.25);
+    return a00 * erfinv3_d(std::sqrt(w) - 5.0);
+  }
+}
+
 // logspace_add: exponential/functions/scalar/logspace_add.hpp:25-42;
 // logspace_sub: scalar/logspace_sub.hpp:20-32 (Log_2: constants/log_2.hpp:29-30)
 template <class T> T logspace_add_o(T a0, T a1) {
@@ -2137,8 +2231,8 @@ enum {
   OR_SINPI = 36, OR_COSPI = 37, OR_TANPI = 38, OR_COTPI = 39, OR_SIND = 40, OR_COSD = 41, OR_TAND = 42,
   OR_COTD = 43,
   // 44..69: the compositions (compose_o; 68 = sinc by pack)
-  OR_SINC = 68, OR_SINCPI = 69, OR_EXPX2 = 75,
-  OR_LAST = OR_EXPX2
+  OR_SINC = 68, OR_SINCPI = 69, OR_EXPX2 = 75, OR_ERFCINV = 77,
+  OR_LAST = OR_ERFCINV
 };
 
 int oracle_abi_version(void) { return 1; }

@@ -132,6 +132,8 @@ FUNCS = {
     "reallog": lambda x: mpmath.log(x),
     "realsqrt": lambda x: mpmath.sqrt(x),
     "expx2": lambda x: mpmath.exp(x * x),
+    "erfinv": lambda x: mpmath.erfinv(x) if abs(x) < 1 else (mpmath.inf * mpmath.sign(x) if abs(x) == 1 else mpmath.nan),
+    "erfcinv": lambda x: mpmath.erfinv(1 - x) if 0 < x < 2 else (mpmath.inf * mpmath.sign(1 - x) if x in (0, 2) else mpmath.nan),
 }
 
 
@@ -294,6 +296,8 @@ COMPOSED_DOMAINS = {
     "reallog": (1e-30, 1e30, [1.0, 2.0, 0.5, 10.0]),
     "realsqrt": (1e-30, 1e30, [1.0, 2.0, 4.0, 0.25]),
     "expx2": (-1e-6, 9.0, [1.0, 0.5, 3.0, 5.0, 8.9, 0.01]),
+    "erfinv": (-1e-20, 1.0, [0.5, -0.5, 0.9966, 0.99, 0.999, 0.9999999, -0.99999, 1e-5]),
+    "erfcinv": (1e-30, 2.0, [1.0, 0.5, 1.5, 0.0034, 1.9966, 1e-10, 1e-20, 1.9999999]),
 }
 
 
@@ -680,6 +684,19 @@ KATS = [
     ("expx2", "-1", "e", 1.0, "modules/core/exponential/unit/scalar/expx2.cpp:44"),
     ("expx2", "1", "e", 1.0, "modules/core/exponential/unit/scalar/expx2.cpp:45"),
     ("expx2", "0", "1", 1.0, "modules/core/exponential/unit/scalar/expx2.cpp:46"),
+    ("erfinv", "nan", "nan", 2, "modules/core/euler/unit/scalar/erfinv.cpp:40"),
+    ("erfinv", "0.52049987781304653768274665389196", "0.5", 2, "modules/core/euler/unit/scalar/erfinv.cpp:42"),
+    ("erfinv", "0.84270079294971486934122063508262", "1", 2, "modules/core/euler/unit/scalar/erfinv.cpp:43"),
+    ("erfinv", "0.99532226501895273416206925636725", "2", 2, "modules/core/euler/unit/scalar/erfinv.cpp:44"),
+    ("erfinv", "0", "0", 2, "modules/core/euler/unit/scalar/erfinv.cpp:45"),
+    ("erfinv", "1", "inf", 2, "modules/core/euler/unit/scalar/erfinv.cpp:46"),
+    ("erfinv", "-1", "-inf", 2, "modules/core/euler/unit/scalar/erfinv.cpp:47"),
+    ("erfcinv", "nan", "nan", 10, "modules/core/euler/unit/scalar/erfcinv.cpp:39"),
+    ("erfcinv", "0", "inf", 10, "modules/core/euler/unit/scalar/erfcinv.cpp:40"),
+    ("erfcinv", "1", "-0", 10, "modules/core/euler/unit/scalar/erfcinv.cpp:42"),
+    ("erfcinv", "0.47950012218695346231725334610804", "0.5", 10, "modules/core/euler/unit/scalar/erfcinv.cpp:43"),
+    ("erfcinv", "0.15729920705028513065877936491739", "1", 10, "modules/core/euler/unit/scalar/erfcinv.cpp:44"),
+    ("erfcinv", "0.00467773498104726583793074363275", "2", 10, "modules/core/euler/unit/scalar/erfcinv.cpp:45"),
     # compositions: the scalar unit tests of NT2 (evaluated in packs and as tails)
     ("coth", "inf", "1", 0.5, "modules/core/hyperbolic/unit/scalar/coth.cpp:39"),
     ("coth", "-inf", "-1", 0.5, "modules/core/hyperbolic/unit/scalar/coth.cpp:40"),

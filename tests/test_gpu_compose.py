@@ -28,7 +28,7 @@ G = os.path.join(os.path.dirname(__file__), "golden")
 COMPOSED = ["coth", "sech", "csch", "acoth", "asech", "acsch", "acot", "asec", "acsc", "acotd", "asecd",
             "acscd", "acosd", "acospi", "asind", "asinpi", "atand", "atanpi", "indeg", "inrad", "secd",
             "cscd", "secpi", "cscpi", "sinc", "sincpi", "sqrt1pm1", "exprecneg", "exprecnegc", "reallog",
-            "realsqrt", "expx2"]
+            "realsqrt", "expx2", "erfinv", "erfcinv"]
 # the larger term of a cancelling difference in NT2's definition
 CANCEL_SCALE = {"asec": np.pi / 2, "asecd": 90.0, "acotd": 90.0}
 
@@ -81,7 +81,9 @@ def test_composed_fast_close_to_exact(op, dt):
     got = ops.map(op, xd, FAST).cpu().numpy()
     ex = ops.map(op, xd, EXACT).cpu().numpy()
     err = classic_ulp(got, ex.astype(np.float64), dt)
-    ok = (err <= 4) | (np.isnan(got) & np.isnan(ex)) | (got == ex)
+    # erfcinv's f32 rational erfcinv2 has cancelling terms: its fused and
+    # unfused evaluations differ by up to 6 ulp (its own bound is ~20)
+    ok = (err <= (8 if op == "erfcinv" else 4)) | (np.isnan(got) & np.isnan(ex)) | (got == ex)
     eps = np.finfo(dt).eps
     if op in CANCEL_SCALE:
         ok |= np.abs(got.astype(np.float64) - ex) <= 4 * eps * CANCEL_SCALE[op]
