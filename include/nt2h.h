@@ -127,6 +127,7 @@ extern "C" {
 #define NT2H_OP_EXPX2 75     /* exp(x^2): exponential/functions/simd/common/expx2.hpp:37-61 */
 #define NT2H_OP_ERFINV 76    /* euler/functions/simd/common/erfinv.hpp:32-108 */
 #define NT2H_OP_ERFCINV 77   /* euler/functions/simd/common/erfcinv.hpp:32-107 */
+#define NT2H_OP_SINHC 78     /* sinh(x)/x: hyperbolic/functions/simd/common/sinhc.hpp:41-80 */
 
 /* Black-Scholes formula variants */
 #define NT2H_BS_TEXTBOOK 0 /* d1 = (log(S/X)+(r+v^2/2)T)/(v sqrt T), call and put, erfc-based normcdf */

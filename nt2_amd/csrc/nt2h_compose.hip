@@ -1,9 +1,9 @@
 // Elementwise maps of NT2's composed functions (C ABI op codes NT2H_OP_COTH
-// to NT2H_OP_ERFCINV): the reciprocal hyperbolic functions and their inverses,
+// to NT2H_OP_SINHC): the reciprocal hyperbolic functions and their inverses,
 // the inverse trigonometric functions of reciprocals and in degree or pi
 // units, the reciprocal trigonometric functions of degree and pi arguments,
 // sinc, and the exponential family's sqrt1pm1, exprecneg(c), reallog,
-// realsqrt and expx2, and the inverse error functions.  The schedule is the other maps' (packs of 8 floats / 4 doubles
+// realsqrt and expx2, the inverse error functions, and sinhc.  The schedule is the other maps' (packs of 8 floats / 4 doubles
 // from index 0, NT2's scalar form for the tail); the functions are in
 // nt2/hip/math_compose.hpp.  A translation unit of its own so that the
 // library's sources compile in parallel.
@@ -66,7 +66,8 @@ template <class P, int OP> struct compose_op {
         else if constexpr (OP == NT2H_OP_REALSQRT) v = dm::sqrtv(x);
         else if constexpr (OP == NT2H_OP_EXPX2) v = dm::expx2_t<P, S>(x);
         else if constexpr (OP == NT2H_OP_ERFINV) v = dm::erfinv_t<P>(x);
-        else v = dm::erfcinv_t<P>(x);
+        else if constexpr (OP == NT2H_OP_ERFCINV) v = dm::erfcinv_t<P>(x);
+        else v = dm::sinhc_t<P>(x);
         r[l] = v;
       }
     }
@@ -95,7 +96,7 @@ static int launch_t(const char* name, int op, size_t n, const T* x, T* y, hipStr
     NT2H_COMPOSE(NT2H_OP_SINC) NT2H_COMPOSE(NT2H_OP_SINCPI)
     NT2H_COMPOSE(NT2H_OP_SQRT1PM1) NT2H_COMPOSE(NT2H_OP_EXPRECNEG) NT2H_COMPOSE(NT2H_OP_EXPRECNEGC)
     NT2H_COMPOSE(NT2H_OP_REALLOG) NT2H_COMPOSE(NT2H_OP_REALSQRT) NT2H_COMPOSE(NT2H_OP_EXPX2)
-    NT2H_COMPOSE(NT2H_OP_ERFINV) NT2H_COMPOSE(NT2H_OP_ERFCINV)
+    NT2H_COMPOSE(NT2H_OP_ERFINV) NT2H_COMPOSE(NT2H_OP_ERFCINV) NT2H_COMPOSE(NT2H_OP_SINHC)
 #undef NT2H_COMPOSE
     default: return fail(NT2H_EINVAL, name);
   }

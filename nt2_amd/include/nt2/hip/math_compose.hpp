@@ -381,4 +381,33 @@ template <class P, class T> NT2H_FN T erfcinv_t(T a0) {
   }
 }
 
+// sinhc = sinh(x)/x: hyperbolic/functions/simd/common/sinhc.hpp:41-80 (the
+// scalar form scalar/sinhc.hpp:31-67 gives the same value per lane): below
+// |x| = 1 sinhc_kernel (details/sinhc_kernel.hpp: a degree-3 polynomial in x^2
+// for float, 1 + x^2 p(x^2)/q(x^2) for double); above, e = exp(x) and
+// average(e, -1/e)/x, or with e = exp(x/2) beyond Maxlog - Log_2,
+// ((e/2)/x) e; +Inf at +-Inf
+template <class P, class T> NT2H_FN T sinhc_t(T a0) {
+  NT2H_STRICT
+  const T x = absv(a0);
+  if (x < T(1)) {
+    const T x2 = x * x;
+    if constexpr (sizeof(T) == 4) {
+      constexpr float c[4] = {NT2H_F(0x39559e2f), NT2H_F(0x3c087bbe), NT2H_F(0x3e2aaacc), NT2H_F(0x3f800000)};
+      return horner<P>(c, x2);
+    } else {
+      constexpr double p[4] = {NT2H_D(0xbfe9435fe8bb3cd6), NT2H_D(0xc064773a398ff4fe), NT2H_D(0xc0c694b8c71d6182),
+                               NT2H_D(0xc115782bdbf6ab05)};
+      constexpr double q[4] = {NT2H_D(0x3ff0000000000000), NT2H_D(0xc0715b6096e96484), NT2H_D(0x40e1a7ba7ed72245),
+                               NT2H_D(0xc1401a20e4f90044)};
+      return 1.0 + (x2 * horner<P>(p, x2)) / horner<P>(q, x2);
+    }
+  }
+  const T thr = cst<T>(0x42b0c0a5u, 0x40862e42fefa39efull) - cst<T>(0x3f317218u, 0x3fe62e42fefa39efull);
+  const bool big = x > thr;
+  const T e = exp_t<P>(x * (big ? T(0.5) : T(1)));
+  const T r = big ? ((T(0.5) * e) / x) * e : fma<P>(e, T(0.5), -(T(1) / e) * T(0.5)) / x;
+  return __builtin_isinf(x) ? x : r;
+}
+
 }}}  // namespace nt2::hip::dm

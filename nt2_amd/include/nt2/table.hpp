@@ -225,7 +225,7 @@ NT2H_TAG(atand_, same1p) NT2H_TAG(atanpi_, same1p) NT2H_TAG(indeg_, same1) NT2H_
 NT2H_TAG(secd_, same1) NT2H_TAG(cscd_, same1) NT2H_TAG(secpi_, same1) NT2H_TAG(cscpi_, same1)
 NT2H_TAG(sinc_, same1p) NT2H_TAG(sincpi_, same1p) NT2H_TAG(atan2d_, same2)
 NT2H_TAG(sqrt1pm1_, same1) NT2H_TAG(exprecneg_, same1) NT2H_TAG(exprecnegc_, same1) NT2H_TAG(reallog_, same1)
-NT2H_TAG(realsqrt_, same1) NT2H_TAG(expx2_, same1p) NT2H_TAG(erfinv_, same1) NT2H_TAG(erfcinv_, same1) NT2H_TAG(logspace_add_, same2) NT2H_TAG(logspace_sub_, same2)
+NT2H_TAG(realsqrt_, same1) NT2H_TAG(expx2_, same1p) NT2H_TAG(erfinv_, same1) NT2H_TAG(erfcinv_, same1) NT2H_TAG(sinhc_, same1) NT2H_TAG(logspace_add_, same2) NT2H_TAG(logspace_sub_, same2)
 NT2H_TAG(pow_, pow2)
 NT2H_TAG(fma_, same3) NT2H_TAG(fnms_, same3) NT2H_TAG(if_else_, sel3)
 NT2H_TAG(selinc_, selinc2) NT2H_TAG(seldec_, selinc2) NT2H_TAG(if_zero_else_one_, izeo1)
@@ -390,6 +390,7 @@ NT2H_UN(cscpi_, dm::cscpi_t<P>(x), dm::cscpi_t<P>(x), no_integer_form<T>())
 NT2H_UN(sqrt1pm1_, dm::sqrt1pm1(x), dm::sqrt1pm1(x), no_integer_form<T>())
 NT2H_UN(erfinv_, dm::erfinv_t<P>(x), dm::erfinv_t<P>(x), no_integer_form<T>())
 NT2H_UN(erfcinv_, dm::erfcinv_t<P>(x), dm::erfcinv_t<P>(x), no_integer_form<T>())
+NT2H_UN(sinhc_, dm::sinhc_t<P>(x), dm::sinhc_t<P>(x), no_integer_form<T>())
 NT2H_UN(exprecneg_, dm::exprecneg_t<P>(x), dm::exprecneg_t<P>(x), no_integer_form<T>())
 NT2H_UN(exprecnegc_, dm::exprecnegc_t<P>(x), dm::exprecnegc_t<P>(x), no_integer_form<T>())
 NT2H_UN(reallog_, dm::log_t<P>(x), dm::log_t<P>(x), no_integer_form<T>())
@@ -835,7 +836,7 @@ NT2H_FN1(secd, secd_) NT2H_FN1(cscd, cscd_) NT2H_FN1(secpi, secpi_) NT2H_FN1(csc
 NT2H_FN1(sinc, sinc_) NT2H_FN1(sincpi, sincpi_)
 NT2H_FN1(sqrt1pm1, sqrt1pm1_) NT2H_FN1(exprecneg, exprecneg_) NT2H_FN1(exprecnegc, exprecnegc_)
 NT2H_FN1(reallog, reallog_) NT2H_FN1(realsqrt, realsqrt_) NT2H_FN1(expx2, expx2_)
-NT2H_FN1(erfinv, erfinv_) NT2H_FN1(erfcinv, erfcinv_)
+NT2H_FN1(erfinv, erfinv_) NT2H_FN1(erfcinv, erfcinv_) NT2H_FN1(sinhc, sinhc_)
 NT2H_FN1(if_zero_else_one, if_zero_else_one_)
 #undef NT2H_FN1
 #define NT2H_FN2(name, Tag) \

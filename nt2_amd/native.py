@@ -20,7 +20,7 @@ OPS = {"exp": 0, "log": 1, "sqrt": 2, "erfc": 3, "normcdf": 4,
        "asind": 58, "asinpi": 59, "atand": 60, "atanpi": 61, "indeg": 62, "inrad": 63, "secd": 64,
        "cscd": 65, "secpi": 66, "cscpi": 67, "sinc": 68, "sincpi": 69,
        "sqrt1pm1": 70, "exprecneg": 71, "exprecnegc": 72, "reallog": 73, "realsqrt": 74, "expx2": 75,
-       "erfinv": 76, "erfcinv": 77}
+       "erfinv": 76, "erfcinv": 77, "sinhc": 78}
 H2D, D2H, D2D = 1, 2, 3
 
 _lib = None

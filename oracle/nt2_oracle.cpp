@@ -1867,6 +1867,7 @@ template <class T> T cscpi_o(T a0) {
   return (is_flint_v(a0) && a0 != T(0)) ? T(NAN) : r;
 }
 template <class T> T erfinv_o(T a0);
+template <class T> T sinhc_o(T a0);
 template <class T> T erfcinv_o(T a0);
 // one lane of a composed op; kinds as in the C ABI (44..69, 68 = sinc is by pack)
 template <class T> T compose_o(int op, T x, bool sc) {
@@ -1930,6 +1931,7 @@ template <class T> T compose_o(int op, T x, bool sc) {
     }
     case 76: return erfinv_o(x);
     case 77: return erfcinv_o(x);
+    case 78: return sinhc_o(x);
     case 69: {
       const T sp = trig_unit<T, true>(1, x);
       if (std::isinf(x)) return T(0);
@@ -2029,6 +2031,26 @@ template <class T> T erfcinv_o(T a0) {
   }
 }
 
+// sinhc: hyperbolic/functions/simd/common/sinhc.hpp:41-80, details/sinhc_kernel.hpp
+template <class T> T sinhc_o(T a0) {
+  const T x = std::fabs(a0);
+  if (x < T(1)) {
+    const T x2 = x * x;
+    if constexpr (sizeof(T) == 4) {
+      const float c[4] = {F32(0x39559e2f), F32(0x3c087bbe), F32(0x3e2aaacc), F32(0x3f800000)};
+      return horner(c, x2);
+    } else {
+      const double p[4] = {F64(0xbfe9435fe8bb3cd6), F64(0xc064773a398ff4fe), F64(0xc0c694b8c71d6182), F64(0xc115782bdbf6ab05)};
+      const double q[4] = {F64(0x3ff0000000000000), F64(0xc0715b6096e96484), F64(0x40e1a7ba7ed72245), F64(0xc1401a20e4f90044)};
+      return 1.0 + (x2 * horner(p, x2)) / horner(q, x2);
+    }
+  }
+  const T thr = ov::cst<T>(0x42b0c0a5u, 0x40862e42fefa39efull) - ov::cst<T>(0x3f317218u, 0x3fe62e42fefa39efull);
+  const bool big = x > thr;
+  const T e = ov::exp(x * (big ? T(0.5) : T(1)));
+  const T r = big ? ((T(0.5) * e) / x) * e : (e * T(0.5) + (-(T(1) / e)) * T(0.5)) / x;
+  return std::isinf(x) ? x : r;
+}
 // logspace_add: exponential/functions/scalar/logspace_add.hpp:25-42;
 // logspace_sub: scalar/logspace_sub.hpp:20-32 (Log_2: constants/log_2.hpp:29-30)
 template <class T> T logspace_add_o(T a0, T a1) {
@@ -2231,8 +2253,8 @@ enum {
   OR_SINPI = 36, OR_COSPI = 37, OR_TANPI = 38, OR_COTPI = 39, OR_SIND = 40, OR_COSD = 41, OR_TAND = 42,
   OR_COTD = 43,
   // 44..69: the compositions (compose_o; 68 = sinc by pack)
-  OR_SINC = 68, OR_SINCPI = 69, OR_EXPX2 = 75, OR_ERFCINV = 77,
-  OR_LAST = OR_ERFCINV
+  OR_SINC = 68, OR_SINCPI = 69, OR_EXPX2 = 75, OR_SINHC = 78,
+  OR_LAST = OR_SINHC
 };
 
 int oracle_abi_version(void) { return 1; }

@@ -134,6 +134,7 @@ FUNCS = {
     "expx2": lambda x: mpmath.exp(x * x),
     "erfinv": lambda x: mpmath.erfinv(x) if abs(x) < 1 else (mpmath.inf * mpmath.sign(x) if abs(x) == 1 else mpmath.nan),
     "erfcinv": lambda x: mpmath.erfinv(1 - x) if 0 < x < 2 else (mpmath.inf * mpmath.sign(1 - x) if x in (0, 2) else mpmath.nan),
+    "sinhc": lambda x: mpmath.sinh(x) / x if x != 0 else mpmath.mpf(1),
 }
 
 
@@ -298,6 +299,7 @@ COMPOSED_DOMAINS = {
     "expx2": (-1e-6, 9.0, [1.0, 0.5, 3.0, 5.0, 8.9, 0.01]),
     "erfinv": (-1e-20, 1.0, [0.5, -0.5, 0.9966, 0.99, 0.999, 0.9999999, -0.99999, 1e-5]),
     "erfcinv": (1e-30, 2.0, [1.0, 0.5, 1.5, 0.0034, 1.9966, 1e-10, 1e-20, 1.9999999]),
+    "sinhc": (-1e-6, 85.0, [1.0, 0.999, 0.5, 2.0, 10.0, 20.0]),
 }
 
 
@@ -697,6 +699,10 @@ KATS = [
     ("erfcinv", "0.47950012218695346231725334610804", "0.5", 10, "modules/core/euler/unit/scalar/erfcinv.cpp:43"),
     ("erfcinv", "0.15729920705028513065877936491739", "1", 10, "modules/core/euler/unit/scalar/erfcinv.cpp:44"),
     ("erfcinv", "0.00467773498104726583793074363275", "2", 10, "modules/core/euler/unit/scalar/erfcinv.cpp:45"),
+    ("sinhc", "inf", "inf", 0.5, "modules/core/hyperbolic/unit/scalar/sinhc.cpp:37"),
+    ("sinhc", "-inf", "inf", 0.5, "modules/core/hyperbolic/unit/scalar/sinhc.cpp:38"),
+    ("sinhc", "nan", "nan", 0.5, "modules/core/hyperbolic/unit/scalar/sinhc.cpp:39"),
+    ("sinhc", "0", "1", 0.5, "modules/core/hyperbolic/unit/scalar/sinhc.cpp:41"),
     # compositions: the scalar unit tests of NT2 (evaluated in packs and as tails)
     ("coth", "inf", "1", 0.5, "modules/core/hyperbolic/unit/scalar/coth.cpp:39"),
     ("coth", "-inf", "-1", 0.5, "modules/core/hyperbolic/unit/scalar/coth.cpp:40"),

@@ -28,7 +28,7 @@ G = os.path.join(os.path.dirname(__file__), "golden")
 COMPOSED = ["coth", "sech", "csch", "acoth", "asech", "acsch", "acot", "asec", "acsc", "acotd", "asecd",
             "acscd", "acosd", "acospi", "asind", "asinpi", "atand", "atanpi", "indeg", "inrad", "secd",
             "cscd", "secpi", "cscpi", "sinc", "sincpi", "sqrt1pm1", "exprecneg", "exprecnegc", "reallog",
-            "realsqrt", "expx2", "erfinv", "erfcinv"]
+            "realsqrt", "expx2", "erfinv", "erfcinv", "sinhc"]
 # the larger term of a cancelling difference in NT2's definition
 CANCEL_SCALE = {"asec": np.pi / 2, "asecd": 90.0, "acotd": 90.0}
 

@@ -126,7 +126,7 @@ def _bound(name, x, y, dt):
                                   "acoth", "asech", "acsch", "acot", "asec", "acsc", "acotd", "asecd", "acscd",
                                   "acosd", "acospi", "asind", "asinpi", "atand", "atanpi", "indeg", "inrad", "secd",
                                   "cscd", "secpi", "cscpi", "sinc", "sincpi", "sqrt1pm1", "exprecneg", "exprecnegc",
-                                  "reallog", "realsqrt", "expx2", "erfinv", "erfcinv"])
+                                  "reallog", "realsqrt", "expx2", "erfinv", "erfcinv", "sinhc"])
 def test_golden_mpmath(dt, tag, name):
     g = np.load(os.path.join(G, "math_%s.npz" % tag))
     x, y = g[name + "_x"], g[name + "_y"]
@@ -207,7 +207,7 @@ COMPOSED_BOUND = {"coth": (2, 2), "sech": (3, 2), "csch": (3, 2), "acoth": (3, 2
                   "asinpi": (3, 2), "atand": (3, 2), "atanpi": (3, 2), "indeg": (1, 1), "inrad": (1, 1),
                   "secd": (3, 2), "cscd": (4, 3), "secpi": (3, 2), "cscpi": (3, 2), "sinc": (2, 2),
                   "sincpi": (3, 3), "sqrt1pm1": (6, 6), "exprecneg": (3, 3), "exprecnegc": (3, 3),
-                  "reallog": (1, 1), "realsqrt": (0.5, 0.5), "erfinv": (4, 4), "erfcinv": (24, 4)}
+                  "reallog": (1, 1), "realsqrt": (0.5, 0.5), "erfinv": (4, 4), "erfcinv": (24, 4), "sinhc": (3, 2)}
 
 TRIG_VARIANTS = ("cot", "sec", "csc", "sinpi", "cospi", "tanpi", "cotpi", "sind", "cosd", "tand", "cotd")
 UNIT_DOMAIN32 = {"tanpi": 2.0 ** 22, "sind": 2.0 ** 24, "cosd": 2.0 ** 24, "tand": 2.0 ** 24, "cotd": 2.0 ** 24}
@@ -827,7 +827,7 @@ def test_composed_structure(dt):
     x = np.abs(x)
     odd = ["coth", "csch", "acoth", "acsch", "acot", "acsc", "acotd", "acscd", "asind", "asinpi", "atand",
            "atanpi", "indeg", "inrad", "cscd", "cscpi", "erfinv"]
-    even = ["sech", "secd", "secpi", "sinc", "sincpi"]
+    even = ["sech", "secd", "secpi", "sinc", "sincpi", "sinhc"]
     for name in odd + even:
         for tail in (False, True):
             p = _one_lane(mapf, name, x, W, tail)
